@@ -1,0 +1,241 @@
+#!/usr/bin/env python3
+"""Benchmark: batched Newton iterations/s of the linear RC power-grid mesh.
+
+Workload (the batched-refactor configuration): 64x64 RC mesh (4096 nodes +
+1 source branch), branch R log-uniform [0.1, 10] ohm, one decap per node
+log-uniform [10 fF, 1 pF], 5% of nodes with DC current sinks in [0, 1 mA],
+1.0 V at a corner; 256 instances with per-instance +-10% Gaussian R
+perturbation; backward-Euler transient (xmu = 0) at dt = 1 ps.
+
+A bench step is one transient timestep of all 256 instances: the full
+newton_solve (device load + gather, GPU refactor on the fixed AMD-LU order,
+GPU solve, convergence) -- two Newton iterations per step for this linear
+circuit.  value = sum over instances of Newton iterations / wall time.
+
+Also measured in the same run: the same kernels at batch 1 (the plain
+batch-1 GPU path), HIP-event times of the load, refactor and solve phases
+with their roofline, and the reference CPU path (compiled neospice,
+oracle/_ref) on the host cores on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def lu_traffic(eng, ckt, B):
+    """Algorithmic bytes and flops per refactor and per solve (per batch)."""
+    st = eng.stats()
+    nnz_lu = st["nnz_lu"]
+    n, nnz_a = ckt.n, ckt.nnz
+    refactor_bytes = 8.0 * (nnz_a + nnz_lu) * B
+    solve_bytes = (8.0 * nnz_lu + 16.0 * n) * B
+    return refactor_bytes, solve_bytes, st
+
+
+def refactor_flops(eng1, ckt, B):
+    """F_refactor = sum_j sum_{r in U(:,j)} (1 + 2 |L(:,r)|_offdiag) + n over the
+    pivot structure the host fixes for this matrix (same pattern and values)."""
+    import neospice_amd as na
+    vals, _ = eng1.load()
+    s = na.pivot_structure("amdlu", ckt.n, ckt.col_ptr, ckt.row_idx, vals[:, 0])
+    seg, nU, rows = s["seg_ptr"], s["nU"], s["rows"]
+    nl = np.array([seg[k + 1] - seg[k] - nU[k] - 1 for k in range(ckt.n)])
+    f = ckt.n
+    for k in range(ckt.n):
+        f += int(np.sum(1 + 2 * nl[rows[seg[k]:seg[k] + nU[k]]]))
+    return float(f) * B
+
+
+def run_steps(eng, na, dt, first_step, count, t0_step):
+    iters = 0
+    for k in range(count):
+        step = t0_step + k
+        ctx = na.be_ctx(dt, (step + 1) * dt, first=(step == 0))
+        conv, its = eng.tran_step(ctx)
+        if not conv.all():
+            raise RuntimeError("Newton failed at step %d for %d instances" % (step, (~conv).sum()))
+        eng.rotate_states()
+        iters += int(its.sum())
+    return iters
+
+
+def cpu_baseline(ckt, info, R, dt, nsteps, budget_s, parity_steps, x_gpu0):
+    """CPU-baseline leg: the reference's own CPU Newton path (oracle/_ref) on
+    whole instances of the same workload, one instance per host thread
+    (ctypes releases the GIL).  The same leg checks instance 0 of the GPU run
+    against the reference after the same step sequence."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refbind as RB
+    if not RB.available():
+        return None, None
+    res_devs = info["res_devs"]
+    vals0 = {int(d): float(R[k, 0]) for k, d in enumerate(res_devs)}
+    rc = RB.RefCircuit(ckt, values=vals0, options={"xmu": 0.0}, probe=False)
+    x_ref, _, _ = rc.tran_fixed(dt, parity_steps)
+    parity = float(np.max(np.abs(x_gpu0 - x_ref)))
+    cores = min(os.cpu_count() or 1, 16)
+    lock = threading.Lock()
+    tot = {"iters": 0, "inst": 0}
+    deadline = time.perf_counter() + budget_s
+
+    def worker(tid):
+        i = tid
+        while time.perf_counter() < deadline:
+            vals = {int(d): float(R[k, i % R.shape[1]]) for k, d in enumerate(res_devs)}
+            r = RB.RefCircuit(ckt, values=vals, options={"xmu": 0.0}, probe=False)
+            _, its, _ = r.tran_fixed(dt, nsteps)
+            with lock:
+                tot["iters"] += its
+                tot["inst"] += 1
+            i += cores
+
+    t0 = time.perf_counter()
+    th = [threading.Thread(target=worker, args=(t,)) for t in range(cores)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    wall = time.perf_counter() - t0
+    cpu = {"value": round(tot["iters"] / wall, 2), "unit": "Newton iters/s", "cores": cores,
+           "kind": "reference",
+           "sample": "%d whole instances x %d BE steps (%d Newton iterations) of the 64x64 mesh "
+                     "on %d host threads in %.1f s; reference CPU path: newton_solve with "
+                     "AMD-LU replay refactor/solve and device loads" %
+                     (tot["inst"], nsteps, tot["iters"], cores, wall)}
+    return cpu, parity
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--mesh", type=int, default=64)
+    ap.add_argument("--batch1-steps", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
+              "(instances shard across GPUs with no exchange; see DESIGN.md)" % args.gpus)
+        sys.exit(2)
+
+    import torch
+    import neospice_amd as na
+    from neospice_amd import workloads as W
+
+    dt = 1e-12
+    B = args.batch
+    ckt, info = W.rc_mesh(args.mesh)
+    ckt.finalize()
+    R = W.mesh_instance_resistors(info, B)
+    log("bench: mesh %dx%d n=%d nnz=%d, batch %d" % (args.mesh, args.mesh, ckt.n, ckt.nnz, B))
+
+    eng = na.BatchEngine(ckt, batch=B, xmu=0.0)
+    for k, d in enumerate(info["res_devs"]):
+        eng.set_values(int(d), R[k])
+    eng.tran_begin()
+    log("bench: DC preamble done, solver", eng.solver_name)
+    run_steps(eng, na, dt, True, args.warmup, 0)
+    torch.cuda.synchronize()
+    eng.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    iters = run_steps(eng, na, dt, False, args.steps, args.warmup)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    timing = eng.timing()
+    eng.set_timing(False)
+    value = iters / wall
+    log("bench: batched %d steps, %d instance-iterations in %.3f s -> %.1f iters/s"
+        % (args.steps, iters, wall, value))
+    refactor_bytes, solve_bytes, st = lu_traffic(eng, ckt, B)
+    x_gpu0 = eng.solution()[:, 0].copy()
+
+    # batch-1 GPU path: the same kernels, one instance
+    e1 = na.BatchEngine(ckt, batch=1, xmu=0.0)
+    for k, d in enumerate(info["res_devs"]):
+        e1.set_values(int(d), R[k, :1])
+    e1.tran_begin()
+    run_steps(e1, na, dt, True, args.warmup, 0)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    it1 = run_steps(e1, na, dt, False, args.batch1_steps, args.warmup)
+    torch.cuda.synchronize()
+    w1 = time.perf_counter() - t1
+    batch1_value = it1 / w1
+    log("bench: batch-1 %d steps, %d iterations in %.3f s -> %.1f iters/s"
+        % (args.batch1_steps, it1, w1, batch1_value))
+
+    ms_refactor = timing["refactor_ms"] / max(1, iters // B)
+    ms_solve = timing["solve_ms"] / max(1, iters // B)
+    ms_load = timing["load_ms"] / max(1, iters // B)
+    fl = refactor_flops(e1, ckt, B)
+    phases = {
+        "load_gather_ms": ms_load, "refactor_ms": ms_refactor, "solve_ms": ms_solve,
+        "refactor_GBs": refactor_bytes / (ms_refactor * 1e-3) / 1e9,
+        "refactor_GFLOPs": fl / (ms_refactor * 1e-3) / 1e9,
+        "refactor_flops": fl,
+        "solve_GBs": solve_bytes / (ms_solve * 1e-3) / 1e9,
+        "levels_refactor": st["levels_refactor"], "levels_solve": st["levels_solve"],
+        "nnz_lu": st["nnz_lu"],
+    }
+    achieved = refactor_bytes / (ms_refactor * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": "k_refactor_level (all levels of one refactor)",
+                "bytes_per_launch": refactor_bytes}
+
+    cpu, parity = None, None
+    if not args.no_cpu:
+        try:
+            cpu, parity = cpu_baseline(ckt, info, R, dt, 10, args.cpu_seconds,
+                                       args.warmup + args.steps, x_gpu0)
+            log("bench: instance 0 vs reference CPU path, max |dx| =", parity)
+        except Exception as e:  # the baseline is reported, not required
+            log("bench: cpu baseline unavailable:", e)
+
+    out = {
+        "metric": "batched Newton iters/s vs batch-1 GPU @1e-9 rel; adjoint grad/forward time",
+        "value": round(value, 2),
+        "unit": "Newton iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded 64x64 RC mesh, per-instance R perturbation)",
+        "config": {"workload": "rc_mesh_64x64_batched_refactor_be", "instances": B,
+                   "unknowns": ckt.n, "nnz": ckt.nnz, "dt": dt, "integration": "backward Euler",
+                   "solver": eng.solver_name},
+        "batch1_gpu_value": round(batch1_value, 2),
+        "speedup_vs_batch1_gpu": round(value / batch1_value, 2),
+        "parity_max_abs_diff_vs_reference": parity,
+        "phases": phases,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,154 @@
+/* neospice_amd: MI355X (gfx950) Newton hot path for neospice, C ABI.
+ *
+ * Two seams of the reference are exposed, both for a batch of B circuit
+ * instances that share one topology (B = 1 is the plain drop-in):
+ *
+ *   nsa_lu_*      the solver interface ISolver (src/core/solver_iface.hpp:16-41):
+ *                 symbolic / numeric / refactorize / solve, plus the
+ *                 transposed solve the adjoint needs (SparseMatrix::
+ *                 solve_transposed, src/solver/solve.cpp:102-141).
+ *   nsa_engine_*  one Newton-iteration pipeline over a finalized circuit:
+ *                 Device::evaluate for every device (src/devices/device.hpp:85-86),
+ *                 the NumericMatrix accumulation, factor, solve and the
+ *                 NIiter bookkeeping of newton_solve (src/core/newton.cpp:51-419).
+ *   nsa_circuit_* the typed Circuit builders and finalize() the engine consumes
+ *                 (src/core/circuit.hpp:179-273).
+ *
+ * Conventions
+ *   - Every function returns an int status: 0 = OK, 1 = singular (the
+ *     `true` of ISolver::numeric/refactorize), negative = error:
+ *     -1 misuse order (std::logic_error in the reference), -2 bad argument
+ *     or size (std::invalid_argument), -3 runtime failure
+ *     (std::runtime_error), -4 HIP error, -5 a reference feature this
+ *     build does not cover.  nsa_last_error() returns the message of the
+ *     calling thread's last error.  No C++ exception crosses the ABI.
+ *   - Batched arrays are unknown-major with the instance index fastest:
+ *     value(i, inst) = a[i * B + inst].  Matrix values follow the slot
+ *     order of the reference's NumericMatrix (CSC, sorted by column then
+ *     row).  Node indices are 0-based MNA indices, -1 = ground.
+ *   - The caller owns every host buffer; handles own their device memory.
+ *   - A handle is used by one host thread at a time (the reference keeps
+ *     its analysis contexts thread_local, src/core/newton.cpp:22).
+ */
+#ifndef NEOSPICE_AMD_H
+#define NEOSPICE_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nsa_circuit nsa_circuit;
+typedef struct nsa_engine nsa_engine;
+typedef struct nsa_lu nsa_lu;
+
+/* Mirrors SimOptions (src/core/types.hpp:24-66) fields used on this path. */
+typedef struct {
+    double abstol, reltol, vntol, trtol, chgtol, gmin, diag_gmin, xmu, src_fact;
+    int32_t itl1, itl4, method, engine; /* engine: 0 auto, 1 markowitz, 2 amdlu */
+} nsa_options;
+
+/* Mirrors the IntegratorCtx fields device loads read (types.hpp:73-94). */
+typedef struct {
+    int32_t mode, order, method, pad;
+    double ag0, ag1, ag2, xmu_ratio, time, src_fact;
+} nsa_ctx;
+
+const char* nsa_last_error(void);
+void nsa_default_options(nsa_options* opt);
+
+/* ---- circuit (Circuit typed API, src/api/circuit_typed.cpp) ---- */
+nsa_circuit* nsa_circuit_create(void);
+void nsa_circuit_destroy(nsa_circuit* c);
+int nsa_circuit_node(nsa_circuit* c, const char* name, int32_t* index);
+int nsa_circuit_r(nsa_circuit* c, const char* name, int32_t a, int32_t b, double ohms, int32_t* dev);
+int nsa_circuit_c(nsa_circuit* c, const char* name, int32_t a, int32_t b, double farads, int32_t* dev);
+int nsa_circuit_v(nsa_circuit* c, const char* name, int32_t p, int32_t n, double dc, int32_t* dev);
+int nsa_circuit_v_pulse(nsa_circuit* c, const char* name, int32_t p, int32_t n,
+                        const double pulse[7], int32_t* dev);
+int nsa_circuit_v_pwl(nsa_circuit* c, const char* name, int32_t p, int32_t n, int32_t npts,
+                      const double* t, const double* v, int32_t* dev);
+int nsa_circuit_i(nsa_circuit* c, const char* name, int32_t p, int32_t n, double dc, int32_t* dev);
+int nsa_circuit_model(nsa_circuit* c, const char* text);
+int nsa_circuit_d(nsa_circuit* c, const char* name, int32_t a, int32_t k, const char* model, int32_t* dev);
+int nsa_circuit_m(nsa_circuit* c, const char* name, int32_t d, int32_t g, int32_t s, int32_t b,
+                  const char* model, double w, double l, int32_t* dev);
+/* Circuit::finalize (src/core/circuit.cpp:201-283). */
+int nsa_circuit_finalize(nsa_circuit* c);
+/* n (unknowns), nodes, nnz, states, is_linear */
+int nsa_circuit_info(const nsa_circuit* c, int32_t out[5]);
+int nsa_circuit_pattern(const nsa_circuit* c, int32_t* col_ptr, int32_t* row_idx);
+int nsa_circuit_load_order(const nsa_circuit* c, int32_t* order);
+int nsa_circuit_branch(const nsa_circuit* c, int32_t dev, int32_t* branch);
+
+/* ---- batched Newton engine ---- */
+int nsa_engine_create(nsa_circuit* c, int32_t batch, const nsa_options* opt, nsa_engine** out);
+void nsa_engine_destroy(nsa_engine* e);
+/* Per-instance primary value of device `dev` (R ohms, C farads, V/I dc), B values. */
+int nsa_engine_set_values(nsa_engine* e, int32_t dev, const double* vals);
+/* per_instance = 0: ctx[0] for all instances; 1: ctx[inst]. */
+int nsa_engine_set_ctx(nsa_engine* e, const nsa_ctx* ctx, int32_t per_instance);
+/* newton_solve with iteration limit itl; converged/iterations per instance. */
+int nsa_engine_newton(nsa_engine* e, int32_t itl, int32_t* converged, int32_t* iterations);
+/* Host copies, n x B (solution) / states x B (state slot `which` 0..3). */
+int nsa_engine_get_solution(nsa_engine* e, double* x);
+int nsa_engine_set_solution(nsa_engine* e, const double* x);
+int nsa_engine_get_state(nsa_engine* e, int32_t which, double* s);
+int nsa_engine_set_state(nsa_engine* e, int32_t which, const double* s);
+/* state1..3 = state0 (transient.cpp:791-802) / rotate (circuit.cpp:328-334). */
+int nsa_engine_seed_states(nsa_engine* e);
+int nsa_engine_rotate_states(nsa_engine* e);
+/* Fresh solver on the same pattern (a new make_solver + symbolic). */
+int nsa_engine_reset_solver(nsa_engine* e);
+/* One device load at the current solution: matrix values nnz x B, rhs n x B. */
+int nsa_engine_load(nsa_engine* e, double* values, double* rhs);
+/* A^T x = b on the factors of the last Newton iteration (b in place, n x B). */
+int nsa_engine_solve_transposed(nsa_engine* e, double* b);
+/* Name of the engine in use ("markowitz" / "amdlu"), pivot statistics. */
+const char* nsa_engine_solver_name(nsa_engine* e);
+/* out: cpu pivots, gpu refactors, nnz(L+U), refactor levels, solve levels
+ * (of the first pivot group), gpu solves, device loads, pivot groups. */
+int nsa_engine_stats(nsa_engine* e, int64_t out[8]);
+/* HIP-event timing on the engine's stream: accumulated milliseconds of
+ * device load+gather, GPU refactor and GPU solve since timing was enabled. */
+int nsa_engine_set_timing(nsa_engine* e, int32_t on);
+int nsa_engine_timing(nsa_engine* e, double out[3]);
+
+/* ---- solver seam (ISolver) ---- */
+int nsa_lu_create(int32_t batch, int32_t n, const int32_t* col_ptr, const int32_t* row_idx,
+                  int32_t is_linear, int32_t engine, nsa_lu** out);
+void nsa_lu_destroy(nsa_lu* s);
+/* values: host, nnz x B.  status: per instance 0 ok / 1 singular.  Returns
+ * 1 if any instance is singular. */
+int nsa_lu_numeric(nsa_lu* s, const double* values, double diag_gmin, int32_t* status);
+int nsa_lu_refactorize(nsa_lu* s, const double* values, double diag_gmin, int32_t* status);
+/* rhs: host, n x B, overwritten with the solution. */
+int nsa_lu_solve(nsa_lu* s, double* rhs);
+int nsa_lu_solve_transposed(nsa_lu* s, double* rhs);
+const char* nsa_lu_name(nsa_lu* s);
+/* Pivot order and fill: n, nnz(L+U), levels of refactor, levels of solve. */
+int nsa_lu_info(nsa_lu* s, int32_t out[4]);
+/* Pivot-space structure: row_perm (n), col_perm (n), seg_ptr (n+1), rows (nnz(L+U)). */
+int nsa_lu_structure(nsa_lu* s, int32_t* row_perm, int32_t* col_perm, int32_t* seg_ptr,
+                     int32_t* rows);
+
+/* ---- host-only pivot computations (no GPU needed) ----
+ * The pivot order and fill pattern a fresh numeric factorization of
+ * `values` fixes: engine 1 = Sparse 1.3 Markowitz (NeoSolver::numeric,
+ * src/core/neo_solver.cpp:148-174), engine 2 = min-degree + Gilbert-Peierls
+ * (AmdLuSolver::symbolic + factor_, src/core/amd_lu_solver.cpp:11-197).
+ * Pivot space k = 0..n-1: b'[k] = b[row_perm[k]], x[col_perm[k]] = y[k];
+ * column k holds nU[k] U rows (processing order), the diagonal, then L rows,
+ * at rows[seg_ptr[k] .. seg_ptr[k+1]).  Pass rows = NULL to get *nnz_lu only.
+ * Returns 1 when the matrix is singular. */
+int nsa_pivot_structure(int32_t engine, int32_t n, const int32_t* col_ptr, const int32_t* row_idx,
+                        const double* values, double diag_gmin, int32_t* nnz_lu,
+                        int32_t* row_perm, int32_t* col_perm, int32_t* seg_ptr, int32_t* nU,
+                        int32_t* rows);
+/* amd_ordering (src/core/amd.cpp:14-143). */
+int nsa_min_degree_order(int32_t n, const int32_t* col_ptr, const int32_t* row_idx, int32_t* perm);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,387 @@
+"""neospice_amd: the neospice Newton hot path on MI355X.
+
+Python mirror of the reference's typed Circuit API (src/core/circuit.hpp:
+179-226) and of the analyses that drive Newton (DC operating point,
+transient), over the C ABI of libneospice_amd.so.  Batched: one circuit
+topology, B instances with per-instance device values.
+
+    ckt = Circuit()
+    a, b = ckt.node("a"), ckt.node("b")
+    ckt.V("v1", a, GND, 1.0); ckt.R("r1", a, b, 1e3); ckt.C("c1", b, GND, 1e-9)
+    eng = BatchEngine(ckt, batch=256)
+    x = eng.dc_op()                       # (n, B)
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import Ctx, Options, check, f64p, i32p, as_f64, as_i32, lib
+
+GND = -1
+
+# CKTmode bits (ngspice cktdefs.h; src/core/newton.cpp:109-115)
+MODETRAN, MODEAC, MODEDCOP, MODETRANOP, MODEDCTRANCURVE = 0x1, 0x2, 0x10, 0x20, 0x40
+MODEINITFLOAT, MODEINITJCT, MODEINITFIX = 0x100, 0x200, 0x400
+MODEINITSMSIG, MODEINITTRAN, MODEINITPRED, MODEUIC = 0x800, 0x1000, 0x2000, 0x10000
+
+ENGINES = {"auto": 0, "markowitz": 1, "amdlu": 2}
+
+
+def default_options(**kw):
+    o = Options()
+    lib().nsa_default_options(C.byref(o))
+    for k, v in kw.items():
+        if k == "engine" and isinstance(v, str):
+            v = ENGINES[v]
+        if k == "method" and isinstance(v, str):
+            v = 1 if v == "gear" else 0
+        setattr(o, k, v)
+    return o
+
+
+class Circuit:
+    """Typed circuit builder (records the calls; builds the native circuit on finalize)."""
+
+    def __init__(self):
+        self.ops = []
+        self.nodes = {}
+        self.devices = []     # (kind, name)
+        self._native = None
+
+    # -- builders (names and argument order follow circuit.hpp:179-226) --
+    def node(self, name):
+        if name in ("0", "gnd", "GND"):
+            return GND
+        if name not in self.nodes:
+            self.nodes[name] = len(self.nodes)
+            self.ops.append(("node", name))
+        return self.nodes[name]
+
+    def _dev(self, kind, name, *args):
+        self._check_open()
+        self.ops.append((kind, name) + tuple(args))
+        self.devices.append((kind, name))
+        return len(self.devices) - 1
+
+    def R(self, name, a, b, ohms):
+        return self._dev("R", name, a, b, float(ohms))
+
+    def C(self, name, a, b, farads):
+        return self._dev("C", name, a, b, float(farads))
+
+    def V(self, name, p, n, dc):
+        return self._dev("V", name, p, n, float(dc))
+
+    def V_pulse(self, name, p, n, v1, v2, td=0.0, tr=-1.0, tf=-1.0, pw=-1.0, per=-1.0):
+        return self._dev("Vpulse", name, p, n, (v1, v2, td, tr, tf, pw, per))
+
+    def V_pwl(self, name, p, n, times, values):
+        return self._dev("Vpwl", name, p, n, tuple(map(float, times)), tuple(map(float, values)))
+
+    def I(self, name, p, n, dc):
+        return self._dev("I", name, p, n, float(dc))
+
+    def model(self, text):
+        self._check_open()
+        self.ops.append(("model", text))
+
+    def D(self, name, a, k, model):
+        return self._dev("D", name, a, k, model)
+
+    def M(self, name, d, g, s, b, model, w, l):
+        return self._dev("M", name, d, g, s, b, model, float(w), float(l))
+
+    def _check_open(self):
+        if self._native is not None:
+            raise RuntimeError("Cannot add to a finalized circuit")
+
+    # -- native --
+    def finalize(self):
+        if self._native is not None:
+            return self
+        L = lib()
+        h = L.nsa_circuit_create()
+        self._native = h
+        idx = C.c_int32()
+        dev = C.c_int32()
+        for op in self.ops:
+            k = op[0]
+            if k == "node":
+                check(L.nsa_circuit_node(h, op[1].encode(), C.byref(idx)))
+            elif k == "model":
+                check(L.nsa_circuit_model(h, op[1].encode()))
+            elif k in ("R", "C", "V", "I"):
+                f = {"R": L.nsa_circuit_r, "C": L.nsa_circuit_c, "V": L.nsa_circuit_v,
+                     "I": L.nsa_circuit_i}[k]
+                check(f(h, op[1].encode(), op[2], op[3], op[4], C.byref(dev)))
+            elif k == "Vpulse":
+                p = as_f64(op[4])
+                check(L.nsa_circuit_v_pulse(h, op[1].encode(), op[2], op[3], f64p(p), C.byref(dev)))
+            elif k == "Vpwl":
+                t, v = as_f64(op[4]), as_f64(op[5])
+                check(L.nsa_circuit_v_pwl(h, op[1].encode(), op[2], op[3], len(t), f64p(t), f64p(v),
+                                          C.byref(dev)))
+            elif k == "D":
+                check(L.nsa_circuit_d(h, op[1].encode(), op[2], op[3], op[4].encode(), C.byref(dev)))
+            elif k == "M":
+                check(L.nsa_circuit_m(h, op[1].encode(), op[2], op[3], op[4], op[5], op[6].encode(),
+                                      op[7], op[8], C.byref(dev)))
+        check(L.nsa_circuit_finalize(h))
+        info = np.zeros(5, np.int32)
+        check(L.nsa_circuit_info(h, i32p(info)))
+        self.n, self.num_nodes, self.nnz, self.num_states, linear = map(int, info)
+        self.is_linear = bool(linear)
+        self.col_ptr = np.zeros(self.n + 1, np.int32)
+        self.row_idx = np.zeros(self.nnz, np.int32)
+        check(L.nsa_circuit_pattern(h, i32p(self.col_ptr), i32p(self.row_idx)))
+        self.load_order = np.zeros(len(self.devices), np.int32)
+        check(L.nsa_circuit_load_order(h, i32p(self.load_order)))
+        self.branch = np.zeros(len(self.devices), np.int32)
+        for d in range(len(self.devices)):
+            check(L.nsa_circuit_branch(h, d, C.byref(dev)))
+            self.branch[d] = dev.value
+        return self
+
+    @property
+    def native(self):
+        if self._native is None:
+            self.finalize()
+        return self._native
+
+    def device_index(self, name):
+        for i, (_, nm) in enumerate(self.devices):
+            if nm == name:
+                return i
+        raise KeyError(name)
+
+    def __del__(self):
+        if getattr(self, "_native", None) is not None:
+            try:
+                lib().nsa_circuit_destroy(self._native)
+            except Exception:
+                pass
+
+
+def make_ctx(mode, order=1, method=0, ag=(0.0, 0.0, 0.0), xmu_ratio=1.0, time=0.0, src_fact=1.0):
+    c = Ctx()
+    c.mode, c.order, c.method = int(mode), int(order), int(method)
+    c.ag0, c.ag1, c.ag2 = map(float, ag)
+    c.xmu_ratio, c.time, c.src_fact = float(xmu_ratio), float(time), float(src_fact)
+    return c
+
+
+def be_ctx(dt, time, first):
+    """Context of a backward-Euler step (fill_integrator_context, transient.cpp:398-447)."""
+    return make_ctx(MODETRAN | (MODEINITTRAN if first else MODEINITPRED), 1, 0,
+                    (1.0 / dt, -1.0 / dt, 0.0), 0.0, time)
+
+
+class BatchEngine:
+    """B instances of one circuit on the GPU (nsa_engine_*)."""
+
+    def __init__(self, circuit, batch=1, **opts):
+        circuit.finalize()
+        self.circuit = circuit
+        self.B = int(batch)
+        self.opts = default_options(**opts)
+        h = C.c_void_p()
+        check(lib().nsa_engine_create(circuit.native, self.B, C.byref(self.opts), C.byref(h)))
+        self.h = h
+        self.n = circuit.n
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().nsa_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_values(self, dev, vals):
+        if isinstance(dev, str):
+            dev = self.circuit.device_index(dev)
+        v = as_f64(np.broadcast_to(np.asarray(vals, np.float64), (self.B,)))
+        check(lib().nsa_engine_set_values(self.h, int(dev), f64p(v)))
+
+    def set_ctx(self, ctx):
+        if isinstance(ctx, Ctx):
+            check(lib().nsa_engine_set_ctx(self.h, C.byref(ctx), 0))
+        else:
+            arr = (Ctx * self.B)(*ctx)
+            check(lib().nsa_engine_set_ctx(self.h, arr, 1))
+
+    def newton(self, itl):
+        conv = np.zeros(self.B, np.int32)
+        its = np.zeros(self.B, np.int32)
+        check(lib().nsa_engine_newton(self.h, int(itl), i32p(conv), i32p(its)))
+        return conv.astype(bool), its
+
+    def solution(self):
+        x = np.zeros((self.n, self.B))
+        check(lib().nsa_engine_get_solution(self.h, f64p(x)))
+        return x
+
+    def set_solution(self, x):
+        x = as_f64(np.broadcast_to(x, (self.n, self.B)) if np.ndim(x) else np.full((self.n, self.B), x))
+        check(lib().nsa_engine_set_solution(self.h, f64p(x)))
+
+    def state(self, which=0):
+        s = np.zeros((self.circuit.num_states, self.B))
+        check(lib().nsa_engine_get_state(self.h, int(which), f64p(s)))
+        return s
+
+    def set_state(self, which, s):
+        s = as_f64(s)
+        check(lib().nsa_engine_set_state(self.h, int(which), f64p(s)))
+
+    def load(self):
+        vals = np.zeros((self.circuit.nnz, self.B))
+        rhs = np.zeros((self.n, self.B))
+        check(lib().nsa_engine_load(self.h, f64p(vals), f64p(rhs)))
+        return vals, rhs
+
+    def solve_transposed(self, b):
+        b = as_f64(np.array(b, dtype=np.float64).reshape(self.n, self.B))
+        check(lib().nsa_engine_solve_transposed(self.h, f64p(b)))
+        return b
+
+    def seed_states(self):
+        check(lib().nsa_engine_seed_states(self.h))
+
+    def rotate_states(self):
+        check(lib().nsa_engine_rotate_states(self.h))
+
+    def reset_solver(self):
+        check(lib().nsa_engine_reset_solver(self.h))
+
+    @property
+    def solver_name(self):
+        return lib().nsa_engine_solver_name(self.h).decode()
+
+    def stats(self):
+        s = np.zeros(8, np.int64)
+        check(lib().nsa_engine_stats(self.h, s.ctypes.data_as(_lib.P_I64)))
+        return {"cpu_pivot": int(s[0]), "gpu_refactor": int(s[1]), "nnz_lu": int(s[2]),
+                "levels_refactor": int(s[3]), "levels_solve": int(s[4]), "gpu_solve": int(s[5]),
+                "loads": int(s[6]), "pivot_groups": int(s[7])}
+
+    def set_timing(self, on=True):
+        check(lib().nsa_engine_set_timing(self.h, int(bool(on))))
+
+    def timing(self):
+        """Accumulated HIP-event milliseconds: load+gather, refactor, solve."""
+        t = np.zeros(3)
+        check(lib().nsa_engine_timing(self.h, f64p(t)))
+        return {"load_ms": float(t[0]), "refactor_ms": float(t[1]), "solve_ms": float(t[2])}
+
+    # -- analyses --
+    def dc_op(self, mode=MODEDCOP | MODEINITJCT, x0=0.0):
+        """solve_dc's direct Newton attempt (dc.cpp:104-107)."""
+        self.set_solution(x0)
+        self.set_ctx(make_ctx(mode, 1, int(self.opts.method), (0, 0, 0), 1.0, 0.0,
+                              self.opts.src_fact))
+        conv, its = self.newton(self.opts.itl1)
+        if not conv.all():
+            raise RuntimeError("DC operating point failed to converge for instances %s"
+                               % np.nonzero(~conv)[0][:10].tolist())
+        return self.solution(), its
+
+    def tran_begin(self, x0=0.0):
+        """DC preamble of solve_transient (transient.cpp:710-802): TRANOP Newton on
+        its own solver, a fresh solver for the transient, state1..3 = state0."""
+        x, its = self.dc_op(MODETRANOP | MODEINITJCT, x0)
+        self.reset_solver()
+        self.seed_states()
+        return x, its
+
+    def tran_step(self, ctx, itl=None):
+        """One transient Newton solve; the caller accepts (rotate) or rejects."""
+        self.set_ctx(ctx)
+        return self.newton(self.opts.itl4 if itl is None else itl)
+
+
+class Solver:
+    """The ISolver seam (nsa_lu_*) for a batch of matrices on one pattern."""
+
+    def __init__(self, n, col_ptr, row_idx, batch=1, is_linear=True, engine="auto"):
+        self.n, self.B = int(n), int(batch)
+        self.nnz = int(col_ptr[n])
+        cp, ri = as_i32(col_ptr), as_i32(row_idx)
+        h = C.c_void_p()
+        check(lib().nsa_lu_create(self.B, self.n, i32p(cp), i32p(ri), int(bool(is_linear)),
+                                  ENGINES[engine] if isinstance(engine, str) else int(engine),
+                                  C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().nsa_lu_destroy(self.h)
+            self.h = None
+
+    def _vals(self, values):
+        return as_f64(np.asarray(values, np.float64).reshape(self.nnz, self.B))
+
+    def numeric(self, values, diag_gmin=0.0):
+        st = np.zeros(self.B, np.int32)
+        check(lib().nsa_lu_numeric(self.h, f64p(self._vals(values)), float(diag_gmin), i32p(st)))
+        return st.astype(bool)
+
+    def refactorize(self, values, diag_gmin=0.0):
+        st = np.zeros(self.B, np.int32)
+        check(lib().nsa_lu_refactorize(self.h, f64p(self._vals(values)), float(diag_gmin), i32p(st)))
+        return st.astype(bool)
+
+    def solve(self, rhs):
+        b = as_f64(np.array(rhs, np.float64).reshape(self.n, self.B))
+        check(lib().nsa_lu_solve(self.h, f64p(b)))
+        return b
+
+    def solve_transposed(self, rhs):
+        b = as_f64(np.array(rhs, np.float64).reshape(self.n, self.B))
+        check(lib().nsa_lu_solve_transposed(self.h, f64p(b)))
+        return b
+
+    @property
+    def name(self):
+        return lib().nsa_lu_name(self.h).decode()
+
+    def info(self):
+        o = np.zeros(4, np.int32)
+        check(lib().nsa_lu_info(self.h, i32p(o)))
+        return {"n": int(o[0]), "nnz_lu": int(o[1]), "levels_refactor": int(o[2]),
+                "levels_solve": int(o[3])}
+
+    def structure(self):
+        info = self.info()
+        rp = np.zeros(self.n, np.int32)
+        cp = np.zeros(self.n, np.int32)
+        seg = np.zeros(self.n + 1, np.int32)
+        rows = np.zeros(info["nnz_lu"], np.int32)
+        check(lib().nsa_lu_structure(self.h, i32p(rp), i32p(cp), i32p(seg), i32p(rows)))
+        return {"row_perm": rp, "col_perm": cp, "seg_ptr": seg, "rows": rows}
+
+
+def pivot_structure(engine, n, col_ptr, row_idx, values, diag_gmin=0.0):
+    """Host-only: pivot order and fill of a fresh factorization (nsa_pivot_structure).
+    Returns None when singular."""
+    eng = ENGINES[engine] if isinstance(engine, str) else int(engine)
+    cp, ri, v = as_i32(col_ptr), as_i32(row_idx), as_f64(values)
+    nlu = C.c_int32()
+    rc = check(lib().nsa_pivot_structure(eng, n, i32p(cp), i32p(ri), f64p(v), float(diag_gmin),
+                                         C.byref(nlu), None, None, None, None, None))
+    if rc == 1:
+        return None
+    out = {"row_perm": np.zeros(n, np.int32), "col_perm": np.zeros(n, np.int32),
+           "seg_ptr": np.zeros(n + 1, np.int32), "nU": np.zeros(n, np.int32),
+           "rows": np.zeros(nlu.value, np.int32)}
+    check(lib().nsa_pivot_structure(eng, n, i32p(cp), i32p(ri), f64p(v), float(diag_gmin),
+                                    C.byref(nlu), i32p(out["row_perm"]), i32p(out["col_perm"]),
+                                    i32p(out["seg_ptr"]), i32p(out["nU"]), i32p(out["rows"])))
+    return out
+
+
+def min_degree_order(n, col_ptr, row_idx):
+    perm = np.zeros(n, np.int32)
+    check(lib().nsa_min_degree_order(n, i32p(as_i32(col_ptr)), i32p(as_i32(row_idx)), i32p(perm)))
+    return perm

@@ -1,0 +1,292 @@
+#include "batch_lu.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+#include "common.hpp"
+#include "gpu_dev.hpp"
+
+namespace nsa {
+
+// make_solver.cpp:37 -- AMD-LU is auto-selected for linear circuits with at
+// least this many unknowns.
+static constexpr int kAmdLuAutoThreshold = 256;
+
+// Per-instance call kinds handed to the refactor kernel.
+enum : int32_t { KIND_REFACTOR = 0, KIND_VERIFY = 1, KIND_TRUSTED = 2, KIND_SKIP = 3 };
+
+BatchLu::BatchLu(int batch, hipStream_t stream) : B_(batch), stream_(stream) {
+    if (batch <= 0) throw Error(NSA_ERR_ARGUMENT, "BatchLu: batch must be positive");
+    NSA_HIP(hipMalloc(&d_fail_, sizeof(int32_t) * B_));
+    NSA_HIP(hipMalloc(&d_fail_col_, sizeof(int32_t) * B_));
+    NSA_HIP(hipMalloc(&d_kind_, sizeof(int32_t) * B_));
+    h_kind_.assign(B_, -1);
+    group_of_.assign(B_, -1);
+}
+
+BatchLu::~BatchLu() {
+    groups_.clear();
+    for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
+    (void)hipFree(d_fail_);
+    (void)hipFree(d_fail_col_);
+    (void)hipFree(d_kind_);
+}
+
+void BatchLu::symbolic(int n, const int32_t* col_ptr, const int32_t* row_idx, bool is_linear,
+                       int engine) {
+    n_ = n;
+    acp_.assign(col_ptr, col_ptr + n + 1);
+    ari_.assign(row_idx, row_idx + col_ptr[n]);
+    groups_.clear();
+    group_of_.assign(B_, -1);
+    factored_ = false;
+    committed_ = false;
+    auto_amd_ = false;
+    if (engine == ENGINE_AMDLU) {
+        form_ = FORM_GP;
+    } else if (engine == ENGINE_MARKOWITZ) {
+        form_ = FORM_S13;
+    } else {
+        form_ = (n >= kAmdLuAutoThreshold && is_linear) ? FORM_GP : FORM_S13;
+        auto_amd_ = form_ == FORM_GP;
+    }
+    if (form_ == FORM_GP) {
+        q_ = min_degree_order(n, acp_.data(), ari_.data());
+        if ((int)q_.size() != n) {
+            q_.resize(n);
+            for (int i = 0; i < n; ++i) q_[i] = i;
+        }
+    }
+}
+
+std::vector<double> BatchLu::instance_values(const double* dA, int inst) {
+    std::vector<double> v(acp_[n_]);
+    NSA_HIP(hipMemcpy2DAsync(v.data(), sizeof(double), dA + inst, sizeof(double) * B_,
+                             sizeof(double), v.size(), hipMemcpyDeviceToHost, stream_));
+    NSA_HIP(hipStreamSynchronize(stream_));
+    return v;
+}
+
+// CPU pivot computation for one instance's values.  Markowitz continues
+// from `from` (the state of the instance's previous numeric(), reuse path)
+// or starts fresh; AMD-LU always re-pivots from scratch (factor_).  Returns
+// true when singular.
+bool BatchLu::cpu_order(const std::vector<double>& vals, double gmin, const Sparse13* from,
+                        LuStructure& out, std::unique_ptr<Sparse13>& state) {
+    ++n_cpu_pivot;
+    if (form_ == FORM_S13) {
+        state = from ? from->clone() : std::make_unique<Sparse13>(n_, acp_.data(), ari_.data());
+        if (state->numeric(vals.data(), gmin) != 0) return true;
+        out = state->structure();
+        return false;
+    }
+    state.reset();
+    return !gp_factor_structure(n_, acp_.data(), ari_.data(), vals.data(), gmin, q_, out);
+}
+
+int BatchLu::find_group(const LuStructure& s) const {
+    for (size_t g = 0; g < groups_.size(); ++g)
+        if (groups_[g].lu && groups_[g].lu->structure().same_as(s)) return (int)g;
+    return -1;
+}
+
+void BatchLu::rebuild_group(int g) {
+    Group& G = groups_[g];
+    LuStructure s = G.lu->structure();
+    std::sort(G.inst.begin(), G.inst.end());
+    const bool all = (int)G.inst.size() == B_;
+    G.lu.reset();
+    G.lu = std::make_unique<GpuLu>(s, (int)G.inst.size(), stream_, all ? nullptr : &G.inst, B_);
+}
+
+void BatchLu::run_gpu(const double* dA, double gmin, const std::vector<int32_t>& kind,
+                      const std::vector<int>& which, std::vector<int32_t>& fail) {
+    launch_fill_i32(d_fail_, FAIL_NONE, B_, stream_);
+    launch_fill_i32(d_fail_col_, 0x7fffffff, B_, stream_);
+    if (kind != h_kind_) {
+        h_kind_ = kind;
+        NSA_HIP(hipMemcpyAsync(d_kind_, h_kind_.data(), sizeof(int32_t) * B_,
+                               hipMemcpyHostToDevice, stream_));
+    }
+    if (timing_) NSA_HIP(hipEventRecord(ev_[0], stream_));
+    for (int g : which) groups_[g].lu->refactor(dA, gmin, d_kind_, d_fail_, d_fail_col_);
+    if (timing_) { NSA_HIP(hipEventRecord(ev_[1], stream_)); pend_rf_ = true; }
+    ++n_gpu_refactor;
+    fail.resize(B_);
+    NSA_HIP(hipMemcpyAsync(fail.data(), d_fail_, sizeof(int32_t) * B_, hipMemcpyDeviceToHost,
+                           stream_));
+    NSA_HIP(hipStreamSynchronize(stream_));
+    collect_timing();
+}
+
+int BatchLu::factor(const double* dA, double gmin, const int32_t* numeric_mask, int32_t* status) {
+    if (acp_.empty()) throw Error(NSA_ERR_LOGIC, "BatchLu: symbolic() not called");
+    std::vector<int32_t> kind(B_, KIND_VERIFY);
+    if (numeric_mask)
+        for (int i = 0; i < B_; ++i) kind[i] = numeric_mask[i] ? KIND_VERIFY : KIND_REFACTOR;
+    std::fill(status, status + B_, 0);
+    if (groups_.empty()) {
+        // First factorization.  A refactorize() on a fresh solver is a
+        // numeric(): NeoSolver throws and newton_solve retries with numeric()
+        // (newton.cpp:235-242); AmdLuSolver runs factor_ (amd_lu_solver.cpp:390-404).
+        std::fill(kind.begin(), kind.end(), KIND_VERIFY);
+        auto vals = instance_values(dA, 0);
+        LuStructure s;
+        std::unique_ptr<Sparse13> st;
+        bool sing = cpu_order(vals, gmin, nullptr, s, st);
+        if (sing && auto_amd_ && !committed_) {
+            // AutoFallbackSolver (make_solver.cpp:85-101): a singular first
+            // AMD-LU factorization switches to Markowitz.
+            form_ = FORM_S13;
+            auto_amd_ = false;
+            sing = cpu_order(vals, gmin, nullptr, s, st);
+        }
+        committed_ = true;
+        if (sing) {
+            std::fill(status, status + B_, 1);
+            return B_;
+        }
+        Group G;
+        G.inst.resize(B_);
+        for (int i = 0; i < B_; ++i) { G.inst[i] = i; group_of_[i] = 0; }
+        G.lu = std::make_unique<GpuLu>(s, B_, stream_);
+        G.s13 = std::move(st);
+        groups_.push_back(std::move(G));
+        kind[0] = KIND_TRUSTED;  // its order was just computed from its own values
+    }
+    std::vector<int> all_groups(groups_.size());
+    for (size_t g = 0; g < groups_.size(); ++g) all_groups[g] = (int)g;
+    std::vector<int32_t> fail;
+    run_gpu(dA, gmin, kind, all_groups, fail);
+    int nsing = 0;
+    std::vector<char> changed(groups_.size(), 0);
+    std::vector<int32_t> rerun_kind(B_, KIND_SKIP);
+    bool moved = false;
+    for (int i = 0; i < B_; ++i) {
+        const int32_t f = fail[i];
+        if (f == FAIL_NONE) continue;
+        const int k = kind[i];
+        // Sparse 1.3 refactorize: exact zero pivot is reported as singular
+        // (factor.cpp:997, 1025, 1045); newton_solve then asks for numeric().
+        if (form_ == FORM_S13 && k == KIND_REFACTOR) { status[i] = 1; ++nsing; continue; }
+        // A fresh AMD-LU factorization would be singular, or an exact zero
+        // on an order just computed for these values.
+        if ((form_ == FORM_GP && k == KIND_VERIFY && f == FAIL_ZERO_PIVOT) || k == KIND_TRUSTED) {
+            status[i] = 1;
+            ++nsing;
+            continue;
+        }
+        // numeric() whose order no longer holds, or an AMD-LU replay rejected
+        // by its growth check (answered with a full factor_, amd_lu_solver.cpp:395-404):
+        // re-pivot this instance on the CPU from its own values.
+        const int g = group_of_[i];
+        auto vals = instance_values(dA, i);
+        LuStructure s;
+        std::unique_ptr<Sparse13> st;
+        if (cpu_order(vals, gmin, groups_[g].s13.get(), s, st)) {
+            status[i] = 1;
+            ++nsing;
+            continue;
+        }
+        if (groups_[g].lu->structure().same_as(s)) {
+            // Same order: the GPU values are the re-pivoting factorization's.
+            continue;
+        }
+        int dst = find_group(s);
+        if (dst < 0) {
+            Group G;
+            std::vector<int32_t> one{i};
+            G.lu = std::make_unique<GpuLu>(s, 1, stream_, &one, B_);
+            G.s13 = std::move(st);
+            groups_.push_back(std::move(G));
+            changed.push_back(1);
+            dst = (int)groups_.size() - 1;
+        }
+        auto& src_inst = groups_[g].inst;
+        src_inst.erase(std::find(src_inst.begin(), src_inst.end(), i));
+        groups_[dst].inst.push_back(i);
+        group_of_[i] = dst;
+        changed[g] = 1;
+        changed[dst] = 1;
+        rerun_kind[i] = KIND_TRUSTED;
+        ++n_regroup;
+        moved = true;
+    }
+    if (moved) {
+        // Drop emptied groups, rebuild the plans whose membership changed and
+        // recompute their members: moved instances on their new order, the
+        // others exactly as before (same order, same values).
+        std::vector<Group> keep;
+        std::vector<char> keep_changed;
+        for (size_t g = 0; g < groups_.size(); ++g) {
+            if (groups_[g].inst.empty()) continue;
+            keep.push_back(std::move(groups_[g]));
+            keep_changed.push_back(changed[g]);
+        }
+        groups_ = std::move(keep);
+        std::vector<int> redo;
+        for (size_t g = 0; g < groups_.size(); ++g) {
+            for (int32_t i : groups_[g].inst) group_of_[i] = (int)g;
+            if (keep_changed[g]) { rebuild_group((int)g); redo.push_back((int)g); }
+        }
+        for (int g : redo)
+            for (int32_t i : groups_[g].inst)
+                if (rerun_kind[i] == KIND_SKIP && !status[i]) rerun_kind[i] = KIND_TRUSTED;
+        run_gpu(dA, gmin, rerun_kind, redo, fail);
+        for (int i = 0; i < B_; ++i)
+            if (rerun_kind[i] != KIND_SKIP && fail[i] != FAIL_NONE && !status[i]) {
+                status[i] = 1;
+                ++nsing;
+            }
+    }
+    factored_ = true;
+    return nsing;
+}
+
+int BatchLu::refactorize(const double* dA, double gmin, int32_t* status) {
+    // NeoSolver::refactorize before numeric() is a logic_error
+    // (neo_solver.cpp:178-180); AmdLuSolver falls through to factor_.
+    if (groups_.empty() && form_ == FORM_S13)
+        throw Error(NSA_ERR_LOGIC, "refactorize: numeric() not called");
+    std::vector<int32_t> mask(B_, 0);
+    return factor(dA, gmin, mask.data(), status);
+}
+
+void BatchLu::solve(double* db) {
+    if (!factored()) throw Error(NSA_ERR_LOGIC, "BatchLu::solve: numeric() not called");
+    if (timing_) NSA_HIP(hipEventRecord(ev_[2], stream_));
+    for (auto& G : groups_) G.lu->solve(db, false);
+    if (timing_) { NSA_HIP(hipEventRecord(ev_[3], stream_)); pend_sv_ = true; }
+    ++n_gpu_solve;
+}
+
+void BatchLu::solve_t(double* db) {
+    if (!factored()) throw Error(NSA_ERR_LOGIC, "BatchLu::solve_t: numeric() not called");
+    for (auto& G : groups_) G.lu->solve(db, true);
+}
+
+void BatchLu::set_timing(bool on) {
+    if (on && !ev_[0])
+        for (auto& e : ev_) NSA_HIP(hipEventCreate(&e));
+    timing_ = on;
+    ms_refactor = ms_solve = 0.0;
+    pend_rf_ = pend_sv_ = false;
+}
+
+// Reads the elapsed time of the last recorded refactor / solve once the
+// stream has passed them (callers synchronize first).
+void BatchLu::collect_timing() {
+    float ms = 0.f;
+    if (pend_rf_) {
+        NSA_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+        ms_refactor += ms;
+        pend_rf_ = false;
+    }
+    if (pend_sv_) {
+        NSA_HIP(hipEventElapsedTime(&ms, ev_[2], ev_[3]));
+        ms_solve += ms;
+        pend_sv_ = false;
+    }
+}
+
+}  // namespace nsa

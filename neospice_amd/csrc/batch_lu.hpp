@@ -1,0 +1,105 @@
+// The solver seam: the reference's ISolver contract (src/core/solver_iface.hpp:16-41)
+// for a batch of B instances that share one sparsity pattern.
+//
+//   symbolic(pattern)       -> choose the engine like make_solver
+//                              (make_solver.cpp:156-176) and keep the pattern
+//   numeric(A, gmin)        -> pivot order fixed on the CPU from real values,
+//                              factors computed on the GPU; later calls
+//                              verify on the GPU that the order still holds
+//   refactorize(A, gmin)    -> GPU refactor on the fixed order
+//   solve(b) / solve_t(b)   -> GPU forward/back substitution (in place)
+//
+// Every instance behaves like its own reference solver: pivot orders are
+// value-dependent, so instances are partitioned into structure groups, one
+// GPU plan per distinct pivot order.  An instance whose order changes (a
+// failed threshold check on numeric(), or a rejected AMD-LU replay) is
+// re-pivoted on the CPU from its own values and moved to the group of its
+// new order.
+//
+// A and b live in device memory, instance-interleaved (v[i * B + inst]).
+// Per-instance outcomes are reported in status[] (0 ok, 1 singular).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <vector>
+
+#include "gpu_lu.hpp"
+#include "lu_cpu.hpp"
+#include "lu_structure.hpp"
+
+namespace nsa {
+
+enum Engine : int32_t { ENGINE_AUTO = 0, ENGINE_MARKOWITZ = 1, ENGINE_AMDLU = 2 };
+
+class BatchLu {
+public:
+    BatchLu(int batch, hipStream_t stream);
+    ~BatchLu();
+
+    void symbolic(int n, const int32_t* col_ptr, const int32_t* row_idx, bool is_linear,
+                  int engine = ENGINE_AUTO);
+    // mask (host, B entries, may be null = all): instances whose call is a
+    // numeric(); the others are refactorize().  Returns the number of
+    // singular instances; status[inst] = 1 for each.
+    int factor(const double* dA, double diag_gmin, const int32_t* numeric_mask, int32_t* status);
+    int numeric(const double* dA, double diag_gmin, int32_t* status) {
+        return factor(dA, diag_gmin, nullptr, status);
+    }
+    int refactorize(const double* dA, double diag_gmin, int32_t* status);
+    void solve(double* db);
+    void solve_t(double* db);
+
+    const char* name() const { return form_ == FORM_S13 ? "markowitz" : "amdlu"; }
+    bool factored() const { return !groups_.empty() && factored_; }
+    int num_groups() const { return (int)groups_.size(); }
+    // The plan of structure group g (group 0 holds the first instance's order).
+    GpuLu* gpu(int g = 0) { return groups_.empty() ? nullptr : groups_[g].lu.get(); }
+    const LuStructure* structure(int g = 0) const {
+        return groups_.empty() ? nullptr : &groups_[g].lu->structure();
+    }
+    int group_of(int inst) const { return group_of_[inst]; }
+    int n() const { return n_; }
+    int batch() const { return B_; }
+    // counters for tests and the bench
+    long n_cpu_pivot = 0;     // CPU pivot computations
+    long n_gpu_refactor = 0;  // GPU refactors of the batch (each = all levels of every group)
+    long n_gpu_solve = 0;
+    long n_regroup = 0;       // instances moved to another pivot order
+    // HIP-event timing of the GPU refactor / solve on this object's stream
+    // (enabled by set_timing); accumulated milliseconds.
+    void set_timing(bool on);
+    void collect_timing();
+    double ms_refactor = 0.0, ms_solve = 0.0;
+
+private:
+    struct Group {
+        std::unique_ptr<GpuLu> lu;
+        std::vector<int32_t> inst;
+        std::unique_ptr<Sparse13> s13;  // Markowitz: CPU state after the numeric() that fixed the order
+    };
+    std::vector<double> instance_values(const double* dA, int inst);
+    bool cpu_order(const std::vector<double>& vals, double gmin, const Sparse13* from,
+                   LuStructure& out, std::unique_ptr<Sparse13>& state);
+    int find_group(const LuStructure& s) const;
+    void rebuild_group(int g);
+    void run_gpu(const double* dA, double gmin, const std::vector<int32_t>& kind,
+                 const std::vector<int>& which, std::vector<int32_t>& fail);
+
+    int B_;
+    hipStream_t stream_;
+    int n_ = 0;
+    std::vector<int32_t> acp_, ari_;
+    LuForm form_ = FORM_S13;
+    bool auto_amd_ = false, committed_ = false;
+    std::vector<int32_t> q_;
+    std::vector<Group> groups_;
+    std::vector<int32_t> group_of_;
+    bool factored_ = false;
+    int32_t *d_fail_ = nullptr, *d_fail_col_ = nullptr, *d_kind_ = nullptr;
+    std::vector<int32_t> h_kind_;
+    bool timing_ = false, pend_rf_ = false, pend_sv_ = false;
+    hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+}  // namespace nsa

@@ -1,0 +1,382 @@
+// extern "C" boundary (include/neospice_amd.h).  Every entry point catches
+// C++ exceptions and turns them into status codes.
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/neospice_amd.h"
+#include "batch_lu.hpp"
+#include "circuit.hpp"
+#include "common.hpp"
+#include "engine.hpp"
+#include "lu_cpu.hpp"
+
+struct nsa_circuit {
+    nsa::Circuit c;
+};
+
+struct nsa_engine {
+    nsa_circuit* ckt;
+    std::unique_ptr<nsa::BatchEngine> e;
+};
+
+struct nsa_lu {
+    hipStream_t stream = nullptr;
+    std::unique_ptr<nsa::BatchLu> lu;
+    double* dA = nullptr;
+    double* db = nullptr;
+    int n = 0, nnz = 0, B = 0;
+    ~nsa_lu() {
+        lu.reset();
+        if (dA) (void)hipFree(dA);
+        if (db) (void)hipFree(db);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <class F>
+int guard(F&& f) {
+    try {
+        return f();
+    } catch (const nsa::Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::invalid_argument& e) {
+        g_err = e.what();
+        return nsa::NSA_ERR_ARGUMENT;
+    } catch (const std::logic_error& e) {
+        g_err = e.what();
+        return nsa::NSA_ERR_LOGIC;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nsa::NSA_ERR_RUNTIME;
+    }
+}
+
+nsa::Options to_opts(const nsa_options* o) {
+    nsa::Options r;
+    if (!o) return r;
+    r.abstol = o->abstol; r.reltol = o->reltol; r.vntol = o->vntol; r.trtol = o->trtol;
+    r.chgtol = o->chgtol; r.gmin = o->gmin; r.diag_gmin = o->diag_gmin; r.xmu = o->xmu;
+    r.src_fact = o->src_fact; r.itl1 = o->itl1; r.itl4 = o->itl4; r.method = o->method;
+    r.engine = o->engine;
+    return r;
+}
+
+int set_dev(int32_t* dev, int32_t v) {
+    if (dev) *dev = v;
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+const char* nsa_last_error(void) { return g_err.c_str(); }
+
+void nsa_default_options(nsa_options* o) {
+    nsa::Options d;
+    o->abstol = d.abstol; o->reltol = d.reltol; o->vntol = d.vntol; o->trtol = d.trtol;
+    o->chgtol = d.chgtol; o->gmin = d.gmin; o->diag_gmin = d.diag_gmin; o->xmu = d.xmu;
+    o->src_fact = d.src_fact; o->itl1 = d.itl1; o->itl4 = d.itl4; o->method = d.method;
+    o->engine = d.engine;
+}
+
+nsa_circuit* nsa_circuit_create(void) { return new nsa_circuit(); }
+void nsa_circuit_destroy(nsa_circuit* c) { delete c; }
+
+int nsa_circuit_node(nsa_circuit* c, const char* name, int32_t* index) {
+    return guard([&] { *index = c->c.node(name); return 0; });
+}
+int nsa_circuit_r(nsa_circuit* c, const char* name, int32_t a, int32_t b, double v, int32_t* dev) {
+    return guard([&] { return set_dev(dev, c->c.R(name, a, b, v)); });
+}
+int nsa_circuit_c(nsa_circuit* c, const char* name, int32_t a, int32_t b, double v, int32_t* dev) {
+    return guard([&] { return set_dev(dev, c->c.C(name, a, b, v)); });
+}
+int nsa_circuit_v(nsa_circuit* c, const char* name, int32_t p, int32_t n, double v, int32_t* dev) {
+    return guard([&] { return set_dev(dev, c->c.V(name, p, n, v)); });
+}
+int nsa_circuit_v_pulse(nsa_circuit* c, const char* name, int32_t p, int32_t n,
+                        const double pulse[7], int32_t* dev) {
+    return guard([&] { return set_dev(dev, c->c.V_pulse(name, p, n, pulse)); });
+}
+int nsa_circuit_v_pwl(nsa_circuit* c, const char* name, int32_t p, int32_t n, int32_t npts,
+                      const double* t, const double* v, int32_t* dev) {
+    return guard([&] {
+        return set_dev(dev, c->c.V_pwl(name, p, n, std::vector<double>(t, t + npts),
+                                        std::vector<double>(v, v + npts)));
+    });
+}
+int nsa_circuit_i(nsa_circuit* c, const char* name, int32_t p, int32_t n, double v, int32_t* dev) {
+    return guard([&] { return set_dev(dev, c->c.I(name, p, n, v)); });
+}
+int nsa_circuit_model(nsa_circuit* c, const char* text) {
+    return guard([&] { c->c.model(text); return 0; });
+}
+int nsa_circuit_d(nsa_circuit* c, const char* name, int32_t a, int32_t k, const char* model,
+                  int32_t* dev) {
+    return guard([&] { return set_dev(dev, c->c.D(name, a, k, model)); });
+}
+int nsa_circuit_m(nsa_circuit* c, const char* name, int32_t d, int32_t g, int32_t s, int32_t b,
+                  const char* model, double w, double l, int32_t* dev) {
+    return guard([&] { return set_dev(dev, c->c.M(name, d, g, s, b, model, w, l)); });
+}
+int nsa_circuit_finalize(nsa_circuit* c) {
+    return guard([&] { c->c.finalize(); return 0; });
+}
+int nsa_circuit_info(const nsa_circuit* c, int32_t out[5]) {
+    return guard([&] {
+        if (!c->c.finalized()) throw nsa::Error(nsa::NSA_ERR_LOGIC, "circuit not finalized");
+        out[0] = c->c.num_vars();
+        out[1] = c->c.num_nodes();
+        out[2] = c->c.nnz();
+        out[3] = c->c.num_states();
+        out[4] = c->c.is_linear() ? 1 : 0;
+        return 0;
+    });
+}
+int nsa_circuit_pattern(const nsa_circuit* c, int32_t* col_ptr, int32_t* row_idx) {
+    return guard([&] {
+        std::memcpy(col_ptr, c->c.col_ptr().data(), sizeof(int32_t) * c->c.col_ptr().size());
+        std::memcpy(row_idx, c->c.row_idx().data(), sizeof(int32_t) * c->c.row_idx().size());
+        return 0;
+    });
+}
+int nsa_circuit_load_order(const nsa_circuit* c, int32_t* order) {
+    return guard([&] {
+        std::memcpy(order, c->c.load_order().data(), sizeof(int32_t) * c->c.load_order().size());
+        return 0;
+    });
+}
+int nsa_circuit_branch(const nsa_circuit* c, int32_t dev, int32_t* branch) {
+    return guard([&] { *branch = c->c.devices().at(dev).branch; return 0; });
+}
+
+int nsa_engine_create(nsa_circuit* c, int32_t batch, const nsa_options* opt, nsa_engine** out) {
+    return guard([&] {
+        auto e = std::make_unique<nsa_engine>();
+        e->ckt = c;
+        e->e = std::make_unique<nsa::BatchEngine>(c->c, batch, to_opts(opt));
+        *out = e.release();
+        return 0;
+    });
+}
+void nsa_engine_destroy(nsa_engine* e) { delete e; }
+
+int nsa_engine_set_values(nsa_engine* e, int32_t dev, const double* vals) {
+    return guard([&] { e->e->set_device_values(dev, vals); return 0; });
+}
+int nsa_engine_set_ctx(nsa_engine* e, const nsa_ctx* ctx, int32_t per_instance) {
+    return guard([&] {
+        static_assert(sizeof(nsa_ctx) == sizeof(nsa::DevCtx), "ctx layout");
+        const int B = e->e->batch();
+        std::vector<nsa::DevCtx> v(B);
+        for (int i = 0; i < B; ++i)
+            std::memcpy(&v[i], per_instance ? &ctx[i] : &ctx[0], sizeof(nsa::DevCtx));
+        e->e->set_ctx(v);
+        return 0;
+    });
+}
+int nsa_engine_newton(nsa_engine* e, int32_t itl, int32_t* converged, int32_t* iterations) {
+    return guard([&] {
+        auto r = e->e->newton(itl);
+        const int B = e->e->batch();
+        if (converged) std::memcpy(converged, r.converged.data(), sizeof(int32_t) * B);
+        if (iterations) std::memcpy(iterations, r.iterations.data(), sizeof(int32_t) * B);
+        return 0;
+    });
+}
+int nsa_engine_get_solution(nsa_engine* e, double* x) {
+    return guard([&] { e->e->get_solution(x); return 0; });
+}
+int nsa_engine_set_solution(nsa_engine* e, const double* x) {
+    return guard([&] { e->e->set_solution(x); return 0; });
+}
+int nsa_engine_get_state(nsa_engine* e, int32_t which, double* s) {
+    return guard([&] {
+        if (which < 0 || which > 3) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "state index 0..3");
+        e->e->get_state(which, s);
+        return 0;
+    });
+}
+int nsa_engine_set_state(nsa_engine* e, int32_t which, const double* s) {
+    return guard([&] {
+        if (which < 0 || which > 3) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "state index 0..3");
+        e->e->set_state(which, s);
+        return 0;
+    });
+}
+int nsa_engine_seed_states(nsa_engine* e) {
+    return guard([&] { e->e->seed_states_from_state0(); return 0; });
+}
+int nsa_engine_rotate_states(nsa_engine* e) {
+    return guard([&] { e->e->rotate_states(); return 0; });
+}
+int nsa_engine_reset_solver(nsa_engine* e) {
+    return guard([&] { e->e->reset_solver(); return 0; });
+}
+int nsa_engine_load(nsa_engine* e, double* values, double* rhs) {
+    return guard([&] {
+        e->e->load_only();
+        const auto& c = e->e->circuit();
+        const size_t B = e->e->batch();
+        NSA_HIP(hipMemcpy(values, e->e->d_matrix(), sizeof(double) * c.nnz() * B,
+                          hipMemcpyDeviceToHost));
+        NSA_HIP(hipMemcpy(rhs, e->e->d_rhs(), sizeof(double) * c.num_vars() * B,
+                          hipMemcpyDeviceToHost));
+        return 0;
+    });
+}
+int nsa_engine_solve_transposed(nsa_engine* e, double* b) {
+    return guard([&] {
+        const size_t bytes = sizeof(double) * e->e->n() * e->e->batch();
+        double* d = e->e->d_rhs();
+        NSA_HIP(hipMemcpy(d, b, bytes, hipMemcpyHostToDevice));
+        e->e->lu().solve_t(d);
+        NSA_HIP(hipMemcpy(b, d, bytes, hipMemcpyDeviceToHost));
+        return 0;
+    });
+}
+const char* nsa_engine_solver_name(nsa_engine* e) { return e->e->lu().name(); }
+int nsa_engine_stats(nsa_engine* e, int64_t out[8]) {
+    return guard([&] {
+        auto& lu = e->e->lu();
+        std::memset(out, 0, sizeof(int64_t) * 8);
+        out[0] = lu.n_cpu_pivot;
+        out[1] = lu.n_gpu_refactor;
+        if (lu.structure()) {
+            out[2] = lu.structure()->nlu();
+            out[3] = lu.gpu()->nlevels_refactor();
+            out[4] = lu.gpu()->nlevels_solve();
+        }
+        out[5] = lu.n_gpu_solve;
+        out[6] = e->e->n_loads;
+        out[7] = lu.num_groups();
+        return 0;
+    });
+}
+
+int nsa_engine_set_timing(nsa_engine* e, int32_t on) {
+    return guard([&] { e->e->set_timing(on != 0); return 0; });
+}
+
+int nsa_engine_timing(nsa_engine* e, double out[3]) {
+    return guard([&] {
+        out[0] = e->e->ms_load_gpu;
+        out[1] = e->e->lu().ms_refactor;
+        out[2] = e->e->lu().ms_solve;
+        return 0;
+    });
+}
+
+int nsa_lu_create(int32_t batch, int32_t n, const int32_t* col_ptr, const int32_t* row_idx,
+                  int32_t is_linear, int32_t engine, nsa_lu** out) {
+    return guard([&] {
+        auto s = std::make_unique<nsa_lu>();
+        NSA_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        s->n = n;
+        s->nnz = col_ptr[n];
+        s->B = batch;
+        s->lu = std::make_unique<nsa::BatchLu>(batch, s->stream);
+        s->lu->symbolic(n, col_ptr, row_idx, is_linear != 0, engine);
+        NSA_HIP(hipMalloc(&s->dA, sizeof(double) * std::max(1, s->nnz) * batch));
+        NSA_HIP(hipMalloc(&s->db, sizeof(double) * std::max(1, n) * batch));
+        *out = s.release();
+        return 0;
+    });
+}
+void nsa_lu_destroy(nsa_lu* s) { delete s; }
+
+static int lu_factor(nsa_lu* s, const double* values, double gmin, int32_t* status, bool numeric) {
+    NSA_HIP(hipMemcpyAsync(s->dA, values, sizeof(double) * s->nnz * s->B, hipMemcpyHostToDevice,
+                           s->stream));
+    std::vector<int32_t> st(s->B, 0);
+    const int nsing = numeric ? s->lu->numeric(s->dA, gmin, st.data())
+                              : s->lu->refactorize(s->dA, gmin, st.data());
+    if (status) std::memcpy(status, st.data(), sizeof(int32_t) * s->B);
+    return nsing ? 1 : 0;
+}
+int nsa_lu_numeric(nsa_lu* s, const double* values, double gmin, int32_t* status) {
+    return guard([&] { return lu_factor(s, values, gmin, status, true); });
+}
+int nsa_lu_refactorize(nsa_lu* s, const double* values, double gmin, int32_t* status) {
+    return guard([&] { return lu_factor(s, values, gmin, status, false); });
+}
+static int lu_solve(nsa_lu* s, double* rhs, bool t) {
+    const size_t bytes = sizeof(double) * s->n * s->B;
+    NSA_HIP(hipMemcpyAsync(s->db, rhs, bytes, hipMemcpyHostToDevice, s->stream));
+    if (t) s->lu->solve_t(s->db); else s->lu->solve(s->db);
+    NSA_HIP(hipMemcpyAsync(rhs, s->db, bytes, hipMemcpyDeviceToHost, s->stream));
+    NSA_HIP(hipStreamSynchronize(s->stream));
+    return 0;
+}
+int nsa_lu_solve(nsa_lu* s, double* rhs) {
+    return guard([&] { return lu_solve(s, rhs, false); });
+}
+int nsa_lu_solve_transposed(nsa_lu* s, double* rhs) {
+    return guard([&] { return lu_solve(s, rhs, true); });
+}
+const char* nsa_lu_name(nsa_lu* s) { return s->lu->name(); }
+int nsa_lu_info(nsa_lu* s, int32_t out[4]) {
+    return guard([&] {
+        if (!s->lu->structure()) throw nsa::Error(nsa::NSA_ERR_LOGIC, "numeric() not called");
+        out[0] = s->n;
+        out[1] = s->lu->structure()->nlu();
+        out[2] = s->lu->gpu()->nlevels_refactor();
+        out[3] = s->lu->gpu()->nlevels_solve();
+        return 0;
+    });
+}
+int nsa_lu_structure(nsa_lu* s, int32_t* row_perm, int32_t* col_perm, int32_t* seg_ptr,
+                     int32_t* rows) {
+    return guard([&] {
+        const auto* st = s->lu->structure();
+        if (!st) throw nsa::Error(nsa::NSA_ERR_LOGIC, "numeric() not called");
+        std::memcpy(row_perm, st->row_perm.data(), sizeof(int32_t) * st->n);
+        std::memcpy(col_perm, st->col_perm.data(), sizeof(int32_t) * st->n);
+        std::memcpy(seg_ptr, st->cptr.data(), sizeof(int32_t) * (st->n + 1));
+        std::memcpy(rows, st->rows.data(), sizeof(int32_t) * st->nlu());
+        return 0;
+    });
+}
+
+int nsa_pivot_structure(int32_t engine, int32_t n, const int32_t* col_ptr, const int32_t* row_idx,
+                        const double* values, double diag_gmin, int32_t* nnz_lu,
+                        int32_t* row_perm, int32_t* col_perm, int32_t* seg_ptr, int32_t* nU,
+                        int32_t* rows) {
+    return guard([&] {
+        nsa::LuStructure st;
+        if (engine == nsa::ENGINE_AMDLU) {
+            auto q = nsa::min_degree_order(n, col_ptr, row_idx);
+            if (!nsa::gp_factor_structure(n, col_ptr, row_idx, values, diag_gmin, q, st)) return 1;
+        } else if (engine == nsa::ENGINE_MARKOWITZ) {
+            nsa::Sparse13 s13(n, col_ptr, row_idx);
+            if (s13.numeric(values, diag_gmin) != 0) return 1;
+            st = s13.structure();
+        } else {
+            throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "engine must be 1 (markowitz) or 2 (amdlu)");
+        }
+        if (nnz_lu) *nnz_lu = st.nlu();
+        if (rows) {
+            std::memcpy(row_perm, st.row_perm.data(), sizeof(int32_t) * n);
+            std::memcpy(col_perm, st.col_perm.data(), sizeof(int32_t) * n);
+            std::memcpy(seg_ptr, st.cptr.data(), sizeof(int32_t) * (n + 1));
+            std::memcpy(nU, st.nU.data(), sizeof(int32_t) * n);
+            std::memcpy(rows, st.rows.data(), sizeof(int32_t) * st.nlu());
+        }
+        return 0;
+    });
+}
+
+int nsa_min_degree_order(int32_t n, const int32_t* col_ptr, const int32_t* row_idx, int32_t* perm) {
+    return guard([&] {
+        auto q = nsa::min_degree_order(n, col_ptr, row_idx);
+        std::memcpy(perm, q.data(), sizeof(int32_t) * q.size());
+        return 0;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,292 @@
+// Device loads (one lane per device instance, instance index fastest) and
+// the Newton bookkeeping kernels.  Device kernels write each stamp call's
+// value into its own "site"; the gather then sums the sites of every matrix
+// slot / rhs row in load order, so no atomics are needed and every slot is
+// accumulated in the reference's order (NumericMatrix::add from 0.0).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "common.hpp"
+#include "circuit.hpp"
+#include "gpu_dev.hpp"
+
+namespace nsa {
+
+namespace {
+
+constexpr int TPB = 256;
+
+inline int blocks_for(long work) { return (int)((work + TPB - 1) / TPB); }
+
+__global__ void k_res_sites(int count, const double* __restrict__ ohms,
+                            double* __restrict__ contrib, int site, int B) {
+    const long gid = (long)blockIdx.x * TPB + threadIdx.x;
+    if (gid >= (long)count * B) return;
+    const int d = (int)(gid / B), inst = (int)(gid % B);
+    const size_t sB = (size_t)B;
+    // Resistor::evaluate (resistor.cpp:33-42)
+    const double g = 1.0 / ohms[gid];
+    double* c = contrib + (size_t)(site + 4 * d) * sB + inst;
+    c[0] = g;
+    c[sB] = -g;
+    c[2 * sB] = -g;
+    c[3 * sB] = g;
+}
+
+__global__ void k_const_sites(int count, const double* __restrict__ vals,
+                              double* __restrict__ contrib, int site, int B) {
+    const long gid = (long)blockIdx.x * TPB + threadIdx.x;
+    if (gid >= (long)count * B) return;
+    const int d = (int)(gid / B), inst = (int)(gid % B);
+    contrib[(size_t)(site + d) * B + inst] = vals[d];
+}
+
+// Capacitor::evaluate, state-vector path (capacitor.cpp:28-110).
+__global__ void k_cap(CapTable t, const DevCtx* __restrict__ ctx, const double* __restrict__ x,
+                      double* __restrict__ s0, double* __restrict__ s1,
+                      const double* __restrict__ s2, const int32_t* __restrict__ done,
+                      double* __restrict__ cm, double* __restrict__ cr, int B) {
+    const long gid = (long)blockIdx.x * TPB + threadIdx.x;
+    if (gid >= (long)t.count * B) return;
+    const int d = (int)(gid / B), inst = (int)(gid % B);
+    if (done && done[inst]) return;
+    const size_t sB = (size_t)B;
+    const DevCtx c = ctx[inst];
+    double* m = cm + (size_t)(t.site + 4 * d) * sB + inst;
+    double* r = cr + (size_t)(t.rsite + 2 * d) * sB + inst;
+    double geq = 0.0, ceq = 0.0;
+    if (c.mode & (MODETRAN | MODEAC | MODETRANOP)) {
+        const int np = t.nodes[2 * d], nn = t.nodes[2 * d + 1];
+        const double cap = t.farads[gid];
+        const bool cond1 = ((c.mode & MODEDC) && (c.mode & MODEINITJCT)) ||
+                           ((c.mode & MODEUIC) && (c.mode & MODEINITTRAN));
+        double vcap;
+        if (cond1) {
+            vcap = 0.0;  // no IC= on capacitors in this build
+        } else {
+            const double vp = np >= 0 ? x[(size_t)np * sB + inst] : 0.0;
+            const double vn = nn >= 0 ? x[(size_t)nn * sB + inst] : 0.0;
+            vcap = vp - vn;
+        }
+        const size_t q = (size_t)t.state[d] * sB + inst;  // state slot 0: charge
+        const size_t iq = q + sB;                          // state slot 1: current
+        if (c.mode & (MODETRAN | MODEAC)) {
+            if (c.mode & MODEINITPRED) {
+                s0[q] = s1[q];
+            } else {
+                s0[q] = cap * vcap;
+                if (c.mode & MODEINITTRAN) s1[q] = s0[q];
+            }
+            int order = c.order;
+            if (order < 1) order = 1;
+            if (order > 2) order = 2;
+            double deriv;
+            if (order == 1) {
+                deriv = c.ag0 * s0[q] + c.ag1 * s1[q];
+            } else if (c.method == 0) {
+                deriv = -s1[iq] * c.xmu_ratio + c.ag0 * s0[q] + c.ag1 * s1[q];
+            } else {
+                deriv = c.ag0 * s0[q] + c.ag1 * s1[q] + c.ag2 * s2[q];
+            }
+            s0[iq] = deriv;
+            geq = c.ag0 * cap;
+            ceq = s0[iq] - c.ag0 * s0[q];
+            if (c.mode & MODEINITTRAN) s1[iq] = s0[iq];
+        } else {
+            s0[q] = cap * vcap;
+        }
+    }
+    // multiplier m = 1 (m * geq == geq exactly)
+    m[0] = geq;
+    m[sB] = -geq;
+    m[2 * sB] = -geq;
+    m[3 * sB] = geq;
+    r[0] = -ceq;
+    r[sB] = ceq;
+}
+
+// VSource::value_at (vsource.cpp:80-155) for DC, PULSE and PWL.
+__device__ double wave_value(const VsrcTable& t, int d, double dcv, double time) {
+    const int w = t.wave[d];
+    if (w == WAVE_PULSE) {
+        const double* p = t.pulse + 7 * d;  // v1 v2 td tr tf pw per
+        const double v1 = p[0], v2 = p[1], td = p[2], tr = p[3], tf = p[4], pw = p[5], per = p[6];
+        if (time < td) return v1;
+        const double t_rel = time - td;
+        const double t_in = (per > 0.0) ? fmod(t_rel, per) : t_rel;
+        if (t_in < tr) return v1 + (v2 - v1) * (t_in / tr);
+        if (t_in < tr + pw) return v2;
+        if (t_in < tr + pw + tf) {
+            const double tfall = t_in - (tr + pw);
+            return v2 + (v1 - v2) * (tfall / tf);
+        }
+        return v1;
+    }
+    if (w == WAVE_PWL) {
+        const int a = t.pwl_ptr[d], b = t.pwl_ptr[d + 1];
+        if (a == b) return dcv;
+        if (time <= t.pwl_t[a]) return t.pwl_v[a];
+        if (time >= t.pwl_t[b - 1]) return t.pwl_v[b - 1];
+        int hi = a + 1;
+        for (; hi < b; ++hi)
+            if (t.pwl_t[hi] >= time) break;
+        const int lo = hi - 1;
+        const double t0 = t.pwl_t[lo], v0 = t.pwl_v[lo];
+        const double t1 = t.pwl_t[hi], v1 = t.pwl_v[hi];
+        if (t1 == t0) return v1;
+        const double frac = (time - t0) / (t1 - t0);
+        return v0 + (v1 - v0) * frac;
+    }
+    return dcv;
+}
+
+// VSource::evaluate rhs part (vsource.cpp:193-212).  The +-1 matrix stamps
+// are constant sites.
+__global__ void k_vsrc(VsrcTable t, const DevCtx* __restrict__ ctx,
+                       const int32_t* __restrict__ done, double* __restrict__ cr, int B) {
+    const long gid = (long)blockIdx.x * TPB + threadIdx.x;
+    if (gid >= (long)t.count * B) return;
+    const int d = (int)(gid / B), inst = (int)(gid % B);
+    if (done && done[inst]) return;
+    const DevCtx c = ctx[inst];
+    const double dcv = t.dc[gid];
+    double val;
+    if (t.dc_given[d] && t.wave[d] != WAVE_DC && (c.mode & (MODEDCOP | MODEDCTRANCURVE)))
+        val = dcv;
+    else
+        val = wave_value(t, d, dcv, c.time);
+    val *= c.src_fact;
+    cr[(size_t)(t.rsite + d) * B + inst] = val;
+}
+
+// ISource::evaluate (isource.cpp:157-174), DC sources.
+__global__ void k_isrc(IsrcTable t, const DevCtx* __restrict__ ctx,
+                       const int32_t* __restrict__ done, double* __restrict__ cr, int B) {
+    const long gid = (long)blockIdx.x * TPB + threadIdx.x;
+    if (gid >= (long)t.count * B) return;
+    const int d = (int)(gid / B), inst = (int)(gid % B);
+    if (done && done[inst]) return;
+    const double I = t.dc[gid] * ctx[inst].src_fact;
+    double* r = cr + (size_t)(t.rsite + 2 * d) * B + inst;
+    r[0] = -I;
+    r[(size_t)B] = I;
+}
+
+__global__ void k_gather(int nrows, const int32_t* __restrict__ ptr,
+                         const int32_t* __restrict__ sites, const int32_t* __restrict__ ptr2,
+                         const int32_t* __restrict__ sites2, const double* __restrict__ contrib,
+                         const double* __restrict__ contrib2, double* __restrict__ out, int B) {
+    const long gid = (long)blockIdx.x * TPB + threadIdx.x;
+    if (gid >= (long)nrows * B) return;
+    const int r = (int)(gid / B), inst = (int)(gid % B);
+    const size_t sB = (size_t)B;
+    double acc = 0.0;
+    for (int p = ptr[r]; p < ptr[r + 1]; ++p) acc += contrib[(size_t)sites[p] * sB + inst];
+    if (ptr2) {
+        double acc2 = 0.0;
+        for (int p = ptr2[r]; p < ptr2[r + 1]; ++p) acc2 += contrib2[(size_t)sites2[p] * sB + inst];
+        acc += acc2;
+    }
+    out[gid] = acc;
+}
+
+// Newton convergence test (newton.cpp:267-292): any failing unknown marks
+// its instance non-converged.
+__global__ void k_conv(int n, int num_nodes, const double* __restrict__ xn,
+                       const double* __restrict__ xo, double reltol, double vntol, double abstol,
+                       const int32_t* __restrict__ done, int32_t* __restrict__ nonconv, int B) {
+    const long gid = (long)blockIdx.x * TPB + threadIdx.x;
+    if (gid >= (long)n * B) return;
+    const int i = (int)(gid / B), inst = (int)(gid % B);
+    if (done && done[inst]) return;
+    const double vn = xn[gid], vo = xo[gid];
+    bool bad;
+    if (isnan(vn) || isinf(vn)) {
+        bad = true;
+    } else {
+        const double diff = fabs(vn - vo);
+        const double tol = reltol * fmax(fabs(vn), fabs(vo)) + (i < num_nodes ? vntol : abstol);
+        bad = diff > tol;
+    }
+    if (bad) atomicOr(&nonconv[inst], 1);
+}
+
+__global__ void k_commit(int n, const double* __restrict__ src, double* __restrict__ dst,
+                         const int32_t* __restrict__ mask, int mask_value, int B) {
+    const long gid = (long)blockIdx.x * TPB + threadIdx.x;
+    if (gid >= (long)n * B) return;
+    const int inst = (int)(gid % B);
+    if (mask && mask[inst] != mask_value) return;
+    dst[gid] = src[gid];
+}
+
+__global__ void k_fill_i32(int32_t* p, int32_t v, int count) {
+    const long gid = (long)blockIdx.x * TPB + threadIdx.x;
+    if (gid < count) p[gid] = v;
+}
+
+}  // namespace
+
+void launch_res_sites(int count, const double* ohms, double* contrib, int site, int B,
+                      hipStream_t s) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_res_sites, dim3(blocks_for((long)count * B)), dim3(TPB), 0, s, count,
+                       ohms, contrib, site, B);
+}
+
+void launch_const_sites(int count, const double* vals, double* contrib, int site, int B,
+                        hipStream_t s) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_const_sites, dim3(blocks_for((long)count * B)), dim3(TPB), 0, s, count,
+                       vals, contrib, site, B);
+}
+
+void launch_cap(const CapTable& t, const DevCtx* ctx, const double* x, double* s0, double* s1,
+                double* s2, const int32_t* done, double* cm, double* cr, int B, hipStream_t s) {
+    if (!t.count) return;
+    hipLaunchKernelGGL(k_cap, dim3(blocks_for((long)t.count * B)), dim3(TPB), 0, s, t, ctx, x, s0,
+                       s1, s2, done, cm, cr, B);
+}
+
+void launch_vsrc(const VsrcTable& t, const DevCtx* ctx, const int32_t* done, double* cr, int B,
+                 hipStream_t s) {
+    if (!t.count) return;
+    hipLaunchKernelGGL(k_vsrc, dim3(blocks_for((long)t.count * B)), dim3(TPB), 0, s, t, ctx, done,
+                       cr, B);
+}
+
+void launch_isrc(const IsrcTable& t, const DevCtx* ctx, const int32_t* done, double* cr, int B,
+                 hipStream_t s) {
+    if (!t.count) return;
+    hipLaunchKernelGGL(k_isrc, dim3(blocks_for((long)t.count * B)), dim3(TPB), 0, s, t, ctx, done,
+                       cr, B);
+}
+
+void launch_gather(int nrows, const int32_t* ptr, const int32_t* sites, const int32_t* ptr2,
+                   const int32_t* sites2, const double* contrib, const double* contrib2,
+                   double* out, int B, hipStream_t s) {
+    if (!nrows) return;
+    hipLaunchKernelGGL(k_gather, dim3(blocks_for((long)nrows * B)), dim3(TPB), 0, s, nrows, ptr,
+                       sites, ptr2, sites2, contrib, contrib2, out, B);
+}
+
+void launch_conv(int n, int num_nodes, const double* xnew, const double* xold, double reltol,
+                 double vntol, double abstol, const int32_t* done, int32_t* nonconv, int B,
+                 hipStream_t s) {
+    hipLaunchKernelGGL(k_conv, dim3(blocks_for((long)n * B)), dim3(TPB), 0, s, n, num_nodes,
+                       xnew, xold, reltol, vntol, abstol, done, nonconv, B);
+}
+
+void launch_commit(int n, const double* src, double* dst, const int32_t* mask, int mask_value,
+                   int B, hipStream_t s) {
+    hipLaunchKernelGGL(k_commit, dim3(blocks_for((long)n * B)), dim3(TPB), 0, s, n, src, dst,
+                       mask, mask_value, B);
+}
+
+void launch_fill_i32(int32_t* p, int32_t v, int count, hipStream_t s) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(count)), dim3(TPB), 0, s, p, v, count);
+}
+
+}  // namespace nsa

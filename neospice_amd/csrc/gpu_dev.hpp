@@ -1,0 +1,87 @@
+// Device-evaluation and Newton helper kernels (launch wrappers).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace nsa {
+
+// Per-instance integrator context (the fields of the reference's
+// IntegratorCtx that device loads read, src/core/types.hpp:73-94).
+struct DevCtx {
+    int32_t mode;
+    int32_t order;
+    int32_t method;
+    int32_t pad;
+    double ag0, ag1, ag2;
+    double xmu_ratio;
+    double time;      // current_time of the step being solved
+    double src_fact;  // SimOptions::src_fact
+};
+
+// CKTmode bits (ngspice cktdefs.h, used by src/core/newton.cpp:109-115).
+enum ModeBits : int32_t {
+    MODETRAN = 0x1,
+    MODEAC = 0x2,
+    MODEDCOP = 0x10,
+    MODETRANOP = 0x20,
+    MODEDCTRANCURVE = 0x40,
+    MODEDC = 0x70,
+    MODEINITFLOAT = 0x100,
+    MODEINITJCT = 0x200,
+    MODEINITFIX = 0x400,
+    MODEINITSMSIG = 0x800,
+    MODEINITTRAN = 0x1000,
+    MODEINITPRED = 0x2000,
+    MODEUIC = 0x10000,
+    INITF_MASK = 0x3F00,
+};
+
+struct CapTable {
+    int32_t count;
+    const int32_t* nodes;      // 2 per device
+    const int32_t* state;      // state base per device
+    const double* farads;      // count x B
+    int32_t site;              // first matrix site (4 per device)
+    int32_t rsite;             // first rhs site (2 per device)
+};
+
+struct VsrcTable {
+    int32_t count;
+    const int32_t* wave;       // WAVE_* per device
+    const double* dc;          // count x B
+    const int32_t* dc_given;   // per device
+    const double* pulse;       // 7 per device
+    const int32_t* pwl_ptr;    // count+1
+    const double* pwl_t;
+    const double* pwl_v;
+    int32_t rsite;             // 1 per device
+};
+
+struct IsrcTable {
+    int32_t count;
+    const double* dc;          // count x B
+    int32_t rsite;             // 2 per device
+};
+
+void launch_res_sites(int count, const double* ohms, double* contrib, int site, int B,
+                      hipStream_t s);
+void launch_const_sites(int count, const double* vals, double* contrib, int site, int B,
+                        hipStream_t s);
+void launch_cap(const CapTable& t, const DevCtx* ctx, const double* x, double* s0, double* s1,
+                double* s2, const int32_t* done, double* cm, double* cr, int B, hipStream_t s);
+void launch_vsrc(const VsrcTable& t, const DevCtx* ctx, const int32_t* done, double* cr, int B,
+                 hipStream_t s);
+void launch_isrc(const IsrcTable& t, const DevCtx* ctx, const int32_t* done, double* cr, int B,
+                 hipStream_t s);
+void launch_gather(int nrows, const int32_t* ptr, const int32_t* sites, const int32_t* ptr2,
+                   const int32_t* sites2, const double* contrib, const double* contrib2,
+                   double* out, int B, hipStream_t s);
+void launch_conv(int n, int num_nodes, const double* xnew, const double* xold, double reltol,
+                 double vntol, double abstol, const int32_t* done, int32_t* nonconv, int B,
+                 hipStream_t s);
+void launch_commit(int n, const double* src, double* dst, const int32_t* mask, int mask_value,
+                   int B, hipStream_t s);
+void launch_fill_i32(int32_t* p, int32_t v, int count, hipStream_t s);
+
+}  // namespace nsa

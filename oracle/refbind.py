@@ -1,0 +1,316 @@
+"""ctypes wrapper of the compiled reference (oracle/_ref/libneospice_ref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the
+CPU-baseline leg of bench.py, never by the product.  It replays a
+neospice_amd.Circuit (the recorded builder calls) into the reference's own
+Circuit through its typed API, so both sides see identical inputs; node k
+of the product (0-based) is node "n<k+1>" of the reference, created in the
+same order, so MNA indices coincide.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_SO = os.path.join(HERE, "_ref", "libneospice_ref.so")
+ORC_SO = os.path.join(HERE, "_ref", "liboracle.so")
+
+P_I32 = C.POINTER(C.c_int32)
+P_F64 = C.POINTER(C.c_double)
+VP = C.c_void_p
+_ref = None
+_orc = None
+
+
+def available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        L = C.CDLL(REF_SO)
+        sig = {
+            "ref_last_error": (C.c_char_p, []),
+            "ref_ckt_new": (VP, [C.c_int]),
+            "ref_ckt_free": (None, [VP]),
+            "ref_ckt_model": (C.c_int, [VP, C.c_char_p]),
+            "ref_ckt_r": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int, C.c_double]),
+            "ref_ckt_c": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int, C.c_double]),
+            "ref_ckt_v": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int, C.c_double]),
+            "ref_ckt_i": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int, C.c_double]),
+            "ref_ckt_v_pulse": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int] + [C.c_double] * 7),
+            "ref_ckt_v_pwl": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int, C.c_int, P_F64, P_F64]),
+            "ref_ckt_d": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int, C.c_char_p]),
+            "ref_ckt_m": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p,
+                                    C.c_double, C.c_double]),
+            "ref_ckt_ic": (C.c_int, [VP, C.c_int, C.c_double]),
+            "ref_ckt_option": (C.c_int, [VP, C.c_char_p, C.c_double]),
+            "ref_ckt_finalize": (C.c_int, [VP, C.c_int]),
+            "ref_ckt_num_vars": (C.c_int, [VP]),
+            "ref_ckt_num_nodes": (C.c_int, [VP]),
+            "ref_ckt_num_states": (C.c_int, [VP]),
+            "ref_ckt_nnz": (C.c_int, [VP]),
+            "ref_ckt_is_linear": (C.c_int, [VP]),
+            "ref_ckt_num_devices": (C.c_int, [VP]),
+            "ref_ckt_pattern": (None, [VP, P_I32, P_I32]),
+            "ref_ckt_load_order": (C.c_int, [VP, P_I32]),
+            "ref_ckt_branch_index": (C.c_int, [VP, C.c_int]),
+            "ref_ckt_dead_nodes": (C.c_int, [VP, P_I32]),
+            "ref_ckt_state": (P_F64, [VP, C.c_int]),
+            "ref_ckt_set_ctx": (None, [VP, C.c_int, C.c_int, C.c_int] + [C.c_double] * 7),
+            "ref_ckt_load": (C.c_int, [VP, P_F64, P_F64, P_F64]),
+            "ref_dc": (C.c_int, [VP, P_F64, P_F64]),
+            "ref_tran": (C.c_int, [VP, C.c_double, C.c_double, C.c_int, P_F64, P_F64, P_F64,
+                                   P_I32]),
+            "ref_tran_fixed": (C.c_int, [VP, C.c_double, C.c_int, P_F64, P_F64]),
+            "ref_tran_num_records": (C.c_int, [VP]),
+            "ref_tran_records": (None, [VP, P_I32, P_F64]),
+            "ref_newton": (C.c_int, [VP, P_F64, C.c_int]),
+            "ref_sens": (C.c_int, [VP, C.c_int, C.c_int, P_F64, C.c_char_p, C.c_int]),
+            "ref_solver_new": (VP, [C.c_int, C.c_int, P_I32, P_I32]),
+            "ref_solver_free": (None, [VP]),
+            "ref_solver_numeric": (C.c_int, [VP, P_F64, C.c_double]),
+            "ref_solver_refactorize": (C.c_int, [VP, P_F64, C.c_double]),
+            "ref_solver_solve": (C.c_int, [VP, P_F64]),
+            "ref_solver_solve_transposed": (C.c_int, [VP, P_F64]),
+            "ref_solver_fast_count": (C.c_int, [VP]),
+            "ref_solver_fallback_count": (C.c_int, [VP]),
+            "ref_neo_dump": (C.c_int, [VP, P_I32, P_I32, P_I32, P_I32, P_F64]),
+            "ref_neo_fillins": (C.c_int, [VP]),
+            "ref_amd_dump": (C.c_int64, [VP, P_I32, P_I32, P_I32, P_I32, P_F64, P_I32, P_I32, P_F64]),
+            "ref_amd_ordering": (C.c_int, [C.c_int, P_I32, P_I32, P_I32]),
+        }
+        for k, (r, a) in sig.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _ref = L
+    return _ref
+
+
+def orc():
+    """The plain-C restatement (oracle/oracle.c)."""
+    global _orc
+    if _orc is None:
+        L = C.CDLL(ORC_SO)
+        L.orc_amd_order.argtypes = [C.c_int, P_I32, P_I32, P_I32]
+        L.orc_gp_factor.argtypes = [C.c_int, P_I32, P_I32, P_F64, C.c_double, C.c_double, P_I32,
+                                    P_I32, P_I32, P_I32, P_F64, C.c_int, P_I32, P_I32, P_F64,
+                                    C.c_int]
+        L.orc_gp_refactor.argtypes = [C.c_int, P_I32, P_I32, P_F64, C.c_double, C.c_double, P_I32,
+                                      P_I32, P_I32, P_I32, P_F64, P_I32, P_I32, P_F64]
+        L.orc_gp_solve.argtypes = [C.c_int, P_I32, P_I32, P_I32, P_I32, P_F64, P_I32, P_I32,
+                                   P_F64, P_F64]
+        L.orc_gp_solve_t.argtypes = L.orc_gp_solve.argtypes
+        _orc = L
+    return _orc
+
+
+def i32(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(P_I32)
+
+
+def f64(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(P_F64)
+
+
+def _chk(rc):
+    if rc < 0:
+        raise RuntimeError("reference: " + ref().ref_last_error().decode())
+    return rc
+
+
+class RefCircuit:
+    """A reference Circuit built from a neospice_amd.Circuit's recorded calls.
+    `values` optionally overrides device primary values {device index: value}."""
+
+    def __init__(self, ckt, values=None, options=None, probe=True):
+        L = ref()
+        self.L = L
+        nn = len(ckt.nodes)
+        self.h = C.c_void_p(L.ref_ckt_new(nn))
+        values = values or {}
+        d = 0
+        for op in ckt.ops:
+            k = op[0]
+            node = lambda x: x + 1  # product 0-based (GND=-1) -> reference "n<k+1>" (0=GND)
+            if k == "node":
+                continue
+            if k == "model":
+                _chk(L.ref_ckt_model(self.h, op[1].encode()))
+                continue
+            v = values.get(d)
+            if k in ("R", "C", "V", "I"):
+                f = {"R": L.ref_ckt_r, "C": L.ref_ckt_c, "V": L.ref_ckt_v, "I": L.ref_ckt_i}[k]
+                _chk(f(self.h, op[1].encode(), node(op[2]), node(op[3]), op[4] if v is None else v))
+            elif k == "Vpulse":
+                _chk(L.ref_ckt_v_pulse(self.h, op[1].encode(), node(op[2]), node(op[3]), *op[4]))
+            elif k == "Vpwl":
+                t, pt = f64(op[4])
+                vv, pv = f64(op[5])
+                _chk(L.ref_ckt_v_pwl(self.h, op[1].encode(), node(op[2]), node(op[3]), len(t), pt, pv))
+            elif k == "D":
+                _chk(L.ref_ckt_d(self.h, op[1].encode(), node(op[2]), node(op[3]), op[4].encode()))
+            elif k == "M":
+                _chk(L.ref_ckt_m(self.h, op[1].encode(), node(op[2]), node(op[3]), node(op[4]),
+                                 node(op[5]), op[6].encode(), op[7], op[8]))
+            d += 1
+        for key, val in (options or {}).items():
+            _chk(L.ref_ckt_option(self.h, key.encode(), float(val)))
+        _chk(L.ref_ckt_finalize(self.h, 1 if probe else 0))
+        self.n = L.ref_ckt_num_vars(self.h)
+        self.num_nodes = L.ref_ckt_num_nodes(self.h)
+        self.nnz = L.ref_ckt_nnz(self.h)
+        self.ndev = L.ref_ckt_num_devices(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.ref_ckt_free(self.h)
+            self.h = None
+
+    def pattern(self):
+        cp = np.zeros(self.n + 1, np.int32)
+        ri = np.zeros(self.nnz, np.int32)
+        self.L.ref_ckt_pattern(self.h, i32(cp)[1], i32(ri)[1])
+        return cp, ri
+
+    def load_order(self):
+        o = np.zeros(self.ndev, np.int32)
+        k = self.L.ref_ckt_load_order(self.h, o.ctypes.data_as(P_I32))
+        return o[:k]
+
+    def dc(self):
+        x = np.zeros(self.n)
+        sec = C.c_double()
+        its = _chk(self.L.ref_dc(self.h, x.ctypes.data_as(P_F64), C.byref(sec)))
+        return x, its, sec.value
+
+    def tran(self, tstep, tstop, max_points=200000):
+        t = np.zeros(max_points)
+        s = np.zeros(max_points * self.n)
+        sec = C.c_double()
+        its = C.c_int32()
+        k = _chk(self.L.ref_tran(self.h, tstep, tstop, max_points, t.ctypes.data_as(P_F64),
+                                 s.ctypes.data_as(P_F64), C.byref(sec), C.byref(its)))
+        return t[:k].copy(), s[:k * self.n].reshape(k, self.n).copy(), its.value, sec.value
+
+    def tran_fixed(self, dt, nsteps):
+        """Fixed-step backward-Euler stepping through the reference's own
+        newton_solve / rotate_state; returns (final solution, Newton
+        iterations of the stepping loop, seconds)."""
+        x = np.zeros(self.n)
+        sec = C.c_double()
+        its = _chk(self.L.ref_tran_fixed(self.h, dt, nsteps, x.ctypes.data_as(P_F64), C.byref(sec)))
+        return x, its, sec.value
+
+    def records(self):
+        k = self.L.ref_tran_num_records(self.h)
+        ints = np.zeros(4 * k, np.int32)
+        dbl = np.zeros(10 * k)
+        self.L.ref_tran_records(self.h, ints.ctypes.data_as(P_I32), dbl.ctypes.data_as(P_F64))
+        return ints.reshape(k, 4), dbl.reshape(k, 10)
+
+    def set_ctx(self, mode, order=1, method=0, ag=(0, 0, 0), delta=0.0, dold1=0.0, xmu_ratio=1.0,
+                time=0.0):
+        self.L.ref_ckt_set_ctx(self.h, mode, order, method, ag[0], ag[1], ag[2], delta, dold1,
+                               xmu_ratio, time)
+
+    def load(self, x):
+        x, px = f64(x)
+        vals = np.zeros(self.nnz)
+        rhs = np.zeros(self.n)
+        _chk(self.L.ref_ckt_load(self.h, px, vals.ctypes.data_as(P_F64), rhs.ctypes.data_as(P_F64)))
+        return vals, rhs
+
+    def state(self, which, count):
+        p = self.L.ref_ckt_state(self.h, which)
+        return np.ctypeslib.as_array(p, shape=(count,)).copy()
+
+
+class RefSolver:
+    """A reference NeoSolver (kind 0) or AmdLuSolver (kind 1) on a CSC pattern."""
+
+    def __init__(self, kind, n, col_ptr, row_idx):
+        self.L = ref()
+        self.n = n
+        self.cp, pcp = i32(col_ptr)
+        self.ri, pri = i32(row_idx)
+        self.h = C.c_void_p(self.L.ref_solver_new(kind, n, pcp, pri))
+        self.kind = kind
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.ref_solver_free(self.h)
+            self.h = None
+
+    def numeric(self, vals, gmin=0.0):
+        v, pv = f64(vals)
+        return bool(_chk(self.L.ref_solver_numeric(self.h, pv, gmin)))
+
+    def refactorize(self, vals, gmin=0.0):
+        v, pv = f64(vals)
+        return bool(_chk(self.L.ref_solver_refactorize(self.h, pv, gmin)))
+
+    def solve(self, b):
+        b, pb = f64(np.array(b, dtype=np.float64))
+        _chk(self.L.ref_solver_solve(self.h, pb))
+        return b
+
+    def solve_transposed(self, b):
+        b, pb = f64(np.array(b, dtype=np.float64))
+        _chk(self.L.ref_solver_solve_transposed(self.h, pb))
+        return b
+
+    def neo_dump(self):
+        n = self.n
+        i2r = np.zeros(n, np.int32)
+        i2c = np.zeros(n, np.int32)
+        cnt = self.L.ref_neo_dump(self.h, None, None, None, None, None)
+        cp = np.zeros(n + 1, np.int32)
+        rows = np.zeros(cnt, np.int32)
+        vals = np.zeros(cnt)
+        self.L.ref_neo_dump(self.h, i2r.ctypes.data_as(P_I32), i2c.ctypes.data_as(P_I32),
+                            cp.ctypes.data_as(P_I32), rows.ctypes.data_as(P_I32),
+                            vals.ctypes.data_as(P_F64))
+        return {"i2e_row": i2r, "i2e_col": i2c, "col_ptr": cp, "rows": rows, "vals": vals}
+
+    def amd_dump(self):
+        n = self.n
+        sz = self.L.ref_amd_dump(self.h, *([None] * 8))
+        lnz, unz = sz >> 32, sz & 0xFFFFFFFF
+        out = {k: np.zeros(n, np.int32) for k in ("q", "pinv")}
+        out["Lp"] = np.zeros(n + 1, np.int32)
+        out["Up"] = np.zeros(n + 1, np.int32)
+        out["Li"] = np.zeros(lnz, np.int32)
+        out["Ui"] = np.zeros(unz, np.int32)
+        out["Lx"] = np.zeros(lnz)
+        out["Ux"] = np.zeros(unz)
+        args = [out[k] for k in ("q", "pinv", "Lp", "Li", "Lx", "Up", "Ui", "Ux")]
+        self.L.ref_amd_dump(self.h, *[a.ctypes.data_as(P_I32 if a.dtype == np.int32 else P_F64)
+                                      for a in args])
+        return out
+
+
+def split_newton_calls(ints, dbls):
+    """Groups probe records into Newton calls: a transient call starts at a
+    record whose mode carries INITTRAN or INITPRED (newton.cpp:396-402).
+    Returns a list of dicts with the call's first-iteration context, the
+    number of iterations and whether the step was accepted."""
+    calls = []
+    for k in range(len(ints)):
+        mode = int(ints[k, 0])
+        if mode & 0x1 and mode & (0x1000 | 0x2000):
+            calls.append({"first": k, "iters": 0, "mode": mode, "order": int(ints[k, 1]),
+                          "method": int(ints[k, 2]), "ag": tuple(dbls[k, 0:3]),
+                          "delta": float(dbls[k, 3]), "xmu_ratio": float(dbls[k, 7]),
+                          "time": float(dbls[k, 8]), "acc0": int(ints[k, 3])})
+        if calls and mode & 0x1:
+            calls[-1]["iters"] += 1
+    for i, c in enumerate(calls):
+        # the run ends on an accepted step (transient.cpp:873, 965-1004)
+        c["accepted"] = calls[i + 1]["acc0"] > c["acc0"] if i + 1 < len(calls) else True
+    return calls
