@@ -22,6 +22,8 @@ BatchLu::BatchLu(int batch, hipStream_t stream) : B_(batch), stream_(stream) {
     NSA_HIP(hipMalloc(&d_kind_, sizeof(int32_t) * B_));
     h_kind_.assign(B_, -1);
     group_of_.assign(B_, -1);
+    NSA_HIP(hipHostMalloc((void**)&h_kind_pin_, sizeof(int32_t) * B_, hipHostMallocDefault));
+    NSA_HIP(hipHostMalloc((void**)&h_fail_pin_, sizeof(int32_t) * B_, hipHostMallocDefault));
 }
 
 BatchLu::~BatchLu() {
@@ -30,6 +32,10 @@ BatchLu::~BatchLu() {
     (void)hipFree(d_fail_);
     (void)hipFree(d_fail_col_);
     (void)hipFree(d_kind_);
+    (void)hipFree(d_vals_);
+    (void)hipHostFree(h_kind_pin_);
+    (void)hipHostFree(h_fail_pin_);
+    if (h_vals_pin_) (void)hipHostFree(h_vals_pin_);
 }
 
 void BatchLu::symbolic(int n, const int32_t* col_ptr, const int32_t* row_idx, bool is_linear,
@@ -60,11 +66,17 @@ void BatchLu::symbolic(int n, const int32_t* col_ptr, const int32_t* row_idx, bo
 }
 
 std::vector<double> BatchLu::instance_values(const double* dA, int inst) {
-    std::vector<double> v(acp_[n_]);
-    NSA_HIP(hipMemcpy2DAsync(v.data(), sizeof(double), dA + inst, sizeof(double) * B_,
-                             sizeof(double), v.size(), hipMemcpyDeviceToHost, stream_));
+    const int nnz = acp_[n_];
+    if (!d_vals_) {
+        NSA_HIP(hipMalloc(&d_vals_, sizeof(double) * std::max(1, nnz)));
+        NSA_HIP(hipHostMalloc((void**)&h_vals_pin_, sizeof(double) * std::max(1, nnz),
+                              hipHostMallocDefault));
+    }
+    launch_take_instance(dA, B_, inst, nnz, d_vals_, stream_);
+    NSA_HIP(hipMemcpyAsync(h_vals_pin_, d_vals_, sizeof(double) * nnz, hipMemcpyDeviceToHost,
+                           stream_));
     NSA_HIP(hipStreamSynchronize(stream_));
-    return v;
+    return std::vector<double>(h_vals_pin_, h_vals_pin_ + nnz);
 }
 
 // CPU pivot computation for one instance's values.  Markowitz continues
@@ -104,8 +116,10 @@ void BatchLu::run_gpu(const double* dA, double gmin, const std::vector<int32_t>&
     launch_fill_i32(d_fail_, FAIL_NONE, B_, stream_);
     launch_fill_i32(d_fail_col_, 0x7fffffff, B_, stream_);
     if (kind != h_kind_) {
+        // the stream is idle here: every run ends with a synchronize
         h_kind_ = kind;
-        NSA_HIP(hipMemcpyAsync(d_kind_, h_kind_.data(), sizeof(int32_t) * B_,
+        std::memcpy(h_kind_pin_, kind.data(), sizeof(int32_t) * B_);
+        NSA_HIP(hipMemcpyAsync(d_kind_, h_kind_pin_, sizeof(int32_t) * B_,
                                hipMemcpyHostToDevice, stream_));
     }
     if (timing_) NSA_HIP(hipEventRecord(ev_[0], stream_));
@@ -113,9 +127,10 @@ void BatchLu::run_gpu(const double* dA, double gmin, const std::vector<int32_t>&
     if (timing_) { NSA_HIP(hipEventRecord(ev_[1], stream_)); pend_rf_ = true; }
     ++n_gpu_refactor;
     fail.resize(B_);
-    NSA_HIP(hipMemcpyAsync(fail.data(), d_fail_, sizeof(int32_t) * B_, hipMemcpyDeviceToHost,
+    NSA_HIP(hipMemcpyAsync(h_fail_pin_, d_fail_, sizeof(int32_t) * B_, hipMemcpyDeviceToHost,
                            stream_));
     NSA_HIP(hipStreamSynchronize(stream_));
+    std::memcpy(fail.data(), h_fail_pin_, sizeof(int32_t) * B_);
     collect_timing();
 }
 

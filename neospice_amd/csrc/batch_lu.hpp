@@ -98,6 +98,10 @@ private:
     bool factored_ = false;
     int32_t *d_fail_ = nullptr, *d_fail_col_ = nullptr, *d_kind_ = nullptr;
     std::vector<int32_t> h_kind_;
+    // pinned staging for every host<->device transfer of the factor driver
+    int32_t *h_kind_pin_ = nullptr, *h_fail_pin_ = nullptr;
+    double* h_vals_pin_ = nullptr;   // one instance's matrix values (nnz)
+    double* d_vals_ = nullptr;
     bool timing_ = false, pend_rf_ = false, pend_sv_ = false;
     hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
 };

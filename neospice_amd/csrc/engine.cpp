@@ -59,6 +59,8 @@ BatchEngine::BatchEngine(const Circuit& ckt, int batch, const Options& opt)
     d_ctx_ = dalloc<DevCtx>(allocs_, B_);
     d_done_ = dalloc<int32_t>(allocs_, B_);
     d_nonconv_ = dalloc<int32_t>(allocs_, B_);
+    NSA_HIP(hipHostMalloc((void**)&h_flags_, sizeof(int32_t) * 4 * B_, hipHostMallocDefault));
+    NSA_HIP(hipHostMalloc((void**)&h_ctx_pin_, sizeof(DevCtx) * B_, hipHostMallocDefault));
     DevCtx c{};
     c.mode = MODEDCOP | MODEINITJCT;
     c.order = 1;
@@ -74,8 +76,25 @@ BatchEngine::BatchEngine(const Circuit& ckt, int batch, const Options& opt)
 BatchEngine::~BatchEngine() {
     lu_.reset();
     for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
+    if (stream_) (void)hipStreamSynchronize(stream_);
     for (void* p : allocs_) (void)hipFree(p);
+    if (h_flags_) (void)hipHostFree(h_flags_);
+    if (h_ctx_pin_) (void)hipHostFree(h_ctx_pin_);
     if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void BatchEngine::upload_ctx() {
+    // the previous context copy may still read the staging area
+    NSA_HIP(hipStreamSynchronize(stream_));
+    std::memcpy(h_ctx_pin_, h_ctx_.data(), sizeof(DevCtx) * B_);
+    NSA_HIP(hipMemcpyAsync(d_ctx_, h_ctx_pin_, sizeof(DevCtx) * B_, hipMemcpyHostToDevice, stream_));
+}
+
+// Callers guarantee that the stream has passed the previous copy from `slot`.
+void BatchEngine::upload_flags(int slot, const std::vector<int32_t>& v, int32_t* dst) {
+    int32_t* h = h_flags_ + (size_t)slot * B_;
+    std::memcpy(h, v.data(), sizeof(int32_t) * B_);
+    NSA_HIP(hipMemcpyAsync(dst, h, sizeof(int32_t) * B_, hipMemcpyHostToDevice, stream_));
 }
 
 void BatchEngine::build_tables() {
@@ -208,12 +227,12 @@ void BatchEngine::set_device_values(int dev, const double* vals) {
         default: throw Error(NSA_ERR_ARGUMENT, "set_device_values: unsupported device kind");
     }
     std::vector<double> v(vals, vals + B_);
-    if (d.kind == DK_R)
+    if (d.kind == DK_R) {
         for (auto& x : v) if (!std::isfinite(x) || x == 0.0) x = 1e-3;
+        r_dirty_ = true;  // conductance sites are recomputed before the next load
+    }
     NSA_HIP(hipMemcpyAsync(dst + (size_t)d.family_index * B_, v.data(), sizeof(double) * B_,
                            hipMemcpyHostToDevice, stream_));
-    if (d.kind == DK_R)
-        launch_res_sites(nR_, d_r_ohms_, d_cm_, r_site_, B_, stream_);
     NSA_HIP(hipStreamSynchronize(stream_));
 }
 
@@ -224,11 +243,14 @@ void BatchEngine::set_ctx(const DevCtx& c) {
 void BatchEngine::set_ctx(const std::vector<DevCtx>& c) {
     if ((int)c.size() != B_) throw Error(NSA_ERR_ARGUMENT, "set_ctx: need one context per instance");
     h_ctx_ = c;
-    NSA_HIP(hipMemcpyAsync(d_ctx_, h_ctx_.data(), sizeof(DevCtx) * B_, hipMemcpyHostToDevice,
-                           stream_));
+    upload_ctx();
 }
 
 void BatchEngine::load(const int32_t* d_done) {
+    if (r_dirty_) {
+        launch_res_sites(nR_, d_r_ohms_, d_cm_, r_site_, B_, stream_);
+        r_dirty_ = false;
+    }
     if (timing_) NSA_HIP(hipEventRecord(ev_[0], stream_));
     launch_cap(cap_, d_ctx_, d_x_, d_s_[0], d_s_[1], d_s_[2], d_done, d_cm_, d_cr_, B_, stream_);
     launch_vsrc(vsrc_, d_ctx_, d_done, d_cr_, B_, stream_);
@@ -325,15 +347,15 @@ NewtonOutcome BatchEngine::newton(int itl) {
             if (!done[i] && numeric_mask[i] && !skip[i]) force_numeric[i] = 0;
         // solve; proposed solution replaces the solution of live instances
         lu_->solve(d_rhs_);
-        std::vector<int32_t> live(B_);
-        for (int i = 0; i < B_; ++i) live[i] = (!done[i] && !skip[i]) ? 1 : 0;
-        NSA_HIP(hipMemcpyAsync(d_nonconv_, live.data(), sizeof(int32_t) * B_,
-                               hipMemcpyHostToDevice, stream_));
+        std::vector<int32_t> live(B_), not_live(B_);
+        for (int i = 0; i < B_; ++i) {
+            live[i] = (!done[i] && !skip[i]) ? 1 : 0;
+            not_live[i] = 1 - live[i];
+        }
+        upload_flags(0, live, d_nonconv_);
         launch_commit(n_, d_rhs_, d_x_, d_nonconv_, 1, B_, stream_);
         // convergence (newton.cpp:267-292); d_done_ doubles as "not live"
-        for (int i = 0; i < B_; ++i) live[i] = live[i] ? 0 : 1;
-        NSA_HIP(hipMemcpyAsync(d_done_, live.data(), sizeof(int32_t) * B_, hipMemcpyHostToDevice,
-                               stream_));
+        upload_flags(1, not_live, d_done_);
         launch_fill_i32(d_nonconv_, 0, B_, stream_);
         launch_conv(n_, num_nodes, d_x_, d_xold_, opt_.reltol, opt_.vntol, opt_.abstol, d_done_,
                     d_nonconv_, B_, stream_);
@@ -373,24 +395,22 @@ NewtonOutcome BatchEngine::newton(int itl) {
         }
         if (any_restore) {
             // converged: return the previous iterate (newton.cpp:378-381)
-            NSA_HIP(hipMemcpyAsync(d_nonconv_, restore.data(), sizeof(int32_t) * B_,
-                                   hipMemcpyHostToDevice, stream_));
+            upload_flags(2, restore, d_nonconv_);
             launch_commit(n_, d_xold_, d_x_, d_nonconv_, 1, B_, stream_);
         }
         // mode changes for the next iteration's loads
         for (int i = 0; i < B_; ++i) h_ctx_[i].mode = ctx[i].mode;
-        NSA_HIP(hipMemcpyAsync(d_ctx_, h_ctx_.data(), sizeof(DevCtx) * B_, hipMemcpyHostToDevice,
+        std::memcpy(h_ctx_pin_, h_ctx_.data(), sizeof(DevCtx) * B_);  // stream synced above
+        NSA_HIP(hipMemcpyAsync(d_ctx_, h_ctx_pin_, sizeof(DevCtx) * B_, hipMemcpyHostToDevice,
                                stream_));
-        NSA_HIP(hipMemcpyAsync(d_done_, done.data(), sizeof(int32_t) * B_, hipMemcpyHostToDevice,
-                               stream_));
+        upload_flags(3, done, d_done_);
         ms_load += std::chrono::duration<double, std::milli>(t1 - t0).count();
         ms_factor += std::chrono::duration<double, std::milli>(t2 - t1).count();
         ms_solve += std::chrono::duration<double, std::milli>(t3 - t2).count();
     }
     // restore the caller's mode (newton.cpp:224, 380, 417)
     for (int i = 0; i < B_; ++i) h_ctx_[i].mode = saved_mode[i];
-    NSA_HIP(hipMemcpyAsync(d_ctx_, h_ctx_.data(), sizeof(DevCtx) * B_, hipMemcpyHostToDevice,
-                           stream_));
+    upload_ctx();
     NSA_HIP(hipStreamSynchronize(stream_));
     for (int i = 0; i < B_; ++i) out.total_iterations += out.iterations[i];
     return out;

@@ -101,6 +101,14 @@ private:
     DevCtx* d_ctx_ = nullptr;
     int32_t *d_done_ = nullptr, *d_nonconv_ = nullptr;
     std::vector<DevCtx> h_ctx_;
+    // Pinned staging for the per-iteration host->device flags and contexts
+    // (4 x B int32: live, not-live, restore, done) so that asynchronous
+    // copies never read host memory that is rewritten or freed.
+    int32_t* h_flags_ = nullptr;
+    DevCtx* h_ctx_pin_ = nullptr;
+    void upload_ctx();
+    void upload_flags(int slot, const std::vector<int32_t>& v, int32_t* dst);
+    bool r_dirty_ = false;
     // family tables
     CapTable cap_{};
     VsrcTable vsrc_{};

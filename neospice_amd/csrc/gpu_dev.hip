@@ -226,6 +226,12 @@ __global__ void k_fill_i32(int32_t* p, int32_t v, int count) {
     if (gid < count) p[gid] = v;
 }
 
+__global__ void k_take_instance(const double* __restrict__ A, int B, int inst, int count,
+                                double* __restrict__ out) {
+    const long gid = (long)blockIdx.x * TPB + threadIdx.x;
+    if (gid < count) out[gid] = A[gid * B + inst];
+}
+
 }  // namespace
 
 void launch_res_sites(int count, const double* ohms, double* contrib, int site, int B,
@@ -287,6 +293,12 @@ void launch_commit(int n, const double* src, double* dst, const int32_t* mask, i
 void launch_fill_i32(int32_t* p, int32_t v, int count, hipStream_t s) {
     if (!count) return;
     hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(count)), dim3(TPB), 0, s, p, v, count);
+}
+
+void launch_take_instance(const double* A, int B, int inst, int count, double* out, hipStream_t s) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_take_instance, dim3(blocks_for(count)), dim3(TPB), 0, s, A, B, inst, count,
+                       out);
 }
 
 }  // namespace nsa

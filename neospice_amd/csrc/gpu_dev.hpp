@@ -83,5 +83,7 @@ void launch_conv(int n, int num_nodes, const double* xnew, const double* xold, d
 void launch_commit(int n, const double* src, double* dst, const int32_t* mask, int mask_value,
                    int B, hipStream_t s);
 void launch_fill_i32(int32_t* p, int32_t v, int count, hipStream_t s);
+// out[k] = A[k * B + inst], k < count (one instance of a slot-major batch)
+void launch_take_instance(const double* A, int B, int inst, int count, double* out, hipStream_t s);
 
 }  // namespace nsa
