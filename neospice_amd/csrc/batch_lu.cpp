@@ -27,7 +27,11 @@ BatchLu::BatchLu(int batch, hipStream_t stream) : B_(batch), stream_(stream) {
 }
 
 BatchLu::~BatchLu() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
     groups_.clear();
+    for (auto st : side_) (void)hipStreamDestroy(st);
+    for (auto e : join_ev_) (void)hipEventDestroy(e);
+    if (fork_ev_) (void)hipEventDestroy(fork_ev_);
     for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
     (void)hipFree(d_fail_);
     (void)hipFree(d_fail_col_);
@@ -111,6 +115,32 @@ void BatchLu::rebuild_group(int g) {
     G.lu = std::make_unique<GpuLu>(s, (int)G.inst.size(), stream_, all ? nullptr : &G.inst, B_);
 }
 
+template <class F>
+void BatchLu::for_groups(const std::vector<int>& which, F&& launch) {
+    if (which.size() <= 1) {
+        for (int g : which) launch(g, stream_);
+        return;
+    }
+    const size_t nside = which.size() - 1;
+    if (!fork_ev_) NSA_HIP(hipEventCreateWithFlags(&fork_ev_, hipEventDisableTiming));
+    while (side_.size() < nside) {
+        hipStream_t st;
+        NSA_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        side_.push_back(st);
+        hipEvent_t e;
+        NSA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        join_ev_.push_back(e);
+    }
+    NSA_HIP(hipEventRecord(fork_ev_, stream_));
+    for (size_t i = 1; i < which.size(); ++i) {
+        NSA_HIP(hipStreamWaitEvent(side_[i - 1], fork_ev_, 0));
+        launch(which[i], side_[i - 1]);
+        NSA_HIP(hipEventRecord(join_ev_[i - 1], side_[i - 1]));
+    }
+    launch(which[0], stream_);
+    for (size_t i = 1; i < which.size(); ++i) NSA_HIP(hipStreamWaitEvent(stream_, join_ev_[i - 1], 0));
+}
+
 void BatchLu::run_gpu(const double* dA, double gmin, const std::vector<int32_t>& kind,
                       const std::vector<int>& which, std::vector<int32_t>& fail) {
     launch_fill_i32(d_fail_, FAIL_NONE, B_, stream_);
@@ -123,7 +153,9 @@ void BatchLu::run_gpu(const double* dA, double gmin, const std::vector<int32_t>&
                                hipMemcpyHostToDevice, stream_));
     }
     if (timing_) NSA_HIP(hipEventRecord(ev_[0], stream_));
-    for (int g : which) groups_[g].lu->refactor(dA, gmin, d_kind_, d_fail_, d_fail_col_);
+    for_groups(which, [&](int g, hipStream_t st) {
+        groups_[g].lu->refactor(dA, gmin, d_kind_, d_fail_, d_fail_col_, st);
+    });
     if (timing_) { NSA_HIP(hipEventRecord(ev_[1], stream_)); pend_rf_ = true; }
     ++n_gpu_refactor;
     fail.resize(B_);
@@ -270,14 +302,14 @@ int BatchLu::refactorize(const double* dA, double gmin, int32_t* status) {
 void BatchLu::solve(double* db) {
     if (!factored()) throw Error(NSA_ERR_LOGIC, "BatchLu::solve: numeric() not called");
     if (timing_) NSA_HIP(hipEventRecord(ev_[2], stream_));
-    for (auto& G : groups_) G.lu->solve(db, false);
+    for_groups(all_groups(), [&](int g, hipStream_t st) { groups_[g].lu->solve(db, false, st); });
     if (timing_) { NSA_HIP(hipEventRecord(ev_[3], stream_)); pend_sv_ = true; }
     ++n_gpu_solve;
 }
 
 void BatchLu::solve_t(double* db) {
     if (!factored()) throw Error(NSA_ERR_LOGIC, "BatchLu::solve_t: numeric() not called");
-    for (auto& G : groups_) G.lu->solve(db, true);
+    for_groups(all_groups(), [&](int g, hipStream_t st) { groups_[g].lu->solve(db, true, st); });
 }
 
 void BatchLu::set_timing(bool on) {

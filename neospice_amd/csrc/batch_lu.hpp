@@ -104,6 +104,18 @@ private:
     double* d_vals_ = nullptr;
     bool timing_ = false, pend_rf_ = false, pend_sv_ = false;
     hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
+    // Pivot groups run concurrently: group 0 on the main stream, group g on
+    // side stream g-1, forked from and joined back into the main stream.
+    std::vector<hipStream_t> side_;
+    std::vector<hipEvent_t> join_ev_;
+    hipEvent_t fork_ev_ = nullptr;
+    template <class F>
+    void for_groups(const std::vector<int>& which, F&& launch);
+    std::vector<int> all_groups() const {
+        std::vector<int> w(groups_.size());
+        for (size_t g = 0; g < w.size(); ++g) w[g] = (int)g;
+        return w;
+    }
 };
 
 }  // namespace nsa

@@ -390,10 +390,20 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
     return __hiloint2double(hi, lo);
 }
 
+// Operands of one thread's (or wave's) first row of a level, loaded one
+// level ahead: the row record and its first terms' values and sources.
+struct RowPrefetch {
+    int4 ri;
+    double v[8];
+    int s[8];
+};
+
 // Runs one program level by level.  Each row's subtraction chain is
 // sequential in term order.  Thread mode: one thread per row, term loads
 // issued eight at a time.  Wave mode: one wave per row, the lanes form the
 // products of 64 terms at once and the chain then consumes them in order.
+// The next level's first-row operands are loaded while the current level
+// runs (they depend on nothing the current level writes).
 template <bool kSV>
 __device__ void run_program(const DevProgram& p, double* y, const double* LU, const double* SV) {
     const int tid = threadIdx.x;
@@ -401,6 +411,7 @@ __device__ void run_program(const DevProgram& p, double* y, const double* LU, co
     const int w = __builtin_amdgcn_readfirstlane(tid / kWave);
     const bool skip = (p.op & OP_SKIPZERO) != 0;
     const bool has_diag = (p.op & (OP_MUL_DIAG | OP_DIV_DIAG | OP_MUL_NZ)) != 0;
+    auto val = [&](int q) { return kSV ? SV[p.sv_base + q] : LU[p.pos[q]]; };
     auto finish = [&](double acc, int dix) {
         if (has_diag) {
             const double d = kSV ? SV[dix] : LU[dix];
@@ -410,20 +421,49 @@ __device__ void run_program(const DevProgram& p, double* y, const double* LU, co
         }
         return acc;
     };
+    // Loads are clamped rather than skipped: term arrays carry padding.
+    auto prefetch = [&](int l, RowPrefetch& f) {
+        const int r0 = p.level_ptr[l], r1 = p.level_ptr[l + 1];
+        const bool tmode = p.level_mode[l] == 0;
+        f.ri = p.rows[min((tmode ? r0 + tid : r0 + w), r1 - 1)];
+        const int last = max(f.ri.z - 1, 0);
+        if (tmode) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int q = f.ri.y + min(j, last);
+                f.v[j] = val(q);
+                f.s[j] = p.src[q];
+            }
+        } else {
+            const int q = f.ri.y + min(lane, last);
+            f.v[0] = val(q);
+            f.s[0] = p.src[q];
+        }
+    };
+    if (p.nlevels == 0) return;
+    RowPrefetch cur, nxt;
+    prefetch(0, cur);
     for (int l = 0; l < p.nlevels; ++l) {
+        if (l + 1 < p.nlevels) prefetch(l + 1, nxt);
         const int r0 = p.level_ptr[l], r1 = p.level_ptr[l + 1];
         if (p.level_mode[l] == 0) {
             for (int idx = r0 + tid; idx < r1; idx += kSvThreads) {
-                const int4 ri = p.rows[idx];
+                const bool first = idx < r0 + kSvThreads;
+                const int4 ri = first ? cur.ri : p.rows[idx];
                 double acc = y[ri.x];
                 for (int j0 = 0; j0 < ri.z; j0 += 8) {
                     double v[8];
                     int sx[8];
+                    if (first && j0 == 0) {
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        const int q = ri.y + min(j0 + j, ri.z - 1);
-                        v[j] = kSV ? SV[p.sv_base + q] : LU[p.pos[q]];
-                        sx[j] = p.src[q];
+                        for (int j = 0; j < 8; ++j) { v[j] = cur.v[j]; sx[j] = cur.s[j]; }
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            const int q = ri.y + min(j0 + j, ri.z - 1);
+                            v[j] = val(q);
+                            sx[j] = p.src[q];
+                        }
                     }
 #pragma unroll
                     for (int j = 0; j < 8; ++j) {
@@ -441,12 +481,21 @@ __device__ void run_program(const DevProgram& p, double* y, const double* LU, co
             }
         } else {
             for (int idx = r0 + w; idx < r1; idx += kSvThreads / kWave) {
-                const int4 ri = p.rows[idx];
+                const bool first = idx < r0 + kSvThreads / kWave;
+                const int4 ri = first ? cur.ri : p.rows[idx];
                 double acc = y[ri.x];
                 for (int c = 0; c < ri.z; c += kWave) {
-                    const int q = ri.y + min(c + lane, ri.z - 1);
-                    const double v = kSV ? SV[p.sv_base + q] : LU[p.pos[q]];
-                    const double ys = y[p.src[q]];
+                    double v;
+                    int sx;
+                    if (first && c == 0) {
+                        v = cur.v[0];
+                        sx = cur.s[0];
+                    } else {
+                        const int q = ri.y + min(c + lane, ri.z - 1);
+                        v = val(q);
+                        sx = p.src[q];
+                    }
+                    const double ys = y[sx];
                     // a skipped term subtracts +0.0, which leaves acc unchanged
                     const double prod = skip ? (ys != 0.0 ? ys * v : 0.0) : v * ys;
                     const int m = min(kWave, ri.z - c);
@@ -457,6 +506,7 @@ __device__ void run_program(const DevProgram& p, double* y, const double* LU, co
             }
         }
         __syncthreads();
+        cur = nxt;
     }
 }
 
@@ -777,6 +827,7 @@ void GpuLu::upload_program(const TriProgram& t, DevProgram& d, std::vector<int32
     }
     if (sv_pos) {
         sv_pos->insert(sv_pos->end(), pos.begin(), pos.end());
+        sv_pos->push_back(0);   // padding for clamped prefetches
         if (has_diag) {
             for (size_t idx = 0; idx < t.order.size(); ++idx) {
                 const int tg = rows[idx].x;
@@ -787,6 +838,9 @@ void GpuLu::upload_program(const TriProgram& t, DevProgram& d, std::vector<int32
         }
     }
     if (s_.n > 32767) throw Error(NSA_ERR_UNSUPPORTED, "GpuLu: solve sources exceed int16");
+    // padding: clamped prefetches of a row without terms read one past the end
+    src.push_back(0);
+    pos.push_back(0);
     d.level_ptr = upload(allocs_, t.level_ptr);
     d.level_mode = upload(allocs_, mode);
     d.rows = upload(allocs_, rows);
@@ -795,7 +849,7 @@ void GpuLu::upload_program(const TriProgram& t, DevProgram& d, std::vector<int32
 }
 
 void GpuLu::refactor(const double* dA, double gmin, const int32_t* dkind, int32_t* dfail,
-                     int32_t* dfail_col) {
+                     int32_t* dfail_col, hipStream_t on) {
     RfArgs a{};
     a.n = s_.n; a.T0 = T0_; a.t = tail_; a.nlev = nlev_; a.wsz = wsz_; a.wregion = wregion_; a.nws = nws_;
     a.union_bytes = (int)(rf_lds_ - (size_t)std::max(tail_, 1) * 8);
@@ -813,11 +867,12 @@ void GpuLu::refactor(const double* dA, double gmin, const int32_t* dkind, int32_
     a.A = dA; a.LU = dLU_; a.nlu = s_.nlu(); a.B = B_; a.imap = d_imap_; a.Bg = Bglobal_;
     a.gmin = gmin; a.form = (int)s_.form; a.pivot_tol = 1e-3; a.kind = dkind;
     a.fail = dfail; a.fail_col = dfail_col;
-    hipLaunchKernelGGL(k_refactor, dim3(B_), dim3(kRfThreads), rf_lds_, stream_, a);
+    hipLaunchKernelGGL(k_refactor, dim3(B_), dim3(kRfThreads), rf_lds_, on ? on : stream_, a);
     NSA_HIP(hipGetLastError());
 }
 
-void GpuLu::solve(double* db, bool transposed) {
+void GpuLu::solve(double* db, bool transposed, hipStream_t on) {
+    hipStream_t st = on ? on : stream_;
     SolveArgs a{};
     a.n = s_.n;
     // plain: y[k] = b[row_perm[k]]; L then U; x[col_perm[k]] = y[k]
@@ -835,9 +890,9 @@ void GpuLu::solve(double* db, bool transposed) {
     a.B = B_;
     a.Bg = Bglobal_;
     if (transposed)
-        hipLaunchKernelGGL(k_solve<false>, dim3(B_), dim3(kSvThreads), (size_t)s_.n * 8, stream_, a);
+        hipLaunchKernelGGL(k_solve<false>, dim3(B_), dim3(kSvThreads), (size_t)s_.n * 8, st, a);
     else
-        hipLaunchKernelGGL(k_solve<true>, dim3(B_), dim3(kSvThreads), (size_t)s_.n * 8, stream_, a);
+        hipLaunchKernelGGL(k_solve<true>, dim3(B_), dim3(kSvThreads), (size_t)s_.n * 8, st, a);
     NSA_HIP(hipGetLastError());
 }
 

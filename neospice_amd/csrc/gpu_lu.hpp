@@ -65,12 +65,13 @@ public:
     // numeric whose pivot order must be re-checked, 2 numeric on an order
     // just computed from these values, 3 skip.  Failures go to fail[inst]
     // (FailCode) and fail_col[inst].  Asynchronous on the stream.
+    // `on` overrides the plan's stream.
     void refactor(const double* dA, double diag_gmin, const int32_t* dkind, int32_t* dfail,
-                  int32_t* dfail_col);
+                  int32_t* dfail_col, hipStream_t on = nullptr);
 
     // In-place solve of b (device, n x Bglobal) for this plan's instances;
     // transposed = A^T x = b.
-    void solve(double* db, bool transposed);
+    void solve(double* db, bool transposed, hipStream_t on = nullptr);
 
     const LuStructure& structure() const { return s_; }
     int batch() const { return B_; }
