@@ -95,32 +95,46 @@ def cpu_baseline(ckt, info, R, dt, nsteps, budget_s, parity_steps, x_gpu0):
     for tid in range(cores):
         vals = {int(d): float(R[k, tid % R.shape[1]]) for k, d in enumerate(res_devs)}
         circuits.append(RB.RefCircuit(ckt, values=vals, options={"xmu": 0.0}, probe=False))
-    tot = [[0, 0.0, 0] for _ in range(cores)]   # iterations, stepping seconds, runs
-    deadline = time.perf_counter() + budget_s
+    def leg(nthreads, seconds):
+        tot = [[0, 0.0, 0] for _ in range(nthreads)]   # iterations, stepping seconds, runs
+        deadline = time.perf_counter() + seconds
 
-    def worker(tid):
-        while time.perf_counter() < deadline:
-            _, its, secs = circuits[tid].tran_fixed(dt, nsteps)
-            tot[tid][0] += its
-            tot[tid][1] += secs
-            tot[tid][2] += 1
+        def worker(tid):
+            while time.perf_counter() < deadline:
+                _, its, secs = circuits[tid].tran_fixed(dt, nsteps)
+                tot[tid][0] += its
+                tot[tid][1] += secs
+                tot[tid][2] += 1
 
-    t0 = time.perf_counter()
-    th = [threading.Thread(target=worker, args=(t,)) for t in range(cores)]
-    for t in th:
-        t.start()
-    for t in th:
-        t.join()
-    wall = time.perf_counter() - t0
-    rate = sum(it / s for it, s, _ in tot if s > 0)
-    iters = sum(it for it, _, _ in tot)
-    runs = sum(r for _, _, r in tot)
+        t0 = time.perf_counter()
+        th = [threading.Thread(target=worker, args=(t,)) for t in range(nthreads)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        wall = time.perf_counter() - t0
+        rate = sum(it / sec for it, sec, _ in tot if sec > 0)
+        return rate, sum(it for it, _, _ in tot), sum(r for _, _, r in tot), wall
+
+    rate1, _, _, _ = leg(1, budget_s * 0.25)
+    rate, iters, runs, wall = leg(cores, budget_s * 0.75)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     cpu = {"value": round(rate, 2), "unit": "Newton iters/s", "cores": cores,
            "kind": "reference",
+           "value_1core": round(rate1, 2),
+           "cpu_model": model,
            "sample": "%d runs of %d BE steps (%d Newton iterations) of the 64x64 mesh, one "
-                     "instance per host thread on %d threads within %.1f s; rate = sum over "
-                     "threads of iterations / stepping time; reference CPU path: newton_solve "
-                     "with AMD-LU replay refactor/solve and device loads" %
+                     "instance per host thread on %d threads within %.1f s (plus a 1-thread leg); "
+                     "rate = sum over threads of iterations / stepping time; reference CPU path: "
+                     "newton_solve with AMD-LU replay refactor/solve and device loads" %
                      (runs, nsteps, iters, cores, wall)}
     return cpu, parity
 
@@ -159,19 +173,23 @@ def main():
     log("bench: DC preamble done, solver", eng.solver_name)
     run_steps(eng, na, dt, True, args.warmup, 0)
     torch.cuda.synchronize()
-    eng.set_timing(True)
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     iters = run_steps(eng, na, dt, False, args.steps, args.warmup)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    timing = eng.timing()
-    eng.set_timing(False)
     value = iters / wall
     log("bench: batched %d steps, %d instance-iterations in %.3f s -> %.1f iters/s"
         % (args.steps, iters, wall, value))
+    # phase breakdown on a few more steps, HIP events on the engine's stream
+    nph = max(1, min(5, args.steps))
+    eng.set_timing(True)
+    it_ph = run_steps(eng, na, dt, False, nph, args.warmup + args.steps)
+    torch.cuda.synchronize()
+    timing = eng.timing()
+    eng.set_timing(False)
     refactor_bytes, solve_bytes, st = lu_traffic(eng, ckt, B)
     x_gpu0 = eng.solution()[:, 0].copy()
+    total_steps = args.warmup + args.steps + nph
 
     # batch-1 GPU path: the same kernels, one instance
     e1 = na.BatchEngine(ckt, batch=1, xmu=0.0)
@@ -188,9 +206,10 @@ def main():
     log("bench: batch-1 %d steps, %d iterations in %.3f s -> %.1f iters/s"
         % (args.batch1_steps, it1, w1, batch1_value))
 
-    ms_refactor = timing["refactor_ms"] / max(1, iters // B)
-    ms_solve = timing["solve_ms"] / max(1, iters // B)
-    ms_load = timing["load_ms"] / max(1, iters // B)
+    nit = max(1, it_ph // B)      # batched Newton iterations in the phase run
+    ms_refactor = timing["refactor_ms"] / nit
+    ms_solve = timing["solve_ms"] / nit
+    ms_load = timing["load_ms"] / nit
     fl = refactor_flops(e1, ckt, B)
     phases = {
         "load_gather_ms": ms_load, "refactor_ms": ms_refactor, "solve_ms": ms_solve,
@@ -199,19 +218,22 @@ def main():
         "refactor_flops": fl,
         "solve_GBs": solve_bytes / (ms_solve * 1e-3) / 1e9,
         "levels_refactor": st["levels_refactor"], "levels_solve": st["levels_solve"],
-        "nnz_lu": st["nnz_lu"],
+        "nnz_lu": st["nnz_lu"], "pivot_groups": st["pivot_groups"],
     }
     achieved = refactor_bytes / (ms_refactor * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "k_refactor_level (all levels of one refactor)",
-                "bytes_per_launch": refactor_bytes}
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                "kernel": "k_refactor",
+                "launch_ms": round(ms_refactor, 4),
+                "bytes_per_launch": refactor_bytes,
+                "flops_per_launch": fl,
+                "fp64_TFLOPs": round(fl / (ms_refactor * 1e-3) / 1e12, 4),
+                "fp64_frac": round(fl / (ms_refactor * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 6)}
 
     cpu, parity = None, None
     if not args.no_cpu:
         try:
-            cpu, parity = cpu_baseline(ckt, info, R, dt, 10, args.cpu_seconds,
-                                       args.warmup + args.steps, x_gpu0)
+            cpu, parity = cpu_baseline(ckt, info, R, dt, 10, args.cpu_seconds, total_steps, x_gpu0)
             log("bench: instance 0 vs reference CPU path, max |dx| =", parity)
         except Exception as e:  # the baseline is reported, not required
             log("bench: cpu baseline unavailable:", e)

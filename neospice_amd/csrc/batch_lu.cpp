@@ -163,6 +163,9 @@ void BatchLu::run_gpu(const double* dA, double gmin, const std::vector<int32_t>&
                            stream_));
     NSA_HIP(hipStreamSynchronize(stream_));
     std::memcpy(fail.data(), h_fail_pin_, sizeof(int32_t) * B_);
+    for (int i = 0; i < B_; ++i)
+        if (fail[i] == FAIL_TIMEOUT)
+            throw Error(NSA_ERR_RUNTIME, "GPU refactor: a dependency wait ran out of its bound");
     collect_timing();
 }
 

@@ -38,6 +38,7 @@ enum FailCode : int32_t {
     FAIL_ZERO_PIVOT = 1,   // exact zero pivot (refactorize) or singular (numeric)
     FAIL_GROWTH = 2,       // GP replay pivot-growth check failed
     FAIL_REORDER = 3,      // pivot no longer threshold-acceptable: CPU re-pivot
+    FAIL_TIMEOUT = 4,      // a kernel's bounded wait for a dependency ran out (a bug)
 };
 
 }  // namespace nsa

@@ -27,12 +27,19 @@ constexpr int kWave = 64;
 constexpr int kRfThreads = 1024;              // refactor workgroup: 16 waves
 constexpr int kRfWaves = kRfThreads / kWave;
 constexpr int kSvThreads = 512;               // solve workgroup
-constexpr int kPf = 8;                        // U-entry steps whose operands are in flight
-
+constexpr int kPf = 16;                       // U-entry steps whose operands are in flight
+constexpr int kContinued = 1 << 30;           // step descriptor: continuation chunk of an L column
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __hiloint2double(hi, lo);
 }
 
 __device__ __forceinline__ double wave_max(double v) {
@@ -58,16 +65,24 @@ __device__ __forceinline__ void flag(int32_t* fail, int32_t* fail_col, int ginst
 
 struct RfArgs {
     int n, T0, t, nlev, wsz, wregion, nws, union_bytes;
-    const int32_t *lev_ptr, *lev_cols;
+    const int32_t* lev_ptr;   // nlev + 1 into the column positions
+    const int4* cmeta;        // per column position: (k, cb, len, nU), (d0, nd, 0, 0)
     const int32_t *cptr, *nU, *rows, *a_src, *gmin_pos, *desc_ptr;
-    // per step: (U slot | chunk length << 16, first L position of the chunk,
-    // diagonal position of r, offset of the chunk's relative slots)
+    // per step: (U slot | chunk length << 16 | continuation flag, first L
+    // position of the chunk, pivot column r, offset of the chunk's relative
+    // slots)
     const int4* desc;
     const int16_t* rel;    // per L entry of a chunk: slot in the column, -1 = not here
     const int32_t *cand_ptr, *cand_pos, *diag_cand;
-    const int32_t *tpos_ptr, *tpos, *twidx, *lrow_ptr, *ucol_ptr, *urow_ptr, *tcand_ptr, *tdiag_cand;
-    const int16_t *lrow, *ucol, *urow, *tcand;
-    const int32_t *dstep_ptr, *dtask_widx, *dtask_ptr, *dterm;
+    const int32_t *tposmap, *tcand_ptr, *tdiag_cand;   // tposmap: 16 LU positions per thread (-1: none)
+    const int16_t* tcand;
+    const uint32_t* tmask;   // per trailing step: L rows, U columns, U rows (3 x 128 bits)
+    // deferred updates: per trailing step a table index (-1: none); per table
+    // and thread a task range; per task (register, first product, end)
+    const int32_t *dstep_ptr, *dtask_ptr, *dterm;
+    const int4* dtask;
+    int ndterm;
+    double* dprod;         // per instance: ndterm deferred products
     const int32_t* sv_pos;
     int nsv;
     double* SV;
@@ -86,73 +101,59 @@ struct RfArgs {
 // One column by one wave.  partial: a trailing-block column, only the
 // updates from pivots before the block; its values stay raw for the dense
 // phase and its running |U| maximum goes to cmx.
-__device__ __forceinline__ void rf_column(const RfArgs& a, int k, bool partial, double* ws,
-                                          int4* wdesc, double* LU, int ginst, int kind, double* cmx) {
+__device__ __forceinline__ void rf_column(const RfArgs& a, const int4 m0, const int4 m1, bool partial,
+                                          double* ws, int4* wdesc, double* LU, int ginst, int kind,
+                                          double* cmx) {
     const int lane = threadIdx.x & (kWave - 1);
     const bool s13 = a.form == FORM_S13;
-    const int cb = a.cptr[k];
-    const int len = a.cptr[k + 1] - cb;
-    const int gp = a.gmin_pos[k];
-    // Column load: originals from A, fill zero, diag_gmin after the load
-    // (NeoSolver::load_real + add_diag_gmin; GP scatter + x[col] += gmin).
-    // Four chunks of loads are in flight at once.
+    const int k = m0.x, cb = m0.y, len = m0.z;
+    // Column load: the loaded values (A plus diag_gmin) already sit in the
+    // column's own positions; four chunks in flight at once.
     for (int s0 = 0; s0 < len; s0 += 4 * kWave) {
-        int src[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) src[c] = a.a_src[cb + min(s0 + c * kWave + lane, len - 1)];
         double v[4];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) v[c] = a.A[(size_t)max(src[c], 0) * a.Bg + ginst];
+        for (int c = 0; c < 4; ++c) v[c] = LU[cb + min(s0 + c * kWave + lane, len - 1)];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int sl = s0 + c * kWave + lane;
-            if (sl < len) {
-                double x = src[c] >= 0 ? v[c] : 0.0;
-                if (a.gmin != 0.0 && cb + sl == gp) x += a.gmin;
-                ws[sl] = x;
-            }
-        }
+        for (int c = 0; c < 4; ++c)
+            if (s0 + c * kWave + lane < len) ws[s0 + c * kWave + lane] = v[c];
     }
     wave_sync();
 
-    const int d0 = a.desc_ptr[k];
-    const int nd = a.desc_ptr[k + 1] - d0;
-    double colmax = 0.0;
+    const int d0 = m1.x;
+    const int nd = m1.y;
     // The column's step descriptors are staged in the wave's LDS; the
     // descriptor of step u + kPf is read together with step u's operands.
     for (int u = lane; u < nd; u += kWave) wdesc[u] = a.desc[d0 + u];
     wave_sync();
-    // Ring of prefetched steps.  Per step: U slot and chunk length packed and
-    // r's diagonal value; per lane: its two relative slots and its two L
-    // values.  Every load is unconditional (indices clamped) and the main
-    // loop is straight-line, so each wait covers exactly the step it feeds.
+    // Ring of prefetched steps.  Per step: U slot and chunk length packed
+    // (wave-uniform) and r's stored diagonal (Sparse 1.3); per lane: its
+    // relative slot and its L value.  Every load is unconditional (indices
+    // clamped) and the main loop is straight-line, so each wait covers
+    // exactly the step it feeds.  U values stay in the column workspace
+    // (no later step of the column reads or targets their slots) and are
+    // written out with the column.
     int usnl[kPf];
-    int16_t q0r[kPf], q1r[kPf];
-    double l0[kPf], l1[kPf], dg[kPf];
+    int16_t qr[kPf];
+    double lv[kPf], dg[kPf];
     auto fetch = [&](const int4 d, int p) {
         usnl[p] = d.x;
-        const int nl = d.x >> 16;
-        const int last = nl > 0 ? nl - 1 : 0;
-        const int e0 = min(lane, last), e1 = min(lane + kWave, last);
-        q0r[p] = a.rel[d.w + e0];
-        q1r[p] = a.rel[d.w + e1];
-        l0[p] = LU[d.y + e0];
-        l1[p] = LU[d.y + e1];
-        dg[p] = LU[d.z];
+        const int nl = (d.x >> 16) & 0x7f;
+        const int e = min(lane, nl > 0 ? nl - 1 : 0);
+        qr[p] = a.rel[d.w + e];
+        lv[p] = LU[d.y + e];
+        if (s13) dg[p] = LU[d.y - 1];   // r's stored diagonal (first chunk: just above L(:, r))
     };
     auto step = [&](int p) {
         const int us = usnl[p] & 0xffff;
-        const int nl = usnl[p] >> 16;
-        const bool m0 = lane < nl && q0r[p] >= 0, m1 = lane + kWave < nl && q1r[p] >= 0;
-        // the pivot row's slot is never a target of its own step, so the
-        // three reads go out together
+        const int nl = (usnl[p] >> 16) & 0x7f;
+        const bool scale = s13 && !(usnl[p] & kContinued);   // a continuation finds U(r,j) in place
+        const bool m = lane < nl && qr[p] >= 0;
+        // the pivot row's slot is never a target of its own step
         const double xr = ws[us];
-        const double t0 = ws[m0 ? q0r[p] : 0], t1 = ws[m1 ? q1r[p] : 0];
-        const double uval = s13 ? xr * dg[p] : xr;
-        LU[cb + us] = uval;   // every lane stores the same value
-        colmax = fmax(colmax, fabs(uval));
-        if (m0) ws[q0r[p]] = t0 - uval * l0[p];
-        if (m1) ws[q1r[p]] = t1 - uval * l1[p];
+        const double tv = ws[m ? qr[p] : 0];
+        const double uval = scale ? xr * dg[p] : xr;
+        if (m) ws[qr[p]] = tv - uval * lv[p];
+        if (scale && lane == 0) ws[us] = uval;
         wave_sync();
     };
     if (nd > 0) {
@@ -171,6 +172,17 @@ __device__ __forceinline__ void rf_column(const RfArgs& a, int k, bool partial, 
         for (int p = 0; p < kPf; ++p)
             if (nfull * kPf + p < nd) step(p);
     }
+    // U part of the column: write out, and its largest magnitude
+    const int nu = m0.w;
+    double colmax = 0.0;
+    for (int u = lane; u < nu; u += kWave) {
+        const double v = ws[u];
+        if (!partial || a.rows[cb + u] < a.T0) {
+            LU[cb + u] = v;
+            colmax = fmax(colmax, fabs(v));
+        }
+    }
+    if (!s13) colmax = wave_max(colmax);
 
     if (partial) {
         for (int s = lane; s < len; s += kWave)
@@ -178,7 +190,7 @@ __device__ __forceinline__ void rf_column(const RfArgs& a, int k, bool partial, 
         if (lane == 0) cmx[k - a.T0] = colmax;
         return;
     }
-    const int dslot = a.nU[k];
+    const int dslot = m0.w;
     const double xd = ws[dslot];
     if (s13) {
         double lmax = 0.0;
@@ -230,120 +242,213 @@ __device__ __forceinline__ void rf_column(const RfArgs& a, int k, bool partial, 
     }
 }
 
-// The trailing block of one instance, right-looking in LDS by the whole
-// workgroup: W[j * t + i] = entry (tail row i, tail column j).
-__device__ __forceinline__ void rf_tail(const RfArgs& a, double* W, double* LU, int ginst, int kind,
-                                        const double* cmx, double* sh_piv) {
+__device__ __forceinline__ bool bit128(const uint32_t* m, int i) { return (m[i >> 5] >> (i & 31)) & 1u; }
+
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+// The trailing block of one instance (t <= 128), right-looking, held in
+// registers: thread (ti, tj) = (tid % 32, tid / 32) owns entries
+// (ti + 32a, tj + 32b), a, b < 4.  Per step s the wave holding column s
+// checks the pivot and publishes the column (L, scaled for AMD-LU), the
+// owners of row s publish the row, and after one barrier every thread
+// applies the rank-1 update to its entries -- exactly where L(i,s) and
+// U(s,j) are both in the pattern (128-bit masks per step), as in the
+// sparse reference.  Deferred sparse updates of a step are applied by the
+// owner of their target entry.
+__device__ __forceinline__ void rf_tail(const RfArgs& a, double* lds, double* LU, int ginst, int kind,
+                                        double* cmx, uint32_t* msk, const double* dprod) {
     const int t = a.t;
     const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int ti = tid & 31, tj = tid >> 5;
+    const int wv = tid / kWave;
     const bool s13 = a.form == FORM_S13;
-    for (int i = tid; i < t * t; i += kRfThreads) W[i] = 0.0;
+    double* Lcol = lds;          // 2 x 128 (double-buffered by step parity)
+    double* Urow = lds + 256;    // 2 x 128
+    double* shpv = lds + 512;    // 2
+    double* colbuf = lds + 516;  // 128: raw pivot column for the candidate check
+    for (int i = tid; i < t * 12; i += kRfThreads) msk[i] = a.tmask[i];
+    double R[4][4];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int pos = a.tposmap[tid * 16 + q];
+        R[q >> 2][q & 3] = pos >= 0 ? LU[pos] : 0.0;
+    }
     __syncthreads();
-    const int npos = a.tpos_ptr[t];
-    for (int q = tid; q < npos; q += kRfThreads) W[a.twidx[q]] = LU[a.tpos[q]];
-    __syncthreads();
-    int code_all = FAIL_NONE, fcol = 0x7fffffff;
     for (int s = 0; s < t; ++s) {
-        double* colS = W + (size_t)s * t;
-        if (tid < kWave) {
-            const int lane = tid;
-            const double piv = colS[s];
+        const int buf = s & 1;
+        const int ts = s & 31, bs = s >> 5;    // column s: owners tj == ts, register column bs
+        const uint32_t* lm = msk + 12 * s;     // L rows of column s
+        const uint32_t* um = lm + 4;           // U columns of row s
+        if (wv == (ts >> 1)) {
+            // the wave holding column s (its half tj == ts owns it)
+            const bool own = tj == ts;
+            const int src = ts + 32 * (ts & 1);    // lane owning (s, s)
+            double cv[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                cv[q] = bs == 0 ? R[q][0] : bs == 1 ? R[q][1] : bs == 2 ? R[q][2] : R[q][3];
+            const double mine = bs == 0 ? cv[0] : bs == 1 ? cv[1] : bs == 2 ? cv[2] : cv[3];
+            const double piv = readlane_f64(mine, src);
             int code = FAIL_NONE;
             if (!s13) {
-                double cm = 0.0;
-                for (int q = a.urow_ptr[s] + lane; q < a.urow_ptr[s + 1]; q += kWave)
-                    cm = fmax(cm, fabs(colS[a.urow[q]]));
-                cm = wave_max(cm);
-                if (lane == 0) {
-                    cm = fmax(cm, cmx[s]);
-                    const double aukk = fabs(piv);
-                    if (aukk > cm) cm = aukk;
-                    if (kind == 0 && (piv == 0.0 || (cm > 0.0 && aukk < a.pivot_tol * cm)))
-                        code = FAIL_GROWTH;
-                    if (kind == 1) {
-                        const int c0 = a.tcand_ptr[s], c1 = a.tcand_ptr[s + 1];
-                        int ipiv = -1;
-                        double amax = -1.0;
-                        for (int c = c0; c < c1; ++c) {
-                            const double v = fabs(colS[a.tcand[c]]);
-                            if (v > amax) { amax = v; ipiv = c - c0; }
-                        }
-                        const int dc = a.tdiag_cand[s];
-                        if (ipiv < 0 || amax <= 0.0) {
-                            code = FAIL_ZERO_PIVOT;
-                        } else {
-                            if (dc >= 0 && fabs(colS[a.tcand[c0 + dc]]) >= amax * a.pivot_tol) ipiv = dc;
-                            if (a.tcand[c0 + ipiv] != s) code = FAIL_REORDER;
+                double cm = fmax(cmx[s], 0.0);
+                const double aukk = fabs(piv);
+                if (aukk > cm) cm = aukk;
+                if (kind == 0 && (piv == 0.0 || (cm > 0.0 && aukk < a.pivot_tol * cm))) code = FAIL_GROWTH;
+                if (kind == 1) {
+                    // first maximum over the candidates in reach order, then
+                    // the diagonal within tolerance (amd_lu_solver.cpp:145-160)
+                    if (own)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) colbuf[ti + 32 * q] = cv[q];
+                    wave_sync();
+                    const int c0 = a.tcand_ptr[s], nc = a.tcand_ptr[s + 1] - c0;
+                    double v0 = -1.0, v1 = -1.0;
+                    if (lane < nc) v0 = fabs(colbuf[a.tcand[c0 + lane]]);
+                    if (lane + kWave < nc) v1 = fabs(colbuf[a.tcand[c0 + lane + kWave]]);
+                    const double amax = wave_max(fmax(v0, v1));
+                    int c = 0x7fffffff;
+                    if (lane + kWave < nc && v1 == amax) c = lane + kWave;
+                    if (lane < nc && v0 == amax) c = lane;
+                    int ipiv = wave_min_i(c);
+                    const int dc = a.tdiag_cand[s];
+                    if (nc == 0 || ipiv == 0x7fffffff || amax <= 0.0) {
+                        code = FAIL_ZERO_PIVOT;
+                    } else {
+                        if (dc >= 0 && fabs(colbuf[a.tcand[c0 + dc]]) >= amax * a.pivot_tol) ipiv = dc;
+                        if (a.tcand[c0 + ipiv] != s) code = FAIL_REORDER;
+                    }
+                }
+                // L(i, s) = x_i / U(s, s)
+                if (own) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int i = ti + 32 * q;
+                        if (i > s && i < t && bit128(lm, i)) {
+                            cv[q] = cv[q] / piv;
+                            Lcol[buf * 128 + i] = cv[q];
                         }
                     }
-                    *sh_piv = piv;
                 }
             } else {
-                double lm = 0.0;
                 if (kind == 1) {
-                    for (int q = a.lrow_ptr[s] + lane; q < a.lrow_ptr[s + 1]; q += kWave)
-                        lm = fmax(lm, fabs(colS[a.lrow[q]]));
-                    lm = wave_max(lm);
+                    double m = 0.0;
+                    if (own)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int i = ti + 32 * q;
+                            if (i > s && i < t && bit128(lm, i)) m = fmax(m, fabs(cv[q]));
+                        }
+                    m = wave_max(m);
+                    if (!(m * 1e-3 < fabs(piv))) code = FAIL_REORDER;
+                } else if (piv == 0.0) {
+                    code = FAIL_ZERO_PIVOT;
                 }
-                if (lane == 0) {
-                    if (kind == 1) {
-                        if (!(lm * 1e-3 < fabs(piv))) code = FAIL_REORDER;
-                    } else if (piv == 0.0) {
-                        code = FAIL_ZERO_PIVOT;
+                const double pv = (piv != 0.0) ? 1.0 / piv : piv;   // the diagonal stores 1 / pivot
+                if (own) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int i = ti + 32 * q;
+                        if (i > s && i < t && bit128(lm, i)) Lcol[buf * 128 + i] = cv[q];
+                        if (i == s) cv[q] = pv;
                     }
-                    *sh_piv = (piv != 0.0) ? 1.0 / piv : piv;
+                }
+                if (lane == src) shpv[buf] = pv;
+            }
+            if (own) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (bs == 0) R[q][0] = cv[q];
+                    else if (bs == 1) R[q][1] = cv[q];
+                    else if (bs == 2) R[q][2] = cv[q];
+                    else R[q][3] = cv[q];
                 }
             }
-            if (lane == 0 && code != FAIL_NONE) {
-                if (code > code_all) code_all = code;
-                if (a.T0 + s < fcol) fcol = a.T0 + s;
+            if (lane == src && code != FAIL_NONE) flag(a.fail, a.fail_col, ginst, code, a.T0 + s);
+        }
+        if (ti == ts) {
+            // owners of row s: publish U(s, j) (raw) and fold it into the
+            // columns' running |U| maxima
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int j = tj + 32 * b;
+                const double raw = bs == 0 ? R[0][b] : bs == 1 ? R[1][b] : bs == 2 ? R[2][b] : R[3][b];
+                if (j > s && j < t && bit128(um, j)) {
+                    Urow[buf * 128 + j] = raw;
+                    if (!s13) cmx[j] = fmax(cmx[j], fabs(raw));
+                }
             }
         }
         __syncthreads();
-        const double pv = *sh_piv;
-        if (!s13) {
-            for (int q = a.lrow_ptr[s] + tid; q < a.lrow_ptr[s + 1]; q += kRfThreads) {
-                const int i = a.lrow[q];
-                colS[i] = colS[i] / pv;
-            }
-        } else {
-            for (int q = a.ucol_ptr[s] + tid; q < a.ucol_ptr[s + 1]; q += kRfThreads) {
-                const int j = a.ucol[q];
-                W[(size_t)j * t + s] = W[(size_t)j * t + s] * pv;
-            }
-            if (tid == 0) colS[s] = pv;
+        const double pv = s13 ? shpv[buf] : 1.0;
+        double lv[4], uv[4];
+        bool lok[4], uok[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = ti + 32 * q, j = tj + 32 * q;
+            lok[q] = i > s && i < t && bit128(lm, i);
+            uok[q] = j > s && j < t && bit128(um, j);
+            lv[q] = lok[q] ? Lcol[buf * 128 + i] : 0.0;
+            const double ur = uok[q] ? Urow[buf * 128 + j] : 0.0;
+            // U(s, j) = x_s(j) * (1 / pivot) for Sparse 1.3, x_s(j) for AMD-LU
+            uv[q] = s13 ? ur * pv : ur;
         }
-        __syncthreads();
-        const int l0 = a.lrow_ptr[s], nl = a.lrow_ptr[s + 1] - l0;
-        const int u0 = a.ucol_ptr[s], nuc = a.ucol_ptr[s + 1] - u0;
-        const int work = nl * nuc;
-        for (int q = tid; q < work; q += kRfThreads) {
-            const int i = a.lrow[l0 + q % nl];
-            const int j = a.ucol[u0 + q / nl];
-            W[(size_t)j * t + i] -= colS[i] * W[(size_t)j * t + s];
+        if (s13 && ti == ts) {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                if (!uok[b]) continue;
+                if (bs == 0) R[0][b] = uv[b];
+                else if (bs == 1) R[1][b] = uv[b];
+                else if (bs == 2) R[2][b] = uv[b];
+                else R[3][b] = uv[b];
+            }
         }
-        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                if (lok[q] && uok[b]) R[q][b] = R[q][b] - lv[q] * uv[b];
         // Updates from earlier (sparse) pivots that the reference's reach
-        // order places after pivot s in a column: one task per target entry,
-        // its terms in that order.
-        const int k0 = a.dstep_ptr[s], k1 = a.dstep_ptr[s + 1];
-        if (k1 > k0) {
-            for (int q = k0 + tid; q < k1; q += kRfThreads) {
-                double acc = W[a.dtask_widx[q]];
-                for (int e = a.dtask_ptr[q]; e < a.dtask_ptr[q + 1]; ++e)
-                    acc -= LU[a.dterm[2 * e]] * LU[a.dterm[2 * e + 1]];
-                W[a.dtask_widx[q]] = acc;
+        // order places after pivot s in a column, by the target's owner, in
+        // that order; the products were formed before the dense phase.
+        const int dt = a.dstep_ptr[s];
+        if (dt >= 0) {
+            const int k0 = a.dtask_ptr[dt * (kRfThreads + 1) + tid];
+            const int k1 = a.dtask_ptr[dt * (kRfThreads + 1) + tid + 1];
+            for (int k = k0; k < k1; ++k) {
+                const int4 tk = a.dtask[k];   // (register, first product, end, -)
+                double acc = 0.0;
+#pragma unroll
+                for (int q = 0; q < 16; ++q)
+                    if (tk.x == q) acc = R[q >> 2][q & 3];
+                for (int e = tk.y; e < tk.z; e += 16) {
+                    double pr[16];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) pr[u] = dprod[min(e + u, tk.z - 1)];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u)
+                        if (e + u < tk.z) acc -= pr[u];
+                }
+#pragma unroll
+                for (int q = 0; q < 16; ++q)
+                    if (tk.x == q) R[q >> 2][q & 3] = acc;
             }
-            __syncthreads();
         }
     }
-    for (int q = tid; q < npos; q += kRfThreads) LU[a.tpos[q]] = W[a.twidx[q]];
-    if (tid == 0) flag(a.fail, a.fail_col, ginst, code_all, fcol);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int pos = a.tposmap[tid * 16 + q];
+        if (pos >= 0) LU[pos] = R[q >> 2][q & 3];
+    }
 }
 
 __global__ __launch_bounds__(kRfThreads) void k_refactor(RfArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ double sh_piv;
     const int inst = block_instance(blockIdx.x, a.B);
     const int ginst = a.imap ? a.imap[inst] : inst;
     const int kind = a.kind ? a.kind[ginst] : 0;
@@ -351,19 +456,66 @@ __global__ __launch_bounds__(kRfThreads) void k_refactor(RfArgs a) {
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     double* LU = a.LU + (size_t)inst * a.nlu;
     double* cmx = reinterpret_cast<double*>(smem + a.union_bytes);
+    // after the union, the |U| maxima and the trailing masks: the level table
+    int* lvl = reinterpret_cast<int*>(smem + a.union_bytes + (size_t)a.t * 56);
     double* ws = reinterpret_cast<double*>(smem + (size_t)w * a.wregion);
     int4* wdesc = reinterpret_cast<int4*>(smem + (size_t)w * a.wregion + (size_t)a.wsz * 8);
-    for (int l = 0; l < a.nlev; ++l) {
-        if (w < a.nws)
-            for (int c = a.lev_ptr[l] + w; c < a.lev_ptr[l + 1]; c += a.nws)
-                rf_column(a, a.lev_cols[c], false, ws, wdesc, LU, ginst, kind, cmx);
+    // Load phase (NeoSolver::load_real + add_diag_gmin; GP scatter + gmin):
+    // every factor position takes its A value (fill: zero), eight loads in
+    // flight per thread.
+    for (int p0 = threadIdx.x; p0 < a.nlu; p0 += 8 * kRfThreads) {
+        int src[8];
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) src[u] = a.a_src[min(p0 + u * kRfThreads, a.nlu - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = a.A[(size_t)max(src[u], 0) * a.Bg + ginst];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (p0 + u * kRfThreads < a.nlu) LU[p0 + u * kRfThreads] = src[u] >= 0 ? v[u] : 0.0;
+    }
+    for (int l = threadIdx.x; l <= a.nlev; l += kRfThreads) lvl[l] = a.lev_ptr[l];
+    __syncthreads();
+    if (a.gmin != 0.0) {
+        for (int k = threadIdx.x; k < a.n; k += kRfThreads)
+            if (a.gmin_pos[k] >= 0) LU[a.gmin_pos[k]] += a.gmin;
+        __syncthreads();
+    }
+    // Sparse levels then the trailing columns' partial pass (level nlev).
+    // Column positions: level order for the sparse columns, then the
+    // trailing columns; each wave loads the metadata of its next column
+    // ahead.
+    int4 m0 = make_int4(0, 0, 0, 0), m1 = m0;
+    int pc = -1;   // column position whose metadata m0, m1 hold
+    auto meta = [&](int c, int4& x0, int4& x1) { x0 = a.cmeta[2 * c]; x1 = a.cmeta[2 * c + 1]; };
+    for (int l = 0; l <= a.nlev; ++l) {
+        const bool part = l == a.nlev;
+        const int cbeg = part ? a.T0 : lvl[l], cend = part ? a.n : lvl[l + 1];
+        if (w < a.nws) {
+            for (int c = cbeg + w; c < cend; c += a.nws) {
+                if (pc != c) meta(c, m0, m1);
+                // the wave's next column: in this level, else its first one
+                // of the next level (or of the partial pass)
+                int cn = c + a.nws;
+                if (cn >= cend) cn = (l + 1 < a.nlev) ? lvl[l + 1] + w : (l + 1 == a.nlev ? a.T0 + w : a.n);
+                int4 n0 = m0, n1 = m1;
+                if (cn < a.n) meta(cn, n0, n1);
+                rf_column(a, m0, m1, part, ws, wdesc, LU, ginst, kind, cmx);
+                m0 = n0;
+                m1 = n1;
+                pc = cn < a.n ? cn : -1;
+            }
+        }
         __syncthreads();
     }
     if (a.t > 0) {
-        if (w < a.nws)
-            for (int j = w; j < a.t; j += a.nws) rf_column(a, a.T0 + j, true, ws, wdesc, LU, ginst, kind, cmx);
+        // products L(i, r) * U(r, j) of the deferred updates, all final now
+        double* dprod = a.dprod + (size_t)inst * a.ndterm;
+        for (int e = threadIdx.x; e < a.ndterm; e += kRfThreads)
+            dprod[e] = LU[a.dterm[2 * e]] * LU[a.dterm[2 * e + 1]];
         __syncthreads();
-        rf_tail(a, reinterpret_cast<double*>(smem), LU, ginst, kind, cmx, &sh_piv);
+        rf_tail(a, reinterpret_cast<double*>(smem), LU, ginst, kind, cmx,
+                reinterpret_cast<uint32_t*>(smem + a.union_bytes + (size_t)a.t * 8), dprod);
         __syncthreads();
     }
     double* SV = a.SV + (size_t)inst * a.nsv;
@@ -383,15 +535,9 @@ struct SolveArgs {
     int B, Bg;
 };
 
-__device__ __forceinline__ double readlane_f64(double v, int l) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)b, l);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-    return __hiloint2double(hi, lo);
-}
-
 // Operands of one thread's (or wave's) first row of a level, loaded one
-// level ahead: the row record and its first terms' values and sources.
+// level ahead: the row record and its first terms' values and sources
+// (thread mode: 8 terms; wave mode: 8 chunks of 64).
 struct RowPrefetch {
     int4 ri;
     double v[8];
@@ -405,7 +551,8 @@ struct RowPrefetch {
 // The next level's first-row operands are loaded while the current level
 // runs (they depend on nothing the current level writes).
 template <bool kSV>
-__device__ void run_program(const DevProgram& p, double* y, const double* LU, const double* SV) {
+__device__ void run_program(const DevProgram& p, const int* lvl, const int* lmode, double* y,
+                            const double* LU, const double* SV) {
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(tid / kWave);
@@ -421,32 +568,35 @@ __device__ void run_program(const DevProgram& p, double* y, const double* LU, co
         }
         return acc;
     };
-    // Loads are clamped rather than skipped: term arrays carry padding.
-    auto prefetch = [&](int l, RowPrefetch& f) {
-        const int r0 = p.level_ptr[l], r1 = p.level_ptr[l + 1];
-        const bool tmode = p.level_mode[l] == 0;
-        f.ri = p.rows[min((tmode ? r0 + tid : r0 + w), r1 - 1)];
+    // Two-stage prefetch: the record of this thread's (wave's) first row
+    // two levels ahead, its term operands one level ahead.  Loads are
+    // clamped rather than skipped: the term arrays carry padding.
+    auto record = [&](int l) {
+        const int r0 = lvl[l], r1 = lvl[l + 1];
+        return p.rows[min((lmode[l] == 0 ? r0 + tid : r0 + w), r1 - 1)];
+    };
+    auto operands = [&](int l, RowPrefetch& f) {
+        const bool tmode = lmode[l] == 0;
         const int last = max(f.ri.z - 1, 0);
-        if (tmode) {
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int q = f.ri.y + min(j, last);
-                f.v[j] = val(q);
-                f.s[j] = p.src[q];
-            }
-        } else {
-            const int q = f.ri.y + min(lane, last);
-            f.v[0] = val(q);
-            f.s[0] = p.src[q];
+        for (int j = 0; j < 8; ++j) {
+            const int q = f.ri.y + min(tmode ? j : j * kWave + lane, last);
+            f.v[j] = val(q);
+            f.s[j] = p.src[q];
         }
     };
     if (p.nlevels == 0) return;
     RowPrefetch cur, nxt;
-    prefetch(0, cur);
+    int4 rec2 = make_int4(0, 0, 0, 0);
+    nxt.ri = rec2;
+    cur.ri = record(0);
+    operands(0, cur);
+    if (p.nlevels > 1) nxt.ri = record(1);
     for (int l = 0; l < p.nlevels; ++l) {
-        if (l + 1 < p.nlevels) prefetch(l + 1, nxt);
-        const int r0 = p.level_ptr[l], r1 = p.level_ptr[l + 1];
-        if (p.level_mode[l] == 0) {
+        if (l + 1 < p.nlevels) operands(l + 1, nxt);
+        if (l + 2 < p.nlevels) rec2 = record(l + 2);
+        const int r0 = lvl[l], r1 = lvl[l + 1];
+        if (lmode[l] == 0) {
             for (int idx = r0 + tid; idx < r1; idx += kSvThreads) {
                 const bool first = idx < r0 + kSvThreads;
                 const int4 ri = first ? cur.ri : p.rows[idx];
@@ -484,22 +634,32 @@ __device__ void run_program(const DevProgram& p, double* y, const double* LU, co
                 const bool first = idx < r0 + kSvThreads / kWave;
                 const int4 ri = first ? cur.ri : p.rows[idx];
                 double acc = y[ri.x];
-                for (int c = 0; c < ri.z; c += kWave) {
-                    double v;
-                    int sx;
-                    if (first && c == 0) {
-                        v = cur.v[0];
-                        sx = cur.s[0];
+                for (int c0 = 0; c0 < ri.z; c0 += 8 * kWave) {
+                    // up to 8 chunks of operands in flight at once
+                    double v[8];
+                    int sx[8];
+                    if (first && c0 == 0) {
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) { v[j] = cur.v[j]; sx[j] = cur.s[j]; }
                     } else {
-                        const int q = ri.y + min(c + lane, ri.z - 1);
-                        v = val(q);
-                        sx = p.src[q];
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            const int q = ri.y + min(c0 + j * kWave + lane, ri.z - 1);
+                            v[j] = val(q);
+                            sx[j] = p.src[q];
+                        }
                     }
-                    const double ys = y[sx];
-                    // a skipped term subtracts +0.0, which leaves acc unchanged
-                    const double prod = skip ? (ys != 0.0 ? ys * v : 0.0) : v * ys;
-                    const int m = min(kWave, ri.z - c);
-                    for (int j = 0; j < m; ++j) acc -= readlane_f64(prod, j);
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const int c = c0 + j * kWave;
+                        if (c < ri.z) {
+                            const double ys = y[sx[j]];
+                            // a skipped term subtracts +0.0, which leaves acc unchanged
+                            const double prod = skip ? (ys != 0.0 ? ys * v[j] : 0.0) : v[j] * ys;
+                            const int m = min(kWave, ri.z - c);
+                            for (int k = 0; k < m; ++k) acc -= readlane_f64(prod, k);
+                        }
+                    }
                 }
                 acc = finish(acc, ri.w);
                 if (lane == 0) y[ri.x] = acc;
@@ -507,6 +667,7 @@ __device__ void run_program(const DevProgram& p, double* y, const double* LU, co
         }
         __syncthreads();
         cur = nxt;
+        nxt.ri = rec2;
     }
 }
 
@@ -519,10 +680,19 @@ __global__ __launch_bounds__(kSvThreads) void k_solve(SolveArgs a) {
     const int ginst = a.imap ? a.imap[inst] : inst;
     const double* LU = a.LU + (size_t)inst * a.nlu;
     const double* SV = a.SV + (size_t)inst * a.nsv;
+    // level tables of both programs after the vector
+    int* lv1 = reinterpret_cast<int*>(smem + (size_t)a.n * 8);
+    int* lm1 = lv1 + a.p1.nlevels + 1;
+    int* lv2 = lm1 + a.p1.nlevels;
+    int* lm2 = lv2 + a.p2.nlevels + 1;
     for (int k = threadIdx.x; k < a.n; k += kSvThreads) y[k] = a.b[(size_t)a.in_perm[k] * a.Bg + ginst];
+    for (int l = threadIdx.x; l <= a.p1.nlevels; l += kSvThreads) lv1[l] = a.p1.level_ptr[l];
+    for (int l = threadIdx.x; l < a.p1.nlevels; l += kSvThreads) lm1[l] = a.p1.level_mode[l];
+    for (int l = threadIdx.x; l <= a.p2.nlevels; l += kSvThreads) lv2[l] = a.p2.level_ptr[l];
+    for (int l = threadIdx.x; l < a.p2.nlevels; l += kSvThreads) lm2[l] = a.p2.level_mode[l];
     __syncthreads();
-    run_program<kSV>(a.p1, y, LU, SV);
-    run_program<kSV>(a.p2, y, LU, SV);
+    run_program<kSV>(a.p1, lv1, lm1, y, LU, SV);
+    run_program<kSV>(a.p2, lv2, lm2, y, LU, SV);
     for (int k = threadIdx.x; k < a.n; k += kSvThreads) a.b[(size_t)a.out_perm[k] * a.Bg + ginst] = y[k];
 }
 
@@ -570,7 +740,6 @@ GpuLu::GpuLu(const LuStructure& s, int batch, hipStream_t stream,
     if ((size_t)n * 8 > lds_max)
         throw Error(NSA_ERR_UNSUPPORTED, "GpuLu: solve vector exceeds LDS (n too large)");
     int tmax = 128;
-    while (tmax > 1 && (size_t)tmax * tmax * 8 + (size_t)tmax * 8 > lds_max) --tmax;
     // The dense phase applies updates from trailing pivots in ascending
     // pivot order; the trailing block is kept to sizes where every trailing
     // column lists its trailing U rows in that order (always for Sparse 1.3).
@@ -618,7 +787,6 @@ GpuLu::GpuLu(const LuStructure& s, int batch, hipStream_t stream,
         for (int k = 0; k < T0_; ++k) lev_cols[fill[lev[k]]++] = k;
     }
     d_lev_ptr_ = upload(allocs_, lev_ptr);
-    d_lev_cols_ = upload(allocs_, lev_cols);
     // Step descriptors and relative slots.  In a trailing column, a pivot r
     // before the block that the U list puts after a trailing pivot s is
     // "deferred": the sparse pass applies it to the rows before the block
@@ -644,12 +812,12 @@ GpuLu::GpuLu(const LuStructure& s, int batch, hipStream_t stream,
             const bool deferred = k >= T0_ && cur >= 0;
             if (rel.size() + (size_t)nl[r] >= (size_t)INT_MAX)
                 throw Error(NSA_ERR_UNSUPPORTED, "GpuLu: plan too large");
-            // L(:, r) in chunks of 128 (two entries per lane); a continuation
+            // L(:, r) in chunks of 64 (one entry per lane); a continuation
             // chunk recomputes the same U value (x_r is not among its targets)
-            for (int c0 = 0; c0 < std::max(nl[r], 1); c0 += 2 * kWave) {
-                const int cl = std::min(2 * kWave, nl[r] - c0);
-                desc.push_back(make_int4((p - cb) | (std::max(cl, 0) << 16), dpos[r] + 1 + c0, dpos[r],
-                                         (int)rel.size()));
+            for (int c0 = 0; c0 < std::max(nl[r], 1); c0 += kWave) {
+                const int cl = std::min(kWave, nl[r] - c0);
+                desc.push_back(make_int4((p - cb) | (std::max(cl, 0) << 16) | (c0 > 0 ? kContinued : 0),
+                                         dpos[r] + 1 + c0, r, (int)rel.size()));
                 for (int e = dpos[r] + 1 + c0; e < dpos[r] + 1 + c0 + cl; ++e) {
                     const int i = s_.rows[e];
                     if (slot[i] < 0) throw Error(NSA_ERR_LOGIC, "GpuLu: fill pattern not closed");
@@ -674,20 +842,35 @@ GpuLu::GpuLu(const LuStructure& s, int batch, hipStream_t stream,
         for (int p = cb; p < s_.cptr[k + 1]; ++p) slot[s_.rows[p]] = -1;
         if (k >= T0_) std::fill(task_of.begin(), task_of.end(), -1);
     }
-    std::vector<int32_t> dstep_ptr(t + 1, 0), dtask_widx, dtask_ptr(1, 0), dterm;
+    // Deferred tasks by trailing step, regrouped by the thread owning the
+    // target entry: tail row i, column j -> thread (i % 32) + 32 (j % 32),
+    // register (i / 32) * 4 + j / 32.
+    std::vector<int32_t> dstep_ptr(std::max(t, 1), -1), dtask_ptr, dterm;
+    std::vector<int4> dtask;
     for (int s0 = 0; s0 < t; ++s0) {
+        if (step_tasks[s0].empty()) continue;
+        std::vector<std::vector<int4>> by_tid(kRfThreads);
         for (auto& tk : step_tasks[s0]) {
-            dtask_widx.push_back(tk.first);
+            const int j = tk.first / t, i = tk.first % t;
+            const int e0 = (int)(dterm.size() / 2);
             dterm.insert(dterm.end(), tk.second.begin(), tk.second.end());
-            dtask_ptr.push_back((int32_t)(dterm.size() / 2));
+            const int e1 = (int)(dterm.size() / 2);
+            by_tid[(i & 31) + 32 * (j & 31)].push_back(make_int4((i >> 5) * 4 + (j >> 5), e0, e1, 0));
         }
-        dstep_ptr[s0 + 1] = (int32_t)dtask_widx.size();
+        dstep_ptr[s0] = (int32_t)(dtask_ptr.size() / (kRfThreads + 1));
+        for (int th = 0; th < kRfThreads; ++th) {
+            dtask_ptr.push_back((int32_t)dtask.size());
+            dtask.insert(dtask.end(), by_tid[th].begin(), by_tid[th].end());
+        }
+        dtask_ptr.push_back((int32_t)dtask.size());
     }
     n_deferred_terms_ = (int64_t)dterm.size() / 2;
     d_dstep_ptr_ = upload(allocs_, dstep_ptr);
-    d_dtask_widx_ = upload(allocs_, dtask_widx);
     d_dtask_ptr_ = upload(allocs_, dtask_ptr);
+    d_dtask_ = upload(allocs_, dtask);
     d_dterm_ = upload(allocs_, dterm);
+    NSA_HIP(hipMalloc(&d_dprod_, std::max<size_t>(1, dterm.size() / 2 * (size_t)B_) * sizeof(double)));
+    allocs_.push_back(d_dprod_);
     d_cptr_ = upload(allocs_, s_.cptr);
     d_nU_ = upload(allocs_, s_.nU);
     d_rows_ = upload(allocs_, s_.rows);
@@ -696,14 +879,26 @@ GpuLu::GpuLu(const LuStructure& s, int batch, hipStream_t stream,
     d_desc_ptr_ = upload(allocs_, desc_ptr);
     int maxnd = 1;
     for (int k = 0; k < n; ++k) maxnd = std::max(maxnd, desc_ptr[k + 1] - desc_ptr[k]);
+    // column positions: the sparse columns in level order, then the trailing ones
+    std::vector<int4> cmeta(2 * (size_t)std::max(n, 1));
+    for (int c = 0; c < n; ++c) {
+        const int k = c < T0_ ? lev_cols[c] : c;
+        cmeta[2 * c] = make_int4(k, s_.cptr[k], s_.cptr[k + 1] - s_.cptr[k], s_.nU[k]);
+        cmeta[2 * c + 1] = make_int4(desc_ptr[k], desc_ptr[k + 1] - desc_ptr[k], 0, 0);
+    }
+    d_cmeta_ = upload(allocs_, cmeta);
     // LDS: per-wave regions (column workspace + staged descriptors),
     // overlaid by the trailing block, then the trailing columns' |U| maxima.
     wregion_ = (int)align16((size_t)wsz_ * 8 + (size_t)maxnd * 16);
-    const size_t cmx_bytes = (size_t)std::max(t, 1) * 8;
-    nws_ = (int)std::min<size_t>(kRfWaves, (lds_max - cmx_bytes) / (size_t)wregion_);
+    // after the union: |U| maxima of the trailing columns (t doubles), then
+    // the trailing structure masks (12 words per step)
+    const size_t cmx_bytes = (size_t)std::max(t, 1) * 8 + (size_t)std::max(t, 1) * 48;
+    const size_t flag_bytes = (size_t)(nlev_ + 1) * 4;   // the level table
+    if (cmx_bytes + flag_bytes >= lds_max) throw Error(NSA_ERR_UNSUPPORTED, "GpuLu: level table exceeds LDS");
+    nws_ = (int)std::min<size_t>(kRfWaves, (lds_max - cmx_bytes - flag_bytes) / (size_t)wregion_);
     if (nws_ < 1) throw Error(NSA_ERR_UNSUPPORTED, "GpuLu: column workspace exceeds LDS");
-    const size_t union_bytes = align16(std::max((size_t)nws_ * wregion_, (size_t)t * t * 8));
-    rf_lds_ = union_bytes + cmx_bytes;
+    union_bytes_ = align16(std::max((size_t)nws_ * wregion_, (size_t)(516 + 128) * 8));
+    rf_lds_ = union_bytes_ + cmx_bytes + flag_bytes;
     if (rf_lds_ > lds_max) throw Error(NSA_ERR_UNSUPPORTED, "GpuLu: refactor LDS budget");
     desc.push_back(make_int4(0, 0, 0, 0));
     d_desc_ = upload(allocs_, desc);
@@ -717,11 +912,13 @@ GpuLu::GpuLu(const LuStructure& s, int batch, hipStream_t stream,
     d_diag_cand_ = upload(allocs_, dc);
     d_row_perm_ = upload(allocs_, s_.row_perm);
     d_col_perm_ = upload(allocs_, s_.col_perm);
-    // ---- dense trailing block lists (tail-local indices)
-    std::vector<int32_t> tpos_ptr(t + 1, 0), tpos, twidx;
-    std::vector<int32_t> lrow_ptr(t + 1, 0), urow_ptr(t + 1, 0), ucol_ptr(t + 1, 0);
-    std::vector<int16_t> lrow, urow;
-    std::vector<std::vector<int16_t>> ucol_lists(t);
+    // ---- dense trailing block: per-thread register positions and structure masks
+    if (t > 128) throw Error(NSA_ERR_LOGIC, "GpuLu: trailing block larger than 128");
+    std::vector<int32_t> tposmap((size_t)kRfThreads * 16, -1);
+    std::vector<uint32_t> tmask((size_t)std::max(t, 1) * 12, 0u);
+    auto setbit = [&](int step, int word0, int i) {
+        tmask[(size_t)step * 12 + word0 + (i >> 5)] |= 1u << (i & 31);
+    };
     std::vector<int32_t> tcand_ptr(t + 1, 0), tdiag(std::max(t, 1), -1);
     std::vector<int16_t> tcand;
     for (int j = 0; j < t; ++j) {
@@ -730,35 +927,21 @@ GpuLu::GpuLu(const LuStructure& s, int batch, hipStream_t stream,
             const int r = s_.rows[p];
             if (r < T0_) continue;
             const int i = r - T0_;
-            tpos.push_back(p);
-            twidx.push_back(j * t + i);
-            if (i > j) lrow.push_back((int16_t)i);
-            if (i < j) { urow.push_back((int16_t)i); ucol_lists[i].push_back((int16_t)j); }
+            tposmap[(size_t)((i & 31) + 32 * (j & 31)) * 16 + (i >> 5) * 4 + (j >> 5)] = p;
+            if (i > j) setbit(j, 0, i);                       // L(i, j): an L row of step j
+            if (i < j) { setbit(i, 4, j); setbit(j, 8, i); }  // U(i, j): a U column of step i, a U row of column j
         }
-        tpos_ptr[j + 1] = (int32_t)tpos.size();
-        lrow_ptr[j + 1] = (int32_t)lrow.size();
-        urow_ptr[j + 1] = (int32_t)urow.size();
         if (s_.form == FORM_GP && !s_.cand_ptr.empty()) {
             for (int c = s_.cand_ptr[k]; c < s_.cand_ptr[k + 1]; ++c)
                 tcand.push_back((int16_t)(s_.rows[s_.cand_pos[c]] - T0_));
             tdiag[j] = s_.diag_cand[k];
         }
         tcand_ptr[j + 1] = (int32_t)tcand.size();
+        if (tcand_ptr[j + 1] - tcand_ptr[j] > 2 * kWave)
+            throw Error(NSA_ERR_UNSUPPORTED, "GpuLu: trailing step with more than 128 candidates");
     }
-    std::vector<int16_t> ucol;
-    for (int i = 0; i < t; ++i) {
-        ucol.insert(ucol.end(), ucol_lists[i].begin(), ucol_lists[i].end());
-        ucol_ptr[i + 1] = (int32_t)ucol.size();
-    }
-    d_tpos_ptr_ = upload(allocs_, tpos_ptr);
-    d_tpos_ = upload(allocs_, tpos);
-    d_twidx_ = upload(allocs_, twidx);
-    d_lrow_ptr_ = upload(allocs_, lrow_ptr);
-    d_lrow_ = upload(allocs_, lrow);
-    d_urow_ptr_ = upload(allocs_, urow_ptr);
-    d_urow_ = upload(allocs_, urow);
-    d_ucol_ptr_ = upload(allocs_, ucol_ptr);
-    d_ucol_ = upload(allocs_, ucol);
+    d_tposmap_ = upload(allocs_, tposmap);
+    d_tmask_ = upload(allocs_, tmask);
     d_tcand_ptr_ = upload(allocs_, tcand_ptr);
     d_tcand_ = upload(allocs_, tcand);
     d_tdiag_cand_ = upload(allocs_, tdiag);
@@ -772,6 +955,11 @@ GpuLu::GpuLu(const LuStructure& s, int batch, hipStream_t stream,
     upload_program(tb, tbwd_, nullptr);
     nsv_ = (int)sv_pos.size();
     d_sv_pos_ = upload(allocs_, sv_pos);
+    // solve LDS: the vector, then both programs' level tables
+    sv_lds_ = (size_t)n * 8 + (size_t)(2 * fwd_.nlevels + 1 + 2 * bwd_.nlevels + 1) * 4;
+    sv_lds_t_ = (size_t)n * 8 + (size_t)(2 * tfwd_.nlevels + 1 + 2 * tbwd_.nlevels + 1) * 4;
+    if (std::max(sv_lds_, sv_lds_t_) > lds_max)
+        throw Error(NSA_ERR_UNSUPPORTED, "GpuLu: solve vector and level tables exceed LDS");
     NSA_HIP(hipMalloc(&dSV_, std::max<size_t>(1, (size_t)nsv_ * B_) * sizeof(double)));
     allocs_.push_back(dSV_);
     // 256 doubles of padding past the last instance for clamped reads
@@ -782,12 +970,12 @@ GpuLu::GpuLu(const LuStructure& s, int batch, hipStream_t stream,
     if (rf_lds_ > 65536)
         NSA_HIP(hipFuncSetAttribute((const void*)k_refactor, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)rf_lds_));
-    if ((size_t)n * 8 > 65536) {
+    if (sv_lds_ > 65536)
         NSA_HIP(hipFuncSetAttribute((const void*)k_solve<true>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, n * 8));
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)sv_lds_));
+    if (sv_lds_t_ > 65536)
         NSA_HIP(hipFuncSetAttribute((const void*)k_solve<false>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, n * 8));
-    }
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)sv_lds_t_));
     NSA_HIP(hipStreamSynchronize(stream_));
 }
 
@@ -818,11 +1006,12 @@ void GpuLu::upload_program(const TriProgram& t, DevProgram& d, std::vector<int32
             }
             rows[idx] = make_int4(tg, first, cnt, has_diag ? t.dpos[tg] : -1);
         }
-        // latency model (cycles): a round of dependent loads ~2000, a chained
-        // subtraction ~10 (thread) or ~12 (wave, with the lane read)
+        // latency model (cycles): a round of dependent loads ~2000 (8 terms per
+        // thread, 8 x 64 per wave), a chained subtraction ~10 (thread) or ~12
+        // (wave, with the lane read)
         const double R = r1 - r0;
         const double thread_cost = std::ceil(R / kSvThreads) * (std::ceil(maxc / 8.0) * 2000 + 10.0 * maxc);
-        const double wave_cost = std::ceil(R / (kSvThreads / kWave)) * (std::ceil(maxc / 64.0) * 2000 + 12.0 * maxc);
+        const double wave_cost = std::ceil(R / (kSvThreads / kWave)) * (std::ceil(maxc / 512.0) * 2000 + 12.0 * maxc);
         mode[l] = wave_cost < thread_cost ? 1 : 0;
     }
     if (sv_pos) {
@@ -852,17 +1041,17 @@ void GpuLu::refactor(const double* dA, double gmin, const int32_t* dkind, int32_
                      int32_t* dfail_col, hipStream_t on) {
     RfArgs a{};
     a.n = s_.n; a.T0 = T0_; a.t = tail_; a.nlev = nlev_; a.wsz = wsz_; a.wregion = wregion_; a.nws = nws_;
-    a.union_bytes = (int)(rf_lds_ - (size_t)std::max(tail_, 1) * 8);
-    a.lev_ptr = d_lev_ptr_; a.lev_cols = d_lev_cols_;
+    a.union_bytes = (int)union_bytes_;
+    a.lev_ptr = d_lev_ptr_; a.cmeta = d_cmeta_;
     a.cptr = d_cptr_; a.nU = d_nU_; a.rows = d_rows_; a.a_src = d_asrc_; a.gmin_pos = d_gpos_;
     a.desc_ptr = d_desc_ptr_; a.desc = d_desc_; a.rel = d_rel_;
     a.cand_ptr = d_cand_ptr_; a.cand_pos = d_cand_pos_; a.diag_cand = d_diag_cand_;
-    a.tpos_ptr = d_tpos_ptr_; a.tpos = d_tpos_; a.twidx = d_twidx_;
-    a.lrow_ptr = d_lrow_ptr_; a.ucol_ptr = d_ucol_ptr_; a.urow_ptr = d_urow_ptr_;
-    a.tcand_ptr = d_tcand_ptr_; a.tdiag_cand = d_tdiag_cand_;
-    a.lrow = d_lrow_; a.ucol = d_ucol_; a.urow = d_urow_; a.tcand = d_tcand_;
-    a.dstep_ptr = d_dstep_ptr_; a.dtask_widx = d_dtask_widx_; a.dtask_ptr = d_dtask_ptr_;
+    a.tposmap = d_tposmap_;
+    a.tcand_ptr = d_tcand_ptr_; a.tdiag_cand = d_tdiag_cand_; a.tcand = d_tcand_; a.tmask = d_tmask_;
+    a.dstep_ptr = d_dstep_ptr_; a.dtask = d_dtask_; a.dtask_ptr = d_dtask_ptr_;
     a.dterm = d_dterm_;
+    a.ndterm = (int)n_deferred_terms_;
+    a.dprod = d_dprod_;
     a.sv_pos = d_sv_pos_; a.nsv = nsv_; a.SV = dSV_;
     a.A = dA; a.LU = dLU_; a.nlu = s_.nlu(); a.B = B_; a.imap = d_imap_; a.Bg = Bglobal_;
     a.gmin = gmin; a.form = (int)s_.form; a.pivot_tol = 1e-3; a.kind = dkind;
@@ -889,10 +1078,11 @@ void GpuLu::solve(double* db, bool transposed, hipStream_t on) {
     a.imap = d_imap_;
     a.B = B_;
     a.Bg = Bglobal_;
+    const size_t lds = transposed ? sv_lds_t_ : sv_lds_;
     if (transposed)
-        hipLaunchKernelGGL(k_solve<false>, dim3(B_), dim3(kSvThreads), (size_t)s_.n * 8, st, a);
+        hipLaunchKernelGGL(k_solve<false>, dim3(B_), dim3(kSvThreads), lds, st, a);
     else
-        hipLaunchKernelGGL(k_solve<true>, dim3(B_), dim3(kSvThreads), (size_t)s_.n * 8, st, a);
+        hipLaunchKernelGGL(k_solve<true>, dim3(B_), dim3(kSvThreads), lds, st, a);
     NSA_HIP(hipGetLastError());
 }
 

@@ -15,7 +15,7 @@
 //   partial pass   trailing-block columns: A load plus the updates from
 //                  pivots before the block;
 //   dense tail     the trailing block (<= 128 columns) factored right-looking
-//                  in LDS by the whole workgroup;
+//                  by the whole workgroup, held in registers;
 //   solve gather   the factor values the plain solve reads, copied into the
 //                  solve's own term order so that it streams them.
 // Every value is bit-identical to the reference: each entry receives its
@@ -94,10 +94,11 @@ private:
     hipStream_t stream_;
     std::vector<void*> allocs_;
     int tail_ = 0, T0_ = 0, nlev_ = 0, wsz_ = 0, wregion_ = 0, nws_ = 0;
-    size_t rf_lds_ = 0;
+    size_t rf_lds_ = 0, union_bytes_ = 0, sv_lds_ = 0, sv_lds_t_ = 0;
     int64_t n_deferred_terms_ = 0;
     // sparse phase
-    int32_t *d_lev_ptr_ = nullptr, *d_lev_cols_ = nullptr;
+    int32_t* d_lev_ptr_ = nullptr;
+    int4* d_cmeta_ = nullptr;
     int32_t *d_cptr_ = nullptr, *d_nU_ = nullptr, *d_rows_ = nullptr, *d_asrc_ = nullptr,
             *d_gpos_ = nullptr, *d_desc_ptr_ = nullptr;
     int4* d_desc_ = nullptr;
@@ -105,14 +106,14 @@ private:
     int32_t *d_cand_ptr_ = nullptr, *d_cand_pos_ = nullptr, *d_diag_cand_ = nullptr,
             *d_row_perm_ = nullptr, *d_col_perm_ = nullptr;
     // dense tail
-    int32_t *d_tpos_ptr_ = nullptr, *d_tpos_ = nullptr, *d_twidx_ = nullptr;
-    int32_t *d_lrow_ptr_ = nullptr, *d_ucol_ptr_ = nullptr, *d_urow_ptr_ = nullptr;
-    int16_t *d_lrow_ = nullptr, *d_ucol_ = nullptr, *d_urow_ = nullptr;
+    int32_t* d_tposmap_ = nullptr;
+    uint32_t* d_tmask_ = nullptr;
     int32_t* d_tcand_ptr_ = nullptr;
     int16_t* d_tcand_ = nullptr;
     int32_t* d_tdiag_cand_ = nullptr;
-    int32_t *d_dstep_ptr_ = nullptr, *d_dtask_widx_ = nullptr, *d_dtask_ptr_ = nullptr,
-            *d_dterm_ = nullptr;
+    int32_t *d_dstep_ptr_ = nullptr, *d_dtask_ptr_ = nullptr, *d_dterm_ = nullptr;
+    int4* d_dtask_ = nullptr;
+    double* d_dprod_ = nullptr;   // deferred products, ndterm per instance
     // solve-ordered copy of the values the plain solve reads
     int32_t* d_sv_pos_ = nullptr;
     int nsv_ = 0;
