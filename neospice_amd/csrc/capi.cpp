@@ -72,6 +72,25 @@ int set_dev(int32_t* dev, int32_t v) {
 }
 }  // namespace
 
+
+// A CSC pattern as the reference's SparsityPattern::to_csc produces it:
+// n >= 1, col_ptr[0] = 0, non-decreasing, rows in [0, n) strictly
+// increasing within a column.
+static void check_csc(int32_t n, const int32_t* col_ptr, const int32_t* row_idx) {
+    if (n <= 0) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "pattern: n must be positive");
+    if (!col_ptr || !row_idx) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "pattern: null pointer");
+    if (col_ptr[0] != 0) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "pattern: col_ptr[0] != 0");
+    for (int32_t c = 0; c < n; ++c) {
+        if (col_ptr[c + 1] < col_ptr[c]) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "pattern: col_ptr decreasing");
+        for (int32_t p = col_ptr[c]; p < col_ptr[c + 1]; ++p) {
+            if (row_idx[p] < 0 || row_idx[p] >= n)
+                throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "pattern: row index out of range");
+            if (p > col_ptr[c] && row_idx[p] <= row_idx[p - 1])
+                throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "pattern: rows not strictly increasing in a column");
+        }
+    }
+}
+
 extern "C" {
 
 const char* nsa_last_error(void) { return g_err.c_str(); }
@@ -275,6 +294,8 @@ int nsa_engine_timing(nsa_engine* e, double out[3]) {
 int nsa_lu_create(int32_t batch, int32_t n, const int32_t* col_ptr, const int32_t* row_idx,
                   int32_t is_linear, int32_t engine, nsa_lu** out) {
     return guard([&] {
+        check_csc(n, col_ptr, row_idx);
+        if (batch <= 0 || !out) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "nsa_lu_create: batch/out");
         auto s = std::make_unique<nsa_lu>();
         NSA_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
         s->n = n;
@@ -348,6 +369,8 @@ int nsa_pivot_structure(int32_t engine, int32_t n, const int32_t* col_ptr, const
                         int32_t* row_perm, int32_t* col_perm, int32_t* seg_ptr, int32_t* nU,
                         int32_t* rows) {
     return guard([&] {
+        check_csc(n, col_ptr, row_idx);
+        if (!values || !nnz_lu) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "nsa_pivot_structure: null pointer");
         nsa::LuStructure st;
         if (engine == nsa::ENGINE_AMDLU) {
             auto q = nsa::min_degree_order(n, col_ptr, row_idx);
@@ -373,6 +396,8 @@ int nsa_pivot_structure(int32_t engine, int32_t n, const int32_t* col_ptr, const
 
 int nsa_min_degree_order(int32_t n, const int32_t* col_ptr, const int32_t* row_idx, int32_t* perm) {
     return guard([&] {
+        check_csc(n, col_ptr, row_idx);
+        if (!perm) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "nsa_min_degree_order: null perm");
         auto q = nsa::min_degree_order(n, col_ptr, row_idx);
         std::memcpy(perm, q.data(), sizeof(int32_t) * q.size());
         return 0;

@@ -104,8 +104,109 @@ def orc():
         L.orc_gp_solve.argtypes = [C.c_int, P_I32, P_I32, P_I32, P_I32, P_F64, P_I32, P_I32,
                                    P_F64, P_F64]
         L.orc_gp_solve_t.argtypes = L.orc_gp_solve.argtypes
+        L.orc_s13_refactor.argtypes = [C.POINTER(OrcS13), P_I32, P_I32, P_F64, C.c_double]
+        L.orc_s13_solve.argtypes = [C.POINTER(OrcS13), P_F64, P_F64]
+        L.orc_s13_solve_t.argtypes = [C.POINTER(OrcS13), P_F64, P_F64]
         _orc = L
     return _orc
+
+
+class OrcS13(C.Structure):
+    """orc_s13 (oracle.h): a Sparse-1.3 factored structure."""
+    _fields_ = [("n", C.c_int), ("i2e_row", P_I32), ("i2e_col", P_I32), ("cptr", P_I32),
+                ("rows", P_I32), ("vals", P_F64)]
+
+
+class OracleGP:
+    """The oracle's AMD-LU pipeline on one matrix: minimum-degree order,
+    Gilbert-Peierls factor, replay, solves."""
+
+    def __init__(self, n, col_ptr, row_idx, values, gmin=0.0, tol=1e-3):
+        self.n = n
+        self.cp, _ = i32(col_ptr)
+        self.ri, _ = i32(row_idx)
+        O = orc()
+        self.q = np.zeros(n, np.int32)
+        O.orc_amd_order(n, self.cp.ctypes.data_as(P_I32), self.ri.ctypes.data_as(P_I32),
+                        self.q.ctypes.data_as(P_I32))
+        cap = n * n + 1
+        self.pinv = np.zeros(n, np.int32)
+        self.Lp = np.zeros(n + 1, np.int32)
+        self.Up = np.zeros(n + 1, np.int32)
+        self.Li = np.zeros(cap, np.int32)
+        self.Ui = np.zeros(cap, np.int32)
+        self.Lx = np.zeros(cap)
+        self.Ux = np.zeros(cap)
+        self.tol = tol
+        v, pv = f64(values)
+        self.singular = O.orc_gp_factor(
+            n, self.cp.ctypes.data_as(P_I32), self.ri.ctypes.data_as(P_I32), pv, gmin, tol,
+            self.q.ctypes.data_as(P_I32), self.pinv.ctypes.data_as(P_I32),
+            self.Lp.ctypes.data_as(P_I32), self.Li.ctypes.data_as(P_I32),
+            self.Lx.ctypes.data_as(P_F64), cap, self.Up.ctypes.data_as(P_I32),
+            self.Ui.ctypes.data_as(P_I32), self.Ux.ctypes.data_as(P_F64), cap) != 0
+
+    def _args(self):
+        return [a.ctypes.data_as(P_I32 if a.dtype == np.int32 else P_F64)
+                for a in (self.q, self.pinv, self.Lp, self.Li, self.Lx, self.Up, self.Ui, self.Ux)]
+
+    def refactor(self, values, gmin=0.0):
+        """True when the replay is accepted by the growth check."""
+        v, pv = f64(values)
+        a = self._args()
+        return orc().orc_gp_refactor(self.n, self.cp.ctypes.data_as(P_I32),
+                                     self.ri.ctypes.data_as(P_I32), pv, gmin, self.tol,
+                                     *a) == 1
+
+    def solve(self, b):
+        x, px = f64(np.array(b, dtype=np.float64))
+        orc().orc_gp_solve(self.n, *self._args(), px)
+        return x
+
+    def solve_t(self, b):
+        x, px = f64(np.array(b, dtype=np.float64))
+        orc().orc_gp_solve_t(self.n, *self._args(), px)
+        return x
+
+
+class OracleS13:
+    """The oracle's Sparse-1.3 refactor and solves on a fixed pivot order
+    and fill given in the internal numbering: i2e_row/i2e_col (1-based
+    external index of internal row/column i+1), the column lists cptr/rows
+    (1-based internal rows, ascending)."""
+
+    def __init__(self, n, i2e_row, i2e_col, cptr, rows):
+        self.n = n
+        self.keep = [i32(i2e_row)[0], i32(i2e_col)[0], i32(cptr)[0], i32(rows)[0]]
+        self.vals = np.zeros(len(self.keep[3]))
+        self.s = OrcS13(n, *[k.ctypes.data_as(P_I32) for k in self.keep],
+                        self.vals.ctypes.data_as(P_F64))
+
+    @classmethod
+    def from_structure(cls, st):
+        """From a pivot-space structure (row_perm, col_perm, seg_ptr, rows;
+        0-based, column segments in ascending row order)."""
+        return cls(len(st["row_perm"]), np.asarray(st["row_perm"]) + 1, np.asarray(st["col_perm"]) + 1,
+                   st["seg_ptr"], np.asarray(st["rows"]) + 1)
+
+    def refactor(self, col_ptr, row_idx, values, gmin=0.0):
+        """0 ok, else the 1-based internal step of an exact zero pivot."""
+        cp, pcp = i32(col_ptr)
+        ri, pri = i32(row_idx)
+        v, pv = f64(values)
+        return orc().orc_s13_refactor(C.byref(self.s), pcp, pri, pv, gmin)
+
+    def solve(self, b):
+        b, pb = f64(np.array(b, dtype=np.float64))
+        x = np.zeros(self.n)
+        orc().orc_s13_solve(C.byref(self.s), pb, x.ctypes.data_as(P_F64))
+        return x
+
+    def solve_t(self, b):
+        b, pb = f64(np.array(b, dtype=np.float64))
+        x = np.zeros(self.n)
+        orc().orc_s13_solve_t(C.byref(self.s), pb, x.ctypes.data_as(P_F64))
+        return x
 
 
 def i32(a):
