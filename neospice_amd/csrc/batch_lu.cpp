@@ -178,7 +178,7 @@ int BatchLu::factor(const double* dA, double gmin, const int32_t* numeric_mask, 
     if (groups_.empty()) {
         // First factorization.  A refactorize() on a fresh solver is a
         // numeric(): NeoSolver throws and newton_solve retries with numeric()
-        // (newton.cpp:235-242); AmdLuSolver runs factor_ (amd_lu_solver.cpp:390-404).
+        // (newton.cpp:235-242); AmdLuSolver runs factor_ (amd_lu_solver.cpp:280-301).
         std::fill(kind.begin(), kind.end(), KIND_VERIFY);
         auto vals = instance_values(dA, 0);
         LuStructure s;
@@ -227,7 +227,7 @@ int BatchLu::factor(const double* dA, double gmin, const int32_t* numeric_mask, 
             continue;
         }
         // numeric() whose order no longer holds, or an AMD-LU replay rejected
-        // by its growth check (answered with a full factor_, amd_lu_solver.cpp:395-404):
+        // by its growth check (answered with a full factor_, amd_lu_solver.cpp:291-300):
         // re-pivot this instance on the CPU from its own values.
         const int g = group_of_[i];
         auto vals = instance_values(dA, i);

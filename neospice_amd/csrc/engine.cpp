@@ -40,9 +40,9 @@ BatchEngine::BatchEngine(const Circuit& ckt, int batch, const Options& opt)
     : ckt_(ckt), B_(batch), opt_(opt) {
     if (!ckt.finalized()) throw Error(NSA_ERR_LOGIC, "BatchEngine: circuit not finalized");
     if (batch <= 0) throw Error(NSA_ERR_ARGUMENT, "BatchEngine: batch must be positive");
-    if (!ckt.family(DK_D).empty() || !ckt.family(DK_M1).empty()) {
-        // Nonlinear families are evaluated by their own kernels (added per family).
-    }
+    if (!ckt.family(DK_D).empty() || !ckt.family(DK_M1).empty())
+        throw Error(NSA_ERR_UNSUPPORTED,
+                    "BatchEngine: diode and MOS1 loads are not on the GPU path in this build");
     n_ = ckt.num_vars();
     nnz_ = ckt.nnz();
     ns_ = ckt.num_states();

@@ -5,7 +5,7 @@
 //   Sparse 1.3 (SparseMatrix::factor, factor.cpp:1000-1047): for each U entry
 //     r of column j in ascending row order, U(r,j) = x_r * (1/pivot_r) and
 //     x_t -= U(r,j) * L(t,r); L kept unnormalised; diagonal = 1/x_j.
-//   AMD-LU replay (refactor_replay_, amd_lu_solver.cpp:325-373): for each U
+//   AMD-LU replay (refactor_replay_, amd_lu_solver.cpp:214-271): for each U
 //     entry in the recorded reach order, U(r,j) = x_r and x_t -= L(t,r) * U(r,j);
 //     pivot U(j,j) = x_j with the growth check; L(t,j) = x_t / U(j,j).
 // Everything is compiled without FMA contraction.

@@ -37,7 +37,7 @@ LuStructure structure_from_gp(int n, const int32_t* acp, const int32_t* ari,
     s.cptr.assign(n + 1, 0);
     s.nU.assign(n, 0);
     for (int k = 0; k < n; ++k) {
-        const int nu = Up[k + 1] - Up[k] - 1;          // U diag is last (amd_lu_solver.cpp:160)
+        const int nu = Up[k + 1] - Up[k] - 1;          // U diag is last (amd_lu_solver.cpp:158-160)
         const int nl = Lp[k + 1] - Lp[k] - 1;          // unit L diag is first (:163)
         s.nU[k] = nu;
         s.cptr[k + 1] = s.cptr[k] + nu + 1 + nl;
@@ -62,7 +62,7 @@ LuStructure structure_from_gp(int n, const int32_t* acp, const int32_t* ari,
             s.a_pos[p] = pos;
         }
     }
-    // diag_gmin lands on x[col] = pivot row pinv[col] (amd_lu_solver.cpp:225, 333).
+    // diag_gmin lands on x[col] = pivot row pinv[col] (amd_lu_solver.cpp:121, 227-229).
     s.gmin_pos.assign(n, -1);
     for (int k = 0; k < n; ++k) s.gmin_pos[k] = find_in_col(s, k, pinv[q[k]]);
     s.row_perm.assign(n, 0);
@@ -260,10 +260,10 @@ void build_solve_programs(const LuStructure& s, TriProgram& fwd, TriProgram& bwd
         for (int t = 0; t < n; ++t) { ttb[t] = lcol[t]; sort_key(ttb[t], true); }
         tbwd.op = OP_MUL_DIAG;
     } else {
-        // amd_lu_solver.cpp:427-433: unit-L forward scatter, j ascending.
+        // amd_lu_solver.cpp:320-327: unit-L forward scatter, j ascending.
         for (int t = 0; t < n; ++t) { tf[t] = lrow[t]; sort_key(tf[t], true); }
         fwd.op = OP_PLAIN;
-        // amd_lu_solver.cpp:436-442: backward, j descending: x[j] /= U(j,j),
+        // amd_lu_solver.cpp:329-336: backward, j descending: x[j] /= U(j,j),
         // then scatter U(:, j).  Target t gathers U(t, c), c descending, and
         // divides last.
         for (int t = 0; t < n; ++t) { tb[t] = urow[t]; sort_key(tb[t], false); }
