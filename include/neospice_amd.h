@@ -52,6 +52,7 @@ typedef struct {
 typedef struct {
     int32_t mode, order, method, pad;
     double ag0, ag1, ag2, xmu_ratio, time, src_fact;
+    double delta, delta_old1; /* CKTdelta and CKTdeltaOld[1] (predictor ratio) */
 } nsa_ctx;
 
 const char* nsa_last_error(void);

@@ -163,18 +163,21 @@ class Circuit:
                 pass
 
 
-def make_ctx(mode, order=1, method=0, ag=(0.0, 0.0, 0.0), xmu_ratio=1.0, time=0.0, src_fact=1.0):
+def make_ctx(mode, order=1, method=0, ag=(0.0, 0.0, 0.0), xmu_ratio=1.0, time=0.0, src_fact=1.0,
+             delta=0.0, delta_old1=0.0):
     c = Ctx()
     c.mode, c.order, c.method = int(mode), int(order), int(method)
     c.ag0, c.ag1, c.ag2 = map(float, ag)
     c.xmu_ratio, c.time, c.src_fact = float(xmu_ratio), float(time), float(src_fact)
+    c.delta, c.delta_old1 = float(delta), float(delta_old1)
     return c
 
 
 def be_ctx(dt, time, first):
-    """Context of a backward-Euler step (fill_integrator_context, transient.cpp:398-447)."""
+    """Context of a fixed backward-Euler step (fill_integrator_context,
+    transient.cpp:398-447): delta = deltaOld[1] = dt."""
     return make_ctx(MODETRAN | (MODEINITTRAN if first else MODEINITPRED), 1, 0,
-                    (1.0 / dt, -1.0 / dt, 0.0), 0.0, time)
+                    (1.0 / dt, -1.0 / dt, 0.0), 0.0, time, 1.0, dt, dt)
 
 
 class BatchEngine:

@@ -27,7 +27,8 @@ class Options(C.Structure):
 class Ctx(C.Structure):
     _fields_ = [("mode", C.c_int32), ("order", C.c_int32), ("method", C.c_int32), ("pad", C.c_int32),
                 ("ag0", C.c_double), ("ag1", C.c_double), ("ag2", C.c_double),
-                ("xmu_ratio", C.c_double), ("time", C.c_double), ("src_fact", C.c_double)]
+                ("xmu_ratio", C.c_double), ("time", C.c_double), ("src_fact", C.c_double),
+                ("delta", C.c_double), ("delta_old1", C.c_double)]
 
 
 _SIGS = {

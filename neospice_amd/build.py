@@ -17,6 +17,7 @@ OBJ = os.path.join(HERE, "_build")
 
 SOURCES = [
     "circuit.cpp",
+    "devices.cpp",
     "lu_structure.cpp",
     "lu_cpu.cpp",
     "batch_lu.cpp",
@@ -24,6 +25,7 @@ SOURCES = [
     "capi.cpp",
     "gpu_lu.hip",
     "gpu_dev.hip",
+    "gpu_nonlinear.hip",
 ]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]

@@ -6,6 +6,7 @@
 #include <sstream>
 
 #include "common.hpp"
+#include "devices.hpp"
 
 namespace nsa {
 
@@ -202,6 +203,8 @@ int32_t Circuit::D(const std::string& name, int32_t a, int32_t k, const std::str
     const int32_t mi = find_model(model);
     if (mi < 0 || models_[mi].type != "d")
         throw Error(NSA_ERR_ARGUMENT, "D device: model not found or wrong type: " + model);
+    if (card_needs_internal_nodes(models_[mi]))
+        throw Error(NSA_ERR_UNSUPPORTED, "D device " + name + ": RS != 0 (internal node) not covered");
     Device d{DK_D, name};
     d.node[0] = a; d.node[1] = k;
     d.model = mi;
@@ -213,6 +216,11 @@ int32_t Circuit::M(const std::string& name, int32_t dn, int32_t g, int32_t s, in
     const int32_t mi = find_model(model);
     if (mi < 0 || (models_[mi].type != "nmos" && models_[mi].type != "pmos"))
         throw Error(NSA_ERR_ARGUMENT, "M device: model not found or wrong type: " + model);
+    if (models_[mi].level != 1)
+        throw Error(NSA_ERR_UNSUPPORTED, "M device " + name + ": only LEVEL=1 (MOS1) is covered");
+    if (card_needs_internal_nodes(models_[mi]))
+        throw Error(NSA_ERR_UNSUPPORTED,
+                    "M device " + name + ": RD/RS/RSH != 0 (internal nodes) not covered");
     Device d{DK_M1, name};
     d.node[0] = dn; d.node[1] = g; d.node[2] = s; d.node[3] = b;
     d.model = mi;

@@ -6,6 +6,7 @@
 #include <cstring>
 
 #include "common.hpp"
+#include "devices.hpp"
 
 namespace nsa {
 
@@ -40,9 +41,6 @@ BatchEngine::BatchEngine(const Circuit& ckt, int batch, const Options& opt)
     : ckt_(ckt), B_(batch), opt_(opt) {
     if (!ckt.finalized()) throw Error(NSA_ERR_LOGIC, "BatchEngine: circuit not finalized");
     if (batch <= 0) throw Error(NSA_ERR_ARGUMENT, "BatchEngine: batch must be positive");
-    if (!ckt.family(DK_D).empty() || !ckt.family(DK_M1).empty())
-        throw Error(NSA_ERR_UNSUPPORTED,
-                    "BatchEngine: diode and MOS1 loads are not on the GPU path in this build");
     n_ = ckt.num_vars();
     nnz_ = ckt.nnz();
     ns_ = ckt.num_states();
@@ -59,6 +57,11 @@ BatchEngine::BatchEngine(const Circuit& ckt, int batch, const Options& opt)
     d_ctx_ = dalloc<DevCtx>(allocs_, B_);
     d_done_ = dalloc<int32_t>(allocs_, B_);
     d_nonconv_ = dalloc<int32_t>(allocs_, B_);
+    d_devnc_ = dalloc<int32_t>(allocs_, B_);
+    NSA_HIP(hipMemsetAsync(d_devnc_, 0, sizeof(int32_t) * B_, stream_));
+    tol_.gmin = opt_.gmin;
+    tol_.reltol = opt_.reltol;
+    tol_.abstol = opt_.abstol;
     NSA_HIP(hipHostMalloc((void**)&h_flags_, sizeof(int32_t) * 4 * B_, hipHostMallocDefault));
     NSA_HIP(hipHostMalloc((void**)&h_ctx_pin_, sizeof(DevCtx) * B_, hipHostMallocDefault));
     DevCtx c{};
@@ -124,6 +127,40 @@ void BatchEngine::build_tables() {
         cap_state.push_back(devs[d].state_base);
         cap_val.push_back(devs[d].value);
     }
+    // diodes: 7 matrix sites, 2 rhs sites (DIOload stamp order)
+    DeviceEnv env;
+    env.gmin = opt_.gmin;
+    env.reltol = opt_.reltol;
+    env.abstol = opt_.abstol;
+    env.vntol = opt_.vntol;
+    const auto& fd = ckt_.family(DK_D);
+    dio_.count = (int)fd.size();
+    dio_.site = site;
+    dio_.rsite = rsite;
+    std::vector<int32_t> dio_nodes, dio_state;
+    std::vector<DioParam> dio_par;
+    for (int d : fd) {
+        dev_site_[d] = site; site += 7;
+        dev_rsite_[d] = rsite; rsite += 2;
+        dio_nodes.push_back(devs[d].node[0]);
+        dio_nodes.push_back(devs[d].node[1]);
+        dio_state.push_back(devs[d].state_base);
+        dio_par.push_back(diode_param(ckt_.models().at(devs[d].model), devs[d], env));
+    }
+    // MOS1: 22 matrix sites, 4 rhs sites (MOS1load stamp order)
+    const auto& fm = ckt_.family(DK_M1);
+    mos_.count = (int)fm.size();
+    mos_.site = site;
+    mos_.rsite = rsite;
+    std::vector<int32_t> mos_nodes, mos_state;
+    std::vector<Mos1Param> mos_par;
+    for (int d : fm) {
+        dev_site_[d] = site; site += 22;
+        dev_rsite_[d] = rsite; rsite += 4;
+        for (int k = 0; k < 4; ++k) mos_nodes.push_back(devs[d].node[k]);
+        mos_state.push_back(devs[d].state_base);
+        mos_par.push_back(mos1_param(ckt_.models().at(devs[d].model), devs[d], env));
+    }
     const_site_ = site;
     std::vector<double> const_vals;
     for (int d : fv) {
@@ -162,12 +199,13 @@ void BatchEngine::build_tables() {
     std::vector<std::vector<int32_t>> slot_sites(nnz_), rows_d(n_), rows_u(n_);
     for (int d : ckt_.load_order()) {
         const Device& dv = devs[d];
-        if (dv.kind == DK_D || dv.kind == DK_M1)
-            throw Error(NSA_ERR_UNSUPPORTED, "nonlinear device families are not wired in this engine yet");
         for (size_t k = 0; k < dv.mat_slot.size(); ++k)
             if (dv.mat_slot[k] >= 0) slot_sites[dv.mat_slot[k]].push_back(dev_site_[d] + (int)k);
+        // UCB devices stamp the circuit-global one-based rhs, which
+        // newton_solve adds to rhs after all loads (newton.cpp:166-168)
+        auto& rows = (dv.kind == DK_D || dv.kind == DK_M1) ? rows_u : rows_d;
         for (size_t k = 0; k < dv.rhs_row.size(); ++k)
-            if (dv.rhs_row[k] >= 0) rows_d[dv.rhs_row[k]].push_back(dev_rsite_[d] + (int)k);
+            if (dv.rhs_row[k] >= 0) rows[dv.rhs_row[k]].push_back(dev_rsite_[d] + (int)k);
     }
     for (size_t k = 0; k < ckt_.dead_nodes().size(); ++k) {
         const int nd = ckt_.dead_nodes()[k];
@@ -211,6 +249,15 @@ void BatchEngine::build_tables() {
     d_i_val_ = dupload(allocs_, replicate(ival, B_));
     isrc_.dc = d_i_val_;
     d_const_vals_ = dupload(allocs_, const_vals);
+    dio_.nodes = dupload(allocs_, dio_nodes);
+    dio_.state = dupload(allocs_, dio_state);
+    dio_.par = dupload(allocs_, dio_par);
+    mos_.nodes = dupload(allocs_, mos_nodes);
+    mos_.state = dupload(allocs_, mos_state);
+    mos_.par = dupload(allocs_, mos_par);
+    const size_t n_aux = (size_t)std::max(1, mos_.count) * kMos1Aux * B_;
+    mos_.aux = dalloc<double>(allocs_, n_aux);
+    NSA_HIP(hipMemset(mos_.aux, 0, n_aux * sizeof(double)));
     // constant sites are written once
     launch_res_sites(nR_, d_r_ohms_, d_cm_, r_site_, B_, stream_);
     launch_const_sites(n_const_, d_const_vals_, d_cm_, const_site_, B_, stream_);
@@ -255,6 +302,10 @@ void BatchEngine::load(const int32_t* d_done) {
     launch_cap(cap_, d_ctx_, d_x_, d_s_[0], d_s_[1], d_s_[2], d_done, d_cm_, d_cr_, B_, stream_);
     launch_vsrc(vsrc_, d_ctx_, d_done, d_cr_, B_, stream_);
     launch_isrc(isrc_, d_ctx_, d_done, d_cr_, B_, stream_);
+    launch_diode(dio_, d_ctx_, d_x_, d_s_[0], d_s_[1], d_s_[2], d_done, d_cm_, d_cr_, d_devnc_,
+                 tol_, B_, stream_);
+    launch_mos1(mos_, d_ctx_, d_x_, d_s_[0], d_s_[1], d_s_[2], d_done, d_cm_, d_cr_, d_devnc_,
+                tol_, B_, stream_);
     launch_gather(nnz_, d_mptr_, d_msites_, nullptr, nullptr, d_cm_, nullptr, d_A_, B_, stream_);
     launch_gather(n_, d_rptr_d_, d_rsites_d_, d_rptr_u_, d_rsites_u_, d_cr_, d_cr_, d_rhs_, B_,
                   stream_);
@@ -309,6 +360,7 @@ NewtonOutcome BatchEngine::newton(int itl) {
         // old_solution = solution (newton.cpp:138)
         NSA_HIP(hipMemcpyAsync(d_xold_, d_x_, sizeof(double) * n_ * B_, hipMemcpyDeviceToDevice,
                                stream_));
+        launch_fill_i32(d_devnc_, 0, B_, stream_);
         load(d_done_);
         if (time_phases) NSA_HIP(hipStreamSynchronize(stream_));
         auto t1 = clk::now();
@@ -359,6 +411,10 @@ NewtonOutcome BatchEngine::newton(int itl) {
         launch_fill_i32(d_nonconv_, 0, B_, stream_);
         launch_conv(n_, num_nodes, d_x_, d_xold_, opt_.reltol, opt_.vntol, opt_.abstol, d_done_,
                     d_nonconv_, B_, stream_);
+        // device convergence (newton.cpp:321-334): the loads' CKTnoncon and
+        // MOS1's current test on the proposed solution
+        launch_mos1_conv(mos_, d_x_, d_s_[0], d_done_, d_devnc_, tol_, B_, stream_);
+        launch_or_i32(d_nonconv_, d_devnc_, B_, stream_);
         NSA_HIP(hipMemcpyAsync(nonconv.data(), d_nonconv_, sizeof(int32_t) * B_,
                                hipMemcpyDeviceToHost, stream_));
         NSA_HIP(hipStreamSynchronize(stream_));

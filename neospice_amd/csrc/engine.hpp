@@ -113,6 +113,12 @@ private:
     CapTable cap_{};
     VsrcTable vsrc_{};
     IsrcTable isrc_{};
+    DioTable dio_{};
+    Mos1Table mos_{};
+    DevTol tol_{};
+    // per-instance device non-convergence of the current iteration
+    // (CKTnoncon of the loads, device_converged after the solve)
+    int32_t* d_devnc_ = nullptr;
     int nR_ = 0, r_site_ = 0;
     double* d_r_ohms_ = nullptr;
     double *d_c_val_ = nullptr, *d_v_val_ = nullptr, *d_i_val_ = nullptr;

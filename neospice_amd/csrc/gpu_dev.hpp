@@ -4,6 +4,8 @@
 
 #include <cstdint>
 
+#include "devices.hpp"
+
 namespace nsa {
 
 // Per-instance integrator context (the fields of the reference's
@@ -17,6 +19,8 @@ struct DevCtx {
     double xmu_ratio;
     double time;      // current_time of the step being solved
     double src_fact;  // SimOptions::src_fact
+    double delta;       // CKTdelta: the step being solved
+    double delta_old1;  // CKTdeltaOld[1]: the previous step (predictor ratio)
 };
 
 // CKTmode bits (ngspice cktdefs.h, used by src/core/newton.cpp:109-115).
@@ -64,6 +68,36 @@ struct IsrcTable {
     int32_t rsite;             // 2 per device
 };
 
+// Level-1 diodes (RS = 0: posPrime is the anode).  7 matrix sites and 2 rhs
+// sites per device, in DIOload's stamp order.
+struct DioTable {
+    int32_t count;
+    const int32_t* nodes;   // 2 per device: anode, cathode (-1 = ground)
+    const int32_t* state;   // state base per device (5 slots)
+    const DioParam* par;    // per device
+    int32_t site, rsite;
+};
+
+// MOS1 (RD = RS = 0: dNodePrime = d, sNodePrime = s).  22 matrix sites and
+// 4 rhs sites per device, in MOS1load's stamp order.  `aux` holds the
+// instance fields that outlive one load (count x kMos1Aux x B).
+enum Mos1Aux : int32_t {
+    M1_VON = 0, M1_MODE, M1_CD, M1_CBS, M1_CBD, M1_GBS, M1_GBD, M1_GM, M1_GDS, M1_GMBS, kMos1Aux
+};
+struct Mos1Table {
+    int32_t count;
+    const int32_t* nodes;   // 4 per device: d, g, s, b
+    const int32_t* state;   // state base per device (17 slots)
+    const Mos1Param* par;   // per device
+    double* aux;
+    int32_t site, rsite;
+};
+
+// Tolerances a device load reads from SimOptions.
+struct DevTol {
+    double gmin, reltol, abstol;
+};
+
 void launch_res_sites(int count, const double* ohms, double* contrib, int site, int B,
                       hipStream_t s);
 void launch_const_sites(int count, const double* vals, double* contrib, int site, int B,
@@ -74,6 +108,19 @@ void launch_vsrc(const VsrcTable& t, const DevCtx* ctx, const int32_t* done, dou
                  hipStream_t s);
 void launch_isrc(const IsrcTable& t, const DevCtx* ctx, const int32_t* done, double* cr, int B,
                  hipStream_t s);
+// DIOload / MOS1load; a device that limits its junction voltages or fails
+// its own convergence test sets devnc[inst] (CKTnoncon, device_converged).
+void launch_diode(const DioTable& t, const DevCtx* ctx, const double* x, double* s0, double* s1,
+                  const double* s2, const int32_t* done, double* cm, double* cr, int32_t* devnc,
+                  DevTol tol, int B, hipStream_t s);
+void launch_mos1(const Mos1Table& t, const DevCtx* ctx, const double* x, double* s0, double* s1,
+                 const double* s2, const int32_t* done, double* cm, double* cr, int32_t* devnc,
+                 DevTol tol, int B, hipStream_t s);
+// MOS1Device::conv_test on the proposed solution x (mos1_device.cpp:376-416).
+void launch_mos1_conv(const Mos1Table& t, const double* x, const double* s0,
+                      const int32_t* skip, int32_t* devnc, DevTol tol, int B, hipStream_t s);
+// dst[i] |= src[i], i < count
+void launch_or_i32(int32_t* dst, const int32_t* src, int count, hipStream_t s);
 void launch_gather(int nrows, const int32_t* ptr, const int32_t* sites, const int32_t* ptr2,
                    const int32_t* sites2, const double* contrib, const double* contrib2,
                    double* out, int B, hipStream_t s);

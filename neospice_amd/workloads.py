@@ -65,3 +65,62 @@ def mesh_instance_resistors(info, batch, config_index=1, sigma=0.10):
     for b, r in enumerate(rs):
         out[:, b] = nominal * (1.0 + sigma * r.standard_normal(len(nominal)))
     return out
+
+
+def diode_mesh(m=32, config_index=5, r_range=(1.0, 100.0), c_range=(10e-15, 100e-15),
+               n_pwl=16, v_max=1.2, t_stop=1e-9):
+    """Nonlinear mesh: m x m nodes, a resistor on every grid edge
+    (log-uniform in r_range), a capacitor and a diode (IS=1e-14, N=1,
+    anode at the node, cathode at ground) per node, and a PWL voltage
+    source at node (0, 0) with n_pwl seeded breakpoints in [0, v_max] over
+    [0, t_stop]."""
+    g = rng(config_index)
+    ckt = Circuit()
+    ckt.model(".model dmesh D(IS=1e-14 N=1)")
+    nid = [[ckt.node("n%d_%d" % (i, j)) for j in range(m)] for i in range(m)]
+    res_devs, res_vals = [], []
+    for i in range(m):
+        for j in range(m):
+            for di, dj, tag in ((0, 1, "h"), (1, 0, "v")):
+                ii, jj = i + di, j + dj
+                if ii < m and jj < m:
+                    r = float(np.exp(g.uniform(np.log(r_range[0]), np.log(r_range[1]))))
+                    res_devs.append(ckt.R("r%s%d_%d" % (tag, i, j), nid[i][j], nid[ii][jj], r))
+                    res_vals.append(r)
+    for i in range(m):
+        for j in range(m):
+            c = float(np.exp(g.uniform(np.log(c_range[0]), np.log(c_range[1]))))
+            ckt.C("c%d_%d" % (i, j), nid[i][j], GND, c)
+            ckt.D("d%d_%d" % (i, j), nid[i][j], GND, "dmesh")
+    t = np.sort(g.uniform(0.0, t_stop, n_pwl - 1))
+    t = np.concatenate([[0.0], t])
+    v = g.uniform(0.0, v_max, n_pwl)
+    v[0] = 0.0
+    ckt.V_pwl("vin", nid[0][0], GND, t.tolist(), v.tolist())
+    info = {"res_devs": np.array(res_devs), "res_vals": np.array(res_vals), "m": m,
+            "pwl_t": t, "pwl_v": v}
+    return ckt, info
+
+
+def inverter_chain(stages=50, vdd=1.8, r_load=1e3, c_load=2e-15):
+    """MOS1 inverter chain: LEVEL=1 NMOS/PMOS (VTO +-0.5, KP 120u/40u,
+    LAMBDA 0.02), a PULSE input 0 -> vdd (50 ps edges, 1 ns period) and,
+    per stage, an RC load (series r_load to a node with c_load to ground)
+    that drives the next stage: 2 nodes per stage, 2 + 2*stages + 2
+    unknowns."""
+    ckt = Circuit()
+    ckt.model(".model nch NMOS(LEVEL=1 VTO=0.5 KP=120u LAMBDA=0.02)")
+    ckt.model(".model pch PMOS(LEVEL=1 VTO=-0.5 KP=40u LAMBDA=0.02)")
+    nvdd, nin = ckt.node("vdd"), ckt.node("in")
+    ckt.V("vsup", nvdd, GND, vdd)
+    ckt.V_pulse("vin", nin, GND, 0.0, vdd, 0.0, 50e-12, 50e-12, 0.45e-9, 1e-9)
+    drive = nin
+    for k in range(stages):
+        o = ckt.node("o%d" % k)
+        l = ckt.node("l%d" % k)
+        ckt.M("mn%d" % k, o, drive, GND, GND, "nch", 1e-6, 0.18e-6)
+        ckt.M("mp%d" % k, o, drive, nvdd, nvdd, "pch", 2e-6, 0.18e-6)
+        ckt.R("rl%d" % k, o, l, r_load)
+        ckt.C("cl%d" % k, l, GND, c_load)
+        drive = l
+    return ckt, {"stages": stages}

@@ -407,7 +407,8 @@ def split_newton_calls(ints, dbls):
         if mode & 0x1 and mode & (0x1000 | 0x2000):
             calls.append({"first": k, "iters": 0, "mode": mode, "order": int(ints[k, 1]),
                           "method": int(ints[k, 2]), "ag": tuple(dbls[k, 0:3]),
-                          "delta": float(dbls[k, 3]), "xmu_ratio": float(dbls[k, 7]),
+                          "delta": float(dbls[k, 3]), "dold1": float(dbls[k, 5]),
+                          "xmu_ratio": float(dbls[k, 7]),
                           "time": float(dbls[k, 8]), "acc0": int(ints[k, 3])})
         if calls and mode & 0x1:
             calls[-1]["iters"] += 1

@@ -64,7 +64,8 @@ def replay(eng, calls):
     xs = []
     last = eng.solution()
     for c in calls:
-        ctx = na.make_ctx(c["mode"], c["order"], c["method"], c["ag"], c["xmu_ratio"], c["time"])
+        ctx = na.make_ctx(c["mode"], c["order"], c["method"], c["ag"], c["xmu_ratio"], c["time"],
+                          delta=c["delta"], delta_old1=c["dold1"])
         conv, its = eng.tran_step(ctx)
         if c["accepted"]:
             assert conv.all()

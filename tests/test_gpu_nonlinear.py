@@ -1,0 +1,163 @@
+"""GPU parity of the nonlinear device loads (level-1 diode, MOS1) and the
+batched Newton engine on circuits built from them, against the compiled
+reference:
+
+  * stamp level: matrix values and rhs of one load at a seeded iterate, in
+    DC and transient modes, with the device states carried through the same
+    load sequence on both sides;
+  * DC operating point (direct Newton): iteration counts equal, solution
+    within 1e-9 relative;
+  * transient: the engine replays the reference's own accepted/rejected
+    step sequence; waveforms within 1e-6 relative.
+
+The device math uses exp/log/pow/sqrt; the GPU's libm rounds some results
+differently from glibc in the last ulp, so these comparisons carry
+tolerances (stated per test) instead of bit equality."""
+import numpy as np
+import pytest
+
+import neospice_amd as na
+import refbind as RB
+from neospice_amd import workloads as W
+
+pytestmark = [pytest.mark.gpu, pytest.mark.ref]
+
+LOAD_RTOL = 1e-11
+
+
+def close_load(v, v_ref, what):
+    scale = max(1e-30, float(np.abs(v_ref).max()))
+    err = np.abs(v - v_ref)
+    bad = err > LOAD_RTOL * np.maximum(np.abs(v_ref), 1e-6 * scale)
+    assert not bad.any(), (what, int(np.argmax(err)), float(err.max()), scale)
+
+
+def diode_ckt(m):
+    ckt, info = W.diode_mesh(m, t_stop=100e-12)
+    ckt.finalize()
+    return ckt, info
+
+
+def mos_ckt(stages):
+    ckt, info = W.inverter_chain(stages)
+    ckt.finalize()
+    return ckt, info
+
+
+def load_sequence(ckt, seed, B=2):
+    """The same sequence of loads on the engine and the reference: DC
+    INITJCT, DC INITFLOAT at a random iterate, TRANOP, then transient INITTRAN
+    and INITFLOAT loads (trapezoidal)."""
+    rc = RB.RefCircuit(ckt, probe=False)
+    eng = na.BatchEngine(ckt, batch=B)
+    rng = np.random.default_rng(seed)
+    dt = 1e-12
+    seq = [
+        ("dc_jct", na.MODEDCOP | na.MODEINITJCT, 1, (0, 0, 0)),
+        ("dc_float", na.MODEDCOP | na.MODEINITFLOAT, 1, (0, 0, 0)),
+        ("dc_float2", na.MODEDCOP | na.MODEINITFLOAT, 1, (0, 0, 0)),
+        ("tranop", na.MODETRANOP | na.MODEINITFLOAT, 1, (0, 0, 0)),
+        ("tran_init", na.MODETRAN | na.MODEINITTRAN, 1, (1 / dt, -1 / dt, 0)),
+        ("tran_float", na.MODETRAN | na.MODEINITFLOAT, 1, (1 / dt, -1 / dt, 0)),
+        ("tran_trap", na.MODETRAN | na.MODEINITFLOAT, 2, (2 / dt, -2 / dt, 0)),
+    ]
+    for name, mode, order, ag in seq:
+        x = rng.uniform(-0.2, 1.0, ckt.n)
+        eng.set_solution(np.repeat(x[:, None], B, axis=1))
+        eng.set_ctx(na.make_ctx(mode, order, 0, ag, 1.0, 2 * dt, 1.0, dt, dt))
+        rc.set_ctx(mode, order, 0, ag, dt, dt, 1.0, 2 * dt)
+        v_ref, r_ref = rc.load(x)
+        v, r = eng.load()
+        for i in range(B):
+            close_load(v[:, i], v_ref, name + " matrix")
+            close_load(r[:, i], r_ref, name + " rhs")
+
+
+def test_diode_loads():
+    ckt, _ = diode_ckt(4)
+    load_sequence(ckt, 11)
+
+
+def test_mos1_loads():
+    ckt, _ = mos_ckt(4)
+    load_sequence(ckt, 12)
+
+
+def dc_parity(ckt, B, set_values=None, values_ref=None):
+    eng = na.BatchEngine(ckt, batch=B)
+    if set_values:
+        set_values(eng)
+    x, its = eng.dc_op()
+    for i in range(B):
+        rc = RB.RefCircuit(ckt, values=values_ref(i) if values_ref else None, probe=False)
+        x_ref, its_ref, _ = rc.dc()
+        assert its[i] == its_ref, (i, its[i], its_ref)
+        tol = 1e-9 * np.maximum(np.abs(x_ref), 1e-3) + 1e-12
+        assert (np.abs(x[:, i] - x_ref) <= tol).all(), (i, np.max(np.abs(x[:, i] - x_ref)))
+
+
+def test_diode_dc_op_batched():
+    ckt, info = diode_ckt(8)
+    B = 3
+    rs = W.instance_rngs(5, B)
+    R = np.stack([info["res_vals"] * (1 + 0.1 * r.standard_normal(len(info["res_vals"])))
+                  for r in rs], axis=1)
+
+    def setv(eng):
+        for k, d in enumerate(info["res_devs"]):
+            eng.set_values(int(d), R[k])
+    dc_parity(ckt, B, setv,
+              lambda i: {int(d): float(R[k, i]) for k, d in enumerate(info["res_devs"])})
+
+
+def test_mos1_dc_op():
+    ckt, _ = mos_ckt(10)
+    dc_parity(ckt, 2)
+
+
+def replay(eng, calls):
+    xs, its = [], []
+    last = eng.solution()
+    for c in calls:
+        ctx = na.make_ctx(c["mode"], c["order"], c["method"], c["ag"], c["xmu_ratio"], c["time"],
+                          delta=c["delta"], delta_old1=c["dold1"])
+        conv, it = eng.tran_step(ctx)
+        its.append((int(it[0]), c["iters"]))
+        if c["accepted"]:
+            assert conv.all(), c
+            eng.rotate_states()
+            last = eng.solution()
+            xs.append((c["time"], last.copy()))
+        else:
+            eng.set_solution(last)
+    return xs, its
+
+
+def tran_parity(ckt, tstep, tstop, rtol=1e-6):
+    rc = RB.RefCircuit(ckt)
+    t_ref, x_ref, _, _ = rc.tran(tstep, tstop)
+    calls = RB.split_newton_calls(*rc.records())
+    eng = na.BatchEngine(ckt, batch=1)
+    x0, _ = eng.tran_begin()
+    tol0 = 1e-9 * np.maximum(np.abs(x_ref[0]), 1e-3) + 1e-12
+    assert (np.abs(x0[:, 0] - x_ref[0]) <= tol0).all()
+    xs, its = replay(eng, calls)
+    assert len(xs) == len(t_ref) - 1
+    worst = 0.0
+    for k, (t, x) in enumerate(xs):
+        assert t == t_ref[k + 1]
+        scale = np.maximum(np.abs(x_ref[k + 1]), 1e-3)
+        worst = max(worst, float(np.max(np.abs(x[:, 0] - x_ref[k + 1]) / scale)))
+    mism = sum(a != b for a, b in its)
+    assert worst <= rtol, (worst, mism, len(its))
+    return worst, mism
+
+
+def test_diode_transient_replay():
+    ckt, _ = diode_ckt(5)
+    tran_parity(ckt, 2e-12, 100e-12)
+
+
+def test_mos1_transient_replay():
+    ckt, _ = mos_ckt(5)
+    tran_parity(ckt, 10e-12, 1e-9)
