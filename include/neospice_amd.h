@@ -106,6 +106,33 @@ int nsa_engine_load(nsa_engine* e, double* values, double* rhs);
 /* A^T x = b on the factors of the last Newton iteration (b in place, n x B). */
 int nsa_engine_solve_transposed(nsa_engine* e, double* b);
 /* Name of the engine in use ("markowitz" / "amdlu"), pivot statistics. */
+/* ---- parameters and adjoints ----
+ * No counterpart in the reference, whose only sensitivity analysis is the
+ * forward-difference .sens (src/core/sens.cpp:57-186: one extra DC solve per
+ * R, V and I); its batched / Monte Carlo and adjoint execution is listed as
+ * planned (ROADMAP.md:128-141).  Family codes match the device kinds. */
+enum { NSA_FAMILY_R = 0, NSA_FAMILY_C = 1, NSA_FAMILY_V = 2, NSA_FAMILY_I = 3 };
+int nsa_engine_family_count(nsa_engine* e, int32_t family, int32_t* count);
+/* All per-instance values of a family at once: count x B, instance fastest
+ * (R ohms, C farads, V / I dc value). */
+int nsa_engine_set_family_values(nsa_engine* e, int32_t family, const double* vals);
+int nsa_engine_get_family_values(nsa_engine* e, int32_t family, double* vals);
+/* DC adjoint on the factors of the last Newton solve: J^T lambda = dL/dx
+ * (n x B), then dL/dR, dL/dC (zero), dL/dV, dL/dI (count x B each; any
+ * output may be null). */
+int nsa_engine_dc_adjoint(nsa_engine* e, const double* dLdx, double* lambda, double* gR,
+                          double* gC, double* gV, double* gI);
+/* Fixed-step transient from the current solution: TRANOP operating point,
+ * then nsteps steps of dt (backward Euler first, then the options' method);
+ * probe node voltages out ((nsteps+1) x nprobe x B); a checkpoint every
+ * ckpt_every steps for nsa_engine_tran_adjoint. */
+int nsa_engine_tran_forward(nsa_engine* e, int32_t nsteps, double dt, const int32_t* probes,
+                            int32_t nprobe, int32_t ckpt_every, double* probe_out);
+/* Discrete adjoint of the last tran_forward: dL/d(probe values) in
+ * ((nsteps+1) x nprobe x B); dL/dR and dL/dC out (count x B). */
+int nsa_engine_tran_adjoint(nsa_engine* e, const double* dLdprobe, double* gR, double* gC);
+/* Wall milliseconds of the last tran_forward and tran_adjoint. */
+int nsa_engine_adjoint_timing(nsa_engine* e, double out[2]);
 const char* nsa_engine_solver_name(nsa_engine* e);
 /* out: cpu pivots, gpu refactors, nnz(L+U), refactor levels, solve levels
  * (of the first pivot group), gpu solves, device loads, pivot groups. */

@@ -92,6 +92,10 @@ class Circuit:
     def M(self, name, d, g, s, b, model, w, l):
         return self._dev("M", name, d, g, s, b, model, float(w), float(l))
 
+    def ops_devices(self):
+        """The recorded device calls, one per device index (kind, name, pins...)."""
+        return [op for op in self.ops if op[0] not in ("node", "model")]
+
     def _check_open(self):
         if self._native is not None:
             raise RuntimeError("Cannot add to a finalized circuit")
@@ -141,6 +145,12 @@ class Circuit:
         for d in range(len(self.devices)):
             check(L.nsa_circuit_branch(h, d, C.byref(dev)))
             self.branch[d] = dev.value
+        # device indices of each family in family (load) order: the row order
+        # of the engine's per-family value and gradient arrays
+        kinds = {"R": ("R",), "C": ("C",), "V": ("V", "Vpulse", "Vpwl"), "I": ("I",),
+                 "D": ("D",), "M": ("M",)}
+        self.family = {f: np.array([d for d in self.load_order if self.devices[d][0] in ks],
+                                   np.int32) for f, ks in kinds.items()}
         return self
 
     @property
@@ -161,6 +171,9 @@ class Circuit:
                 lib().nsa_circuit_destroy(self._native)
             except Exception:
                 pass
+
+
+FAMILIES = {"R": 0, "C": 1, "V": 2, "I": 3}
 
 
 def make_ctx(mode, order=1, method=0, ag=(0.0, 0.0, 0.0), xmu_ratio=1.0, time=0.0, src_fact=1.0,
@@ -277,6 +290,63 @@ class BatchEngine:
         t = np.zeros(3)
         check(lib().nsa_engine_timing(self.h, f64p(t)))
         return {"load_ms": float(t[0]), "refactor_ms": float(t[1]), "solve_ms": float(t[2])}
+
+    # -- parameters and adjoints --
+    def family_count(self, family):
+        c = C.c_int32()
+        check(lib().nsa_engine_family_count(self.h, FAMILIES[family], C.byref(c)))
+        return c.value
+
+    def set_family_values(self, family, vals):
+        """All per-instance values of a family: (count, B) array (R ohms, C
+        farads, V / I dc)."""
+        n = self.family_count(family)
+        v = as_f64(np.broadcast_to(np.asarray(vals, np.float64), (n, self.B)))
+        check(lib().nsa_engine_set_family_values(self.h, FAMILIES[family], f64p(v)))
+
+    def family_values(self, family):
+        v = np.zeros((self.family_count(family), self.B))
+        check(lib().nsa_engine_get_family_values(self.h, FAMILIES[family], f64p(v)))
+        return v
+
+    def dc_adjoint(self, dLdx):
+        """Gradients of a loss through the operating point of the last Newton
+        solve, given dL/dx (n x B): dict with lambda and dL/dR, dL/dC, dL/dV,
+        dL/dI (count x B)."""
+        g = as_f64(np.asarray(dLdx, np.float64).reshape(self.n, self.B))
+        out = {"lambda": np.zeros((self.n, self.B))}
+        for f in ("R", "C", "V", "I"):
+            out[f] = np.zeros((max(1, self.family_count(f)), self.B))
+        check(lib().nsa_engine_dc_adjoint(self.h, f64p(g), f64p(out["lambda"]), f64p(out["R"]),
+                                          f64p(out["C"]), f64p(out["V"]), f64p(out["I"])))
+        for f in ("R", "C", "V", "I"):
+            out[f] = out[f][:self.family_count(f)]
+        return out
+
+    def tran_forward(self, nsteps, dt, probes, ckpt_every=25, x0=0.0):
+        """Fixed-step transient (TRANOP operating point, then nsteps steps:
+        backward Euler first, then the options' method); returns the probe
+        voltages, (nsteps+1, nprobe, B)."""
+        self.set_solution(x0)
+        p = as_i32(probes)
+        out = np.zeros((nsteps + 1, len(p), self.B))
+        check(lib().nsa_engine_tran_forward(self.h, int(nsteps), float(dt), i32p(p), len(p),
+                                            int(ckpt_every), f64p(out)))
+        return out
+
+    def tran_adjoint(self, dLdprobe):
+        """dL/dR and dL/dC (count x B) of the last tran_forward, given dL/d
+        probe values shaped like its output."""
+        g = as_f64(np.asarray(dLdprobe, np.float64))
+        gR = np.zeros((max(1, self.family_count("R")), self.B))
+        gC = np.zeros((max(1, self.family_count("C")), self.B))
+        check(lib().nsa_engine_tran_adjoint(self.h, f64p(g), f64p(gR), f64p(gC)))
+        return gR[:self.family_count("R")], gC[:self.family_count("C")]
+
+    def adjoint_timing(self):
+        t = np.zeros(2)
+        check(lib().nsa_engine_adjoint_timing(self.h, f64p(t)))
+        return {"forward_ms": float(t[0]), "adjoint_ms": float(t[1])}
 
     # -- analyses --
     def dc_op(self, mode=MODEDCOP | MODEINITJCT, x0=0.0):

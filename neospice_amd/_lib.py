@@ -66,6 +66,13 @@ _SIGS = {
     "nsa_engine_reset_solver": (C.c_int, [VP]),
     "nsa_engine_load": (C.c_int, [VP, P_F64, P_F64]),
     "nsa_engine_solve_transposed": (C.c_int, [VP, P_F64]),
+    "nsa_engine_family_count": (C.c_int, [VP, I32, P_I32]),
+    "nsa_engine_set_family_values": (C.c_int, [VP, I32, P_F64]),
+    "nsa_engine_get_family_values": (C.c_int, [VP, I32, P_F64]),
+    "nsa_engine_dc_adjoint": (C.c_int, [VP, P_F64, P_F64, P_F64, P_F64, P_F64, P_F64]),
+    "nsa_engine_tran_forward": (C.c_int, [VP, I32, C.c_double, P_I32, I32, I32, P_F64]),
+    "nsa_engine_tran_adjoint": (C.c_int, [VP, P_F64, P_F64, P_F64]),
+    "nsa_engine_adjoint_timing": (C.c_int, [VP, P_F64]),
     "nsa_engine_solver_name": (C.c_char_p, [VP]),
     "nsa_engine_stats": (C.c_int, [VP, P_I64]),
     "nsa_engine_set_timing": (C.c_int, [VP, I32]),

@@ -22,10 +22,12 @@ SOURCES = [
     "lu_cpu.cpp",
     "batch_lu.cpp",
     "engine.cpp",
+    "adjoint.cpp",
     "capi.cpp",
     "gpu_lu.hip",
     "gpu_dev.hip",
     "gpu_nonlinear.hip",
+    "gpu_adjoint.hip",
 ]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]

@@ -259,6 +259,52 @@ int nsa_engine_solve_transposed(nsa_engine* e, double* b) {
         return 0;
     });
 }
+namespace {
+nsa::DevKind value_family(int32_t family) {
+    if (family < NSA_FAMILY_R || family > NSA_FAMILY_I)
+        throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "family must be NSA_FAMILY_R/C/V/I");
+    return (nsa::DevKind)family;
+}
+}  // namespace
+int nsa_engine_family_count(nsa_engine* e, int32_t family, int32_t* count) {
+    return guard([&] { *count = e->e->family_count(value_family(family)); return 0; });
+}
+int nsa_engine_set_family_values(nsa_engine* e, int32_t family, const double* vals) {
+    return guard([&] { e->e->set_family_values(value_family(family), vals); return 0; });
+}
+int nsa_engine_get_family_values(nsa_engine* e, int32_t family, double* vals) {
+    return guard([&] { e->e->get_family_values(value_family(family), vals); return 0; });
+}
+int nsa_engine_dc_adjoint(nsa_engine* e, const double* dLdx, double* lambda, double* gR,
+                          double* gC, double* gV, double* gI) {
+    return guard([&] {
+        if (!dLdx) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "dc_adjoint: dLdx is null");
+        e->e->dc_adjoint(dLdx, lambda, gR, gC, gV, gI);
+        return 0;
+    });
+}
+int nsa_engine_tran_forward(nsa_engine* e, int32_t nsteps, double dt, const int32_t* probes,
+                            int32_t nprobe, int32_t ckpt_every, double* probe_out) {
+    return guard([&] {
+        if (nprobe > 0 && !probes) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "tran_forward: probes is null");
+        e->e->tran_forward(nsteps, dt, probes, nprobe, ckpt_every, probe_out);
+        return 0;
+    });
+}
+int nsa_engine_tran_adjoint(nsa_engine* e, const double* dLdprobe, double* gR, double* gC) {
+    return guard([&] {
+        if (!dLdprobe) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "tran_adjoint: dLdprobe is null");
+        e->e->tran_adjoint(dLdprobe, gR, gC);
+        return 0;
+    });
+}
+int nsa_engine_adjoint_timing(nsa_engine* e, double out[2]) {
+    return guard([&] {
+        out[0] = e->e->ms_forward;
+        out[1] = e->e->ms_adjoint;
+        return 0;
+    });
+}
 const char* nsa_engine_solver_name(nsa_engine* e) { return e->e->lu().name(); }
 int nsa_engine_stats(nsa_engine* e, int64_t out[8]) {
     return guard([&] {

@@ -77,6 +77,7 @@ BatchEngine::BatchEngine(const Circuit& ckt, int batch, const Options& opt)
 }
 
 BatchEngine::~BatchEngine() {
+    free_tran();
     lu_.reset();
     for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamSynchronize(stream_);

@@ -76,6 +76,33 @@ public:
     // stamp-level parity checks).  Matrix and rhs end up in d_matrix/d_rhs.
     void load_only();
 
+    // ---- parameters and adjoints (adjoint.cpp)
+    int family_count(DevKind k) const { return (int)ckt_.family(k).size(); }
+    // All per-instance values of a family (R ohms, C farads, V/I dc) at
+    // once: count x B, device-major with the instance fastest.
+    void set_family_values(DevKind k, const double* h_vals);
+    void get_family_values(DevKind k, double* h_vals) const;
+    // DC operating-point adjoint on the factors of the last newton() call:
+    // J^T lambda = dL/dx, then dL/dp = -lambda^T dF/dp for every R (ohms),
+    // C (farads: zero in DC), V and I (dc value).  Host arrays; any output
+    // may be null.
+    void dc_adjoint(const double* h_dLdx, double* h_lambda, double* h_gR, double* h_gC,
+                    double* h_gV, double* h_gI);
+    // Fixed-step transient from the current solution as the initial guess:
+    // the TRANOP operating point, then nsteps steps of dt (backward Euler
+    // first, then the options' method at order 2).  Records the probe nodes'
+    // voltages ((nsteps+1) x nprobe x B) and keeps a checkpoint of the full
+    // engine state every ckpt_every steps for tran_adjoint.
+    void tran_forward(int nsteps, double dt, const int32_t* probes, int nprobe, int ckpt_every,
+                      double* h_probes);
+    // Discrete adjoint of the last tran_forward: given dL/d(probe values),
+    // accumulates dL/dR and dL/dC (count x B).  Segments between checkpoints
+    // are recomputed forward, then swept backward with one Jacobian load,
+    // refactor and transposed solve per step.
+    void tran_adjoint(const double* h_dLdprobe, double* h_gR, double* h_gC);
+    // Wall-clock milliseconds of the last tran_forward / tran_adjoint.
+    double ms_forward = 0.0, ms_adjoint = 0.0;
+
     // Timing breakdown of the last newton() call, milliseconds.
     double ms_load = 0, ms_factor = 0, ms_solve = 0;
     bool time_phases = false;
@@ -87,6 +114,13 @@ public:
 
 private:
     void build_tables();
+    void ensure_adjoint();
+    double* family_values_ptr(DevKind k) const;
+    void factor_current();
+    void jacobian_at(const double* d_x, const DevCtx& c);
+    DevCtx step_ctx(int k, double dt) const;
+    void save_checkpoint(int slot);
+    void restore_checkpoint(int slot);
     void load(const int32_t* d_done);
 
     const Circuit& ckt_;
@@ -133,6 +167,24 @@ private:
     std::vector<int32_t> dev_rsite_; // first rhs site per device
     bool timing_ = false;
     hipEvent_t ev_[2] = {nullptr, nullptr};
+    // adjoint tables and buffers (allocated on first use)
+    bool adj_ready_ = false;
+    int32_t *d_r_nodes_ = nullptr, *d_v_branch_ = nullptr, *d_i_nodes_ = nullptr;
+    int32_t *d_cinc_ptr_ = nullptr, *d_cinc_ent_ = nullptr;
+    double *d_lam_ = nullptr, *d_carry_ = nullptr, *d_ibar_ = nullptr, *d_q_ = nullptr;
+    double *d_gR_ = nullptr, *d_gC_ = nullptr, *d_gV_ = nullptr, *d_gI_ = nullptr;
+    // transient record of the last tran_forward
+    int tr_steps_ = 0, tr_ckpt_ = 0, tr_nprobe_ = 0;
+    double tr_dt_ = 0.0;
+    std::vector<DevCtx> tr_ctx_;       // per step 0..nsteps (0 = TRANOP)
+    int32_t* d_probes_ = nullptr;
+    double* d_probe_vals_ = nullptr;   // (nsteps+1) x nprobe x B
+    double* d_ckpt_ = nullptr;         // checkpoints: x, s0..s3, MOS1 aux
+    size_t ckpt_stride_ = 0;
+    double* d_segx_ = nullptr;         // (ckpt+1) x n x B solutions of one segment
+    double* d_dlp_ = nullptr;          // dL/d(probe values)
+    std::vector<void*> tr_allocs_;
+    void free_tran();
 };
 
 }  // namespace nsa

@@ -39,6 +39,10 @@ enum ModeBits : int32_t {
     MODEINITPRED = 0x2000,
     MODEUIC = 0x10000,
     INITF_MASK = 0x3F00,
+    // Not an ngspice bit: evaluate the devices at the given solution with no
+    // junction limiting or prediction, for the adjoint's Jacobian at a
+    // converged point (where the forward's last load did not limit either).
+    MODEADJOINT = 0x40000000,
 };
 
 struct CapTable {

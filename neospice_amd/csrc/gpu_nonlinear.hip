@@ -208,7 +208,10 @@ __global__ void k_diode(DioTable t, const DevCtx* __restrict__ ctx, const double
     const int fl = p.flags;
     int Check = 1;
     double vd;
-    if (mode & MODEINITSMSIG) {
+    if (mode & MODEADJOINT) {
+        vd = v_pos - v_neg;
+        Check = 0;
+    } else if (mode & MODEINITSMSIG) {
         vd = s0[sv];
     } else if (mode & MODEINITTRAN) {
         vd = s1[sv];
@@ -471,7 +474,9 @@ __global__ void k_mos1(Mos1Table t, const DevCtx* __restrict__ ctx, const double
         const double vgdo = s0[S(vgsS)] - s0[S(vdsS)];
         von = type * aux[M1_VON * sB];
         // limiting
-        if (s0[S(vdsS)] >= 0) {
+        if (mode & MODEADJOINT) {
+            Check = 0;
+        } else if (s0[S(vdsS)] >= 0) {
             vgs = fetlim(vgs, s0[S(vgsS)], von);
             vds = vgs - vgd;
             vds = limvds(vds, s0[S(vdsS)]);
@@ -482,7 +487,8 @@ __global__ void k_mos1(Mos1Table t, const DevCtx* __restrict__ ctx, const double
             vds = -limvds(-vds, -(s0[S(vdsS)]));  // CKTfixLimit = 0
             vgs = vgd + vds;
         }
-        if (vds >= 0) {
+        if (mode & MODEADJOINT) {
+        } else if (vds >= 0) {
             vbs = pnjlim(vbs, s0[S(vbsS)], vt, p.sourceVcrit, &Check);
             vbd = vbs - vds;
         } else {

@@ -68,7 +68,7 @@ def mesh_instance_resistors(info, batch, config_index=1, sigma=0.10):
 
 
 def diode_mesh(m=32, config_index=5, r_range=(1.0, 100.0), c_range=(10e-15, 100e-15),
-               n_pwl=16, v_max=1.2, t_stop=1e-9):
+               n_pwl=16, v_max=1.2, t_stop=1e-9, diodes=True):
     """Nonlinear mesh: m x m nodes, a resistor on every grid edge
     (log-uniform in r_range), a capacitor and a diode (IS=1e-14, N=1,
     anode at the node, cathode at ground) per node, and a PWL voltage
@@ -91,7 +91,8 @@ def diode_mesh(m=32, config_index=5, r_range=(1.0, 100.0), c_range=(10e-15, 100e
         for j in range(m):
             c = float(np.exp(g.uniform(np.log(c_range[0]), np.log(c_range[1]))))
             ckt.C("c%d_%d" % (i, j), nid[i][j], GND, c)
-            ckt.D("d%d_%d" % (i, j), nid[i][j], GND, "dmesh")
+            if diodes:
+                ckt.D("d%d_%d" % (i, j), nid[i][j], GND, "dmesh")
     t = np.sort(g.uniform(0.0, t_stop, n_pwl - 1))
     t = np.concatenate([[0.0], t])
     v = g.uniform(0.0, v_max, n_pwl)

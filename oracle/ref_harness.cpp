@@ -470,6 +470,89 @@ int ref_tran_fixed(void* p, double dt, int nsteps, double* solution_out, double*
     });
 }
 
+// The same fixed-step sequence with the step order of solve_transient's
+// controller at a constant step: backward Euler for the first step, then
+// order 2 of the circuit's integration method (trapezoidal with the
+// options' xmu, fill_integrator_context, transient.cpp:398-447).  Writes
+// every accepted solution ((nsteps+1) x n, row 0 = operating point).
+int ref_tran_steps(void* p, double dt, int nsteps, double* solutions_out, double* seconds) {
+    auto* h = static_cast<RefCkt*>(p);
+    return guarded([&] {
+        Circuit& ckt = h->ckt;
+        ckt.finalize_if_needed();
+        const int n = ckt.num_vars();
+        ckt.integrator_ctx.options = &ckt.options;
+        std::vector<double> sol(n, 0.0);
+        {
+            auto dc_solver = make_solver(n, ckt.is_linear());
+            dc_solver->symbolic(ckt.pattern());
+            ckt.integrator_ctx.mode = 0x20 | 0x200;  // MODETRANOP | MODEINITJCT
+            NewtonResult r = newton_solve(ckt, *dc_solver, sol, ckt.options);
+            if (!r.converged) throw std::runtime_error("DC preamble did not converge");
+        }
+        if (solutions_out) std::copy(sol.begin(), sol.end(), solutions_out);
+        auto solver = make_solver(n, ckt.is_linear());
+        solver->symbolic(ckt.pattern());
+        NewtonWorkspace ws(ckt.pattern());
+        const int ns = ckt.num_states();
+        std::copy_n(ckt.state0(), ns, ckt.state1());
+        std::copy_n(ckt.state0(), ns, ckt.state2());
+        std::copy_n(ckt.state0(), ns, ckt.state3());
+        SimOptions topts = ckt.options;
+        topts.max_iter = ckt.options.itl4;
+        const bool gear = ckt.integrator_ctx.integrate_method == 1;
+        int iters = 0;
+        auto t0 = std::chrono::steady_clock::now();
+        for (int k = 0; k < nsteps; ++k) {
+            auto& c = ckt.integrator_ctx;
+            const double t = (k + 1) * dt;
+            const int order = (k == 0) ? 1 : 2;
+            c.order = order;
+            c.delta = dt;
+            c.current_time = t;
+            c.delta_old[0] = dt;
+            c.delta_old[1] = dt;
+            c.mode = 0x1 | (k == 0 ? 0x1000 : 0x2000);
+            if (order == 1) {
+                c.ag[0] = 1.0 / dt;
+                c.ag[1] = -1.0 / dt;
+                c.ag[2] = 0.0;
+                c.xmu_ratio = 0.0;
+            } else if (!gear) {
+                const double om = 1.0 - ckt.options.xmu;
+                c.ag[0] = 1.0 / (dt * om);
+                c.ag[1] = -1.0 / (dt * om);
+                c.ag[2] = 0.0;
+                c.xmu_ratio = ckt.options.xmu / om;
+            } else {
+                const double h_old = dt, sum = dt + h_old;
+                c.ag[0] = (2.0 * dt + h_old) / (dt * sum);
+                c.ag[1] = -(dt + h_old) / (dt * h_old);
+                c.ag[2] = dt / (h_old * sum);
+                c.xmu_ratio = 0.0;
+            }
+            for (auto& dev : ckt.devices()) {
+                if (auto* cp = dynamic_cast<Capacitor*>(dev.get())) {
+                    cp->set_transient(dt);
+                    cp->set_integrator_order(order);
+                } else if (auto* vs = dynamic_cast<VSource*>(dev.get())) {
+                    vs->set_time(t);
+                } else if (auto* is = dynamic_cast<ISource*>(dev.get())) {
+                    is->set_time(t);
+                }
+            }
+            NewtonResult r = newton_solve(ckt, *solver, sol, topts, ws);
+            if (!r.converged) return -1;
+            iters += r.iterations;
+            ckt.rotate_state();
+            if (solutions_out) std::copy(sol.begin(), sol.end(), solutions_out + (size_t)(k + 1) * n);
+        }
+        auto t1 = std::chrono::steady_clock::now();
+        if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+        return iters;
+    });
+}
+
 int ref_tran_num_records(void* p) { return (int)static_cast<RefCkt*>(p)->recs.size(); }
 
 // Record layout: 4 ints (mode, order, method, accepted_before) and

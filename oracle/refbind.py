@@ -69,6 +69,7 @@ def ref():
             "ref_tran_records": (None, [VP, P_I32, P_F64]),
             "ref_newton": (C.c_int, [VP, P_F64, C.c_int]),
             "ref_sens": (C.c_int, [VP, C.c_int, C.c_int, P_F64, C.c_char_p, C.c_int]),
+            "ref_tran_steps": (C.c_int, [VP, C.c_double, C.c_int, P_F64, P_F64]),
             "ref_solver_new": (VP, [C.c_int, C.c_int, P_I32, P_I32]),
             "ref_solver_free": (None, [VP]),
             "ref_solver_numeric": (C.c_int, [VP, P_F64, C.c_double]),
@@ -307,6 +308,30 @@ class RefCircuit:
         sec = C.c_double()
         its = _chk(self.L.ref_tran_fixed(self.h, dt, nsteps, x.ctypes.data_as(P_F64), C.byref(sec)))
         return x, its, sec.value
+
+    def tran_steps(self, dt, nsteps):
+        """Fixed-step transient with the controller's constant-step order
+        sequence (backward Euler, then order 2 of the circuit's method):
+        all solutions ((nsteps+1) x n), Newton iterations, seconds."""
+        xs = np.zeros((nsteps + 1) * self.n)
+        sec = C.c_double()
+        its = _chk(self.L.ref_tran_steps(self.h, dt, nsteps, xs.ctypes.data_as(P_F64),
+                                         C.byref(sec)))
+        return xs.reshape(nsteps + 1, self.n), its, sec.value
+
+    def sens(self, out_node, max_out=4096):
+        """.sens of v(out_node) (sens.cpp:57-186): {element:parameter: value}."""
+        vals = np.zeros(max_out)
+        stride = 64
+        names = C.create_string_buffer(max_out * stride)
+        k = _chk(self.L.ref_sens(self.h, out_node, max_out, vals.ctypes.data_as(P_F64), names,
+                                 stride))
+        raw = names.raw
+        out = {}
+        for i in range(k):
+            nm = raw[i * stride:(i + 1) * stride].split(b"\0", 1)[0].decode()
+            out[nm] = float(vals[i])
+        return out
 
     def records(self):
         k = self.L.ref_tran_num_records(self.h)
