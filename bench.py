@@ -14,8 +14,11 @@ circuit.  value = sum over instances of Newton iterations / wall time.
 
 Also measured in the same run: the same kernels at batch 1 (the plain
 batch-1 GPU path), HIP-event times of the load, refactor and solve phases
-with their roofline, and the reference CPU path (compiled neospice,
-oracle/_ref) on the host cores on a bounded sample.
+with their roofline, the reference CPU path (compiled neospice,
+oracle/_ref) on the host cores on a bounded sample, and the second half of
+the metric, adjoint gradient time / forward time, on the transient-adjoint
+workload (32x32 diode-clamped RC mesh, 128 instances, 500 trapezoidal
+steps, checkpoints every 25 steps; see adjoint_c5).
 """
 import argparse
 import json
@@ -35,6 +38,20 @@ FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def measured_traffic():
+    """HBM bytes per batched refactor of this workload from the committed PMC
+    summary (FETCH_SIZE + WRITE_SIZE scaled by a plain copy's counters), or
+    None when it is absent."""
+    path = os.path.join(ROOT, "profiles", "k_refactor_pmc_rc_mesh_64x64_b256.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_refactor:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def lu_traffic(eng, ckt, B):
@@ -139,6 +156,44 @@ def cpu_baseline(ckt, info, R, dt, nsteps, budget_s, parity_steps, x_gpu0):
     return cpu, parity
 
 
+def adjoint_c5(na, W, mesh=32, batch=128, nsteps=500, dt=2e-12, ckpt=25, nprobe=64):
+    """Transient adjoint of the diode-clamped RC mesh (32x32, diode IS=1e-14
+    N=1 per node, 16-point PWL drive): 500 trapezoidal steps of 2 ps over
+    128 instances with seeded per-instance +-10% R and C perturbation; loss
+    = L2 error of 64 seeded probe waveforms against a seeded target;
+    gradient w.r.t. every R and C with recomputation checkpoints every 25
+    steps.  grad time includes the checkpoint recomputation."""
+    ckt, _ = W.diode_mesh(mesh)
+    ckt.finalize()
+    eng = na.BatchEngine(ckt, batch=batch)
+    R, C = eng.family_values("R"), eng.family_values("C")
+    for i, r in enumerate(W.instance_rngs(5, batch)):
+        R[:, i] *= 1 + 0.1 * r.standard_normal(R.shape[0])
+        C[:, i] *= 1 + 0.1 * r.standard_normal(C.shape[0])
+    eng.set_family_values("R", R)
+    eng.set_family_values("C", C)
+    g = np.random.default_rng(W.BASE_SEED + 5)
+    probes = np.sort(g.choice(ckt.num_nodes, nprobe, replace=False)).astype(np.int32)
+    target = g.uniform(0.0, 0.8, (nsteps + 1, nprobe, 1))
+    st0 = eng.stats()
+    t0 = time.perf_counter()
+    v = eng.tran_forward(nsteps, dt, probes, ckpt_every=ckpt)
+    t1 = time.perf_counter()
+    its = eng.stats()["gpu_refactor"] + eng.stats()["cpu_pivot"] - st0["gpu_refactor"] - st0["cpu_pivot"]
+    eng.tran_adjoint(2.0 * (v - target))
+    t2 = time.perf_counter()
+    fwd, grad = t1 - t0, t2 - t1
+    log("bench: adjoint forward %.3f s, gradient %.3f s, ratio %.3f" % (fwd, grad, grad / fwd))
+    return {"workload": "diode_mesh_%dx%d_trap_%dx%gps_b%d" % (mesh, mesh, nsteps, dt * 1e12, batch),
+            "unknowns": ckt.n, "nnz": ckt.nnz, "instances": batch, "steps": nsteps,
+            "probes": nprobe, "checkpoint_every": ckpt, "params": int(R.shape[0] + C.shape[0]),
+            "forward_ms": round(fwd * 1e3, 2), "grad_ms": round(grad * 1e3, 2),
+            "grad_over_forward": round(grad / fwd, 3),
+            "batched_factorizations_forward": int(its),
+            "note": "grad = checkpoint recomputation + backward sweep (one Jacobian load, refactor "
+                    "and transposed solve per step)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,6 +204,11 @@ def main():
     ap.add_argument("--batch1-steps", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain device copy of known size (for scaling the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--adjoint-steps", type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -156,6 +216,15 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.calibrate_copy:
+        n = 1 << 25  # 256 MiB of float64 read and written per copy
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "copies": 3}))
+        return
     import neospice_amd as na
     from neospice_amd import workloads as W
 
@@ -221,14 +290,17 @@ def main():
         "nnz_lu": st["nnz_lu"], "pivot_groups": st["pivot_groups"],
     }
     achieved = refactor_bytes / (ms_refactor * 1e-3) / 1e9
+    # rocprof PMC traffic applies to the default workload only
+    traffic = measured_traffic() if (args.mesh == 64 and B == 256) else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
                 "kernel": "k_refactor",
                 "launch_ms": round(ms_refactor, 4),
                 "bytes_per_launch": refactor_bytes,
                 "flops_per_launch": fl,
                 "fp64_TFLOPs": round(fl / (ms_refactor * 1e-3) / 1e12, 4),
-                "fp64_frac": round(fl / (ms_refactor * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 6)}
+                "fp64_frac": round(fl / (ms_refactor * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 6),
+                "traffic_GBs": (round(traffic / (ms_refactor * 1e-3) / 1e9, 1) if traffic else None)}
 
     cpu, parity = None, None
     if not args.no_cpu:
@@ -237,6 +309,10 @@ def main():
             log("bench: instance 0 vs reference CPU path, max |dx| =", parity)
         except Exception as e:  # the baseline is reported, not required
             log("bench: cpu baseline unavailable:", e)
+
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_c5(na, W, nsteps=args.adjoint_steps)
 
     out = {
         "metric": "batched Newton iters/s vs batch-1 GPU @1e-9 rel; adjoint grad/forward time",
@@ -260,6 +336,7 @@ def main():
         "phases": phases,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "adjoint": adjoint,
     }
     print(json.dumps(out))
 
