@@ -29,6 +29,8 @@ constexpr int kRfWaves = kRfThreads / kWave;
 constexpr int kSvThreads = 512;               // solve workgroup
 constexpr int kPf = 16;                       // U-entry steps whose operands are in flight
 constexpr int kContinued = 1 << 30;           // step descriptor: continuation chunk of an L column
+constexpr int kSweepMax = 4 * kWave;          // solve column sweep: up to 4 targets per lane
+constexpr int kSweepPf = 8;                   // sweep steps whose operands are in flight
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -539,28 +541,111 @@ struct SolveArgs {
 // level ahead: the row record and its first terms' values and sources
 // (thread mode: 8 terms; wave mode: 8 chunks of 64).
 struct RowPrefetch {
-    int4 ri;
+    int4 ri;     // record of this thread's (wave's) first row of the level
+    int4 rn;     // record of the level three ahead, in flight
     double v[8];
     int s[8];
 };
+
+// Column sweep over a program's densest end (DevProgram::sweep_*), one
+// wave.  Lane l holds the targets l + 64 m (sweep order) in registers; the
+// block of 64 targets the current steps finalize is always register 0 (the
+// registers rotate after each block).  Step s takes target s's value from
+// its lane, applies the diagonal operation, stores it, and subtracts its
+// contribution from every later target with a term from it, so each target
+// receives its terms in step order -- the order of its row in the reference
+// loop.  A row-gather level costs its longest chain of dependent
+// subtractions; a sweep step costs one, for all targets at once.
+struct SweepArgs {
+    int n, sv;
+    const int32_t* tgt;
+    const uint8_t* bits;
+};
+
+template <int kOp>
+__device__ __forceinline__ void run_sweep(const SweepArgs p, double* y, const double* SV) {
+    constexpr bool skip = (kOp & OP_SKIPZERO) != 0;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int D = p.n;
+    const double* V = SV + p.sv;
+    double a[4];
+    int tg[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        tg[m] = p.tgt[min(lane + m * kWave, D - 1)];
+        a[m] = y[tg[m]];
+    }
+    // Operands of step s (four values per lane, relative to the step's own
+    // block of 64 targets, and the lane's presence bits) live in ring slot
+    // s % kSweepPf; every load is a vector load issued kSweepPf steps ahead.
+    double ring[kSweepPf][4];
+    int bring[kSweepPf];
+    auto fetch = [&](int st, int slot) {
+        st = min(st, D - 1);
+        const int base = (st / kWave) * kWave + lane;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) ring[slot][m] = V[(size_t)st * D + min(base + m * kWave, D - 1)];
+        bring[slot] = p.bits[(size_t)st * kWave + lane];
+    };
+#pragma unroll
+    for (int q = 0; q < kSweepPf; ++q) fetch(q, q);
+    for (int mb = 0; mb * kWave < D; ++mb) {
+#pragma unroll
+        for (int tt = 0; tt < kWave; ++tt) {
+            const int st = mb * kWave + tt;
+            if (st >= D) break;
+            const int slot = tt % kSweepPf;
+            double v[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) v[m] = ring[slot][m];
+            const int bits = bring[slot] >> mb;
+            fetch(st + kSweepPf, slot);
+            double x = readlane_f64(a[0], tt);
+            if constexpr ((kOp & (OP_MUL_DIAG | OP_DIV_DIAG | OP_MUL_NZ)) != 0) {
+                const double d = readlane_f64(v[0], tt);   // the block diagonal
+                if constexpr ((kOp & OP_MUL_DIAG) != 0) x = x * d;
+                else if constexpr ((kOp & OP_DIV_DIAG) != 0) x = x / d;
+                else if (x != 0.0) x = x * d;
+            }
+            if (lane == tt) y[tg[0]] = x;
+            if (!(skip && x == 0.0)) {
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    if (m + mb < 4) {
+                        const double na = a[m] - v[m] * x;
+                        a[m] = ((bits >> m) & 1) ? na : a[m];
+                    }
+                }
+            }
+        }
+        a[0] = a[1]; a[1] = a[2]; a[2] = a[3];
+        tg[0] = tg[1]; tg[1] = tg[2]; tg[2] = tg[3];
+    }
+}
 
 // Runs one program level by level.  Each row's subtraction chain is
 // sequential in term order.  Thread mode: one thread per row, term loads
 // issued eight at a time.  Wave mode: one wave per row, the lanes form the
 // products of 64 terms at once and the chain then consumes them in order.
-// The next level's first-row operands are loaded while the current level
-// runs (they depend on nothing the current level writes).
+//
+// Prefetch: the operands of each thread's first row are loaded two levels
+// ahead and its row record three levels ahead (none of them depend on
+// values a level writes).  The three buffers rotate by name -- the level
+// loop is unrolled three times -- so no buffer is ever copied: a copy of a
+// register that a load is still filling would make the compiler wait for
+// every outstanding load at the end of each level.
 template <bool kSV>
-__device__ void run_program(const DevProgram& p, const int* lvl, const int* lmode, double* y,
-                            const double* LU, const double* SV) {
+__device__ __forceinline__ void run_program(const DevProgram& p, const int* lvl, const int* lmode,
+                                            double* y, const double* LU, const double* SV) {
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(tid / kWave);
     const bool skip = (p.op & OP_SKIPZERO) != 0;
     const bool has_diag = (p.op & (OP_MUL_DIAG | OP_DIV_DIAG | OP_MUL_NZ)) != 0;
+    const int nlev = p.nlevels;
     auto val = [&](int q) { return kSV ? SV[p.sv_base + q] : LU[p.pos[q]]; };
     auto finish = [&](double acc, int dix) {
-        if (has_diag) {
+        if (has_diag && dix >= 0) {   // dix < 0: a partial row the sweep finishes
             const double d = kSV ? SV[dix] : LU[dix];
             if (p.op & OP_MUL_DIAG) acc = acc * d;
             else if (p.op & OP_DIV_DIAG) acc = acc / d;
@@ -568,14 +653,15 @@ __device__ void run_program(const DevProgram& p, const int* lvl, const int* lmod
         }
         return acc;
     };
-    // Two-stage prefetch: the record of this thread's (wave's) first row
-    // two levels ahead, its term operands one level ahead.  Loads are
-    // clamped rather than skipped: the term arrays carry padding.
+    // Loads are clamped rather than skipped: the level table is read at a
+    // valid level and the term arrays carry padding.
     auto record = [&](int l) {
+        l = min(l, nlev - 1);
         const int r0 = lvl[l], r1 = lvl[l + 1];
         return p.rows[min((lmode[l] == 0 ? r0 + tid : r0 + w), r1 - 1)];
     };
     auto operands = [&](int l, RowPrefetch& f) {
+        l = min(l, nlev - 1);
         const bool tmode = lmode[l] == 0;
         const int last = max(f.ri.z - 1, 0);
 #pragma unroll
@@ -585,16 +671,7 @@ __device__ void run_program(const DevProgram& p, const int* lvl, const int* lmod
             f.s[j] = p.src[q];
         }
     };
-    if (p.nlevels == 0) return;
-    RowPrefetch cur, nxt;
-    int4 rec2 = make_int4(0, 0, 0, 0);
-    nxt.ri = rec2;
-    cur.ri = record(0);
-    operands(0, cur);
-    if (p.nlevels > 1) nxt.ri = record(1);
-    for (int l = 0; l < p.nlevels; ++l) {
-        if (l + 1 < p.nlevels) operands(l + 1, nxt);
-        if (l + 2 < p.nlevels) rec2 = record(l + 2);
+    auto level = [&](int l, const RowPrefetch& cur) {
         const int r0 = lvl[l], r1 = lvl[l + 1];
         if (lmode[l] == 0) {
             for (int idx = r0 + tid; idx < r1; idx += kSvThreads) {
@@ -656,8 +733,16 @@ __device__ void run_program(const DevProgram& p, const int* lvl, const int* lmod
                             const double ys = y[sx[j]];
                             // a skipped term subtracts +0.0, which leaves acc unchanged
                             const double prod = skip ? (ys != 0.0 ? ys * v[j] : 0.0) : v[j] * ys;
-                            const int m = min(kWave, ri.z - c);
-                            for (int k = 0; k < m; ++k) acc -= readlane_f64(prod, k);
+                            // the row record is wave-uniform: a scalar term count lets
+                            // a full chunk run as 64 straight-line subtractions with
+                            // immediate lane indices (same order as the tail loop)
+                            const int m = __builtin_amdgcn_readfirstlane(min(kWave, ri.z - c));
+                            if (m == kWave) {
+#pragma unroll
+                                for (int k = 0; k < kWave; ++k) acc -= readlane_f64(prod, k);
+                            } else {
+                                for (int k = 0; k < m; ++k) acc -= readlane_f64(prod, k);
+                            }
                         }
                     }
                 }
@@ -666,8 +751,31 @@ __device__ void run_program(const DevProgram& p, const int* lvl, const int* lmod
             }
         }
         __syncthreads();
-        cur = nxt;
-        nxt.ri = rec2;
+    };
+    // One level: start the operands of level l + 2 (their record arrived a
+    // level ago) and the record of level l + 3 (into the slot the current
+    // buffer no longer needs), then run level l.
+    auto body = [&](int l, RowPrefetch& cur, RowPrefetch& ahead2) {
+        ahead2.ri = ahead2.rn;
+        operands(l + 2, ahead2);
+        cur.rn = record(l + 3);
+        level(l, cur);
+    };
+    if (nlev > 0) {
+    RowPrefetch F0, F1, F2;
+    F0.ri = record(0);
+    operands(0, F0);
+    F1.ri = record(1);
+    operands(1, F1);
+    F2.rn = record(2);
+    int l = 0;
+    for (; l + 3 <= nlev; l += 3) {
+        body(l, F0, F2);
+        body(l + 1, F1, F0);
+        body(l + 2, F2, F1);
+    }
+    if (l < nlev) body(l, F0, F2);
+    if (l + 1 < nlev) body(l + 1, F1, F0);
     }
 }
 
@@ -692,6 +800,25 @@ __global__ __launch_bounds__(kSvThreads) void k_solve(SolveArgs a) {
     for (int l = threadIdx.x; l < a.p2.nlevels; l += kSvThreads) lm2[l] = a.p2.level_mode[l];
     __syncthreads();
     run_program<kSV>(a.p1, lv1, lm1, y, LU, SV);
+    if constexpr (kSV) {
+        // The forward program's sweep ends it, the backward program's starts
+        // it; both run here, one after the other, by the first wave.
+#pragma nounroll
+        for (int k = 0; k < 2; ++k) {
+            const bool on = k == 0 ? (a.p1.sweep_n > 0 && !a.p1.sweep_first)
+                                   : (a.p2.sweep_n > 0 && a.p2.sweep_first);
+            if (!on) continue;
+            const SweepArgs sw = k == 0 ? SweepArgs{a.p1.sweep_n, a.p1.sweep_sv, a.p1.sweep_tgt, a.p1.sweep_bits}
+                                        : SweepArgs{a.p2.sweep_n, a.p2.sweep_sv, a.p2.sweep_tgt, a.p2.sweep_bits};
+            const int op = k == 0 ? a.p1.op : a.p2.op;
+            if (threadIdx.x < kWave) {
+                if (op == OP_PLAIN) run_sweep<OP_PLAIN>(sw, y, SV);
+                else if (op == OP_DIV_DIAG) run_sweep<OP_DIV_DIAG>(sw, y, SV);
+                else run_sweep<OP_SKIPZERO | OP_MUL_NZ>(sw, y, SV);
+            }
+            __syncthreads();
+        }
+    }
     run_program<kSV>(a.p2, lv2, lm2, y, LU, SV);
     for (int k = threadIdx.x; k < a.n; k += kSvThreads) a.b[(size_t)a.out_perm[k] * a.Bg + ginst] = y[k];
 }
@@ -947,7 +1074,7 @@ GpuLu::GpuLu(const LuStructure& s, int batch, hipStream_t stream,
     d_tdiag_cand_ = upload(allocs_, tdiag);
     // ---- solve programs and the solve-ordered value copy of the plain solve
     TriProgram f, b, tf, tb;
-    build_solve_programs(s_, f, b, tf, tb);
+    build_solve_programs(s_, f, b, tf, tb, kSweepMax);
     std::vector<int32_t> sv_pos;
     upload_program(f, fwd_, &sv_pos);
     upload_program(b, bwd_, &sv_pos);
@@ -1004,7 +1131,8 @@ void GpuLu::upload_program(const TriProgram& t, DevProgram& d, std::vector<int32
                 src.push_back((int16_t)t.term_src[q]);
                 pos.push_back(t.term_pos[q]);
             }
-            rows[idx] = make_int4(tg, first, cnt, has_diag ? t.dpos[tg] : -1);
+            const bool part = !t.partial.empty() && t.partial[tg];
+            rows[idx] = make_int4(tg, first, cnt, (has_diag && !part) ? t.dpos[tg] : -1);
         }
         // latency model (cycles): a round of dependent loads ~2000 (8 terms per
         // thread, 8 x 64 per wave), a chained subtraction ~10 (thread) or ~12
@@ -1020,10 +1148,36 @@ void GpuLu::upload_program(const TriProgram& t, DevProgram& d, std::vector<int32
         if (has_diag) {
             for (size_t idx = 0; idx < t.order.size(); ++idx) {
                 const int tg = rows[idx].x;
+                if (rows[idx].w < 0) continue;   // partial target: the sweep finishes it
                 if (t.dpos[tg] < 0) throw Error(NSA_ERR_LOGIC, "GpuLu: solve program without diagonal");
                 rows[idx].w = (int)sv_pos->size();
                 sv_pos->push_back(t.dpos[tg]);
             }
+        }
+        if (t.sweep_n > 0) {
+            // D x D value block: row st holds step st's terms on later targets
+            // and, on the diagonal, target st's own diagonal value (absent
+            // terms read position 0 and are masked out)
+            const int D = t.sweep_n;
+            if (t.op != OP_PLAIN && t.op != OP_DIV_DIAG && t.op != (OP_SKIPZERO | OP_MUL_NZ))
+                throw Error(NSA_ERR_LOGIC, "GpuLu: no sweep kernel for this solve operation");
+            d.sweep_n = D;
+            d.sweep_first = t.sweep_first ? 1 : 0;
+            d.sweep_sv = (int)sv_pos->size();
+            std::vector<uint8_t> bits((size_t)D * kWave, 0);
+            for (int st = 0; st < D; ++st) {
+                const int tg = t.sweep_tgt[st];
+                if (has_diag && t.dpos[tg] < 0)
+                    throw Error(NSA_ERR_LOGIC, "GpuLu: swept target without diagonal");
+                for (int r = 0; r < D; ++r) {
+                    int q = t.sweep_pos[(size_t)st * D + r];
+                    if (q >= 0) bits[(size_t)st * kWave + r % kWave] |= (uint8_t)(1u << (r / kWave));
+                    if (r == st) q = has_diag ? t.dpos[tg] : 0;
+                    sv_pos->push_back(q >= 0 ? q : 0);
+                }
+            }
+            d.sweep_tgt = upload(allocs_, t.sweep_tgt);
+            d.sweep_bits = upload(allocs_, bits);
         }
     }
     if (s_.n > 32767) throw Error(NSA_ERR_UNSUPPORTED, "GpuLu: solve sources exceed int16");

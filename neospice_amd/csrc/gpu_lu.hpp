@@ -46,6 +46,16 @@ struct DevProgram {
     int4* rows = nullptr;            // (target, first term, term count, diagonal value index)
     int16_t* src = nullptr;          // per term: source index
     int32_t* pos = nullptr;          // per term: LU position (programs that read LU directly)
+    // column sweep (TriProgram::sweep_*): one wave, before (descending) or
+    // after (ascending) the levels; values in a sweep_n x sweep_n block of the
+    // solve-ordered copy at sweep_sv (row = step, column = target; the step's
+    // own diagonal on the block diagonal), presence bits per step and lane
+    // (bit m: target lane + 64 m)
+    int32_t sweep_n = 0;
+    int32_t sweep_first = 0;
+    int32_t sweep_sv = 0;
+    int32_t* sweep_tgt = nullptr;
+    uint8_t* sweep_bits = nullptr;
 };
 
 class GpuLu {

@@ -181,12 +181,17 @@ RefactorSchedule build_refactor_schedule(const LuStructure& s) {
 
 namespace {
 
+struct Term { int32_t key, pos, src; };
+
 // Levels of a row-gather program: target t waits for every source.
-void level_program(TriProgram& pr, int n, bool ascending) {
-    std::vector<int32_t> lev(n, 0);
+// Targets with skip[t] are not in the program; as sources they are ready
+// before it starts (a descending program's sweep runs first).
+void level_program(TriProgram& pr, int n, bool ascending, const std::vector<uint8_t>* skip) {
+    std::vector<int32_t> lev(n, -1);
     int32_t maxlev = -1;
     for (int it = 0; it < n; ++it) {
         const int t = ascending ? it : n - 1 - it;
+        if (skip && (*skip)[t]) continue;
         int32_t l = 0;
         for (int p = pr.term_ptr[t]; p < pr.term_ptr[t + 1]; ++p)
             l = std::max(l, lev[pr.term_src[p]] + 1);
@@ -195,14 +200,53 @@ void level_program(TriProgram& pr, int n, bool ascending) {
     }
     const int nl = maxlev + 1;
     pr.level_ptr.assign(nl + 1, 0);
-    for (int t = 0; t < n; ++t) pr.level_ptr[lev[t] + 1]++;
+    for (int t = 0; t < n; ++t)
+        if (lev[t] >= 0) pr.level_ptr[lev[t] + 1]++;
     for (int l = 0; l < nl; ++l) pr.level_ptr[l + 1] += pr.level_ptr[l];
-    pr.order.resize(n);
+    pr.order.resize(pr.level_ptr[nl]);
     std::vector<int32_t> fill(pr.level_ptr.begin(), pr.level_ptr.end() - 1);
-    for (int t = 0; t < n; ++t) pr.order[fill[lev[t]]++] = t;
+    for (int t = 0; t < n; ++t)
+        if (lev[t] >= 0) pr.order[fill[lev[t]]++] = t;
 }
 
-struct Term { int32_t key, pos, src; };
+// Plans the column sweep over the last (ascending) or first (descending) D
+// targets.  Requires every target's terms to be in processing order, which
+// the callers guarantee by their sort; builds the step x target position
+// table and cuts the swept terms out of the target lists.
+void plan_sweep(TriProgram& pr, std::vector<std::vector<Term>>& terms, int n, bool ascending,
+                int sweep_max, std::vector<uint8_t>& skip) {
+    const int D = std::min(sweep_max, n);
+    skip.assign(n, 0);
+    if (D <= 0) return;
+    pr.sweep_n = D;
+    pr.sweep_first = !ascending;
+    pr.sweep_tgt.resize(D);
+    std::vector<int32_t> rel(n, -1);
+    for (int r = 0; r < D; ++r) {
+        const int t = ascending ? n - D + r : n - 1 - r;
+        pr.sweep_tgt[r] = t;
+        rel[t] = r;
+    }
+    pr.sweep_pos.assign((size_t)D * D, -1);
+    pr.partial.assign(n, 0);
+    for (int r = 0; r < D; ++r) {
+        const int t = pr.sweep_tgt[r];
+        std::vector<Term> keep;
+        for (const Term& tm : terms[t]) {
+            const int sidx = rel[tm.src];
+            if (sidx < 0) {
+                keep.push_back(tm);
+            } else {
+                if (sidx >= r) throw Error(NSA_ERR_LOGIC, "plan_sweep: term from a later step");
+                pr.sweep_pos[(size_t)sidx * D + r] = tm.pos;
+            }
+        }
+        // in processing order every pre-segment term precedes the swept ones
+        terms[t] = keep;
+        if (ascending && !keep.empty()) pr.partial[t] = 1;
+        else skip[t] = 1;
+    }
+}
 
 void pack_terms(TriProgram& pr, int n, std::vector<std::vector<Term>>& terms) {
     pr.term_ptr.assign(n + 1, 0);
@@ -219,7 +263,7 @@ void pack_terms(TriProgram& pr, int n, std::vector<std::vector<Term>>& terms) {
 }  // namespace
 
 void build_solve_programs(const LuStructure& s, TriProgram& fwd, TriProgram& bwd,
-                          TriProgram& tfwd, TriProgram& tbwd) {
+                          TriProgram& tfwd, TriProgram& tbwd, int sweep_max) {
     const int n = s.n;
     // Entries by kind, each (row, col, pos) in pivot space.
     std::vector<std::vector<Term>> lrow(n), urow(n), lcol(n), ucol(n);
@@ -275,15 +319,22 @@ void build_solve_programs(const LuStructure& s, TriProgram& fwd, TriProgram& bwd
         for (int t = 0; t < n; ++t) ttb[t] = lcol[t];
         tbwd.op = OP_PLAIN;
     }
+    // Sweeps: the forward programs of both forms gather ascending sources in
+    // an ascending program; the AMD-LU backward gathers descending sources in
+    // a descending one.  Sparse 1.3's backward row dot runs ascending inside
+    // a descending program, so it keeps its levels.
+    std::vector<uint8_t> fskip, bskip;
+    plan_sweep(fwd, tf, n, true, sweep_max, fskip);
+    if (s.form != FORM_S13) plan_sweep(bwd, tb, n, false, sweep_max, bskip);
     pack_terms(fwd, n, tf);
     pack_terms(bwd, n, tb);
     pack_terms(tfwd, n, ttf);
     pack_terms(tbwd, n, ttb);
     fwd.dpos = dp; bwd.dpos = dp; tfwd.dpos = dp; tbwd.dpos = dp;
-    level_program(fwd, n, true);
-    level_program(bwd, n, false);
-    level_program(tfwd, n, true);
-    level_program(tbwd, n, false);
+    level_program(fwd, n, true, fskip.empty() ? nullptr : &fskip);
+    level_program(bwd, n, false, bskip.empty() ? nullptr : &bskip);
+    level_program(tfwd, n, true, nullptr);
+    level_program(tbwd, n, false, nullptr);
 }
 
 }  // namespace nsa

@@ -88,6 +88,17 @@ struct TriProgram {
     std::vector<int32_t> term_pos;   // LU positions
     std::vector<int32_t> term_src;   // source index (pivot space)
     std::vector<int32_t> dpos;       // n, diagonal position per target (or -1)
+    // Column sweep over the densest end of the program (0 = none): the last
+    // sweep_n targets of an ascending program or the first sweep_n of a
+    // descending one, in processing order (sweep_tgt).  sweep_pos[s * n_s + r]
+    // is the LU position of the term that step s (finalizing sweep_tgt[s])
+    // applies to sweep_tgt[r], or -1.  Ascending: those targets keep only
+    // their terms from before the segment in the level program, without the
+    // diagonal operation (partial[t] = 1); descending: they leave it.
+    int32_t sweep_n = 0;
+    bool sweep_first = false;
+    std::vector<int32_t> sweep_tgt, sweep_pos;
+    std::vector<uint8_t> partial;
     int32_t nlevels() const { return (int32_t)level_ptr.size() - 1; }
 };
 
@@ -101,7 +112,11 @@ struct RefactorSchedule {
 };
 
 RefactorSchedule build_refactor_schedule(const LuStructure& s);
+// sweep_max > 0 plans a column sweep of up to sweep_max targets for the
+// plain-solve programs whose per-target term order is the processing order
+// (each target's chain then stays the reference's when it is applied step
+// by step); the transposed programs never sweep.
 void build_solve_programs(const LuStructure& s, TriProgram& fwd, TriProgram& bwd,
-                          TriProgram& tfwd, TriProgram& tbwd);
+                          TriProgram& tfwd, TriProgram& tbwd, int sweep_max = 0);
 
 }  // namespace nsa
