@@ -29,6 +29,7 @@ constexpr int kRfWaves = kRfThreads / kWave;
 constexpr int kSvThreads = 512;               // solve workgroup
 constexpr int kPf = 16;                       // U-entry steps whose operands are in flight
 constexpr int kContinued = 1 << 30;           // step descriptor: continuation chunk of an L column
+constexpr int kSpinMax = 1 << 24;              // bounded wait for a column (about 0.5 s)
 constexpr int kSweepMax = 4 * kWave;          // solve column sweep: up to 4 targets per lane
 constexpr int kSweepPf = 8;                   // sweep steps whose operands are in flight
 __device__ __forceinline__ void wave_sync() {
@@ -100,12 +101,33 @@ struct RfArgs {
     int32_t *fail, *fail_col;
 };
 
+// Completion bits of the columns (LDS, one bit per column).  A column's
+// factor values are published by a workgroup-scope release fence followed
+// by setting its bit; a reader that sees the bit set issues an acquire
+// fence before loading them.
+__device__ __forceinline__ bool col_done(uint32_t* done, int r) {
+    return (__hip_atomic_load(&done[r >> 5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >> (r & 31)) & 1u;
+}
+
+__device__ __forceinline__ void col_publish(uint32_t* done, int k) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if ((threadIdx.x & (kWave - 1)) == 0)
+        __hip_atomic_fetch_or(&done[k >> 5], 1u << (k & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // One column by one wave.  partial: a trailing-block column, only the
 // updates from pivots before the block; its values stay raw for the dense
 // phase and its running |U| maximum goes to cmx.
+//
+// Columns run as soon as a wave takes them, not level by level: the step
+// of pivot r reads L(:, r) only once column r's bit is set.  A scan at the
+// start finds the first step whose pivot column is not finished yet; the
+// operands of that step and of every later one are fetched after a wait
+// for the pivot's bit (columns are taken in a topological order, so the
+// wait always ends; a bounded spin flags FAIL_TIMEOUT if it did not).
 __device__ __forceinline__ void rf_column(const RfArgs& a, const int4 m0, const int4 m1, bool partial,
                                           double* ws, int4* wdesc, double* LU, int ginst, int kind,
-                                          double* cmx) {
+                                          double* cmx, uint32_t* done) {
     const int lane = threadIdx.x & (kWave - 1);
     const bool s13 = a.form == FORM_S13;
     const int k = m0.x, cb = m0.y, len = m0.z;
@@ -127,6 +149,27 @@ __device__ __forceinline__ void rf_column(const RfArgs& a, const int4 m0, const 
     // descriptor of step u + kPf is read together with step u's operands.
     for (int u = lane; u < nd; u += kWave) wdesc[u] = a.desc[d0 + u];
     wave_sync();
+    int pend = nd;   // first step whose pivot column was not finished at the scan
+    for (int u0 = 0; u0 < nd; u0 += kWave) {
+        const bool open = u0 + lane < nd && !col_done(done, wdesc[u0 + lane].z);
+        const unsigned long long b = __ballot(open);
+        if (b) {
+            pend = u0 + __ffsll(b) - 1;
+            break;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    auto await = [&](int r) {
+        int spins = 0;
+        while (!col_done(done, r)) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > kSpinMax) {
+                if (lane == 0) flag(a.fail, a.fail_col, ginst, FAIL_TIMEOUT, k);
+                break;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    };
     // Ring of prefetched steps.  Per step: U slot and chunk length packed
     // (wave-uniform) and r's stored diagonal (Sparse 1.3); per lane: its
     // relative slot and its L value.  Every load is unconditional (indices
@@ -137,7 +180,10 @@ __device__ __forceinline__ void rf_column(const RfArgs& a, const int4 m0, const 
     int usnl[kPf];
     int16_t qr[kPf];
     double lv[kPf], dg[kPf];
-    auto fetch = [&](const int4 d, int p) {
+    auto fetch = [&](int u, int p) {
+        u = min(u, nd - 1);
+        const int4 d = wdesc[u];
+        if (u >= pend) await(d.z);
         usnl[p] = d.x;
         const int nl = (d.x >> 16) & 0x7f;
         const int e = min(lane, nl > 0 ? nl - 1 : 0);
@@ -160,14 +206,13 @@ __device__ __forceinline__ void rf_column(const RfArgs& a, const int4 m0, const 
     };
     if (nd > 0) {
 #pragma unroll
-        for (int p = 0; p < kPf; ++p) fetch(wdesc[min(p, nd - 1)], p);
+        for (int p = 0; p < kPf; ++p) fetch(p, p);
         const int nfull = nd / kPf;
         for (int r = 0; r < nfull; ++r) {
 #pragma unroll
             for (int p = 0; p < kPf; ++p) {
-                const int4 dn = wdesc[min(r * kPf + p + kPf, nd - 1)];
                 step(p);
-                fetch(dn, p);
+                fetch(r * kPf + p + kPf, p);
             }
         }
 #pragma unroll
@@ -458,8 +503,11 @@ __global__ __launch_bounds__(kRfThreads) void k_refactor(RfArgs a) {
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     double* LU = a.LU + (size_t)inst * a.nlu;
     double* cmx = reinterpret_cast<double*>(smem + a.union_bytes);
-    // after the union, the |U| maxima and the trailing masks: the level table
-    int* lvl = reinterpret_cast<int*>(smem + a.union_bytes + (size_t)a.t * 56);
+    // after the union, the |U| maxima and the trailing masks: the column
+    // completion bits, then the counter that deals out the columns
+    uint32_t* done = reinterpret_cast<uint32_t*>(smem + a.union_bytes + (size_t)a.t * 56);
+    const int nwords = (a.n + 31) / 32;
+    int* next_col = reinterpret_cast<int*>(done + nwords);
     double* ws = reinterpret_cast<double*>(smem + (size_t)w * a.wregion);
     int4* wdesc = reinterpret_cast<int4*>(smem + (size_t)w * a.wregion + (size_t)a.wsz * 8);
     // Load phase (NeoSolver::load_real + add_diag_gmin; GP scatter + gmin):
@@ -476,40 +524,40 @@ __global__ __launch_bounds__(kRfThreads) void k_refactor(RfArgs a) {
         for (int u = 0; u < 8; ++u)
             if (p0 + u * kRfThreads < a.nlu) LU[p0 + u * kRfThreads] = src[u] >= 0 ? v[u] : 0.0;
     }
-    for (int l = threadIdx.x; l <= a.nlev; l += kRfThreads) lvl[l] = a.lev_ptr[l];
+    for (int i = threadIdx.x; i < nwords; i += kRfThreads) done[i] = 0u;
+    if (threadIdx.x == 0) *next_col = 0;
     __syncthreads();
     if (a.gmin != 0.0) {
         for (int k = threadIdx.x; k < a.n; k += kRfThreads)
             if (a.gmin_pos[k] >= 0) LU[a.gmin_pos[k]] += a.gmin;
         __syncthreads();
     }
-    // Sparse levels then the trailing columns' partial pass (level nlev).
-    // Column positions: level order for the sparse columns, then the
-    // trailing columns; each wave loads the metadata of its next column
-    // ahead.
-    int4 m0 = make_int4(0, 0, 0, 0), m1 = m0;
-    int pc = -1;   // column position whose metadata m0, m1 hold
+    // Sparse columns, then the trailing columns' partial pass.  Column
+    // positions: the sparse columns in level order, then the trailing ones
+    // (a topological order); each wave takes the next position from a
+    // counter and loads its metadata while it runs the current one.
     auto meta = [&](int c, int4& x0, int4& x1) { x0 = a.cmeta[2 * c]; x1 = a.cmeta[2 * c + 1]; };
-    for (int l = 0; l <= a.nlev; ++l) {
-        const bool part = l == a.nlev;
-        const int cbeg = part ? a.T0 : lvl[l], cend = part ? a.n : lvl[l + 1];
-        if (w < a.nws) {
-            for (int c = cbeg + w; c < cend; c += a.nws) {
-                if (pc != c) meta(c, m0, m1);
-                // the wave's next column: in this level, else its first one
-                // of the next level (or of the partial pass)
-                int cn = c + a.nws;
-                if (cn >= cend) cn = (l + 1 < a.nlev) ? lvl[l + 1] + w : (l + 1 == a.nlev ? a.T0 + w : a.n);
-                int4 n0 = m0, n1 = m1;
-                if (cn < a.n) meta(cn, n0, n1);
-                rf_column(a, m0, m1, part, ws, wdesc, LU, ginst, kind, cmx);
-                m0 = n0;
-                m1 = n1;
-                pc = cn < a.n ? cn : -1;
-            }
+    auto take = [&]() {
+        int c = 0;
+        if ((threadIdx.x & (kWave - 1)) == 0) c = atomicAdd(next_col, 1);
+        return __builtin_amdgcn_readfirstlane(c);
+    };
+    if (w < a.nws) {
+        int c = take();
+        int4 m0 = make_int4(0, 0, 0, 0), m1 = m0;
+        if (c < a.n) meta(c, m0, m1);
+        while (c < a.n) {
+            const int cn = take();
+            int4 n0 = m0, n1 = m1;
+            if (cn < a.n) meta(cn, n0, n1);
+            rf_column(a, m0, m1, c >= a.T0, ws, wdesc, LU, ginst, kind, cmx, done);
+            col_publish(done, m0.x);
+            c = cn;
+            m0 = n0;
+            m1 = n1;
         }
-        __syncthreads();
     }
+    __syncthreads();
     if (a.t > 0) {
         // products L(i, r) * U(r, j) of the deferred updates, all final now
         double* dprod = a.dprod + (size_t)inst * a.ndterm;
@@ -1020,8 +1068,9 @@ GpuLu::GpuLu(const LuStructure& s, int batch, hipStream_t stream,
     // after the union: |U| maxima of the trailing columns (t doubles), then
     // the trailing structure masks (12 words per step)
     const size_t cmx_bytes = (size_t)std::max(t, 1) * 8 + (size_t)std::max(t, 1) * 48;
-    const size_t flag_bytes = (size_t)(nlev_ + 1) * 4;   // the level table
-    if (cmx_bytes + flag_bytes >= lds_max) throw Error(NSA_ERR_UNSUPPORTED, "GpuLu: level table exceeds LDS");
+    // the column completion bits and the dealing counter
+    const size_t flag_bytes = align16((size_t)((n + 31) / 32) * 4 + 4);
+    if (cmx_bytes + flag_bytes >= lds_max) throw Error(NSA_ERR_UNSUPPORTED, "GpuLu: completion bits exceed LDS");
     nws_ = (int)std::min<size_t>(kRfWaves, (lds_max - cmx_bytes - flag_bytes) / (size_t)wregion_);
     if (nws_ < 1) throw Error(NSA_ERR_UNSUPPORTED, "GpuLu: column workspace exceeds LDS");
     union_bytes_ = align16(std::max((size_t)nws_ * wregion_, (size_t)(516 + 128) * 8));
