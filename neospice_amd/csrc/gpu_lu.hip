@@ -289,6 +289,8 @@ __device__ __forceinline__ void rf_column(const RfArgs& a, const int4 m0, const 
     }
 }
 
+__host__ __device__ constexpr size_t align16_d(size_t x) { return (x + 15) & ~(size_t)15; }
+
 __device__ __forceinline__ bool bit128(const uint32_t* m, int i) { return (m[i >> 5] >> (i & 31)) & 1u; }
 
 __device__ __forceinline__ int wave_min_i(int v) {
@@ -326,11 +328,14 @@ __device__ __forceinline__ void rf_tail(const RfArgs& a, double* lds, double* LU
         R[q >> 2][q & 3] = pos >= 0 ? LU[pos] : 0.0;
     }
     __syncthreads();
+    int dt_next = a.dstep_ptr[0];   // deferred-task table of the next step, loaded a step ahead
     for (int s = 0; s < t; ++s) {
         const int buf = s & 1;
         const int ts = s & 31, bs = s >> 5;    // column s: owners tj == ts, register column bs
         const uint32_t* lm = msk + 12 * s;     // L rows of column s
         const uint32_t* um = lm + 4;           // U columns of row s
+        const int dt = dt_next;
+        dt_next = a.dstep_ptr[min(s + 1, t - 1)];
         if (wv == (ts >> 1)) {
             // the wave holding column s (its half tj == ts owns it)
             const bool own = tj == ts;
@@ -371,14 +376,21 @@ __device__ __forceinline__ void rf_tail(const RfArgs& a, double* lds, double* LU
                         if (a.tcand[c0 + ipiv] != s) code = FAIL_REORDER;
                     }
                 }
-                // L(i, s) = x_i / U(s, s)
+                // L(i, s) = x_i / U(s, s); the four quotients are formed
+                // unconditionally (independent, so they overlap) and kept
+                // where the pattern has the entry
                 if (own) {
+                    const uint4 lw = *reinterpret_cast<const uint4*>(lm);
+                    const uint32_t lwa[4] = {lw.x, lw.y, lw.z, lw.w};
+                    double dv[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) dv[q] = cv[q] / piv;
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const int i = ti + 32 * q;
-                        if (i > s && i < t && bit128(lm, i)) {
-                            cv[q] = cv[q] / piv;
-                            Lcol[buf * 128 + i] = cv[q];
+                        if (i > s && i < t && ((lwa[q] >> ti) & 1u)) {
+                            cv[q] = dv[q];
+                            Lcol[buf * 128 + i] = dv[q];
                         }
                     }
                 }
@@ -420,14 +432,20 @@ __device__ __forceinline__ void rf_tail(const RfArgs& a, double* lds, double* LU
         }
         if (ti == ts) {
             // owners of row s: publish U(s, j) (raw) and fold it into the
-            // columns' running |U| maxima
+            // columns' running |U| maxima (the maxima are read first, all
+            // four at once; each owner has its own columns j)
+            const uint4 uw = *reinterpret_cast<const uint4*>(um);
+            const uint32_t uwa[4] = {uw.x, uw.y, uw.z, uw.w};
+            double cm[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) cm[b] = s13 ? 0.0 : cmx[min(tj + 32 * b, t - 1)];
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const int j = tj + 32 * b;
                 const double raw = bs == 0 ? R[0][b] : bs == 1 ? R[1][b] : bs == 2 ? R[2][b] : R[3][b];
-                if (j > s && j < t && bit128(um, j)) {
+                if (j > s && j < t && ((uwa[b] >> tj) & 1u)) {
                     Urow[buf * 128 + j] = raw;
-                    if (!s13) cmx[j] = fmax(cmx[j], fabs(raw));
+                    if (!s13) cmx[j] = fmax(cm[b], fabs(raw));
                 }
             }
         }
@@ -435,13 +453,17 @@ __device__ __forceinline__ void rf_tail(const RfArgs& a, double* lds, double* LU
         const double pv = s13 ? shpv[buf] : 1.0;
         double lv[4], uv[4];
         bool lok[4], uok[4];
+        const uint4 lw = *reinterpret_cast<const uint4*>(lm);
+        const uint4 uw = *reinterpret_cast<const uint4*>(um);
+        const uint32_t lwa[4] = {lw.x, lw.y, lw.z, lw.w}, uwa[4] = {uw.x, uw.y, uw.z, uw.w};
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int i = ti + 32 * q, j = tj + 32 * q;
-            lok[q] = i > s && i < t && bit128(lm, i);
-            uok[q] = j > s && j < t && bit128(um, j);
-            lv[q] = lok[q] ? Lcol[buf * 128 + i] : 0.0;
-            const double ur = uok[q] ? Urow[buf * 128 + j] : 0.0;
+            lok[q] = i > s && i < t && ((lwa[q] >> ti) & 1u);
+            uok[q] = j > s && j < t && ((uwa[q] >> tj) & 1u);
+            // unused operands (absent entries) are read but never applied
+            lv[q] = Lcol[buf * 128 + i];
+            const double ur = Urow[buf * 128 + j];
             // U(s, j) = x_s(j) * (1 / pivot) for Sparse 1.3, x_s(j) for AMD-LU
             uv[q] = s13 ? ur * pv : ur;
         }
@@ -463,7 +485,6 @@ __device__ __forceinline__ void rf_tail(const RfArgs& a, double* lds, double* LU
         // Updates from earlier (sparse) pivots that the reference's reach
         // order places after pivot s in a column, by the target's owner, in
         // that order; the products were formed before the dense phase.
-        const int dt = a.dstep_ptr[s];
         if (dt >= 0) {
             const int k0 = a.dtask_ptr[dt * (kRfThreads + 1) + tid];
             const int k1 = a.dtask_ptr[dt * (kRfThreads + 1) + tid + 1];
@@ -505,7 +526,7 @@ __global__ __launch_bounds__(kRfThreads) void k_refactor(RfArgs a) {
     double* cmx = reinterpret_cast<double*>(smem + a.union_bytes);
     // after the union, the |U| maxima and the trailing masks: the column
     // completion bits, then the counter that deals out the columns
-    uint32_t* done = reinterpret_cast<uint32_t*>(smem + a.union_bytes + (size_t)a.t * 56);
+    uint32_t* done = reinterpret_cast<uint32_t*>(smem + a.union_bytes + align16_d((size_t)a.t * 8) + (size_t)a.t * 48);
     const int nwords = (a.n + 31) / 32;
     int* next_col = reinterpret_cast<int*>(done + nwords);
     double* ws = reinterpret_cast<double*>(smem + (size_t)w * a.wregion);
@@ -561,15 +582,35 @@ __global__ __launch_bounds__(kRfThreads) void k_refactor(RfArgs a) {
     if (a.t > 0) {
         // products L(i, r) * U(r, j) of the deferred updates, all final now
         double* dprod = a.dprod + (size_t)inst * a.ndterm;
-        for (int e = threadIdx.x; e < a.ndterm; e += kRfThreads)
-            dprod[e] = LU[a.dterm[2 * e]] * LU[a.dterm[2 * e + 1]];
+        for (int e0 = threadIdx.x; e0 < a.ndterm; e0 += 4 * kRfThreads) {
+            int2 q[4];
+            double l[4], u[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) q[j] = reinterpret_cast<const int2*>(a.dterm)[min(e0 + j * kRfThreads, a.ndterm - 1)];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) { l[j] = LU[q[j].x]; u[j] = LU[q[j].y]; }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (e0 + j * kRfThreads < a.ndterm) dprod[e0 + j * kRfThreads] = l[j] * u[j];
+        }
         __syncthreads();
         rf_tail(a, reinterpret_cast<double*>(smem), LU, ginst, kind, cmx,
-                reinterpret_cast<uint32_t*>(smem + a.union_bytes + (size_t)a.t * 8), dprod);
+                reinterpret_cast<uint32_t*>(smem + a.union_bytes + align16_d((size_t)a.t * 8)), dprod);
         __syncthreads();
     }
     double* SV = a.SV + (size_t)inst * a.nsv;
-    for (int i = threadIdx.x; i < a.nsv; i += kRfThreads) SV[i] = LU[a.sv_pos[i]];
+    // solve-ordered copy, eight gathers in flight per thread
+    for (int i0 = threadIdx.x; i0 < a.nsv; i0 += 8 * kRfThreads) {
+        int q[8];
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) q[u] = a.sv_pos[min(i0 + u * kRfThreads, a.nsv - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = LU[q[u]];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (i0 + u * kRfThreads < a.nsv) SV[i0 + u * kRfThreads] = v[u];
+    }
 }
 
 struct SolveArgs {
@@ -1065,9 +1106,10 @@ GpuLu::GpuLu(const LuStructure& s, int batch, hipStream_t stream,
     // LDS: per-wave regions (column workspace + staged descriptors),
     // overlaid by the trailing block, then the trailing columns' |U| maxima.
     wregion_ = (int)align16((size_t)wsz_ * 8 + (size_t)maxnd * 16);
-    // after the union: |U| maxima of the trailing columns (t doubles), then
-    // the trailing structure masks (12 words per step)
-    const size_t cmx_bytes = (size_t)std::max(t, 1) * 8 + (size_t)std::max(t, 1) * 48;
+    // after the union: |U| maxima of the trailing columns (t doubles, padded
+    // to 16 bytes), then the trailing structure masks (12 words per step,
+    // read 4 at a time)
+    const size_t cmx_bytes = align16((size_t)std::max(t, 1) * 8) + (size_t)std::max(t, 1) * 48;
     // the column completion bits and the dealing counter
     const size_t flag_bytes = align16((size_t)((n + 31) / 32) * 4 + 4);
     if (cmx_bytes + flag_bytes >= lds_max) throw Error(NSA_ERR_UNSUPPORTED, "GpuLu: completion bits exceed LDS");
