@@ -120,11 +120,9 @@ __device__ __forceinline__ void col_publish(uint32_t* done, int k) {
 // phase and its running |U| maximum goes to cmx.
 //
 // Columns run as soon as a wave takes them, not level by level: the step
-// of pivot r reads L(:, r) only once column r's bit is set.  A scan at the
-// start finds the first step whose pivot column is not finished yet; the
-// operands of that step and of every later one are fetched after a wait
-// for the pivot's bit (columns are taken in a topological order, so the
-// wait always ends; a bounded spin flags FAIL_TIMEOUT if it did not).
+// of pivot r reads L(:, r) only once column r's bit is set (columns are
+// taken in a topological order, so every wait ends; a bounded spin flags
+// FAIL_TIMEOUT if one did not).
 __device__ __forceinline__ void rf_column(const RfArgs& a, const int4 m0, const int4 m1, bool partial,
                                           double* ws, int4* wdesc, double* LU, int ginst, int kind,
                                           double* cmx, uint32_t* done) {
@@ -149,16 +147,15 @@ __device__ __forceinline__ void rf_column(const RfArgs& a, const int4 m0, const 
     // descriptor of step u + kPf is read together with step u's operands.
     for (int u = lane; u < nd; u += kWave) wdesc[u] = a.desc[d0 + u];
     wave_sync();
-    int pend = nd;   // first step whose pivot column was not finished at the scan
-    for (int u0 = 0; u0 < nd; u0 += kWave) {
-        const bool open = u0 + lane < nd && !col_done(done, wdesc[u0 + lane].z);
-        const unsigned long long b = __ballot(open);
-        if (b) {
-            pend = u0 + __ffsll(b) - 1;
-            break;
+    // first step at or after u whose pivot column is not finished (nd: none)
+    auto first_open = [&](int u) {
+        for (int u0 = u; u0 < nd; u0 += kWave) {
+            const bool open = u0 + lane < nd && !col_done(done, wdesc[u0 + lane].z);
+            const unsigned long long b = __ballot(open);
+            if (b) return u0 + __ffsll(b) - 1;
         }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        return nd;
+    };
     auto await = [&](int r) {
         int spins = 0;
         while (!col_done(done, r)) {
@@ -168,22 +165,19 @@ __device__ __forceinline__ void rf_column(const RfArgs& a, const int4 m0, const 
                 break;
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     };
     // Ring of prefetched steps.  Per step: U slot and chunk length packed
     // (wave-uniform) and r's stored diagonal (Sparse 1.3); per lane: its
     // relative slot and its L value.  Every load is unconditional (indices
-    // clamped) and the main loop is straight-line, so each wait covers
-    // exactly the step it feeds.  U values stay in the column workspace
-    // (no later step of the column reads or targets their slots) and are
-    // written out with the column.
+    // clamped to the segment) and the ring loop is straight-line, so each
+    // wait covers exactly the step it feeds.  U values stay in the column
+    // workspace (no later step of the column reads or targets their slots)
+    // and are written out with the column.
     int usnl[kPf];
     int16_t qr[kPf];
     double lv[kPf], dg[kPf];
-    auto fetch = [&](int u, int p) {
-        u = min(u, nd - 1);
-        const int4 d = wdesc[u];
-        if (u >= pend) await(d.z);
+    auto fetch = [&](int u, int p, int u1) {
+        const int4 d = wdesc[min(u, u1 - 1)];
         usnl[p] = d.x;
         const int nl = (d.x >> 16) & 0x7f;
         const int e = min(lane, nl > 0 ? nl - 1 : 0);
@@ -196,28 +190,48 @@ __device__ __forceinline__ void rf_column(const RfArgs& a, const int4 m0, const 
         const int nl = (usnl[p] >> 16) & 0x7f;
         const bool scale = s13 && !(usnl[p] & kContinued);   // a continuation finds U(r,j) in place
         const bool m = lane < nl && qr[p] >= 0;
-        // the pivot row's slot is never a target of its own step
+        // the pivot row's slot is never a target of its own step, so both
+        // reads are independent; the empty asm takes both values, which
+        // keeps the target read from sinking into the masked update below
+        // (it would then wait for the pivot read before it is issued)
         const double xr = ws[us];
         const double tv = ws[m ? qr[p] : 0];
+        asm volatile("" ::"v"(xr), "v"(tv));
         const double uval = scale ? xr * dg[p] : xr;
         if (m) ws[qr[p]] = tv - uval * lv[p];
         if (scale && lane == 0) ws[us] = uval;
         wave_sync();
     };
-    if (nd > 0) {
+    // Segments: the steps from u0 whose pivot columns are finished run
+    // through the ring; at a step whose pivot is still open the wave waits
+    // for that one column and scans on.  The ring thus never waits for a
+    // dependency with ready steps queued behind it.
+    int u0 = 0;
+    while (u0 < nd) {
+        int u1 = first_open(u0);
+        if (u1 == u0) {
+            await(wdesc[u0].z);
+            u1 = first_open(u0 + 1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        const int cnt = u1 - u0;
 #pragma unroll
-        for (int p = 0; p < kPf; ++p) fetch(p, p);
-        const int nfull = nd / kPf;
+        for (int p = 0; p < kPf; ++p) fetch(u0 + p, p, u1);
+        const int nfull = cnt / kPf;
         for (int r = 0; r < nfull; ++r) {
 #pragma unroll
             for (int p = 0; p < kPf; ++p) {
                 step(p);
-                fetch(r * kPf + p + kPf, p);
+                fetch(u0 + r * kPf + p + kPf, p, u1);
             }
         }
 #pragma unroll
         for (int p = 0; p < kPf; ++p)
-            if (nfull * kPf + p < nd) step(p);
+            if (nfull * kPf + p < cnt) step(p);
+        // the ring's last fetches (clamped, unused) are drained here, so the
+        // next segment starts with no loads outstanding
+        __builtin_amdgcn_s_waitcnt(0x0f70);   // vmcnt(0): vmcnt bits 3:0 and 15:14 zero, expcnt 7, lgkmcnt 15
+        u0 = u1;
     }
     // U part of the column: write out, and its largest magnitude
     const int nu = m0.w;
