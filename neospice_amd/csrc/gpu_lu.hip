@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <type_traits>
 #include <utility>
 
 #include "common.hpp"
@@ -166,41 +167,65 @@ __device__ __forceinline__ void rf_column(const RfArgs& a, const int4 m0, const 
             }
         }
     };
-    // Ring of prefetched steps.  Per step: U slot and chunk length packed
-    // (wave-uniform) and r's stored diagonal (Sparse 1.3); per lane: its
-    // relative slot and its L value.  Every load is unconditional (indices
-    // clamped to the segment) and the ring loop is straight-line, so each
-    // wait covers exactly the step it feeds.  U values stay in the column
-    // workspace (no later step of the column reads or targets their slots)
-    // and are written out with the column.
-    int usnl[kPf];
-    int16_t qr[kPf];
-    double lv[kPf], dg[kPf];
-    auto fetch = [&](int u, int p, int u1) {
-        const int4 d = wdesc[min(u, u1 - 1)];
-        usnl[p] = d.x;
-        const int nl = (d.x >> 16) & 0x7f;
-        const int e = min(lane, nl > 0 ? nl - 1 : 0);
-        qr[p] = a.rel[d.w + e];
-        lv[p] = LU[d.y + e];
-        if (s13) dg[p] = LU[d.y - 1];   // r's stored diagonal (first chunk: just above L(:, r))
-    };
-    auto step = [&](int p) {
-        const int us = usnl[p] & 0xffff;
-        const int nl = (usnl[p] >> 16) & 0x7f;
-        const bool scale = s13 && !(usnl[p] & kContinued);   // a continuation finds U(r,j) in place
-        const bool m = lane < nl && qr[p] >= 0;
-        // the pivot row's slot is never a target of its own step, so both
-        // reads are independent; the empty asm takes both values, which
-        // keeps the target read from sinking into the masked update below
-        // (it would then wait for the pivot read before it is issued)
-        const double xr = ws[us];
-        const double tv = ws[m ? qr[p] : 0];
-        asm volatile("" ::"v"(xr), "v"(tv));
-        const double uval = scale ? xr * dg[p] : xr;
-        if (m) ws[qr[p]] = tv - uval * lv[p];
-        if (scale && lane == 0) ws[us] = uval;
-        wave_sync();
+    // Ring of prefetched steps, D deep.  Per step: U slot and chunk length
+    // packed (wave-uniform) and r's stored diagonal (Sparse 1.3); per lane:
+    // its relative slot and its L value.  Every load is unconditional
+    // (indices clamped to the segment) and the ring loop is straight-line,
+    // so each wait covers exactly the step it feeds.  U values stay in the
+    // column workspace (no later step of the column reads or targets their
+    // slots) and are written out with the column.  The depth follows the
+    // segment's length: a ring fetches D steps past the segment's last
+    // group, and every vector load costs the address unit about as much
+    // as a full one, so a short segment gets a short ring.
+    auto ring = [&](auto depth, const int u0, const int u1) {
+        constexpr int D = decltype(depth)::value;
+        int usnl[D];
+        int16_t qr[D];
+        double lv[D], dg[D];
+        auto fetch = [&](int u, int p) {
+            const int4 d = wdesc[min(u, u1 - 1)];
+            usnl[p] = d.x;
+            const int nl = (d.x >> 16) & 0x7f;
+            const int e = min(lane, nl > 0 ? nl - 1 : 0);
+            qr[p] = a.rel[d.w + e];
+            lv[p] = LU[d.y + e];
+            if (s13) dg[p] = LU[d.y - 1];   // r's stored diagonal (first chunk: just above L(:, r))
+        };
+        auto step = [&](int p) {
+            const int us = usnl[p] & 0xffff;
+            const int nl = (usnl[p] >> 16) & 0x7f;
+            const bool scale = s13 && !(usnl[p] & kContinued);   // a continuation finds U(r,j) in place
+            const bool m = lane < nl && qr[p] >= 0;
+            // the pivot row's slot is never a target of its own step, so both
+            // reads are independent; the empty asm takes both values, which
+            // keeps the target read from sinking into the masked update below
+            // (it would then wait for the pivot read before it is issued)
+            const double xr = ws[us];
+            const double tv = ws[m ? qr[p] : 0];
+            asm volatile("" ::"v"(xr), "v"(tv));
+            const double uval = scale ? xr * dg[p] : xr;
+            if (m) ws[qr[p]] = tv - uval * lv[p];
+            if (scale && lane == 0) ws[us] = uval;
+            wave_sync();
+        };
+        const int cnt = u1 - u0;
+        const int ng = (cnt + D - 1) / D;   // groups of D steps
+#pragma unroll
+        for (int p = 0; p < D; ++p) fetch(u0 + p, p);
+        for (int g = 0; g + 1 < ng; ++g) {
+#pragma unroll
+            for (int p = 0; p < D; ++p) {
+                step(p);
+                fetch(u0 + (g + 1) * D + p, p);
+            }
+        }
+        const int base = (ng - 1) * D;
+#pragma unroll
+        for (int p = 0; p < D; ++p)
+            if (base + p < cnt) step(p);
+        // the ring's last fetches (clamped, unused) are drained here, so the
+        // next segment starts with no loads outstanding
+        __builtin_amdgcn_s_waitcnt(0x0f70);   // vmcnt(0): vmcnt bits 3:0 and 15:14 zero, expcnt 7, lgkmcnt 15
     };
     // Segments: the steps from u0 whose pivot columns are finished run
     // through the ring; at a step whose pivot is still open the wave waits
@@ -215,22 +240,11 @@ __device__ __forceinline__ void rf_column(const RfArgs& a, const int4 m0, const 
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         const int cnt = u1 - u0;
-#pragma unroll
-        for (int p = 0; p < kPf; ++p) fetch(u0 + p, p, u1);
-        const int nfull = cnt / kPf;
-        for (int r = 0; r < nfull; ++r) {
-#pragma unroll
-            for (int p = 0; p < kPf; ++p) {
-                step(p);
-                fetch(u0 + r * kPf + p + kPf, p, u1);
-            }
-        }
-#pragma unroll
-        for (int p = 0; p < kPf; ++p)
-            if (nfull * kPf + p < cnt) step(p);
-        // the ring's last fetches (clamped, unused) are drained here, so the
-        // next segment starts with no loads outstanding
-        __builtin_amdgcn_s_waitcnt(0x0f70);   // vmcnt(0): vmcnt bits 3:0 and 15:14 zero, expcnt 7, lgkmcnt 15
+        if (cnt >= 2 * kPf) ring(std::integral_constant<int, kPf>{}, u0, u1);
+        else if (cnt >= 12) ring(std::integral_constant<int, 8>{}, u0, u1);
+        else if (cnt >= 4) ring(std::integral_constant<int, 4>{}, u0, u1);
+        else if (cnt >= 2) ring(std::integral_constant<int, 2>{}, u0, u1);
+        else ring(std::integral_constant<int, 1>{}, u0, u1);
         u0 = u1;
     }
     // U part of the column: write out, and its largest magnitude
