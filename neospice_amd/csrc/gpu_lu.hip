@@ -31,6 +31,10 @@ constexpr int kSvThreads = 512;               // solve workgroup
 constexpr int kPf = 16;                       // U-entry steps whose operands are in flight
 constexpr int kContinued = 1 << 30;           // step descriptor: continuation chunk of an L column
 constexpr int kSpinMax = 1 << 24;              // bounded wait for a column (about 0.5 s)
+constexpr int kBufferFlags = 0x00020000;       // raw buffer resource, dword 3 (gfx9 family)
+constexpr unsigned kOutOfRange = 0xfffffff0u;  // buffer offset past any buffer here: load returns 0
+
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 constexpr int kSweepMax = 4 * kWave;          // solve column sweep: up to 4 targets per lane
 constexpr int kSweepPf = 8;                   // sweep steps whose operands are in flight
 __device__ __forceinline__ void wave_sync() {
@@ -77,6 +81,7 @@ struct RfArgs {
     // slots)
     const int4* desc;
     const int16_t* rel;    // per L entry of a chunk: slot in the column, -1 = not here
+    int nrel;              // entries of rel
     const int32_t *cand_ptr, *cand_pos, *diag_cand;
     const int32_t *tposmap, *tcand_ptr, *tdiag_cand;   // tposmap: 16 LU positions per thread (-1: none)
     const int16_t* tcand;
@@ -130,6 +135,8 @@ __device__ __forceinline__ void rf_column(const RfArgs& a, const int4 m0, const 
     const int lane = threadIdx.x & (kWave - 1);
     const bool s13 = a.form == FORM_S13;
     const int k = m0.x, cb = m0.y, len = m0.z;
+    const __amdgpu_buffer_rsrc_t lu_rsrc = __builtin_amdgcn_make_buffer_rsrc(LU, 0, a.nlu * 8, kBufferFlags);
+    const __amdgpu_buffer_rsrc_t rel_rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)a.rel, 0, a.nrel * 2, kBufferFlags);
     // Column load: the loaded values (A plus diag_gmin) already sit in the
     // column's own positions; four chunks in flight at once.
     for (int s0 = 0; s0 < len; s0 += 4 * kWave) {
@@ -186,10 +193,17 @@ __device__ __forceinline__ void rf_column(const RfArgs& a, const int4 m0, const 
             const int4 d = wdesc[min(u, u1 - 1)];
             usnl[p] = d.x;
             const int nl = (d.x >> 16) & 0x7f;
-            const int e = min(lane, nl > 0 ? nl - 1 : 0);
-            qr[p] = a.rel[d.w + e];
-            lv[p] = LU[d.y + e];
-            if (s13) dg[p] = LU[d.y - 1];   // r's stored diagonal (first chunk: just above L(:, r))
+            // Buffer loads: lanes past the chunk (and the diagonal load of
+            // AMD-LU) take an offset past the buffer's end, which returns 0
+            // without a memory access; every fetch issues the same three
+            // loads, so each wait stays exact.
+            const bool in = lane < nl;
+            qr[p] = (int16_t)__builtin_amdgcn_raw_buffer_load_b16(rel_rsrc, in ? (unsigned)(d.w + lane) * 2u : kOutOfRange, 0, 0);
+            const u32x2 lw = __builtin_amdgcn_raw_buffer_load_b64(lu_rsrc, in ? (unsigned)(d.y + lane) * 8u : kOutOfRange, 0, 0);
+            lv[p] = __hiloint2double((int)lw.y, (int)lw.x);
+            // r's stored diagonal (Sparse 1.3; first chunk: just above L(:, r))
+            const u32x2 gw = __builtin_amdgcn_raw_buffer_load_b64(lu_rsrc, s13 ? (unsigned)(d.y - 1) * 8u : kOutOfRange, 0, 0);
+            dg[p] = __hiloint2double((int)gw.y, (int)gw.x);
         };
         auto step = [&](int p) {
             const int us = usnl[p] & 0xffff;
@@ -1150,6 +1164,10 @@ GpuLu::GpuLu(const LuStructure& s, int batch, hipStream_t stream,
     d_desc_ = upload(allocs_, desc);
     rel.push_back(-1);   // a clamped read of an empty chunk at the end stays in bounds
     d_rel_ = upload(allocs_, rel);
+    nrel_ = (int)rel.size();
+    // the refactor reads the factor and rel through buffer resources (32-bit byte ranges)
+    if ((int64_t)s_.nlu() * 8 >= INT_MAX || (int64_t)nrel_ * 2 >= INT_MAX)
+        throw Error(NSA_ERR_UNSUPPORTED, "GpuLu: factor exceeds a buffer resource's range");
     std::vector<int32_t> cp = s_.cand_ptr, cpos = s_.cand_pos, dc = s_.diag_cand;
     if (cp.empty()) cp.assign(n + 1, 0);
     if (dc.empty()) dc.assign(n, -1);
@@ -1317,7 +1335,7 @@ void GpuLu::refactor(const double* dA, double gmin, const int32_t* dkind, int32_
     a.union_bytes = (int)union_bytes_;
     a.lev_ptr = d_lev_ptr_; a.cmeta = d_cmeta_;
     a.cptr = d_cptr_; a.nU = d_nU_; a.rows = d_rows_; a.a_src = d_asrc_; a.gmin_pos = d_gpos_;
-    a.desc_ptr = d_desc_ptr_; a.desc = d_desc_; a.rel = d_rel_;
+    a.desc_ptr = d_desc_ptr_; a.desc = d_desc_; a.rel = d_rel_; a.nrel = nrel_;
     a.cand_ptr = d_cand_ptr_; a.cand_pos = d_cand_pos_; a.diag_cand = d_diag_cand_;
     a.tposmap = d_tposmap_;
     a.tcand_ptr = d_tcand_ptr_; a.tdiag_cand = d_tdiag_cand_; a.tcand = d_tcand_; a.tmask = d_tmask_;

@@ -113,6 +113,7 @@ private:
             *d_gpos_ = nullptr, *d_desc_ptr_ = nullptr;
     int4* d_desc_ = nullptr;
     int16_t* d_rel_ = nullptr;
+    int nrel_ = 0;
     int32_t *d_cand_ptr_ = nullptr, *d_cand_pos_ = nullptr, *d_diag_cand_ = nullptr,
             *d_row_perm_ = nullptr, *d_col_perm_ = nullptr;
     // dense tail
