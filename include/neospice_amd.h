@@ -53,6 +53,7 @@ typedef struct {
     int32_t mode, order, method, pad;
     double ag0, ag1, ag2, xmu_ratio, time, src_fact;
     double delta, delta_old1; /* CKTdelta and CKTdeltaOld[1] (predictor ratio) */
+    double delta_old2;        /* CKTdeltaOld[2] (order-2 truncation error) */
 } nsa_ctx;
 
 const char* nsa_last_error(void);
@@ -99,6 +100,17 @@ int nsa_engine_set_state(nsa_engine* e, int32_t which, const double* s);
 /* state1..3 = state0 (transient.cpp:791-802) / rotate (circuit.cpp:328-334). */
 int nsa_engine_seed_states(nsa_engine* e);
 int nsa_engine_rotate_states(nsa_engine* e);
+/* Device local-truncation-error step proposal for the current context and
+ * states, after a step converged and before the states rotate: every
+ * device's compute_trunc (capacitor.cpp:242-250, dio_device.cpp:278-291,
+ * mos1_device.cpp:352-365; ckt_terr.hpp:19-64) and their minimum, as transient.cpp:478-485 and
+ * convergence.cpp:948-951 take it.  dt_min: B values (1e30 when no device
+ * proposes a step); dt_dev: NULL or nsa_engine_trunc_devices() x B,
+ * instance fastest.  Orders above 2 are not kept (four state buffers). */
+int nsa_engine_trunc(nsa_engine* e, double* dt_min, double* dt_dev);
+/* Devices with a truncation row (capacitors, diodes, then MOS1), as circuit device
+ * indices in dt_dev's row order; dev_index may be NULL.  Returns the count. */
+int nsa_engine_trunc_devices(nsa_engine* e, int32_t* dev_index);
 /* Fresh solver on the same pattern (a new make_solver + symbolic). */
 int nsa_engine_reset_solver(nsa_engine* e);
 /* One device load at the current solution: matrix values nnz x B, rhs n x B. */

@@ -177,12 +177,14 @@ FAMILIES = {"R": 0, "C": 1, "V": 2, "I": 3}
 
 
 def make_ctx(mode, order=1, method=0, ag=(0.0, 0.0, 0.0), xmu_ratio=1.0, time=0.0, src_fact=1.0,
-             delta=0.0, delta_old1=0.0):
+             delta=0.0, delta_old1=0.0, delta_old2=None):
     c = Ctx()
     c.mode, c.order, c.method = int(mode), int(order), int(method)
     c.ag0, c.ag1, c.ag2 = map(float, ag)
     c.xmu_ratio, c.time, c.src_fact = float(xmu_ratio), float(time), float(src_fact)
     c.delta, c.delta_old1 = float(delta), float(delta_old1)
+    # CKTdeltaOld[2]; a constant-step history unless given
+    c.delta_old2 = float(delta_old1 if delta_old2 is None else delta_old2)
     return c
 
 
@@ -267,6 +269,23 @@ class BatchEngine:
 
     def rotate_states(self):
         check(lib().nsa_engine_rotate_states(self.h))
+
+    def trunc(self, per_device=False):
+        """Device truncation-error step proposal after a converged step,
+        before rotate_states (nsa_engine_trunc): the minimum over devices
+        per instance (B,), and with per_device the devices' circuit indices
+        and their values (ndev x B)."""
+        B = self.B
+        dt = np.zeros(B)
+        if not per_device:
+            check(lib().nsa_engine_trunc(self.h, f64p(dt), None))
+            return dt
+        nd = check(lib().nsa_engine_trunc_devices(self.h, None))
+        idx = np.zeros(max(nd, 1), np.int32)
+        check(lib().nsa_engine_trunc_devices(self.h, i32p(idx)))
+        dev = np.zeros((max(nd, 1), B))
+        check(lib().nsa_engine_trunc(self.h, f64p(dt), f64p(dev)))
+        return dt, idx[:nd], dev[:nd]
 
     def reset_solver(self):
         check(lib().nsa_engine_reset_solver(self.h))

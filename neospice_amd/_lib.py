@@ -28,7 +28,7 @@ class Ctx(C.Structure):
     _fields_ = [("mode", C.c_int32), ("order", C.c_int32), ("method", C.c_int32), ("pad", C.c_int32),
                 ("ag0", C.c_double), ("ag1", C.c_double), ("ag2", C.c_double),
                 ("xmu_ratio", C.c_double), ("time", C.c_double), ("src_fact", C.c_double),
-                ("delta", C.c_double), ("delta_old1", C.c_double)]
+                ("delta", C.c_double), ("delta_old1", C.c_double), ("delta_old2", C.c_double)]
 
 
 _SIGS = {
@@ -63,6 +63,8 @@ _SIGS = {
     "nsa_engine_set_state": (C.c_int, [VP, I32, P_F64]),
     "nsa_engine_seed_states": (C.c_int, [VP]),
     "nsa_engine_rotate_states": (C.c_int, [VP]),
+    "nsa_engine_trunc": (C.c_int, [VP, P_F64, P_F64]),
+    "nsa_engine_trunc_devices": (C.c_int, [VP, P_I32]),
     "nsa_engine_reset_solver": (C.c_int, [VP]),
     "nsa_engine_load": (C.c_int, [VP, P_F64, P_F64]),
     "nsa_engine_solve_transposed": (C.c_int, [VP, P_F64]),

@@ -234,6 +234,20 @@ int nsa_engine_seed_states(nsa_engine* e) {
 int nsa_engine_rotate_states(nsa_engine* e) {
     return guard([&] { e->e->rotate_states(); return 0; });
 }
+int nsa_engine_trunc(nsa_engine* e, double* dt_min, double* dt_dev) {
+    return guard([&] {
+        if (!dt_min) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "nsa_engine_trunc: dt_min is NULL");
+        e->e->trunc(dt_min, dt_dev);
+        return 0;
+    });
+}
+int nsa_engine_trunc_devices(nsa_engine* e, int32_t* dev_index) {
+    return guard([&] {
+        const auto& v = e->e->trunc_devices();
+        if (dev_index) std::copy(v.begin(), v.end(), dev_index);
+        return (int)v.size();
+    });
+}
 int nsa_engine_reset_solver(nsa_engine* e) {
     return guard([&] { e->e->reset_solver(); return 0; });
 }

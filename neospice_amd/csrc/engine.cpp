@@ -259,6 +259,16 @@ void BatchEngine::build_tables() {
     const size_t n_aux = (size_t)std::max(1, mos_.count) * kMos1Aux * B_;
     mos_.aux = dalloc<double>(allocs_, n_aux);
     NSA_HIP(hipMemset(mos_.aux, 0, n_aux * sizeof(double)));
+    // truncation-error rows: the capacitors, the diodes, then the MOS1 devices
+    std::vector<int32_t> tbase, tkind;
+    for (int d : fc) { trunc_dev_.push_back(d); tbase.push_back(devs[d].state_base); tkind.push_back(0); }
+    for (int d : fd) { trunc_dev_.push_back(d); tbase.push_back(devs[d].state_base); tkind.push_back(2); }
+    for (int d : fm) { trunc_dev_.push_back(d); tbase.push_back(devs[d].state_base); tkind.push_back(1); }
+    trunc_.ndev = (int)trunc_dev_.size();
+    trunc_.base = dupload(allocs_, tbase);
+    trunc_.kind = dupload(allocs_, tkind);
+    d_tmin_ = dalloc<unsigned long long>(allocs_, (size_t)B_);
+    d_tdev_ = dalloc<double>(allocs_, (size_t)std::max(1, trunc_.ndev) * B_);
     // constant sites are written once
     launch_res_sites(nR_, d_r_ohms_, d_cm_, r_site_, B_, stream_);
     launch_const_sites(n_const_, d_const_vals_, d_cm_, const_site_, B_, stream_);
@@ -490,6 +500,25 @@ void BatchEngine::set_state(int which, const double* in) {
     if (!ns_) return;
     NSA_HIP(hipMemcpyAsync(d_s_[which], in, sizeof(double) * ns_ * B_, hipMemcpyHostToDevice, stream_));
     NSA_HIP(hipStreamSynchronize(stream_));
+}
+
+void BatchEngine::trunc(double* h_min, double* h_dev) {
+    const double big = 1e30;
+    unsigned long long big_bits;
+    std::memcpy(&big_bits, &big, sizeof(big));
+    launch_fill_u64(d_tmin_, big_bits, B_, stream_);
+    const TruncOpts o{opt_.abstol, opt_.reltol, opt_.chgtol, opt_.trtol};
+    launch_trunc(trunc_, d_ctx_, d_s_[0], d_s_[1], d_s_[2], d_s_[3], B_, o, h_dev ? d_tdev_ : nullptr,
+                 d_tmin_, stream_);
+    NSA_HIP(hipGetLastError());
+    std::vector<unsigned long long> bits(B_);
+    NSA_HIP(hipMemcpyAsync(bits.data(), d_tmin_, sizeof(unsigned long long) * B_, hipMemcpyDeviceToHost,
+                           stream_));
+    if (h_dev && trunc_.ndev)
+        NSA_HIP(hipMemcpyAsync(h_dev, d_tdev_, sizeof(double) * trunc_.ndev * B_, hipMemcpyDeviceToHost,
+                               stream_));
+    NSA_HIP(hipStreamSynchronize(stream_));
+    for (int i = 0; i < B_; ++i) std::memcpy(&h_min[i], &bits[i], sizeof(double));
 }
 
 void BatchEngine::seed_states_from_state0() {

@@ -69,6 +69,15 @@ public:
     void seed_states_from_state0();  // state1..3 = state0
     void rotate_states();            // state3 <- state2 <- state1 <- state0 (copy)
 
+    // Device local-truncation-error step proposal (compute_trunc of every
+    // device, transient.cpp:478-485 and convergence.cpp:948-951) for the
+    // current context and states, called after a step converges and before
+    // rotate_states.  h_min (B): the minimum over the devices per instance
+    // (1e30: no device proposes); h_dev (optional, trunc_devices() x B,
+    // instance fastest): each device's value.
+    void trunc(double* h_min, double* h_dev);
+    const std::vector<int32_t>& trunc_devices() const { return trunc_dev_; }
+
     // A new solver on the same pattern, as a fresh make_solver + symbolic.
     void reset_solver();
 
@@ -131,6 +140,10 @@ private:
     std::vector<void*> allocs_;
     double *d_x_ = nullptr, *d_xold_ = nullptr, *d_rhs_ = nullptr, *d_A_ = nullptr;
     double* d_s_[4] = {nullptr, nullptr, nullptr, nullptr};
+    TruncTable trunc_{};
+    std::vector<int32_t> trunc_dev_;          // circuit device index per trunc row
+    unsigned long long* d_tmin_ = nullptr;    // B
+    double* d_tdev_ = nullptr;                // trunc rows x B
     double *d_cm_ = nullptr, *d_cr_ = nullptr;
     DevCtx* d_ctx_ = nullptr;
     int32_t *d_done_ = nullptr, *d_nonconv_ = nullptr;

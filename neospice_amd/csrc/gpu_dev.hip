@@ -232,7 +232,88 @@ __global__ void k_take_instance(const double* __restrict__ A, int B, int inst, i
     if (gid < count) out[gid] = A[gid * B + inst];
 }
 
+// ckt_terr (src/devices/ckt_terr.hpp:19-64) for the charge at state q of
+// one instance, folded into dt the way the reference folds it (min of the
+// running value and the proposal; a NaN proposal leaves it).  std::max and
+// std::min keep their argument order: (a < b) ? b : a and (b < a) ? b : a.
+__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
+
+__device__ __forceinline__ double terr(int q, const double* s0, const double* s1, const double* s2,
+                                       const double* s3, int B, int inst, const DevCtx& c,
+                                       const TruncOpts& o, double dt) {
+    const int order = c.order;
+    const size_t iq = (size_t)q * B + inst, ic = iq + B;   // ccap = qcap + 1
+    const double volttol = o.abstol + o.reltol * smax(fabs(s0[ic]), fabs(s1[ic]));
+    double chargetol = smax(fabs(s0[iq]), fabs(s1[iq]));
+    chargetol = o.reltol * smax(chargetol, o.chgtol) / c.delta;
+    const double tol = smax(volttol, chargetol);
+    // divided differences of the charge history (order <= 2: four states)
+    const double* st[4] = {s0, s1, s2, s3};
+    const double dold[3] = {c.delta, c.delta_old1, c.delta_old2};   // CKTdeltaOld[0] is CKTdelta
+    double diff[4], deltmp[3];
+    for (int i = order + 1; i >= 0; --i) diff[i] = st[i][iq];
+    for (int i = 0; i <= order; ++i) deltmp[i] = dold[i];
+    int j = order;
+    for (;;) {
+        for (int i = 0; i <= j; ++i) diff[i] = (diff[i] - diff[i + 1]) / deltmp[i];
+        if (--j < 0) break;
+        for (int i = 0; i <= j; ++i) deltmp[i] = deltmp[i + 1] + dold[i];
+    }
+    // LTE coefficients of the integration methods (the reference's tables)
+    const double gear[2] = {0.5, 0.2222222222};
+    const double trap[2] = {0.5, 0.08333333333};
+    const double factor = c.method == 1 ? gear[order - 1] : trap[order - 1];
+    double del = o.trtol * tol / smax(o.abstol, factor * fabs(diff[0]));
+    if (order == 2) del = sqrt(del);
+    return (del < dt) ? del : dt;
+}
+
+__global__ void k_trunc(TruncTable t, const DevCtx* __restrict__ ctx, const double* __restrict__ s0,
+                        const double* __restrict__ s1, const double* __restrict__ s2,
+                        const double* __restrict__ s3, int B, TruncOpts o, double* __restrict__ dt_dev,
+                        unsigned long long* __restrict__ dt_min) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= t.ndev * B) return;
+    const int inst = idx % B, dev = idx / B;
+    const DevCtx c = ctx[inst];
+    double dt = 1e30;
+    // compute_trunc's guard; orders above 2 would need more than four states
+    if (c.order >= 1 && c.order <= 2 && c.delta > 0.0) {
+        const int base = t.base[dev];
+        if (t.kind[dev] == 0) {
+            dt = terr(base, s0, s1, s2, s3, B, inst, c, o, dt);
+        } else if (t.kind[dev] == 2) {
+            dt = terr(base + 3, s0, s1, s2, s3, B, inst, c, o, dt);   // DIOcapCharge
+        } else {
+            const int qoff[5] = {5, 8, 11, 13, 15};   // qgs, qgd, qgb, qbd, qbs
+            for (int k = 0; k < 5; ++k) dt = terr(base + qoff[k], s0, s1, s2, s3, B, inst, c, o, dt);
+        }
+    }
+    if (dt_dev) dt_dev[idx] = dt;
+    // dt >= 0: the bit patterns of non-negative doubles order like the values
+    atomicMin(&dt_min[inst], (unsigned long long)__double_as_longlong(dt));
+}
+
+__global__ void k_fill_u64(unsigned long long* p, unsigned long long v, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = v;
+}
+
 }  // namespace
+
+void launch_trunc(const TruncTable& t, const DevCtx* ctx, const double* s0, const double* s1,
+                  const double* s2, const double* s3, int B, const TruncOpts& o, double* dt_dev,
+                  unsigned long long* dt_min, hipStream_t s) {
+    const int total = t.ndev * B;
+    if (total <= 0) return;
+    hipLaunchKernelGGL(k_trunc, dim3((total + 255) / 256), dim3(256), 0, s, t, ctx, s0, s1, s2, s3, B, o,
+                       dt_dev, dt_min);
+}
+
+void launch_fill_u64(unsigned long long* p, unsigned long long v, int count, hipStream_t s) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_fill_u64, dim3((count + 255) / 256), dim3(256), 0, s, p, v, count);
+}
 
 void launch_res_sites(int count, const double* ohms, double* contrib, int site, int B,
                       hipStream_t s) {

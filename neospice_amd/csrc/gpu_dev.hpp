@@ -21,6 +21,7 @@ struct DevCtx {
     double src_fact;  // SimOptions::src_fact
     double delta;       // CKTdelta: the step being solved
     double delta_old1;  // CKTdeltaOld[1]: the previous step (predictor ratio)
+    double delta_old2;  // CKTdeltaOld[2] (truncation error of order 2)
 };
 
 // CKTmode bits (ngspice cktdefs.h, used by src/core/newton.cpp:109-115).
@@ -134,6 +135,27 @@ void launch_conv(int n, int num_nodes, const double* xnew, const double* xold, d
 void launch_commit(int n, const double* src, double* dst, const int32_t* mask, int mask_value,
                    int B, hipStream_t s);
 void launch_fill_i32(int32_t* p, int32_t v, int count, hipStream_t s);
+
+// Local truncation error (ckt_terr, src/devices/ckt_terr.hpp:19-64) of the
+// devices that propose a step: capacitors (one charge at their state base,
+// capacitor.cpp:242-250), diodes (the depletion charge at base + 3,
+// dio_device.cpp:278-291) and MOS1 (qgs, qgd, qgb, qbd, qbs at base + 5, 8,
+// 11, 13, 15; mos1_device.cpp:352-365).
+struct TruncTable {
+    int ndev;
+    const int32_t* base;   // state base per device
+    const int32_t* kind;   // 0 capacitor, 1 MOS1, 2 diode
+};
+struct TruncOpts {
+    double abstol, reltol, chgtol, trtol;
+};
+// dt_dev (ndev x B, optional) gets each device's compute_trunc; dt_min (B,
+// the bits of non-negative doubles, filled with those of 1e30 first) their
+// minimum per instance.
+void launch_trunc(const TruncTable& t, const DevCtx* ctx, const double* s0, const double* s1,
+                  const double* s2, const double* s3, int B, const TruncOpts& o, double* dt_dev,
+                  unsigned long long* dt_min, hipStream_t s);
+void launch_fill_u64(unsigned long long* p, unsigned long long v, int count, hipStream_t s);
 // out[k] = A[k * B + inst], k < count (one instance of a slot-major batch)
 void launch_take_instance(const double* A, int B, int inst, int count, double* out, hipStream_t s);
 

@@ -475,7 +475,13 @@ int ref_tran_fixed(void* p, double dt, int nsteps, double* solution_out, double*
 // order 2 of the circuit's integration method (trapezoidal with the
 // options' xmu, fill_integrator_context, transient.cpp:398-447).  Writes
 // every accepted solution ((nsteps+1) x n, row 0 = operating point).
-int ref_tran_steps(void* p, double dt, int nsteps, double* solutions_out, double* seconds) {
+// Fixed steps of dt (backward Euler, then order 2 of the circuit's method).
+// With trunc_out (nsteps x num devices), each step also records every
+// device's compute_trunc after its Newton solve and before the states
+// rotate (where transient.cpp:478-485 evaluates it), with the constant-step
+// history CKTdeltaOld[0..2] = dt.
+static int fixed_steps(void* p, double dt, int nsteps, double* solutions_out, double* seconds,
+                       double* trunc_out) {
     auto* h = static_cast<RefCkt*>(p);
     return guarded([&] {
         Circuit& ckt = h->ckt;
@@ -544,6 +550,12 @@ int ref_tran_steps(void* p, double dt, int nsteps, double* solutions_out, double
             NewtonResult r = newton_solve(ckt, *solver, sol, topts, ws);
             if (!r.converged) return -1;
             iters += r.iterations;
+            if (trunc_out) {
+                c.delta_old[2] = dt;
+                const size_t nd = ckt.devices().size();
+                for (size_t i = 0; i < nd; ++i)
+                    trunc_out[(size_t)k * nd + i] = ckt.devices()[i]->compute_trunc(c, ckt.options);
+            }
             ckt.rotate_state();
             if (solutions_out) std::copy(sol.begin(), sol.end(), solutions_out + (size_t)(k + 1) * n);
         }
@@ -552,6 +564,16 @@ int ref_tran_steps(void* p, double dt, int nsteps, double* solutions_out, double
         return iters;
     });
 }
+
+int ref_tran_steps(void* p, double dt, int nsteps, double* solutions_out, double* seconds) {
+    return fixed_steps(p, dt, nsteps, solutions_out, seconds, nullptr);
+}
+
+int ref_tran_trunc(void* p, double dt, int nsteps, double* solutions_out, double* trunc_out) {
+    return fixed_steps(p, dt, nsteps, solutions_out, nullptr, trunc_out);
+}
+
+int ref_num_devices(void* p) { return (int)static_cast<RefCkt*>(p)->ckt.devices().size(); }
 
 int ref_tran_num_records(void* p) { return (int)static_cast<RefCkt*>(p)->recs.size(); }
 

@@ -70,6 +70,8 @@ def ref():
             "ref_newton": (C.c_int, [VP, P_F64, C.c_int]),
             "ref_sens": (C.c_int, [VP, C.c_int, C.c_int, P_F64, C.c_char_p, C.c_int]),
             "ref_tran_steps": (C.c_int, [VP, C.c_double, C.c_int, P_F64, P_F64]),
+            "ref_tran_trunc": (C.c_int, [VP, C.c_double, C.c_int, P_F64, P_F64]),
+            "ref_num_devices": (C.c_int, [VP]),
             "ref_solver_new": (VP, [C.c_int, C.c_int, P_I32, P_I32]),
             "ref_solver_free": (None, [VP]),
             "ref_solver_numeric": (C.c_int, [VP, P_F64, C.c_double]),
@@ -318,6 +320,18 @@ class RefCircuit:
         its = _chk(self.L.ref_tran_steps(self.h, dt, nsteps, xs.ctypes.data_as(P_F64),
                                          C.byref(sec)))
         return xs.reshape(nsteps + 1, self.n), its, sec.value
+
+    def tran_trunc(self, dt, nsteps):
+        """The fixed-step transient of tran_steps, recording after each step
+        (before the states rotate) every device's compute_trunc with
+        CKTdeltaOld[0..2] = dt: solutions ((nsteps+1) x n) and
+        (nsteps x num devices) step proposals (1e30: none)."""
+        nd = self.L.ref_num_devices(self.h)
+        xs = np.zeros((nsteps + 1) * self.n)
+        tr = np.zeros(nsteps * max(nd, 1))
+        _chk(self.L.ref_tran_trunc(self.h, dt, nsteps, xs.ctypes.data_as(P_F64),
+                                   tr.ctypes.data_as(P_F64)))
+        return xs.reshape(nsteps + 1, self.n), tr[:nsteps * nd].reshape(nsteps, nd)
 
     def sens(self, out_node, max_out=4096):
         """.sens of v(out_node) (sens.cpp:57-186): {element:parameter: value}."""
