@@ -641,17 +641,18 @@ __global__ __launch_bounds__(kRfThreads) void k_refactor(RfArgs a) {
         __syncthreads();
     }
     double* SV = a.SV + (size_t)inst * a.nsv;
-    // solve-ordered copy, eight gathers in flight per thread
+    // solve-ordered copy, eight gathers in flight per thread; entries the
+    // sweeps mask (position -1) are not copied
     for (int i0 = threadIdx.x; i0 < a.nsv; i0 += 8 * kRfThreads) {
         int q[8];
         double v[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) q[u] = a.sv_pos[min(i0 + u * kRfThreads, a.nsv - 1)];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = LU[q[u]];
+        for (int u = 0; u < 8; ++u) v[u] = q[u] >= 0 ? LU[q[u]] : 0.0;
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-            if (i0 + u * kRfThreads < a.nsv) SV[i0 + u * kRfThreads] = v[u];
+            if (i0 + u * kRfThreads < a.nsv && q[u] >= 0) SV[i0 + u * kRfThreads] = v[u];
     }
 }
 
@@ -1309,8 +1310,10 @@ void GpuLu::upload_program(const TriProgram& t, DevProgram& d, std::vector<int32
                 for (int r = 0; r < D; ++r) {
                     int q = t.sweep_pos[(size_t)st * D + r];
                     if (q >= 0) bits[(size_t)st * kWave + r % kWave] |= (uint8_t)(1u << (r / kWave));
-                    if (r == st) q = has_diag ? t.dpos[tg] : 0;
-                    sv_pos->push_back(q >= 0 ? q : 0);
+                    if (r == st) q = has_diag ? t.dpos[tg] : -1;
+                    // -1: an absent term (or the unused diagonal of a plain
+                    // program); the sweep masks it, so the copy skips it
+                    sv_pos->push_back(q);
                 }
             }
             d.sweep_tgt = upload(allocs_, t.sweep_tgt);
