@@ -31,6 +31,7 @@ constexpr int kSvThreads = 512;               // solve workgroup
 constexpr int kPf = 16;                       // U-entry steps whose operands are in flight
 constexpr int kContinued = 1 << 30;           // step descriptor: continuation chunk of an L column
 constexpr int kSpinMax = 1 << 24;              // bounded wait for a column (about 0.5 s)
+constexpr int kChunk = kWave - 1;              // L entries per step (lane 63 carries the diagonal)
 constexpr int kBufferFlags = 0x00020000;       // raw buffer resource, dword 3 (gfx9 family)
 constexpr unsigned kOutOfRange = 0xfffffff0u;  // buffer offset past any buffer here: load returns 0
 
@@ -188,22 +189,23 @@ __device__ __forceinline__ void rf_column(const RfArgs& a, const int4 m0, const 
         constexpr int D = decltype(depth)::value;
         int usnl[D];
         int16_t qr[D];
-        double lv[D], dg[D];
+        double lv[D];
         auto fetch = [&](int u, int p) {
             const int4 d = wdesc[min(u, u1 - 1)];
             usnl[p] = d.x;
             const int nl = (d.x >> 16) & 0x7f;
-            // Buffer loads: lanes past the chunk (and the diagonal load of
-            // AMD-LU) take an offset past the buffer's end, which returns 0
-            // without a memory access; every fetch issues the same three
-            // loads, so each wait stays exact.
+            // Buffer loads: lanes past the chunk take an offset past the
+            // buffer's end, which returns 0 without a memory access; every
+            // fetch issues the same two loads, so each wait stays exact.
+            // Chunks hold at most 63 entries, so lane 63 is always past the
+            // chunk: for Sparse 1.3 it loads r's stored diagonal (first
+            // chunk: just above L(:, r)) with the chunk.
             const bool in = lane < nl;
+            const unsigned lo = in ? (unsigned)(d.y + lane) * 8u
+                                   : (lane == kWave - 1 && s13) ? (unsigned)(d.y - 1) * 8u : kOutOfRange;
             qr[p] = (int16_t)__builtin_amdgcn_raw_buffer_load_b16(rel_rsrc, in ? (unsigned)(d.w + lane) * 2u : kOutOfRange, 0, 0);
-            const u32x2 lw = __builtin_amdgcn_raw_buffer_load_b64(lu_rsrc, in ? (unsigned)(d.y + lane) * 8u : kOutOfRange, 0, 0);
+            const u32x2 lw = __builtin_amdgcn_raw_buffer_load_b64(lu_rsrc, lo, 0, 0);
             lv[p] = __hiloint2double((int)lw.y, (int)lw.x);
-            // r's stored diagonal (Sparse 1.3; first chunk: just above L(:, r))
-            const u32x2 gw = __builtin_amdgcn_raw_buffer_load_b64(lu_rsrc, s13 ? (unsigned)(d.y - 1) * 8u : kOutOfRange, 0, 0);
-            dg[p] = __hiloint2double((int)gw.y, (int)gw.x);
         };
         auto step = [&](int p) {
             const int us = usnl[p] & 0xffff;
@@ -217,7 +219,7 @@ __device__ __forceinline__ void rf_column(const RfArgs& a, const int4 m0, const 
             const double xr = ws[us];
             const double tv = ws[m ? qr[p] : 0];
             asm volatile("" ::"v"(xr), "v"(tv));
-            const double uval = scale ? xr * dg[p] : xr;
+            const double uval = scale ? xr * readlane_f64(lv[p], kWave - 1) : xr;
             if (m) ws[qr[p]] = tv - uval * lv[p];
             if (scale && lane == 0) ws[us] = uval;
             wave_sync();
@@ -1071,10 +1073,11 @@ GpuLu::GpuLu(const LuStructure& s, int batch, hipStream_t stream,
             const bool deferred = k >= T0_ && cur >= 0;
             if (rel.size() + (size_t)nl[r] >= (size_t)INT_MAX)
                 throw Error(NSA_ERR_UNSUPPORTED, "GpuLu: plan too large");
-            // L(:, r) in chunks of 64 (one entry per lane); a continuation
-            // chunk recomputes the same U value (x_r is not among its targets)
-            for (int c0 = 0; c0 < std::max(nl[r], 1); c0 += kWave) {
-                const int cl = std::min(kWave, nl[r] - c0);
+            // L(:, r) in chunks of 63 (one entry per lane; lane 63 carries
+            // the diagonal for Sparse 1.3); a continuation chunk recomputes
+            // the same U value (x_r is not among its targets)
+            for (int c0 = 0; c0 < std::max(nl[r], 1); c0 += kChunk) {
+                const int cl = std::min(kChunk, nl[r] - c0);
                 desc.push_back(make_int4((p - cb) | (std::max(cl, 0) << 16) | (c0 > 0 ? kContinued : 0),
                                          dpos[r] + 1 + c0, r, (int)rel.size()));
                 for (int e = dpos[r] + 1 + c0; e < dpos[r] + 1 + c0 + cl; ++e) {
