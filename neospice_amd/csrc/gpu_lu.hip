@@ -36,6 +36,7 @@ constexpr int kBufferFlags = 0x00020000;       // raw buffer resource, dword 3 (
 constexpr unsigned kOutOfRange = 0xfffffff0u;  // buffer offset past any buffer here: load returns 0
 
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kSweepMax = 4 * kWave;          // solve column sweep: up to 4 targets per lane
 constexpr int kSweepPf = 8;                   // sweep steps whose operands are in flight
 __device__ __forceinline__ void wave_sync() {
@@ -176,51 +177,75 @@ __device__ __forceinline__ void rf_column(const RfArgs& a, const int4 m0, const 
         }
     };
     // Ring of prefetched steps, D deep.  Per step: U slot and chunk length
-    // packed (wave-uniform) and r's stored diagonal (Sparse 1.3); per lane:
-    // its relative slot and its L value.  Every load is unconditional
-    // (indices clamped to the segment) and the ring loop is straight-line,
-    // so each wait covers exactly the step it feeds.  U values stay in the
-    // column workspace (no later step of the column reads or targets their
-    // slots) and are written out with the column.  The depth follows the
-    // segment's length: a ring fetches D steps past the segment's last
-    // group, and every vector load costs the address unit about as much
-    // as a full one, so a short segment gets a short ring.
+    // packed (wave-uniform); per lane: its relative slot and its L value.
+    // Every load is unconditional (indices clamped to the segment) and the
+    // ring loop is straight-line, so each wait covers exactly the step it
+    // feeds.  U values stay in the column workspace (no later step of the
+    // column reads or targets their slots) and are written out with the
+    // column.  The address unit, which bounds this loop, spends about the
+    // same on any vector load, so loads are few: rings of 8 or 16 start at
+    // a step group and fetch a lane's relative slots for eight steps with
+    // one load; shorter rings (short segments) load one slot per step.
+    // The depth follows the segment's length, since a ring fetches up to D
+    // steps past the segment's end.
     auto ring = [&](auto depth, const int u0, const int u1) {
         constexpr int D = decltype(depth)::value;
+        constexpr bool kGrouped = D % 8 == 0;   // u0 is then a multiple of 8
         int usnl[D];
         int16_t qr[D];
         double lv[D];
+        u32x4 gq[kGrouped ? D / 8 : 1];
         auto fetch = [&](int u, int p) {
-            const int4 d = wdesc[min(u, u1 - 1)];
+            const int uc = min(u, u1 - 1);
+            const int4 d = wdesc[uc];
             usnl[p] = d.x;
             const int nl = (d.x >> 16) & 0x7f;
+            if constexpr (kGrouped) {
+                // a group's slots are loaded at its last slot's fetch: the
+                // steps of the group before it still read the previous ring
+                // cycle's register (a clamped uc stays in the same group)
+                if (p % 8 == 7)
+                    gq[p / 8] = __builtin_amdgcn_raw_buffer_load_b128(rel_rsrc, (unsigned)(d.w * 512 + lane * 8) * 2u, 0, 0);
+            } else {
+                qr[p] = (int16_t)__builtin_amdgcn_raw_buffer_load_b16(
+                    rel_rsrc, (unsigned)(d.w * 512 + lane * 8 + (uc & 7)) * 2u, 0, 0);
+            }
             // Buffer loads: lanes past the chunk take an offset past the
-            // buffer's end, which returns 0 without a memory access; every
-            // fetch issues the same two loads, so each wait stays exact.
+            // buffer's end, which returns 0 without a memory access.
             // Chunks hold at most 63 entries, so lane 63 is always past the
             // chunk: for Sparse 1.3 it loads r's stored diagonal (first
             // chunk: just above L(:, r)) with the chunk.
             const bool in = lane < nl;
             const unsigned lo = in ? (unsigned)(d.y + lane) * 8u
                                    : (lane == kWave - 1 && s13) ? (unsigned)(d.y - 1) * 8u : kOutOfRange;
-            qr[p] = (int16_t)__builtin_amdgcn_raw_buffer_load_b16(rel_rsrc, in ? (unsigned)(d.w + lane) * 2u : kOutOfRange, 0, 0);
             const u32x2 lw = __builtin_amdgcn_raw_buffer_load_b64(lu_rsrc, lo, 0, 0);
             lv[p] = __hiloint2double((int)lw.y, (int)lw.x);
+        };
+        auto slot_of = [&](int p) -> int {
+            if constexpr (kGrouped) {
+                const u32x4 g = gq[p / 8];
+                const int j = p % 8;
+                const unsigned w = j < 2 ? g.x : j < 4 ? g.y : j < 6 ? g.z : g.w;
+                return (int)(int16_t)(w >> (16 * (j & 1)));
+            } else {
+                return qr[p];
+            }
         };
         auto step = [&](int p) {
             const int us = usnl[p] & 0xffff;
             const int nl = (usnl[p] >> 16) & 0x7f;
             const bool scale = s13 && !(usnl[p] & kContinued);   // a continuation finds U(r,j) in place
-            const bool m = lane < nl && qr[p] >= 0;
+            const int q = slot_of(p);
+            const bool m = lane < nl && q >= 0;
             // the pivot row's slot is never a target of its own step, so both
             // reads are independent; the empty asm takes both values, which
             // keeps the target read from sinking into the masked update below
             // (it would then wait for the pivot read before it is issued)
             const double xr = ws[us];
-            const double tv = ws[m ? qr[p] : 0];
+            const double tv = ws[m ? q : 0];
             asm volatile("" ::"v"(xr), "v"(tv));
             const double uval = scale ? xr * readlane_f64(lv[p], kWave - 1) : xr;
-            if (m) ws[qr[p]] = tv - uval * lv[p];
+            if (m) ws[q] = tv - uval * lv[p];
             if (scale && lane == 0) ws[us] = uval;
             wave_sync();
         };
@@ -243,10 +268,21 @@ __device__ __forceinline__ void rf_column(const RfArgs& a, const int4 m0, const 
         // next segment starts with no loads outstanding
         __builtin_amdgcn_s_waitcnt(0x0f70);   // vmcnt(0): vmcnt bits 3:0 and 15:14 zero, expcnt 7, lgkmcnt 15
     };
+    // steps [s, e) through the ring its length calls for (grouped rings
+    // only from a group boundary)
+    auto run = [&](int s, int e) {
+        const int cnt = e - s;
+        if (cnt >= 2 * kPf && (s & 7) == 0) ring(std::integral_constant<int, kPf>{}, s, e);
+        else if (cnt >= 12 && (s & 7) == 0) ring(std::integral_constant<int, 8>{}, s, e);
+        else if (cnt >= 4) ring(std::integral_constant<int, 4>{}, s, e);
+        else if (cnt >= 2) ring(std::integral_constant<int, 2>{}, s, e);
+        else ring(std::integral_constant<int, 1>{}, s, e);
+    };
     // Segments: the steps from u0 whose pivot columns are finished run
-    // through the ring; at a step whose pivot is still open the wave waits
-    // for that one column and scans on.  The ring thus never waits for a
-    // dependency with ready steps queued behind it.
+    // through the rings; at a step whose pivot is still open the wave waits
+    // for that one column and scans on.  A dependency thus never holds
+    // ready steps behind it.  A long segment that starts inside a step
+    // group first runs up to the group's end.
     int u0 = 0;
     while (u0 < nd) {
         int u1 = first_open(u0);
@@ -255,12 +291,12 @@ __device__ __forceinline__ void rf_column(const RfArgs& a, const int4 m0, const 
             u1 = first_open(u0 + 1);
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        const int cnt = u1 - u0;
-        if (cnt >= 2 * kPf) ring(std::integral_constant<int, kPf>{}, u0, u1);
-        else if (cnt >= 12) ring(std::integral_constant<int, 8>{}, u0, u1);
-        else if (cnt >= 4) ring(std::integral_constant<int, 4>{}, u0, u1);
-        else if (cnt >= 2) ring(std::integral_constant<int, 2>{}, u0, u1);
-        else ring(std::integral_constant<int, 1>{}, u0, u1);
+        int s = u0;
+        if (u1 - s >= 20 && (s & 7) != 0) {
+            run(s, (s | 7) + 1);
+            s = (s | 7) + 1;
+        }
+        run(s, u1);
         u0 = u1;
     }
     // U part of the column: write out, and its largest magnitude
@@ -1053,14 +1089,22 @@ GpuLu::GpuLu(const LuStructure& s, int batch, hipStream_t stream,
     // "deferred": the sparse pass applies it to the rows before the block
     // only (rel = -1 for the block's rows), and the dense phase applies it
     // to the block's rows right after step s, in list order (dtask lists).
+    //
+    // Relative slots are grouped by eight consecutive steps of a column:
+    // group G holds, for lane l and step j = u % 8 of the group, the slot
+    // of chunk entry l at rel[G * 512 + l * 8 + j] (-1: none), so one
+    // 16-byte load per lane brings a lane's slots for eight steps.  desc.w
+    // is the step's group.
     std::vector<int32_t> desc_ptr(n + 1, 0);
     std::vector<int4> desc;
     std::vector<int16_t> rel;
+    int ngroups = 0;
     std::vector<std::vector<std::pair<int32_t, std::vector<int32_t>>>> step_tasks(std::max(t, 1));
     std::vector<int32_t> slot(n, -1);
     std::vector<int32_t> task_of(std::max(t, 1), -1);  // target row -> task in step_tasks[cur]
     for (int k = 0; k < n; ++k) {
         const int cb = s_.cptr[k];
+        const int d_begin = (int)desc.size(), g_begin = ngroups;
         for (int p = cb; p < s_.cptr[k + 1]; ++p) slot[s_.rows[p]] = p - cb;
         int cur = -1;                        // last trailing pivot seen in this list
         for (int p = cb; p < cb + s_.nU[k]; ++p) {
@@ -1071,19 +1115,25 @@ GpuLu::GpuLu(const LuStructure& s, int batch, hipStream_t stream,
                 continue;
             }
             const bool deferred = k >= T0_ && cur >= 0;
-            if (rel.size() + (size_t)nl[r] >= (size_t)INT_MAX)
+            if ((size_t)(ngroups + 2) * 512 * 2 >= (size_t)INT_MAX)
                 throw Error(NSA_ERR_UNSUPPORTED, "GpuLu: plan too large");
             // L(:, r) in chunks of 63 (one entry per lane; lane 63 carries
             // the diagonal for Sparse 1.3); a continuation chunk recomputes
             // the same U value (x_r is not among its targets)
             for (int c0 = 0; c0 < std::max(nl[r], 1); c0 += kChunk) {
                 const int cl = std::min(kChunk, nl[r] - c0);
+                const int u = (int)desc.size() - d_begin;   // step index in the column
+                const int G = g_begin + u / 8;
+                if (u % 8 == 0) {
+                    ngroups = G + 1;
+                    rel.resize((size_t)ngroups * 512, (int16_t)-1);
+                }
                 desc.push_back(make_int4((p - cb) | (std::max(cl, 0) << 16) | (c0 > 0 ? kContinued : 0),
-                                         dpos[r] + 1 + c0, r, (int)rel.size()));
-                for (int e = dpos[r] + 1 + c0; e < dpos[r] + 1 + c0 + cl; ++e) {
-                    const int i = s_.rows[e];
+                                         dpos[r] + 1 + c0, r, G));
+                for (int e = 0; e < cl; ++e) {
+                    const int i = s_.rows[dpos[r] + 1 + c0 + e];
                     if (slot[i] < 0) throw Error(NSA_ERR_LOGIC, "GpuLu: fill pattern not closed");
-                    rel.push_back((deferred && i >= T0_) ? (int16_t)-1 : (int16_t)slot[i]);
+                    rel[(size_t)G * 512 + e * 8 + u % 8] = (deferred && i >= T0_) ? (int16_t)-1 : (int16_t)slot[i];
                 }
             }
             if (!deferred) continue;
@@ -1166,7 +1216,7 @@ GpuLu::GpuLu(const LuStructure& s, int batch, hipStream_t stream,
     if (rf_lds_ > lds_max) throw Error(NSA_ERR_UNSUPPORTED, "GpuLu: refactor LDS budget");
     desc.push_back(make_int4(0, 0, 0, 0));
     d_desc_ = upload(allocs_, desc);
-    rel.push_back(-1);   // a clamped read of an empty chunk at the end stays in bounds
+    if (rel.empty()) rel.assign(512, (int16_t)-1);
     d_rel_ = upload(allocs_, rel);
     nrel_ = (int)rel.size();
     // the refactor reads the factor and rel through buffer resources (32-bit byte ranges)
