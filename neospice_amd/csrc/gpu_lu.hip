@@ -197,7 +197,11 @@ __device__ __forceinline__ void rf_column(const RfArgs& a, const int4 m0, const 
         u32x4 gq[kGrouped ? D / 8 : 1];
         auto fetch = [&](int u, int p) {
             const int uc = min(u, u1 - 1);
-            const int4 d = wdesc[uc];
+            // the descriptor is wave-uniform: scalar copies keep the
+            // offsets' selects below free of exec-mask branches
+            const int4 dv = wdesc[uc];
+            const int4 d = make_int4(__builtin_amdgcn_readfirstlane(dv.x), __builtin_amdgcn_readfirstlane(dv.y),
+                                     0, __builtin_amdgcn_readfirstlane(dv.w));
             usnl[p] = d.x;
             const int nl = (d.x >> 16) & 0x7f;
             if constexpr (kGrouped) {
@@ -215,9 +219,12 @@ __device__ __forceinline__ void rf_column(const RfArgs& a, const int4 m0, const 
             // Chunks hold at most 63 entries, so lane 63 is always past the
             // chunk: for Sparse 1.3 it loads r's stored diagonal (first
             // chunk: just above L(:, r)) with the chunk.
-            const bool in = lane < nl;
-            const unsigned lo = in ? (unsigned)(d.y + lane) * 8u
-                                   : (lane == kWave - 1 && s13) ? (unsigned)(d.y - 1) * 8u : kOutOfRange;
+            // (bitwise selects: a conditional expression here compiles to
+            // exec-mask branches)
+            const unsigned in_mask = 0u - (unsigned)(lane < nl);
+            const unsigned dg_mask = 0u - (unsigned)(lane == kWave - 1 && s13);
+            const unsigned past = ((unsigned)(d.y - 1) * 8u & dg_mask) | (kOutOfRange & ~dg_mask);
+            const unsigned lo = ((unsigned)(d.y + lane) * 8u & in_mask) | (past & ~in_mask);
             const u32x2 lw = __builtin_amdgcn_raw_buffer_load_b64(lu_rsrc, lo, 0, 0);
             lv[p] = __hiloint2double((int)lw.y, (int)lw.x);
         };
@@ -232,9 +239,10 @@ __device__ __forceinline__ void rf_column(const RfArgs& a, const int4 m0, const 
             }
         };
         auto step = [&](int p) {
-            const int us = usnl[p] & 0xffff;
-            const int nl = (usnl[p] >> 16) & 0x7f;
-            const bool scale = s13 && !(usnl[p] & kContinued);   // a continuation finds U(r,j) in place
+            const int ud = __builtin_amdgcn_readfirstlane(usnl[p]);   // uniform: scalar branches below
+            const int us = ud & 0xffff;
+            const int nl = (ud >> 16) & 0x7f;
+            const bool scale = s13 && !(ud & kContinued);   // a continuation finds U(r,j) in place
             const int q = slot_of(p);
             const bool m = lane < nl && q >= 0;
             // the pivot row's slot is never a target of its own step, so both
@@ -246,7 +254,9 @@ __device__ __forceinline__ void rf_column(const RfArgs& a, const int4 m0, const 
             asm volatile("" ::"v"(xr), "v"(tv));
             const double uval = scale ? xr * readlane_f64(lv[p], kWave - 1) : xr;
             if (m) ws[q] = tv - uval * lv[p];
-            if (scale && lane == 0) ws[us] = uval;
+            if (scale) {
+                if (lane == 0) ws[us] = uval;
+            }
             wave_sync();
         };
         const int cnt = u1 - u0;
