@@ -419,9 +419,13 @@ __device__ __forceinline__ void rf_tail(const RfArgs& a, double* lds, double* LU
     }
     __syncthreads();
     int dt_next = a.dstep_ptr[0];   // deferred-task table of the next step, loaded a step ahead
-    for (int s = 0; s < t; ++s) {
+    // One pivot step.  The register column of pivot s (bs = s / 32) is a
+    // constant of each instantiation, so column s's registers are read and
+    // written directly instead of through four-way selects.
+    auto step = [&](auto bs_tag, const int s) {
+        constexpr int bs = decltype(bs_tag)::value;
         const int buf = s & 1;
-        const int ts = s & 31, bs = s >> 5;    // column s: owners tj == ts, register column bs
+        const int ts = s & 31;                 // column s: owners tj == ts, register column bs
         const uint32_t* lm = msk + 12 * s;     // L rows of column s
         const uint32_t* um = lm + 4;           // U columns of row s
         const int dt = dt_next;
@@ -432,9 +436,8 @@ __device__ __forceinline__ void rf_tail(const RfArgs& a, double* lds, double* LU
             const int src = ts + 32 * (ts & 1);    // lane owning (s, s)
             double cv[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                cv[q] = bs == 0 ? R[q][0] : bs == 1 ? R[q][1] : bs == 2 ? R[q][2] : R[q][3];
-            const double mine = bs == 0 ? cv[0] : bs == 1 ? cv[1] : bs == 2 ? cv[2] : cv[3];
+            for (int q = 0; q < 4; ++q) cv[q] = R[q][bs];
+            const double mine = cv[bs];
             const double piv = readlane_f64(mine, src);
             int code = FAIL_NONE;
             if (!s13) {
@@ -511,12 +514,7 @@ __device__ __forceinline__ void rf_tail(const RfArgs& a, double* lds, double* LU
             }
             if (own) {
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    if (bs == 0) R[q][0] = cv[q];
-                    else if (bs == 1) R[q][1] = cv[q];
-                    else if (bs == 2) R[q][2] = cv[q];
-                    else R[q][3] = cv[q];
-                }
+                for (int q = 0; q < 4; ++q) R[q][bs] = cv[q];
             }
             if (lane == src && code != FAIL_NONE) flag(a.fail, a.fail_col, ginst, code, a.T0 + s);
         }
@@ -532,7 +530,7 @@ __device__ __forceinline__ void rf_tail(const RfArgs& a, double* lds, double* LU
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const int j = tj + 32 * b;
-                const double raw = bs == 0 ? R[0][b] : bs == 1 ? R[1][b] : bs == 2 ? R[2][b] : R[3][b];
+                const double raw = R[bs][b];
                 if (j > s && j < t && ((uwa[b] >> tj) & 1u)) {
                     Urow[buf * 128 + j] = raw;
                     if (!s13) cmx[j] = fmax(cm[b], fabs(raw));
@@ -559,13 +557,8 @@ __device__ __forceinline__ void rf_tail(const RfArgs& a, double* lds, double* LU
         }
         if (s13 && ti == ts) {
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                if (!uok[b]) continue;
-                if (bs == 0) R[0][b] = uv[b];
-                else if (bs == 1) R[1][b] = uv[b];
-                else if (bs == 2) R[2][b] = uv[b];
-                else R[3][b] = uv[b];
-            }
+            for (int b = 0; b < 4; ++b)
+                if (uok[b]) R[bs][b] = uv[b];
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -597,7 +590,11 @@ __device__ __forceinline__ void rf_tail(const RfArgs& a, double* lds, double* LU
                     if (tk.x == q) R[q >> 2][q & 3] = acc;
             }
         }
-    }
+    };
+    for (int s = 0; s < min(t, 32); ++s) step(std::integral_constant<int, 0>{}, s);
+    for (int s = 32; s < min(t, 64); ++s) step(std::integral_constant<int, 1>{}, s);
+    for (int s = 64; s < min(t, 96); ++s) step(std::integral_constant<int, 2>{}, s);
+    for (int s = 96; s < t; ++s) step(std::integral_constant<int, 3>{}, s);
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int pos = a.tposmap[tid * 16 + q];
