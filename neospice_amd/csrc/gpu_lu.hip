@@ -736,7 +736,7 @@ struct RowPrefetch {
 struct SweepArgs {
     int n, sv;
     const int32_t* tgt;
-    const uint8_t* bits;
+    const int32_t* bits;
 };
 
 template <int kOp>
@@ -1362,14 +1362,14 @@ void GpuLu::upload_program(const TriProgram& t, DevProgram& d, std::vector<int32
             d.sweep_n = D;
             d.sweep_first = t.sweep_first ? 1 : 0;
             d.sweep_sv = (int)sv_pos->size();
-            std::vector<uint8_t> bits((size_t)D * kWave, 0);
+            std::vector<int32_t> bits((size_t)D * kWave, 0);
             for (int st = 0; st < D; ++st) {
                 const int tg = t.sweep_tgt[st];
                 if (has_diag && t.dpos[tg] < 0)
                     throw Error(NSA_ERR_LOGIC, "GpuLu: swept target without diagonal");
                 for (int r = 0; r < D; ++r) {
                     int q = t.sweep_pos[(size_t)st * D + r];
-                    if (q >= 0) bits[(size_t)st * kWave + r % kWave] |= (uint8_t)(1u << (r / kWave));
+                    if (q >= 0) bits[(size_t)st * kWave + r % kWave] |= 1 << (r / kWave);
                     if (r == st) q = has_diag ? t.dpos[tg] : -1;
                     // -1: an absent term (or the unused diagonal of a plain
                     // program); the sweep masks it, so the copy skips it

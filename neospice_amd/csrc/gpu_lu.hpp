@@ -50,12 +50,13 @@ struct DevProgram {
     // after (ascending) the levels; values in a sweep_n x sweep_n block of the
     // solve-ordered copy at sweep_sv (row = step, column = target; the step's
     // own diagonal on the block diagonal), presence bits per step and lane
-    // (bit m: target lane + 64 m)
+    // (bit m: target lane + 64 m; a full word each, so that the kernel's
+    // prefetch of them needs no conversion that would wait for the load)
     int32_t sweep_n = 0;
     int32_t sweep_first = 0;
     int32_t sweep_sv = 0;
     int32_t* sweep_tgt = nullptr;
-    uint8_t* sweep_bits = nullptr;
+    int32_t* sweep_bits = nullptr;
 };
 
 class GpuLu {
