@@ -27,7 +27,7 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kRfThreads = 1024;              // refactor workgroup: 16 waves
 constexpr int kRfWaves = kRfThreads / kWave;
-constexpr int kSvThreads = 512;               // solve workgroup
+constexpr int kSvThreads = 768;               // solve workgroup
 constexpr int kPf = 16;                       // U-entry steps whose operands are in flight
 constexpr int kContinued = 1 << 30;           // step descriptor: continuation chunk of an L column
 constexpr int kSpinMax = 1 << 24;              // bounded wait for a column (about 0.5 s)
