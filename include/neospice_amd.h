@@ -75,6 +75,12 @@ int nsa_circuit_model(nsa_circuit* c, const char* text);
 int nsa_circuit_d(nsa_circuit* c, const char* name, int32_t a, int32_t k, const char* model, int32_t* dev);
 int nsa_circuit_m(nsa_circuit* c, const char* name, int32_t d, int32_t g, int32_t s, int32_t b,
                   const char* model, double w, double l, int32_t* dev);
+/* The constants the BSIM4v7 load of device `dev` (a LEVEL=14 M device) uses,
+ * as the reference holds them after BSIM4v7temp in its model, size-dependent
+ * (pParam) and instance records (bsim4v7_temp.cpp:98-2317): NSA_BSIM4_NPARAM
+ * doubles, in the order documented in neospice_amd/csrc/bsim4.hpp (B4Param). */
+#define NSA_BSIM4_NPARAM 110
+int nsa_circuit_bsim4_params(const nsa_circuit* c, int32_t dev, double* out);
 /* Circuit::finalize (src/core/circuit.cpp:201-283). */
 int nsa_circuit_finalize(nsa_circuit* c);
 /* n (unknowns), nodes, nnz, states, is_linear */

@@ -47,6 +47,7 @@ _SIGS = {
     "nsa_circuit_d": (C.c_int, [VP, C.c_char_p, I32, I32, C.c_char_p, P_I32]),
     "nsa_circuit_m": (C.c_int, [VP, C.c_char_p, I32, I32, I32, I32, C.c_char_p, C.c_double,
                                 C.c_double, P_I32]),
+    "nsa_circuit_bsim4_params": (C.c_int, [VP, I32, P_F64]),
     "nsa_circuit_finalize": (C.c_int, [VP]),
     "nsa_circuit_info": (C.c_int, [VP, P_I32]),
     "nsa_circuit_pattern": (C.c_int, [VP, P_I32, P_I32]),

@@ -18,6 +18,7 @@ OBJ = os.path.join(HERE, "_build")
 SOURCES = [
     "circuit.cpp",
     "devices.cpp",
+    "bsim4.cpp",
     "lu_structure.cpp",
     "lu_cpu.cpp",
     "batch_lu.cpp",
@@ -27,6 +28,7 @@ SOURCES = [
     "gpu_lu.hip",
     "gpu_dev.hip",
     "gpu_nonlinear.hip",
+    "gpu_bsim4.hip",
     "gpu_adjoint.hip",
 ]
 

@@ -1,0 +1,61 @@
+// BSIM4v7 (LEVEL=14) per-device constants for the GPU load, computed once
+// on the host: the model defaulting of BSIM4v7setup
+// (src/devices/bsim4v7/bsim4v7_setup.cpp:60-1500), the size-dependent and
+// instance part of BSIM4v7temp (bsim4v7_temp.cpp:98-2317) and the geometry
+// of BSIM4v7PAeffGeo (bsim4v7_geo.cpp:63-161).
+//
+// Scope: the model's default operating modes.  Cards are accepted when
+// every parameter they give is in a list of value-only parameters (the
+// threshold, mobility, velocity, output-resistance, short-channel and
+// overlap parameters below), which change numbers but not which branches of
+// BSIM4v7temp and BSIM4v7load run.  Everything else -- mode selectors
+// (capMod, mobMod, rdsMod, rgateMod, rbodyMod, trnqsMod, igcMod, igbMod,
+// gidlMod, tempMod, mtrlMod ...), binning (L/W/P prefixed), junction,
+// gate-current, GIDL, impact-ionisation, stress and temperature parameters
+// -- is refused with NSA_ERR_UNSUPPORTED, as are instances other than W, L
+// with NF = 1 and simulation temperatures other than TNOM.  Under these
+// cards the following terms of BSIM4v7load are identically zero and are not
+// restated: substrate current, GIDL/GISL, gate tunnelling currents, the
+// trap-assisted junction tunnelling currents, source/drain junction
+// currents beyond gmin (zero area, zero sidewall saturation current),
+// external source/drain and gate resistances, NQS.
+//
+// The arithmetic is the reference's, in its order, with glibc
+// transcendentals, so every entry is bit-equal to the field the reference
+// holds in its model, size-dependent (pParam) or instance record.
+#pragma once
+#include <cstdint>
+
+#include "circuit.hpp"
+#include "devices.hpp"
+
+namespace nsa {
+
+struct B4Param {
+    // model
+    double type, vtm, coxe, coxp, toxe, toxp, factor1, vcrit, bdos, ados, temp_ratio;
+    double PhiBS, PhiBSWS, PhiBSWGS, PhiBD, PhiBSWD, PhiBSWGD;
+    double MJS, MJSWS, MJSWGS, MJD, MJSWD, MJSWGD;
+    // junction capacitances of this instance (zero-bias, area/perimeter applied)
+    double czbs, czbssw, czbsswg, czbd, czbdsw, czbdswg;
+    // size-dependent (pParam)
+    double leff, weff, leffCV, weffCV, phi, sqrtPhi, Xdep0, vbi, k1, k1ox, k3, k3b;
+    double dvt0, dvt1, dvt2, dvt0w, dvt1w, dvt2w, lpe0, lpeb, kt1, kt1l, kt2, w0;
+    double theta0vb0, etab, nfactor, cdsc, cdscb, cdscd, cit, mstar, voffcbn, cdep0;
+    double dwg, dwb, prwg, prwb, rds0, rdswmin, xj, a0, ags, b0, b1, keta;
+    double ua, ub, uc, ud, a2, delta, pvag, pclm, litl, thetaRout, pdiblb, pditsd;
+    double pscbe1, pscbe2, noff, voffcv, abulkCVfactor, acde, ldeb, moin;
+    double cgdo, cgso, cgbo, cgdl, cgsl, ckappad, ckappas;
+    // instance
+    double vth0, k2ox, eta0, vbsc, u0temp, vsattemp, vtfbphi2, vfbzb;
+    int32_t off, pad;
+};
+
+// Throws Error(NSA_ERR_UNSUPPORTED) for a card, instance or temperature
+// outside the scope above and Error(NSA_ERR_ARGUMENT) for an invalid one.
+B4Param bsim4_param(const ModelCard& card, const Device& d, const DeviceEnv& env);
+
+// Whether a model card selects BSIM4v7 (LEVEL=14, mosfet_common.cpp:479-481).
+inline bool is_bsim4_level(int level) { return level == 14; }
+
+}  // namespace nsa

@@ -8,6 +8,7 @@
 #include "batch_lu.hpp"
 #include "circuit.hpp"
 #include "common.hpp"
+#include "bsim4.hpp"
 #include "engine.hpp"
 #include "lu_cpu.hpp"
 
@@ -142,6 +143,20 @@ int nsa_circuit_d(nsa_circuit* c, const char* name, int32_t a, int32_t k, const 
 int nsa_circuit_m(nsa_circuit* c, const char* name, int32_t d, int32_t g, int32_t s, int32_t b,
                   const char* model, double w, double l, int32_t* dev) {
     return guard([&] { return set_dev(dev, c->c.M(name, d, g, s, b, model, w, l)); });
+}
+int nsa_circuit_bsim4_params(const nsa_circuit* c, int32_t dev, double* out) {
+    return guard([&] {
+        static_assert(sizeof(nsa::B4Param) == NSA_BSIM4_NPARAM * sizeof(double) + 8,
+                      "B4Param layout");
+        if (!out) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "out is NULL");
+        const auto& devs = c->c.devices();
+        if (dev < 0 || dev >= (int32_t)devs.size() || devs[dev].kind != nsa::DK_B4)
+            throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "not a BSIM4v7 device");
+        const nsa::B4Param p =
+            nsa::bsim4_param(c->c.models().at(devs[dev].model), devs[dev], nsa::DeviceEnv{});
+        std::memcpy(out, &p, NSA_BSIM4_NPARAM * sizeof(double));
+        return 0;
+    });
 }
 int nsa_circuit_finalize(nsa_circuit* c) {
     return guard([&] { c->c.finalize(); return 0; });

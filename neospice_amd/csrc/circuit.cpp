@@ -1,5 +1,7 @@
 #include "circuit.hpp"
 
+#include "bsim4.hpp"
+
 #include <algorithm>
 #include <cctype>
 #include <cmath>
@@ -80,6 +82,7 @@ ModelCard parse_model_card(const std::string& text) {
 int32_t load_rank(DevKind k) {
     switch (k) {
         case DK_M1: return 3;
+        case DK_B4: return 3;   // "M" family, like MOS1
         case DK_C: return 17;
         case DK_D: return 22;
         case DK_I: return 31;
@@ -94,6 +97,7 @@ int32_t state_count(DevKind k) {
         case DK_C: return 2;    // capacitor.hpp:61
         case DK_D: return 5;    // dio_device.hpp:50
         case DK_M1: return 17;  // mos1_device.hpp:64
+        case DK_B4: return 29;  // BSIM4v7numStates (bsim4v7_def.hpp:545)
         default: return 0;
     }
 }
@@ -181,9 +185,11 @@ int32_t Circuit::model(const std::string& text) {
         if (m.get("rs", 0.0) != 0.0)
             throw Error(NSA_ERR_UNSUPPORTED, "diode RS != 0 (internal node) is not supported");
     } else if (m.type == "nmos" || m.type == "pmos") {
-        if (m.level != 1)
-            throw Error(NSA_ERR_UNSUPPORTED, "only MOSFET LEVEL=1 is supported by this build");
-        if (m.get("rd", 0.0) != 0.0 || m.get("rs", 0.0) != 0.0 || m.get("rsh", 0.0) != 0.0)
+        if (m.level != 1 && !is_bsim4_level(m.level))
+            throw Error(NSA_ERR_UNSUPPORTED,
+                        "only MOSFET LEVEL=1 (MOS1) and LEVEL=14 (BSIM4v7) are supported by this build");
+        if (m.level == 1 &&
+            (m.get("rd", 0.0) != 0.0 || m.get("rs", 0.0) != 0.0 || m.get("rsh", 0.0) != 0.0))
             throw Error(NSA_ERR_UNSUPPORTED, "MOS1 RD/RS/RSH (internal nodes) are not supported");
     } else {
         throw Error(NSA_ERR_UNSUPPORTED, "unsupported model type " + m.type);
@@ -216,8 +222,18 @@ int32_t Circuit::M(const std::string& name, int32_t dn, int32_t g, int32_t s, in
     const int32_t mi = find_model(model);
     if (mi < 0 || (models_[mi].type != "nmos" && models_[mi].type != "pmos"))
         throw Error(NSA_ERR_ARGUMENT, "M device: model not found or wrong type: " + model);
+    if (is_bsim4_level(models_[mi].level)) {
+        Device d{DK_B4, name};
+        d.node[0] = dn; d.node[1] = g; d.node[2] = s; d.node[3] = b;
+        d.model = mi;
+        d.w = w; d.l = l;
+        d.w_given = true; d.l_given = true;
+        bsim4_param(models_[mi], d, DeviceEnv{});  // refuses cards outside the covered subset
+        return add(d);
+    }
     if (models_[mi].level != 1)
-        throw Error(NSA_ERR_UNSUPPORTED, "M device " + name + ": only LEVEL=1 (MOS1) is covered");
+        throw Error(NSA_ERR_UNSUPPORTED,
+                    "M device " + name + ": only LEVEL=1 (MOS1) and LEVEL=14 (BSIM4v7) are covered");
     if (card_needs_internal_nodes(models_[mi]))
         throw Error(NSA_ERR_UNSUPPORTED,
                     "M device " + name + ": RD/RS/RSH != 0 (internal nodes) not covered");
@@ -231,7 +247,7 @@ int32_t Circuit::M(const std::string& name, int32_t dn, int32_t g, int32_t s, in
 
 bool Circuit::is_linear() const {
     for (const auto& d : devs_)
-        if (d.kind == DK_D || d.kind == DK_M1) return false;
+        if (d.kind == DK_D || d.kind == DK_M1 || d.kind == DK_B4) return false;
     return true;
 }
 
@@ -254,6 +270,11 @@ namespace {
 //   posPrimePosPrime, posPosPrime, negPosPrime, posPrimePos, posPrimeNeg.
 // MOS1load (mos1_load.cpp:846-883): rhs g, b, dPrime, sPrime; then the 22
 //   matrix stamps in the order listed there.
+// BSIM4v7load (bsim4v7_load.cpp:4883-5024): rhs dPrime, gPrime, bPrime,
+//   sPrime; then the GP, DP, SP and BP rows (columns gp/dp/sp/bp within each
+//   row's order there).  With no internal nodes the primed nodes are the
+//   terminals; the D/S external-node stamps and the GIDL/GISL stamps are
+//   identically zero under the covered cards and are not lowered.
 void stamp_lists(const Device& d, std::vector<std::pair<int32_t, int32_t>>& mat,
                  std::vector<int32_t>& rhs) {
     mat.clear();
@@ -286,6 +307,13 @@ void stamp_lists(const Device& d, std::vector<std::pair<int32_t, int32_t>>& mat,
                    {D, DP},  {G, B},   {G, DP},  {G, SP},  {S, SP},  {B, G},
                    {B, DP},  {B, SP},  {DP, D},  {DP, G},  {DP, B},  {DP, SP},
                    {SP, G},  {SP, S},  {SP, B},  {SP, DP}};
+            break;
+        }
+        case DK_B4: {
+            const int32_t D = d.node[0], G = d.node[1], S = d.node[2], B = d.node[3];
+            rhs = {D, G, B, S};
+            mat = {{G, G}, {G, D}, {G, S}, {G, B}, {D, D}, {D, G}, {D, S}, {D, B},
+                   {S, D}, {S, G}, {S, S}, {S, B}, {B, D}, {B, G}, {B, S}, {B, B}};
             break;
         }
         default: break;

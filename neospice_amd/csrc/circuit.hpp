@@ -1,6 +1,6 @@
 // Circuit description and finalization, mirroring the reference's Circuit
 // (src/core/circuit.hpp:157-418) for the device families on the Newton hot
-// path: R, C, V, I, level-1 diode and MOS1.
+// path: R, C, V, I, level-1 diode, MOS1 and BSIM4v7.
 //
 // finalize() reproduces the reference's numbering and ordering exactly,
 // because they fix the MNA unknowns and the floating-point summation order
@@ -22,7 +22,9 @@
 
 namespace nsa {
 
-enum DevKind : int32_t { DK_R = 0, DK_C = 1, DK_V = 2, DK_I = 3, DK_D = 4, DK_M1 = 5, DK_COUNT = 6 };
+enum DevKind : int32_t {
+    DK_R = 0, DK_C = 1, DK_V = 2, DK_I = 3, DK_D = 4, DK_M1 = 5, DK_B4 = 6, DK_COUNT = 7
+};
 enum Wave : int32_t { WAVE_DC = 0, WAVE_PULSE = 1, WAVE_PWL = 2 };
 
 struct ModelCard {

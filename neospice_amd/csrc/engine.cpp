@@ -1,5 +1,7 @@
 #include "engine.hpp"
 
+#include "bsim4.hpp"
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -162,6 +164,20 @@ void BatchEngine::build_tables() {
         mos_state.push_back(devs[d].state_base);
         mos_par.push_back(mos1_param(ckt_.models().at(devs[d].model), devs[d], env));
     }
+    // BSIM4v7: 16 matrix sites, 4 rhs sites (BSIM4v7load stamp order)
+    const auto& fb = ckt_.family(DK_B4);
+    b4_.count = (int)fb.size();
+    b4_.site = site;
+    b4_.rsite = rsite;
+    std::vector<int32_t> b4_nodes, b4_state;
+    std::vector<B4Param> b4_par;
+    for (int d : fb) {
+        dev_site_[d] = site; site += 16;
+        dev_rsite_[d] = rsite; rsite += 4;
+        for (int k = 0; k < 4; ++k) b4_nodes.push_back(devs[d].node[k]);
+        b4_state.push_back(devs[d].state_base);
+        b4_par.push_back(bsim4_param(ckt_.models().at(devs[d].model), devs[d], env));
+    }
     const_site_ = site;
     std::vector<double> const_vals;
     for (int d : fv) {
@@ -204,7 +220,7 @@ void BatchEngine::build_tables() {
             if (dv.mat_slot[k] >= 0) slot_sites[dv.mat_slot[k]].push_back(dev_site_[d] + (int)k);
         // UCB devices stamp the circuit-global one-based rhs, which
         // newton_solve adds to rhs after all loads (newton.cpp:166-168)
-        auto& rows = (dv.kind == DK_D || dv.kind == DK_M1) ? rows_u : rows_d;
+        auto& rows = (dv.kind == DK_D || dv.kind == DK_M1 || dv.kind == DK_B4) ? rows_u : rows_d;
         for (size_t k = 0; k < dv.rhs_row.size(); ++k)
             if (dv.rhs_row[k] >= 0) rows[dv.rhs_row[k]].push_back(dev_rsite_[d] + (int)k);
     }
@@ -259,11 +275,19 @@ void BatchEngine::build_tables() {
     const size_t n_aux = (size_t)std::max(1, mos_.count) * kMos1Aux * B_;
     mos_.aux = dalloc<double>(allocs_, n_aux);
     NSA_HIP(hipMemset(mos_.aux, 0, n_aux * sizeof(double)));
-    // truncation-error rows: the capacitors, the diodes, then the MOS1 devices
+    b4_.nodes = dupload(allocs_, b4_nodes);
+    b4_.state = dupload(allocs_, b4_state);
+    b4_.par = dupload(allocs_, b4_par);
+    const size_t n_b4aux = (size_t)std::max(1, b4_.count) * kB4Aux * B_;
+    b4_.aux = dalloc<double>(allocs_, n_b4aux);
+    NSA_HIP(hipMemset(b4_.aux, 0, n_b4aux * sizeof(double)));
+    // truncation-error rows: the capacitors, the diodes, the MOS1, then the
+    // BSIM4v7 devices
     std::vector<int32_t> tbase, tkind;
     for (int d : fc) { trunc_dev_.push_back(d); tbase.push_back(devs[d].state_base); tkind.push_back(0); }
     for (int d : fd) { trunc_dev_.push_back(d); tbase.push_back(devs[d].state_base); tkind.push_back(2); }
     for (int d : fm) { trunc_dev_.push_back(d); tbase.push_back(devs[d].state_base); tkind.push_back(1); }
+    for (int d : fb) { trunc_dev_.push_back(d); tbase.push_back(devs[d].state_base); tkind.push_back(3); }
     trunc_.ndev = (int)trunc_dev_.size();
     trunc_.base = dupload(allocs_, tbase);
     trunc_.kind = dupload(allocs_, tkind);
@@ -316,6 +340,8 @@ void BatchEngine::load(const int32_t* d_done) {
     launch_diode(dio_, d_ctx_, d_x_, d_s_[0], d_s_[1], d_s_[2], d_done, d_cm_, d_cr_, d_devnc_,
                  tol_, B_, stream_);
     launch_mos1(mos_, d_ctx_, d_x_, d_s_[0], d_s_[1], d_s_[2], d_done, d_cm_, d_cr_, d_devnc_,
+                tol_, B_, stream_);
+    launch_bsim4(b4_, d_ctx_, d_x_, d_s_[0], d_s_[1], d_s_[2], d_done, d_cm_, d_cr_, d_devnc_,
                 tol_, B_, stream_);
     launch_gather(nnz_, d_mptr_, d_msites_, nullptr, nullptr, d_cm_, nullptr, d_A_, B_, stream_);
     launch_gather(n_, d_rptr_d_, d_rsites_d_, d_rptr_u_, d_rsites_u_, d_cr_, d_cr_, d_rhs_, B_,

@@ -162,6 +162,7 @@ private:
     IsrcTable isrc_{};
     DioTable dio_{};
     Mos1Table mos_{};
+    B4Table b4_{};
     DevTol tol_{};
     // per-instance device non-convergence of the current iteration
     // (CKTnoncon of the loads, device_converged after the solve)

@@ -284,6 +284,10 @@ __global__ void k_trunc(TruncTable t, const DevCtx* __restrict__ ctx, const doub
             dt = terr(base, s0, s1, s2, s3, B, inst, c, o, dt);
         } else if (t.kind[dev] == 2) {
             dt = terr(base + 3, s0, s1, s2, s3, B, inst, c, o, dt);   // DIOcapCharge
+        } else if (t.kind[dev] == 3) {
+            // BSIM4v7 qb, qg, qd (bsim4v7_device.cpp:1015-1017)
+            const int qoff[3] = {11, 13, 15};
+            for (int k = 0; k < 3; ++k) dt = terr(base + qoff[k], s0, s1, s2, s3, B, inst, c, o, dt);
         } else {
             const int qoff[5] = {5, 8, 11, 13, 15};   // qgs, qgd, qgb, qbd, qbs
             for (int k = 0; k < 5; ++k) dt = terr(base + qoff[k], s0, s1, s2, s3, B, inst, c, o, dt);

@@ -98,6 +98,21 @@ struct Mos1Table {
     int32_t site, rsite;
 };
 
+// BSIM4v7: the instance fields BSIM4v7load reads back on the next load
+// (the convergence test's linearisation and the limiting's von).
+enum B4Aux : int32_t {
+    B4_VON = 0, B4_MODE, B4_CD, B4_CBS, B4_CBD, B4_GBS, B4_GBD, B4_GM, B4_GDS, B4_GMBS, kB4Aux
+};
+struct B4Param;
+struct B4Table {
+    int32_t count;
+    const int32_t* nodes;   // 4 per device: d, g, s, b
+    const int32_t* state;   // state base per device (29 slots)
+    const B4Param* par;     // per device
+    double* aux;            // count x kB4Aux x B
+    int32_t site, rsite;    // 16 matrix sites, 4 rhs sites per device
+};
+
 // Tolerances a device load reads from SimOptions.
 struct DevTol {
     double gmin, reltol, abstol;
@@ -121,6 +136,11 @@ void launch_diode(const DioTable& t, const DevCtx* ctx, const double* x, double*
 void launch_mos1(const Mos1Table& t, const DevCtx* ctx, const double* x, double* s0, double* s1,
                  const double* s2, const int32_t* done, double* cm, double* cr, int32_t* devnc,
                  DevTol tol, int B, hipStream_t s);
+// BSIM4v7load for the default-mode cards (bsim4v7_load.cpp:69-5104); the
+// load's own current test sets devnc (device_converged).
+void launch_bsim4(const B4Table& t, const DevCtx* ctx, const double* x, double* s0, double* s1,
+                  const double* s2, const int32_t* done, double* cm, double* cr, int32_t* devnc,
+                  DevTol tol, int B, hipStream_t s);
 // MOS1Device::conv_test on the proposed solution x (mos1_device.cpp:376-416).
 void launch_mos1_conv(const Mos1Table& t, const double* x, const double* s0,
                       const int32_t* skip, int32_t* devnc, DevTol tol, int B, hipStream_t s);
@@ -144,7 +164,7 @@ void launch_fill_i32(int32_t* p, int32_t v, int count, hipStream_t s);
 struct TruncTable {
     int ndev;
     const int32_t* base;   // state base per device
-    const int32_t* kind;   // 0 capacitor, 1 MOS1, 2 diode
+    const int32_t* kind;   // 0 capacitor, 1 MOS1, 2 diode, 3 BSIM4v7
 };
 struct TruncOpts {
     double abstol, reltol, chgtol, trtol;

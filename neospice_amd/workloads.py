@@ -125,3 +125,51 @@ def inverter_chain(stages=50, vdd=1.8, r_load=1e3, c_load=2e-15):
         ckt.C("cl%d" % k, l, GND, c_load)
         drive = l
     return ckt, {"stages": stages}
+
+
+def ring_oscillator(stages=31, vdd=1.0, wn=0.5e-6, wp=1.0e-6, l=0.05e-6, toxe=1.8e-9, vth0=0.4,
+                    mismatch_rng=None, sigma_vth0=0.03, sigma_toxe=0.01, kick=True):
+    """BSIM4v7 ring oscillator: `stages` (odd) CMOS inverters in a loop,
+    NMOS W/L = wn/l and PMOS wp/l on the default LEVEL=14 card with TOXE and
+    VTH0 +-vth0, VDD = vdd.  With `mismatch_rng`, every transistor gets its
+    own card with Gaussian relative mismatch of sigma_vth0 on VTH0 and
+    sigma_toxe on TOXE.  A ring's only operating point is metastable (Newton
+    does not find it for long rings, in the reference either), so `kick`
+    adds an enable transistor that holds stage 0's input low until 20 ps:
+    the operating point is then a settled chain, and the ring oscillates
+    once the enable releases.  Unknowns: stages + vdd (+ enable node) +
+    source branches."""
+    ckt = Circuit()
+    shared = mismatch_rng is None
+    if shared:
+        ckt.model(".model nmod nmos level=14 toxe=%.17g vth0=%.17g" % (toxe, vth0))
+        ckt.model(".model pmod pmos level=14 toxe=%.17g vth0=%.17g" % (toxe, -vth0))
+    nvdd = ckt.node("vdd")
+    ckt.V("vsup", nvdd, GND, vdd)
+    nodes = [ckt.node("n%d" % k) for k in range(stages)]
+    for k in range(stages):
+        a, o = nodes[k], nodes[(k + 1) % stages]
+        names = []
+        for pol, sign in (("n", 1.0), ("p", -1.0)):
+            if shared:
+                names.append(pol + "mod")
+            else:
+                dv = 1.0 + sigma_vth0 * mismatch_rng.standard_normal()
+                dt = 1.0 + sigma_toxe * mismatch_rng.standard_normal()
+                nm = "%s%d" % (pol, k)
+                ckt.model(".model %s %smos level=14 toxe=%.17g vth0=%.17g" %
+                          (nm, pol, toxe * dt, sign * vth0 * dv))
+                names.append(nm)
+        ckt.M("mn%d" % k, o, a, GND, GND, names[0], wn, l)
+        ckt.M("mp%d" % k, o, a, nvdd, nvdd, names[1], wp, l)
+    if kick:
+        # enable: a wide NMOS holds stage 0's input low until its gate falls
+        # (t = 20 ps), which gives the ring a well-defined operating point
+        ne = ckt.node("en")
+        ckt.V_pulse("ven", ne, GND, vdd, 0.0, 20e-12, 10e-12, 10e-12, 1.0, 2.0)
+        if shared:
+            ckt.M("mrst", nodes[0], ne, GND, GND, "nmod", 8 * wn, l)
+        else:
+            ckt.model(".model nrst nmos level=14 toxe=%.17g vth0=%.17g" % (toxe, vth0))
+            ckt.M("mrst", nodes[0], ne, GND, GND, "nrst", 8 * wn, l)
+    return ckt, {"stages": stages, "nodes": nodes}

@@ -45,6 +45,7 @@
 #include "solver/matrix.hpp"
 #include "core/neo_solver.hpp"
 #include "core/amd_lu_solver.hpp"
+#include "devices/bsim4v7/bsim4v7_device.hpp"
 #undef private
 #undef protected
 
@@ -571,6 +572,71 @@ int ref_tran_steps(void* p, double dt, int nsteps, double* solutions_out, double
 
 int ref_tran_trunc(void* p, double dt, int nsteps, double* solutions_out, double* trunc_out) {
     return fixed_steps(p, dt, nsteps, solutions_out, nullptr, trunc_out);
+}
+
+// The BSIM4v7 constants of device `dev` after BSIM4v7temp (the device must
+// have been evaluated once), in the order of the product's B4Param record:
+// model, junction capacitances, size-dependent (pParam), instance fields.
+int ref_bsim4_fields(void* p, int dev, double* out) {
+    auto* h = static_cast<RefCkt*>(p);
+    return guarded([&] {
+        auto* b = dynamic_cast<BSIM4v7Device*>(h->ckt.devices().at(dev).get());
+        if (!b || !b->temp_done_) throw std::runtime_error("not an evaluated BSIM4v7 device");
+        const auto* m = b->model_;
+        const auto& i = b->inst_;
+        const auto* q = i.pParam;
+        const double nf = i.BSIM4v7nf;
+        const double v[] = {
+            (double)m->BSIM4v7type, m->BSIM4v7vtm, m->BSIM4v7coxe, m->BSIM4v7coxp, m->BSIM4v7toxe,
+            m->BSIM4v7toxp, m->BSIM4v7factor1, m->BSIM4v7vcrit, m->BSIM4v7bdos, m->BSIM4v7ados,
+            h->ckt.options.temp / m->BSIM4v7tnom - 1.0,
+            m->BSIM4v7PhiBS, m->BSIM4v7PhiBSWS, m->BSIM4v7PhiBSWGS, m->BSIM4v7PhiBD,
+            m->BSIM4v7PhiBSWD, m->BSIM4v7PhiBSWGD,
+            m->BSIM4v7SbulkJctBotGradingCoeff, m->BSIM4v7SbulkJctSideGradingCoeff,
+            m->BSIM4v7SbulkJctGateSideGradingCoeff, m->BSIM4v7DbulkJctBotGradingCoeff,
+            m->BSIM4v7DbulkJctSideGradingCoeff, m->BSIM4v7DbulkJctGateSideGradingCoeff,
+            m->BSIM4v7SunitAreaTempJctCap * i.BSIM4v7Aseff,
+            m->BSIM4v7SunitLengthSidewallTempJctCap * i.BSIM4v7Pseff,
+            m->BSIM4v7SunitLengthGateSidewallTempJctCap * q->BSIM4v7weffCJ * nf,
+            m->BSIM4v7DunitAreaTempJctCap * i.BSIM4v7Adeff,
+            m->BSIM4v7DunitLengthSidewallTempJctCap * i.BSIM4v7Pdeff,
+            m->BSIM4v7DunitLengthGateSidewallTempJctCap * q->BSIM4v7weffCJ * nf,
+            q->BSIM4v7leff, q->BSIM4v7weff, q->BSIM4v7leffCV, q->BSIM4v7weffCV, q->BSIM4v7phi,
+            q->BSIM4v7sqrtPhi, q->BSIM4v7Xdep0, q->BSIM4v7vbi, q->BSIM4v7k1, q->BSIM4v7k1ox,
+            q->BSIM4v7k3, q->BSIM4v7k3b,
+            q->BSIM4v7dvt0, q->BSIM4v7dvt1, q->BSIM4v7dvt2, q->BSIM4v7dvt0w, q->BSIM4v7dvt1w,
+            q->BSIM4v7dvt2w, q->BSIM4v7lpe0, q->BSIM4v7lpeb, q->BSIM4v7kt1, q->BSIM4v7kt1l,
+            q->BSIM4v7kt2, q->BSIM4v7w0,
+            q->BSIM4v7theta0vb0, q->BSIM4v7etab, q->BSIM4v7nfactor, q->BSIM4v7cdsc,
+            q->BSIM4v7cdscb, q->BSIM4v7cdscd, q->BSIM4v7cit, q->BSIM4v7mstar, q->BSIM4v7voffcbn,
+            q->BSIM4v7cdep0,
+            q->BSIM4v7dwg, q->BSIM4v7dwb, q->BSIM4v7prwg, q->BSIM4v7prwb, q->BSIM4v7rds0,
+            q->BSIM4v7rdswmin, q->BSIM4v7xj, q->BSIM4v7a0, q->BSIM4v7ags, q->BSIM4v7b0,
+            q->BSIM4v7b1, q->BSIM4v7keta,
+            q->BSIM4v7ua, q->BSIM4v7ub, q->BSIM4v7uc, q->BSIM4v7ud, q->BSIM4v7a2, q->BSIM4v7delta,
+            q->BSIM4v7pvag, q->BSIM4v7pclm, q->BSIM4v7litl, q->BSIM4v7thetaRout,
+            q->BSIM4v7pdiblb, q->BSIM4v7pditsd,
+            q->BSIM4v7pscbe1, q->BSIM4v7pscbe2, q->BSIM4v7noff, q->BSIM4v7voffcv,
+            q->BSIM4v7abulkCVfactor, q->BSIM4v7acde, q->BSIM4v7ldeb, q->BSIM4v7moin,
+            q->BSIM4v7cgdo, q->BSIM4v7cgso, q->BSIM4v7cgbo, q->BSIM4v7cgdl, q->BSIM4v7cgsl,
+            q->BSIM4v7ckappad, q->BSIM4v7ckappas,
+            i.BSIM4v7vth0, i.BSIM4v7k2ox, i.BSIM4v7eta0, i.BSIM4v7vbsc, i.BSIM4v7u0temp,
+            i.BSIM4v7vsattemp, i.BSIM4v7vtfbphi2, i.BSIM4v7vfbzb};
+        static_assert(sizeof(v) / sizeof(v[0]) == 110, "field count");
+        std::memcpy(out, v, sizeof(v));
+        return 0;
+    });
+}
+
+// Devices whose device_converged(solution) is false after the last load.
+int ref_ckt_devnoncon(void* p, const double* solution) {
+    auto* h = static_cast<RefCkt*>(p);
+    return guarded([&] {
+        std::vector<double> sol(solution, solution + h->ckt.num_vars());
+        int bad = 0;
+        for (Device* dev : h->ckt.device_load_order()) bad += dev->device_converged(sol) ? 0 : 1;
+        return bad;
+    });
 }
 
 int ref_num_devices(void* p) { return (int)static_cast<RefCkt*>(p)->ckt.devices().size(); }

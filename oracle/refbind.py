@@ -45,6 +45,8 @@ def ref():
             "ref_ckt_d": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int, C.c_char_p]),
             "ref_ckt_m": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p,
                                     C.c_double, C.c_double]),
+            "ref_bsim4_fields": (C.c_int, [VP, C.c_int, P_F64]),
+            "ref_ckt_devnoncon": (C.c_int, [VP, P_F64]),
             "ref_ckt_ic": (C.c_int, [VP, C.c_int, C.c_double]),
             "ref_ckt_option": (C.c_int, [VP, C.c_char_p, C.c_double]),
             "ref_ckt_finalize": (C.c_int, [VP, C.c_int]),

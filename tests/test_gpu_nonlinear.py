@@ -161,3 +161,44 @@ def test_diode_transient_replay():
 def test_mos1_transient_replay():
     ckt, _ = mos_ckt(5)
     tran_parity(ckt, 10e-12, 1e-9)
+
+
+# ---- BSIM4v7 (LEVEL=14, default-mode cards)
+
+def ring_ckt(stages, mismatch_seed=None):
+    rng = np.random.default_rng(mismatch_seed) if mismatch_seed is not None else None
+    ckt, info = W.ring_oscillator(stages, mismatch_rng=rng)
+    ckt.finalize()
+    return ckt, info
+
+
+def test_bsim4_loads():
+    ckt, _ = ring_ckt(5, mismatch_seed=3)
+    load_sequence(ckt, 13)
+
+
+@pytest.mark.parametrize("stages", [5, 11])
+def test_bsim4_dc_op(stages):
+    ckt, _ = ring_ckt(stages, mismatch_seed=4)
+    dc_parity(ckt, 2)
+
+
+def test_bsim4_dc_op_long_ring():
+    """On a 31-stage ring the direct Newton from MODEINITJCT passes through
+    iterates of 1e11 V (the exponentials clamp), where the last-ulp
+    differences of the device libm against glibc change the path and the
+    iteration count; the operating point itself is unique and compared to
+    the DC bar."""
+    ckt, _ = ring_ckt(31, mismatch_seed=4)
+    eng = na.BatchEngine(ckt, batch=2)
+    x, its = eng.dc_op()
+    x_ref, its_ref, _ = RB.RefCircuit(ckt, probe=False).dc()
+    tol = 1e-9 * np.maximum(np.abs(x_ref), 1e-3) + 1e-12
+    for i in range(2):
+        assert (np.abs(x[:, i] - x_ref) <= tol).all(), (i, np.max(np.abs(x[:, i] - x_ref)))
+
+
+def test_bsim4_transient_replay():
+    # the reference's own adaptive step sequence (steps capped at 1 ps)
+    ckt, _ = ring_ckt(7, mismatch_seed=5)
+    tran_parity(ckt, 1e-12, 0.3e-9)
