@@ -194,6 +194,88 @@ def adjoint_c5(na, W, mesh=32, batch=128, nsteps=500, dt=2e-12, ckpt=25, nprobe=
                     "and transposed solve per step)"}
 
 
+def bsim4_ring(na, W, stages=117, batch=256, steps=30, warmup=3, b1_steps=10, dt=1e-12,
+               cpu_steps=10, seed=31):
+    """The BSIM4v7 ring-oscillator batch: `batch` instances of a `stages`-stage
+    ring (LEVEL=14 cards, TOXE 1.8 nm, VTH0 +-0.4 V, W/L 0.5u/0.05u and
+    1u/0.05u, VDD 1 V) with per-transistor, per-instance Gaussian mismatch of
+    3% on VTH0 and 1% on TOXE, fixed 1 ps backward-Euler steps (the adaptive
+    controller's cap).  Batched and batch-1 Newton iterations/s, and the
+    reference CPU path on instance 0 (its own fixed-step loop) for parity and
+    as the CPU baseline."""
+    import re
+    import torch
+    rng = np.random.default_rng(seed)
+    ckt, info = W.ring_oscillator(stages, mismatch_rng=np.random.default_rng(seed + 1))
+    ckt.finalize()
+    texts = {op[1].split()[1].lower(): op[1] for op in ckt.ops if op[0] == "model"}
+    devs = ckt.ops_devices()
+    mdevs = [i for i, op in enumerate(devs) if op[0] == "M" and op[1] != "mrst"]
+    mism = {}
+    for dv in mdevs:
+        text = texts[devs[dv][6].lower()]
+        for par, sig in (("vth0", 0.03), ("toxe", 0.01)):
+            base = float(re.search(par + r"=([-0-9.e+]+)", text).group(1))
+            v = base * (1.0 + sig * rng.standard_normal(batch))
+            v[0] = base          # instance 0 keeps the circuit's own cards
+            mism[(dv, par)] = v
+
+    def make(B):
+        eng = na.BatchEngine(ckt, batch=B, xmu=0.0, itl1=400)
+        for (dv, par), v in mism.items():
+            eng.set_model_values(dv, par, v[:B])
+        eng.tran_begin()
+        return eng
+
+    eng = make(batch)
+    log("bench: ring %d stages n=%d nnz=%d, batch %d, DC done" % (stages, ckt.n, ckt.nnz, batch))
+    run_steps(eng, na, dt, True, warmup, 0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    iters = run_steps(eng, na, dt, False, steps, warmup)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    eng.set_timing(True)
+    it_ph = run_steps(eng, na, dt, False, 3, warmup + steps)
+    torch.cuda.synchronize()
+    timing = eng.timing()
+    eng.set_timing(False)
+    nit = max(1, it_ph // batch)
+    x0 = eng.solution()[:, 0].copy()
+    total = warmup + steps + 3
+    e1 = make(1)
+    run_steps(e1, na, dt, True, warmup, 0)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    it1 = run_steps(e1, na, dt, False, b1_steps, warmup)
+    torch.cuda.synchronize()
+    w1 = time.perf_counter() - t1
+    out = {"workload": "bsim4v7_ring_%d_stages_be_1ps_b%d" % (stages, batch), "instances": batch,
+           "unknowns": ckt.n, "nnz": ckt.nnz, "devices_bsim4": len(mdevs) + 1,
+           "value": round(iters / wall, 2), "unit": "Newton iters/s",
+           "ms_per_step": round(wall / steps * 1e3, 4),
+           "batch1_gpu_value": round(it1 / w1, 2),
+           "speedup_vs_batch1_gpu": round((iters / wall) / (it1 / w1), 2),
+           "phases_ms_per_iter": {"load_gather": timing["load_ms"] / nit,
+                                  "refactor": timing["refactor_ms"] / nit,
+                                  "solve": timing["solve_ms"] / nit}}
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import refbind as RB
+        rc = RB.RefCircuit(ckt, probe=False)
+        xr, its_r, sec = rc.tran_fixed(dt, total)
+        out["parity_max_abs_diff_vs_reference"] = float(np.max(np.abs(x0 - xr)))
+        rc2 = RB.RefCircuit(ckt, probe=False)
+        _, its_c, sec_c = rc2.tran_fixed(dt, cpu_steps)
+        out["cpu_baseline"] = {"value": round(its_c / sec_c, 2), "unit": "Newton iters/s", "cores": 1,
+                               "kind": "reference",
+                               "sample": "instance 0, %d fixed 1 ps BE steps on one core" % cpu_steps}
+    except Exception as e:  # reported, not required
+        log("bench: ring reference leg unavailable:", e)
+    log("bench: ring", out)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,6 +287,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-ring", action="store_true")
+    ap.add_argument("--ring-stages", type=int, default=117)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of known size (for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -314,6 +398,10 @@ def main():
     if not args.no_adjoint:
         adjoint = adjoint_c5(na, W, nsteps=args.adjoint_steps)
 
+    ring = None
+    if not args.no_ring:
+        ring = bsim4_ring(na, W, stages=args.ring_stages)
+
     out = {
         "metric": "batched Newton iters/s vs batch-1 GPU @1e-9 rel; adjoint grad/forward time",
         "value": round(value, 2),
@@ -337,6 +425,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "adjoint": adjoint,
+        "bsim4_ring": ring,
     }
     print(json.dumps(out))
 

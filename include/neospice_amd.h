@@ -94,6 +94,13 @@ int nsa_engine_create(nsa_circuit* c, int32_t batch, const nsa_options* opt, nsa
 void nsa_engine_destroy(nsa_engine* e);
 /* Per-instance primary value of device `dev` (R ohms, C farads, V/I dc), B values. */
 int nsa_engine_set_values(nsa_engine* e, int32_t dev, const double* vals);
+/* Per-instance value (vals[B]) of card parameter `param` of BSIM4v7 device
+ * `dev` -- the .step / Monte-Carlo form of a per-instance model card (the
+ * reference evaluates one card per device; instance i here is the reference
+ * circuit whose card for `dev` gives `param` = vals[i]).  `param` is one of
+ * the card value parameters the BSIM4v7 build covers; repeated calls
+ * accumulate per instance. */
+int nsa_engine_set_model_values(nsa_engine* e, int32_t dev, const char* param, const double* vals);
 /* per_instance = 0: ctx[0] for all instances; 1: ctx[inst]. */
 int nsa_engine_set_ctx(nsa_engine* e, const nsa_ctx* ctx, int32_t per_instance);
 /* newton_solve with iteration limit itl; converged/iterations per instance. */

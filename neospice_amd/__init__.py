@@ -222,6 +222,13 @@ class BatchEngine:
         v = as_f64(np.broadcast_to(np.asarray(vals, np.float64), (self.B,)))
         check(lib().nsa_engine_set_values(self.h, int(dev), f64p(v)))
 
+    def set_model_values(self, dev, param, vals):
+        """Per-instance value of a BSIM4v7 card parameter of device `dev`."""
+        if isinstance(dev, str):
+            dev = self.circuit.device_index(dev)
+        v = as_f64(np.broadcast_to(np.asarray(vals, np.float64), (self.B,)))
+        check(lib().nsa_engine_set_model_values(self.h, int(dev), param.encode(), f64p(v)))
+
     def set_ctx(self, ctx):
         if isinstance(ctx, Ctx):
             check(lib().nsa_engine_set_ctx(self.h, C.byref(ctx), 0))

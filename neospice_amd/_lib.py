@@ -56,6 +56,7 @@ _SIGS = {
     "nsa_engine_create": (C.c_int, [VP, I32, C.POINTER(Options), C.POINTER(VP)]),
     "nsa_engine_destroy": (None, [VP]),
     "nsa_engine_set_values": (C.c_int, [VP, I32, P_F64]),
+    "nsa_engine_set_model_values": (C.c_int, [VP, I32, C.c_char_p, P_F64]),
     "nsa_engine_set_ctx": (C.c_int, [VP, C.POINTER(Ctx), I32]),
     "nsa_engine_newton": (C.c_int, [VP, I32, P_I32, P_I32]),
     "nsa_engine_get_solution": (C.c_int, [VP, P_F64]),

@@ -51,6 +51,9 @@ struct B4Param {
     int32_t off, pad;
 };
 
+// The doubles before `off`: the record's numeric fields.
+constexpr int kB4Fields = 110;
+
 // Throws Error(NSA_ERR_UNSUPPORTED) for a card, instance or temperature
 // outside the scope above and Error(NSA_ERR_ARGUMENT) for an invalid one.
 B4Param bsim4_param(const ModelCard& card, const Device& d, const DeviceEnv& env);

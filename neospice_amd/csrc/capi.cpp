@@ -203,6 +203,13 @@ void nsa_engine_destroy(nsa_engine* e) { delete e; }
 int nsa_engine_set_values(nsa_engine* e, int32_t dev, const double* vals) {
     return guard([&] { e->e->set_device_values(dev, vals); return 0; });
 }
+int nsa_engine_set_model_values(nsa_engine* e, int32_t dev, const char* param, const double* vals) {
+    return guard([&] {
+        if (!param || !vals) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "param/vals is NULL");
+        e->e->set_model_values(dev, param, vals);
+        return 0;
+    });
+}
 int nsa_engine_set_ctx(nsa_engine* e, const nsa_ctx* ctx, int32_t per_instance) {
     return guard([&] {
         static_assert(sizeof(nsa_ctx) == sizeof(nsa::DevCtx), "ctx layout");

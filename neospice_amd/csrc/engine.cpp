@@ -3,6 +3,7 @@
 #include "bsim4.hpp"
 
 #include <algorithm>
+#include <cctype>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -278,6 +279,8 @@ void BatchEngine::build_tables() {
     b4_.nodes = dupload(allocs_, b4_nodes);
     b4_.state = dupload(allocs_, b4_state);
     b4_.par = dupload(allocs_, b4_par);
+    b4_host_ = b4_par;
+    env_ = env;
     const size_t n_b4aux = (size_t)std::max(1, b4_.count) * kB4Aux * B_;
     b4_.aux = dalloc<double>(allocs_, n_b4aux);
     NSA_HIP(hipMemset(b4_.aux, 0, n_b4aux * sizeof(double)));
@@ -316,6 +319,40 @@ void BatchEngine::set_device_values(int dev, const double* vals) {
     NSA_HIP(hipMemcpyAsync(dst + (size_t)d.family_index * B_, v.data(), sizeof(double) * B_,
                            hipMemcpyHostToDevice, stream_));
     NSA_HIP(hipStreamSynchronize(stream_));
+}
+
+void BatchEngine::set_model_values(int dev, const std::string& param, const double* vals) {
+    const auto& devs = ckt_.devices();
+    if (dev < 0 || dev >= (int)devs.size() || devs[dev].kind != DK_B4)
+        throw Error(NSA_ERR_ARGUMENT, "set_model_values: not a BSIM4v7 device");
+    const Device& d = devs[dev];
+    const size_t per_dev = (size_t)kB4Fields * B_;
+    if (!d_b4_inst_) {
+        // switch the table to per-instance records, each a copy of the shared one
+        d_b4_inst_ = dalloc<double>(allocs_, per_dev * b4_.count);
+        std::vector<double> all(per_dev * b4_.count);
+        for (int k = 0; k < b4_.count; ++k) {
+            const double* f = reinterpret_cast<const double*>(&b4_host_[k]);
+            for (int j = 0; j < kB4Fields; ++j)
+                for (int i = 0; i < B_; ++i) all[k * per_dev + (size_t)j * B_ + i] = f[j];
+        }
+        NSA_HIP(hipMemcpy(d_b4_inst_, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice));
+        b4_over_.assign(b4_.count, std::vector<std::map<std::string, double>>(B_));
+        b4_.par_inst = d_b4_inst_;
+    }
+    std::string key = param;
+    for (auto& ch : key) ch = (char)std::tolower((unsigned char)ch);
+    std::vector<double> col(per_dev);
+    for (int i = 0; i < B_; ++i) {
+        b4_over_[d.family_index][i][key] = vals[i];
+        ModelCard card = ckt_.models().at(d.model);
+        for (const auto& kv : b4_over_[d.family_index][i]) card.p[kv.first] = kv.second;
+        const B4Param p = bsim4_param(card, d, env_);
+        const double* f = reinterpret_cast<const double*>(&p);
+        for (int j = 0; j < kB4Fields; ++j) col[(size_t)j * B_ + i] = f[j];
+    }
+    NSA_HIP(hipMemcpy(d_b4_inst_ + (size_t)d.family_index * per_dev, col.data(),
+                      per_dev * sizeof(double), hipMemcpyHostToDevice));
 }
 
 void BatchEngine::set_ctx(const DevCtx& c) {

@@ -8,10 +8,13 @@
 #include <hip/hip_runtime.h>
 
 #include <memory>
+#include <map>
+#include <string>
 #include <vector>
 
 #include "batch_lu.hpp"
 #include "circuit.hpp"
+#include "bsim4.hpp"
 #include "gpu_dev.hpp"
 
 namespace nsa {
@@ -46,6 +49,10 @@ public:
     // Per-instance override of a device's primary value (R ohms, C farads,
     // V/I dc); host array of B values.
     void set_device_values(int dev, const double* vals);
+    // Per-instance value of a BSIM4v7 card parameter of device dev (one of
+    // the card's value parameters): the device's constants are recomputed
+    // per instance with the parameter overridden by vals[inst].
+    void set_model_values(int dev, const std::string& param, const double* vals);
     // Integrator context: one for all instances, or per instance.
     void set_ctx(const DevCtx& c);
     void set_ctx(const std::vector<DevCtx>& c);
@@ -163,6 +170,10 @@ private:
     DioTable dio_{};
     Mos1Table mos_{};
     B4Table b4_{};
+    DeviceEnv env_{};
+    std::vector<B4Param> b4_host_;                              // shared records
+    std::vector<std::vector<std::map<std::string, double>>> b4_over_;  // per device, instance
+    double* d_b4_inst_ = nullptr;
     DevTol tol_{};
     // per-instance device non-convergence of the current iteration
     // (CKTnoncon of the loads, device_converged after the solve)

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstddef>
 
 #include "bsim4.hpp"
 #include "dev_limits.hpp"
@@ -41,6 +42,10 @@ constexpr double kDelta4 = 0.02;
 // CONSTvt0 (ucb_compat.hpp:12)
 constexpr double kVt0 = 1.38064852e-23 * 300.15 / 1.6021766208e-19;
 
+// field f of the device's constant record (pb, ps set in the kernel)
+#define PF(f) (pb[(offsetof(B4Param, f) / sizeof(double)) * ps])
+static_assert(kB4Fields == (int)(offsetof(B4Param, off) / sizeof(double)), "B4Param layout");
+
 // UCB MAX (ucb_compat.hpp:38-43)
 __device__ inline double ucb_max(double a, double b) { return (a > b) ? a : b; }
 
@@ -55,7 +60,11 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
     if (done && done[inst]) return;
     const size_t sB = (size_t)B;
     const DevCtx c = ctx[inst];
-    const B4Param& p = t.par[d];
+    // the device's constants: one shared record, or per-instance records
+    // stored field-major (a wave reads 64 consecutive instances of a field)
+    const double* pb = t.par_inst ? t.par_inst + (size_t)d * kB4Fields * sB + inst
+                                  : reinterpret_cast<const double*>(t.par + d);
+    const size_t ps = t.par_inst ? sB : 1;
     const int mode = c.mode;
     const size_t st = (size_t)t.state[d] * sB + inst;
     auto S = [&](int k) { return st + (size_t)k * sB; };
@@ -65,7 +74,7 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
     double* aux = t.aux + (size_t)d * kB4Aux * sB + inst;
     const int nD = t.nodes[4 * d], nG = t.nodes[4 * d + 1], nS = t.nodes[4 * d + 2],
               nB = t.nodes[4 * d + 3];
-    const double type = p.type;
+    const double type = PF(type);
     const bool charge = ((mode & (MODEDCTRANCURVE | MODEAC | MODETRAN | MODEINITSMSIG)) ||
                          ((mode & MODETRANOP) && (mode & MODEUIC)));
 
@@ -98,7 +107,7 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
         if ((vds == 0.0) && (vgs == 0.0) && (vbs == 0.0) &&
             ((mode & (MODETRAN | MODEAC | MODEDCOP | MODEDCTRANCURVE)) || (!(mode & MODEUIC)))) {
             vds = 0.1;
-            vgs = type * p.vth0 + 0.1;
+            vgs = type * PF(vth0) + 0.1;
             vbs = 0.0;
         }
     } else {
@@ -153,10 +162,10 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
             vgs = vgd + vds;
         }
         if (vds >= 0.0) {
-            vbs = pnjlim(vbs, s0vbs, kVt0, p.vcrit, &Check);
+            vbs = pnjlim(vbs, s0vbs, kVt0, PF(vcrit), &Check);
             vbd = vbs - vds;
         } else {
-            vbd = pnjlim(vbd, s0vbd, kVt0, p.vcrit, &Check);
+            vbd = pnjlim(vbd, s0vbd, kVt0, PF(vcrit), &Check);
             vbs = vbd + vds;
         }
     }
@@ -180,62 +189,62 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
         Vds = -vds;
         Vbs = vbd;
     }
-    const double toxe = p.toxe, epssub = kEpsSi;
+    const double toxe = PF(toxe), epssub = kEpsSi;
 
     // ---- Vbseff, Phis, Xdep (:1013-1038)
     double T0, T1, T2, T3, T4, T5, T6, T7, T8, T9, T10, T11, T12, T13;
     double Vbseff, dVbseff_dVb;
-    T0 = Vbs - p.vbsc - 0.001;
-    T1 = sqrt(T0 * T0 - 0.004 * p.vbsc);
+    T0 = Vbs - PF(vbsc) - 0.001;
+    T1 = sqrt(T0 * T0 - 0.004 * PF(vbsc));
     if (T0 >= 0.0) {
-        Vbseff = p.vbsc + 0.5 * (T0 + T1);
+        Vbseff = PF(vbsc) + 0.5 * (T0 + T1);
         dVbseff_dVb = 0.5 * (1.0 + T0 / T1);
     } else {
         T2 = -0.002 / (T1 - T0);
-        Vbseff = p.vbsc * (1.0 + T2);
-        dVbseff_dVb = T2 * p.vbsc / T1;
+        Vbseff = PF(vbsc) * (1.0 + T2);
+        dVbseff_dVb = T2 * PF(vbsc) / T1;
     }
-    T9 = 0.95 * p.phi;
+    T9 = 0.95 * PF(phi);
     T0 = T9 - Vbseff - 0.001;
     T1 = sqrt(T0 * T0 + 0.004 * T9);
     Vbseff = T9 - 0.5 * (T0 + T1);
     dVbseff_dVb *= 0.5 * (1.0 + T0 / T1);
-    const double Phis = p.phi - Vbseff;
+    const double Phis = PF(phi) - Vbseff;
     const double dPhis_dVb = -1.0;
     const double sqrtPhis = sqrt(Phis);
     const double dsqrtPhis_dVb = -0.5 / sqrtPhis;
-    const double Xdep = p.Xdep0 * sqrtPhis / p.sqrtPhi;
-    const double dXdep_dVb = (p.Xdep0 / p.sqrtPhi) * dsqrtPhis_dVb;
-    const double Leff = p.leff;
-    const double Vtm = p.vtm;
+    const double Xdep = PF(Xdep0) * sqrtPhis / PF(sqrtPhi);
+    const double dXdep_dVb = (PF(Xdep0) / PF(sqrtPhi)) * dsqrtPhis_dVb;
+    const double Leff = PF(leff);
+    const double Vtm = PF(vtm);
 
     // ---- Vth (:1044-1141)
     T3 = sqrt(Xdep);
-    const double V0 = p.vbi - p.phi;
-    T0 = p.dvt2 * Vbseff;
+    const double V0 = PF(vbi) - PF(phi);
+    T0 = PF(dvt2) * Vbseff;
     if (T0 >= -0.5) {
         T1 = 1.0 + T0;
-        T2 = p.dvt2;
+        T2 = PF(dvt2);
     } else {
         T4 = 1.0 / (3.0 + 8.0 * T0);
         T1 = (1.0 + 3.0 * T0) * T4;
-        T2 = p.dvt2 * T4 * T4;
+        T2 = PF(dvt2) * T4 * T4;
     }
-    const double lt1 = p.factor1 * T3 * T1;
-    const double dlt1_dVb = p.factor1 * (0.5 / T3 * T1 * dXdep_dVb + T3 * T2);
-    T0 = p.dvt2w * Vbseff;
+    const double lt1 = PF(factor1) * T3 * T1;
+    const double dlt1_dVb = PF(factor1) * (0.5 / T3 * T1 * dXdep_dVb + T3 * T2);
+    T0 = PF(dvt2w) * Vbseff;
     if (T0 >= -0.5) {
         T1 = 1.0 + T0;
-        T2 = p.dvt2w;
+        T2 = PF(dvt2w);
     } else {
         T4 = 1.0 / (3.0 + 8.0 * T0);
         T1 = (1.0 + 3.0 * T0) * T4;
-        T2 = p.dvt2w * T4 * T4;
+        T2 = PF(dvt2w) * T4 * T4;
     }
-    const double ltw = p.factor1 * T3 * T1;
-    const double dltw_dVb = p.factor1 * (0.5 / T3 * T1 * dXdep_dVb + T3 * T2);
+    const double ltw = PF(factor1) * T3 * T1;
+    const double dltw_dVb = PF(factor1) * (0.5 / T3 * T1 * dXdep_dVb + T3 * T2);
     double Theta0, dTheta0_dVb;
-    T0 = p.dvt1 * Leff / lt1;
+    T0 = PF(dvt1) * Leff / lt1;
     if (T0 < kExpThreshold) {
         T1 = exp(T0);
         T2 = T1 - 1.0;
@@ -248,11 +257,11 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
         Theta0 = 1.0 / (kMaxExp - 2.0);
         dTheta0_dVb = 0.0;
     }
-    const double thetavth = p.dvt0 * Theta0;
+    const double thetavth = PF(dvt0) * Theta0;
     const double Delt_vth = thetavth * V0;
-    const double dDelt_vth_dVb = p.dvt0 * dTheta0_dVb * V0;
+    const double dDelt_vth_dVb = PF(dvt0) * dTheta0_dVb * V0;
     double dT5_dVb;
-    T0 = p.dvt1w * p.weff * Leff / ltw;
+    T0 = PF(dvt1w) * PF(weff) * Leff / ltw;
     if (T0 < kExpThreshold) {
         T1 = exp(T0);
         T2 = T1 - 1.0;
@@ -265,14 +274,14 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
         T5 = 1.0 / (kMaxExp - 2.0);
         dT5_dVb = 0.0;
     }
-    T0 = p.dvt0w * T5;
+    T0 = PF(dvt0w) * T5;
     T2 = T0 * V0;
-    const double dT2_dVb_w = p.dvt0w * dT5_dVb * V0;
-    const double TempRatio = p.temp_ratio;
-    T0 = sqrt(1.0 + p.lpe0 / Leff);
-    T1 = p.k1ox * (T0 - 1.0) * p.sqrtPhi + (p.kt1 + p.kt1l / Leff + p.kt2 * Vbseff) * TempRatio;
-    const double Vth_NarrowW = toxe * p.phi / (p.weff + p.w0);
-    T3 = p.eta0 + p.etab * Vbseff;
+    const double dT2_dVb_w = PF(dvt0w) * dT5_dVb * V0;
+    const double TempRatio = PF(temp_ratio);
+    T0 = sqrt(1.0 + PF(lpe0) / Leff);
+    T1 = PF(k1ox) * (T0 - 1.0) * PF(sqrtPhi) + (PF(kt1) + PF(kt1l) / Leff + PF(kt2) * Vbseff) * TempRatio;
+    const double Vth_NarrowW = toxe * PF(phi) / (PF(weff) + PF(w0));
+    T3 = PF(eta0) + PF(etab) * Vbseff;
     if (T3 < 1.0e-4) {
         T9 = 1.0 / (3.0 - 2.0e4 * T3);
         T3 = (2.0e-4 - T3) * T9;
@@ -280,34 +289,34 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
     } else {
         T4 = 1.0;
     }
-    const double dDIBL_Sft_dVd = T3 * p.theta0vb0;
+    const double dDIBL_Sft_dVd = T3 * PF(theta0vb0);
     const double DIBL_Sft = dDIBL_Sft_dVd * Vds;
-    const double Lpe_Vb = sqrt(1.0 + p.lpeb / Leff);
-    const double Vth = type * p.vth0 + (p.k1ox * sqrtPhis - p.k1 * p.sqrtPhi) * Lpe_Vb -
-                       p.k2ox * Vbseff - Delt_vth - T2 + (p.k3 + p.k3b * Vbseff) * Vth_NarrowW +
+    const double Lpe_Vb = sqrt(1.0 + PF(lpeb) / Leff);
+    const double Vth = type * PF(vth0) + (PF(k1ox) * sqrtPhis - PF(k1) * PF(sqrtPhi)) * Lpe_Vb -
+                       PF(k2ox) * Vbseff - Delt_vth - T2 + (PF(k3) + PF(k3b) * Vbseff) * Vth_NarrowW +
                        T1 - DIBL_Sft;
-    const double dVth_dVb = Lpe_Vb * p.k1ox * dsqrtPhis_dVb - p.k2ox - dDelt_vth_dVb - dT2_dVb_w +
-                            p.k3b * Vth_NarrowW - p.etab * Vds * p.theta0vb0 * T4 +
-                            p.kt2 * TempRatio;
+    const double dVth_dVb = Lpe_Vb * PF(k1ox) * dsqrtPhis_dVb - PF(k2ox) - dDelt_vth_dVb - dT2_dVb_w +
+                            PF(k3b) * Vth_NarrowW - PF(etab) * Vds * PF(theta0vb0) * T4 +
+                            PF(kt2) * TempRatio;
     const double dVth_dVd = -dDIBL_Sft_dVd;
 
     // ---- n (:1145-1165)
     double n, dn_dVb, dn_dVd;
     {
         const double tmp1 = epssub / Xdep;
-        const double tmp2 = p.nfactor * tmp1;
-        const double tmp3 = p.cdsc + p.cdscb * Vbseff + p.cdscd * Vds;
-        const double tmp4 = (tmp2 + tmp3 * Theta0 + p.cit) / p.coxe;
+        const double tmp2 = PF(nfactor) * tmp1;
+        const double tmp3 = PF(cdsc) + PF(cdscb) * Vbseff + PF(cdscd) * Vds;
+        const double tmp4 = (tmp2 + tmp3 * Theta0 + PF(cit)) / PF(coxe);
         if (tmp4 >= -0.5) {
             n = 1.0 + tmp4;
-            dn_dVb = (-tmp2 / Xdep * dXdep_dVb + tmp3 * dTheta0_dVb + p.cdscb * Theta0) / p.coxe;
-            dn_dVd = p.cdscd * Theta0 / p.coxe;
+            dn_dVb = (-tmp2 / Xdep * dXdep_dVb + tmp3 * dTheta0_dVb + PF(cdscb) * Theta0) / PF(coxe);
+            dn_dVd = PF(cdscd) * Theta0 / PF(coxe);
         } else {
             T0 = 1.0 / (3.0 + 8.0 * tmp4);
             n = (1.0 + 3.0 * tmp4) * T0;
             T0 *= T0;
-            dn_dVb = (-tmp2 / Xdep * dXdep_dVb + tmp3 * dTheta0_dVb + p.cdscb * Theta0) / p.coxe * T0;
-            dn_dVd = p.cdscd * Theta0 / p.coxe * T0;
+            dn_dVb = (-tmp2 / Xdep * dXdep_dVb + tmp3 * dTheta0_dVb + PF(cdscb) * Theta0) / PF(coxe) * T0;
+            dn_dVd = PF(cdscd) * Theta0 / PF(coxe) * T0;
         }
     }
     const double von_new = Vth;
@@ -318,13 +327,13 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
     const double Vgst = Vgs_eff - Vth;
     double dT10_dVg, dT10_dVd, dT10_dVb, dT9_dVg, dT9_dVd, dT9_dVb;
     T0 = n * Vtm;
-    T1 = p.mstar * Vgst;
+    T1 = PF(mstar) * Vgst;
     T2 = T1 / T0;
     if (T2 > kExpThreshold) {
         T10 = T1;
-        dT10_dVg = p.mstar * dVgs_eff_dVg;
-        dT10_dVd = -dVth_dVd * p.mstar;
-        dT10_dVb = -dVth_dVb * p.mstar;
+        dT10_dVg = PF(mstar) * dVgs_eff_dVg;
+        dT10_dVd = -dVth_dVd * PF(mstar);
+        dT10_dVb = -dVth_dVb * PF(mstar);
     } else if (T2 < -kExpThreshold) {
         T10 = Vtm * log(1.0 + kMinExp);
         dT10_dVg = 0.0;
@@ -335,32 +344,32 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
         const double ExpVgst = exp(T2);
         T3 = Vtm * log(1.0 + ExpVgst);
         T10 = n * T3;
-        dT10_dVg = p.mstar * ExpVgst / (1.0 + ExpVgst);
+        dT10_dVg = PF(mstar) * ExpVgst / (1.0 + ExpVgst);
         dT10_dVb = T3 * dn_dVb - dT10_dVg * (dVth_dVb + Vgst * dn_dVb / n);
         dT10_dVd = T3 * dn_dVd - dT10_dVg * (dVth_dVd + Vgst * dn_dVd / n);
         dT10_dVg *= dVgs_eff_dVg;
     }
-    T1 = p.voffcbn - (1.0 - p.mstar) * Vgst;
+    T1 = PF(voffcbn) - (1.0 - PF(mstar)) * Vgst;
     T2 = T1 / T0;
     if (T2 < -kExpThreshold) {
-        T3 = p.coxe * kMinExp / p.cdep0;
-        T9 = p.mstar + T3 * n;
+        T3 = PF(coxe) * kMinExp / PF(cdep0);
+        T9 = PF(mstar) + T3 * n;
         dT9_dVg = 0.0;
         dT9_dVd = dn_dVd * T3;
         dT9_dVb = dn_dVb * T3;
     } else if (T2 > kExpThreshold) {
-        T3 = p.coxe * kMaxExp / p.cdep0;
-        T9 = p.mstar + T3 * n;
+        T3 = PF(coxe) * kMaxExp / PF(cdep0);
+        T9 = PF(mstar) + T3 * n;
         dT9_dVg = 0.0;
         dT9_dVd = dn_dVd * T3;
         dT9_dVb = dn_dVb * T3;
     } else {
         const double ExpVgst = exp(T2);
-        T3 = p.coxe / p.cdep0;
+        T3 = PF(coxe) / PF(cdep0);
         T4 = T3 * ExpVgst;
         T5 = T1 * T4 / T0;
-        T9 = p.mstar + n * T4;
-        dT9_dVg = T3 * (p.mstar - 1.0) * ExpVgst / Vtm;
+        T9 = PF(mstar) + n * T4;
+        dT9_dVg = T3 * (PF(mstar) - 1.0) * ExpVgst / Vtm;
         dT9_dVb = T4 * dn_dVb - dT9_dVg * dVth_dVb - T5 * dn_dVb;
         dT9_dVd = T4 * dn_dVd - dT9_dVg * dVth_dVd - T5 * dn_dVd;
         dT9_dVg *= dVgs_eff_dVg;
@@ -372,10 +381,10 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
     double dVgsteff_dVb = (T9 * dT10_dVb - T10 * dT9_dVb) / T11;
 
     // ---- effective width, Rds (:1310-1347)
-    T9 = sqrtPhis - p.sqrtPhi;
-    double Weff = p.weff - 2.0 * (p.dwg * Vgsteff + p.dwb * T9);
-    double dWeff_dVg = -2.0 * p.dwg;
-    double dWeff_dVb = -2.0 * p.dwb * dsqrtPhis_dVb;
+    T9 = sqrtPhis - PF(sqrtPhi);
+    double Weff = PF(weff) - 2.0 * (PF(dwg) * Vgsteff + PF(dwb) * T9);
+    double dWeff_dVg = -2.0 * PF(dwg);
+    double dWeff_dVb = -2.0 * PF(dwb) * dsqrtPhis_dVb;
     if (Weff < 2.0e-8) {
         T0 = 1.0 / (6.0e-8 - 2.0 * Weff);
         Weff = 2.0e-8 * (4.0e-8 - Weff) * T0;
@@ -385,17 +394,17 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
     }
     double Rds, dRds_dVg, dRds_dVb;
     {
-        T0 = 1.0 + p.prwg * Vgsteff;
-        const double dT0_dVg = -p.prwg / T0 / T0;
-        T1 = p.prwb * T9;
-        const double dT1_dVb = p.prwb * dsqrtPhis_dVb;
+        T0 = 1.0 + PF(prwg) * Vgsteff;
+        const double dT0_dVg = -PF(prwg) / T0 / T0;
+        T1 = PF(prwb) * T9;
+        const double dT1_dVb = PF(prwb) * dsqrtPhis_dVb;
         T2 = 1.0 / T0 + T1;
         T3 = T2 + sqrt(T2 * T2 + 0.01);
         double dT3_dVg = 1.0 + T2 / (T3 - T2);
         const double dT3_dVb = dT3_dVg * dT1_dVb;
         dT3_dVg *= dT0_dVg;
-        T4 = p.rds0 * 0.5;
-        Rds = p.rdswmin + T3 * T4;
+        T4 = PF(rds0) * 0.5;
+        Rds = PF(rdswmin) + T3 * T4;
         dRds_dVg = T4 * dT3_dVg;
         dRds_dVb = T4 * dT3_dVb;
     }
@@ -403,22 +412,22 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
     // ---- Abulk (:1350-1403)
     double Abulk, Abulk0, dAbulk_dVg, dAbulk_dVb, dAbulk0_dVb;
     {
-        T9 = 0.5 * p.k1ox * Lpe_Vb / sqrtPhis;
-        T1 = T9 + p.k2ox - p.k3b * Vth_NarrowW;
+        T9 = 0.5 * PF(k1ox) * Lpe_Vb / sqrtPhis;
+        T1 = T9 + PF(k2ox) - PF(k3b) * Vth_NarrowW;
         const double dT1_dVb = -T9 / sqrtPhis * dsqrtPhis_dVb;
-        T9 = sqrt(p.xj * Xdep);
+        T9 = sqrt(PF(xj) * Xdep);
         const double tmp1 = Leff + 2.0 * T9;
         T5 = Leff / tmp1;
-        const double tmp2 = p.a0 * T5;
-        const double tmp3 = p.weff + p.b1;
-        const double tmp4 = p.b0 / tmp3;
+        const double tmp2 = PF(a0) * T5;
+        const double tmp3 = PF(weff) + PF(b1);
+        const double tmp4 = PF(b0) / tmp3;
         T2 = tmp2 + tmp4;
         const double dT2_dVb = -T9 / tmp1 / Xdep * dXdep_dVb;
         T6 = T5 * T5;
         T7 = T5 * T6;
         Abulk0 = 1.0 + T1 * T2;
         dAbulk0_dVb = T1 * tmp2 * dT2_dVb + T2 * dT1_dVb;
-        T8 = p.ags * p.a0 * T7;
+        T8 = PF(ags) * PF(a0) * T7;
         dAbulk_dVg = -T1 * T8;
         Abulk = Abulk0 + dAbulk_dVg * Vgsteff;
         dAbulk_dVb = dAbulk0_dVb - T8 * Vgsteff * (dT1_dVb + 3.0 * T1 * dT2_dVb);
@@ -437,14 +446,14 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
     }
     {
         double dT0_dVb;
-        T2 = p.keta * Vbseff;
+        T2 = PF(keta) * Vbseff;
         if (T2 >= -0.9) {
             T0 = 1.0 / (1.0 + T2);
-            dT0_dVb = -p.keta * T0 * T0;
+            dT0_dVb = -PF(keta) * T0 * T0;
         } else {
             T1 = 1.0 / (0.8 + T2);
             T0 = (17.0 + 20.0 * T2) * T1;
-            dT0_dVb = -p.keta * T1 * T1;
+            dT0_dVb = -PF(keta) * T1 * T1;
         }
         dAbulk_dVg *= T0;
         dAbulk_dVb = dAbulk_dVb * T0 + Abulk * dT0_dVb;
@@ -458,20 +467,20 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
     {
         const double T14 = 0.0;
         T0 = Vgsteff + Vth + Vth - T14;
-        T2 = p.ua + p.uc * Vbseff;
+        T2 = PF(ua) + PF(uc) * Vbseff;
         T3 = T0 / toxe;
         T12 = sqrt(Vth * Vth + 0.0001);
         T9 = 1.0 / (Vgsteff + 2 * T12);
         T10 = T9 * toxe;
-        T8 = p.ud * T10 * T10 * Vth;
+        T8 = PF(ud) * T10 * T10 * Vth;
         T6 = T8 * Vth;
-        T5 = T3 * (T2 + p.ub * T3) + T6;
+        T5 = T3 * (T2 + PF(ub) * T3) + T6;
         T7 = -2.0 * T6 * T9;
         T11 = T7 * Vth / T12;
-        double dDenomi_dVg = (T2 + 2.0 * p.ub * T3) / toxe;
+        double dDenomi_dVg = (T2 + 2.0 * PF(ub) * T3) / toxe;
         T13 = 2.0 * (dDenomi_dVg + T11 + T8);
         double dDenomi_dVd = T13 * dVth_dVd;
-        double dDenomi_dVb = T13 * dVth_dVb + p.uc * T3;
+        double dDenomi_dVb = T13 * dVth_dVb + PF(uc) * T3;
         dDenomi_dVg += T7;
         double Denomi;
         if (T5 >= -0.8) {
@@ -484,7 +493,7 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
             dDenomi_dVd *= T9;
             dDenomi_dVb *= T9;
         }
-        ueff = p.u0temp / Denomi;
+        ueff = PF(u0temp) / Denomi;
         T9 = -ueff / Denomi;
         dueff_dVg = T9 * dDenomi_dVg;
         dueff_dVd = T9 * dDenomi_dVd;
@@ -492,15 +501,15 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
     }
 
     // ---- Vdsat (:1517-1616); A1 = 0: Lambda = A2
-    const double WVCox = Weff * p.vsattemp * p.coxe;
+    const double WVCox = Weff * PF(vsattemp) * PF(coxe);
     const double WVCoxRds = WVCox * Rds;
-    const double Esat = 2.0 * p.vsattemp / ueff;
+    const double Esat = 2.0 * PF(vsattemp) / ueff;
     const double EsatL = Esat * Leff;
     T0 = -EsatL / ueff;
     const double dEsatL_dVg = T0 * dueff_dVg;
     const double dEsatL_dVd = T0 * dueff_dVd;
     const double dEsatL_dVb = T0 * dueff_dVb;
-    const double Lambda = p.a2;
+    const double Lambda = PF(a2);
     const double dLambda_dVg = 0.0;
     const double Vgst2Vtm = Vgsteff + 2.0 * Vtm;
     double tmp1, tmp2, tmp3;
@@ -558,13 +567,13 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
     // ---- Vdseff (:1619-1657)
     double Vdseff, dVdseff_dVg, dVdseff_dVd, dVdseff_dVb;
     {
-        T1 = Vdsat - Vds - p.delta;
+        T1 = Vdsat - Vds - PF(delta);
         const double dT1_dVg = dVdsat_dVg;
         const double dT1_dVd = dVdsat_dVd - 1.0;
         const double dT1_dVb = dVdsat_dVb;
-        T2 = sqrt(T1 * T1 + 4.0 * p.delta * Vdsat);
+        T2 = sqrt(T1 * T1 + 4.0 * PF(delta) * Vdsat);
         T0 = T1 / T2;
-        T9 = 2.0 * p.delta;
+        T9 = 2.0 * PF(delta);
         T3 = T9 / T2;
         const double dT2_dVg = T0 * dT1_dVg + T3 * dVdsat_dVg;
         const double dT2_dVd = T0 * dT1_dVd + T3 * dVdsat_dVd;
@@ -618,16 +627,16 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
     // ---- Idl (:1728-1785)
     double Idl, dIdl_dVg, dIdl_dVd, dIdl_dVb;
     {
-        tmp1 = p.vtfbphi2;
-        tmp2 = 2.0e8 * p.toxp;
+        tmp1 = PF(vtfbphi2);
+        tmp2 = 2.0e8 * PF(toxp);
         double dT0_dVg = 1.0 / tmp2;
         T0 = (Vgsteff + tmp1) * dT0_dVg;
-        tmp3 = exp(p.bdos * 0.7 * log(T0));
+        tmp3 = exp(PF(bdos) * 0.7 * log(T0));
         T1 = 1.0 + tmp3;
-        T2 = p.bdos * 0.7 * tmp3 / T0;
-        const double Tcen = p.ados * 1.9e-9 / T1;
+        T2 = PF(bdos) * 0.7 * tmp3 / T0;
+        const double Tcen = PF(ados) * 1.9e-9 / T1;
         const double dTcen_dVg = -Tcen * T2 * dT0_dVg / T1;
-        const double Coxeff = epssub * p.coxp / (epssub + p.coxp * Tcen);
+        const double Coxeff = epssub * PF(coxp) / (epssub + PF(coxp) * Tcen);
         const double dCoxeff_dVg = -Coxeff * Coxeff * dTcen_dVg / epssub;
         const double CoxeffWovL = Coxeff * Weff / Leff;
         const double beta = ueff * CoxeffWovL;
@@ -665,7 +674,7 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
     // ---- output resistance: FP = 1 (FPROUT = 0), VACLM, VADIBL (:1789-1897)
     const double FP = 1.0, dFP_dVg = 0.0;
     double PvagTerm, dPvagTerm_dVg, dPvagTerm_dVb, dPvagTerm_dVd;
-    T8 = p.pvag / EsatL;
+    T8 = PF(pvag) / EsatL;
     T9 = T8 * Vgsteff;
     if (T9 > -0.9) {
         PvagTerm = 1.0 + T9;
@@ -682,7 +691,7 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
         dPvagTerm_dVd = -T9 * dEsatL_dVd;
     }
     double Cclm, dCclm_dVg, dCclm_dVd, dCclm_dVb, VACLM, dVACLM_dVg, dVACLM_dVd, dVACLM_dVb;
-    if ((p.pclm > kMinExp) && (diffVds > 1.0e-10)) {
+    if ((PF(pclm) > kMinExp) && (diffVds > 1.0e-10)) {
         T0 = 1.0 + Rds * Idl;
         const double dT0_dVg = dRds_dVg * Idl + Rds * dIdl_dVg;
         const double dT0_dVd = Rds * dIdl_dVd;
@@ -692,7 +701,7 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
         const double dT1_dVg = (dVdsat_dVg - T2 * dEsatL_dVg / Leff) / Esat;
         const double dT1_dVd = (dVdsat_dVd - T2 * dEsatL_dVd / Leff) / Esat;
         const double dT1_dVb = (dVdsat_dVb - T2 * dEsatL_dVb / Leff) / Esat;
-        Cclm = FP * PvagTerm * T0 * T1 / (p.pclm * p.litl);
+        Cclm = FP * PvagTerm * T0 * T1 / (PF(pclm) * PF(litl));
         dCclm_dVg = Cclm * (dFP_dVg / FP + dPvagTerm_dVg / PvagTerm + dT0_dVg / T0 + dT1_dVg / T1);
         dCclm_dVb = Cclm * (dPvagTerm_dVb / PvagTerm + dT0_dVb / T0 + dT1_dVb / T1);
         dCclm_dVd = Cclm * (dPvagTerm_dVd / PvagTerm + dT0_dVd / T0 + dT1_dVd / T1);
@@ -706,7 +715,7 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
         dCclm_dVd = dCclm_dVg = dCclm_dVb = 0.0;
     }
     double VADIBL, dVADIBL_dVg, dVADIBL_dVb, dVADIBL_dVd;
-    if (p.thetaRout > kMinExp) {
+    if (PF(thetaRout) > kMinExp) {
         T8 = Abulk * Vdsat;
         T0 = Vgst2Vtm * T8;
         const double dT0_dVg = Vgst2Vtm * Abulk * dVdsat_dVg + T8 + Vgst2Vtm * Vdsat * dAbulk_dVg;
@@ -717,23 +726,23 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
         const double dT1_dVb = Abulk * dVdsat_dVb + dAbulk_dVb * Vdsat;
         const double dT1_dVd = Abulk * dVdsat_dVd;
         T9 = T1 * T1;
-        T2 = p.thetaRout;
+        T2 = PF(thetaRout);
         VADIBL = (Vgst2Vtm - T0 / T1) / T2;
         dVADIBL_dVg = (1.0 - dT0_dVg / T1 + T0 * dT1_dVg / T9) / T2;
         dVADIBL_dVb = (-dT0_dVb / T1 + T0 * dT1_dVb / T9) / T2;
         dVADIBL_dVd = (-dT0_dVd / T1 + T0 * dT1_dVd / T9) / T2;
-        T7 = p.pdiblb * Vbseff;
+        T7 = PF(pdiblb) * Vbseff;
         if (T7 >= -0.9) {
             T3 = 1.0 / (1.0 + T7);
             VADIBL *= T3;
             dVADIBL_dVg *= T3;
-            dVADIBL_dVb = (dVADIBL_dVb - VADIBL * p.pdiblb) * T3;
+            dVADIBL_dVb = (dVADIBL_dVb - VADIBL * PF(pdiblb)) * T3;
             dVADIBL_dVd *= T3;
         } else {
             T4 = 1.0 / (0.8 + T7);
             T3 = (17.0 + 20.0 * T7) * T4;
             dVADIBL_dVg *= T3;
-            dVADIBL_dVb = dVADIBL_dVb * T3 - VADIBL * p.pdiblb * T4 * T4;
+            dVADIBL_dVb = dVADIBL_dVb * T3 - VADIBL * PF(pdiblb) * T4 * T4;
             dVADIBL_dVd *= T3;
             VADIBL *= T3;
         }
@@ -753,16 +762,16 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
     const double VADITS = kMaxExp, dVADITS_dVg = 0.0, dVADITS_dVd = 0.0;
     // VASCBE (:1929-1947)
     double VASCBE, dVASCBE_dVg, dVASCBE_dVd, dVASCBE_dVb;
-    if ((p.pscbe2 > 0.0) && (p.pscbe1 >= 0.0)) {
-        if (diffVds > p.pscbe1 * p.litl / kExpThreshold) {
-            T0 = p.pscbe1 * p.litl / diffVds;
-            VASCBE = Leff * exp(T0) / p.pscbe2;
+    if ((PF(pscbe2) > 0.0) && (PF(pscbe1) >= 0.0)) {
+        if (diffVds > PF(pscbe1) * PF(litl) / kExpThreshold) {
+            T0 = PF(pscbe1) * PF(litl) / diffVds;
+            VASCBE = Leff * exp(T0) / PF(pscbe2);
             T1 = T0 * VASCBE / diffVds;
             dVASCBE_dVg = T1 * dVdseff_dVg;
             dVASCBE_dVd = -T1 * (1.0 - dVdseff_dVd);
             dVASCBE_dVb = T1 * dVdseff_dVb;
         } else {
-            VASCBE = kMaxExp * Leff / p.pscbe2;
+            VASCBE = kMaxExp * Leff / PF(pscbe2);
             dVASCBE_dVg = dVASCBE_dVd = dVASCBE_dVb = 0.0;
         }
     } else {
@@ -824,16 +833,16 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
             VbseffCV = Vbseff;
             dVbseffCV_dVb = 1.0;
         } else {
-            VbseffCV = p.phi - Phis;
+            VbseffCV = PF(phi) - Phis;
             dVbseffCV_dVb = -dPhis_dVb;
         }
-        const double CoxWL = p.coxe * p.weffCV * p.leffCV * 1.0;
+        const double CoxWL = PF(coxe) * PF(weffCV) * PF(leffCV) * 1.0;
         {
-            const double noff = n * p.noff;
-            const double dnoff_dVd = p.noff * dn_dVd;
-            const double dnoff_dVb = p.noff * dn_dVb;
+            const double noff = n * PF(noff);
+            const double dnoff_dVd = PF(noff) * dn_dVd;
+            const double dnoff_dVb = PF(noff) * dn_dVb;
             T0 = Vtm * noff;
-            const double voffcv = p.voffcv;
+            const double voffcv = PF(voffcv);
             const double VgstNVt = (Vgst - voffcv) / T0;
             if (VgstNVt > kExpThreshold) {
                 Vgsteff = Vgst - voffcv;
@@ -857,36 +866,36 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
                 dVgsteff_dVg *= dVgs_eff_dVg;
             }
         }
-        double V3 = p.vfbzb - Vgs_eff + VbseffCV - kDelta3;
-        if (p.vfbzb <= 0.0) T0 = sqrt(V3 * V3 - 4.0 * kDelta3 * p.vfbzb);
-        else T0 = sqrt(V3 * V3 + 4.0 * kDelta3 * p.vfbzb);
+        double V3 = PF(vfbzb) - Vgs_eff + VbseffCV - kDelta3;
+        if (PF(vfbzb) <= 0.0) T0 = sqrt(V3 * V3 - 4.0 * kDelta3 * PF(vfbzb));
+        else T0 = sqrt(V3 * V3 + 4.0 * kDelta3 * PF(vfbzb));
         T1 = 0.5 * (1.0 + V3 / T0);
-        const double Vfbeff = p.vfbzb - 0.5 * (V3 + T0);
+        const double Vfbeff = PF(vfbzb) - 0.5 * (V3 + T0);
         const double dVfbeff_dVg = T1 * dVgs_eff_dVg;
         const double dVfbeff_dVb = -T1 * dVbseffCV_dVb;
-        const double Cox = p.coxp;
-        double Tox = 1.0e8 * p.toxp;
-        T0 = (Vgs_eff - VbseffCV - p.vfbzb) / Tox;
+        const double Cox = PF(coxp);
+        double Tox = 1.0e8 * PF(toxp);
+        T0 = (Vgs_eff - VbseffCV - PF(vfbzb)) / Tox;
         const double dT0_dVg = dVgs_eff_dVg / Tox;
         const double dT0_dVb = -dVbseffCV_dVb / Tox;
         double Tcen, dTcen_dVg, dTcen_dVb, dTcen_dVd;
-        double tmp = T0 * p.acde;
+        double tmp = T0 * PF(acde);
         if ((-kExpThreshold < tmp) && (tmp < kExpThreshold)) {
-            Tcen = p.ldeb * exp(tmp);
-            dTcen_dVg = p.acde * Tcen;
+            Tcen = PF(ldeb) * exp(tmp);
+            dTcen_dVg = PF(acde) * Tcen;
             dTcen_dVb = dTcen_dVg * dT0_dVb;
             dTcen_dVg *= dT0_dVg;
         } else if (tmp <= -kExpThreshold) {
-            Tcen = p.ldeb * kMinExp;
+            Tcen = PF(ldeb) * kMinExp;
             dTcen_dVg = dTcen_dVb = 0.0;
         } else {
-            Tcen = p.ldeb * kMaxExp;
+            Tcen = PF(ldeb) * kMaxExp;
             dTcen_dVg = dTcen_dVb = 0.0;
         }
-        const double LINK = 1.0e-3 * p.toxp;
-        V3 = p.ldeb - Tcen - LINK;
-        const double V4 = sqrt(V3 * V3 + 4.0 * LINK * p.ldeb);
-        Tcen = p.ldeb - 0.5 * (V3 + V4);
+        const double LINK = 1.0e-3 * PF(toxp);
+        V3 = PF(ldeb) - Tcen - LINK;
+        const double V4 = sqrt(V3 * V3 + 4.0 * LINK * PF(ldeb));
+        Tcen = PF(ldeb) - 0.5 * (V3 + V4);
         T1 = 0.5 * (1.0 + V3 / V4);
         dTcen_dVg *= T1;
         dTcen_dVb *= T1;
@@ -897,24 +906,24 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
         double dCoxeff_dVg = T2 * T2 * T3;
         double dCoxeff_dVb = dCoxeff_dVg * dTcen_dVb;
         dCoxeff_dVg *= dTcen_dVg;
-        double CoxWLcen = CoxWL * Coxeff / p.coxe;
-        const double Qac0 = CoxWLcen * (Vfbeff - p.vfbzb);
+        double CoxWLcen = CoxWL * Coxeff / PF(coxe);
+        const double Qac0 = CoxWLcen * (Vfbeff - PF(vfbzb));
         double QovCox = Qac0 / Coxeff;
         const double dQac0_dVg = CoxWLcen * dVfbeff_dVg + QovCox * dCoxeff_dVg;
         const double dQac0_dVb = CoxWLcen * dVfbeff_dVb + QovCox * dCoxeff_dVb;
-        T0 = 0.5 * p.k1ox;
+        T0 = 0.5 * PF(k1ox);
         T3 = Vgs_eff - Vfbeff - VbseffCV - Vgsteff;
-        if (p.k1ox == 0.0) {
+        if (PF(k1ox) == 0.0) {
             T1 = 0.0;
             T2 = 0.0;
         } else if (T3 < 0.0) {
-            T1 = T0 + T3 / p.k1ox;
+            T1 = T0 + T3 / PF(k1ox);
             T2 = CoxWLcen;
         } else {
             T1 = sqrt(T0 * T0 + T3);
             T2 = CoxWLcen * T0 / T1;
         }
-        const double Qsub0 = CoxWLcen * p.k1ox * (T1 - T0);
+        const double Qsub0 = CoxWLcen * PF(k1ox) * (T1 - T0);
         QovCox = Qsub0 / Coxeff;
         const double dQsub0_dVg = T2 * (dVgs_eff_dVg - dVfbeff_dVg - dVgsteff_dVg) + QovCox * dCoxeff_dVg;
         const double dQsub0_dVd = -T2 * dVgsteff_dVd;
@@ -922,12 +931,12 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
             -T2 * (dVfbeff_dVb + dVbseffCV_dVb + dVgsteff_dVb) + QovCox * dCoxeff_dVb;
         // gate-bias dependent delta Phis
         double Denomi;
-        if (p.k1ox <= 0.0) {
-            Denomi = 0.25 * p.moin * Vtm;
-            T0 = 0.5 * p.sqrtPhi;
+        if (PF(k1ox) <= 0.0) {
+            Denomi = 0.25 * PF(moin) * Vtm;
+            T0 = 0.5 * PF(sqrtPhi);
         } else {
-            Denomi = p.moin * Vtm * p.k1ox * p.k1ox;
-            T0 = p.k1ox * p.sqrtPhi;
+            Denomi = PF(moin) * Vtm * PF(k1ox) * PF(k1ox);
+            T0 = PF(k1ox) * PF(sqrtPhi);
         }
         T1 = 2.0 * T0 + Vgsteff;
         const double DeltaPhi = Vtm * log(1.0 + T1 * Vgsteff / Denomi);
@@ -938,11 +947,11 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
         const double VgDP = 0.5 * (T0 + T1);
         const double dVgDP_dVg = 0.5 * (dT0_dVg2 + (T0 * dT0_dVg2 + 0.002) / T1);
         Tox += Tox;
-        T0 = (Vgsteff + p.vtfbphi2) / Tox;
-        tmp = exp(p.bdos * 0.7 * log(T0));
+        T0 = (Vgsteff + PF(vtfbphi2)) / Tox;
+        tmp = exp(PF(bdos) * 0.7 * log(T0));
         T1 = 1.0 + tmp;
-        T2 = p.bdos * 0.7 * tmp / (T0 * Tox);
-        Tcen = p.ados * 1.9e-9 / T1;
+        T2 = PF(bdos) * 0.7 * tmp / (T0 * Tox);
+        Tcen = PF(ados) * 1.9e-9 / T1;
         dTcen_dVg = -Tcen * T2 / T1;
         dTcen_dVd = dTcen_dVg * dVgsteff_dVd;
         dTcen_dVb = dTcen_dVg * dVgsteff_dVb;
@@ -955,9 +964,9 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
         const double dCoxeff_dVd = dCoxeff_dVg * dTcen_dVd;
         dCoxeff_dVb = dCoxeff_dVg * dTcen_dVb;
         dCoxeff_dVg *= dTcen_dVg;
-        CoxWLcen = CoxWL * Coxeff / p.coxe;
-        const double AbulkCV = Abulk0 * p.abulkCVfactor;
-        const double dAbulkCV_dVb = p.abulkCVfactor * dAbulk0_dVb;
+        CoxWLcen = CoxWL * Coxeff / PF(coxe);
+        const double AbulkCV = Abulk0 * PF(abulkCVfactor);
+        const double dAbulkCV_dVb = PF(abulkCVfactor) * dAbulk0_dVb;
         const double VdsatCV = VgDP / AbulkCV;
         T0 = VdsatCV - Vds - kDelta4;
         const double dT0_dVg3 = dVgDP_dVg / AbulkCV;
@@ -1054,37 +1063,37 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
     // ---- junction C-V (:3843-3960)
     double qbs = 0.0, qbd = 0.0, capbs = 0.0, capbd = 0.0;
     if (charge) {
-        const double czbd = p.czbd, czbs = p.czbs, czbdsw = p.czbdsw, czbdswg = p.czbdswg;
-        const double czbssw = p.czbssw, czbsswg = p.czbsswg;
+        const double czbd = PF(czbd), czbs = PF(czbs), czbdsw = PF(czbdsw), czbdswg = PF(czbdswg);
+        const double czbssw = PF(czbssw), czbsswg = PF(czbsswg);
         if (vbs_jct == 0.0) {
             qbs = 0.0;
             capbs = czbs + czbssw + czbsswg;
         } else if (vbs_jct < 0.0) {
             if (czbs > 0.0) {
-                const double arg = 1.0 - vbs_jct / p.PhiBS;
-                const double sarg = (p.MJS == 0.5) ? 1.0 / sqrt(arg) : exp(-p.MJS * log(arg));
-                qbs = p.PhiBS * czbs * (1.0 - arg * sarg) / (1.0 - p.MJS);
+                const double arg = 1.0 - vbs_jct / PF(PhiBS);
+                const double sarg = (PF(MJS) == 0.5) ? 1.0 / sqrt(arg) : exp(-PF(MJS) * log(arg));
+                qbs = PF(PhiBS) * czbs * (1.0 - arg * sarg) / (1.0 - PF(MJS));
                 capbs = czbs * sarg;
             } else {
                 qbs = 0.0;
                 capbs = 0.0;
             }
             if (czbssw > 0.0) {
-                const double arg = 1.0 - vbs_jct / p.PhiBSWS;
-                const double sarg = (p.MJSWS == 0.5) ? 1.0 / sqrt(arg) : exp(-p.MJSWS * log(arg));
-                qbs += p.PhiBSWS * czbssw * (1.0 - arg * sarg) / (1.0 - p.MJSWS);
+                const double arg = 1.0 - vbs_jct / PF(PhiBSWS);
+                const double sarg = (PF(MJSWS) == 0.5) ? 1.0 / sqrt(arg) : exp(-PF(MJSWS) * log(arg));
+                qbs += PF(PhiBSWS) * czbssw * (1.0 - arg * sarg) / (1.0 - PF(MJSWS));
                 capbs += czbssw * sarg;
             }
             if (czbsswg > 0.0) {
-                const double arg = 1.0 - vbs_jct / p.PhiBSWGS;
-                const double sarg = (p.MJSWGS == 0.5) ? 1.0 / sqrt(arg) : exp(-p.MJSWGS * log(arg));
-                qbs += p.PhiBSWGS * czbsswg * (1.0 - arg * sarg) / (1.0 - p.MJSWGS);
+                const double arg = 1.0 - vbs_jct / PF(PhiBSWGS);
+                const double sarg = (PF(MJSWGS) == 0.5) ? 1.0 / sqrt(arg) : exp(-PF(MJSWGS) * log(arg));
+                qbs += PF(PhiBSWGS) * czbsswg * (1.0 - arg * sarg) / (1.0 - PF(MJSWGS));
                 capbs += czbsswg * sarg;
             }
         } else {
             T0 = czbs + czbssw + czbsswg;
-            T1 = vbs_jct * (czbs * p.MJS / p.PhiBS + czbssw * p.MJSWS / p.PhiBSWS +
-                            czbsswg * p.MJSWGS / p.PhiBSWGS);
+            T1 = vbs_jct * (czbs * PF(MJS) / PF(PhiBS) + czbssw * PF(MJSWS) / PF(PhiBSWS) +
+                            czbsswg * PF(MJSWGS) / PF(PhiBSWGS));
             qbs = vbs_jct * (T0 + 0.5 * T1);
             capbs = T0 + T1;
         }
@@ -1093,30 +1102,30 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
             capbd = czbd + czbdsw + czbdswg;
         } else if (vbd_jct < 0.0) {
             if (czbd > 0.0) {
-                const double arg = 1.0 - vbd_jct / p.PhiBD;
-                const double sarg = (p.MJD == 0.5) ? 1.0 / sqrt(arg) : exp(-p.MJD * log(arg));
-                qbd = p.PhiBD * czbd * (1.0 - arg * sarg) / (1.0 - p.MJD);
+                const double arg = 1.0 - vbd_jct / PF(PhiBD);
+                const double sarg = (PF(MJD) == 0.5) ? 1.0 / sqrt(arg) : exp(-PF(MJD) * log(arg));
+                qbd = PF(PhiBD) * czbd * (1.0 - arg * sarg) / (1.0 - PF(MJD));
                 capbd = czbd * sarg;
             } else {
                 qbd = 0.0;
                 capbd = 0.0;
             }
             if (czbdsw > 0.0) {
-                const double arg = 1.0 - vbd_jct / p.PhiBSWD;
-                const double sarg = (p.MJSWD == 0.5) ? 1.0 / sqrt(arg) : exp(-p.MJSWD * log(arg));
-                qbd += p.PhiBSWD * czbdsw * (1.0 - arg * sarg) / (1.0 - p.MJSWD);
+                const double arg = 1.0 - vbd_jct / PF(PhiBSWD);
+                const double sarg = (PF(MJSWD) == 0.5) ? 1.0 / sqrt(arg) : exp(-PF(MJSWD) * log(arg));
+                qbd += PF(PhiBSWD) * czbdsw * (1.0 - arg * sarg) / (1.0 - PF(MJSWD));
                 capbd += czbdsw * sarg;
             }
             if (czbdswg > 0.0) {
-                const double arg = 1.0 - vbd_jct / p.PhiBSWGD;
-                const double sarg = (p.MJSWGD == 0.5) ? 1.0 / sqrt(arg) : exp(-p.MJSWGD * log(arg));
-                qbd += p.PhiBSWGD * czbdswg * (1.0 - arg * sarg) / (1.0 - p.MJSWGD);
+                const double arg = 1.0 - vbd_jct / PF(PhiBSWGD);
+                const double sarg = (PF(MJSWGD) == 0.5) ? 1.0 / sqrt(arg) : exp(-PF(MJSWGD) * log(arg));
+                qbd += PF(PhiBSWGD) * czbdswg * (1.0 - arg * sarg) / (1.0 - PF(MJSWGD));
                 capbd += czbdswg * sarg;
             }
         } else {
             T0 = czbd + czbdsw + czbdswg;
-            T1 = vbd_jct * (czbd * p.MJD / p.PhiBD + czbdsw * p.MJSWD / p.PhiBSWD +
-                            czbdswg * p.MJSWGD / p.PhiBSWGD);
+            T1 = vbd_jct * (czbd * PF(MJD) / PF(PhiBD) + czbdsw * PF(MJSWD) / PF(PhiBSWD) +
+                            czbdswg * PF(MJSWGD) / PF(PhiBSWGD));
             qbd = vbd_jct * (T0 + 0.5 * T1);
             capbd = T0 + T1;
         }
@@ -1154,29 +1163,29 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
         T0 = vgdx + kDelta1;
         T1 = sqrt(T0 * T0 + 4.0 * kDelta1);
         T2 = 0.5 * (T0 - T1);
-        T3 = p.weffCV * p.cgdl;
-        T4 = sqrt(1.0 - 4.0 * T2 / p.ckappad);
-        const double cgdo = p.cgdo + T3 - T3 * (1.0 - 1.0 / T4) * (0.5 - 0.5 * T0 / T1);
-        const double qgdo = (p.cgdo + T3) * vgdx - T3 * (T2 + 0.5 * p.ckappad * (T4 - 1.0));
+        T3 = PF(weffCV) * PF(cgdl);
+        T4 = sqrt(1.0 - 4.0 * T2 / PF(ckappad));
+        const double cgdo = PF(cgdo) + T3 - T3 * (1.0 - 1.0 / T4) * (0.5 - 0.5 * T0 / T1);
+        const double qgdo = (PF(cgdo) + T3) * vgdx - T3 * (T2 + 0.5 * PF(ckappad) * (T4 - 1.0));
         T0 = vgsx + kDelta1;
         T1 = sqrt(T0 * T0 + 4.0 * kDelta1);
         T2 = 0.5 * (T0 - T1);
-        T3 = p.weffCV * p.cgsl;
-        T4 = sqrt(1.0 - 4.0 * T2 / p.ckappas);
-        const double cgso = p.cgso + T3 - T3 * (1.0 - 1.0 / T4) * (0.5 - 0.5 * T0 / T1);
-        const double qgso = (p.cgso + T3) * vgsx - T3 * (T2 + 0.5 * p.ckappas * (T4 - 1.0));
+        T3 = PF(weffCV) * PF(cgsl);
+        T4 = sqrt(1.0 - 4.0 * T2 / PF(ckappas));
+        const double cgso = PF(cgso) + T3 - T3 * (1.0 - 1.0 / T4) * (0.5 - 0.5 * T0 / T1);
+        const double qgso = (PF(cgso) + T3) * vgsx - T3 * (T2 + 0.5 * PF(ckappas) * (T4 - 1.0));
         const double ag0 = c.ag0;
         double qsrc;
         if (dmode > 0) {
             qdrn -= qgdo;
-            gcggb = (cggb + cgdo + cgso + p.cgbo) * ag0;
+            gcggb = (cggb + cgdo + cgso + PF(cgbo)) * ag0;
             gcgdb = (cgdb - cgdo) * ag0;
             gcgsb = (cgsb - cgso) * ag0;
             gcgbb = -(gcggb + gcgdb + gcgsb);
             gcdgb = (cdgb - cgdo) * ag0;
             gcsgb = -(cggb + cbgb + cdgb + cgso) * ag0;
-            gcbgb = (cbgb - p.cgbo) * ag0;
-            const double qgb = p.cgbo * vgb;
+            gcbgb = (cbgb - PF(cgbo)) * ag0;
+            const double qgb = PF(cgbo) * vgb;
             qgate += qgdo + qgso + qgb;
             qbulk -= qgb;
             qsrc = -(qgate + qbulk + qdrn);
@@ -1191,14 +1200,14 @@ __global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double*
             gcbbb = -(gcbdb + gcbgb + gcbsb + 0.0);
         } else {
             qsrc = qdrn - qgso;
-            gcggb = (cggb + cgdo + cgso + p.cgbo) * ag0;
+            gcggb = (cggb + cgdo + cgso + PF(cgbo)) * ag0;
             gcgdb = (cgsb - cgdo) * ag0;
             gcgsb = (cgdb - cgso) * ag0;
             gcgbb = -(gcggb + gcgdb + gcgsb);
             gcdgb = -(cggb + cbgb + cdgb + cgdo) * ag0;
             gcsgb = (cdgb - cgso) * ag0;
-            gcbgb = (cbgb - p.cgbo) * ag0;
-            const double qgb = p.cgbo * vgb;
+            gcbgb = (cbgb - PF(cgbo)) * ag0;
+            const double qgb = PF(cgbo) * vgb;
             qgate += qgdo + qgso + qgb;
             qbulk -= qgb;
             qdrn = -(qgate + qbulk + qsrc);

@@ -108,7 +108,10 @@ struct B4Table {
     int32_t count;
     const int32_t* nodes;   // 4 per device: d, g, s, b
     const int32_t* state;   // state base per device (29 slots)
-    const B4Param* par;     // per device
+    const B4Param* par;     // per device, shared by the instances
+    // per-instance constants instead (null: use par): for device d, field f
+    // and instance i at par_inst[(d * fields + f) * B + i]
+    const double* par_inst;
     double* aux;            // count x kB4Aux x B
     int32_t site, rsite;    // 16 matrix sites, 4 rhs sites per device
 };

@@ -202,3 +202,54 @@ def test_bsim4_transient_replay():
     # the reference's own adaptive step sequence (steps capped at 1 ps)
     ckt, _ = ring_ckt(7, mismatch_seed=5)
     tran_parity(ckt, 1e-12, 0.3e-9)
+
+
+class _Ops:
+    """A recorded circuit with some model cards rewritten (for the
+    reference side of a per-instance card parameter)."""
+
+    def __init__(self, ckt, cards):
+        self.nodes = ckt.nodes
+        self.ops = []
+        for op in ckt.ops:
+            if op[0] == "model":
+                name = op[1].split()[1].lower()
+                self.ops.append(("model", cards.get(name, op[1])))
+            else:
+                self.ops.append(op)
+
+
+def test_bsim4_per_instance_cards():
+    """Monte-Carlo VTH0/TOXE per instance (set_model_values): instance i
+    against the reference circuit whose cards carry instance i's values."""
+    import re
+    ckt, _ = ring_ckt(11, mismatch_seed=6)
+    B = 3
+    eng = na.BatchEngine(ckt, batch=B)
+    rng = np.random.default_rng(9)
+    texts = {op[1].split()[1].lower(): op[1] for op in ckt.ops if op[0] == "model"}
+    devs = ckt.ops_devices()
+    mdevs = [i for i, op in enumerate(devs) if op[0] == "M"][:8]
+    cards = [dict() for _ in range(B)]
+    for dv in mdevs:
+        model = devs[dv][6].lower()
+        text = texts[model]
+        vals = {}
+        for par, sig in (("vth0", 0.03), ("toxe", 0.01)):
+            base = float(re.search(par + r"=([-0-9.e+]+)", text).group(1))
+            vals[par] = base * (1.0 + sig * rng.standard_normal(B))
+            eng.set_model_values(dv, par, vals[par])
+        for i in range(B):
+            t = texts[model]
+            for par in vals:
+                t = re.sub(par + r"=[-0-9.e+]+", "%s=%.17g" % (par, vals[par][i]), t)
+            cards[i][model] = t
+    x, its = eng.dc_op()
+    for i in range(B):
+        rc = RB.RefCircuit(_Ops(ckt, cards[i]), probe=False)
+        x_ref, its_ref, _ = rc.dc()
+        assert its[i] == its_ref, (i, its[i], its_ref)
+        tol = 1e-9 * np.maximum(np.abs(x_ref), 1e-3) + 1e-12
+        assert (np.abs(x[:, i] - x_ref) <= tol).all(), (i, np.max(np.abs(x[:, i] - x_ref)))
+    # the instances differ from each other
+    assert np.abs(x[:, 0] - x[:, 1]).max() > 1e-9
