@@ -173,6 +173,36 @@ __global__ void k_isrc(IsrcTable t, const DevCtx* __restrict__ ctx,
     r[(size_t)B] = I;
 }
 
+constexpr int kWaveSize = 64;
+
+// The sum of a row's sites, in site order from 0.0 (the slot's load-order
+// accumulation).  Loads are issued kChunk at a time ahead of the in-order
+// adds, so a long row (a supply node touched by hundreds of devices) costs
+// one memory latency per chunk instead of one per site.
+__device__ inline double sum_sites(const int32_t* __restrict__ ptr, const int32_t* __restrict__ sites,
+                                   const double* __restrict__ contrib, int r, int inst, size_t sB,
+                                   bool uniform) {
+    constexpr int kChunk = 32;
+    int p0 = ptr[r], p1 = ptr[r + 1];
+    if (uniform) {
+        p0 = __builtin_amdgcn_readfirstlane(p0);
+        p1 = __builtin_amdgcn_readfirstlane(p1);
+    }
+    double acc = 0.0;
+    for (int p = p0; p < p1; p += kChunk) {
+        int s[kChunk];
+        double v[kChunk];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) s[u] = sites[min(p + u, p1 - 1)];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) v[u] = contrib[(size_t)s[u] * sB + inst];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u)
+            if (p + u < p1) acc += v[u];
+    }
+    return acc;
+}
+
 __global__ void k_gather(int nrows, const int32_t* __restrict__ ptr,
                          const int32_t* __restrict__ sites, const int32_t* __restrict__ ptr2,
                          const int32_t* __restrict__ sites2, const double* __restrict__ contrib,
@@ -181,13 +211,11 @@ __global__ void k_gather(int nrows, const int32_t* __restrict__ ptr,
     if (gid >= (long)nrows * B) return;
     const int r = (int)(gid / B), inst = (int)(gid % B);
     const size_t sB = (size_t)B;
-    double acc = 0.0;
-    for (int p = ptr[r]; p < ptr[r + 1]; ++p) acc += contrib[(size_t)sites[p] * sB + inst];
-    if (ptr2) {
-        double acc2 = 0.0;
-        for (int p = ptr2[r]; p < ptr2[r + 1]; ++p) acc2 += contrib2[(size_t)sites2[p] * sB + inst];
-        acc += acc2;
-    }
+    // B a multiple of the wave size: the row is wave-uniform, so the site
+    // list is read through the scalar cache
+    const bool uniform = (B & (kWaveSize - 1)) == 0;
+    double acc = sum_sites(ptr, sites, contrib, r, inst, sB, uniform);
+    if (ptr2) acc += sum_sites(ptr2, sites2, contrib2, r, inst, sB, uniform);
     out[gid] = acc;
 }
 
