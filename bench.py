@@ -250,6 +250,15 @@ def bsim4_ring(na, W, stages=117, batch=256, steps=30, warmup=3, b1_steps=10, dt
     it1 = run_steps(e1, na, dt, False, b1_steps, warmup)
     torch.cuda.synchronize()
     w1 = time.perf_counter() - t1
+    # the BSIM4v7 load kernel against HBM: algorithmic bytes per device-instance
+    # of a backward-Euler corrector load with per-instance cards -- the
+    # 110-double constant record (880 B), 3 node voltages (24 B), 4 state
+    # voltages in (32 B), 3 charges of state1 in (24 B), 13 state slots out
+    # (104 B), 10 carried instance fields in and out (160 B), 20 stamp sites
+    # out (160 B): 1384 B
+    b4_bytes = 1384.0 * (len(mdevs) + 1) * batch
+    b4_ms = timing["bsim4_ms"] / max(1, timing["loads"])
+    b4_gbs = b4_bytes / (b4_ms * 1e-3) / 1e9
     out = {"workload": "bsim4v7_ring_%d_stages_be_1ps_b%d" % (stages, batch), "instances": batch,
            "unknowns": ckt.n, "nnz": ckt.nnz, "devices_bsim4": len(mdevs) + 1,
            "value": round(iters / wall, 2), "unit": "Newton iters/s",
@@ -258,7 +267,10 @@ def bsim4_ring(na, W, stages=117, batch=256, steps=30, warmup=3, b1_steps=10, dt
            "speedup_vs_batch1_gpu": round((iters / wall) / (it1 / w1), 2),
            "phases_ms_per_iter": {"load_gather": timing["load_ms"] / nit,
                                   "refactor": timing["refactor_ms"] / nit,
-                                  "solve": timing["solve_ms"] / nit}}
+                                  "solve": timing["solve_ms"] / nit},
+           "roofline": {"bound": "hbm", "kernel": "k_bsim4", "achieved": round(b4_gbs, 2),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(b4_gbs / HBM_PEAK_GBS, 6),
+                        "traffic": None, "launch_ms": round(b4_ms, 5), "bytes_per_launch": b4_bytes}}
     try:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import refbind as RB
@@ -289,6 +301,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-ring", action="store_true")
     ap.add_argument("--ring-stages", type=int, default=117)
+    ap.add_argument("--ring-only", action="store_true",
+                    help="only the BSIM4v7 ring-oscillator leg (its JSON object)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of known size (for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -311,6 +325,10 @@ def main():
         return
     import neospice_amd as na
     from neospice_amd import workloads as W
+    if args.ring_only:
+        print(json.dumps(bsim4_ring(na, W, stages=args.ring_stages, steps=args.steps,
+                                    warmup=args.warmup, b1_steps=args.batch1_steps)))
+        return
 
     dt = 1e-12
     B = args.batch

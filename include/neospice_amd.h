@@ -162,10 +162,11 @@ const char* nsa_engine_solver_name(nsa_engine* e);
 /* out: cpu pivots, gpu refactors, nnz(L+U), refactor levels, solve levels
  * (of the first pivot group), gpu solves, device loads, pivot groups. */
 int nsa_engine_stats(nsa_engine* e, int64_t out[8]);
-/* HIP-event timing on the engine's stream: accumulated milliseconds of
- * device load+gather, GPU refactor and GPU solve since timing was enabled. */
+/* HIP-event timing on the engine's stream since timing was enabled:
+ * accumulated milliseconds of device load+gather, GPU refactor, GPU solve and
+ * the BSIM4v7 load kernel alone, then the number of loads timed. */
 int nsa_engine_set_timing(nsa_engine* e, int32_t on);
-int nsa_engine_timing(nsa_engine* e, double out[3]);
+int nsa_engine_timing(nsa_engine* e, double out[5]);
 
 /* ---- solver seam (ISolver) ---- */
 int nsa_lu_create(int32_t batch, int32_t n, const int32_t* col_ptr, const int32_t* row_idx,

@@ -313,9 +313,10 @@ class BatchEngine:
 
     def timing(self):
         """Accumulated HIP-event milliseconds: load+gather, refactor, solve."""
-        t = np.zeros(3)
+        t = np.zeros(5)
         check(lib().nsa_engine_timing(self.h, f64p(t)))
-        return {"load_ms": float(t[0]), "refactor_ms": float(t[1]), "solve_ms": float(t[2])}
+        return {"load_ms": float(t[0]), "refactor_ms": float(t[1]), "solve_ms": float(t[2]),
+                "bsim4_ms": float(t[3]), "loads": int(t[4])}
 
     # -- parameters and adjoints --
     def family_count(self, family):

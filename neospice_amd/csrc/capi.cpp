@@ -364,11 +364,13 @@ int nsa_engine_set_timing(nsa_engine* e, int32_t on) {
     return guard([&] { e->e->set_timing(on != 0); return 0; });
 }
 
-int nsa_engine_timing(nsa_engine* e, double out[3]) {
+int nsa_engine_timing(nsa_engine* e, double out[5]) {
     return guard([&] {
         out[0] = e->e->ms_load_gpu;
         out[1] = e->e->lu().ms_refactor;
         out[2] = e->e->lu().ms_solve;
+        out[3] = e->e->ms_bsim4_gpu;
+        out[4] = (double)e->e->n_loads;
         return 0;
     });
 }

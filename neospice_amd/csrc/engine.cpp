@@ -378,8 +378,10 @@ void BatchEngine::load(const int32_t* d_done) {
                  tol_, B_, stream_);
     launch_mos1(mos_, d_ctx_, d_x_, d_s_[0], d_s_[1], d_s_[2], d_done, d_cm_, d_cr_, d_devnc_,
                 tol_, B_, stream_);
+    if (timing_ && b4_.count) NSA_HIP(hipEventRecord(ev_[2], stream_));
     launch_bsim4(b4_, d_ctx_, d_x_, d_s_[0], d_s_[1], d_s_[2], d_done, d_cm_, d_cr_, d_devnc_,
                 tol_, B_, stream_);
+    if (timing_ && b4_.count) NSA_HIP(hipEventRecord(ev_[3], stream_));
     launch_gather(nnz_, d_mptr_, d_msites_, nullptr, nullptr, d_cm_, nullptr, d_A_, B_, stream_);
     launch_gather(n_, d_rptr_d_, d_rsites_d_, d_rptr_u_, d_rsites_u_, d_cr_, d_cr_, d_rhs_, B_,
                   stream_);
@@ -390,6 +392,10 @@ void BatchEngine::load(const int32_t* d_done) {
         NSA_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
         ms_load_gpu += ms;
         ++n_loads;
+        if (b4_.count) {
+            NSA_HIP(hipEventElapsedTime(&ms, ev_[2], ev_[3]));
+            ms_bsim4_gpu += ms;
+        }
     }
 }
 
@@ -398,6 +404,7 @@ void BatchEngine::set_timing(bool on) {
         for (auto& e : ev_) NSA_HIP(hipEventCreate(&e));
     timing_ = on;
     ms_load_gpu = 0.0;
+    ms_bsim4_gpu = 0.0;
     n_loads = 0;
     lu_->set_timing(on);
 }

@@ -126,6 +126,7 @@ public:
     // the solver's refactor / solve; accumulated since set_timing(true).
     void set_timing(bool on);
     double ms_load_gpu = 0.0;
+    double ms_bsim4_gpu = 0.0;  // the BSIM4v7 load kernel alone (HIP events)
     long n_loads = 0;
 
 private:
@@ -191,7 +192,7 @@ private:
     std::vector<int32_t> dev_site_;  // first matrix site per device (insertion index)
     std::vector<int32_t> dev_rsite_; // first rhs site per device
     bool timing_ = false;
-    hipEvent_t ev_[2] = {nullptr, nullptr};
+    hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
     // adjoint tables and buffers (allocated on first use)
     bool adj_ready_ = false;
     int32_t *d_r_nodes_ = nullptr, *d_v_branch_ = nullptr, *d_i_nodes_ = nullptr;

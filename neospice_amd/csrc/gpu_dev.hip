@@ -176,18 +176,14 @@ __global__ void k_isrc(IsrcTable t, const DevCtx* __restrict__ ctx,
 constexpr int kWaveSize = 64;
 
 // The sum of a row's sites, in site order from 0.0 (the slot's load-order
-// accumulation).  Loads are issued kChunk at a time ahead of the in-order
-// adds, so a long row (a supply node touched by hundreds of devices) costs
-// one memory latency per chunk instead of one per site.
-__device__ inline double sum_sites(const int32_t* __restrict__ ptr, const int32_t* __restrict__ sites,
-                                   const double* __restrict__ contrib, int r, int inst, size_t sB,
-                                   bool uniform) {
-    constexpr int kChunk = 32;
-    int p0 = ptr[r], p1 = ptr[r + 1];
-    if (uniform) {
-        p0 = __builtin_amdgcn_readfirstlane(p0);
-        p1 = __builtin_amdgcn_readfirstlane(p1);
-    }
+// accumulation).  Rows of more than 4 sites issue their loads a chunk at a
+// time ahead of the in-order adds: 8, or 32 for long rows (a supply node
+// touched by hundreds of devices), which then cost one memory latency per
+// chunk instead of one per site.
+template <int kChunk>
+__device__ inline double sum_chunks(const int32_t* __restrict__ sites,
+                                    const double* __restrict__ contrib, int p0, int p1, int inst,
+                                    size_t sB) {
     double acc = 0.0;
     for (int p = p0; p < p1; p += kChunk) {
         int s[kChunk];
@@ -200,6 +196,21 @@ __device__ inline double sum_sites(const int32_t* __restrict__ ptr, const int32_
         for (int u = 0; u < kChunk; ++u)
             if (p + u < p1) acc += v[u];
     }
+    return acc;
+}
+
+__device__ inline double sum_sites(const int32_t* __restrict__ ptr, const int32_t* __restrict__ sites,
+                                   const double* __restrict__ contrib, int r, int inst, size_t sB,
+                                   bool uniform) {
+    int p0 = ptr[r], p1 = ptr[r + 1];
+    if (uniform) {
+        p0 = __builtin_amdgcn_readfirstlane(p0);
+        p1 = __builtin_amdgcn_readfirstlane(p1);
+    }
+    if (p1 - p0 > 16) return sum_chunks<32>(sites, contrib, p0, p1, inst, sB);
+    if (p1 - p0 > 4) return sum_chunks<8>(sites, contrib, p0, p1, inst, sB);
+    double acc = 0.0;
+    for (int p = p0; p < p1; ++p) acc += contrib[(size_t)sites[p] * sB + inst];
     return acc;
 }
 
