@@ -49,7 +49,11 @@ static_assert(kB4Fields == (int)(offsetof(B4Param, off) / sizeof(double)), "B4Pa
 // UCB MAX (ucb_compat.hpp:38-43)
 __device__ inline double ucb_max(double a, double b) { return (a > b) ? a : b; }
 
-__global__ void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double* __restrict__ x,
+// The lane's live state (~140 doubles at the charge model) fits in 280
+// VGPRs without spilling when the kernel may run one wave per SIMD; at the
+// compiler's default occupancy target it spilled 680 bytes per lane to
+// scratch, which doubled its HBM traffic and its duration.
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double* __restrict__ x,
                         double* __restrict__ s0, double* __restrict__ s1,
                         const double* __restrict__ s2, const int32_t* __restrict__ done,
                         double* __restrict__ cm, double* __restrict__ cr,
