@@ -109,12 +109,14 @@ void BatchEngine::ensure_adjoint() {
     const size_t nB = (size_t)n_ * B_;
     d_lam_ = alloc<double>(allocs_, nB);
     d_carry_ = alloc<double>(allocs_, nB);
-    d_ibar_ = alloc<double>(allocs_, (size_t)fc.size() * B_);
-    d_q_ = alloc<double>(allocs_, (size_t)fc.size() * B_);
-    d_gR_ = alloc<double>(allocs_, (size_t)family_count(DK_R) * B_);
-    d_gC_ = alloc<double>(allocs_, (size_t)fc.size() * B_);
-    d_gV_ = alloc<double>(allocs_, (size_t)family_count(DK_V) * B_);
-    d_gI_ = alloc<double>(allocs_, (size_t)family_count(DK_I) * B_);
+    // at least one row per family: the clears and copies below use max(1, count)
+    auto rows = [](size_t k) { return std::max<size_t>(1, k); };
+    d_ibar_ = alloc<double>(allocs_, rows(fc.size()) * B_);
+    d_q_ = alloc<double>(allocs_, rows(fc.size()) * B_);
+    d_gR_ = alloc<double>(allocs_, rows(family_count(DK_R)) * B_);
+    d_gC_ = alloc<double>(allocs_, rows(fc.size()) * B_);
+    d_gV_ = alloc<double>(allocs_, rows(family_count(DK_V)) * B_);
+    d_gI_ = alloc<double>(allocs_, rows(family_count(DK_I)) * B_);
     adj_ready_ = true;
 }
 
@@ -314,6 +316,8 @@ void BatchEngine::tran_adjoint(const double* h_dLdprobe, double* h_gR, double* h
         if (p.tJctCap != 0.0 || p.tJctSWCap != 0.0 || p.tTransitTime != 0.0)
             throw Error(NSA_ERR_UNSUPPORTED, "tran_adjoint: diodes with charge storage");
     }
+    if (!ckt_.family(DK_B4).empty())
+        throw Error(NSA_ERR_UNSUPPORTED, "tran_adjoint: BSIM4v7 devices (charge storage)");
     for (int d : ckt_.family(DK_M1)) {
         const Mos1Param p = mos1_param(ckt_.models().at(ckt_.devices()[d].model),
                                        ckt_.devices()[d], DeviceEnv{});

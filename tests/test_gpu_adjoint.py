@@ -226,3 +226,49 @@ def test_adjoint_linear_complex_step(which):
         cs = complex_step(lambda p: loss(f, p), vals[f], rows)
         scale = np.abs(cs).max()
         np.testing.assert_allclose(g[f][:, 0], cs, rtol=1e-9, atol=1e-11 * scale, err_msg=f)
+
+
+def test_dc_adjoint_bsim4_central_fd():
+    """BSIM4v7 inverters with resistive loads at a DC bias: dL/dR and dL/dV
+    through the BSIM4v7 Jacobian (its load at the converged solution,
+    unlimited) against central differences."""
+    B = 2
+    c = na.Circuit()
+    c.model(".model nb nmos level=14 toxe=1.8e-9 vth0=0.4")
+    c.model(".model pb pmos level=14 toxe=1.8e-9 vth0=-0.4")
+    vdd, nin = c.node("vdd"), c.node("in")
+    c.V("vsup", vdd, na.GND, 1.0)
+    c.V("vin", nin, na.GND, 0.42)
+    drive = nin
+    rng = np.random.default_rng(4)
+    for k in range(3):
+        o, l = c.node("o%d" % k), c.node("l%d" % k)
+        c.M("mn%d" % k, o, drive, na.GND, na.GND, "nb", 0.5e-6, 0.05e-6)
+        c.M("mp%d" % k, o, drive, vdd, vdd, "pb", 1e-6, 0.05e-6)
+        c.R("rl%d" % k, o, l, rng.uniform(2e3, 8e3))
+        c.R("rg%d" % k, l, na.GND, rng.uniform(2e4, 8e4))
+        drive = l
+    ckt = c.finalize()
+    eng = na.BatchEngine(ckt, batch=B, **TIGHT)
+    w = np.random.default_rng(5).standard_normal((ckt.n, B))
+    dc_loss(eng, w)
+    g = eng.dc_adjoint(w)
+    for fam, rows in (("R", [0, 2, 5]), ("V", [0, 1])):
+        base = eng.family_values(fam)
+        fd = central_fd(eng, fam, rows, base, lambda: dc_loss(eng, w))
+        for j, k in enumerate(rows):
+            np.testing.assert_allclose(g[fam][k], fd[j], rtol=1e-5,
+                                       atol=1e-6 * np.abs(g[fam]).max(), err_msg=f"{fam}{k}")
+
+
+def test_tran_adjoint_refuses_bsim4():
+    c = na.Circuit()
+    c.model(".model nb nmos level=14 toxe=1.8e-9 vth0=0.4")
+    a = c.node("a")
+    c.V("va", a, na.GND, 0.5)
+    c.M("m1", a, a, na.GND, na.GND, "nb", 0.5e-6, 0.05e-6)
+    ckt = c.finalize()
+    eng = na.BatchEngine(ckt, batch=1)
+    with pytest.raises(Exception):
+        eng.tran_forward(3, 1e-12, np.array([0]))
+        eng.tran_adjoint(np.zeros((4, 1, 1)))
