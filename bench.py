@@ -54,6 +54,19 @@ def measured_traffic():
     return None
 
 
+def ring_load_traffic():
+    """HBM bytes per k_bsim4 launch on the 117-stage, 256-instance ring batch
+    from the committed PMC summary, or None when it is absent."""
+    path = os.path.join(ROOT, "profiles", "k_bsim4_bsim4v7_ring_117_b256.md")
+    try:
+        for line in open(path):
+            if line.startswith("| traffic_bytes_per_launch |"):
+                return float(line.split("|")[2])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def lu_traffic(eng, ckt, B):
     """Algorithmic bytes and flops per refactor and per solve (per batch)."""
     st = eng.stats()
@@ -270,7 +283,8 @@ def bsim4_ring(na, W, stages=117, batch=256, steps=30, warmup=3, b1_steps=10, dt
                                   "solve": timing["solve_ms"] / nit},
            "roofline": {"bound": "hbm", "kernel": "k_bsim4", "achieved": round(b4_gbs, 2),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(b4_gbs / HBM_PEAK_GBS, 6),
-                        "traffic": None, "launch_ms": round(b4_ms, 5), "bytes_per_launch": b4_bytes}}
+                        "traffic": (ring_load_traffic() if (stages == 117 and batch == 256) else None),
+                        "launch_ms": round(b4_ms, 5), "bytes_per_launch": b4_bytes}}
     try:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import refbind as RB
