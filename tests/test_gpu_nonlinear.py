@@ -198,10 +198,11 @@ def test_bsim4_dc_op_long_ring():
         assert (np.abs(x[:, i] - x_ref) <= tol).all(), (i, np.max(np.abs(x[:, i] - x_ref)))
 
 
-def test_bsim4_transient_replay():
+@pytest.mark.parametrize("stages,tstop", [(7, 0.3e-9), (31, 0.15e-9)])
+def test_bsim4_transient_replay(stages, tstop):
     # the reference's own adaptive step sequence (steps capped at 1 ps)
-    ckt, _ = ring_ckt(7, mismatch_seed=5)
-    tran_parity(ckt, 1e-12, 0.3e-9)
+    ckt, _ = ring_ckt(stages, mismatch_seed=5)
+    tran_parity(ckt, 1e-12, tstop)
 
 
 class _Ops:
