@@ -168,6 +168,65 @@ int nsa_engine_stats(nsa_engine* e, int64_t out[8]);
 int nsa_engine_set_timing(nsa_engine* e, int32_t on);
 int nsa_engine_timing(nsa_engine* e, double out[5]);
 
+/* ---- bundles: instances of different topologies in one batch ----
+ * The reference runs a Monte Carlo / .step sweep one circuit at a time, each
+ * a fresh Circuit with its own solver (run_step_sweep,
+ * src/api/neospice.cpp:229-334; Simulator::run_dc / run_transient,
+ * src/api/neospice.hpp:50-93).  A bundle holds one finalized circuit per
+ * instance (instances may differ in topology and size) and advances all of
+ * them together: solutions are packed instance after instance
+ * (x[xoff[i] + k], xoff from nsa_bundle_offsets).  Circuits must outlive the
+ * bundle.  Every instance is factored by Sparse 1.3, as make_solver chooses
+ * for nonlinear circuits and for linear ones under 256 unknowns
+ * (src/core/make_solver.cpp:156-176); other circuits are refused (-5). */
+typedef struct nsa_bundle nsa_bundle;
+int nsa_bundle_create(nsa_circuit* const* circuits, int32_t count, const nsa_options* opt,
+                      nsa_bundle** out);
+void nsa_bundle_destroy(nsa_bundle* b);
+/* instances, total unknowns, total matrix slots, total state slots */
+int nsa_bundle_info(nsa_bundle* b, int32_t out[4]);
+/* xoff, aoff: count+1 entries each (either may be NULL) */
+int nsa_bundle_offsets(nsa_bundle* b, int32_t* xoff, int32_t* aoff);
+/* newton_solve's direct DC attempt (src/core/dc.cpp:104-107) on every
+ * instance in `mode` (MODEDCOP or MODETRANOP with MODEINITJCT) from the
+ * current solution; converged / iterations per instance. */
+int nsa_bundle_dc(nsa_bundle* b, int32_t mode, int32_t* converged, int32_t* iterations);
+int nsa_bundle_get_solution(nsa_bundle* b, double* x);
+int nsa_bundle_set_solution(nsa_bundle* b, const double* x);
+/* solve_transient (src/core/transient.cpp:694-1066) for every instance:
+ * tran_begin runs the MODETRANOP operating point from zeros, the state
+ * seeding, source defaults, breakpoints and the first step (-3 if an
+ * instance's operating point does not converge); tran_advance runs batched
+ * adaptive steps until every instance has reached t_until (its own step
+ * controller: LTE accept/reject, order promotion, breakpoints, ringing and
+ * Newton-failure step cuts).  Returns 1 when every instance is finished.
+ * probes: per-instance unknown indices (probe_ptr[count+1]) recorded at t=0
+ * and after every accepted step, up to max_points rows per instance. */
+int nsa_bundle_tran_begin(nsa_bundle* b, double tstep, double tstop, const int32_t* probe_ptr,
+                          const int32_t* probes, int32_t max_points);
+int nsa_bundle_tran_advance(nsa_bundle* b, double t_until, int32_t* rounds);
+/* accepted steps, rejected steps, failed, finished, Newton iterations of
+ * converged steps (TransientResult::status.iterations), all iterations run,
+ * loop passes, DC iterations */
+int nsa_bundle_tran_status(nsa_bundle* b, int32_t inst, int64_t out[8]);
+/* accepted steps of an instance: end time, step, order, integration method,
+ * Newton iterations (any output may be NULL); returns the count */
+int nsa_bundle_tran_steps(nsa_bundle* b, int32_t inst, double* t, double* dt, int32_t* order,
+                          int32_t* method, int32_t* iters);
+/* recorded probe rows of an instance (rows x nprobe), rows out */
+int nsa_bundle_tran_probes(nsa_bundle* b, int32_t inst, int32_t* rows, double* out);
+/* iterations, CPU pivot computations, plans, graph launches, host-assisted
+ * iterations, max refactor levels, max solve levels, max nnz(L+U) */
+int nsa_bundle_stats(nsa_bundle* b, int64_t out[8]);
+/* HIP-event timing of device-driven iterations (loads+gathers, LU,
+ * convergence/control, the BSIM4v7 load alone; count) */
+int nsa_bundle_set_timing(nsa_bundle* b, int32_t on);
+int nsa_bundle_timing(nsa_bundle* b, double out[5]);
+/* 1 (default): device-driven iterations replay a captured HIP graph */
+int nsa_bundle_set_graph(nsa_bundle* b, int32_t on);
+/* plan (pivot structure) index of an instance, -1 before its first factorization */
+int nsa_bundle_plan_of(nsa_bundle* b, int32_t inst);
+
 /* ---- solver seam (ISolver) ---- */
 int nsa_lu_create(int32_t batch, int32_t n, const int32_t* col_ptr, const int32_t* row_idx,
                   int32_t is_linear, int32_t engine, nsa_lu** out);

@@ -485,3 +485,131 @@ def min_degree_order(n, col_ptr, row_idx):
     perm = np.zeros(n, np.int32)
     check(lib().nsa_min_degree_order(n, i32p(as_i32(col_ptr)), i32p(as_i32(row_idx)), i32p(perm)))
     return perm
+
+
+class Bundle:
+    """Instances of different circuits (topologies and sizes may differ)
+    advanced together on the GPU (nsa_bundle_*): the batched form of the
+    reference's one-circuit-at-a-time .step / Monte Carlo sweep
+    (run_step_sweep, src/api/neospice.cpp:229-334).  Solutions are packed
+    instance after instance; `solution(i)` returns instance i's unknowns."""
+
+    def __init__(self, circuits, **opts):
+        self.circuits = list(circuits)
+        for c in self.circuits:
+            c.finalize()
+        self.N = len(self.circuits)
+        self.opts = default_options(**opts)
+        arr = (C.c_void_p * self.N)(*[c.native for c in self.circuits])
+        h = C.c_void_p()
+        check(lib().nsa_bundle_create(arr, self.N, C.byref(self.opts), C.byref(h)))
+        self.h = h
+        info = np.zeros(4, np.int32)
+        check(lib().nsa_bundle_info(self.h, i32p(info)))
+        self.total_n, self.total_nnz, self.num_states = int(info[1]), int(info[2]), int(info[3])
+        self.xoff = np.zeros(self.N + 1, np.int32)
+        self.aoff = np.zeros(self.N + 1, np.int32)
+        check(lib().nsa_bundle_offsets(self.h, i32p(self.xoff), i32p(self.aoff)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().nsa_bundle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def solution(self, i=None):
+        x = np.zeros(self.total_n)
+        check(lib().nsa_bundle_get_solution(self.h, f64p(x)))
+        if i is None:
+            return x
+        return x[self.xoff[i]:self.xoff[i + 1]].copy()
+
+    def set_solution(self, x):
+        x = as_f64(np.broadcast_to(np.asarray(x, np.float64), (self.total_n,)))
+        check(lib().nsa_bundle_set_solution(self.h, f64p(x)))
+
+    def dc_op(self, mode=MODEDCOP | MODEINITJCT, x0=0.0):
+        """Direct Newton attempt of every instance's operating point."""
+        self.set_solution(x0)
+        conv = np.zeros(self.N, np.int32)
+        its = np.zeros(self.N, np.int32)
+        check(lib().nsa_bundle_dc(self.h, int(mode), i32p(conv), i32p(its)))
+        return conv.astype(bool), its
+
+    def tran_begin(self, tstep, tstop, probes=None, max_points=1):
+        """solve_transient's preamble for every instance; probes: one list
+        of unknown indices per instance (or None)."""
+        if probes is not None:
+            ptr = np.zeros(self.N + 1, np.int32)
+            flat = []
+            for i, p in enumerate(probes):
+                flat.extend(int(v) for v in p)
+                ptr[i + 1] = len(flat)
+            flat = as_i32(flat if flat else [0])
+            self._nprobe = np.diff(ptr)
+            check(lib().nsa_bundle_tran_begin(self.h, float(tstep), float(tstop), i32p(ptr), i32p(flat),
+                                              int(max_points)))
+        else:
+            self._nprobe = np.zeros(self.N, np.int32)
+            check(lib().nsa_bundle_tran_begin(self.h, float(tstep), float(tstop), None, None, 1))
+
+    def tran_advance(self, t_until):
+        """Adaptive steps until every instance reaches t_until; True when all
+        instances are finished."""
+        r = C.c_int32()
+        rc = check(lib().nsa_bundle_tran_advance(self.h, float(t_until), C.byref(r)))
+        self.last_rounds = r.value
+        return rc == 1
+
+    def tran(self, tstep, tstop, probes=None, max_points=1):
+        self.tran_begin(tstep, tstop, probes, max_points)
+        self.tran_advance(tstop)
+
+    def tran_status(self, i):
+        o = np.zeros(8, np.int64)
+        check(lib().nsa_bundle_tran_status(self.h, int(i), o.ctypes.data_as(_lib.P_I64)))
+        return {"accepted": int(o[0]), "rejected": int(o[1]), "failed": bool(o[2]), "finished": bool(o[3]),
+                "newton_iters": int(o[4]), "all_iters": int(o[5]), "loop_passes": int(o[6]),
+                "dc_iters": int(o[7])}
+
+    def tran_steps(self, i):
+        """Accepted steps of instance i: dict of arrays t, dt, order, method, iters."""
+        n = self.tran_status(i)["accepted"]
+        t, dt = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        order, method, iters = (np.zeros(max(n, 1), np.int32) for _ in range(3))
+        check(lib().nsa_bundle_tran_steps(self.h, int(i), f64p(t), f64p(dt), i32p(order), i32p(method),
+                                          i32p(iters)))
+        return {"t": t[:n], "dt": dt[:n], "order": order[:n], "method": method[:n], "iters": iters[:n]}
+
+    def tran_probes(self, i):
+        """Recorded probe rows of instance i: (rows, nprobe) array (row 0: t = 0)."""
+        npb = int(self._nprobe[i])
+        rows = C.c_int32()
+        check(lib().nsa_bundle_tran_probes(self.h, int(i), C.byref(rows), None))
+        out = np.zeros((max(rows.value, 1), max(npb, 1)))
+        check(lib().nsa_bundle_tran_probes(self.h, int(i), C.byref(rows), f64p(out)))
+        return out[:rows.value, :npb]
+
+    def stats(self):
+        s = np.zeros(8, np.int64)
+        check(lib().nsa_bundle_stats(self.h, s.ctypes.data_as(_lib.P_I64)))
+        return {"iterations": int(s[0]), "cpu_pivot": int(s[1]), "plans": int(s[2]),
+                "graph_launches": int(s[3]), "host_iterations": int(s[4]),
+                "max_refactor_levels": int(s[5]), "max_solve_levels": int(s[6]), "max_nnz_lu": int(s[7])}
+
+    def set_timing(self, on=True):
+        check(lib().nsa_bundle_set_timing(self.h, int(bool(on))))
+
+    def timing(self):
+        t = np.zeros(5)
+        check(lib().nsa_bundle_timing(self.h, f64p(t)))
+        return {"load_ms": float(t[0]), "lu_ms": float(t[1]), "post_ms": float(t[2]),
+                "bsim4_ms": float(t[3]), "iterations_timed": int(t[4])}
+
+    def set_graph(self, on=True):
+        check(lib().nsa_bundle_set_graph(self.h, int(bool(on))))
+
+    def plan_of(self, i):
+        return check(lib().nsa_bundle_plan_of(self.h, int(i)))

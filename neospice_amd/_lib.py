@@ -93,6 +93,23 @@ _SIGS = {
     "nsa_pivot_structure": (C.c_int, [I32, I32, P_I32, P_I32, P_F64, C.c_double, P_I32, P_I32, P_I32,
                                       P_I32, P_I32, P_I32]),
     "nsa_min_degree_order": (C.c_int, [I32, P_I32, P_I32, P_I32]),
+    "nsa_bundle_create": (C.c_int, [C.POINTER(VP), I32, C.POINTER(Options), C.POINTER(VP)]),
+    "nsa_bundle_destroy": (None, [VP]),
+    "nsa_bundle_info": (C.c_int, [VP, P_I32]),
+    "nsa_bundle_offsets": (C.c_int, [VP, P_I32, P_I32]),
+    "nsa_bundle_dc": (C.c_int, [VP, I32, P_I32, P_I32]),
+    "nsa_bundle_get_solution": (C.c_int, [VP, P_F64]),
+    "nsa_bundle_set_solution": (C.c_int, [VP, P_F64]),
+    "nsa_bundle_tran_begin": (C.c_int, [VP, C.c_double, C.c_double, P_I32, P_I32, I32]),
+    "nsa_bundle_tran_advance": (C.c_int, [VP, C.c_double, P_I32]),
+    "nsa_bundle_tran_status": (C.c_int, [VP, I32, P_I64]),
+    "nsa_bundle_tran_steps": (C.c_int, [VP, I32, P_F64, P_F64, P_I32, P_I32, P_I32]),
+    "nsa_bundle_tran_probes": (C.c_int, [VP, I32, P_I32, P_F64]),
+    "nsa_bundle_stats": (C.c_int, [VP, P_I64]),
+    "nsa_bundle_set_timing": (C.c_int, [VP, I32]),
+    "nsa_bundle_timing": (C.c_int, [VP, P_F64]),
+    "nsa_bundle_set_graph": (C.c_int, [VP, I32]),
+    "nsa_bundle_plan_of": (C.c_int, [VP, I32]),
 }
 
 _lib = None

@@ -24,12 +24,16 @@ SOURCES = [
     "batch_lu.cpp",
     "engine.cpp",
     "adjoint.cpp",
+    "small_lu.cpp",
+    "bundle.cpp",
     "capi.cpp",
     "gpu_lu.hip",
     "gpu_dev.hip",
     "gpu_nonlinear.hip",
     "gpu_bsim4.hip",
     "gpu_adjoint.hip",
+    "gpu_small_lu.hip",
+    "gpu_bundle.hip",
 ]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]

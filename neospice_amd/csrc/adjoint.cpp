@@ -78,6 +78,7 @@ void BatchEngine::set_family_values(DevKind k, const double* h_vals) {
 
 void BatchEngine::get_family_values(DevKind k, double* h_vals) const {
     const size_t cnt = (size_t)family_count(k) * B_;
+    NSA_HIP(hipStreamSynchronize(stream_));  // values are written on the engine's stream
     if (cnt) NSA_HIP(hipMemcpy(h_vals, family_values_ptr(k), sizeof(double) * cnt, hipMemcpyDeviceToHost));
 }
 
@@ -137,6 +138,7 @@ void BatchEngine::dc_adjoint(const double* h_dLdx, double* h_lambda, double* h_g
     launch_grad_src(nV, d_v_branch_, nI, d_i_nodes_, d_ctx_, d_lam_, d_gV_, d_gI_, B_, stream_);
     NSA_HIP(hipStreamSynchronize(stream_));
     if (h_lambda) NSA_HIP(hipMemcpy(h_lambda, d_lam_, sizeof(double) * nB, hipMemcpyDeviceToHost));
+    NSA_HIP(hipStreamSynchronize(stream_));
     if (h_gR && nR) NSA_HIP(hipMemcpy(h_gR, d_gR_, sizeof(double) * nR * B_, hipMemcpyDeviceToHost));
     if (h_gC && nC) std::memset(h_gC, 0, sizeof(double) * nC * B_);  // capacitors are open in DC
     if (h_gV && nV) NSA_HIP(hipMemcpy(h_gV, d_gV_, sizeof(double) * nV * B_, hipMemcpyDeviceToHost));

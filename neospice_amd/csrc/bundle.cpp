@@ -1,0 +1,1195 @@
+#include "bundle.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <atomic>
+#include <exception>
+#include <thread>
+
+#include "common.hpp"
+#include "devices.hpp"
+#include "gpu_bundle.hpp"
+
+namespace nsa {
+
+namespace {
+
+template <class T>
+T* dalloc(std::vector<void*>& allocs, size_t count) {
+    T* p = nullptr;
+    NSA_HIP(hipMalloc(&p, std::max<size_t>(1, count) * sizeof(T)));
+    allocs.push_back(p);
+    return p;
+}
+
+template <class T>
+T* dupload(std::vector<void*>& allocs, const std::vector<T>& v) {
+    T* p = dalloc<T>(allocs, v.size());
+    if (!v.empty()) NSA_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return p;
+}
+
+void csr(const std::vector<std::vector<int32_t>>& L, std::vector<int32_t>& ptr, std::vector<int32_t>& idx) {
+    ptr.assign(L.size() + 1, 0);
+    for (size_t i = 0; i < L.size(); ++i) ptr[i + 1] = ptr[i] + (int32_t)L[i].size();
+    idx.clear();
+    idx.reserve(ptr.back());
+    for (auto& l : L) idx.insert(idx.end(), l.begin(), l.end());
+}
+
+// Runs f(i) for i < count on up to `threads` host threads.
+template <class F>
+void parallel_for(int count, F&& f) {
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    const int nt = std::min({hw, 16, count});
+    if (nt <= 1) {
+        for (int i = 0; i < count; ++i) f(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    std::atomic<int> next{0};
+    std::vector<std::exception_ptr> err(nt);
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&, t] {
+            try {
+                for (int i; (i = next.fetch_add(1)) < count;) f(i);
+            } catch (...) {
+                err[t] = std::current_exception();
+            }
+        });
+    for (auto& x : th) x.join();
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
+}
+
+// solve_transient's constants (transient.cpp:45-78).
+constexpr double kInitialStopDivisor = 100.0;
+constexpr double kInitialStepDivisor = 10.0;
+constexpr double kMinTimeStepRatio = 1e-11;
+constexpr int kMaxTransientIterations = 500000;
+constexpr int kNoRecentBreakpoint = 1000;
+constexpr double kNewtonFailureDtFactor = 8.0;
+constexpr double kDeviceLteRejectRatio = 0.9;
+constexpr double kMaxDtGrowthFactor = 2.0;
+constexpr int kRingingMinStepsAfterBp = 5;
+constexpr int kLteMinStepCount = 2;
+
+}  // namespace
+
+Bundle::Bundle(const std::vector<const Circuit*>& circuits, const Options& opt)
+    : ckt_(circuits), opt_(opt) {
+    NI_ = (int)circuits.size();
+    if (NI_ <= 0) throw Error(NSA_ERR_ARGUMENT, "Bundle: no instances");
+    for (const Circuit* c : circuits) {
+        if (!c) throw Error(NSA_ERR_ARGUMENT, "Bundle: null circuit");
+        if (!c->finalized()) throw Error(NSA_ERR_LOGIC, "Bundle: circuit not finalized");
+        if (c->family(DK_B4).empty() && c->family(DK_M1).empty() && c->family(DK_D).empty() &&
+            c->num_vars() >= 256 && c->is_linear() && opt.engine != ENGINE_MARKOWITZ)
+            throw Error(NSA_ERR_UNSUPPORTED,
+                        "Bundle: linear circuits of 256 or more unknowns use AMD-LU (BatchEngine)");
+        if (opt.engine == ENGINE_AMDLU)
+            throw Error(NSA_ERR_UNSUPPORTED, "Bundle: instances are factored with Sparse 1.3");
+    }
+    NSA_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    xoff_.assign(NI_ + 1, 0);
+    aoff_.assign(NI_ + 1, 0);
+    n_.resize(NI_);
+    nnodes_.resize(NI_);
+    for (int i = 0; i < NI_; ++i) {
+        n_[i] = ckt_[i]->num_vars();
+        nnodes_[i] = ckt_[i]->num_nodes();
+        xoff_[i + 1] = xoff_[i] + n_[i];
+        aoff_[i + 1] = aoff_[i] + ckt_[i]->nnz();
+    }
+    tot_n_ = xoff_[NI_];
+    tot_nnz_ = aoff_[NI_];
+    d_x_ = dalloc<double>(allocs_, tot_n_);
+    d_xold_ = dalloc<double>(allocs_, tot_n_);
+    d_rhs_ = dalloc<double>(allocs_, tot_n_);
+    d_A_ = dalloc<double>(allocs_, tot_nnz_);
+    d_LU_ = dalloc<double>(allocs_, (size_t)NI_ * kSluMaxLu);
+    NSA_HIP(hipMemset(d_x_, 0, sizeof(double) * tot_n_));
+    d_ctx_ = dalloc<DevCtx>(allocs_, NI_);
+    d_nt_ = dalloc<NtInst>(allocs_, NI_);
+    d_done_ = dalloc<int32_t>(allocs_, NI_);
+    d_devnc_ = dalloc<int32_t>(allocs_, NI_);
+    d_kind_ = dalloc<int32_t>(allocs_, NI_);
+    d_kind_host_ = dalloc<int32_t>(allocs_, NI_);
+    d_fail_ = dalloc<int32_t>(allocs_, NI_);
+    d_fail_col_ = dalloc<int32_t>(allocs_, NI_);
+    NSA_HIP(hipMemset(d_done_, 0, sizeof(int32_t) * NI_));
+    NSA_HIP(hipMemset(d_devnc_, 0, sizeof(int32_t) * NI_));
+    NSA_HIP(hipHostMalloc((void**)&h_ctx_pin_, sizeof(DevCtx) * NI_, hipHostMallocDefault));
+    NSA_HIP(hipHostMalloc((void**)&h_nt_pin_, sizeof(NtInst) * NI_, hipHostMallocDefault));
+    NSA_HIP(hipHostMalloc((void**)&h_kind_pin_, sizeof(int32_t) * 2 * NI_, hipHostMallocDefault));
+    std::vector<int32_t> nn(n_.begin(), n_.end()), nd(nnodes_.begin(), nnodes_.end());
+    d_ixoff_ = dupload(allocs_, std::vector<int32_t>(xoff_.begin(), xoff_.end()));
+    d_in_ = dupload(allocs_, nn);
+    d_inn_ = dupload(allocs_, nd);
+    tol_.gmin = opt_.gmin;
+    tol_.reltol = opt_.reltol;
+    tol_.abstol = opt_.abstol;
+    build_tables();
+    plan_.assign(NI_, -1);
+    s13_.resize(NI_);
+    h_inst_.resize(NI_);
+    for (int i = 0; i < NI_; ++i) h_inst_[i] = SluInst{-1, aoff_[i], i * kSluMaxLu, xoff_[i]};
+    d_inst_ = dalloc<SluInst>(allocs_, NI_);
+    upload_inst();
+    DevCtx c{};
+    c.mode = MODEDCOP | MODEINITJCT;
+    c.order = 1;
+    c.xmu_ratio = 1.0;
+    c.src_fact = opt_.src_fact;
+    set_ctx(std::vector<DevCtx>(NI_, c));
+    NSA_HIP(hipStreamSynchronize(stream_));
+}
+
+Bundle::~Bundle() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
+    for (void* p : tran_allocs_) (void)hipFree(p);
+    for (void* p : allocs_) (void)hipFree(p);
+    if (h_ctx_pin_) (void)hipHostFree(h_ctx_pin_);
+    if (h_nt_pin_) (void)hipHostFree(h_nt_pin_);
+    if (h_kind_pin_) (void)hipHostFree(h_kind_pin_);
+    if (h_acc_pin_) (void)hipHostFree(h_acc_pin_);
+    if (h_tmin_pin_) (void)hipHostFree(h_tmin_pin_);
+    if (h_ring_pin_) (void)hipHostFree(h_ring_pin_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+// Packed tables.  BSIM4v7 devices of every instance form one lane-strided
+// batch (slot k of lane g at k * G + g, in the site, state, constant and
+// auxiliary arrays, which it heads); every other family is a batch of one
+// with its devices' sites and states packed after it.
+void Bundle::build_tables() {
+    DeviceEnv env;
+    env.gmin = opt_.gmin;
+    env.reltol = opt_.reltol;
+    env.abstol = opt_.abstol;
+    env.vntol = opt_.vntol;
+    int G4 = 0;
+    for (const Circuit* c : ckt_) G4 += (int)c->family(DK_B4).size();
+    const int nsB4 = state_count(DK_B4);
+    // per (instance, device): first site / rhs site / state, and the stride
+    // of the device's local slots (G4 for BSIM4v7, 1 otherwise)
+    std::vector<std::vector<int32_t>> site(NI_), rsite(NI_), sbase(NI_), stride(NI_);
+    for (int i = 0; i < NI_; ++i) {
+        const size_t nd = ckt_[i]->devices().size();
+        site[i].assign(nd, -1);
+        rsite[i].assign(nd, -1);
+        sbase[i].assign(nd, -1);
+        stride[i].assign(nd, 1);
+    }
+    // ---- BSIM4v7, lane-strided
+    std::vector<int32_t> b4_pin(4 * (size_t)G4), b4_inst(G4);
+    std::vector<double> b4_par((size_t)kB4Fields * G4);
+    {
+        int g = 0;
+        for (int i = 0; i < NI_; ++i) {
+            const Circuit& c = *ckt_[i];
+            for (int d : c.family(DK_B4)) {
+                const Device& dv = c.devices()[d];
+                for (int k = 0; k < 4; ++k)
+                    b4_pin[(size_t)k * G4 + g] = dv.node[k] < 0 ? -1 : xoff_[i] + dv.node[k];
+                b4_inst[g] = i;
+                const B4Param p = bsim4_param(c.models().at(dv.model), dv, env);
+                const double* f = reinterpret_cast<const double*>(&p);
+                for (int q = 0; q < kB4Fields; ++q) b4_par[(size_t)q * G4 + g] = f[q];
+                site[i][d] = g;
+                rsite[i][d] = g;
+                sbase[i][d] = g;
+                stride[i][d] = G4;
+                ++g;
+            }
+        }
+    }
+    int ms = 16 * G4, rs = 4 * G4, st = nsB4 * G4;
+    const long ns_lane = st;
+    std::vector<int32_t> slot_inst;
+    // ---- resistors
+    std::vector<double> r_ohms;
+    r_site_ = ms;
+    for (int i = 0; i < NI_; ++i) {
+        const Circuit& c = *ckt_[i];
+        for (int d : c.family(DK_R)) { site[i][d] = ms; ms += 4; r_ohms.push_back(c.devices()[d].value); }
+    }
+    nR_ = (int)r_ohms.size();
+    // ---- capacitors
+    std::vector<int32_t> cap_nodes, cap_state, cap_inst;
+    std::vector<double> cap_val;
+    cap_.site = ms;
+    cap_.rsite = rs;
+    for (int i = 0; i < NI_; ++i) {
+        const Circuit& c = *ckt_[i];
+        for (int d : c.family(DK_C)) {
+            const Device& dv = c.devices()[d];
+            site[i][d] = ms; ms += 4;
+            rsite[i][d] = rs; rs += 2;
+            sbase[i][d] = st;
+            for (int k = 0; k < state_count(DK_C); ++k) slot_inst.push_back(i);
+            cap_state.push_back(st); st += state_count(DK_C);
+            for (int k = 0; k < 2; ++k) cap_nodes.push_back(dv.node[k] < 0 ? -1 : xoff_[i] + dv.node[k]);
+            cap_val.push_back(dv.value);
+            cap_inst.push_back(i);
+        }
+    }
+    cap_.count = (int)cap_val.size();
+    // ---- diodes
+    std::vector<int32_t> dio_nodes, dio_state, dio_inst;
+    std::vector<DioParam> dio_par;
+    dio_.site = ms;
+    dio_.rsite = rs;
+    for (int i = 0; i < NI_; ++i) {
+        const Circuit& c = *ckt_[i];
+        for (int d : c.family(DK_D)) {
+            const Device& dv = c.devices()[d];
+            site[i][d] = ms; ms += 7;
+            rsite[i][d] = rs; rs += 2;
+            sbase[i][d] = st;
+            for (int k = 0; k < state_count(DK_D); ++k) slot_inst.push_back(i);
+            dio_state.push_back(st); st += state_count(DK_D);
+            for (int k = 0; k < 2; ++k) dio_nodes.push_back(dv.node[k] < 0 ? -1 : xoff_[i] + dv.node[k]);
+            dio_par.push_back(diode_param(c.models().at(dv.model), dv, env));
+            dio_inst.push_back(i);
+        }
+    }
+    dio_.count = (int)dio_par.size();
+    // ---- MOS1
+    std::vector<int32_t> mos_nodes, mos_state, mos_inst;
+    std::vector<Mos1Param> mos_par;
+    mos_.site = ms;
+    mos_.rsite = rs;
+    for (int i = 0; i < NI_; ++i) {
+        const Circuit& c = *ckt_[i];
+        for (int d : c.family(DK_M1)) {
+            const Device& dv = c.devices()[d];
+            site[i][d] = ms; ms += 22;
+            rsite[i][d] = rs; rs += 4;
+            sbase[i][d] = st;
+            for (int k = 0; k < state_count(DK_M1); ++k) slot_inst.push_back(i);
+            mos_state.push_back(st); st += state_count(DK_M1);
+            for (int k = 0; k < 4; ++k) mos_nodes.push_back(dv.node[k] < 0 ? -1 : xoff_[i] + dv.node[k]);
+            mos_par.push_back(mos1_param(c.models().at(dv.model), dv, env));
+            mos_inst.push_back(i);
+        }
+    }
+    mos_.count = (int)mos_par.size();
+    // ---- constant matrix sites: V source +-1, dead-node pins
+    const_site_ = ms;
+    std::vector<double> const_vals;
+    const double pin = (opt_.gmin > 0.0) ? opt_.gmin : 1e-12;  // newton.cpp:80
+    std::vector<std::vector<int32_t>> pin_site(NI_);
+    for (int i = 0; i < NI_; ++i) {
+        const Circuit& c = *ckt_[i];
+        for (int d : c.family(DK_V)) {
+            site[i][d] = ms; ms += 4;
+            const_vals.insert(const_vals.end(), {1.0, -1.0, 1.0, -1.0});
+        }
+        for (size_t k = 0; k < c.dead_nodes().size(); ++k) {
+            pin_site[i].push_back(ms++);
+            const_vals.push_back(pin);
+        }
+    }
+    n_const_ = (int)const_vals.size();
+    n_msites_ = ms;
+    // ---- voltage and current sources (rhs sites)
+    std::vector<int32_t> vw, vdg, pwl_ptr{0}, v_inst, i_inst;
+    std::vector<double> vdc, pwl_t, pwl_v, ival;
+    vsrc_.rsite = rs;
+    h_pulse_.clear();
+    for (int i = 0; i < NI_; ++i) {
+        const Circuit& c = *ckt_[i];
+        for (int d : c.family(DK_V)) {
+            const Device& dv = c.devices()[d];
+            rsite[i][d] = rs; rs += 1;
+            vw.push_back(dv.wave);
+            vdg.push_back(0);
+            vdc.push_back(dv.value);
+            for (int k = 0; k < 7; ++k) h_pulse_.push_back(dv.pulse[k]);
+            pwl_t.insert(pwl_t.end(), dv.pwl_t.begin(), dv.pwl_t.end());
+            pwl_v.insert(pwl_v.end(), dv.pwl_v.begin(), dv.pwl_v.end());
+            pwl_ptr.push_back((int32_t)pwl_t.size());
+            v_inst.push_back(i);
+        }
+    }
+    vsrc_.count = (int)vdc.size();
+    isrc_.rsite = rs;
+    for (int i = 0; i < NI_; ++i) {
+        const Circuit& c = *ckt_[i];
+        for (int d : c.family(DK_I)) {
+            rsite[i][d] = rs; rs += 2;
+            ival.push_back(c.devices()[d].value);
+            i_inst.push_back(i);
+        }
+    }
+    isrc_.count = (int)ival.size();
+    n_rsites_ = rs;
+    ns_tot_ = st;
+    // ---- gathers in each instance's load order
+    std::vector<std::vector<int32_t>> slot_sites(tot_nnz_), rows_d(tot_n_), rows_u(tot_n_);
+    for (int i = 0; i < NI_; ++i) {
+        const Circuit& c = *ckt_[i];
+        const auto& devs = c.devices();
+        for (int d : c.load_order()) {
+            const Device& dv = devs[d];
+            const int sd = stride[i][d];
+            for (size_t k = 0; k < dv.mat_slot.size(); ++k)
+                if (dv.mat_slot[k] >= 0)
+                    slot_sites[aoff_[i] + dv.mat_slot[k]].push_back(site[i][d] + (int)k * sd);
+            auto& rows = (dv.kind == DK_D || dv.kind == DK_M1 || dv.kind == DK_B4) ? rows_u : rows_d;
+            for (size_t k = 0; k < dv.rhs_row.size(); ++k)
+                if (dv.rhs_row[k] >= 0) rows[xoff_[i] + dv.rhs_row[k]].push_back(rsite[i][d] + (int)k * sd);
+        }
+        for (size_t k = 0; k < c.dead_nodes().size(); ++k) {
+            const int nd = c.dead_nodes()[k];
+            slot_sites[aoff_[i] + c.offset(nd, nd)].push_back(pin_site[i][k]);
+        }
+    }
+    // every index the kernels will follow stays inside its array
+    auto in_range = [](const std::vector<int32_t>& v, long lo, long hi, const char* what) {
+        for (int32_t k : v)
+            if (k < lo || k >= hi) throw Error(NSA_ERR_RUNTIME, std::string("Bundle tables: ") + what);
+    };
+    for (const auto& l : slot_sites) in_range(l, 0, n_msites_, "matrix site");
+    for (const auto& l : rows_d) in_range(l, 0, n_rsites_, "rhs site");
+    for (const auto& l : rows_u) in_range(l, 0, n_rsites_, "rhs site");
+    in_range(b4_pin, -1, tot_n_, "BSIM4v7 terminal");
+    in_range(cap_nodes, -1, tot_n_, "capacitor node");
+    in_range(dio_nodes, -1, tot_n_, "diode node");
+    in_range(mos_nodes, -1, tot_n_, "MOS1 node");
+    in_range(cap_state, 0, ns_tot_ - state_count(DK_C) + 1, "capacitor state");
+    in_range(dio_state, 0, ns_tot_ - state_count(DK_D) + 1, "diode state");
+    in_range(mos_state, 0, ns_tot_ - state_count(DK_M1) + 1, "MOS1 state");
+    if ((long)slot_inst.size() != ns_tot_ - ns_lane) throw Error(NSA_ERR_RUNTIME, "Bundle tables: state owners");
+    if (r_site_ + 4L * nR_ > const_site_ || cap_.site + 4L * cap_.count > dio_.site ||
+        dio_.site + 7L * dio_.count > mos_.site || mos_.site + 22L * mos_.count > const_site_ ||
+        const_site_ + (long)n_const_ != n_msites_)
+        throw Error(NSA_ERR_RUNTIME, "Bundle tables: matrix site blocks");
+    if (cap_.rsite + 2L * cap_.count > dio_.rsite || dio_.rsite + 2L * dio_.count > mos_.rsite ||
+        mos_.rsite + 4L * mos_.count > vsrc_.rsite || vsrc_.rsite + (long)vsrc_.count > isrc_.rsite ||
+        isrc_.rsite + 2L * isrc_.count != n_rsites_)
+        throw Error(NSA_ERR_RUNTIME, "Bundle tables: rhs site blocks");
+    std::vector<int32_t> p, ix;
+    csr(slot_sites, p, ix);
+    d_mptr_ = dupload(allocs_, p);
+    d_msites_ = dupload(allocs_, ix);
+    csr(rows_d, p, ix);
+    d_rptr_d_ = dupload(allocs_, p);
+    d_rsites_d_ = dupload(allocs_, ix);
+    csr(rows_u, p, ix);
+    d_rptr_u_ = dupload(allocs_, p);
+    d_rsites_u_ = dupload(allocs_, ix);
+    // ---- device arrays
+    d_cm_ = dalloc<double>(allocs_, n_msites_);
+    d_cr_ = dalloc<double>(allocs_, n_rsites_);
+    NSA_HIP(hipMemset(d_cm_, 0, sizeof(double) * std::max(1, n_msites_)));
+    NSA_HIP(hipMemset(d_cr_, 0, sizeof(double) * std::max(1, n_rsites_)));
+    for (int k = 0; k < 4; ++k) {
+        d_s_[k] = dalloc<double>(allocs_, ns_tot_);
+        NSA_HIP(hipMemset(d_s_[k], 0, sizeof(double) * std::max(1, ns_tot_)));
+    }
+    d_r_ohms_ = dupload(allocs_, r_ohms);
+    cap_.nodes = dupload(allocs_, cap_nodes);
+    cap_.state = dupload(allocs_, cap_state);
+    cap_.farads = dupload(allocs_, cap_val);
+    cap_.inst_of = dupload(allocs_, cap_inst);
+    vsrc_.wave = dupload(allocs_, vw);
+    vsrc_.dc_given = dupload(allocs_, vdg);
+    vsrc_.dc = dupload(allocs_, vdc);
+    vsrc_.pulse = dupload(allocs_, h_pulse_);
+    vsrc_.pwl_ptr = dupload(allocs_, pwl_ptr);
+    vsrc_.pwl_t = dupload(allocs_, pwl_t);
+    vsrc_.pwl_v = dupload(allocs_, pwl_v);
+    vsrc_.inst_of = dupload(allocs_, v_inst);
+    isrc_.dc = dupload(allocs_, ival);
+    isrc_.inst_of = dupload(allocs_, i_inst);
+    d_const_vals_ = dupload(allocs_, const_vals);
+    dio_.nodes = dupload(allocs_, dio_nodes);
+    dio_.state = dupload(allocs_, dio_state);
+    dio_.par = dupload(allocs_, dio_par);
+    dio_.inst_of = dupload(allocs_, dio_inst);
+    mos_.nodes = dupload(allocs_, mos_nodes);
+    mos_.state = dupload(allocs_, mos_state);
+    mos_.par = dupload(allocs_, mos_par);
+    mos_.inst_of = dupload(allocs_, mos_inst);
+    mos_.aux = dalloc<double>(allocs_, (size_t)std::max(1, mos_.count) * kMos1Aux);
+    NSA_HIP(hipMemset(mos_.aux, 0, sizeof(double) * std::max(1, mos_.count) * kMos1Aux));
+    // BSIM4v7 as one device of G4 lanes
+    b4_.count = G4 > 0 ? 1 : 0;
+    b4_.nodes = dupload(allocs_, std::vector<int32_t>{-1, -1, -1, -1});
+    b4_.state = dupload(allocs_, std::vector<int32_t>{0});
+    b4_.par = nullptr;
+    b4_.par_inst = dupload(allocs_, b4_par);
+    b4_.aux = dalloc<double>(allocs_, (size_t)kB4Aux * std::max(1, G4));
+    NSA_HIP(hipMemset(b4_.aux, 0, sizeof(double) * kB4Aux * std::max(1, G4)));
+    b4_.site = 0;
+    b4_.rsite = 0;
+    b4_.inst_of = dupload(allocs_, b4_inst);
+    b4_.pin = dupload(allocs_, b4_pin);
+    b4_lanes_ = G4;
+    // truncation-error rows: capacitors, diodes, MOS1 (batch of one), BSIM4v7 (lanes)
+    std::vector<int32_t> tb, tk, ti;
+    for (size_t k = 0; k < cap_state.size(); ++k) { tb.push_back(cap_state[k]); tk.push_back(0); ti.push_back(cap_inst[k]); }
+    for (size_t k = 0; k < dio_state.size(); ++k) { tb.push_back(dio_state[k]); tk.push_back(2); ti.push_back(dio_inst[k]); }
+    for (size_t k = 0; k < mos_state.size(); ++k) { tb.push_back(mos_state[k]); tk.push_back(1); ti.push_back(mos_inst[k]); }
+    trunc1_.ndev = (int)tb.size();
+    trunc1_.base = dupload(allocs_, tb);
+    trunc1_.kind = dupload(allocs_, tk);
+    trunc1_.inst_of = dupload(allocs_, ti);
+    trunc4_.ndev = G4 > 0 ? 1 : 0;
+    trunc4_.base = b4_.state;
+    trunc4_.kind = dupload(allocs_, std::vector<int32_t>{3});
+    trunc4_.inst_of = b4_.inst_of;
+    ns_lane_ = ns_lane;
+    d_slot_inst_ = dupload(allocs_, slot_inst);
+    // constant sites
+    launch_res_sites(nR_, d_r_ohms_, d_cm_, r_site_, 1, stream_);
+    launch_const_sites(n_const_, d_const_vals_, d_cm_, const_site_, 1, stream_);
+}
+
+void Bundle::upload_inst() {
+    for (int i = 0; i < NI_; ++i) h_inst_[i].plan = plan_[i];
+    NSA_HIP(hipMemcpyAsync(d_inst_, h_inst_.data(), sizeof(SluInst) * NI_, hipMemcpyHostToDevice,
+                           stream_));
+    NSA_HIP(hipStreamSynchronize(stream_));
+    inst_dirty_ = false;
+}
+
+void Bundle::set_ctx(const std::vector<DevCtx>& c) {
+    if ((int)c.size() != NI_) throw Error(NSA_ERR_ARGUMENT, "Bundle::set_ctx: one context per instance");
+    h_ctx_ = c;
+    NSA_HIP(hipStreamSynchronize(stream_));
+    std::memcpy(h_ctx_pin_, h_ctx_.data(), sizeof(DevCtx) * NI_);
+    NSA_HIP(hipMemcpyAsync(d_ctx_, h_ctx_pin_, sizeof(DevCtx) * NI_, hipMemcpyHostToDevice, stream_));
+}
+
+void Bundle::reset_solver() {
+    plan_.assign(NI_, -1);
+    for (auto& s : s13_) s.reset();
+    inst_dirty_ = true;
+}
+
+void Bundle::get_solution(double* out) const {
+    NSA_HIP(hipMemcpyAsync(out, d_x_, sizeof(double) * tot_n_, hipMemcpyDeviceToHost, stream_));
+    NSA_HIP(hipStreamSynchronize(stream_));
+}
+
+void Bundle::set_solution(const double* in) {
+    NSA_HIP(hipMemcpyAsync(d_x_, in, sizeof(double) * tot_n_, hipMemcpyHostToDevice, stream_));
+    NSA_HIP(hipStreamSynchronize(stream_));
+}
+
+void Bundle::get_state(int which, double* out) const {
+    if (which < 0 || which > 3) throw Error(NSA_ERR_ARGUMENT, "Bundle::get_state: which in 0..3");
+    if (!ns_tot_) return;
+    NSA_HIP(hipMemcpyAsync(out, d_s_[which], sizeof(double) * ns_tot_, hipMemcpyDeviceToHost, stream_));
+    NSA_HIP(hipStreamSynchronize(stream_));
+}
+
+void Bundle::set_timing(bool on) {
+    if (on && !ev_[0])
+        for (auto& e : ev_) NSA_HIP(hipEventCreate(&e));
+    timing_ = on;
+    ms_load = ms_lu = ms_post = ms_bsim4 = 0.0;
+    n_timed = 0;
+}
+
+// ---------------------------------------------------------------- iteration
+
+void Bundle::launch_begin() {
+    launch_nt_begin(d_nt_, NI_, d_ixoff_, d_in_, d_x_, d_xold_, d_ctx_, d_kind_, d_fail_, d_devnc_,
+                    d_done_, stream_);
+}
+
+void Bundle::launch_loads() {
+    double* s0 = d_s_[0];
+    double* s1 = d_s_[1];
+    double* s2 = d_s_[2];
+    launch_cap(cap_, d_ctx_, d_x_, s0, s1, s2, d_done_, d_cm_, d_cr_, 1, stream_);
+    launch_vsrc(vsrc_, d_ctx_, d_done_, d_cr_, 1, stream_);
+    launch_isrc(isrc_, d_ctx_, d_done_, d_cr_, 1, stream_);
+    launch_diode(dio_, d_ctx_, d_x_, s0, s1, s2, d_done_, d_cm_, d_cr_, d_devnc_, tol_, 1, stream_);
+    launch_mos1(mos_, d_ctx_, d_x_, s0, s1, s2, d_done_, d_cm_, d_cr_, d_devnc_, tol_, 1, stream_);
+    if (timing_ && b4_.count) NSA_HIP(hipEventRecord(ev_[1], stream_));
+    launch_bsim4(b4_, d_ctx_, d_x_, s0, s1, s2, d_done_, d_cm_, d_cr_, d_devnc_, tol_, b4_lanes_,
+                 stream_);
+    if (timing_ && b4_.count) NSA_HIP(hipEventRecord(ev_[2], stream_));
+    launch_gather(tot_nnz_, d_mptr_, d_msites_, nullptr, nullptr, d_cm_, nullptr, d_A_, 1, stream_);
+    launch_gather(tot_n_, d_rptr_d_, d_rsites_d_, d_rptr_u_, d_rsites_u_, d_cr_, d_cr_, d_rhs_, 1,
+                  stream_);
+}
+
+void Bundle::launch_lu(const int32_t* d_kind, int mode) {
+    launch_small_lu(plans_, d_inst_, NI_, mode, d_A_, opt_.diag_gmin, d_kind, d_fail_, d_fail_col_,
+                    d_LU_, d_rhs_, d_x_, stream_);
+}
+
+void Bundle::launch_post(int) {
+    // MOS1Device::conv_test on the proposed solution (skipped where nothing was solved)
+    launch_mos1_conv(mos_, d_x_, d_s_[0], d_fail_, d_devnc_, tol_, 1, stream_);
+    launch_nt_post(d_nt_, NI_, d_ixoff_, d_in_, d_inn_, d_x_, d_xold_, d_ctx_, d_kind_, d_fail_,
+                   d_devnc_, d_done_, NtConvTol{opt_.reltol, opt_.vntol, opt_.abstol}, stream_);
+}
+
+void Bundle::capture_graph() {
+    if (graph_exec_) { NSA_HIP(hipGraphExecDestroy(graph_exec_)); graph_exec_ = nullptr; }
+    if (graph_) { NSA_HIP(hipGraphDestroy(graph_)); graph_ = nullptr; }
+    NSA_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    launch_begin();
+    launch_loads();
+    launch_lu(d_kind_, SLU_FACTOR | SLU_SOLVE);
+    launch_post(0);
+    NSA_HIP(hipStreamEndCapture(stream_, &graph_));
+    NSA_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+    graph_words_ = plans_.d_words();
+    graph_nlu_ = plans_.max_nlu();
+    graph_n_ = plans_.max_n();
+}
+
+void Bundle::iteration_device() {
+    if (inst_dirty_) upload_inst();
+    if (timing_) {
+        NSA_HIP(hipEventRecord(ev_[0], stream_));
+        launch_begin();
+        launch_loads();
+        NSA_HIP(hipEventRecord(ev_[3], stream_));
+        launch_lu(d_kind_, SLU_FACTOR | SLU_SOLVE);
+        NSA_HIP(hipEventRecord(ev_[4], stream_));
+        launch_post(0);
+        NSA_HIP(hipEventRecord(ev_[5], stream_));
+        timed_pending_ = true;
+        return;
+    }
+    if (use_graph) {
+        if (!graph_exec_ || graph_words_ != plans_.d_words() || graph_nlu_ != plans_.max_nlu() ||
+            graph_n_ != plans_.max_n())
+            capture_graph();
+        NSA_HIP(hipGraphLaunch(graph_exec_, stream_));
+        ++n_graph_launches;
+        return;
+    }
+    launch_begin();
+    launch_loads();
+    launch_lu(d_kind_, SLU_FACTOR | SLU_SOLVE);
+    launch_post(0);
+}
+
+std::vector<double> Bundle::instance_matrix(int i) {
+    std::vector<double> v(aoff_[i + 1] - aoff_[i]);
+    NSA_HIP(hipMemcpyAsync(v.data(), d_A_ + aoff_[i], sizeof(double) * v.size(), hipMemcpyDeviceToHost,
+                           stream_));
+    NSA_HIP(hipStreamSynchronize(stream_));
+    return v;
+}
+
+// Host part of an iteration for the instances the device cannot finish on
+// its own: the first factorization of a fresh solver (called right after the
+// iteration's loads, before any LU) and numeric() calls whose reused pivot
+// order failed the threshold test (stalled after the LU).  Each instance in
+// `repivot` is re-pivoted on the CPU from its own matrix as
+// NeoSolver::numeric does (a fresh ordering, or the reuse path on the
+// instance's own state), then factored and solved on the GPU on its new
+// plan; with `others`, every other running instance takes its LU call of
+// this iteration too.  The iteration then ends on the device as usual.
+// `nt` is the host copy of the device state (current on entry).
+void Bundle::host_lu_post(std::vector<NtInst>& nt, const std::vector<int>& repivot, bool others) {
+    std::vector<double> A(tot_nnz_);
+    NSA_HIP(hipMemcpyAsync(A.data(), d_A_, sizeof(double) * tot_nnz_, hipMemcpyDeviceToHost, stream_));
+    NSA_HIP(hipStreamSynchronize(stream_));
+    std::vector<int32_t> sing(repivot.size(), 0);
+    std::vector<LuStructure> st(repivot.size());
+    parallel_for((int)repivot.size(), [&](int t) {
+        const int i = repivot[t];
+        const Circuit& c = *ckt_[i];
+        if (!s13_[i])
+            s13_[i] = std::make_unique<Sparse13>(c.num_vars(), c.col_ptr().data(), c.row_idx().data());
+        if (s13_[i]->numeric(A.data() + aoff_[i], opt_.diag_gmin) != 0) {
+            sing[t] = 1;
+            return;
+        }
+        st[t] = s13_[i]->structure();
+    });
+    n_cpu_pivot += (long)repivot.size();
+    std::vector<int32_t> kinds(NI_, SLU_SKIP), fails(NI_, -1);
+    if (others)
+        for (int i = 0; i < NI_; ++i)
+            if (!nt[i].done && !nt[i].stalled && plan_[i] >= 0) {
+                kinds[i] = nt[i].kind;
+                fails[i] = 0;
+            }
+    std::vector<int> failed;
+    for (size_t t = 0; t < repivot.size(); ++t) {
+        const int i = repivot[t];
+        NtInst& s = nt[i];
+        if (sing[t]) {
+            // singular numeric(): the attempt fails (newton.cpp:220-224)
+            s.iter += 1;
+            s.converged = 0;
+            s.done = 1;
+            s.stalled = 0;
+            kinds[i] = SLU_SKIP;
+            failed.push_back(i);
+            continue;
+        }
+        if (!SmallLuPlans::fits(st[t]))
+            throw Error(NSA_ERR_UNSUPPORTED, "Bundle: an instance's factor exceeds the packed LU limits");
+        plan_[i] = plans_.add(st[t]);
+        s.stalled = 0;
+        s.kind = SLU_TRUSTED;
+        kinds[i] = SLU_TRUSTED;
+        fails[i] = 0;
+    }
+    plans_.sync(stream_);
+    upload_inst();
+    std::memcpy(h_nt_pin_, nt.data(), sizeof(NtInst) * NI_);
+    NSA_HIP(hipMemcpyAsync(d_nt_, h_nt_pin_, sizeof(NtInst) * NI_, hipMemcpyHostToDevice, stream_));
+    std::memcpy(h_kind_pin_, kinds.data(), sizeof(int32_t) * NI_);
+    std::memcpy(h_kind_pin_ + NI_, fails.data(), sizeof(int32_t) * NI_);
+    NSA_HIP(hipMemcpyAsync(d_kind_, h_kind_pin_, sizeof(int32_t) * NI_, hipMemcpyHostToDevice, stream_));
+    NSA_HIP(hipMemcpyAsync(d_fail_, h_kind_pin_ + NI_, sizeof(int32_t) * NI_, hipMemcpyHostToDevice,
+                           stream_));
+    // a failed attempt leaves the caller's mode in the context (the pinned
+    // context still holds it) and its completion flag set
+    for (int i : failed) {
+        NSA_HIP(hipMemcpyAsync(&d_ctx_[i].mode, &h_ctx_pin_[i].mode, sizeof(int32_t),
+                               hipMemcpyHostToDevice, stream_));
+        NSA_HIP(hipMemcpyAsync(d_done_ + i, &nt[i].done, sizeof(int32_t), hipMemcpyHostToDevice,
+                               stream_));
+    }
+    launch_lu(d_kind_, SLU_FACTOR | SLU_SOLVE);
+    launch_post(0);
+    NSA_HIP(hipStreamSynchronize(stream_));
+    ++n_host_iterations;
+}
+
+void Bundle::collect_timing() {
+    if (!timed_pending_) return;
+    float ms = 0.f;
+    NSA_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[3]));
+    ms_load += ms;
+    NSA_HIP(hipEventElapsedTime(&ms, ev_[3], ev_[4]));
+    ms_lu += ms;
+    NSA_HIP(hipEventElapsedTime(&ms, ev_[4], ev_[5]));
+    ms_post += ms;
+    if (b4_.count) {
+        NSA_HIP(hipEventElapsedTime(&ms, ev_[1], ev_[2]));
+        ms_bsim4 += ms;
+    }
+    ++n_timed;
+    timed_pending_ = false;
+}
+
+void Bundle::newton(int itl, const int32_t* active, std::vector<int32_t>& converged,
+                    std::vector<int32_t>& iterations) {
+    std::vector<NtInst> nt(NI_);
+    for (int i = 0; i < NI_; ++i) {
+        NtInst s{};
+        s.saved_mode = h_ctx_[i].mode;
+        s.done = (active && !active[i]) ? 1 : 0;
+        s.itl = itl;
+        s.kind = SLU_SKIP;
+        nt[i] = s;
+    }
+    NSA_HIP(hipStreamSynchronize(stream_));
+    std::memcpy(h_nt_pin_, nt.data(), sizeof(NtInst) * NI_);
+    NSA_HIP(hipMemcpyAsync(d_nt_, h_nt_pin_, sizeof(NtInst) * NI_, hipMemcpyHostToDevice, stream_));
+    auto readback = [&]() {
+        NSA_HIP(hipMemcpyAsync(h_nt_pin_, d_nt_, sizeof(NtInst) * NI_, hipMemcpyDeviceToHost, stream_));
+        NSA_HIP(hipStreamSynchronize(stream_));
+        collect_timing();
+        std::memcpy(nt.data(), h_nt_pin_, sizeof(NtInst) * NI_);
+    };
+    bool first = true;
+    // every round advances each running instance by at least one iteration
+    for (int round = 0; round <= itl + 1; ++round) {
+        std::vector<int> fresh;
+        for (int i = 0; i < NI_; ++i)
+            if (!nt[i].done && plan_[i] < 0) fresh.push_back(i);
+        if (!fresh.empty()) {
+            // this iteration's loads, then the CPU orderings and every LU
+            if (inst_dirty_) upload_inst();
+            launch_begin();
+            launch_loads();
+            readback();
+            host_lu_post(nt, fresh, true);
+            readback();
+        } else {
+            // two iterations before the first look (iteration 0 never
+            // converges); one at a time while HIP events time the phases
+            const int k = (first && !timing_) ? 2 : 1;
+            for (int r = 0; r < k; ++r) iteration_device();
+            readback();
+        }
+        first = false;
+        std::vector<int> stalled;
+        for (int i = 0; i < NI_; ++i)
+            if (!nt[i].done && nt[i].stalled) stalled.push_back(i);
+        if (!stalled.empty()) {
+            host_lu_post(nt, stalled, false);
+            readback();
+        }
+        bool all = true;
+        for (int i = 0; i < NI_; ++i) all &= nt[i].done != 0;
+        if (all) break;
+    }
+    converged.assign(NI_, 0);
+    iterations.assign(NI_, 0);
+    for (int i = 0; i < NI_; ++i) {
+        if (active && !active[i]) continue;
+        if (!nt[i].done) throw Error(NSA_ERR_RUNTIME, "Bundle::newton: an instance did not finish");
+        converged[i] = nt[i].converged;
+        iterations[i] = nt[i].iter;
+        n_iterations += nt[i].iter;
+    }
+}
+
+void Bundle::dc_op(int mode, std::vector<int32_t>& converged, std::vector<int32_t>& iterations) {
+    std::vector<DevCtx> c(NI_);
+    for (int i = 0; i < NI_; ++i) {
+        DevCtx x{};
+        x.mode = mode;
+        x.order = 1;
+        x.method = opt_.method;
+        x.xmu_ratio = 1.0;
+        x.src_fact = opt_.src_fact;
+        c[i] = x;
+    }
+    set_ctx(c);
+    newton(opt_.itl1, nullptr, converged, iterations);
+}
+
+
+// ---------------------------------------------------------------- transient
+
+namespace {
+
+// TimeStepController (src/core/timestep.cpp), restated on TranCtl.
+void add_source_breakpoint(TranCtl& c, double t, bool hard) {
+    if (t > c.time && t <= c.tstop) {
+        // CKTminBreak: a breakpoint this close to an existing one only promotes it
+        if (c.min_break > 0.0) {
+            auto lb = c.src_breakpoints.lower_bound(t - c.min_break);
+            if (lb != c.src_breakpoints.end() && lb->first <= t + c.min_break) {
+                if (hard) lb->second = true;
+                return;
+            }
+        }
+        c.breakpoints.insert(t);
+        auto it = c.src_breakpoints.find(t);
+        if (it != c.src_breakpoints.end()) {
+            if (hard) it->second = true;
+        } else {
+            c.src_breakpoints.emplace(t, hard);
+        }
+    }
+}
+
+double next_breakpoint_gap(const TranCtl& c) {
+    if (!c.src_breakpoints.empty()) {
+        const double bp = c.src_breakpoints.begin()->first;
+        if (bp > c.time + 1e-18) return bp - c.time;
+    }
+    if (!c.breakpoints.empty()) {
+        const double bp = *c.breakpoints.begin();
+        if (bp > c.time + 1e-18) return bp - c.time;
+    }
+    return c.tstop - c.time;
+}
+
+double clamp_to_breakpoint(const TranCtl& c, double dt) {
+    const double t_next = c.time + dt;
+    if (!c.breakpoints.empty()) {
+        const double bp = *c.breakpoints.begin();
+        if (t_next > bp - 1e-18) return bp - c.time;
+        if (t_next > bp - 0.1 * dt) return bp - c.time;
+    }
+    return dt;
+}
+
+double clamp_to_end(const TranCtl& c, double dt) {
+    if (c.time + dt > c.tstop) return c.tstop - c.time;
+    return dt;
+}
+
+void advance(TranCtl& c, double dt) {
+    c.time += dt;
+    c.crossed_src_bp = false;
+    c.last_bp_hard = false;
+    while (!c.breakpoints.empty() && *c.breakpoints.begin() <= c.time + 1e-18)
+        c.breakpoints.erase(c.breakpoints.begin());
+    while (!c.src_breakpoints.empty() && c.src_breakpoints.begin()->first <= c.time + 1e-18) {
+        if (c.src_breakpoints.begin()->second) c.last_bp_hard = true;
+        c.src_breakpoints.erase(c.src_breakpoints.begin());
+        c.crossed_src_bp = true;
+    }
+}
+
+// VSource::get_breakpoints (vsource.cpp:233-264) of a PULSE (resolved
+// parameters) or PWL source.
+void source_breakpoints(const Device& d, const double* pulse, double tstop, std::vector<double>& out) {
+    out.clear();
+    if (d.wave == WAVE_PWL) {
+        for (double tp : d.pwl_t)
+            if (tp > 0.0 && tp <= tstop) out.push_back(tp);
+        return;
+    }
+    if (d.wave != WAVE_PULSE) return;
+    const double td = pulse[2], tr = pulse[3], tf = pulse[4], pw = pulse[5], per = pulse[6];
+    if (per <= 0.0) return;
+    const int max_periods = static_cast<int>((tstop - td) / per) + 2;
+    for (int k = 0; k < max_periods; ++k) {
+        const double t0 = td + k * per;
+        if (t0 > tstop) break;
+        const double edges[] = {t0, t0 + tr, t0 + tr + pw, t0 + tr + pw + tf};
+        for (double e : edges)
+            if (e > 0.0 && e <= tstop) out.push_back(e);
+    }
+}
+
+}  // namespace
+
+void Bundle::tran_begin(double tstep, double tstop, const int32_t* probe_ptr, const int32_t* probes,
+                        int max_points) {
+    if (!(tstep > 0.0) || !(tstop > 0.0))
+        throw Error(NSA_ERR_ARGUMENT, "Bundle::tran_begin: tstep and tstop must be positive");
+    tstep_ = tstep;
+    tstop_ = tstop;
+    for (void* p : tran_allocs_) NSA_HIP(hipFree(p));
+    tran_allocs_.clear();
+    // 1. DC operating point in MODETRANOP from zeros (transient.cpp:710-726, 195-205)
+    std::vector<double> z(tot_n_, 0.0);
+    set_solution(z.data());
+    std::vector<int32_t> conv, its;
+    dc_op(MODETRANOP | MODEINITJCT, conv, its);
+    dc_iterations_ = its;
+    for (int i = 0; i < NI_; ++i)
+        if (!conv[i])
+            throw Error(NSA_ERR_RUNTIME, "Bundle::tran_begin: DC operating point did not converge for instance " +
+                                             std::to_string(i));
+    // 2. a fresh solver for the transient (transient.cpp:728-730); state
+    //    history seeded from the operating point (transient.cpp:791-802)
+    reset_solver();
+    for (int k = 1; k < 4; ++k) launch_copy_f64(d_s_[k], d_s_[0], ns_tot_, stream_);
+    // 3. PULSE defaults (VSource::resolve_defaults, vsource.cpp:51-58)
+    std::vector<double> pulse = h_pulse_;
+    for (size_t v = 0; v < pulse.size() / 7; ++v) {
+        double* p = &pulse[7 * v];
+        if (p[3] <= 0) p[3] = tstep;
+        if (p[4] <= 0) p[4] = tstep;
+        if (p[5] <= 0) p[5] = tstop;
+        if (p[6] <= 0) p[6] = tstop;
+    }
+    if (!pulse.empty())
+        NSA_HIP(hipMemcpyAsync(const_cast<double*>(vsrc_.pulse), pulse.data(), sizeof(double) * pulse.size(),
+                               hipMemcpyHostToDevice, stream_));
+    // 4. history ring, LTE and ringing buffers, probes
+    auto tal = [&](size_t bytes) {
+        void* p = nullptr;
+        NSA_HIP(hipMalloc(&p, std::max<size_t>(bytes, 8)));
+        tran_allocs_.push_back(p);
+        return p;
+    };
+    d_h1_ = (double*)tal(sizeof(double) * tot_n_);
+    d_h2_ = (double*)tal(sizeof(double) * tot_n_);
+    d_h3_ = (double*)tal(sizeof(double) * tot_n_);
+    launch_copy_f64(d_h1_, d_x_, tot_n_, stream_);
+    launch_copy_f64(d_h2_, d_x_, tot_n_, stream_);
+    launch_copy_f64(d_h3_, d_x_, tot_n_, stream_);
+    d_tmin_ = (unsigned long long*)tal(sizeof(unsigned long long) * 2 * NI_);
+    d_ring_ = (int32_t*)tal(sizeof(int32_t) * NI_);
+    d_acc_ = (int32_t*)tal(sizeof(int32_t) * NI_);
+    d_att_ = (int32_t*)tal(sizeof(int32_t) * NI_);
+    d_prow_ = (int32_t*)tal(sizeof(int32_t) * NI_);
+    if (!h_acc_pin_) {
+        NSA_HIP(hipHostMalloc((void**)&h_acc_pin_, sizeof(int32_t) * 3 * NI_, hipHostMallocDefault));
+        NSA_HIP(hipHostMalloc((void**)&h_tmin_pin_, sizeof(unsigned long long) * 2 * NI_, hipHostMallocDefault));
+        NSA_HIP(hipHostMalloc((void**)&h_ring_pin_, sizeof(int32_t) * NI_, hipHostMallocDefault));
+    }
+    pptr_.assign(NI_ + 1, 0);
+    std::vector<int32_t> pr;
+    if (probe_ptr) {
+        for (int i = 0; i <= NI_; ++i) pptr_[i] = probe_ptr[i];
+        pr.assign(probes, probes + pptr_[NI_]);
+        for (int i = 0; i < NI_; ++i)
+            for (int q = pptr_[i]; q < pptr_[i + 1]; ++q)
+                if (pr[q] < 0 || pr[q] >= n_[i]) throw Error(NSA_ERR_ARGUMENT, "Bundle::tran_begin: probe index");
+    }
+    max_points_ = std::max(1, max_points);
+    d_pptr_ = (int32_t*)tal(sizeof(int32_t) * (NI_ + 1));
+    NSA_HIP(hipMemcpyAsync(d_pptr_, pptr_.data(), sizeof(int32_t) * (NI_ + 1), hipMemcpyHostToDevice, stream_));
+    d_probe_ = (int32_t*)tal(sizeof(int32_t) * std::max<size_t>(1, pr.size()));
+    if (!pr.empty())
+        NSA_HIP(hipMemcpyAsync(d_probe_, pr.data(), sizeof(int32_t) * pr.size(), hipMemcpyHostToDevice, stream_));
+    d_pvals_ = pr.empty() ? nullptr : (double*)tal(sizeof(double) * pr.size() * max_points_);
+    // the t = 0 point (transient.cpp:805)
+    prow_.assign(NI_, 0);
+    std::vector<int32_t> rec(NI_, 3);
+    NSA_HIP(hipStreamSynchronize(stream_));
+    std::memcpy(h_acc_pin_, rec.data(), sizeof(int32_t) * NI_);
+    std::memcpy(h_acc_pin_ + NI_, prow_.data(), sizeof(int32_t) * NI_);
+    NSA_HIP(hipMemcpyAsync(d_acc_, h_acc_pin_, sizeof(int32_t) * NI_, hipMemcpyHostToDevice, stream_));
+    NSA_HIP(hipMemcpyAsync(d_prow_, h_acc_pin_ + NI_, sizeof(int32_t) * NI_, hipMemcpyHostToDevice, stream_));
+    launch_tr_hist(NI_, d_ixoff_, d_in_, d_x_, d_h1_, d_h2_, d_h3_, d_acc_, d_pptr_, d_probe_, d_prow_,
+                   d_pvals_, max_points_, stream_);
+    // 5. step controllers (transient.cpp:826-903)
+    tc_.assign(NI_, TranCtl());
+    const double max_step = std::min(tstep, tstop / 50.0);
+    std::vector<double> bps;
+    int vcur = 0;
+    for (int i = 0; i < NI_; ++i) {
+        TranCtl& c = tc_[i];
+        c.dt_min = max_step * kMinTimeStepRatio;
+        c.dt_max = max_step;
+        c.ctrl_dt = tstep;
+        c.tstop = tstop;
+        c.time = 0.0;
+        c.proposed_dt = tstep;
+        c.min_break = (max_step > 0.0) ? 5e-5 * max_step : 0.0;
+        // collect_breakpoints (transient.cpp:135-151): sources in device order
+        const Circuit& ck = *ckt_[i];
+        const auto& fv = ck.family(DK_V);
+        std::vector<int> vfam_pos(ck.devices().size(), -1);
+        for (size_t k = 0; k < fv.size(); ++k) vfam_pos[fv[k]] = (int)k;
+        for (size_t d = 0; d < ck.devices().size(); ++d) {
+            const Device& dv = ck.devices()[d];
+            if (dv.kind != DK_V) continue;
+            source_breakpoints(dv, &pulse[7 * (size_t)(vcur + vfam_pos[d])], tstop, bps);
+            for (double bp : bps) add_source_breakpoint(c, bp, true);
+        }
+        vcur += (int)fv.size();
+        c.dt = std::min(tstop / kInitialStopDivisor, tstep) / kInitialStepDivisor;
+        c.use_gear = opt_.method == 1;
+        c.integrate_method = c.use_gear ? 1 : 0;
+        c.saved_delta = c.dt;
+        c.steps_after_bp = kNoRecentBreakpoint;
+        const double bp_gap = next_breakpoint_gap(c);
+        if (bp_gap < tstop) {
+            const double init_save_delta = tstop / 50.0;
+            c.dt = std::min(c.dt, 0.1 * std::min(init_save_delta, bp_gap));
+            c.dt /= 10;
+            c.dt = std::max(c.dt, c.dt_min * 2.0);
+        }
+        c.saved_delta = c.dt;
+        c.prev_prev_dt = c.dt;
+    }
+    NSA_HIP(hipStreamSynchronize(stream_));
+}
+
+bool Bundle::tran_done() const {
+    for (const auto& c : tc_)
+        if (!c.finished) return false;
+    return true;
+}
+
+int Bundle::tran_advance(double t_until) {
+    if (tc_.empty()) throw Error(NSA_ERR_LOGIC, "Bundle::tran_advance: tran_begin not called");
+    int rounds = 0;
+    std::vector<int32_t> conv, its;
+    const TruncOpts to{opt_.abstol, opt_.reltol, opt_.chgtol, opt_.trtol};
+    const double big = 1e30;
+    unsigned long long big_bits;
+    std::memcpy(&big_bits, &big, sizeof(big));
+    for (;;) {
+        std::vector<int32_t> att(NI_, 0);
+        std::vector<DevCtx> ctx = h_ctx_;
+        bool any = false;
+        for (int i = 0; i < NI_; ++i) {
+            TranCtl& c = tc_[i];
+            if (c.finished) continue;
+            if (!(c.time < c.tstop - 1e-18)) { c.finished = true; continue; }
+            if (c.time >= t_until) continue;
+            if (++c.total_iterations > kMaxTransientIterations) {
+                c.failed = c.finished = true;
+                continue;
+            }
+            // clamp to bounds and breakpoints (transient.cpp:894-903)
+            double dt = std::min(c.dt, c.dt_max);
+            dt = std::max(dt, c.dt_min);
+            const double unclamped = dt;
+            dt = clamp_to_breakpoint(c, dt);
+            if (dt < unclamped - 1e-18) c.saved_delta = unclamped;
+            dt = clamp_to_end(c, dt);
+            if (dt < 1e-20) { c.finished = true; continue; }
+            c.dt = dt;
+            // fill_integrator_context (transient.cpp:398-447)
+            DevCtx x{};
+            const bool first = c.step_count == 0;
+            x.order = c.order;
+            x.delta = dt;
+            x.time = c.time + dt;
+            x.delta_old1 = first ? dt : c.prev_dt;
+            x.delta_old2 = c.prev_prev_dt;
+            x.mode = MODETRAN | (first ? MODEINITTRAN : MODEINITPRED);
+            x.method = c.integrate_method;
+            x.src_fact = opt_.src_fact;
+            if (c.order == 1) {
+                x.ag0 = 1.0 / dt;
+                x.ag1 = -1.0 / dt;
+                x.ag2 = 0.0;
+                x.xmu_ratio = 0.0;
+            } else if (c.integrate_method != 1) {
+                const double xmu = opt_.xmu;
+                const double one_minus_xmu = 1.0 - xmu;
+                x.ag0 = 1.0 / (dt * one_minus_xmu);
+                x.ag1 = -1.0 / (dt * one_minus_xmu);
+                x.ag2 = 0.0;
+                x.xmu_ratio = xmu / one_minus_xmu;
+            } else {
+                const double h_old = c.prev_dt;
+                if (h_old > 0.0) {
+                    const double sum = dt + h_old;
+                    x.ag0 = (2.0 * dt + h_old) / (dt * sum);
+                    x.ag1 = -(dt + h_old) / (dt * h_old);
+                    x.ag2 = dt / (h_old * sum);
+                } else {
+                    x.ag0 = 1.0 / dt;
+                    x.ag1 = -1.0 / dt;
+                    x.ag2 = 0.0;
+                }
+                x.xmu_ratio = 0.0;
+            }
+            ctx[i] = x;
+            att[i] = 1;
+            any = true;
+        }
+        if (!any) break;
+        set_ctx(ctx);
+        newton(opt_.itl4, att.data(), conv, its);
+        // every device's compute_trunc at the step's order and at order 2
+        // (evaluate_lte, transient.cpp:469-485; order promotion :946-958),
+        // and the ringing test, for the instances that attempted a step
+        launch_fill_u64(d_tmin_, big_bits, 2 * NI_, stream_);
+        launch_trunc(trunc1_, d_ctx_, d_s_[0], d_s_[1], d_s_[2], d_s_[3], 1, to, nullptr, d_tmin_, stream_, 0);
+        launch_trunc(trunc4_, d_ctx_, d_s_[0], d_s_[1], d_s_[2], d_s_[3], b4_lanes_, to, nullptr, d_tmin_,
+                     stream_, 0);
+        launch_trunc(trunc1_, d_ctx_, d_s_[0], d_s_[1], d_s_[2], d_s_[3], 1, to, nullptr, d_tmin_ + NI_,
+                     stream_, 2);
+        launch_trunc(trunc4_, d_ctx_, d_s_[0], d_s_[1], d_s_[2], d_s_[3], b4_lanes_, to, nullptr,
+                     d_tmin_ + NI_, stream_, 2);
+        std::memcpy(h_acc_pin_ + 2 * NI_, att.data(), sizeof(int32_t) * NI_);
+        NSA_HIP(hipMemcpyAsync(d_att_, h_acc_pin_ + 2 * NI_, sizeof(int32_t) * NI_, hipMemcpyHostToDevice,
+                               stream_));
+        launch_tr_ring(NI_, d_ixoff_, d_inn_, d_x_, d_h1_, d_h2_, opt_.reltol, opt_.vntol, d_att_, d_ring_,
+                       stream_);
+        NSA_HIP(hipMemcpyAsync(h_tmin_pin_, d_tmin_, sizeof(unsigned long long) * 2 * NI_,
+                               hipMemcpyDeviceToHost, stream_));
+        NSA_HIP(hipMemcpyAsync(h_ring_pin_, d_ring_, sizeof(int32_t) * NI_, hipMemcpyDeviceToHost, stream_));
+        NSA_HIP(hipStreamSynchronize(stream_));
+        std::vector<int32_t> acc(NI_, 0);
+        for (int i = 0; i < NI_; ++i) {
+            if (!att[i]) continue;
+            TranCtl& c = tc_[i];
+            c.all_iters += its[i];
+            double dt = c.dt;
+            if (!conv[i]) {
+                // Newton failure (transient.cpp:917-948)
+                dt /= kNewtonFailureDtFactor;
+                c.order = 1;
+                c.order_promote_cooldown = 5;
+                if (dt < c.dt_min) {
+                    if (!c.tried_delmin) {
+                        dt = c.dt_min;
+                        c.tried_delmin = true;
+                    } else {
+                        c.failed = c.finished = true;
+                    }
+                }
+                c.dt = dt;
+                acc[i] = 2;
+                continue;
+            }
+            c.newton_iters += its[i];
+            // evaluate_lte (transient.cpp:469-513), device LTE only
+            double tmin_k, tmin_2;
+            std::memcpy(&tmin_k, &h_tmin_pin_[i], sizeof(double));
+            std::memcpy(&tmin_2, &h_tmin_pin_[NI_ + i], sizeof(double));
+            double device_dt = 1e30;
+            bool accepted = true;
+            if (c.step_count >= kLteMinStepCount) {
+                device_dt = std::min(device_dt, tmin_k);
+                device_dt = std::min(kMaxDtGrowthFactor * dt, device_dt);
+                if (device_dt < dt * kDeviceLteRejectRatio && dt > c.dt_min * 1.01) {
+                    accepted = false;
+                    ++c.rejected;
+                    c.ctrl_dt = dt;
+                }
+            }
+            if (!accepted) {
+                double proposed = c.proposed_dt;
+                if (device_dt < proposed) proposed = device_dt;
+                c.dt = std::max(proposed, c.dt_min);
+                acc[i] = 2;
+                continue;
+            }
+            // accept (transient.cpp:976-1063)
+            const double accepted_dt = dt;
+            c.prev_prev_dt = c.prev_dt;
+            c.prev_dt = dt;
+            advance(c, dt);
+            c.ctrl_dt = dt;
+            ++c.step_count;
+            c.tried_delmin = false;
+            if (c.steps_after_bp < kNoRecentBreakpoint) ++c.steps_after_bp;
+            c.steps.push_back(TranStep{c.time, accepted_dt, h_ctx_[i].order, h_ctx_[i].method, its[i], 0});
+            if (c.crossed_src_bp && c.step_count > 2) {
+                c.steps_after_bp = 0;
+                c.order = 1;
+                const double bp_gap = next_breakpoint_gap(c);
+                double scale = 0.1;  // SimOptions::restart_step_scale
+                if (!c.last_bp_hard) scale = std::sqrt(scale);
+                dt = std::min(dt, scale * std::min(c.saved_delta, bp_gap));
+                dt = std::max(dt, c.dt_min * 2.0);
+            }
+            if (c.order_promote_cooldown > 0) --c.order_promote_cooldown;
+            if (c.order == 1 && c.step_count >= 2 && c.order_promote_cooldown == 0) {
+                if (tmin_2 > 1.05 * dt) c.order = 2;
+            }
+            acc[i] = 1;
+            // detect_and_handle_ringing (transient.cpp:672-688)
+            if (!(c.step_count < 3 || c.use_gear || c.steps_after_bp < kRingingMinStepsAfterBp)) {
+                c.ringing = h_ring_pin_[i] != 0;
+                if (c.ringing) c.ringing_cooldown = 3;
+                if (c.ringing_cooldown > 0 && !c.ringing) --c.ringing_cooldown;
+                c.integrate_method = (c.ringing || c.ringing_cooldown > 0) ? 1 : 0;
+            }
+            if (c.step_count >= kLteMinStepCount) {
+                double proposed = device_dt;
+                proposed = std::min(proposed, kMaxDtGrowthFactor * dt);
+                dt = std::max(proposed, c.dt_min);
+            }
+            c.dt = dt;
+            prow_[i] = c.step_count;
+        }
+        // rotate accepted instances, restore the others (transient.cpp:919-975, 1004, 1027-1037)
+        std::memcpy(h_acc_pin_, acc.data(), sizeof(int32_t) * NI_);
+        std::memcpy(h_acc_pin_ + NI_, prow_.data(), sizeof(int32_t) * NI_);
+        NSA_HIP(hipMemcpyAsync(d_acc_, h_acc_pin_, sizeof(int32_t) * NI_, hipMemcpyHostToDevice, stream_));
+        NSA_HIP(hipMemcpyAsync(d_prow_, h_acc_pin_ + NI_, sizeof(int32_t) * NI_, hipMemcpyHostToDevice,
+                               stream_));
+        launch_tr_states(ns_tot_, ns_lane_, b4_lanes_, b4_.inst_of, d_slot_inst_, d_acc_, d_s_[0], d_s_[1],
+                         d_s_[2], d_s_[3], stream_);
+        launch_tr_hist(NI_, d_ixoff_, d_in_, d_x_, d_h1_, d_h2_, d_h3_, d_acc_, d_pptr_, d_probe_, d_prow_,
+                       d_pvals_, max_points_, stream_);
+        ++rounds;
+    }
+    NSA_HIP(hipStreamSynchronize(stream_));
+    return rounds;
+}
+
+void Bundle::tran_probes(int i, int* count, double* out) const {
+    if (i < 0 || i >= NI_) throw Error(NSA_ERR_ARGUMENT, "Bundle::tran_probes: instance");
+    const int np = tran_probe_count(i);
+    const int rows = std::min(max_points_, (tc_.empty() ? 0 : tc_[i].step_count) + 1);
+    *count = rows;
+    if (!np || !out || !d_pvals_) return;
+    NSA_HIP(hipMemcpyAsync(out, d_pvals_ + (size_t)pptr_[i] * max_points_, sizeof(double) * rows * np,
+                           hipMemcpyDeviceToHost, stream_));
+    NSA_HIP(hipStreamSynchronize(stream_));
+}
+
+}  // namespace nsa

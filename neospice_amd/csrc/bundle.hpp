@@ -1,0 +1,233 @@
+// A bundle: many independent circuit instances, each with its own topology
+// and values (a Monte Carlo / .step sweep over circuits of uneven size),
+// advanced together on one GPU.
+//
+// The reference runs such a sweep one circuit at a time (run_step_sweep,
+// src/api/neospice.cpp:229-334): every instance is its own Circuit with its
+// own solver.  Here every per-instance array is packed instance after
+// instance (unknowns at xoff[i], matrix slots at aoff[i], states in the
+// families' regions), and one launch per phase covers every instance:
+//   * device loads: one lane per device of every instance (BSIM4v7 with its
+//     slots and constants strided by lane, so a wave's loads coalesce);
+//   * gathers: one lane per matrix slot / rhs row of the packed system;
+//   * LU: one wave per instance on its own plan (small_lu), the factor in LDS;
+//   * Newton control: newton_solve's per-iteration bookkeeping
+//     (src/core/newton.cpp:136-412) in a kernel with per-instance state, so
+//     an iteration is a fixed sequence of launches captured in a HIP graph.
+// The host keeps what the reference decides from values on the CPU: the
+// Markowitz pivot orders (one Sparse 1.3 state per instance) and the
+// transient step controller of solve_transient (src/core/transient.cpp:
+// 694-1066, src/core/timestep.cpp), restated per instance.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <set>
+#include <vector>
+
+#include "bsim4.hpp"
+#include "circuit.hpp"
+#include "engine.hpp"
+#include "gpu_dev.hpp"
+#include "lu_cpu.hpp"
+#include "small_lu.hpp"
+
+namespace nsa {
+
+// Per-instance Newton state on the device (newton_solve's locals).
+struct NtInst {
+    int32_t saved_mode;      // the caller's mode, restored on exit (newton.cpp:117, 417)
+    int32_t done;            // finished (converged, failed) or not taking part
+    int32_t converged;
+    int32_t iter;            // iterations so far
+    int32_t itl;             // iteration limit
+    int32_t force_numeric;   // newton.cpp:125
+    int32_t always_reorder;  // newton.cpp:131
+    int32_t kind;            // LU call of the current iteration (SluKind)
+    int32_t stalled;         // a numeric() whose pivot order failed: the host re-pivots
+    int32_t pad[3];
+};
+
+// One accepted transient step of an instance.
+struct TranStep {
+    double t, dt;
+    int32_t order, method, iters, pad;
+};
+
+// Per-instance state of the adaptive transient loop (solve_transient with
+// its TimeStepController): everything the loop body reads or writes.
+struct TranCtl {
+    // TimeStepController (timestep.hpp:9-85)
+    double ctrl_dt = 0.0, time = 0.0, tstop = 0.0, proposed_dt = 0.0, prev_dt = 0.0;
+    double min_break = 0.0;
+    int rejected = 0, order = 1;
+    bool crossed_src_bp = false, last_bp_hard = true, ringing = false;
+    int ringing_cooldown = 0;
+    std::set<double> breakpoints;
+    std::map<double, bool> src_breakpoints;  // time -> HARD
+    // loop locals of solve_transient
+    double dt = 0.0, saved_delta = 0.0, prev_prev_dt = 0.0, dt_min = 0.0, dt_max = 0.0;
+    int step_count = 0, steps_after_bp = 1000, order_promote_cooldown = 0;
+    int total_iterations = 0, integrate_method = 0;
+    bool use_gear = false, tried_delmin = false;
+    bool finished = false, failed = false;
+    long newton_iters = 0;       // iterations of converged steps (TransientResult)
+    long all_iters = 0;          // every iteration run, including failed and rejected steps
+    double attempt_dt = 0.0;     // the step being attempted
+    std::vector<TranStep> steps;
+};
+
+class Bundle {
+public:
+    // One circuit per instance (the same Circuit may appear more than once:
+    // instances are copies).  Every circuit must be finalized.
+    Bundle(const std::vector<const Circuit*>& circuits, const Options& opt);
+    ~Bundle();
+    Bundle(const Bundle&) = delete;
+    Bundle& operator=(const Bundle&) = delete;
+
+    int size() const { return NI_; }
+    int total_n() const { return tot_n_; }
+    int total_nnz() const { return tot_nnz_; }
+    const std::vector<int32_t>& xoff() const { return xoff_; }
+    const std::vector<int32_t>& aoff() const { return aoff_; }
+    hipStream_t stream() const { return stream_; }
+
+    // Integrator context per instance (host copy kept).
+    void set_ctx(const std::vector<DevCtx>& c);
+    const std::vector<DevCtx>& ctx() const { return h_ctx_; }
+
+    // newton_solve on every instance with active[i] != 0 (null: all), from
+    // the current solution with the mode in its context.
+    void newton(int itl, const int32_t* active, std::vector<int32_t>& converged,
+                std::vector<int32_t>& iterations);
+    // Direct Newton attempt of the DC operating point (dc.cpp:104-107 /
+    // transient.cpp:195-205) for every instance, from the current solution.
+    void dc_op(int mode, std::vector<int32_t>& converged, std::vector<int32_t>& iterations);
+    // A fresh solver for every instance (make_solver + symbolic).
+    void reset_solver();
+
+    void get_solution(double* out) const;
+    void set_solution(const double* in);
+    void get_state(int which, double* out) const;  // packed states (family regions)
+    int num_states() const { return ns_tot_; }
+
+    // ---- adaptive transient (solve_transient, transient.cpp:694-1066)
+    // DC operating point (MODETRANOP), fresh solver, state seeding, source
+    // defaults and breakpoints, initial step.  probes: per-instance list of
+    // unknown indices recorded at t = 0 and every accepted step
+    // (probe_ptr[NI+1] into probes; may be null), up to max_points rows.
+    void tran_begin(double tstep, double tstop, const int32_t* probe_ptr, const int32_t* probes,
+                    int max_points);
+    // Runs batched steps until every instance has reached t_until (or tstop,
+    // or failed).  Returns the number of batched rounds.
+    int tran_advance(double t_until);
+    bool tran_done() const;
+    const TranCtl& tran_ctl(int i) const { return tc_[i]; }
+    // probe rows of instance i: (count, values row-major count x nprobe_i)
+    void tran_probes(int i, int* count, double* out) const;
+    int tran_probe_count(int i) const { return pptr_.empty() ? 0 : pptr_[i + 1] - pptr_[i]; }
+
+    // ---- statistics
+    long n_cpu_pivot = 0, n_iterations = 0, n_graph_launches = 0, n_host_iterations = 0;
+    int num_plans() const { return plans_.count(); }
+    const SmallLuPlans& plans() const { return plans_; }
+    int plan_of(int i) const { return plan_[i]; }
+    // HIP-event timing of the phases of device-driven iterations
+    void set_timing(bool on);
+    double ms_load = 0.0, ms_lu = 0.0, ms_post = 0.0, ms_bsim4 = 0.0;
+    long n_timed = 0;
+    bool use_graph = true;
+
+private:
+    void build_tables();
+    void launch_begin();
+    void launch_loads();
+    void launch_lu(const int32_t* d_kind, int mode);
+    void launch_post(int iter_base);
+    void iteration_device();
+    void capture_graph();
+    void host_lu_post(std::vector<NtInst>& nt, const std::vector<int>& repivot, bool others);
+    void collect_timing();
+    std::vector<double> instance_matrix(int i);
+    void upload_inst();
+    void tran_attempt_setup(int i, DevCtx& c);
+    void tran_post_round(const std::vector<int32_t>& attempt, const std::vector<int32_t>& conv,
+                         const std::vector<int32_t>& its);
+
+    std::vector<const Circuit*> ckt_;
+    int NI_ = 0, tot_n_ = 0, tot_nnz_ = 0, ns_tot_ = 0;
+    Options opt_;
+    hipStream_t stream_ = nullptr;
+    std::vector<void*> allocs_;
+    std::vector<int32_t> xoff_, aoff_, n_, nnodes_;
+    // device vectors
+    double *d_x_ = nullptr, *d_xold_ = nullptr, *d_rhs_ = nullptr, *d_A_ = nullptr, *d_LU_ = nullptr;
+    double* d_s_[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *d_cm_ = nullptr, *d_cr_ = nullptr;
+    DevCtx* d_ctx_ = nullptr;
+    std::vector<DevCtx> h_ctx_;
+    DevCtx* h_ctx_pin_ = nullptr;
+    NtInst* d_nt_ = nullptr;
+    NtInst* h_nt_pin_ = nullptr;
+    int32_t *d_done_ = nullptr, *d_devnc_ = nullptr, *d_kind_ = nullptr, *d_fail_ = nullptr,
+            *d_fail_col_ = nullptr, *d_kind_host_ = nullptr;
+    int32_t* h_kind_pin_ = nullptr;
+    int32_t *d_ixoff_ = nullptr, *d_in_ = nullptr, *d_inn_ = nullptr;  // per instance xoff, n, num_nodes
+    // families
+    CapTable cap_{};
+    VsrcTable vsrc_{};
+    IsrcTable isrc_{};
+    DioTable dio_{};
+    Mos1Table mos_{};
+    B4Table b4_{};
+    TruncTable trunc1_{}, trunc4_{};
+    int nR_ = 0, r_site_ = 0, n_const_ = 0, const_site_ = 0, n_msites_ = 0, n_rsites_ = 0;
+    double *d_r_ohms_ = nullptr, *d_const_vals_ = nullptr;
+    int32_t *d_mptr_ = nullptr, *d_msites_ = nullptr, *d_rptr_d_ = nullptr, *d_rsites_d_ = nullptr,
+            *d_rptr_u_ = nullptr, *d_rsites_u_ = nullptr;
+    DevTol tol_{};
+    std::vector<double> h_pulse_;  // 7 per V source, for source-default resolution
+    // LU
+    SmallLuPlans plans_;
+    std::vector<int32_t> plan_;                       // per instance, -1 = no factorization yet
+    std::vector<std::unique_ptr<Sparse13>> s13_;      // per instance CPU pivot state
+    std::vector<SluInst> h_inst_;
+    SluInst* d_inst_ = nullptr;
+    bool inst_dirty_ = true;
+    // graph of one device-driven iteration
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+    bool timing_ = false;
+    hipEvent_t ev_[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    bool timed_pending_ = false;
+    const int32_t* graph_words_ = nullptr;
+    int graph_nlu_ = 0, graph_n_ = 0;
+    int b4_lanes_ = 0;
+    long ns_lane_ = 0;
+    int32_t* d_slot_inst_ = nullptr;
+    // transient
+    std::vector<TranCtl> tc_;
+    double tstep_ = 0.0, tstop_ = 0.0;
+    unsigned long long* d_tmin_ = nullptr;   // 2 x NI: order-k proposals, order-2 proposals
+    int32_t* d_ring_ = nullptr;              // NI ringing flags
+    double *d_h1_ = nullptr, *d_h2_ = nullptr, *d_h3_ = nullptr;  // sol_prev, sol_prev2, sol_prev3
+    int32_t* d_acc_ = nullptr;               // NI: 1 accept, 2 restore, 3 record only, 0 nothing
+    int32_t* d_att_ = nullptr;               // NI: attempted a step this round
+    int32_t* h_acc_pin_ = nullptr;
+    unsigned long long* h_tmin_pin_ = nullptr;
+    int32_t* h_ring_pin_ = nullptr;
+    std::vector<int32_t> pptr_;              // probes
+    int32_t *d_probe_ = nullptr, *d_prow_ = nullptr, *d_pptr_ = nullptr;
+    double* d_pvals_ = nullptr;
+    int max_points_ = 0;
+    std::vector<int32_t> prow_;
+    std::vector<int32_t> dc_iterations_;
+public:
+    const std::vector<int32_t>& dc_iterations() const { return dc_iterations_; }
+private:
+    std::vector<void*> tran_allocs_;
+};
+
+}  // namespace nsa

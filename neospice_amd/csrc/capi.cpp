@@ -11,6 +11,7 @@
 #include "bsim4.hpp"
 #include "engine.hpp"
 #include "lu_cpu.hpp"
+#include "bundle.hpp"
 
 struct nsa_circuit {
     nsa::Circuit c;
@@ -19,6 +20,10 @@ struct nsa_circuit {
 struct nsa_engine {
     nsa_circuit* ckt;
     std::unique_ptr<nsa::BatchEngine> e;
+};
+
+struct nsa_bundle {
+    std::unique_ptr<nsa::Bundle> b;
 };
 
 struct nsa_lu {
@@ -273,25 +278,32 @@ int nsa_engine_trunc_devices(nsa_engine* e, int32_t* dev_index) {
 int nsa_engine_reset_solver(nsa_engine* e) {
     return guard([&] { e->e->reset_solver(); return 0; });
 }
+// The engine's work runs on its own non-blocking stream, so every copy here
+// is ordered on that stream and waited for before returning.
 int nsa_engine_load(nsa_engine* e, double* values, double* rhs) {
     return guard([&] {
         e->e->load_only();
         const auto& c = e->e->circuit();
         const size_t B = e->e->batch();
-        NSA_HIP(hipMemcpy(values, e->e->d_matrix(), sizeof(double) * c.nnz() * B,
-                          hipMemcpyDeviceToHost));
-        NSA_HIP(hipMemcpy(rhs, e->e->d_rhs(), sizeof(double) * c.num_vars() * B,
-                          hipMemcpyDeviceToHost));
+        hipStream_t st = e->e->stream();
+        NSA_HIP(hipMemcpyAsync(values, e->e->d_matrix(), sizeof(double) * c.nnz() * B,
+                               hipMemcpyDeviceToHost, st));
+        NSA_HIP(hipMemcpyAsync(rhs, e->e->d_rhs(), sizeof(double) * c.num_vars() * B,
+                               hipMemcpyDeviceToHost, st));
+        NSA_HIP(hipStreamSynchronize(st));
         return 0;
     });
 }
 int nsa_engine_solve_transposed(nsa_engine* e, double* b) {
     return guard([&] {
+        if (!b) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "nsa_engine_solve_transposed: b is NULL");
         const size_t bytes = sizeof(double) * e->e->n() * e->e->batch();
         double* d = e->e->d_rhs();
-        NSA_HIP(hipMemcpy(d, b, bytes, hipMemcpyHostToDevice));
+        hipStream_t st = e->e->stream();
+        NSA_HIP(hipMemcpyAsync(d, b, bytes, hipMemcpyHostToDevice, st));
         e->e->lu().solve_t(d);
-        NSA_HIP(hipMemcpy(b, d, bytes, hipMemcpyDeviceToHost));
+        NSA_HIP(hipMemcpyAsync(b, d, bytes, hipMemcpyDeviceToHost, st));
+        NSA_HIP(hipStreamSynchronize(st));
         return 0;
     });
 }
@@ -485,6 +497,164 @@ int nsa_min_degree_order(int32_t n, const int32_t* col_ptr, const int32_t* row_i
         auto q = nsa::min_degree_order(n, col_ptr, row_idx);
         std::memcpy(perm, q.data(), sizeof(int32_t) * q.size());
         return 0;
+    });
+}
+
+/* ---- bundles of circuits of different topologies ---- */
+
+int nsa_bundle_create(nsa_circuit* const* circuits, int32_t count, const nsa_options* opt,
+                      nsa_bundle** out) {
+    return guard([&] {
+        if (!circuits || count <= 0 || !out)
+            throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "nsa_bundle_create: need count > 0 circuits");
+        std::vector<const nsa::Circuit*> v(count);
+        for (int i = 0; i < count; ++i) {
+            if (!circuits[i]) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "nsa_bundle_create: null circuit");
+            v[i] = &circuits[i]->c;
+        }
+        auto h = std::make_unique<nsa_bundle>();
+        h->b = std::make_unique<nsa::Bundle>(v, to_opts(opt));
+        *out = h.release();
+        return 0;
+    });
+}
+void nsa_bundle_destroy(nsa_bundle* b) { delete b; }
+
+int nsa_bundle_info(nsa_bundle* b, int32_t out[4]) {
+    return guard([&] {
+        out[0] = b->b->size();
+        out[1] = b->b->total_n();
+        out[2] = b->b->total_nnz();
+        out[3] = b->b->num_states();
+        return 0;
+    });
+}
+int nsa_bundle_offsets(nsa_bundle* b, int32_t* xoff, int32_t* aoff) {
+    return guard([&] {
+        const int n = b->b->size() + 1;
+        if (xoff) std::memcpy(xoff, b->b->xoff().data(), sizeof(int32_t) * n);
+        if (aoff) std::memcpy(aoff, b->b->aoff().data(), sizeof(int32_t) * n);
+        return 0;
+    });
+}
+int nsa_bundle_dc(nsa_bundle* b, int32_t mode, int32_t* converged, int32_t* iterations) {
+    return guard([&] {
+        std::vector<int32_t> c, it;
+        b->b->dc_op(mode, c, it);
+        const int n = b->b->size();
+        if (converged) std::memcpy(converged, c.data(), sizeof(int32_t) * n);
+        if (iterations) std::memcpy(iterations, it.data(), sizeof(int32_t) * n);
+        return 0;
+    });
+}
+int nsa_bundle_get_solution(nsa_bundle* b, double* x) {
+    return guard([&] {
+        if (!x) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "nsa_bundle_get_solution: x is NULL");
+        b->b->get_solution(x);
+        return 0;
+    });
+}
+int nsa_bundle_set_solution(nsa_bundle* b, const double* x) {
+    return guard([&] {
+        if (!x) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "nsa_bundle_set_solution: x is NULL");
+        b->b->set_solution(x);
+        return 0;
+    });
+}
+int nsa_bundle_tran_begin(nsa_bundle* b, double tstep, double tstop, const int32_t* probe_ptr,
+                          const int32_t* probes, int32_t max_points) {
+    return guard([&] {
+        if (probe_ptr && !probes && probe_ptr[b->b->size()] > 0)
+            throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "nsa_bundle_tran_begin: probes is NULL");
+        b->b->tran_begin(tstep, tstop, probe_ptr, probes, max_points);
+        return 0;
+    });
+}
+int nsa_bundle_tran_advance(nsa_bundle* b, double t_until, int32_t* rounds) {
+    return guard([&] {
+        const int r = b->b->tran_advance(t_until);
+        if (rounds) *rounds = r;
+        return b->b->tran_done() ? 1 : 0;
+    });
+}
+int nsa_bundle_tran_status(nsa_bundle* b, int32_t inst, int64_t out[8]) {
+    return guard([&] {
+        if (inst < 0 || inst >= b->b->size()) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "instance out of range");
+        const nsa::TranCtl& c = b->b->tran_ctl(inst);
+        out[0] = (int64_t)c.steps.size();
+        out[1] = c.rejected;
+        out[2] = c.failed ? 1 : 0;
+        out[3] = c.finished ? 1 : 0;
+        out[4] = c.newton_iters;
+        out[5] = c.all_iters;
+        out[6] = c.total_iterations;
+        out[7] = b->b->dc_iterations().empty() ? 0 : b->b->dc_iterations()[inst];
+        return 0;
+    });
+}
+int nsa_bundle_tran_steps(nsa_bundle* b, int32_t inst, double* t, double* dt, int32_t* order,
+                          int32_t* method, int32_t* iters) {
+    return guard([&] {
+        if (inst < 0 || inst >= b->b->size()) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "instance out of range");
+        const auto& st = b->b->tran_ctl(inst).steps;
+        for (size_t k = 0; k < st.size(); ++k) {
+            if (t) t[k] = st[k].t;
+            if (dt) dt[k] = st[k].dt;
+            if (order) order[k] = st[k].order;
+            if (method) method[k] = st[k].method;
+            if (iters) iters[k] = st[k].iters;
+        }
+        return (int)st.size();
+    });
+}
+int nsa_bundle_tran_probes(nsa_bundle* b, int32_t inst, int32_t* rows, double* out) {
+    return guard([&] {
+        int r = 0;
+        b->b->tran_probes(inst, &r, out);
+        if (rows) *rows = r;
+        return 0;
+    });
+}
+int nsa_bundle_stats(nsa_bundle* b, int64_t out[8]) {
+    return guard([&] {
+        const nsa::Bundle& x = *b->b;
+        out[0] = x.n_iterations;
+        out[1] = x.n_cpu_pivot;
+        out[2] = x.num_plans();
+        out[3] = x.n_graph_launches;
+        out[4] = x.n_host_iterations;
+        int64_t lev = 0, slev = 0;
+        for (int p = 0; p < x.num_plans(); ++p) {
+            lev = std::max<int64_t>(lev, x.plans().levels(p));
+            slev = std::max<int64_t>(slev, x.plans().solve_levels(p));
+        }
+        out[5] = lev;
+        out[6] = slev;
+        out[7] = x.plans().max_nlu();
+        return 0;
+    });
+}
+int nsa_bundle_set_timing(nsa_bundle* b, int32_t on) {
+    return guard([&] { b->b->set_timing(on != 0); return 0; });
+}
+int nsa_bundle_timing(nsa_bundle* b, double out[5]) {
+    return guard([&] {
+        const nsa::Bundle& x = *b->b;
+        out[0] = x.ms_load;
+        out[1] = x.ms_lu;
+        out[2] = x.ms_post;
+        out[3] = x.ms_bsim4;
+        out[4] = (double)x.n_timed;
+        return 0;
+    });
+}
+int nsa_bundle_set_graph(nsa_bundle* b, int32_t on) {
+    return guard([&] { b->b->use_graph = on != 0; return 0; });
+}
+int nsa_bundle_plan_of(nsa_bundle* b, int32_t inst) {
+    return guard([&] {
+        if (inst < 0 || inst >= b->b->size()) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "instance out of range");
+        return b->b->plan_of(inst);
     });
 }
 

@@ -553,8 +553,11 @@ NewtonOutcome BatchEngine::newton(int itl) {
     return out;
 }
 
+// Host reads are ordered on the engine's (non-blocking) stream, behind any
+// rotation, seeding or solve still queued there.
 void BatchEngine::get_solution(double* out) const {
-    NSA_HIP(hipMemcpy(out, d_x_, sizeof(double) * n_ * B_, hipMemcpyDeviceToHost));
+    NSA_HIP(hipMemcpyAsync(out, d_x_, sizeof(double) * n_ * B_, hipMemcpyDeviceToHost, stream_));
+    NSA_HIP(hipStreamSynchronize(stream_));
 }
 
 void BatchEngine::set_solution(const double* in) {
@@ -563,7 +566,9 @@ void BatchEngine::set_solution(const double* in) {
 }
 
 void BatchEngine::get_state(int which, double* out) const {
-    if (ns_) NSA_HIP(hipMemcpy(out, d_s_[which], sizeof(double) * ns_ * B_, hipMemcpyDeviceToHost));
+    if (!ns_) return;
+    NSA_HIP(hipMemcpyAsync(out, d_s_[which], sizeof(double) * ns_ * B_, hipMemcpyDeviceToHost, stream_));
+    NSA_HIP(hipStreamSynchronize(stream_));
 }
 
 void BatchEngine::set_state(int which, const double* in) {

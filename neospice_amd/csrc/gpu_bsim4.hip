@@ -61,9 +61,10 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     const long gid = (long)blockIdx.x * TPB + threadIdx.x;
     if (gid >= (long)t.count * B) return;
     const int d = (int)(gid / B), inst = (int)(gid % B);
-    if (done && done[inst]) return;
+    const int ci = t.inst_of ? t.inst_of[gid] : inst;
+    if (done && done[ci]) return;
     const size_t sB = (size_t)B;
-    const DevCtx c = ctx[inst];
+    const DevCtx c = ctx[ci];
     // the device's constants: one shared record, or per-instance records
     // stored field-major (a wave reads 64 consecutive instances of a field)
     const double* pb = t.par_inst ? t.par_inst + (size_t)d * kB4Fields * sB + inst
@@ -78,6 +79,15 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     double* aux = t.aux + (size_t)d * kB4Aux * sB + inst;
     const int nD = t.nodes[4 * d], nG = t.nodes[4 * d + 1], nS = t.nodes[4 * d + 2],
               nB = t.nodes[4 * d + 3];
+    // terminal k's voltage: node of the device (batch of one circuit) or the
+    // lane's own solution index (packed batch of circuits)
+    auto V = [&](int node, int k) -> double {
+        if (t.pin) {
+            const int q = t.pin[(size_t)k * sB + inst];
+            return q >= 0 ? x[q] : 0.0;
+        }
+        return sol(x, node, sB, inst);
+    };
     const double type = PF(type);
     const bool charge = ((mode & (MODEDCTRANCURVE | MODEAC | MODETRAN | MODEINITSMSIG)) ||
                          ((mode & MODETRANOP) && (mode & MODEUIC)));
@@ -95,9 +105,9 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     double cdhat = 0.0, cbhat = 0.0;
     if (mode & MODEADJOINT) {
         // the converged solution itself, unlimited (the adjoint's Jacobian)
-        vds = type * (sol(x, nD, sB, inst) - sol(x, nS, sB, inst));
-        vgs = type * (sol(x, nG, sB, inst) - sol(x, nS, sB, inst));
-        vbs = type * (sol(x, nB, sB, inst) - sol(x, nS, sB, inst));
+        vds = type * (V(nD, 0) - V(nS, 2));
+        vgs = type * (V(nG, 1) - V(nS, 2));
+        vbs = type * (V(nB, 3) - V(nS, 2));
         Check = 0;
     } else if (mode & MODEINITTRAN) {
         vds = s1[S(vdsS)];
@@ -125,9 +135,9 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
             vbs = (1.0 + xfact) * (s1[S(vbsS)]) - (xfact * (s2[S(vbsS)]));
             s0[S(vbdS)] = s0[S(vbsS)] - s0[S(vdsS)];
         } else {
-            vds = type * (sol(x, nD, sB, inst) - sol(x, nS, sB, inst));
-            vgs = type * (sol(x, nG, sB, inst) - sol(x, nS, sB, inst));
-            vbs = type * (sol(x, nB, sB, inst) - sol(x, nS, sB, inst));
+            vds = type * (V(nD, 0) - V(nS, 2));
+            vgs = type * (V(nG, 1) - V(nS, 2));
+            vbs = type * (V(nB, 3) - V(nS, 2));
         }
         const double s0vds = s0[S(vdsS)], s0vgs = s0[S(vgsS)], s0vbs = s0[S(vbsS)],
                      s0vbd = s0[S(vbdS)];
@@ -1311,7 +1321,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     aux[B4_GM * sB] = h_gm_new;
     aux[B4_GDS * sB] = h_gds_new;
     aux[B4_GMBS * sB] = h_gmbs_new;
-    if (noncon) atomicOr(&devnc[inst], 1);
+    if (noncon) atomicOr(&devnc[ci], 1);
 }
 
 }  // namespace

@@ -1,0 +1,272 @@
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "bundle.hpp"
+#include "common.hpp"
+#include "gpu_bundle.hpp"
+
+namespace nsa {
+
+namespace {
+
+constexpr int kW = 64;
+
+__global__ __launch_bounds__(kW) void k_nt_begin(NtInst* __restrict__ nt, const int32_t* __restrict__ xoff,
+                                                const int32_t* __restrict__ nn,
+                                                const double* __restrict__ x,
+                                                double* __restrict__ xold,
+                                                const DevCtx* __restrict__ ctx,
+                                                int32_t* __restrict__ kind, int32_t* __restrict__ fail,
+                                                int32_t* __restrict__ devnc, int32_t* __restrict__ done) {
+    const int i = blockIdx.x, lane = threadIdx.x;
+    NtInst s = nt[i];
+    if (s.done || s.stalled) {
+        if (lane == 0) {
+            kind[i] = SLU_SKIP;
+            fail[i] = -1;
+            done[i] = 1;
+        }
+        return;
+    }
+    const int o = xoff[i], n = nn[i];
+    for (int k = lane; k < n; k += kW) xold[o + k] = x[o + k];
+    if (lane == 0) {
+        devnc[i] = 0;
+        // ngspice niiter.c:107-110: MODEINITJCT forces a reorder
+        if (ctx[i].mode & MODEINITJCT) s.always_reorder = 1;
+        const int k = (s.force_numeric || s.always_reorder) ? SLU_VERIFY : SLU_REFACTOR;
+        s.kind = k;
+        kind[i] = k;
+        fail[i] = 0;
+        done[i] = 0;
+        nt[i] = s;
+    }
+}
+
+__global__ __launch_bounds__(kW) void k_nt_post(NtInst* __restrict__ nt, const int32_t* __restrict__ xoff,
+                                               const int32_t* __restrict__ nn,
+                                               const int32_t* __restrict__ nnodes,
+                                               double* __restrict__ x,
+                                               const double* __restrict__ xold,
+                                               DevCtx* __restrict__ ctx,
+                                               const int32_t* __restrict__ kind,
+                                               const int32_t* __restrict__ fail,
+                                               const int32_t* __restrict__ devnc,
+                                               int32_t* __restrict__ done, NtConvTol tol) {
+    const int i = blockIdx.x, lane = threadIdx.x;
+    NtInst s = nt[i];
+    const int kd = kind[i];
+    if (s.done || s.stalled || kd == SLU_SKIP) return;
+    const int f = fail[i];
+    const int iter = s.iter;
+    int mode = ctx[i].mode;
+    bool finish = false, restore = false;
+    if (f != FAIL_NONE) {
+        if (kd == SLU_REFACTOR) {
+            // singular refactorize(): numeric() next iteration, nothing solved
+            // (newton.cpp:229-234)
+            s.force_numeric = 1;
+        } else if (kd == SLU_VERIFY) {
+            // the order no longer holds: the host re-pivots this instance
+            s.stalled = 1;
+            if (lane == 0) nt[i] = s;
+            return;
+        } else {
+            // singular numeric(): this Newton attempt fails (newton.cpp:220-224)
+            s.converged = 0;
+            finish = true;
+        }
+        s.iter = iter + 1;
+    } else {
+        if (kd != SLU_REFACTOR) s.force_numeric = 0;  // newton.cpp:226
+        // convergence test (newton.cpp:267-292)
+        const int o = xoff[i], n = nn[i], nnod = nnodes[i];
+        bool bad = false;
+        for (int k = lane; k < n; k += kW) {
+            const double vn = x[o + k], vo = xold[o + k];
+            if (isnan(vn) || isinf(vn)) {
+                bad = true;
+            } else {
+                const double diff = fabs(vn - vo);
+                const double t = tol.reltol * fmax(fabs(vn), fabs(vo)) + (k < nnod ? tol.vntol : tol.abstol);
+                if (diff > t) bad = true;
+            }
+        }
+        bool conv = !__any(bad);
+        if (iter == 0) conv = false;        // newton.cpp:318-319
+        if (devnc[i]) conv = false;         // device_converged (newton.cpp:321-334)
+        // init-mode cascade (newton.cpp:370-411)
+        const int m = mode;
+        if (m & MODEINITFLOAT) {
+            if (conv) { restore = true; finish = true; s.converged = 1; }
+        } else if (m & MODEINITJCT) {
+            mode = (m & ~INITF_MASK) | MODEINITFIX;
+            s.force_numeric = 1;
+            s.always_reorder = 0;
+        } else if (m & MODEINITFIX) {
+            if (conv) mode = (m & ~INITF_MASK) | MODEINITFLOAT;
+        } else if (m & MODEINITSMSIG) {
+            mode = (m & ~INITF_MASK) | MODEINITFLOAT;
+        } else if (m & MODEINITTRAN) {
+            mode = (m & ~INITF_MASK) | MODEINITFLOAT;
+            s.force_numeric = 1;
+        } else if (m & MODEINITPRED) {
+            mode = (m & ~INITF_MASK) | MODEINITFLOAT;
+        } else if (conv) {
+            restore = true; finish = true; s.converged = 1;
+        }
+        s.iter = iter + 1;
+        if (restore) {
+            // converged: the previous iterate is returned (newton.cpp:378-381)
+            for (int k = lane; k < n; k += kW) x[o + k] = xold[o + k];
+        }
+    }
+    if (!finish && s.iter >= s.itl) {
+        s.converged = 0;
+        finish = true;
+    }
+    if (finish) {
+        s.done = 1;
+        mode = s.saved_mode;  // newton.cpp:224, 380, 417
+    }
+    if (lane == 0) {
+        nt[i] = s;
+        ctx[i].mode = mode;
+        done[i] = s.done;
+    }
+}
+
+__global__ __launch_bounds__(kW) void k_tr_ring(const int32_t* __restrict__ xoff,
+                                               const int32_t* __restrict__ nnodes,
+                                               const double* __restrict__ x,
+                                               const double* __restrict__ h1,
+                                               const double* __restrict__ h2, double reltol,
+                                               double vntol, const int32_t* __restrict__ sel,
+                                               int32_t* __restrict__ ring) {
+    const int i = blockIdx.x, lane = threadIdx.x;
+    if (!sel[i]) return;
+    const int o = xoff[i], nn = nnodes[i];
+    bool found = false;
+    for (int k = lane; k < nn; k += kW) {
+        const double sol = x[o + k], prev = x[o + k], prev2 = h1[o + k], prev3 = h2[o + k];
+        const double d2c = sol - 2.0 * prev + prev2;
+        const double d2p = prev - 2.0 * prev2 + prev3;
+        const double t = reltol * fabs(sol) + vntol;
+        if (d2c * d2p < 0.0 && fabs(d2c) > t && fabs(d2p) > t) {
+            const double d1c = sol - prev, d1p = prev - prev2;
+            double d1max = fmax(fabs(d1c), fabs(d1p));
+            if (d1max < 1e-30) d1max = 1e-30;
+            const double ratio = fmax(fabs(d2c), fabs(d2p)) / d1max;
+            if (ratio > 0.5) found = true;
+        }
+    }
+    const bool any = __any(found);
+    if (lane == 0) ring[i] = any ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kW) void k_tr_hist(const int32_t* __restrict__ xoff,
+                                               const int32_t* __restrict__ nn, double* __restrict__ x,
+                                               double* __restrict__ h1, double* __restrict__ h2,
+                                               double* __restrict__ h3,
+                                               const int32_t* __restrict__ acc,
+                                               const int32_t* __restrict__ pptr,
+                                               const int32_t* __restrict__ probe,
+                                               const int32_t* __restrict__ prow,
+                                               double* __restrict__ pvals, int max_points) {
+    const int i = blockIdx.x, lane = threadIdx.x;
+    const int a = acc[i];
+    if (!a) return;
+    const int o = xoff[i], n = nn[i];
+    if (a == 1 || a == 3) {
+        if (a == 1)
+        for (int k = lane; k < n; k += kW) {
+            const double v = x[o + k];
+            h3[o + k] = h2[o + k];
+            h2[o + k] = h1[o + k];
+            h1[o + k] = v;
+        }
+        if (pvals) {
+            const int r = prow[i];
+            if (r < max_points) {
+                const int p0 = pptr[i], np = pptr[i + 1] - p0;
+                for (int q = lane; q < np; q += kW)
+                    pvals[(size_t)p0 * max_points + (size_t)r * np + q] = x[o + probe[p0 + q]];
+            }
+        }
+    } else {
+        for (int k = lane; k < n; k += kW) x[o + k] = h1[o + k];
+    }
+}
+
+__global__ void k_tr_states(long ns, long ns_lane, int lanes, const int32_t* __restrict__ lane_inst,
+                            const int32_t* __restrict__ slot_inst, const int32_t* __restrict__ acc,
+                            const double* __restrict__ s0, double* __restrict__ s1,
+                            double* __restrict__ s2, double* __restrict__ s3) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ns) return;
+    const int inst = (k < ns_lane) ? lane_inst[k % lanes] : slot_inst[k - ns_lane];
+    if (acc[inst] != 1) return;
+    s3[k] = s2[k];
+    s2[k] = s1[k];
+    s1[k] = s0[k];
+}
+
+__global__ void k_copy_f64(double* __restrict__ dst, const double* __restrict__ src, long count) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) dst[k] = src[k];
+}
+
+}  // namespace
+
+void launch_nt_begin(NtInst* nt, int ni, const int32_t* xoff, const int32_t* n, const double* x,
+                     double* xold, const DevCtx* ctx, int32_t* kind, int32_t* fail,
+                     int32_t* devnc, int32_t* done, hipStream_t s) {
+    if (ni <= 0) return;
+    hipLaunchKernelGGL(k_nt_begin, dim3(ni), dim3(kW), 0, s, nt, xoff, n, x, xold, ctx, kind, fail,
+                       devnc, done);
+}
+
+void launch_nt_post(NtInst* nt, int ni, const int32_t* xoff, const int32_t* n,
+                    const int32_t* num_nodes, double* x, const double* xold, DevCtx* ctx,
+                    const int32_t* kind, const int32_t* fail, const int32_t* devnc, int32_t* done,
+                    NtConvTol tol, hipStream_t s) {
+    if (ni <= 0) return;
+    hipLaunchKernelGGL(k_nt_post, dim3(ni), dim3(kW), 0, s, nt, xoff, n, num_nodes, x, xold, ctx,
+                       kind, fail, devnc, done, tol);
+}
+
+void launch_tr_ring(int ni, const int32_t* xoff, const int32_t* num_nodes, const double* x,
+                    const double* h1, const double* h2, double reltol, double vntol,
+                    const int32_t* sel, int32_t* ring, hipStream_t s) {
+    if (ni <= 0) return;
+    hipLaunchKernelGGL(k_tr_ring, dim3(ni), dim3(kW), 0, s, xoff, num_nodes, x, h1, h2, reltol,
+                       vntol, sel, ring);
+}
+
+void launch_tr_hist(int ni, const int32_t* xoff, const int32_t* n, double* x, double* h1,
+                    double* h2, double* h3, const int32_t* acc, const int32_t* pptr,
+                    const int32_t* probe, const int32_t* prow, double* pvals, int max_points,
+                    hipStream_t s) {
+    if (ni <= 0) return;
+    hipLaunchKernelGGL(k_tr_hist, dim3(ni), dim3(kW), 0, s, xoff, n, x, h1, h2, h3, acc, pptr, probe,
+                       prow, pvals, max_points);
+}
+
+void launch_tr_states(long ns, long ns_lane, int lanes, const int32_t* lane_inst,
+                      const int32_t* slot_inst, const int32_t* acc, double* s0, double* s1,
+                      double* s2, double* s3, hipStream_t s) {
+    if (ns <= 0) return;
+    const int tpb = 256;
+    hipLaunchKernelGGL(k_tr_states, dim3((unsigned)((ns + tpb - 1) / tpb)), dim3(tpb), 0, s, ns,
+                       ns_lane, lanes > 0 ? lanes : 1, lane_inst, slot_inst, acc, s0, s1, s2, s3);
+}
+
+void launch_copy_f64(double* dst, const double* src, long count, hipStream_t s) {
+    if (count <= 0) return;
+    const int tpb = 256;
+    hipLaunchKernelGGL(k_copy_f64, dim3((unsigned)((count + tpb - 1) / tpb)), dim3(tpb), 0, s, dst,
+                       src, count);
+}
+
+}  // namespace nsa

@@ -1,0 +1,61 @@
+// Kernels of the bundle engine: newton_solve's per-iteration bookkeeping
+// with per-instance state, and the per-instance parts of the transient
+// loop (history rotation, ringing test, state rotation, probes).  One wave
+// per instance unless stated; every instance's unknowns are contiguous
+// (xoff[i] .. xoff[i] + n[i]).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace nsa {
+
+struct NtInst;
+struct DevCtx;
+
+struct NtConvTol {
+    double reltol, vntol, abstol;
+};
+
+// Start of an iteration (newton.cpp:138, 207-210): old solution, cleared
+// device non-convergence, the LU call kind (numeric() when forced or in
+// MODEINITJCT, else refactorize()), and the completion flags the loads read.
+void launch_nt_begin(NtInst* nt, int ni, const int32_t* xoff, const int32_t* n, const double* x,
+                     double* xold, const DevCtx* ctx, int32_t* kind, int32_t* fail,
+                     int32_t* devnc, int32_t* done, hipStream_t s);
+// End of an iteration (newton.cpp:211-412): LU outcome (singular
+// refactorize: forced numeric and retry; singular numeric: the attempt
+// fails; a numeric() whose order no longer holds stalls the instance for
+// the host), convergence test, iteration-0 rule, device convergence, the
+// init-mode cascade, the return of the previous iterate on convergence and
+// the iteration limit.
+void launch_nt_post(NtInst* nt, int ni, const int32_t* xoff, const int32_t* n,
+                    const int32_t* num_nodes, double* x, const double* xold, DevCtx* ctx,
+                    const int32_t* kind, const int32_t* fail, const int32_t* devnc, int32_t* done,
+                    NtConvTol tol, hipStream_t s);
+
+// TimeStepController::check_ringing (timestep.cpp:191-229) on the buffers
+// solve_transient passes after its history rotation (transient.cpp:1039-1048):
+// sol = x, sol_prev = x, sol_prev2 = h1, sol_prev3 = h2.  ring[i] = 1 when
+// an instance rings; only instances with sel[i] != 0 are tested.
+void launch_tr_ring(int ni, const int32_t* xoff, const int32_t* num_nodes, const double* x,
+                    const double* h1, const double* h2, double reltol, double vntol,
+                    const int32_t* sel, int32_t* ring, hipStream_t s);
+// After the step decisions: acc[i] == 1 (accepted) rotates the solution
+// history (h3 = h2, h2 = h1, h1 = x) and records the probes at row
+// prow[i]; acc[i] == 2 (rejected / failed) restores x = h1 (sol_prev).
+void launch_tr_hist(int ni, const int32_t* xoff, const int32_t* n, double* x, double* h1,
+                    double* h2, double* h3, const int32_t* acc, const int32_t* pptr,
+                    const int32_t* probe, const int32_t* prow, double* pvals, int max_points,
+                    hipStream_t s);
+// Circuit::rotate_state (circuit.cpp:328-334) for accepted instances:
+// s3 = s2, s2 = s1, s1 = s0 per state slot.  Slots [0, ns_lane) belong to
+// the lane-strided family (instance lane_inst[slot % lanes]); the others to
+// slot_inst[slot - ns_lane].
+void launch_tr_states(long ns, long ns_lane, int lanes, const int32_t* lane_inst,
+                      const int32_t* slot_inst, const int32_t* acc, double* s0, double* s1,
+                      double* s2, double* s3, hipStream_t s);
+// dst[i] = src[i] for i < count.
+void launch_copy_f64(double* dst, const double* src, long count, hipStream_t s);
+
+}  // namespace nsa

@@ -50,9 +50,10 @@ __global__ void k_cap(CapTable t, const DevCtx* __restrict__ ctx, const double* 
     const long gid = (long)blockIdx.x * TPB + threadIdx.x;
     if (gid >= (long)t.count * B) return;
     const int d = (int)(gid / B), inst = (int)(gid % B);
-    if (done && done[inst]) return;
+    const int ci = t.inst_of ? t.inst_of[gid] : inst;
+    if (done && done[ci]) return;
     const size_t sB = (size_t)B;
-    const DevCtx c = ctx[inst];
+    const DevCtx c = ctx[ci];
     double* m = cm + (size_t)(t.site + 4 * d) * sB + inst;
     double* r = cr + (size_t)(t.rsite + 2 * d) * sB + inst;
     double geq = 0.0, ceq = 0.0;
@@ -148,8 +149,9 @@ __global__ void k_vsrc(VsrcTable t, const DevCtx* __restrict__ ctx,
     const long gid = (long)blockIdx.x * TPB + threadIdx.x;
     if (gid >= (long)t.count * B) return;
     const int d = (int)(gid / B), inst = (int)(gid % B);
-    if (done && done[inst]) return;
-    const DevCtx c = ctx[inst];
+    const int ci = t.inst_of ? t.inst_of[gid] : inst;
+    if (done && done[ci]) return;
+    const DevCtx c = ctx[ci];
     const double dcv = t.dc[gid];
     double val;
     if (t.dc_given[d] && t.wave[d] != WAVE_DC && (c.mode & (MODEDCOP | MODEDCTRANCURVE)))
@@ -166,8 +168,9 @@ __global__ void k_isrc(IsrcTable t, const DevCtx* __restrict__ ctx,
     const long gid = (long)blockIdx.x * TPB + threadIdx.x;
     if (gid >= (long)t.count * B) return;
     const int d = (int)(gid / B), inst = (int)(gid % B);
-    if (done && done[inst]) return;
-    const double I = t.dc[gid] * ctx[inst].src_fact;
+    const int ci = t.inst_of ? t.inst_of[gid] : inst;
+    if (done && done[ci]) return;
+    const double I = t.dc[gid] * ctx[ci].src_fact;
     double* r = cr + (size_t)(t.rsite + 2 * d) * B + inst;
     r[0] = -I;
     r[(size_t)B] = I;
@@ -310,11 +313,13 @@ __device__ __forceinline__ double terr(int q, const double* s0, const double* s1
 __global__ void k_trunc(TruncTable t, const DevCtx* __restrict__ ctx, const double* __restrict__ s0,
                         const double* __restrict__ s1, const double* __restrict__ s2,
                         const double* __restrict__ s3, int B, TruncOpts o, double* __restrict__ dt_dev,
-                        unsigned long long* __restrict__ dt_min) {
+                        unsigned long long* __restrict__ dt_min, int order_override) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= t.ndev * B) return;
     const int inst = idx % B, dev = idx / B;
-    const DevCtx c = ctx[inst];
+    const int ci = t.inst_of ? t.inst_of[idx] : inst;
+    DevCtx c = ctx[ci];
+    if (order_override > 0) c.order = order_override;
     double dt = 1e30;
     // compute_trunc's guard; orders above 2 would need more than four states
     if (c.order >= 1 && c.order <= 2 && c.delta > 0.0) {
@@ -334,7 +339,7 @@ __global__ void k_trunc(TruncTable t, const DevCtx* __restrict__ ctx, const doub
     }
     if (dt_dev) dt_dev[idx] = dt;
     // dt >= 0: the bit patterns of non-negative doubles order like the values
-    atomicMin(&dt_min[inst], (unsigned long long)__double_as_longlong(dt));
+    atomicMin(&dt_min[ci], (unsigned long long)__double_as_longlong(dt));
 }
 
 __global__ void k_fill_u64(unsigned long long* p, unsigned long long v, int count) {
@@ -346,11 +351,11 @@ __global__ void k_fill_u64(unsigned long long* p, unsigned long long v, int coun
 
 void launch_trunc(const TruncTable& t, const DevCtx* ctx, const double* s0, const double* s1,
                   const double* s2, const double* s3, int B, const TruncOpts& o, double* dt_dev,
-                  unsigned long long* dt_min, hipStream_t s) {
+                  unsigned long long* dt_min, hipStream_t s, int order_override) {
     const int total = t.ndev * B;
     if (total <= 0) return;
     hipLaunchKernelGGL(k_trunc, dim3((total + 255) / 256), dim3(256), 0, s, t, ctx, s0, s1, s2, s3, B, o,
-                       dt_dev, dt_min);
+                       dt_dev, dt_min, order_override);
 }
 
 void launch_fill_u64(unsigned long long* p, unsigned long long v, int count, hipStream_t s) {

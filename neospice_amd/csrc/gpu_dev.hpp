@@ -53,6 +53,7 @@ struct CapTable {
     const double* farads;      // count x B
     int32_t site;              // first matrix site (4 per device)
     int32_t rsite;             // first rhs site (2 per device)
+    const int32_t* inst_of;    // null, or the context instance of every lane (see below)
 };
 
 struct VsrcTable {
@@ -65,12 +66,14 @@ struct VsrcTable {
     const double* pwl_t;
     const double* pwl_v;
     int32_t rsite;             // 1 per device
+    const int32_t* inst_of;
 };
 
 struct IsrcTable {
     int32_t count;
     const double* dc;          // count x B
     int32_t rsite;             // 2 per device
+    const int32_t* inst_of;
 };
 
 // Level-1 diodes (RS = 0: posPrime is the anode).  7 matrix sites and 2 rhs
@@ -81,6 +84,7 @@ struct DioTable {
     const int32_t* state;   // state base per device (5 slots)
     const DioParam* par;    // per device
     int32_t site, rsite;
+    const int32_t* inst_of;
 };
 
 // MOS1 (RD = RS = 0: dNodePrime = d, sNodePrime = s).  22 matrix sites and
@@ -96,6 +100,7 @@ struct Mos1Table {
     const Mos1Param* par;   // per device
     double* aux;
     int32_t site, rsite;
+    const int32_t* inst_of;
 };
 
 // BSIM4v7: the instance fields BSIM4v7load reads back on the next load
@@ -114,7 +119,19 @@ struct B4Table {
     const double* par_inst;
     double* aux;            // count x kB4Aux x B
     int32_t site, rsite;    // 16 matrix sites, 4 rhs sites per device
+    const int32_t* inst_of;
+    // null, or the terminals' solution indices per lane (4 x B, terminal
+    // major, -1 = ground) in place of nodes[]: a packed batch of circuits
+    // (one device per lane) reads its instance's own unknowns
+    const int32_t* pin;
 };
+
+// `inst_of` (every table): null for a batch of one circuit, where lane
+// (d, inst) reads ctx[inst], done[inst] and sets devnc[inst].  A packed
+// batch of different circuits lays each family out as one batch whose lanes
+// are the devices of every instance; inst_of[lane] (lane = d * B + inst)
+// then names the instance whose context, completion flag and
+// non-convergence flag the lane uses.
 
 // Tolerances a device load reads from SimOptions.
 struct DevTol {
@@ -168,6 +185,7 @@ struct TruncTable {
     int ndev;
     const int32_t* base;   // state base per device
     const int32_t* kind;   // 0 capacitor, 1 MOS1, 2 diode, 3 BSIM4v7
+    const int32_t* inst_of;
 };
 struct TruncOpts {
     double abstol, reltol, chgtol, trtol;
@@ -175,9 +193,11 @@ struct TruncOpts {
 // dt_dev (ndev x B, optional) gets each device's compute_trunc; dt_min (B,
 // the bits of non-negative doubles, filled with those of 1e30 first) their
 // minimum per instance.
+// order_override > 0 evaluates the devices at that integration order instead
+// of the context's (the order-promotion test, transient.cpp:946-958).
 void launch_trunc(const TruncTable& t, const DevCtx* ctx, const double* s0, const double* s1,
                   const double* s2, const double* s3, int B, const TruncOpts& o, double* dt_dev,
-                  unsigned long long* dt_min, hipStream_t s);
+                  unsigned long long* dt_min, hipStream_t s, int order_override = 0);
 void launch_fill_u64(unsigned long long* p, unsigned long long v, int count, hipStream_t s);
 // out[k] = A[k * B + inst], k < count (one instance of a slot-major batch)
 void launch_take_instance(const double* A, int B, int inst, int count, double* out, hipStream_t s);

@@ -86,9 +86,10 @@ __global__ void k_diode(DioTable t, const DevCtx* __restrict__ ctx, const double
     const long gid = (long)blockIdx.x * TPB + threadIdx.x;
     if (gid >= (long)t.count * B) return;
     const int d = (int)(gid / B), inst = (int)(gid % B);
-    if (done && done[inst]) return;
+    const int ci = t.inst_of ? t.inst_of[gid] : inst;
+    if (done && done[ci]) return;
     const size_t sB = (size_t)B;
-    const DevCtx c = ctx[inst];
+    const DevCtx c = ctx[ci];
     const DioParam& p = t.par[d];
     const int mode = c.mode;
     const size_t sv = (size_t)t.state[d] * sB + inst;  // DIOvoltage
@@ -303,7 +304,7 @@ __global__ void k_diode(DioTable t, const DevCtx* __restrict__ ctx, const double
         const double tl = tol.reltol * smax(fabs(cdhat), fabs(cdv)) + tol.abstol;
         if (!(fabs(cdhat - cdv) <= tl)) noncon = 1;
     }
-    if (noncon) atomicOr(&devnc[inst], 1);
+    if (noncon) atomicOr(&devnc[ci], 1);
     const double cdeq = cd - gd * vd;
     r[0] = cdeq;        // CKTrhs[neg] += cdeq
     r[sB] = -cdeq;      // CKTrhs[posPrime] -= cdeq
@@ -326,9 +327,10 @@ __global__ void k_mos1(Mos1Table t, const DevCtx* __restrict__ ctx, const double
     const long gid = (long)blockIdx.x * TPB + threadIdx.x;
     if (gid >= (long)t.count * B) return;
     const int d = (int)(gid / B), inst = (int)(gid % B);
-    if (done && done[inst]) return;
+    const int ci = t.inst_of ? t.inst_of[gid] : inst;
+    if (done && done[ci]) return;
     const size_t sB = (size_t)B;
-    const DevCtx c = ctx[inst];
+    const DevCtx c = ctx[ci];
     const Mos1Param& p = t.par[d];
     const int mode = c.mode;
     const size_t st = (size_t)t.state[d] * sB + inst;
@@ -533,7 +535,7 @@ __global__ void k_mos1(Mos1Table t, const DevCtx* __restrict__ ctx, const double
     }
     // convergence: OFF devices are exempt only in INITFIX / INITSMSIG
     const bool noncon = (p.off == 0 || !(mode & (MODEINITFIX | MODEINITSMSIG))) && Check == 1;
-    if (noncon) atomicOr(&devnc[inst], 1);
+    if (noncon) atomicOr(&devnc[ci], 1);
     s0[S(vbsS)] = vbs;
     s0[S(vbdS)] = vbd;
     s0[S(vgsS)] = vgs;
@@ -658,7 +660,8 @@ __global__ void k_mos1_conv(Mos1Table t, const double* __restrict__ x,
     const long gid = (long)blockIdx.x * TPB + threadIdx.x;
     if (gid >= (long)t.count * B) return;
     const int d = (int)(gid / B), inst = (int)(gid % B);
-    if (skip && skip[inst]) return;
+    const int ci = t.inst_of ? t.inst_of[gid] : inst;
+    if (skip && skip[ci]) return;
     const size_t sB = (size_t)B;
     const Mos1Param& p = t.par[d];
     const size_t st = (size_t)t.state[d] * sB + inst;
@@ -692,7 +695,7 @@ __global__ void k_mos1_conv(Mos1Table t, const double* __restrict__ x,
         tl = tol.reltol * smax(fabs(cbhat), fabs(cb)) + tol.abstol;
         ok = fabs(cbhat - cb) <= tl;
     }
-    if (!ok) atomicOr(&devnc[inst], 1);
+    if (!ok) atomicOr(&devnc[ci], 1);
 }
 
 __global__ void k_or_i32(int32_t* __restrict__ dst, const int32_t* __restrict__ src, int count) {

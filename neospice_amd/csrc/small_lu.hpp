@@ -1,0 +1,114 @@
+// Packed LU for batches of small circuits: one wave per instance, the
+// instance's factor held in LDS for the whole refactor and solve.
+//
+// Every instance may have its own matrix (its own circuit topology and its
+// own pivot order): instances point at a plan, and instances whose Sparse 1.3
+// structures are identical share one.  Values are packed instance after
+// instance (A at aoff, factors at luoff, vectors at xoff) instead of being
+// interleaved, so batches of different sizes sit side by side.
+//
+// The refactor restates SparseMatrix::factor (src/solver/factor.cpp:980-1052)
+// right-looking: step k takes d_k = 1/x_kk and applies, to every (i, j) with
+// L(i,k) and U(k,j) in the pattern, x_ij -= (x_kj * d_k) * L(i,k).  Entry
+// (i, j) receives exactly the terms of its left-looking column loop, in the
+// same ascending-k order, with the same two roundings per term, so the
+// factor is bit-identical; steps whose entries do not overlap share a level.
+// Sparse 1.3 stores U(k,j) = x_kj * d_k and the reciprocal pivot, which a
+// closing pass writes.  The solves are the level-ordered row gathers of
+// build_solve_programs (solve.cpp:6-47, 102-141).
+//
+// Only FORM_S13 structures are planned here: make_solver picks AMD-LU only
+// for linear circuits of at least 256 unknowns (make_solver.cpp:156-176),
+// which run on the instance-interleaved kernels of gpu_lu.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "lu_structure.hpp"
+
+namespace nsa {
+
+// Per-instance work descriptor of a packed launch.
+struct SluInst {
+    int32_t plan;   // plan id
+    int32_t aoff;   // first matrix value of the instance (slot order)
+    int32_t luoff;  // first factor value
+    int32_t xoff;   // first unknown
+};
+
+// Header of one plan inside the flattened plan data (int32 words; every
+// offset is relative to the plan's first word, and every section starts on
+// a 4-word boundary so that 16-byte records load as one vector).
+enum SluHdr : int32_t {
+    SH_N = 0, SH_NLU, SH_NNZA, SH_NLEV, SH_NTASK,
+    SH_APOS,    // nnz_a: position receiving A slot p
+    SH_GMIN,    // n: position receiving diag_gmin, or -1
+    SH_LEV,     // nlev+1 task offsets (relative to SH_TASK, in tasks)
+    SH_TASK,    // tasks, 4 words each: target, U(k,j), L(i,k), pivot (k,k) positions
+    SH_CPTR,    // n+1 column segments
+    SH_NU,      // n: U entries per column
+    SH_ROWS,    // nlu: pivot-space row of each position
+    SH_PCOL,    // nlu: pivot-space column of each position
+    SH_RPERM, SH_CPERM,
+    SH_PROG,    // 4 programs (fwd, bwd, tfwd, tbwd), SH_PROG + 3*p: nlev, levptr off, rows off
+    SH_WORDS = SH_PROG + 12,
+};
+// Program rows: 4 words each (target, first term, term count, diagonal
+// position or -1); terms: 2 words each (LU position, source).  The program
+// op is fixed by the form: fwd SKIPZERO|MUL_NZ, bwd PLAIN, tfwd SKIPZERO,
+// tbwd MUL_DIAG.
+
+// Kinds per instance (the same codes the interleaved kernels use).
+enum SluKind : int32_t { SLU_REFACTOR = 0, SLU_VERIFY = 1, SLU_TRUSTED = 2, SLU_SKIP = 3 };
+// Launch modes.
+enum SluMode : int32_t { SLU_FACTOR = 1, SLU_SOLVE = 2, SLU_SOLVE_T = 4 };
+
+constexpr int kSluMaxLu = 6144;   // factor values per instance held in LDS
+constexpr int kSluMaxN = 1024;
+
+// The plans of a packed batch, built on the host and mirrored on the device.
+class SmallLuPlans {
+public:
+    ~SmallLuPlans();
+    // Whether a structure fits the packed kernel.
+    static bool fits(const LuStructure& s) {
+        return s.form == FORM_S13 && s.nlu() <= kSluMaxLu && s.n <= kSluMaxN;
+    }
+    // Plan id of `s` (an existing identical plan is reused).
+    int add(const LuStructure& s);
+    const LuStructure& structure(int plan) const { return st_[plan]; }
+    int count() const { return (int)st_.size(); }
+    int max_nlu() const { return max_nlu_; }
+    int max_n() const { return max_n_; }
+    // Refactor levels and the solve programs' levels of a plan (statistics).
+    int levels(int plan) const { return lev_[plan]; }
+    int solve_levels(int plan) const { return slev_[plan]; }
+    // Copies new plans to the device (stream-ordered); returns the device
+    // words and offsets valid until the next sync() that adds plans.
+    void sync(hipStream_t s);
+    const int32_t* d_words() const { return d_words_; }
+    const int32_t* d_offsets() const { return d_off_; }
+    // host copies of the flattened plans (what sync() uploads)
+    const std::vector<int32_t>& host_words() const { return words_; }
+    const std::vector<int32_t>& host_offsets() const { return off_; }
+
+private:
+    std::vector<LuStructure> st_;
+    std::vector<int32_t> words_, off_;
+    std::vector<int> lev_, slev_;
+    int max_nlu_ = 0, max_n_ = 0;
+    int32_t* d_words_ = nullptr;
+    int32_t* d_off_ = nullptr;
+    size_t d_words_cap_ = 0, d_off_cap_ = 0, synced_words_ = 0, synced_plans_ = 0;
+};
+
+// One packed launch: factor (per-instance kinds, fail codes and first
+// failing column), solve (rhs -> x, may alias) or transposed solve.  A, LU,
+// rhs, x are packed device arrays addressed through `inst`.  Asynchronous.
+void launch_small_lu(const SmallLuPlans& plans, const SluInst* d_inst, int ninst, int mode,
+                     const double* A, double diag_gmin, const int32_t* kind, int32_t* fail,
+                     int32_t* fail_col, double* LU, const double* rhs, double* x, hipStream_t s);
+
+}  // namespace nsa

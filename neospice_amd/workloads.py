@@ -173,3 +173,25 @@ def ring_oscillator(stages=31, vdd=1.0, wn=0.5e-6, wp=1.0e-6, l=0.05e-6, toxe=1.
             ckt.model(".model nrst nmos level=14 toxe=%.17g vth0=%.17g" % (toxe, vth0))
             ckt.M("mrst", nodes[0], ne, GND, GND, "nrst", 8 * wn, l)
     return ckt, {"stages": stages, "nodes": nodes}
+
+
+def c3_stage_counts(count, config_index=3, lo=31, hi=201):
+    """Stage counts of the uneven-size ring batch: uniform over the odd
+    numbers lo..hi (seeded)."""
+    odd = np.arange(lo, hi + 1, 2)
+    return rng(config_index).choice(odd, size=count).astype(int)
+
+
+def c3_rings(count=256, config_index=3, stages=None, **kw):
+    """The uneven-size BSIM4v7 ring-oscillator batch: `count` rings with
+    stage counts from c3_stage_counts (or `stages`), each with its own
+    per-transistor mismatch (3 % VTH0, 1 % TOXE) from its own seeded stream.
+    Returns (circuits, stage counts)."""
+    if stages is None:
+        stages = c3_stage_counts(count, config_index)
+    streams = instance_rngs(config_index, len(stages))
+    out = []
+    for s, r in zip(stages, streams):
+        ckt, _ = ring_oscillator(int(s), mismatch_rng=r, **kw)
+        out.append(ckt)
+    return out, np.asarray(stages)
