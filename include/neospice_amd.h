@@ -191,6 +191,14 @@ int nsa_bundle_offsets(nsa_bundle* b, int32_t* xoff, int32_t* aoff);
  * instance in `mode` (MODEDCOP or MODETRANOP with MODEINITJCT) from the
  * current solution; converged / iterations per instance. */
 int nsa_bundle_dc(nsa_bundle* b, int32_t mode, int32_t* converged, int32_t* iterations);
+/* The transient preamble's operating point (compute_dc_operating_point,
+ * src/core/transient.cpp:172-238) from the current solution: direct Newton in
+ * MODETRANOP, then gmin stepping (src/core/convergence.cpp:216-346) for the
+ * instances that failed.  stage per instance: 0 direct, 1 gmin stepping,
+ * -1 not converged (source stepping and pseudo-transient are refused).
+ * skip_direct = 1 runs gmin stepping alone on every instance. */
+int nsa_bundle_dc_ladder(nsa_bundle* b, int32_t skip_direct, int32_t* converged, int32_t* iterations,
+                         int32_t* stage);
 int nsa_bundle_get_solution(nsa_bundle* b, double* x);
 int nsa_bundle_set_solution(nsa_bundle* b, const double* x);
 /* solve_transient (src/core/transient.cpp:694-1066) for every instance:

@@ -538,6 +538,15 @@ class Bundle:
         check(lib().nsa_bundle_dc(self.h, int(mode), i32p(conv), i32p(its)))
         return conv.astype(bool), its
 
+    def op_ladder(self, x0=0.0, skip_direct=False):
+        """The transient preamble's operating point: direct Newton in
+        MODETRANOP, then gmin stepping for the instances that failed.
+        Returns (converged, iterations, stage) per instance."""
+        self.set_solution(x0)
+        conv, its, st = (np.zeros(self.N, np.int32) for _ in range(3))
+        check(lib().nsa_bundle_dc_ladder(self.h, int(bool(skip_direct)), i32p(conv), i32p(its), i32p(st)))
+        return conv.astype(bool), its, st
+
     def tran_begin(self, tstep, tstop, probes=None, max_points=1):
         """solve_transient's preamble for every instance; probes: one list
         of unknown indices per instance (or None)."""

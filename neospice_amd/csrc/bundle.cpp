@@ -125,6 +125,16 @@ Bundle::Bundle(const std::vector<const Circuit*>& circuits, const Options& opt)
     NSA_HIP(hipHostMalloc((void**)&h_kind_pin_, sizeof(int32_t) * 2 * NI_, hipHostMallocDefault));
     std::vector<int32_t> nn(n_.begin(), n_.end()), nd(nnodes_.begin(), nnodes_.end());
     d_ixoff_ = dupload(allocs_, std::vector<int32_t>(xoff_.begin(), xoff_.end()));
+    {
+        std::vector<int32_t> xo(tot_n_);
+        for (int i = 0; i < NI_; ++i)
+            for (int k = xoff_[i]; k < xoff_[i + 1]; ++k) xo[k] = i;
+        d_xinst_ = dupload(allocs_, xo);
+    }
+    d_gmin_ = dupload(allocs_, std::vector<double>(NI_, opt_.diag_gmin));
+    d_mask_ = dalloc<int32_t>(allocs_, 3 * (size_t)NI_);
+    NSA_HIP(hipHostMalloc((void**)&h_mask_pin_, sizeof(int32_t) * 3 * NI_, hipHostMallocDefault));
+    NSA_HIP(hipHostMalloc((void**)&h_gmin_pin_, sizeof(double) * NI_, hipHostMallocDefault));
     d_in_ = dupload(allocs_, nn);
     d_inn_ = dupload(allocs_, nd);
     tol_.gmin = opt_.gmin;
@@ -156,6 +166,8 @@ Bundle::~Bundle() {
     if (h_ctx_pin_) (void)hipHostFree(h_ctx_pin_);
     if (h_nt_pin_) (void)hipHostFree(h_nt_pin_);
     if (h_kind_pin_) (void)hipHostFree(h_kind_pin_);
+    if (h_mask_pin_) (void)hipHostFree(h_mask_pin_);
+    if (h_gmin_pin_) (void)hipHostFree(h_gmin_pin_);
     if (h_acc_pin_) (void)hipHostFree(h_acc_pin_);
     if (h_tmin_pin_) (void)hipHostFree(h_tmin_pin_);
     if (h_ring_pin_) (void)hipHostFree(h_ring_pin_);
@@ -526,7 +538,7 @@ void Bundle::launch_loads() {
 
 void Bundle::launch_lu(const int32_t* d_kind, int mode) {
     launch_small_lu(plans_, d_inst_, NI_, mode, d_A_, opt_.diag_gmin, d_kind, d_fail_, d_fail_col_,
-                    d_LU_, d_rhs_, d_x_, stream_);
+                    d_LU_, d_rhs_, d_x_, stream_, d_gmin_);
 }
 
 void Bundle::launch_post(int) {
@@ -748,6 +760,149 @@ void Bundle::newton(int itl, const int32_t* active, std::vector<int32_t>& conver
     }
 }
 
+// ---------------------------------------------------------------- DC ladder
+
+void Bundle::set_gmin(const std::vector<double>& g) {
+    NSA_HIP(hipStreamSynchronize(stream_));
+    std::memcpy(h_gmin_pin_, g.data(), sizeof(double) * NI_);
+    NSA_HIP(hipMemcpyAsync(d_gmin_, h_gmin_pin_, sizeof(double) * NI_, hipMemcpyHostToDevice, stream_));
+}
+
+// Copies, for the instances with mask[i] != 0, the solution and states 0-2
+// from the buffers `src` (x, s0, s1, s2; null = zeros) into `dst`.
+void Bundle::masked_copy_state(const std::vector<int32_t>& mask, double* const* src, double* const* dst) {
+    bool any = false;
+    for (int32_t m : mask) any |= m != 0;
+    if (!any) return;
+    NSA_HIP(hipStreamSynchronize(stream_));
+    std::memcpy(h_mask_pin_, mask.data(), sizeof(int32_t) * NI_);
+    NSA_HIP(hipMemcpyAsync(d_mask_, h_mask_pin_, sizeof(int32_t) * NI_, hipMemcpyHostToDevice, stream_));
+    launch_masked_copy(tot_n_, 0, 1, d_xinst_, d_xinst_, d_mask_, src ? src[0] : nullptr, dst[0], stream_);
+    for (int k = 1; k < 4; ++k)
+        launch_masked_copy(ns_tot_, ns_lane_, b4_lanes_, b4_.inst_of, d_slot_inst_, d_mask_,
+                           src ? src[k] : nullptr, dst[k], stream_);
+    NSA_HIP(hipStreamSynchronize(stream_));
+}
+
+void Bundle::op_ladder(int firstmode, int continuemode, std::vector<int32_t>& converged,
+                       std::vector<int32_t>& iterations, std::vector<int32_t>& stage,
+                       bool skip_direct) {
+    // 1. direct Newton (transient.cpp:195-205)
+    if (skip_direct) {
+        converged.assign(NI_, 0);
+        iterations.assign(NI_, 0);
+        std::vector<DevCtx> c = h_ctx_;
+        for (auto& x : c) x.mode = firstmode;
+        set_ctx(c);
+    } else {
+        dc_op(firstmode, converged, iterations);
+    }
+    stage.assign(NI_, -1);
+    std::vector<int32_t> failing(NI_, 0);
+    bool any = false;
+    for (int i = 0; i < NI_; ++i) {
+        if (converged[i]) stage[i] = 0;
+        failing[i] = !converged[i];
+        any |= failing[i] != 0;
+    }
+    if (!any) return;
+    // 2. gmin stepping (convergence.cpp:216-346) for the instances that failed
+    if (!d_lad_[0])
+        for (int k = 0; k < 8; ++k)
+            d_lad_[k] = dalloc<double>(allocs_, k % 4 == 0 ? tot_n_ : std::max(1, ns_tot_));
+    double* cur[4] = {d_x_, d_s_[0], d_s_[1], d_s_[2]};
+    double* entry[4] = {d_lad_[0], d_lad_[1], d_lad_[2], d_lad_[3]};
+    double* saved[4] = {d_lad_[4], d_lad_[5], d_lad_[6], d_lad_[7]};
+    masked_copy_state(failing, cur, entry);
+    masked_copy_state(failing, nullptr, cur);   // zero initial guess, clear_state
+    masked_copy_state(failing, cur, saved);
+    const double gmin_factor = 10.0;
+    const double gtarget = std::max(opt_.gmin, 0.0);  // max(gmin, gshunt)
+    const int itl2 = 50;                               // SimOptions::itl2
+    const int step_itl = std::max(itl2, 100);
+    enum { L_IDLE = 0, L_STEP, L_FINAL, L_DONE, L_FAILED };
+    std::vector<int> phase(NI_, L_IDLE), mode(NI_, firstmode), total(NI_, 0);
+    std::vector<double> factor(NI_, gmin_factor), old_gmin(NI_, 1e-2), dg(NI_, 1e-2 / gmin_factor);
+    for (int i = 0; i < NI_; ++i)
+        if (failing[i]) phase[i] = L_STEP;
+    std::vector<int32_t> conv, its;
+    for (int round = 0; round < 100000; ++round) {
+        std::vector<int32_t> active(NI_, 0);
+        std::vector<double> g(NI_, opt_.diag_gmin);
+        std::vector<DevCtx> ctx = h_ctx_;
+        bool live = false;
+        for (int i = 0; i < NI_; ++i) {
+            if (phase[i] != L_STEP && phase[i] != L_FINAL) continue;
+            active[i] = 1;
+            live = true;
+            ctx[i].mode = mode[i];
+            g[i] = (phase[i] == L_STEP) ? dg[i] : 0.0;  // the final solve at gshunt
+        }
+        if (!live) break;
+        set_ctx(ctx);
+        set_gmin(g);
+        // both kinds of solve run the same iteration limit (max(itl2, 100) and
+        // SimOptions::max_iter, 100 by default)
+        newton(std::max(step_itl, opt_.itl1), active.data(), conv, its);
+        std::vector<int32_t> save(NI_, 0), back(NI_, 0), to_entry(NI_, 0);
+        for (int i = 0; i < NI_; ++i) {
+            if (!active[i]) continue;
+            total[i] += its[i];
+            if (phase[i] == L_FINAL) {
+                if (conv[i]) {
+                    phase[i] = L_DONE;
+                } else {
+                    phase[i] = L_FAILED;
+                    to_entry[i] = 1;
+                }
+                continue;
+            }
+            if (conv[i]) {
+                mode[i] = continuemode;
+                if (dg[i] <= gtarget) {
+                    phase[i] = L_FINAL;
+                } else {
+                    save[i] = 1;
+                    // ngspice cktop.c:187-194
+                    if (its[i] <= itl2 / 4) {
+                        factor[i] *= std::sqrt(factor[i]);
+                        if (factor[i] > gmin_factor) factor[i] = gmin_factor;
+                    }
+                    if (its[i] > (3 * itl2 / 4)) factor[i] = std::max(std::sqrt(factor[i]), 1.00005);
+                    old_gmin[i] = dg[i];
+                    if (dg[i] < factor[i] * gtarget) {
+                        factor[i] = dg[i] / gtarget;
+                        dg[i] = gtarget;
+                    } else {
+                        dg[i] /= factor[i];
+                    }
+                }
+            } else if (factor[i] < 1.00005) {
+                phase[i] = L_FAILED;
+                to_entry[i] = 1;
+            } else {
+                factor[i] = std::sqrt(std::sqrt(factor[i]));
+                dg[i] = old_gmin[i] / factor[i];
+                back[i] = 1;
+            }
+        }
+        masked_copy_state(save, cur, saved);
+        masked_copy_state(back, saved, cur);
+        masked_copy_state(to_entry, entry, cur);
+    }
+    set_gmin(std::vector<double>(NI_, opt_.diag_gmin));
+    for (int i = 0; i < NI_; ++i) {
+        if (!failing[i]) continue;
+        if (phase[i] == L_DONE) {
+            converged[i] = 1;
+            iterations[i] = total[i];
+            stage[i] = 1;
+        } else {
+            converged[i] = 0;
+        }
+    }
+}
+
 void Bundle::dc_op(int mode, std::vector<int32_t>& converged, std::vector<int32_t>& iterations) {
     std::vector<DevCtx> c(NI_);
     for (int i = 0; i < NI_; ++i) {
@@ -865,7 +1020,7 @@ void Bundle::tran_begin(double tstep, double tstop, const int32_t* probe_ptr, co
     std::vector<double> z(tot_n_, 0.0);
     set_solution(z.data());
     std::vector<int32_t> conv, its;
-    dc_op(MODETRANOP | MODEINITJCT, conv, its);
+    op_ladder(MODETRANOP | MODEINITJCT, MODETRANOP | MODEINITFLOAT, conv, its, dc_stage_);
     dc_iterations_ = its;
     for (int i = 0; i < NI_; ++i)
         if (!conv[i])

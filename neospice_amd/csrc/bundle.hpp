@@ -105,6 +105,17 @@ public:
     // Direct Newton attempt of the DC operating point (dc.cpp:104-107 /
     // transient.cpp:195-205) for every instance, from the current solution.
     void dc_op(int mode, std::vector<int32_t>& converged, std::vector<int32_t>& iterations);
+    // The operating-point ladder of the transient preamble
+    // (compute_dc_operating_point, transient.cpp:172-238): direct Newton in
+    // firstmode, then gmin stepping (convergence.cpp:216-346) with
+    // per-instance diag_gmin for the instances that failed.  stage[i]: 0
+    // direct, 1 gmin stepping, -1 neither converged (source stepping and
+    // pseudo-transient are not restated).
+    // skip_direct: gmin stepping for every instance, from the current solution.
+    void op_ladder(int firstmode, int continuemode, std::vector<int32_t>& converged,
+                   std::vector<int32_t>& iterations, std::vector<int32_t>& stage,
+                   bool skip_direct = false);
+    const std::vector<int32_t>& dc_stage() const { return dc_stage_; }
     // A fresh solver for every instance (make_solver + symbolic).
     void reset_solver();
 
@@ -150,6 +161,8 @@ private:
     void capture_graph();
     void host_lu_post(std::vector<NtInst>& nt, const std::vector<int>& repivot, bool others);
     void collect_timing();
+    void set_gmin(const std::vector<double>& g);
+    void masked_copy_state(const std::vector<int32_t>& mask, double* const* src, double* const* dst);
     std::vector<double> instance_matrix(int i);
     void upload_inst();
     void tran_attempt_setup(int i, DevCtx& c);
@@ -207,6 +220,13 @@ private:
     int b4_lanes_ = 0;
     long ns_lane_ = 0;
     int32_t* d_slot_inst_ = nullptr;
+    int32_t* d_xinst_ = nullptr;   // owner instance of every unknown
+    double* d_gmin_ = nullptr;     // diag_gmin per instance
+    double* h_gmin_pin_ = nullptr;
+    int32_t* d_mask_ = nullptr;
+    int32_t* h_mask_pin_ = nullptr;
+    double* d_lad_[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    std::vector<int32_t> dc_stage_;
     // transient
     std::vector<TranCtl> tc_;
     double tstep_ = 0.0, tstop_ = 0.0;

@@ -651,6 +651,18 @@ int nsa_bundle_timing(nsa_bundle* b, double out[5]) {
 int nsa_bundle_set_graph(nsa_bundle* b, int32_t on) {
     return guard([&] { b->b->use_graph = on != 0; return 0; });
 }
+int nsa_bundle_dc_ladder(nsa_bundle* b, int32_t skip_direct, int32_t* converged, int32_t* iterations,
+                         int32_t* stage) {
+    return guard([&] {
+        std::vector<int32_t> c, it, st;
+        b->b->op_ladder(0x20 | 0x200, 0x20 | 0x100, c, it, st, skip_direct != 0);
+        const int n = b->b->size();
+        if (converged) std::memcpy(converged, c.data(), sizeof(int32_t) * n);
+        if (iterations) std::memcpy(iterations, it.data(), sizeof(int32_t) * n);
+        if (stage) std::memcpy(stage, st.data(), sizeof(int32_t) * n);
+        return 0;
+    });
+}
 int nsa_bundle_plan_of(nsa_bundle* b, int32_t inst) {
     return guard([&] {
         if (inst < 0 || inst >= b->b->size()) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "instance out of range");

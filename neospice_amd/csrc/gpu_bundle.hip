@@ -212,6 +212,16 @@ __global__ void k_tr_states(long ns, long ns_lane, int lanes, const int32_t* __r
     s1[k] = s0[k];
 }
 
+__global__ void k_masked_copy(long ns, long ns_lane, int lanes, const int32_t* __restrict__ lane_inst,
+                              const int32_t* __restrict__ slot_inst, const int32_t* __restrict__ mask,
+                              const double* __restrict__ src, double* __restrict__ dst, int zero) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ns) return;
+    const int inst = (k < ns_lane) ? lane_inst[k % lanes] : slot_inst[k - ns_lane];
+    if (!mask[inst]) return;
+    dst[k] = zero ? 0.0 : src[k];
+}
+
 __global__ void k_copy_f64(double* __restrict__ dst, const double* __restrict__ src, long count) {
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < count) dst[k] = src[k];
@@ -260,6 +270,15 @@ void launch_tr_states(long ns, long ns_lane, int lanes, const int32_t* lane_inst
     const int tpb = 256;
     hipLaunchKernelGGL(k_tr_states, dim3((unsigned)((ns + tpb - 1) / tpb)), dim3(tpb), 0, s, ns,
                        ns_lane, lanes > 0 ? lanes : 1, lane_inst, slot_inst, acc, s0, s1, s2, s3);
+}
+
+void launch_masked_copy(long ns, long ns_lane, int lanes, const int32_t* lane_inst,
+                        const int32_t* slot_inst, const int32_t* mask, const double* src, double* dst,
+                        hipStream_t s) {
+    if (ns <= 0) return;
+    const int tpb = 256;
+    hipLaunchKernelGGL(k_masked_copy, dim3((unsigned)((ns + tpb - 1) / tpb)), dim3(tpb), 0, s, ns, ns_lane,
+                       lanes > 0 ? lanes : 1, lane_inst, slot_inst, mask, src, dst, src ? 0 : 1);
 }
 
 void launch_copy_f64(double* dst, const double* src, long count, hipStream_t s) {

@@ -55,6 +55,11 @@ void launch_tr_hist(int ni, const int32_t* xoff, const int32_t* n, double* x, do
 void launch_tr_states(long ns, long ns_lane, int lanes, const int32_t* lane_inst,
                       const int32_t* slot_inst, const int32_t* acc, double* s0, double* s1,
                       double* s2, double* s3, hipStream_t s);
+// dst[k] = src[k] (src null: 0) for the slots k of the instances with
+// mask[inst] != 0; slot owners as for launch_tr_states.
+void launch_masked_copy(long ns, long ns_lane, int lanes, const int32_t* lane_inst,
+                        const int32_t* slot_inst, const int32_t* mask, const double* src, double* dst,
+                        hipStream_t s);
 // dst[i] = src[i] for i < count.
 void launch_copy_f64(double* dst, const double* src, long count, hipStream_t s);
 

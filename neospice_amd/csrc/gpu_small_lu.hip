@@ -64,7 +64,8 @@ __device__ inline void run_program(const int32_t* __restrict__ P, int prog, cons
 __global__ __launch_bounds__(kW) void k_small_lu(const int32_t* __restrict__ words,
                                                 const int32_t* __restrict__ offs,
                                                 const SluInst* __restrict__ insts, int mode,
-                                                const double* __restrict__ A, double gmin,
+                                                const double* __restrict__ A, double gmin_all,
+                                                const double* __restrict__ gmin_inst,
                                                 const int32_t* __restrict__ kind,
                                                 int32_t* __restrict__ fail,
                                                 int32_t* __restrict__ fail_col, double* LU,
@@ -81,6 +82,7 @@ __global__ __launch_bounds__(kW) void k_small_lu(const int32_t* __restrict__ wor
     double* y = lds + max_nlu;  // n: reciprocal pivots, then the solve vector
     const int32_t* cptr = P + P[SH_CPTR];
     const int32_t* nU = P + P[SH_NU];
+    const double gmin = gmin_inst ? gmin_inst[ii] : gmin_all;
     if (mode & SLU_FACTOR) {
         for (int p = lane; p < nlu; p += kW) v[p] = 0.0;
         __syncthreads();
@@ -180,12 +182,13 @@ __global__ __launch_bounds__(kW) void k_small_lu(const int32_t* __restrict__ wor
 
 void launch_small_lu(const SmallLuPlans& plans, const SluInst* d_inst, int ninst, int mode,
                      const double* A, double diag_gmin, const int32_t* kind, int32_t* fail,
-                     int32_t* fail_col, double* LU, const double* rhs, double* x, hipStream_t s) {
+                     int32_t* fail_col, double* LU, const double* rhs, double* x, hipStream_t s,
+                     const double* gmin_inst) {
     if (ninst <= 0) return;
     const int max_nlu = plans.max_nlu();
     const size_t lds = sizeof(double) * ((size_t)max_nlu + plans.max_n());
     hipLaunchKernelGGL(k_small_lu, dim3(ninst), dim3(kW), lds, s, plans.d_words(), plans.d_offsets(),
-                       d_inst, mode, A, diag_gmin, kind, fail, fail_col, LU, rhs, x, max_nlu);
+                       d_inst, mode, A, diag_gmin, gmin_inst, kind, fail, fail_col, LU, rhs, x, max_nlu);
     NSA_HIP(hipGetLastError());
 }
 

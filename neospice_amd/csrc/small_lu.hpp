@@ -106,9 +106,12 @@ private:
 
 // One packed launch: factor (per-instance kinds, fail codes and first
 // failing column), solve (rhs -> x, may alias) or transposed solve.  A, LU,
-// rhs, x are packed device arrays addressed through `inst`.  Asynchronous.
+// rhs, x are packed device arrays addressed through `inst`.  diag_gmin is
+// added to every pivot's diagonal, or gmin_inst[i] per instance when given.
+// Asynchronous.
 void launch_small_lu(const SmallLuPlans& plans, const SluInst* d_inst, int ninst, int mode,
                      const double* A, double diag_gmin, const int32_t* kind, int32_t* fail,
-                     int32_t* fail_col, double* LU, const double* rhs, double* x, hipStream_t s);
+                     int32_t* fail_col, double* LU, const double* rhs, double* x, hipStream_t s,
+                     const double* gmin_inst = nullptr);
 
 }  // namespace nsa

@@ -51,6 +51,7 @@
 
 #include "core/circuit.hpp"
 #include "core/dc.hpp"
+#include "core/convergence.hpp"
 #include "core/newton.hpp"
 #include "core/transient.hpp"
 #include "core/sens.hpp"
@@ -822,6 +823,67 @@ int ref_amd_ordering(int n, const int32_t* col_ptr, const int32_t* row_idx, int3
     auto q = amd_ordering(n, col_ptr, row_idx);
     std::copy(q.begin(), q.end(), perm);
     return (int)q.size();
+}
+
+
+// The transient preamble's operating-point ladder (compute_dc_operating_point,
+// transient.cpp:172-238): direct Newton in MODETRANOP from the current
+// solution, then gmin stepping, source stepping and pseudo-transient on one
+// solver.  stage_out: 0..3 for the method that converged, -1 if none did.
+// Returns the iterations of the converged method (total over its solves).
+int ref_dc_ladder(void* p, double* solution, int* stage_out) {
+    auto* h = static_cast<RefCkt*>(p);
+    return guarded([&] {
+        Circuit& ckt = h->ckt;
+        ckt.finalize_if_needed();
+        ckt.integrator_ctx.options = &ckt.options;
+        auto solver = make_solver(ckt.num_vars(), ckt.is_linear());
+        solver->symbolic(ckt.pattern());
+        std::vector<double> sol(solution, solution + ckt.num_vars());
+        const int jct = 0x20 | 0x200, flt = 0x20 | 0x100;
+        *stage_out = -1;
+        ckt.integrator_ctx.mode = jct;
+        NewtonResult r = newton_solve(ckt, *solver, sol, ckt.options);
+        if (r.converged) {
+            *stage_out = 0;
+        } else {
+            r = gmin_stepping(ckt, *solver, sol, ckt.options, jct, flt);
+            if (r.converged) {
+                *stage_out = 1;
+            } else {
+                ckt.integrator_ctx.mode = jct;
+                r = source_stepping(ckt, *solver, sol, ckt.options);
+                if (r.converged) {
+                    *stage_out = 2;
+                } else {
+                    ckt.integrator_ctx.mode = jct;
+                    r = pseudo_transient(ckt, *solver, sol, ckt.options);
+                    if (r.converged) *stage_out = 3;
+                }
+            }
+        }
+        std::copy(sol.begin(), sol.end(), solution);
+        return r.iterations;
+    });
+}
+
+
+// gmin_stepping (convergence.cpp:216-346) alone, from the current solution,
+// in MODETRANOP (first MODEINITJCT, then MODEINITFLOAT), on a fresh solver.
+// Returns its iterations, negative when it did not converge.
+int ref_gmin_stepping(void* p, double* solution) {
+    auto* h = static_cast<RefCkt*>(p);
+    return guarded([&] {
+        Circuit& ckt = h->ckt;
+        ckt.finalize_if_needed();
+        ckt.integrator_ctx.options = &ckt.options;
+        auto solver = make_solver(ckt.num_vars(), ckt.is_linear());
+        solver->symbolic(ckt.pattern());
+        std::vector<double> sol(solution, solution + ckt.num_vars());
+        NewtonResult r = gmin_stepping(ckt, *solver, sol, ckt.options, 0x20 | 0x200, 0x20 | 0x100);
+        std::copy(sol.begin(), sol.end(), solution);
+        return r.converged ? r.iterations : -r.iterations;
+    });
 }
 
 }  // extern "C"

@@ -295,6 +295,22 @@ class RefCircuit:
         its = _chk(self.L.ref_dc(self.h, x.ctypes.data_as(P_F64), C.byref(sec)))
         return x, its, sec.value
 
+    def dc_ladder(self):
+        """The transient preamble's operating point from zeros: direct
+        Newton, gmin stepping, source stepping, pseudo-transient (ref_dc_ladder).
+        Returns (solution, iterations of the method that converged, stage)."""
+        x = np.zeros(self.n)
+        st = C.c_int()
+        its = _chk(self.L.ref_dc_ladder(self.h, x.ctypes.data_as(P_F64), C.byref(st)))
+        return x, its, st.value
+
+    def gmin_stepping(self):
+        """gmin_stepping alone from zeros on a fresh solver; returns
+        (solution, iterations), iterations negative when it failed."""
+        x = np.zeros(self.n)
+        its = self.L.ref_gmin_stepping(self.h, x.ctypes.data_as(P_F64))
+        return x, its
+
     def tran(self, tstep, tstop, max_points=200000):
         t = np.zeros(max_points)
         s = np.zeros(max_points * self.n)

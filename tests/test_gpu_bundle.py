@@ -77,6 +77,49 @@ def test_bundle_graph_matches_launches():
     assert b2.stats()["graph_launches"] == 0
 
 
+def test_bundle_gmin_stepping():
+    """gmin stepping (convergence.cpp:216-346) for every instance of a bundle
+    of rings, each on its own diag_gmin schedule: iteration counts (which fix
+    the schedule) and operating points against the reference's own
+    gmin_stepping."""
+    cks, st = W.c3_rings(stages=[5, 7, 9, 11, 13, 15])
+    for c in cks:
+        c.finalize()
+    b = na.Bundle(cks)
+    conv, its, stage = b.op_ladder(skip_direct=True)
+    assert conv.all() and (stage == 1).all()
+    for i, c in enumerate(cks):
+        x_ref, it_ref = RB.RefCircuit(c, probe=False).gmin_stepping()
+        print("ring", st[i], "iterations", its[i], it_ref, flush=True)
+        assert it_ref > 0
+        assert its[i] == it_ref, (i, its[i], it_ref)
+        tol = DC_RTOL * np.maximum(np.abs(x_ref), 1e-3) + 1e-12
+        assert (np.abs(b.solution(i) - x_ref) <= tol).all(), i
+
+
+def test_bundle_op_ladder_long_rings():
+    """Rings of the uneven-size batch whose direct Newton fails in the
+    reference (95-125 stages): the bundle's operating-point ladder reaches
+    the reference's operating point.  Which method gets there can differ:
+    the direct Newton of a long ring passes through ~1e11 V iterates where
+    last-ulp libm differences change its path."""
+    all_rings, st = W.c3_rings(256)
+    pick = [45, 41, 18, 2]
+    cks = [all_rings[i] for i in pick]
+    for c in cks:
+        c.finalize()
+    b = na.Bundle(cks)
+    conv, its, stage = b.op_ladder()
+    assert conv.all()
+    for i, c in enumerate(cks):
+        x_ref, it_ref, st_ref = RB.RefCircuit(c, probe=False).dc_ladder()
+        print("instance", pick[i], "stages", st[pick[i]], "stage", stage[i], st_ref, "iterations", its[i],
+              it_ref, flush=True)
+        assert st_ref >= 0
+        tol = DC_RTOL * np.maximum(np.abs(x_ref), 1e-3) + 1e-12
+        assert (np.abs(b.solution(i) - x_ref) <= tol).all(), (i, float(np.max(np.abs(b.solution(i) - x_ref))))
+
+
 def ref_adaptive(ckt, tstep, tstop):
     rc = RB.RefCircuit(ckt)
     t, x, _, _ = rc.tran(tstep, tstop)
