@@ -12,6 +12,11 @@
 //   4. U(k,j) = x_kj * (1 / x_kk) and the diagonal holds 1 / x_kk.
 // Solve (solve.cpp:6-47) and transposed solve (solve.cpp:102-141) run the
 // level-ordered row gathers of the plan on the LDS values.
+//
+// A level costs one LDS round trip plus its arithmetic, so the level loops
+// read their descriptors from LDS too: the plan's hot block (task levels and
+// the plain solve's programs) is copied there first when it fits, and each
+// level's first descriptor is fetched while the previous level finishes.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -24,43 +29,187 @@ namespace nsa {
 namespace {
 
 constexpr int kW = 64;
+constexpr size_t kLdsLimit = 64 * 1024;
 
 __device__ inline int wave_min_i32(int v) {
     for (int o = kW / 2; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
     return v;
 }
 
-// One triangular program: per level, lane-strided rows; each row is the
-// reference's own subtraction chain for its target.
-template <int OP>
-__device__ inline void run_program(const int32_t* __restrict__ P, int prog, const double* v, double* y,
-                                   int lane) {
-    const int nl = P[SH_PROG + 3 * prog];
-    const int32_t* lev = P + P[SH_PROG + 3 * prog + 1];
-    const int32_t* rows = P + P[SH_PROG + 3 * prog + 2];
-    for (int l = 0; l < nl; ++l) {
-        const int r0 = lev[l], r1 = lev[l + 1];
-        for (int r = r0 + lane; r < r1; r += kW) {
-            const int4 rec = *reinterpret_cast<const int4*>(rows + 4 * r);
-            const int32_t* tm = P + rec.y;
-            double acc = y[rec.x];
-            for (int q = 0; q < rec.z; ++q) {
-                const int pos = tm[2 * q], src = tm[2 * q + 1];
-                const double ys = y[src];
-                if ((OP & OP_SKIPZERO) && ys == 0.0) continue;
-                acc = acc - v[pos] * ys;
-            }
-            if (OP & OP_MUL_NZ) {
-                if (acc != 0.0) acc = acc * v[rec.w];
-            } else if (OP & OP_MUL_DIAG) {
-                acc = acc * v[rec.w];
-            }
-            y[rec.x] = acc;
-        }
-        __syncthreads();
+// Plan words: the hot block from LDS when it was staged there (STAGED), the
+// rest from global memory.  Staged and unstaged launches are separate
+// instantiations, so every hot-block read is a plain LDS (or global) access
+// rather than a flat one that would wait on both memory counters.
+template <bool STAGED>
+struct PlanView {
+    const int32_t* __restrict__ P;  // the plan in global memory
+    const int32_t* hot;             // the staged hot block (LDS)
+    int32_t hot0;                   // plan offset of the hot block
+    __device__ const int32_t* at(int32_t off) const {
+        if (STAGED) return hot + (off - hot0);
+        return P + off;
     }
+};
+
+// A program row record (small_lu.hpp): target, term count, diagonal
+// position, offset of terms 2.., then position and source of terms 0 and 1.
+struct RowRec {
+    int t, cnt, dp, toff, p0, s0, p1, s1;
+};
+
+__device__ inline RowRec load_row(const int32_t* rows, int r) {
+    const int4 a = *reinterpret_cast<const int4*>(rows + 8 * r);
+    const int4 b = *reinterpret_cast<const int4*>(rows + 8 * r + 4);
+    return RowRec{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 }
 
+// The rest of a row's subtraction chain (terms 2..), operand loads issued
+// eight terms ahead.
+template <int OP>
+__device__ inline double row_tail(double acc, const int32_t* tm, int cnt, const double* v, const double* y) {
+    int q = 0;
+    for (; q + 8 <= cnt; q += 8) {
+        int pos[8], src[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            pos[u] = tm[2 * (q + u)];
+            src[u] = tm[2 * (q + u) + 1];
+        }
+        double a[8], b[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            a[u] = v[pos[u]];
+            b[u] = y[src[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if ((OP & OP_SKIPZERO) && b[u] == 0.0) continue;
+            acc = acc - a[u] * b[u];
+        }
+    }
+    for (; q < cnt; ++q) {
+        const double b = y[tm[2 * q + 1]];
+        if ((OP & OP_SKIPZERO) && b == 0.0) continue;
+        acc = acc - v[tm[2 * q]] * b;
+    }
+    return acc;
+}
+
+// Row `rec` of a triangular program: acc = y[t] - sum v[pos] * y[src] in
+// term order (the reference's subtraction chain for target t), then the
+// diagonal operation.  yt, a0, a1, dv are y[t], v[p0], v[p1], v[dp],
+// loaded ahead by the caller.
+template <int OP, bool STAGED>
+__device__ inline void run_row(const PlanView<STAGED>& pv, const RowRec& rec, double yt, double a0,
+                               double a1, double dv, const double* v, double* y) {
+    // both inline sources are loaded together, unconditionally (unused
+    // slots name entry 0), so the row waits for one LDS round trip
+    const double b0 = y[rec.s0];
+    const double b1 = y[rec.s1];
+    double acc = yt;
+    if (rec.cnt >= 1 && !((OP & OP_SKIPZERO) && b0 == 0.0)) acc = acc - a0 * b0;
+    if (rec.cnt >= 2 && !((OP & OP_SKIPZERO) && b1 == 0.0)) acc = acc - a1 * b1;
+    if (rec.cnt > 2) acc = row_tail<OP>(acc, pv.at(rec.toff), rec.cnt - 2, v, y);
+    if (OP & OP_MUL_NZ) {
+        if (acc != 0.0) acc = acc * dv;
+    } else if (OP & OP_MUL_DIAG) {
+        acc = acc * dv;
+    }
+    y[rec.t] = acc;
+}
+
+// What a level needs of its first row, loaded ahead: the record and the
+// operands that are final before the level starts.
+struct RowAhead {
+    RowRec rec;
+    double yt, a0, a1, dv;
+};
+
+// Level l of a program: runs the level's rows, then fetches what level l+1
+// needs.  `cur` holds this level's ahead data, `nxt` receives the next
+// level's; callers alternate two RowAhead variables so that no prefetched
+// register is ever copied (a copy would wait for every outstanding load).
+template <int OP, bool STAGED>
+__device__ inline void program_level(const PlanView<STAGED>& pv, const int32_t* lev, const int32_t* rows,
+                                     int nl, int last, int l, int& r0, int& r1, int& r2,
+                                     const RowAhead& cur, RowAhead& nxt, const double* v, double* y,
+                                     int lane) {
+    const int r3 = (l + 3 <= nl) ? lev[l + 3] : r2;
+    nxt.rec = load_row(rows, min(r1 + lane, last));
+    if (r0 + lane < r1) run_row<OP, STAGED>(pv, cur.rec, cur.yt, cur.a0, cur.a1, cur.dv, v, y);
+    for (int r = r0 + lane + kW; r < r1; r += kW) {
+        const RowRec rec = load_row(rows, r);
+        run_row<OP, STAGED>(pv, rec, y[rec.t], v[rec.p0], v[rec.p1], v[rec.dp], v, y);
+    }
+    __syncthreads();
+    r0 = r1;
+    r1 = r2;
+    r2 = r3;
+    nxt.yt = y[nxt.rec.t];
+    nxt.a0 = v[nxt.rec.p0];
+    nxt.a1 = v[nxt.rec.p1];
+    nxt.dv = v[nxt.rec.dp];
+}
+
+template <int OP, bool STAGED>
+__device__ inline void run_program(const PlanView<STAGED>& pv, int prog, const double* v, double* y,
+                                   int lane) {
+    const int32_t* P = pv.P;
+    const int nl = P[SH_PROG + 3 * prog];
+    const int32_t* lev = pv.at(P[SH_PROG + 3 * prog + 1]);
+    const int32_t* rows = pv.at(P[SH_PROG + 3 * prog + 2]);
+    if (nl <= 0) return;
+    const int last = lev[nl] - 1;  // rows are never empty (one per target)
+    // Level bounds [r0, r1) and the next level's end r2.  Each level's
+    // first row record is fetched one level ahead at a clamped index (so the
+    // fetch does not wait for the bound that decides whether it is used),
+    // and right after a level's barrier its operands that are already final
+    // -- the factor values and the target's own entry -- are loaded, so a
+    // level waits for one LDS round trip, the source entries of its terms.
+    int r0 = lev[0], r1 = lev[1];
+    int r2 = (nl > 1) ? lev[2] : r1;
+    RowAhead A, B;
+    A.rec = load_row(rows, min(r0 + lane, last));
+    A.yt = y[A.rec.t];
+    A.a0 = v[A.rec.p0];
+    A.a1 = v[A.rec.p1];
+    A.dv = v[A.rec.dp];
+    int l = 0;
+    for (; l + 1 < nl; l += 2) {
+        program_level<OP, STAGED>(pv, lev, rows, nl, last, l, r0, r1, r2, A, B, v, y, lane);
+        program_level<OP, STAGED>(pv, lev, rows, nl, last, l + 1, r0, r1, r2, B, A, v, y, lane);
+    }
+    if (l < nl) program_level<OP, STAGED>(pv, lev, rows, nl, last, l, r0, r1, r2, A, B, v, y, lane);
+}
+
+// One task: x_t = x_t - (x_kj * (1 / x_kk)) * x_ik.  A forward-substitution
+// task (pivot stored as -(pos + 1)) is skipped when its y_k is zero, as the
+// forward loop skips a zero entry (solve.cpp:22-23).
+__device__ inline void run_task(const int4 tk, double* v) {
+    const bool fwd = tk.w < 0;
+    const int piv = fwd ? -tk.w - 1 : tk.w;
+    const double xkk = v[piv], xkj = v[tk.y], xik = v[tk.z], xt = v[tk.x];
+    const double u = xkj * (1.0 / xkk);
+    const double nv = xt - u * xik;
+    v[tk.x] = (fwd && xkj == 0.0) ? xt : nv;
+}
+
+// Level l of the refactor: the level's tasks, then the next level's first
+// task (alternated like RowAhead).
+__device__ inline void factor_level(const int32_t* lev, const int4* task, int nlev, int last, int l,
+                                    int& t0, int& t1, int& t2, const int4& cur, int4& nxt, double* v,
+                                    int lane) {
+    const int t3 = (l + 3 <= nlev) ? lev[l + 3] : t2;
+    nxt = task[min(t1 + lane, last)];
+    if (t0 + lane < t1) run_task(cur, v);
+    for (int t = t0 + lane + kW; t < t1; t += kW) run_task(task[t], v);
+    __syncthreads();
+    t0 = t1;
+    t1 = t2;
+    t2 = t3;
+}
+
+template <bool STAGED>
 __global__ __launch_bounds__(kW) void k_small_lu(const int32_t* __restrict__ words,
                                                 const int32_t* __restrict__ offs,
                                                 const SluInst* __restrict__ insts, int mode,
@@ -69,7 +218,7 @@ __global__ __launch_bounds__(kW) void k_small_lu(const int32_t* __restrict__ wor
                                                 const int32_t* __restrict__ kind,
                                                 int32_t* __restrict__ fail,
                                                 int32_t* __restrict__ fail_col, double* LU,
-                                                const double* rhs, double* x, int max_nlu) {
+                                                const double* rhs, double* x, int max_nlu, int max_n) {
     extern __shared__ double lds[];
     const int lane = threadIdx.x;
     const int ii = blockIdx.x;
@@ -78,18 +227,57 @@ __global__ __launch_bounds__(kW) void k_small_lu(const int32_t* __restrict__ wor
     const SluInst d = insts[ii];
     const int32_t* __restrict__ P = words + offs[d.plan];
     const int n = P[SH_N], nlu = P[SH_NLU];
-    double* v = lds;            // nlu factor values
-    double* y = lds + max_nlu;  // n: reciprocal pivots, then the solve vector
+    // LDS: the factor values, the vector right after them (the forward
+    // substitution's tasks address entry i as value nlu + i), the reciprocal
+    // pivots, then the staged hot block on a 16-byte boundary
+    double* v = lds;
+    double* y = lds + nlu;
+    double* dr = lds + max_nlu + max_n;
+    PlanView<STAGED> pv{P, nullptr, P[SH_HOT]};
+    if (STAGED) {
+        // the hot block, 16 bytes per lane per step (sections start on
+        // 4-word boundaries, so the block is whole int4s)
+        int32_t* hw = reinterpret_cast<int32_t*>(lds + ((max_nlu + 2 * max_n + 1) & ~1));
+        const int4* src = reinterpret_cast<const int4*>(P + pv.hot0);
+        int4* dst = reinterpret_cast<int4*>(hw);
+        const int n4 = (P[SH_HOTLEN] + 3) / 4;
+        int k = lane;
+        for (; k + 3 * kW < n4; k += 4 * kW) {
+            const int4 a = src[k], b = src[k + kW], c = src[k + 2 * kW], e = src[k + 3 * kW];
+            dst[k] = a;
+            dst[k + kW] = b;
+            dst[k + 2 * kW] = c;
+            dst[k + 3 * kW] = e;
+        }
+        for (; k < n4; k += kW) dst[k] = src[k];
+        pv.hot = hw;
+    }
     const int32_t* cptr = P + P[SH_CPTR];
     const int32_t* nU = P + P[SH_NU];
     const double gmin = gmin_inst ? gmin_inst[ii] : gmin_all;
+    const bool tr = (mode & SLU_SOLVE_T) != 0;
+    // a factor-and-solve launch folds the forward substitution into the factor
+    const bool fused = (mode & SLU_FACTOR) && (mode & SLU_SOLVE) && !tr;
+    const int32_t* in_perm = P + P[tr ? SH_CPERM : SH_RPERM];
+    const int32_t* out_perm = P + P[tr ? SH_RPERM : SH_CPERM];
     if (mode & SLU_FACTOR) {
         for (int p = lane; p < nlu; p += kW) v[p] = 0.0;
+        // the right-hand side rides along as column n (zeros: no solve asked)
+        for (int k = lane; k < n; k += kW) y[k] = fused ? rhs[d.xoff + in_perm[k]] : 0.0;
         __syncthreads();
         const int32_t* apos = P + P[SH_APOS];
         const int nnz = P[SH_NNZA];
         const double* a = A + d.aoff;
-        for (int p = lane; p < nnz; p += kW) v[apos[p]] = a[p];
+        int p = lane;
+        for (; p + 3 * kW < nnz; p += 4 * kW) {
+            const int q0 = apos[p], q1 = apos[p + kW], q2 = apos[p + 2 * kW], q3 = apos[p + 3 * kW];
+            const double w0 = a[p], w1 = a[p + kW], w2 = a[p + 2 * kW], w3 = a[p + 3 * kW];
+            v[q0] = w0;
+            v[q1] = w1;
+            v[q2] = w2;
+            v[q3] = w3;
+        }
+        for (; p < nnz; p += kW) v[apos[p]] = a[p];
         __syncthreads();
         if (gmin != 0.0) {
             const int32_t* gp = P + P[SH_GMIN];
@@ -99,28 +287,23 @@ __global__ __launch_bounds__(kW) void k_small_lu(const int32_t* __restrict__ wor
             }
             __syncthreads();
         }
-        // right-looking steps, level by level
+        // right-looking steps, level by level; as in run_program, each
+        // level's first task is fetched one level ahead at a clamped index
         const int nlev = P[SH_NLEV];
-        const int32_t* lev = P + P[SH_LEV];
-        const int4* task = reinterpret_cast<const int4*>(P + P[SH_TASK]);
-        int t_next = lev[0] + lane;
-        int4 tk = (t_next < lev[nlev]) ? task[t_next] : make_int4(0, 0, 0, 0);
-        for (int l = 0; l < nlev; ++l) {
-            const int t1 = lev[l + 1];
-            for (int t = lev[l] + lane; t < t1; t += kW) {
-                const int4 cur = (t == t_next) ? tk : task[t];
-                const double xkk = v[cur.w];
-                const double xkj = v[cur.y];
-                const double xik = v[cur.z];
-                const double xt = v[cur.x];
-                const double u = xkj * (1.0 / xkk);
-                v[cur.x] = xt - u * xik;
+        const int ntask = P[SH_NTASK];
+        if (nlev > 0 && ntask > 0) {
+            const int32_t* lev = pv.at(P[SH_LEV]);
+            const int4* task = reinterpret_cast<const int4*>(pv.at(P[SH_TASK]));
+            const int last = ntask - 1;
+            int t0 = lev[0], t1 = lev[1];
+            int t2 = (nlev > 1) ? lev[2] : t1;
+            int4 ta = task[min(t0 + lane, last)], tb;
+            int l = 0;
+            for (; l + 1 < nlev; l += 2) {
+                factor_level(lev, task, nlev, last, l, t0, t1, t2, ta, tb, v, lane);
+                factor_level(lev, task, nlev, last, l + 1, t0, t1, t2, tb, ta, v, lane);
             }
-            // the first task of the next level, fetched while this level's
-            // stores drain
-            t_next = t1 + lane;
-            if (t_next < lev[nlev]) tk = task[t_next];
-            __syncthreads();
+            if (l < nlev) factor_level(lev, task, nlev, last, l, t0, t1, t2, ta, tb, v, lane);
         }
         // checks (the first failing column, as the reference stops there)
         int bad = 0x7fffffff, code = FAIL_NONE;
@@ -139,15 +322,39 @@ __global__ __launch_bounds__(kW) void k_small_lu(const int32_t* __restrict__ wor
         }
         bad = wave_min_i32(bad);
         if (bad != 0x7fffffff) code = (kd == SLU_VERIFY) ? FAIL_REORDER : FAIL_ZERO_PIVOT;
-        // reciprocal pivots, then the stored U values and diagonal
-        for (int k = lane; k < n; k += kW) y[k] = 1.0 / v[cptr[k] + nU[k]];
+        // reciprocal pivots, then the stored U values and diagonal, and the
+        // forward substitution's closing scale by the stored 1/pivot
+        // (solve.cpp:24, zero entries left alone)
+        for (int k = lane; k < n; k += kW) {
+            const double r = 1.0 / v[cptr[k] + nU[k]];
+            dr[k] = r;
+            const double yk = y[k];
+            if (yk != 0.0) y[k] = yk * r;
+        }
         __syncthreads();
         const int32_t* rows = P + P[SH_ROWS];
         const int32_t* pcol = P + P[SH_PCOL];
-        for (int p = lane; p < nlu; p += kW) {
-            const int r = rows[p], c = pcol[p];
-            if (r < c) v[p] = v[p] * y[r];
-            else if (r == c) v[p] = y[r];
+        {
+            int p = lane;
+            for (; p + 3 * kW < nlu; p += 4 * kW) {
+                int r[4], c[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    r[u] = rows[p + u * kW];
+                    c[u] = pcol[p + u * kW];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int q = p + u * kW;
+                    if (r[u] < c[u]) v[q] = v[q] * dr[r[u]];
+                    else if (r[u] == c[u]) v[q] = dr[r[u]];
+                }
+            }
+            for (; p < nlu; p += kW) {
+                const int r = rows[p], c = pcol[p];
+                if (r < c) v[p] = v[p] * dr[r];
+                else if (r == c) v[p] = dr[r];
+            }
         }
         __syncthreads();
         double* lu = LU + d.luoff;
@@ -162,18 +369,22 @@ __global__ __launch_bounds__(kW) void k_small_lu(const int32_t* __restrict__ wor
         for (int p = lane; p < nlu; p += kW) v[p] = lu[p];
     }
     if (!(mode & (SLU_SOLVE | SLU_SOLVE_T))) return;
-    const bool tr = (mode & SLU_SOLVE_T) != 0;
-    const int32_t* in_perm = P + P[tr ? SH_CPERM : SH_RPERM];
-    const int32_t* out_perm = P + P[tr ? SH_RPERM : SH_CPERM];
-    __syncthreads();
-    for (int k = lane; k < n; k += kW) y[k] = rhs[d.xoff + in_perm[k]];
-    __syncthreads();
-    if (!tr) {
-        run_program<OP_SKIPZERO | OP_MUL_NZ>(P, 0, v, y, lane);
-        run_program<OP_PLAIN>(P, 1, v, y, lane);
+    if (fused) {
+        // forward substitution done with the factor: the backward one
+        run_program<OP_PLAIN, STAGED>(pv, 1, v, y, lane);
     } else {
-        run_program<OP_SKIPZERO>(P, 2, v, y, lane);
-        run_program<OP_MUL_DIAG>(P, 3, v, y, lane);
+        __syncthreads();
+        for (int k = lane; k < n; k += kW) y[k] = rhs[d.xoff + in_perm[k]];
+        __syncthreads();
+        if (!tr) {
+            run_program<OP_SKIPZERO | OP_MUL_NZ, false>(PlanView<false>{P, nullptr, 0}, 0, v, y, lane);
+            run_program<OP_PLAIN, STAGED>(pv, 1, v, y, lane);
+        } else {
+            // the transposed programs are not in the hot block
+            const PlanView<false> g{P, nullptr, 0};
+            run_program<OP_SKIPZERO, false>(g, 2, v, y, lane);
+            run_program<OP_MUL_DIAG, false>(g, 3, v, y, lane);
+        }
     }
     for (int k = lane; k < n; k += kW) x[d.xoff + out_perm[k]] = y[k];
 }
@@ -185,10 +396,21 @@ void launch_small_lu(const SmallLuPlans& plans, const SluInst* d_inst, int ninst
                      int32_t* fail_col, double* LU, const double* rhs, double* x, hipStream_t s,
                      const double* gmin_inst) {
     if (ninst <= 0) return;
-    const int max_nlu = plans.max_nlu();
-    const size_t lds = sizeof(double) * ((size_t)max_nlu + plans.max_n());
-    hipLaunchKernelGGL(k_small_lu, dim3(ninst), dim3(kW), lds, s, plans.d_words(), plans.d_offsets(),
-                       d_inst, mode, A, diag_gmin, gmin_inst, kind, fail, fail_col, LU, rhs, x, max_nlu);
+    const int max_nlu = plans.max_nlu(), max_n = plans.max_n();
+    // the hot block starts on a 16-byte boundary after the values, the
+    // vector and the reciprocal pivots
+    const size_t base = sizeof(double) * (((size_t)max_nlu + 2 * (size_t)max_n + 1) & ~(size_t)1);
+    const size_t hot = sizeof(int32_t) * (((size_t)plans.max_hot() + 3) / 4 * 4);
+    const int stage = (base + hot <= kLdsLimit) ? 1 : 0;
+    const size_t lds = base + (stage ? hot : 0);
+    if (stage)
+        hipLaunchKernelGGL(k_small_lu<true>, dim3(ninst), dim3(kW), lds, s, plans.d_words(),
+                           plans.d_offsets(), d_inst, mode, A, diag_gmin, gmin_inst, kind, fail, fail_col,
+                           LU, rhs, x, max_nlu, max_n);
+    else
+        hipLaunchKernelGGL(k_small_lu<false>, dim3(ninst), dim3(kW), lds, s, plans.d_words(),
+                           plans.d_offsets(), d_inst, mode, A, diag_gmin, gmin_inst, kind, fail, fail_col,
+                           LU, rhs, x, max_nlu, max_n);
     NSA_HIP(hipGetLastError());
 }
 

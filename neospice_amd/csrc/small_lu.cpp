@@ -15,12 +15,19 @@ SmallLuPlans::~SmallLuPlans() {
 namespace {
 
 // Right-looking step tasks of a Sparse 1.3 structure, grouped in levels:
-// a step goes one level after the last step that wrote any entry it reads
+// a task goes one level after the last task that wrote any entry it reads
 // or writes.  Reads of a step are final entries of row and column k, which
 // no later step writes, so read-after-write is the only hazard besides the
 // write order of a target (same rule).
-void build_tasks(const LuStructure& s, std::vector<int32_t>& lev_ptr, std::vector<int32_t>& tasks) {
-    const int n = s.n;
+//
+// With the forward substitution folded in, the right-hand side is one more
+// column: vector entry i sits at position nlu + i, and step k adds, for
+// every L(i,k), the task y_i -= (y_k * (1 / x_kk)) * L(i,k) -- solve.cpp:21-33
+// applies exactly these terms, in ascending k per target, and skips step k
+// when y_k == 0.  Such a task carries its pivot position as -(pos + 1).
+void build_tasks(const LuStructure& s, std::vector<int32_t>& lev_ptr, std::vector<int32_t>& tasks,
+                 bool with_rhs) {
+    const int n = s.n, nlu = s.nlu();
     std::vector<std::vector<int32_t>> lpos(n), upos(n);  // L(:,k) / U(k,:) positions
     std::map<std::pair<int, int>, int32_t> pos_of;        // (row, col) -> position
     for (int k = 0; k < n; ++k)
@@ -30,31 +37,42 @@ void build_tasks(const LuStructure& s, std::vector<int32_t>& lev_ptr, std::vecto
             if (r > k) lpos[k].push_back(p);
             else if (r < k) upos[r].push_back(p);
         }
-    std::vector<int> wlev(s.nlu(), -1);
+    std::vector<int> wlev(nlu + n, -1);
     std::vector<std::vector<int32_t>> by_level;
-    for (int k = 0; k < n; ++k) {
-        if (lpos[k].empty() || upos[k].empty()) continue;
-        const int32_t piv = s.dpos(k);
-        int lev = std::max(wlev[piv], -1);
-        for (int32_t p : lpos[k]) lev = std::max(lev, wlev[p]);
-        for (int32_t p : upos[k]) lev = std::max(lev, wlev[p]);
-        std::vector<int32_t> step;
-        for (int32_t pu : upos[k]) {
-            // column j of U(k, j): the column whose segment holds position pu
-            const int j = (int)(std::upper_bound(s.cptr.begin(), s.cptr.end(), pu) - s.cptr.begin()) - 1;
-            for (int32_t pl : lpos[k]) {
-                const int i = s.rows[pl];
-                auto it = pos_of.find({i, j});
-                if (it == pos_of.end())
-                    throw Error(NSA_ERR_RUNTIME, "small LU plan: update outside the fill pattern");
-                lev = std::max(lev, wlev[it->second]);
-                step.insert(step.end(), {it->second, pu, pl, piv});
-            }
-        }
-        ++lev;
+    auto place = [&](int lev, const std::vector<int32_t>& step) {
         for (size_t t = 0; t < step.size(); t += 4) wlev[step[t]] = lev;
         if ((int)by_level.size() <= lev) by_level.resize(lev + 1);
         by_level[lev].insert(by_level[lev].end(), step.begin(), step.end());
+    };
+    for (int k = 0; k < n; ++k) {
+        const int32_t piv = s.dpos(k);
+        if (!lpos[k].empty() && !upos[k].empty()) {
+            int lev = wlev[piv];
+            for (int32_t p : lpos[k]) lev = std::max(lev, wlev[p]);
+            for (int32_t p : upos[k]) lev = std::max(lev, wlev[p]);
+            std::vector<int32_t> step;
+            for (int32_t pu : upos[k]) {
+                // column j of U(k, j): the column whose segment holds position pu
+                const int j = (int)(std::upper_bound(s.cptr.begin(), s.cptr.end(), pu) - s.cptr.begin()) - 1;
+                for (int32_t pl : lpos[k]) {
+                    const int i = s.rows[pl];
+                    auto it = pos_of.find({i, j});
+                    if (it == pos_of.end())
+                        throw Error(NSA_ERR_RUNTIME, "small LU plan: update outside the fill pattern");
+                    lev = std::max(lev, wlev[it->second]);
+                    step.insert(step.end(), {it->second, pu, pl, piv});
+                }
+            }
+            place(lev + 1, step);
+        }
+        if (with_rhs && !lpos[k].empty()) {
+            const int32_t yk = nlu + k;
+            int lev = std::max(wlev[piv], wlev[yk]);
+            for (int32_t p : lpos[k]) lev = std::max({lev, wlev[p], wlev[nlu + s.rows[p]]});
+            std::vector<int32_t> step;
+            for (int32_t pl : lpos[k]) step.insert(step.end(), {nlu + s.rows[pl], yk, pl, -(piv + 1)});
+            place(lev + 1, step);
+        }
     }
     lev_ptr.assign(1, 0);
     tasks.clear();
@@ -68,32 +86,43 @@ void align4(std::vector<int32_t>& w) {
     while (w.size() % 4) w.push_back(0);
 }
 
-// A triangular program as level-ordered row records and terms.
+// A triangular program as level-ordered row records and terms.  A row
+// record is 8 words: target, term count, diagonal position (or -1), offset
+// of terms 2.. (plan-relative), then the LU position and source of terms 0
+// and 1 (the rows of circuit matrices are mostly that short, so a level
+// needs no term fetch before its operand loads).
 void append_program(const TriProgram& t, std::vector<int32_t>& w, int32_t base, int32_t* hdr3) {
     const int nl = t.nlevels();
     hdr3[0] = nl;
     align4(w);
     hdr3[1] = (int32_t)w.size() - base;
     for (int l = 0; l <= nl; ++l) w.push_back(t.level_ptr[l]);
-    // rows (4 words each), then terms (2 words each)
     align4(w);
     const int nrows = (int)t.order.size();
     const int32_t rows_at = (int32_t)w.size();
     hdr3[2] = rows_at - base;
-    w.resize(w.size() + 4 * (size_t)nrows);
-    int32_t term_base = (int32_t)w.size() - base;
+    w.resize(w.size() + 8 * (size_t)nrows);
+    const int32_t term_base = (int32_t)w.size() - base;
     std::vector<int32_t> terms;
     for (int r = 0; r < nrows; ++r) {
         const int tg = t.order[r];
         const int a = t.term_ptr[tg], b = t.term_ptr[tg + 1];
-        int32_t* rec = &w[rows_at + 4 * (size_t)r];
+        int32_t* rec = &w[rows_at + 8 * (size_t)r];
         rec[0] = tg;
-        rec[1] = term_base + (int32_t)terms.size();  // relative word offset of the first term
-        rec[2] = b - a;
-        rec[3] = t.dpos.empty() ? -1 : t.dpos[tg];
+        rec[1] = b - a;
+        rec[2] = t.dpos.empty() ? -1 : t.dpos[tg];
+        rec[3] = term_base + (int32_t)terms.size();
+        // unused inline slots point at position 0 / source 0 (always valid)
+        rec[4] = rec[5] = rec[6] = rec[7] = 0;
+        if (rec[2] < 0) rec[2] = 0;
         for (int q = a; q < b; ++q) {
-            terms.push_back(t.term_pos[q]);
-            terms.push_back(t.term_src[q]);
+            if (q - a < 2) {
+                rec[4 + 2 * (q - a)] = t.term_pos[q];
+                rec[5 + 2 * (q - a)] = t.term_src[q];
+            } else {
+                terms.push_back(t.term_pos[q]);
+                terms.push_back(t.term_src[q]);
+            }
         }
     }
     w.insert(w.end(), terms.begin(), terms.end());
@@ -120,14 +149,8 @@ int SmallLuPlans::add(const LuStructure& s) {
     H(SH_N) = n;
     H(SH_NLU) = s.nlu();
     H(SH_NNZA) = s.nnz_a;
-    std::vector<int32_t> lev_ptr, tasks;
-    build_tasks(s, lev_ptr, tasks);
-    H(SH_NLEV) = (int32_t)lev_ptr.size() - 1;
-    H(SH_NTASK) = (int32_t)(tasks.size() / 4);
     put(SH_APOS, s.a_pos);
     put(SH_GMIN, s.gmin_pos);
-    put(SH_LEV, lev_ptr);
-    put(SH_TASK, tasks);
     put(SH_CPTR, s.cptr);
     put(SH_NU, s.nU);
     put(SH_ROWS, s.rows);
@@ -140,18 +163,39 @@ int SmallLuPlans::add(const LuStructure& s) {
     TriProgram fwd, bwd, tfwd, tbwd;
     build_solve_programs(s, fwd, bwd, tfwd, tbwd, 0);
     int32_t h3[3];
-    const TriProgram* progs[4] = {&fwd, &bwd, &tfwd, &tbwd};
-    for (int p = 0; p < 4; ++p) {
-        append_program(*progs[p], words_, base, h3);
-        H(SH_PROG + 3 * p) = h3[0];
-        H(SH_PROG + 3 * p + 1) = h3[1];
-        H(SH_PROG + 3 * p + 2) = h3[2];
+    // the forward program (solve-only launches) and the transposed ones
+    const TriProgram* cold[3] = {&fwd, &tfwd, &tbwd};
+    const int cold_id[3] = {0, 2, 3};
+    for (int c = 0; c < 3; ++c) {
+        append_program(*cold[c], words_, base, h3);
+        H(SH_PROG + 3 * cold_id[c]) = h3[0];
+        H(SH_PROG + 3 * cold_id[c] + 1) = h3[1];
+        H(SH_PROG + 3 * cold_id[c] + 2) = h3[2];
     }
+    // the hot block: what the level loops of a factor-and-solve launch read
+    // (the task levels with the forward substitution folded in, and the
+    // backward program), contiguous so that a launch can stage it in LDS
+    std::vector<int32_t> lev_ptr, tasks;
+    build_tasks(s, lev_ptr, tasks, true);
+    H(SH_NLEV) = (int32_t)lev_ptr.size() - 1;
+    H(SH_NTASK) = (int32_t)(tasks.size() / 4);
+    align4(words_);
+    const int32_t hot0 = (int32_t)words_.size() - base;
+    put(SH_LEV, lev_ptr);
+    put(SH_TASK, tasks);
+    append_program(bwd, words_, base, h3);
+    H(SH_PROG + 3) = h3[0];
+    H(SH_PROG + 4) = h3[1];
+    H(SH_PROG + 5) = h3[2];
+    align4(words_);
+    H(SH_HOT) = hot0;
+    H(SH_HOTLEN) = (int32_t)words_.size() - base - hot0;
     st_.push_back(s);
     lev_.push_back(H(SH_NLEV));
     slev_.push_back(fwd.nlevels() + bwd.nlevels());
     max_nlu_ = std::max(max_nlu_, s.nlu());
     max_n_ = std::max(max_n_, n);
+    max_hot_ = std::max(max_hot_, (int)words_[base + SH_HOTLEN]);
     return (int)st_.size() - 1;
 }
 

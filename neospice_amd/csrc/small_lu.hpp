@@ -14,8 +14,10 @@
 // same ascending-k order, with the same two roundings per term, so the
 // factor is bit-identical; steps whose entries do not overlap share a level.
 // Sparse 1.3 stores U(k,j) = x_kj * d_k and the reciprocal pivot, which a
-// closing pass writes.  The solves are the level-ordered row gathers of
-// build_solve_programs (solve.cpp:6-47, 102-141).
+// closing pass writes.  A factor-and-solve launch folds the forward
+// substitution (solve.cpp:21-33) into those levels as one more column; the
+// backward substitution and the solve-only launches run the level-ordered
+// row gathers of build_solve_programs (solve.cpp:6-47, 102-141).
 //
 // Only FORM_S13 structures are planned here: make_solver picks AMD-LU only
 // for linear circuits of at least 256 unknowns (make_solver.cpp:156-176),
@@ -46,19 +48,23 @@ enum SluHdr : int32_t {
     SH_APOS,    // nnz_a: position receiving A slot p
     SH_GMIN,    // n: position receiving diag_gmin, or -1
     SH_LEV,     // nlev+1 task offsets (relative to SH_TASK, in tasks)
-    SH_TASK,    // tasks, 4 words each: target, U(k,j), L(i,k), pivot (k,k) positions
+    SH_TASK,    // tasks, 4 words each: target, U(k,j), L(i,k), pivot (k,k) positions;
+                // forward-substitution tasks: y_i, y_k (at nlu + index), L(i,k), -(pivot + 1)
     SH_CPTR,    // n+1 column segments
     SH_NU,      // n: U entries per column
     SH_ROWS,    // nlu: pivot-space row of each position
     SH_PCOL,    // nlu: pivot-space column of each position
     SH_RPERM, SH_CPERM,
     SH_PROG,    // 4 programs (fwd, bwd, tfwd, tbwd), SH_PROG + 3*p: nlev, levptr off, rows off
-    SH_WORDS = SH_PROG + 12,
+    SH_HOT = SH_PROG + 12,  // first word and length of the block a factor-and-solve
+    SH_HOTLEN,              // launch reads in its level loops (LEV, TASK, bwd)
+    SH_WORDS,
 };
-// Program rows: 4 words each (target, first term, term count, diagonal
-// position or -1); terms: 2 words each (LU position, source).  The program
-// op is fixed by the form: fwd SKIPZERO|MUL_NZ, bwd PLAIN, tfwd SKIPZERO,
-// tbwd MUL_DIAG.
+// Program rows: 8 words each (target, term count, diagonal position (0
+// when the program has no diagonal operation), offset of the terms after
+// the first two, then LU position and source of terms 0 and 1); terms: 2
+// words each (LU position, source).  The program op is fixed by the form:
+// fwd SKIPZERO|MUL_NZ, bwd PLAIN, tfwd SKIPZERO, tbwd MUL_DIAG.
 
 // Kinds per instance (the same codes the interleaved kernels use).
 enum SluKind : int32_t { SLU_REFACTOR = 0, SLU_VERIFY = 1, SLU_TRUSTED = 2, SLU_SKIP = 3 };
@@ -82,6 +88,7 @@ public:
     int count() const { return (int)st_.size(); }
     int max_nlu() const { return max_nlu_; }
     int max_n() const { return max_n_; }
+    int max_hot() const { return max_hot_; }
     // Refactor levels and the solve programs' levels of a plan (statistics).
     int levels(int plan) const { return lev_[plan]; }
     int solve_levels(int plan) const { return slev_[plan]; }
@@ -98,7 +105,7 @@ private:
     std::vector<LuStructure> st_;
     std::vector<int32_t> words_, off_;
     std::vector<int> lev_, slev_;
-    int max_nlu_ = 0, max_n_ = 0;
+    int max_nlu_ = 0, max_n_ = 0, max_hot_ = 0;
     int32_t* d_words_ = nullptr;
     int32_t* d_off_ = nullptr;
     size_t d_words_cap_ = 0, d_off_cap_ = 0, synced_words_ = 0, synced_plans_ = 0;
