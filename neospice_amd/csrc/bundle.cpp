@@ -396,6 +396,22 @@ void Bundle::build_tables() {
     csr(rows_u, p, ix);
     d_rptr_u_ = dupload(allocs_, p);
     d_rsites_u_ = dupload(allocs_, ix);
+    // short / long row lists of the two gathers (a row is long when either
+    // of its site lists exceeds 32 sites)
+    {
+        std::vector<int32_t> ms, ml, rs_, rl;
+        for (int r = 0; r < tot_nnz_; ++r) (slot_sites[r].size() > 32 ? ml : ms).push_back(r);
+        for (int r = 0; r < tot_n_; ++r)
+            (rows_d[r].size() > 32 || rows_u[r].size() > 32 ? rl : rs_).push_back(r);
+        gm_.nshort = (int)ms.size();
+        gm_.nlong = (int)ml.size();
+        gm_.short_rows = dupload(allocs_, ms);
+        gm_.long_rows = dupload(allocs_, ml);
+        gr_.nshort = (int)rs_.size();
+        gr_.nlong = (int)rl.size();
+        gr_.short_rows = dupload(allocs_, rs_);
+        gr_.long_rows = dupload(allocs_, rl);
+    }
     // ---- device arrays
     d_cm_ = dalloc<double>(allocs_, n_msites_);
     d_cr_ = dalloc<double>(allocs_, n_rsites_);
@@ -531,9 +547,12 @@ void Bundle::launch_loads() {
     launch_bsim4(b4_, d_ctx_, d_x_, s0, s1, s2, d_done_, d_cm_, d_cr_, d_devnc_, tol_, b4_lanes_,
                  stream_);
     if (timing_ && b4_.count) NSA_HIP(hipEventRecord(ev_[2], stream_));
-    launch_gather(tot_nnz_, d_mptr_, d_msites_, nullptr, nullptr, d_cm_, nullptr, d_A_, 1, stream_);
-    launch_gather(tot_n_, d_rptr_d_, d_rsites_d_, d_rptr_u_, d_rsites_u_, d_cr_, d_cr_, d_rhs_, 1,
-                  stream_);
+    GatherPacked m = gm_, r = gr_;
+    m.ptr = d_mptr_; m.sites = d_msites_; m.contrib = d_cm_; m.ptr2 = nullptr; m.sites2 = nullptr;
+    m.contrib2 = nullptr; m.out = d_A_;
+    r.ptr = d_rptr_d_; r.sites = d_rsites_d_; r.contrib = d_cr_; r.ptr2 = d_rptr_u_;
+    r.sites2 = d_rsites_u_; r.contrib2 = d_cr_; r.out = d_rhs_;
+    launch_gather_packed(m, r, stream_);
 }
 
 void Bundle::launch_lu(const int32_t* d_kind, int mode) {
@@ -1219,13 +1238,10 @@ int Bundle::tran_advance(double t_until) {
         // (evaluate_lte, transient.cpp:469-485; order promotion :946-958),
         // and the ringing test, for the instances that attempted a step
         launch_fill_u64(d_tmin_, big_bits, 2 * NI_, stream_);
-        launch_trunc(trunc1_, d_ctx_, d_s_[0], d_s_[1], d_s_[2], d_s_[3], 1, to, nullptr, d_tmin_, stream_, 0);
+        launch_trunc(trunc1_, d_ctx_, d_s_[0], d_s_[1], d_s_[2], d_s_[3], 1, to, nullptr, d_tmin_, stream_, 0,
+                     d_tmin_ + NI_);
         launch_trunc(trunc4_, d_ctx_, d_s_[0], d_s_[1], d_s_[2], d_s_[3], b4_lanes_, to, nullptr, d_tmin_,
-                     stream_, 0);
-        launch_trunc(trunc1_, d_ctx_, d_s_[0], d_s_[1], d_s_[2], d_s_[3], 1, to, nullptr, d_tmin_ + NI_,
-                     stream_, 2);
-        launch_trunc(trunc4_, d_ctx_, d_s_[0], d_s_[1], d_s_[2], d_s_[3], b4_lanes_, to, nullptr,
-                     d_tmin_ + NI_, stream_, 2);
+                     stream_, 0, d_tmin_ + NI_);
         std::memcpy(h_acc_pin_ + 2 * NI_, att.data(), sizeof(int32_t) * NI_);
         NSA_HIP(hipMemcpyAsync(d_att_, h_acc_pin_ + 2 * NI_, sizeof(int32_t) * NI_, hipMemcpyHostToDevice,
                                stream_));

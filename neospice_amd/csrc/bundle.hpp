@@ -201,6 +201,7 @@ private:
     int32_t *d_mptr_ = nullptr, *d_msites_ = nullptr, *d_rptr_d_ = nullptr, *d_rsites_d_ = nullptr,
             *d_rptr_u_ = nullptr, *d_rsites_u_ = nullptr;
     DevTol tol_{};
+    GatherPacked gm_{}, gr_{};  // row lists of the matrix and rhs gathers
     std::vector<double> h_pulse_;  // 7 per V source, for source-default resolution
     // LU
     SmallLuPlans plans_;

@@ -233,6 +233,77 @@ __global__ void k_gather(int nrows, const int32_t* __restrict__ ptr,
     out[gid] = acc;
 }
 
+// Gather of a packed batch (B = 1): rows of at most kLongRow sites take one
+// lane each; longer rows (a supply node's diagonal collects a stamp from
+// every device on the rail) take a wave each, whose lanes load the row's
+// site values into LDS with one round trip before one lane adds them in
+// site order -- the same additions from 0.0 as the lane-per-row form.
+constexpr int kLongRow = 32;
+constexpr int kLongChunk = 1024;   // values a long-row wave stages per pass
+
+__device__ inline double gsum_lane(const int32_t* __restrict__ ptr, const int32_t* __restrict__ sites,
+                                   const double* __restrict__ contrib, int r) {
+    const int p0 = ptr[r], p1 = ptr[r + 1];
+    if (p1 - p0 > 4) return sum_chunks<8>(sites, contrib, p0, p1, 0, 1);
+    double acc = 0.0;
+    for (int p = p0; p < p1; ++p) acc += contrib[sites[p]];
+    return acc;
+}
+
+__device__ inline double gsum_wave(const int32_t* __restrict__ ptr, const int32_t* __restrict__ sites,
+                                   const double* __restrict__ contrib, int r, double* buf, int lane) {
+    const int p0 = ptr[r], p1 = ptr[r + 1];
+    double acc = 0.0;
+    for (int c0 = p0; c0 < p1; c0 += kLongChunk) {
+        const int c1 = min(p1, c0 + kLongChunk);
+        for (int p = c0 + lane; p < c1; p += kWaveSize) buf[p - c0] = contrib[sites[p]];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (lane == 0)
+            for (int p = 0; p < c1 - c0; ++p) acc += buf[p];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(TPB) void k_gather_packed(GatherPacked m, GatherPacked r) {
+    __shared__ double buf[TPB / kWaveSize][kLongChunk];
+    const int lane = threadIdx.x & (kWaveSize - 1), wave = threadIdx.x / kWaveSize;
+    const long short_blocks_m = ((long)m.nshort + TPB - 1) / TPB;
+    const long short_blocks_r = ((long)r.nshort + TPB - 1) / TPB;
+    long b = blockIdx.x;
+    // blocks: short rows of m, short rows of r, then long rows of m and r
+    // (TPB / 64 rows per block)
+    if (b < short_blocks_m + short_blocks_r) {
+        const GatherPacked& g = (b < short_blocks_m) ? m : r;
+        if (b >= short_blocks_m) b -= short_blocks_m;
+        const long k = b * TPB + threadIdx.x;
+        if (k >= g.nshort) return;
+        const int row = g.short_rows[k];
+        double acc = gsum_lane(g.ptr, g.sites, g.contrib, row);
+        if (g.ptr2) acc += gsum_lane(g.ptr2, g.sites2, g.contrib2, row);
+        g.out[row] = acc;
+        return;
+    }
+    b -= short_blocks_m + short_blocks_r;
+    const long k = b * (TPB / kWaveSize) + wave;
+    const GatherPacked& g = (k < m.nlong) ? m : r;
+    const long kk = (k < m.nlong) ? k : k - m.nlong;
+    if (kk >= g.nlong) return;
+    const int row = g.long_rows[kk];
+    // either list of a row may be long; each runs the form its length needs
+    double acc = (g.ptr[row + 1] - g.ptr[row] > kLongRow) ? gsum_wave(g.ptr, g.sites, g.contrib, row, buf[wave], lane)
+                                                         : gsum_lane(g.ptr, g.sites, g.contrib, row);
+    if (g.ptr2) {
+        const double a2 = (g.ptr2[row + 1] - g.ptr2[row] > kLongRow)
+                              ? gsum_wave(g.ptr2, g.sites2, g.contrib2, row, buf[wave], lane)
+                              : gsum_lane(g.ptr2, g.sites2, g.contrib2, row);
+        acc += a2;
+    }
+    if (lane == 0) g.out[row] = acc;
+}
+
 // Newton convergence test (newton.cpp:267-292): any failing unknown marks
 // its instance non-converged.
 __global__ void k_conv(int n, int num_nodes, const double* __restrict__ xn,
@@ -310,36 +381,73 @@ __device__ __forceinline__ double terr(int q, const double* s0, const double* s1
     return (del < dt) ? del : dt;
 }
 
+// The minimum of v over the lanes of this wave that share ci, folded into
+// dt_min[ci] with one atomic per distinct ci (devices of one instance sit
+// in consecutive lanes, so a wave usually holds one or two instances).
+// Every lane of the wave must call it (all 64 active); lanes with ci < 0
+// take part in the reductions and update nothing.
+__device__ inline void seg_atomic_min(unsigned long long* dt_min, int ci, unsigned long long v) {
+    unsigned long long todo = __ballot(1);
+    const int lane = __lane_id();
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int lci = __shfl(ci, leader);
+        const bool mine = (ci == lci);
+        unsigned long long m = mine ? v : ~0ull;
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned lo = __shfl_xor((unsigned)(m & 0xffffffffu), o);
+            const unsigned hi = __shfl_xor((unsigned)(m >> 32), o);
+            const unsigned long long w = ((unsigned long long)hi << 32) | lo;
+            m = (w < m) ? w : m;
+        }
+        if (lane == leader && lci >= 0) atomicMin(&dt_min[lci], m);
+        todo &= ~__ballot(mine);
+    }
+}
+
 __global__ void k_trunc(TruncTable t, const DevCtx* __restrict__ ctx, const double* __restrict__ s0,
                         const double* __restrict__ s1, const double* __restrict__ s2,
                         const double* __restrict__ s3, int B, TruncOpts o, double* __restrict__ dt_dev,
-                        unsigned long long* __restrict__ dt_min, int order_override) {
+                        unsigned long long* __restrict__ dt_min, int order_override,
+                        unsigned long long* __restrict__ dt_min2) {
+    // lanes past the end stay for the wave-wide reductions, with no instance
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= t.ndev * B) return;
-    const int inst = idx % B, dev = idx / B;
-    const int ci = t.inst_of ? t.inst_of[idx] : inst;
-    DevCtx c = ctx[ci];
+    const bool valid = idx < t.ndev * B;
+    const int inst = valid ? idx % B : 0, dev = valid ? idx / B : 0;
+    const int ci = !valid ? -1 : (t.inst_of ? t.inst_of[idx] : inst);
+    DevCtx c = ctx[valid ? ci : 0];
     if (order_override > 0) c.order = order_override;
-    double dt = 1e30;
-    // compute_trunc's guard; orders above 2 would need more than four states
-    if (c.order >= 1 && c.order <= 2 && c.delta > 0.0) {
-        const int base = t.base[dev];
-        if (t.kind[dev] == 0) {
-            dt = terr(base, s0, s1, s2, s3, B, inst, c, o, dt);
-        } else if (t.kind[dev] == 2) {
-            dt = terr(base + 3, s0, s1, s2, s3, B, inst, c, o, dt);   // DIOcapCharge
-        } else if (t.kind[dev] == 3) {
-            // BSIM4v7 qb, qg, qd (bsim4v7_device.cpp:1015-1017)
-            const int qoff[3] = {11, 13, 15};
-            for (int k = 0; k < 3; ++k) dt = terr(base + qoff[k], s0, s1, s2, s3, B, inst, c, o, dt);
-        } else {
-            const int qoff[5] = {5, 8, 11, 13, 15};   // qgs, qgd, qgb, qbd, qbs
-            for (int k = 0; k < 5; ++k) dt = terr(base + qoff[k], s0, s1, s2, s3, B, inst, c, o, dt);
+    // the device's proposal at context c (compute_trunc's body)
+    auto device_dt = [&](const DevCtx& cx) {
+        double dt = 1e30;
+        // compute_trunc's guard; orders above 2 would need more than four states
+        if (cx.order >= 1 && cx.order <= 2 && cx.delta > 0.0) {
+            const int base = t.base[dev];
+            if (t.kind[dev] == 0) {
+                dt = terr(base, s0, s1, s2, s3, B, inst, cx, o, dt);
+            } else if (t.kind[dev] == 2) {
+                dt = terr(base + 3, s0, s1, s2, s3, B, inst, cx, o, dt);   // DIOcapCharge
+            } else if (t.kind[dev] == 3) {
+                // BSIM4v7 qb, qg, qd (bsim4v7_device.cpp:1015-1017)
+                const int qoff[3] = {11, 13, 15};
+                for (int k = 0; k < 3; ++k) dt = terr(base + qoff[k], s0, s1, s2, s3, B, inst, cx, o, dt);
+            } else {
+                const int qoff[5] = {5, 8, 11, 13, 15};   // qgs, qgd, qgb, qbd, qbs
+                for (int k = 0; k < 5; ++k) dt = terr(base + qoff[k], s0, s1, s2, s3, B, inst, cx, o, dt);
+            }
         }
-    }
-    if (dt_dev) dt_dev[idx] = dt;
+        return dt;
+    };
+    const double dt = valid ? device_dt(c) : 1e30;
+    if (dt_dev && valid) dt_dev[idx] = dt;
     // dt >= 0: the bit patterns of non-negative doubles order like the values
-    atomicMin(&dt_min[ci], (unsigned long long)__double_as_longlong(dt));
+    seg_atomic_min(dt_min, ci, (unsigned long long)__double_as_longlong(dt));
+    if (dt_min2) {
+        // the same states at order 2 (the order-promotion test)
+        c.order = 2;
+        const double dt2 = valid ? device_dt(c) : 1e30;
+        seg_atomic_min(dt_min2, ci, (unsigned long long)__double_as_longlong(dt2));
+    }
 }
 
 __global__ void k_fill_u64(unsigned long long* p, unsigned long long v, int count) {
@@ -351,11 +459,12 @@ __global__ void k_fill_u64(unsigned long long* p, unsigned long long v, int coun
 
 void launch_trunc(const TruncTable& t, const DevCtx* ctx, const double* s0, const double* s1,
                   const double* s2, const double* s3, int B, const TruncOpts& o, double* dt_dev,
-                  unsigned long long* dt_min, hipStream_t s, int order_override) {
+                  unsigned long long* dt_min, hipStream_t s, int order_override,
+                  unsigned long long* dt_min2) {
     const int total = t.ndev * B;
     if (total <= 0) return;
     hipLaunchKernelGGL(k_trunc, dim3((total + 255) / 256), dim3(256), 0, s, t, ctx, s0, s1, s2, s3, B, o,
-                       dt_dev, dt_min, order_override);
+                       dt_dev, dt_min, order_override, dt_min2);
 }
 
 void launch_fill_u64(unsigned long long* p, unsigned long long v, int count, hipStream_t s) {
@@ -404,6 +513,13 @@ void launch_gather(int nrows, const int32_t* ptr, const int32_t* sites, const in
     if (!nrows) return;
     hipLaunchKernelGGL(k_gather, dim3(blocks_for((long)nrows * B)), dim3(TPB), 0, s, nrows, ptr,
                        sites, ptr2, sites2, contrib, contrib2, out, B);
+}
+
+void launch_gather_packed(const GatherPacked& m, const GatherPacked& r, hipStream_t s) {
+    const long blocks = ((long)m.nshort + TPB - 1) / TPB + ((long)r.nshort + TPB - 1) / TPB +
+                        ((long)m.nlong + r.nlong + TPB / kWaveSize - 1) / (TPB / kWaveSize);
+    if (blocks <= 0) return;
+    hipLaunchKernelGGL(k_gather_packed, dim3((unsigned)blocks), dim3(TPB), 0, s, m, r);
 }
 
 void launch_conv(int n, int num_nodes, const double* xnew, const double* xold, double reltol,

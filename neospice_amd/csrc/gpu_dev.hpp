@@ -169,6 +169,24 @@ void launch_or_i32(int32_t* dst, const int32_t* src, int count, hipStream_t s);
 void launch_gather(int nrows, const int32_t* ptr, const int32_t* sites, const int32_t* ptr2,
                    const int32_t* sites2, const double* contrib, const double* contrib2,
                    double* out, int B, hipStream_t s);
+// The gather of a packed batch of circuits (B = 1): out[row] = the sum of
+// the row's sites in ptr/sites (plus, after it, those in ptr2/sites2) from
+// 0.0 in site order.  Rows are split into short ones (a lane each) and long
+// ones (a wave each); both lists hold row indices.
+struct GatherPacked {
+    int nshort, nlong;
+    const int32_t* short_rows;
+    const int32_t* long_rows;
+    const int32_t* ptr;
+    const int32_t* sites;
+    const double* contrib;
+    const int32_t* ptr2;
+    const int32_t* sites2;
+    const double* contrib2;
+    double* out;
+};
+// One launch for two gathers (the matrix and the rhs).
+void launch_gather_packed(const GatherPacked& m, const GatherPacked& r, hipStream_t s);
 void launch_conv(int n, int num_nodes, const double* xnew, const double* xold, double reltol,
                  double vntol, double abstol, const int32_t* done, int32_t* nonconv, int B,
                  hipStream_t s);
@@ -194,10 +212,13 @@ struct TruncOpts {
 // the bits of non-negative doubles, filled with those of 1e30 first) their
 // minimum per instance.
 // order_override > 0 evaluates the devices at that integration order instead
-// of the context's (the order-promotion test, transient.cpp:946-958).
+// of the context's; dt_min2 (optional) receives the per-instance minimum at
+// order 2 from the same states (the order-promotion test,
+// transient.cpp:946-958).
 void launch_trunc(const TruncTable& t, const DevCtx* ctx, const double* s0, const double* s1,
                   const double* s2, const double* s3, int B, const TruncOpts& o, double* dt_dev,
-                  unsigned long long* dt_min, hipStream_t s, int order_override = 0);
+                  unsigned long long* dt_min, hipStream_t s, int order_override = 0,
+                  unsigned long long* dt_min2 = nullptr);
 void launch_fill_u64(unsigned long long* p, unsigned long long v, int count, hipStream_t s);
 // out[k] = A[k * B + inst], k < count (one instance of a slot-major batch)
 void launch_take_instance(const double* A, int B, int inst, int count, double* out, hipStream_t s);
