@@ -1,24 +1,28 @@
 #!/usr/bin/env python3
-"""Benchmark: batched Newton iterations/s of the linear RC power-grid mesh.
+"""Benchmark: batched Newton iterations/s of the uneven-size BSIM4v7
+ring-oscillator batch, against the same kernels at batch 1.
 
-Workload (the batched-refactor configuration): 64x64 RC mesh (4096 nodes +
-1 source branch), branch R log-uniform [0.1, 10] ohm, one decap per node
-log-uniform [10 fF, 1 pF], 5% of nodes with DC current sinks in [0, 1 mA],
-1.0 V at a corner; 256 instances with per-instance +-10% Gaussian R
-perturbation; backward-Euler transient (xmu = 0) at dt = 1 ps.
+Headline workload: 256 BSIM4v7 ring oscillators (LEVEL=14 default cards,
+TOXE 1.8 nm, VTH0 +-0.4 V, W/L 0.5u/0.05u N and 1u/0.05u P, VDD 1 V) whose
+stage counts are drawn uniformly from the odd numbers 31..201, every
+transistor with its own Gaussian mismatch (3 % VTH0, 1 % TOXE); 2 ns of
+adaptive trapezoidal transient with the step capped at 1 ps, every
+instance with its own step controller, in one bundle (all 256 circuits in
+every launch, the Newton iteration replayed from a captured HIP graph).
 
-A bench step is one transient timestep of all 256 instances: the full
-newton_solve (device load + gather, GPU refactor on the fixed AMD-LU order,
-GPU solve, convergence) -- two Newton iterations per step for this linear
-circuit.  value = sum over instances of Newton iterations / wall time.
+A bench step is one window of tstop / K of that transient for all 256
+instances, so the K timed steps cover the whole 2 ns; the warm-up steps run
+on a separate bundle of the same circuits.  value = sum over instances of
+Newton iterations / wall time of the K windows (the operating point before
+them is reported apart).
 
-Also measured in the same run: the same kernels at batch 1 (the plain
-batch-1 GPU path), HIP-event times of the load, refactor and solve phases
-with their roofline, the reference CPU path (compiled neospice,
-oracle/_ref) on the host cores on a bounded sample, and the second half of
-the metric, adjoint gradient time / forward time, on the transient-adjoint
-workload (32x32 diode-clamped RC mesh, 128 instances, 500 trapezoidal
-steps, checkpoints every 25 steps; see adjoint_c5).
+Also measured in the same run: the batch-1 GPU path (the same kernels, one
+instance per bundle) on a stratified sample of the instances, the parity of
+several instances against the reference's own adaptive run (oracle/_ref),
+HIP-event times of every packed-LU launch over a replay of the whole
+transient with its roofline, the reference CPU path on the host cores (one
+core and all cores), the linear RC-mesh batch, and the second half of the
+metric, adjoint gradient time / forward time.
 """
 import argparse
 import json
@@ -40,6 +44,30 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
+
+
+def profile_value(name, key):
+    """A figure from a committed rocprofv3 summary under profiles/ (a
+    markdown table row `| key | value |`), or None when it is absent."""
+    try:
+        for line in open(os.path.join(ROOT, "profiles", name)):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) >= 2 and cells[0] == key:
+                return float(cells[1])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def measured_traffic():
     """HBM bytes per batched refactor of this workload from the committed PMC
     summary (FETCH_SIZE + WRITE_SIZE scaled by a plain copy's counters), or
@@ -50,19 +78,6 @@ def measured_traffic():
             if line.startswith("traffic_bytes_per_refactor:"):
                 return float(line.split(":", 1)[1])
     except OSError:
-        pass
-    return None
-
-
-def ring_load_traffic():
-    """HBM bytes per k_bsim4 launch on the 117-stage, 256-instance ring batch
-    from the committed PMC summary, or None when it is absent."""
-    path = os.path.join(ROOT, "profiles", "k_bsim4_bsim4v7_ring_117_b256.md")
-    try:
-        for line in open(path):
-            if line.startswith("| traffic_bytes_per_launch |"):
-                return float(line.split("|")[2])
-    except (OSError, ValueError):
         pass
     return None
 
@@ -104,7 +119,7 @@ def run_steps(eng, na, dt, first_step, count, t0_step):
     return iters
 
 
-def cpu_baseline(ckt, info, R, dt, nsteps, budget_s, parity_steps, x_gpu0):
+def mesh_cpu_baseline(ckt, info, R, dt, nsteps, budget_s, parity_steps, x_gpu0):
     """CPU-baseline leg: the reference's own CPU Newton path (oracle/_ref) on
     whole instances of the same workload, one instance per host thread
     (ctypes releases the GIL).  The same leg checks instance 0 of the GPU run
@@ -148,25 +163,101 @@ def cpu_baseline(ckt, info, R, dt, nsteps, budget_s, parity_steps, x_gpu0):
 
     rate1, _, _, _ = leg(1, budget_s * 0.25)
     rate, iters, runs, wall = leg(cores, budget_s * 0.75)
-    model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
     cpu = {"value": round(rate, 2), "unit": "Newton iters/s", "cores": cores,
            "kind": "reference",
            "value_1core": round(rate1, 2),
-           "cpu_model": model,
+           "cpu_model": cpu_model(),
            "sample": "%d runs of %d BE steps (%d Newton iterations) of the 64x64 mesh, one "
                      "instance per host thread on %d threads within %.1f s (plus a 1-thread leg); "
                      "rate = sum over threads of iterations / stepping time; reference CPU path: "
                      "newton_solve with AMD-LU replay refactor/solve and device loads" %
                      (runs, nsteps, iters, cores, wall)}
     return cpu, parity
+
+
+def rc_mesh(na, W, args, B=256, dt=1e-12):
+    """The linear batched-refactor workload: 64x64 RC mesh (4096 nodes + 1
+    source branch), branch R log-uniform [0.1, 10] ohm, one decap per node
+    log-uniform [10 fF, 1 pF], 5% of nodes with DC current sinks in [0, 1 mA],
+    1.0 V at a corner; 256 instances with per-instance +-10% Gaussian R
+    perturbation; backward-Euler steps (xmu = 0) of 1 ps.  A step is one
+    transient timestep of all instances (newton_solve: device load + gather,
+    refactor on the fixed AMD-LU order, solve, convergence)."""
+    import torch
+    ckt, info = W.rc_mesh(args.mesh)
+    ckt.finalize()
+    R = W.mesh_instance_resistors(info, B)
+    log("bench: mesh %dx%d n=%d nnz=%d, batch %d" % (args.mesh, args.mesh, ckt.n, ckt.nnz, B))
+    eng = na.BatchEngine(ckt, batch=B, xmu=0.0)
+    for k, d in enumerate(info["res_devs"]):
+        eng.set_values(int(d), R[k])
+    eng.tran_begin()
+    warm = 3
+    run_steps(eng, na, dt, True, warm, 0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    iters = run_steps(eng, na, dt, False, args.mesh_steps, warm)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    value = iters / wall
+    log("bench: mesh batched %d steps, %d instance-iterations in %.3f s -> %.1f iters/s"
+        % (args.mesh_steps, iters, wall, value))
+    nph = 5
+    eng.set_timing(True)
+    it_ph = run_steps(eng, na, dt, False, nph, warm + args.mesh_steps)
+    torch.cuda.synchronize()
+    timing = eng.timing()
+    eng.set_timing(False)
+    refactor_bytes, solve_bytes, st = lu_traffic(eng, ckt, B)
+    x_gpu0 = eng.solution()[:, 0].copy()
+    total_steps = warm + args.mesh_steps + nph
+
+    e1 = na.BatchEngine(ckt, batch=1, xmu=0.0)
+    for k, d in enumerate(info["res_devs"]):
+        e1.set_values(int(d), R[k, :1])
+    e1.tran_begin()
+    run_steps(e1, na, dt, True, warm, 0)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    it1 = run_steps(e1, na, dt, False, args.mesh_batch1_steps, warm)
+    torch.cuda.synchronize()
+    w1 = time.perf_counter() - t1
+    batch1_value = it1 / w1
+
+    nit = max(1, it_ph // B)
+    ms_refactor = timing["refactor_ms"] / nit
+    ms_solve = timing["solve_ms"] / nit
+    ms_load = timing["load_ms"] / nit
+    fl = refactor_flops(e1, ckt, B)
+    achieved = refactor_bytes / (ms_refactor * 1e-3) / 1e9
+    traffic = measured_traffic() if (args.mesh == 64 and B == 256) else None
+    roofline = {"bound": "hbm", "kernel": "k_refactor", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+                "launch_ms": round(ms_refactor, 4), "bytes_per_launch": refactor_bytes,
+                "flops_per_launch": fl,
+                "fp64_TFLOPs": round(fl / (ms_refactor * 1e-3) / 1e12, 4),
+                "fp64_frac": round(fl / (ms_refactor * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 6),
+                "traffic_source": ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a plain "
+                                   "copy (profiles/k_refactor_pmc_rc_mesh_64x64_b256.md)" if traffic else None)}
+    phases = {"load_gather_ms": ms_load, "refactor_ms": ms_refactor, "solve_ms": ms_solve,
+              "solve_GBs": solve_bytes / (ms_solve * 1e-3) / 1e9,
+              "levels_refactor": st["levels_refactor"], "levels_solve": st["levels_solve"],
+              "nnz_lu": st["nnz_lu"], "pivot_groups": st["pivot_groups"]}
+    cpu, parity = None, None
+    if not args.no_cpu:
+        try:
+            cpu, parity = mesh_cpu_baseline(ckt, info, R, dt, 10, args.cpu_seconds * 0.5, total_steps, x_gpu0)
+        except Exception as e:  # the baseline is reported, not required
+            log("bench: mesh cpu baseline unavailable:", e)
+    return {"workload": "rc_mesh_%dx%d_batched_refactor_be_1ps_b%d" % (args.mesh, args.mesh, B),
+            "value": round(value, 2), "unit": "Newton iters/s", "steps": args.mesh_steps,
+            "ms_per_step": round(wall / args.mesh_steps * 1e3, 4),
+            "unknowns": ckt.n, "nnz": ckt.nnz, "solver": eng.solver_name,
+            "batch1_gpu_value": round(batch1_value, 2),
+            "speedup_vs_batch1_gpu": round(value / batch1_value, 2),
+            "parity_max_abs_diff_vs_reference": parity,
+            "phases": phases, "roofline": roofline, "cpu_baseline": cpu}
+
 
 
 def adjoint_c5(na, W, mesh=32, batch=128, nsteps=500, dt=2e-12, ckpt=25, nprobe=64):
@@ -207,116 +298,258 @@ def adjoint_c5(na, W, mesh=32, batch=128, nsteps=500, dt=2e-12, ckpt=25, nprobe=
                     "and transposed solve per step)"}
 
 
-def bsim4_ring(na, W, stages=117, batch=256, steps=30, warmup=3, b1_steps=10, dt=1e-12,
-               cpu_steps=10, seed=31):
-    """The BSIM4v7 ring-oscillator batch: `batch` instances of a `stages`-stage
-    ring (LEVEL=14 cards, TOXE 1.8 nm, VTH0 +-0.4 V, W/L 0.5u/0.05u and
-    1u/0.05u, VDD 1 V) with per-transistor, per-instance Gaussian mismatch of
-    3% on VTH0 and 1% on TOXE, fixed 1 ps backward-Euler steps (the adaptive
-    controller's cap).  Batched and batch-1 Newton iterations/s, and the
-    reference CPU path on instance 0 (its own fixed-step loop) for parity and
-    as the CPU baseline."""
-    import re
+
+# ------------------------------------------------------------------ C3
+
+C3_TSTEP = 1e-12
+C3_TSTOP = 2e-9               # the workload's transient; --tstop shortens it for profiler runs
+C3_MAX_POINTS = 4000          # accepted points per instance (2 ns / 1 ps + breakpoint steps)
+
+
+def c3_probes(c):
+    n = c.n
+    return sorted({1, n // 4, n // 2, (3 * n) // 4, n - 2, n - 1})
+
+
+def c3_run(na, cks, probes, windows, timing=False):
+    """One adaptive transient of the whole batch in `windows` equal windows:
+    (bundle, seconds of the operating point, seconds of the windows,
+    transient Newton iterations)."""
     import torch
-    rng = np.random.default_rng(seed)
-    ckt, info = W.ring_oscillator(stages, mismatch_rng=np.random.default_rng(seed + 1))
-    ckt.finalize()
-    texts = {op[1].split()[1].lower(): op[1] for op in ckt.ops if op[0] == "model"}
-    devs = ckt.ops_devices()
-    mdevs = [i for i, op in enumerate(devs) if op[0] == "M" and op[1] != "mrst"]
-    mism = {}
-    for dv in mdevs:
-        text = texts[devs[dv][6].lower()]
-        for par, sig in (("vth0", 0.03), ("toxe", 0.01)):
-            base = float(re.search(par + r"=([-0-9.e+]+)", text).group(1))
-            v = base * (1.0 + sig * rng.standard_normal(batch))
-            v[0] = base          # instance 0 keeps the circuit's own cards
-            mism[(dv, par)] = v
-
-    def make(B):
-        eng = na.BatchEngine(ckt, batch=B, xmu=0.0, itl1=400)
-        for (dv, par), v in mism.items():
-            eng.set_model_values(dv, par, v[:B])
-        eng.tran_begin()
-        return eng
-
-    eng = make(batch)
-    log("bench: ring %d stages n=%d nnz=%d, batch %d, DC done" % (stages, ckt.n, ckt.nnz, batch))
-    run_steps(eng, na, dt, True, warmup, 0)
-    torch.cuda.synchronize()
+    b = na.Bundle(cks)
     t0 = time.perf_counter()
-    iters = run_steps(eng, na, dt, False, steps, warmup)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    eng.set_timing(True)
-    it_ph = run_steps(eng, na, dt, False, 3, warmup + steps)
-    torch.cuda.synchronize()
-    timing = eng.timing()
-    eng.set_timing(False)
-    nit = max(1, it_ph // batch)
-    x0 = eng.solution()[:, 0].copy()
-    total = warmup + steps + 3
-    e1 = make(1)
-    run_steps(e1, na, dt, True, warmup, 0)
-    torch.cuda.synchronize()
+    b.tran_begin(C3_TSTEP, C3_TSTOP, probes=probes, max_points=C3_MAX_POINTS)
     t1 = time.perf_counter()
-    it1 = run_steps(e1, na, dt, False, b1_steps, warmup)
+    it0 = b.stats()["iterations"]
+    if timing:
+        b.set_timing(True)
     torch.cuda.synchronize()
-    w1 = time.perf_counter() - t1
-    # the BSIM4v7 load kernel against HBM: algorithmic bytes per device-instance
-    # of a backward-Euler corrector load with per-instance cards -- the
-    # 110-double constant record (880 B), 3 node voltages (24 B), 4 state
-    # voltages in (32 B), 3 charges of state1 in (24 B), 13 state slots out
-    # (104 B), 10 carried instance fields in and out (160 B), 20 stamp sites
-    # out (160 B): 1384 B
-    b4_bytes = 1384.0 * (len(mdevs) + 1) * batch
-    b4_ms = timing["bsim4_ms"] / max(1, timing["loads"])
-    b4_gbs = b4_bytes / (b4_ms * 1e-3) / 1e9
-    out = {"workload": "bsim4v7_ring_%d_stages_be_1ps_b%d" % (stages, batch), "instances": batch,
-           "unknowns": ckt.n, "nnz": ckt.nnz, "devices_bsim4": len(mdevs) + 1,
-           "value": round(iters / wall, 2), "unit": "Newton iters/s",
-           "ms_per_step": round(wall / steps * 1e3, 4),
-           "batch1_gpu_value": round(it1 / w1, 2),
-           "speedup_vs_batch1_gpu": round((iters / wall) / (it1 / w1), 2),
-           "phases_ms_per_iter": {"load_gather": timing["load_ms"] / nit,
-                                  "refactor": timing["refactor_ms"] / nit,
-                                  "solve": timing["solve_ms"] / nit},
-           "roofline": {"bound": "hbm", "kernel": "k_bsim4", "achieved": round(b4_gbs, 2),
-                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(b4_gbs / HBM_PEAK_GBS, 6),
-                        "traffic": (ring_load_traffic() if (stages == 117 and batch == 256) else None),
-                        "launch_ms": round(b4_ms, 5), "bytes_per_launch": b4_bytes}}
-    try:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import refbind as RB
-        rc = RB.RefCircuit(ckt, probe=False)
-        xr, its_r, sec = rc.tran_fixed(dt, total)
-        out["parity_max_abs_diff_vs_reference"] = float(np.max(np.abs(x0 - xr)))
-        rc2 = RB.RefCircuit(ckt, probe=False)
-        _, its_c, sec_c = rc2.tran_fixed(dt, cpu_steps)
-        out["cpu_baseline"] = {"value": round(its_c / sec_c, 2), "unit": "Newton iters/s", "cores": 1,
-                               "kind": "reference",
-                               "sample": "instance 0, %d fixed 1 ps BE steps on one core" % cpu_steps}
-    except Exception as e:  # reported, not required
-        log("bench: ring reference leg unavailable:", e)
-    log("bench: ring", out)
+    t2 = time.perf_counter()
+    done = False
+    for k in range(windows):
+        done = b.tran_advance(C3_TSTOP * (k + 1) / windows)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    if not done:
+        raise RuntimeError("C3: the transient did not finish")
+    return b, t1 - t0, t3 - t2, b.stats()["iterations"] - it0
+
+
+def c3_check_status(b, N):
+    acc = [b.tran_status(i) for i in range(N)]
+    bad = [i for i, a in enumerate(acc) if a["failed"] or not a["finished"]]
+    if bad:
+        raise RuntimeError("C3: instances %s failed or did not finish" % bad[:8])
+    return acc
+
+
+def c3_reference(cks, pick, probes, b, budget_s):
+    """The reference's own adaptive transient (solve_transient through
+    oracle/_ref) of the picked instances: their parity against the batched
+    run, and the one-core CPU rate of those runs."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refbind as RB
+    if not RB.available():
+        return None, None
+    rows, its_tot, sec_tot = [], 0, 0.0
+    t_start = time.perf_counter()
+    for i in pick:
+        if rows and time.perf_counter() - t_start > budget_s:
+            break
+        rc = RB.RefCircuit(cks[i], probe=False)
+        t_ref, x_ref, its, sec = rc.tran(C3_TSTEP, C3_TSTOP, max_points=C3_MAX_POINTS)
+        its_tot += its
+        sec_tot += sec
+        st = b.tran_status(i)
+        steps = b.tran_steps(i)
+        row = {"instance": int(i), "unknowns": cks[i].n, "accepted": st["accepted"],
+               "accepted_reference": len(t_ref) - 1, "reference_newton_iters": its}
+        if st["accepted"] == len(t_ref) - 1:
+            row["max_rel_t"] = float(np.max(np.abs(steps["t"] - t_ref[1:]) / t_ref[1:]))
+            x = b.tran_probes(i)
+            xr = x_ref[:, probes[i]]
+            row["max_rel_wave"] = float(np.max(np.abs(x - xr) / np.maximum(np.abs(xr), 1e-3)))
+        rows.append(row)
+        log("bench: C3 parity", row)
+    ok = all(r["accepted"] == r["accepted_reference"] and r.get("max_rel_wave", 1.0) <= 1e-6 for r in rows)
+    parity = {"instances": rows, "within_1e-6": bool(ok),
+              "note": "each instance's accepted steps and probed waveforms against the reference's own "
+                      "adaptive solve_transient of that ring (rel err vs max(|x|, 1e-3))"}
+    rate1 = its_tot / sec_tot if sec_tot > 0 else None
+    return parity, (rate1, its_tot, sec_tot, len(rows))
+
+
+def c3_cpu_all_cores(cks, seconds):
+    """The reference's adaptive transient of whole instances, one per host
+    thread (ctypes releases the GIL), instances taken round robin."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refbind as RB
+    cores = min(os.cpu_count() or 1, 16)
+    tot = [[0, 0.0, 0] for _ in range(cores)]
+    deadline = time.perf_counter() + seconds
+
+    def worker(tid):
+        i = tid
+        while time.perf_counter() < deadline:
+            rc = RB.RefCircuit(cks[i % len(cks)], probe=False)
+            _, _, its, sec = rc.tran(C3_TSTEP, C3_TSTOP, max_points=C3_MAX_POINTS)
+            tot[tid][0] += its
+            tot[tid][1] += sec
+            tot[tid][2] += 1
+            i += cores
+
+    t0 = time.perf_counter()
+    th = [threading.Thread(target=worker, args=(t,)) for t in range(cores)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    wall = time.perf_counter() - t0
+    rate = sum(it / sec for it, sec, _ in tot if sec > 0)
+    return rate, cores, sum(r for _, _, r in tot), sum(it for it, _, _ in tot), wall
+
+
+def c3_headline(na, W, args):
+    import torch
+    global C3_TSTOP
+    C3_TSTOP = args.tstop
+    N = args.instances
+    cks, stages = W.c3_rings(N)
+    for c in cks:
+        c.finalize()
+    probes = [c3_probes(c) for c in cks]
+    log("bench: C3 %d rings, stages %d..%d (mean %.1f)" % (N, stages.min(), stages.max(), stages.mean()))
+    K = max(1, args.steps)
+    # warm-up: the first W windows on a bundle of their own
+    if args.warmup > 0:
+        bw = na.Bundle(cks)
+        bw.tran_begin(C3_TSTEP, C3_TSTOP, probes=None)
+        for k in range(args.warmup):
+            bw.tran_advance(C3_TSTOP * (k + 1) / K)
+        bw.close()
+    b, dc_s, wall, iters = c3_run(na, cks, probes, K)
+    acc = c3_check_status(b, N)
+    value = iters / wall
+    st = b.stats()
+    log("bench: C3 batched: %d windows, %d Newton iterations in %.3f s -> %.0f iters/s (operating point %.3f s)"
+        % (K, iters, wall, value, dc_s))
+    info = [b.plan_info(i) for i in range(N)]
+
+    # HIP events around every packed-LU launch over a replay of the whole
+    # transient (the kernels launched one by one on the bundle's stream)
+    bt, _, wall_t, _ = c3_run(na, cks, probes, K, timing=True)
+    tim = bt.timing()
+    its_i = np.array([bt.tran_status(i)["all_iters"] for i in range(N)], dtype=np.float64)
+    bt.set_timing(False)
+    launches = max(1, tim["iterations_timed"])
+    lu_ms = tim["lu_ms"] / launches
+    # algorithmic bytes of one refactor + forward/backward solve of an
+    # instance: A read, L+U written by the refactor and read by the solve,
+    # rhs in and x out; summed over the instance-iterations of the replay
+    per_inst = np.array([8.0 * (p["nnz_a"] + 2 * p["nnz_lu"]) + 16.0 * p["n"] for p in info])
+    flops_inst = np.array([float(p["flops_refactor"] + p["flops_solve"]) for p in info])
+    bytes_launch = float(np.dot(its_i, per_inst)) / launches
+    flops_launch = float(np.dot(its_i, flops_inst)) / launches
+    achieved = bytes_launch / (lu_ms * 1e-3) / 1e9
+    traffic = profile_value("k_small_lu_pmc_c3_rings_256.md", "traffic_bytes_per_launch") if N == 256 else None
+    roofline = {"bound": "hbm", "kernel": "k_small_lu", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+                "launch_ms": round(lu_ms, 5), "launches": int(launches),
+                "bytes_per_launch": round(bytes_launch, 1),
+                "flops_per_launch": round(flops_launch, 1),
+                "fp64_TFLOPs": round(flops_launch / (lu_ms * 1e-3) / 1e12, 5),
+                "fp64_frac": round(flops_launch / (lu_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 7),
+                "critical_path_levels": int(max(p["levels"] + p["solve_levels"] for p in info)),
+                "ns_per_level": round(lu_ms * 1e6 / max(p["levels"] + p["solve_levels"] for p in info), 2),
+                "timing": "HIP events on the bundle stream around each k_small_lu launch, %d launches over "
+                          "a replay of the whole 2 ns transient" % launches,
+                "traffic_source": ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the same "
+                                   "counters on a plain copy of known size (profiles/k_small_lu_pmc_c3_rings_256.md)"
+                                   if traffic is not None else None)}
+    kernels = {"iteration_ms": {"load_gather": tim["load_ms"] / launches, "k_bsim4": tim["bsim4_ms"] / launches,
+                                "k_small_lu": lu_ms, "convergence": tim["post_ms"] / launches},
+               "replay_wall_s": round(wall_t, 4)}
+    log("bench: C3 k_small_lu %.4f ms/launch, %.1f GB/s, kernels %s" % (lu_ms, achieved, kernels))
+
+    # batch-1 GPU path: the same kernels, one instance per bundle, on a
+    # sample stratified by stage count
+    order = np.argsort(stages, kind="stable")
+    S = min(args.batch1_sample, N)
+    pick = [int(order[int(round(q))]) for q in np.linspace(0, N - 1, max(S, 1))]
+    it1, w1, same = 0, 0.0, True
+    b1_rows = []
+    for i in (pick if S > 0 else []):
+        b1, _, w, it = c3_run(na, [cks[i]], [probes[i]], K)
+        c3_check_status(b1, 1)
+        it1 += it
+        w1 += w
+        eq = bool(np.array_equal(b1.tran_probes(0), b.tran_probes(i)))
+        same &= eq
+        b1_rows.append({"instance": i, "stages": int(stages[i]), "iters": it, "s": round(w, 4),
+                        "bit_equal_to_batched": eq})
+        b1.close()
+    batch1_value = it1 / w1 if w1 > 0 else None
+    log("bench: C3 batch-1 sample %s -> %s iters/s" % (b1_rows, batch1_value))
+
+    parity, cpu1 = None, None
+    cpu = None
+    if not args.no_cpu:
+        try:
+            parity, cpu1 = c3_reference(cks, pick[::max(1, len(pick) // 4)][:4] + [pick[-1]],
+                                        probes, b, args.cpu_seconds * 0.4)
+            if cpu1 is not None:
+                rate_all, cores, runs, its_all, wall_all = c3_cpu_all_cores(cks, args.cpu_seconds * 0.6)
+                cpu = {"value": round(rate_all, 2), "unit": "Newton iters/s", "cores": cores, "kind": "reference",
+                       "value_1core": round(cpu1[0], 2), "cpu_model": cpu_model(),
+                       "sample": "reference solve_transient (oracle/_ref) of whole 2 ns instances of this batch: "
+                                 "%d runs (%d Newton iterations) one instance per host thread on %d threads "
+                                 "within %.1f s; value_1core: %d instances (%d iterations) in sequence on one "
+                                 "core; rate = sum over threads of iterations / solve_transient time"
+                                 % (runs, its_all, cores, wall_all, cpu1[3], cpu1[1])}
+                log("bench: C3 cpu baseline", cpu)
+        except Exception as e:  # the baseline is reported, not required
+            log("bench: C3 reference leg unavailable:", e)
+
+    accepted = np.array([a["accepted"] for a in acc])
+    config = {"workload": "bsim4v7_rings_%d_uneven_31_201_tran_%gps_adaptive_trap_1ps" % (N, C3_TSTOP * 1e12),
+              "instances": N, "stages_min": int(stages.min()), "stages_max": int(stages.max()),
+              "stages_mean": round(float(stages.mean()), 2),
+              "unknowns_total": b.total_n, "nnz_total": b.total_nnz,
+              "tstop": C3_TSTOP, "max_step": C3_TSTEP, "integration": "trapezoidal, adaptive (device LTE)",
+              "step": "one window of tstop/K of every instance's adaptive transient",
+              "pivot_plans": st["plans"]}
+    out = {"value": value, "wall": wall, "K": K, "config": config,
+           "accepted_steps": {"min": int(accepted.min()), "max": int(accepted.max()),
+                              "mean": round(float(accepted.mean()), 2),
+                              "rejected": int(sum(a["rejected"] for a in acc))},
+           "operating_point_s": round(dc_s, 4),
+           "batch1_gpu_value": round(batch1_value, 2) if batch1_value else None,
+           "speedup_vs_batch1_gpu": round(value / batch1_value, 3) if batch1_value else None,
+           "batch1_sample": b1_rows,
+           "batch1_bit_equal_to_batched": bool(same),
+           "parity": parity, "roofline": roofline, "kernels": kernels, "cpu_baseline": cpu}
     return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=256)
-    ap.add_argument("--mesh", type=int, default=64)
-    ap.add_argument("--batch1-steps", type=int, default=20)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--steps", type=int, default=20,
+                    help="timed windows; together they cover the whole 2 ns transient")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--batch1-sample", type=int, default=8,
+                    help="instances timed one per bundle for the batch-1 rate (0: skip)")
+    ap.add_argument("--tstop", type=float, default=2e-9,
+                    help="transient length (shorter only for profiler runs)")
+    ap.add_argument("--cpu-seconds", type=float, default=16.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--no-ring", action="store_true")
-    ap.add_argument("--ring-stages", type=int, default=117)
-    ap.add_argument("--ring-only", action="store_true",
-                    help="only the BSIM4v7 ring-oscillator leg (its JSON object)")
+    ap.add_argument("--no-mesh", action="store_true")
+    ap.add_argument("--headline-only", action="store_true", help="only the ring batch")
+    ap.add_argument("--mesh", type=int, default=64)
+    ap.add_argument("--mesh-steps", type=int, default=100)
+    ap.add_argument("--mesh-batch1-steps", type=int, default=20)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of known size (for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -339,126 +572,30 @@ def main():
         return
     import neospice_amd as na
     from neospice_amd import workloads as W
-    if args.ring_only:
-        print(json.dumps(bsim4_ring(na, W, stages=args.ring_stages, steps=args.steps,
-                                    warmup=args.warmup, b1_steps=args.batch1_steps)))
-        return
 
-    dt = 1e-12
-    B = args.batch
-    ckt, info = W.rc_mesh(args.mesh)
-    ckt.finalize()
-    R = W.mesh_instance_resistors(info, B)
-    log("bench: mesh %dx%d n=%d nnz=%d, batch %d" % (args.mesh, args.mesh, ckt.n, ckt.nnz, B))
-
-    eng = na.BatchEngine(ckt, batch=B, xmu=0.0)
-    for k, d in enumerate(info["res_devs"]):
-        eng.set_values(int(d), R[k])
-    eng.tran_begin()
-    log("bench: DC preamble done, solver", eng.solver_name)
-    run_steps(eng, na, dt, True, args.warmup, 0)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    iters = run_steps(eng, na, dt, False, args.steps, args.warmup)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    value = iters / wall
-    log("bench: batched %d steps, %d instance-iterations in %.3f s -> %.1f iters/s"
-        % (args.steps, iters, wall, value))
-    # phase breakdown on a few more steps, HIP events on the engine's stream
-    nph = max(1, min(5, args.steps))
-    eng.set_timing(True)
-    it_ph = run_steps(eng, na, dt, False, nph, args.warmup + args.steps)
-    torch.cuda.synchronize()
-    timing = eng.timing()
-    eng.set_timing(False)
-    refactor_bytes, solve_bytes, st = lu_traffic(eng, ckt, B)
-    x_gpu0 = eng.solution()[:, 0].copy()
-    total_steps = args.warmup + args.steps + nph
-
-    # batch-1 GPU path: the same kernels, one instance
-    e1 = na.BatchEngine(ckt, batch=1, xmu=0.0)
-    for k, d in enumerate(info["res_devs"]):
-        e1.set_values(int(d), R[k, :1])
-    e1.tran_begin()
-    run_steps(e1, na, dt, True, args.warmup, 0)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    it1 = run_steps(e1, na, dt, False, args.batch1_steps, args.warmup)
-    torch.cuda.synchronize()
-    w1 = time.perf_counter() - t1
-    batch1_value = it1 / w1
-    log("bench: batch-1 %d steps, %d iterations in %.3f s -> %.1f iters/s"
-        % (args.batch1_steps, it1, w1, batch1_value))
-
-    nit = max(1, it_ph // B)      # batched Newton iterations in the phase run
-    ms_refactor = timing["refactor_ms"] / nit
-    ms_solve = timing["solve_ms"] / nit
-    ms_load = timing["load_ms"] / nit
-    fl = refactor_flops(e1, ckt, B)
-    phases = {
-        "load_gather_ms": ms_load, "refactor_ms": ms_refactor, "solve_ms": ms_solve,
-        "refactor_GBs": refactor_bytes / (ms_refactor * 1e-3) / 1e9,
-        "refactor_GFLOPs": fl / (ms_refactor * 1e-3) / 1e9,
-        "refactor_flops": fl,
-        "solve_GBs": solve_bytes / (ms_solve * 1e-3) / 1e9,
-        "levels_refactor": st["levels_refactor"], "levels_solve": st["levels_solve"],
-        "nnz_lu": st["nnz_lu"], "pivot_groups": st["pivot_groups"],
-    }
-    achieved = refactor_bytes / (ms_refactor * 1e-3) / 1e9
-    # rocprof PMC traffic applies to the default workload only
-    traffic = measured_traffic() if (args.mesh == 64 and B == 256) else None
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
-                "kernel": "k_refactor",
-                "launch_ms": round(ms_refactor, 4),
-                "bytes_per_launch": refactor_bytes,
-                "flops_per_launch": fl,
-                "fp64_TFLOPs": round(fl / (ms_refactor * 1e-3) / 1e12, 4),
-                "fp64_frac": round(fl / (ms_refactor * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 6),
-                "traffic_GBs": (round(traffic / (ms_refactor * 1e-3) / 1e9, 1) if traffic else None)}
-
-    cpu, parity = None, None
-    if not args.no_cpu:
-        try:
-            cpu, parity = cpu_baseline(ckt, info, R, dt, 10, args.cpu_seconds, total_steps, x_gpu0)
-            log("bench: instance 0 vs reference CPU path, max |dx| =", parity)
-        except Exception as e:  # the baseline is reported, not required
-            log("bench: cpu baseline unavailable:", e)
-
-    adjoint = None
-    if not args.no_adjoint:
-        adjoint = adjoint_c5(na, W, nsteps=args.adjoint_steps)
-
-    ring = None
-    if not args.no_ring:
-        ring = bsim4_ring(na, W, stages=args.ring_stages)
-
+    c3 = c3_headline(na, W, args)
+    mesh = None if (args.no_mesh or args.headline_only) else rc_mesh(na, W, args)
+    adjoint = None if (args.no_adjoint or args.headline_only) else adjoint_c5(na, W, nsteps=args.adjoint_steps)
     out = {
         "metric": "batched Newton iters/s vs batch-1 GPU @1e-9 rel; adjoint grad/forward time",
-        "value": round(value, 2),
+        "value": round(c3["value"], 2),
         "unit": "Newton iters/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": c3["K"],
         "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "ms_per_step": round(c3["wall"] / c3["K"] * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded 64x64 RC mesh, per-instance R perturbation)",
-        "config": {"workload": "rc_mesh_64x64_batched_refactor_be", "instances": B,
-                   "unknowns": ckt.n, "nnz": ckt.nnz, "dt": dt, "integration": "backward Euler",
-                   "solver": eng.solver_name},
-        "batch1_gpu_value": round(batch1_value, 2),
-        "speedup_vs_batch1_gpu": round(value / batch1_value, 2),
-        "parity_max_abs_diff_vs_reference": parity,
-        "phases": phases,
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "adjoint": adjoint,
-        "bsim4_ring": ring,
+        "data": "synthetic (seeded BSIM4v7 rings: stage counts and per-transistor mismatch)",
+        "config": c3["config"],
     }
+    for k in ("batch1_gpu_value", "speedup_vs_batch1_gpu", "batch1_bit_equal_to_batched", "batch1_sample",
+              "accepted_steps", "operating_point_s", "parity", "roofline", "kernels", "cpu_baseline"):
+        out[k] = c3[k]
+    out["adjoint"] = adjoint
+    out["rc_mesh"] = mesh
     print(json.dumps(out))
 
 

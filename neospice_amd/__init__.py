@@ -620,5 +620,10 @@ class Bundle:
     def set_graph(self, on=True):
         check(lib().nsa_bundle_set_graph(self.h, int(bool(on))))
 
-    def plan_of(self, i):
-        return check(lib().nsa_bundle_plan_of(self.h, int(i)))
+    def plan_info(self, i):
+        """Pivot structure of instance i: plan id, sizes, levels and the
+        operation counts of one refactor and one forward + backward solve."""
+        o = np.zeros(8, np.int64)
+        check(lib().nsa_bundle_plan_info(self.h, int(i), o.ctypes.data_as(_lib.P_I64)))
+        keys = ("plan", "n", "nnz_a", "nnz_lu", "levels", "solve_levels", "flops_refactor", "flops_solve")
+        return {k: int(v) for k, v in zip(keys, o)}

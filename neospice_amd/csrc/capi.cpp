@@ -663,10 +663,32 @@ int nsa_bundle_dc_ladder(nsa_bundle* b, int32_t skip_direct, int32_t* converged,
         return 0;
     });
 }
-int nsa_bundle_plan_of(nsa_bundle* b, int32_t inst) {
+int nsa_bundle_plan_info(nsa_bundle* b, int32_t inst, int64_t out[8]) {
     return guard([&] {
         if (inst < 0 || inst >= b->b->size()) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "instance out of range");
-        return b->b->plan_of(inst);
+        const int p = b->b->plan_of(inst);
+        for (int k = 0; k < 8; ++k) out[k] = 0;
+        out[0] = p;
+        if (p < 0) return 0;
+        const nsa::LuStructure& s = b->b->plans().structure(p);
+        out[1] = s.n;
+        out[2] = s.nnz_a;
+        out[3] = s.nlu();
+        out[4] = b->b->plans().levels(p);
+        out[5] = b->b->plans().solve_levels(p);
+        // left-looking operation counts of the Sparse 1.3 factor (factor.cpp:1010-1026)
+        // and of the forward and backward substitutions (solve.cpp:6-47)
+        int64_t f = s.n, off = 0;
+        for (int j = 0; j < s.n; ++j) {
+            off += s.cptr[j + 1] - s.cptr[j] - 1;
+            for (int q = s.cptr[j]; q < s.cptr[j] + s.nU[j]; ++q) {
+                const int r = s.rows[q];
+                f += 1 + 2 * (s.cptr[r + 1] - s.cptr[r] - s.nU[r] - 1);
+            }
+        }
+        out[6] = f;
+        out[7] = 2 * off + s.n;
+        return 0;
     });
 }
 
