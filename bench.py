@@ -448,19 +448,28 @@ def c3_headline(na, W, args):
     # rhs in and x out; summed over the instance-iterations of the replay
     per_inst = np.array([8.0 * (p["nnz_a"] + 2 * p["nnz_lu"]) + 16.0 * p["n"] for p in info])
     flops_inst = np.array([float(p["flops_refactor"] + p["flops_solve"]) for p in info])
-    bytes_launch = float(np.dot(its_i, per_inst)) / launches
+    # plan indices count once per launch for each distinct plan in use
+    plan_words, plan_its = {}, {}
+    for i, p in enumerate(info):
+        plan_words[p["plan"]] = p["plan_words"]
+        plan_its[p["plan"]] = max(plan_its.get(p["plan"], 0.0), its_i[i])
+    index_bytes = sum(4.0 * w * min(1.0, plan_its[q] / launches) for q, w in plan_words.items())
+    bytes_launch = float(np.dot(its_i, per_inst)) / launches + index_bytes
     flops_launch = float(np.dot(its_i, flops_inst)) / launches
     achieved = bytes_launch / (lu_ms * 1e-3) / 1e9
+    # the launch lasts as long as its longest chain of dependent levels
+    crit = max(p["levels"] + p["bwd_levels"] for p in info)
     traffic = profile_value("k_small_lu_pmc_c3_rings_256.md", "traffic_bytes_per_launch") if N == 256 else None
     roofline = {"bound": "hbm", "kernel": "k_small_lu", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
                 "launch_ms": round(lu_ms, 5), "launches": int(launches),
                 "bytes_per_launch": round(bytes_launch, 1),
+                "index_bytes_per_launch": round(index_bytes, 1),
                 "flops_per_launch": round(flops_launch, 1),
                 "fp64_TFLOPs": round(flops_launch / (lu_ms * 1e-3) / 1e12, 5),
                 "fp64_frac": round(flops_launch / (lu_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 7),
-                "critical_path_levels": int(max(p["levels"] + p["solve_levels"] for p in info)),
-                "ns_per_level": round(lu_ms * 1e6 / max(p["levels"] + p["solve_levels"] for p in info), 2),
+                "critical_path_levels": int(crit),
+                "ns_per_level": round(lu_ms * 1e6 / crit, 2),
                 "timing": "HIP events on the bundle stream around each k_small_lu launch, %d launches over "
                           "a replay of the whole 2 ns transient" % launches,
                 "traffic_source": ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the same "

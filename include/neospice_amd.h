@@ -233,11 +233,14 @@ int nsa_bundle_timing(nsa_bundle* b, double out[5]);
 /* 1 (default): device-driven iterations replay a captured HIP graph */
 int nsa_bundle_set_graph(nsa_bundle* b, int32_t on);
 /* pivot structure of an instance: plan index (-1 before its first
-   factorization), n, nnz(A), nnz(L+U), refactor levels, solve levels, and
+   factorization), n, nnz(A), nnz(L+U), levels of a factor-and-solve launch
+   (the refactor with the forward substitution folded in, then the
+   backward substitution), and
    the operation counts of one refactor (SparseMatrix::factor's left-looking
    loop, src/solver/factor.cpp:1010-1026) and of one forward + backward
-   solve (src/solver/solve.cpp:6-47) */
-int nsa_bundle_plan_info(nsa_bundle* b, int32_t inst, int64_t out[8]);
+   solve (src/solver/solve.cpp:6-47), and the int32 plan words such a
+   launch reads */
+int nsa_bundle_plan_info(nsa_bundle* b, int32_t inst, int64_t out[9]);
 
 /* ---- solver seam (ISolver) ---- */
 int nsa_lu_create(int32_t batch, int32_t n, const int32_t* col_ptr, const int32_t* row_idx,

@@ -623,7 +623,8 @@ class Bundle:
     def plan_info(self, i):
         """Pivot structure of instance i: plan id, sizes, levels and the
         operation counts of one refactor and one forward + backward solve."""
-        o = np.zeros(8, np.int64)
+        o = np.zeros(9, np.int64)
         check(lib().nsa_bundle_plan_info(self.h, int(i), o.ctypes.data_as(_lib.P_I64)))
-        keys = ("plan", "n", "nnz_a", "nnz_lu", "levels", "solve_levels", "flops_refactor", "flops_solve")
+        keys = ("plan", "n", "nnz_a", "nnz_lu", "levels", "bwd_levels", "flops_refactor", "flops_solve",
+                "plan_words")
         return {k: int(v) for k, v in zip(keys, o)}

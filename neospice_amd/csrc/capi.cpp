@@ -663,11 +663,11 @@ int nsa_bundle_dc_ladder(nsa_bundle* b, int32_t skip_direct, int32_t* converged,
         return 0;
     });
 }
-int nsa_bundle_plan_info(nsa_bundle* b, int32_t inst, int64_t out[8]) {
+int nsa_bundle_plan_info(nsa_bundle* b, int32_t inst, int64_t out[9]) {
     return guard([&] {
         if (inst < 0 || inst >= b->b->size()) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "instance out of range");
         const int p = b->b->plan_of(inst);
-        for (int k = 0; k < 8; ++k) out[k] = 0;
+        for (int k = 0; k < 9; ++k) out[k] = 0;
         out[0] = p;
         if (p < 0) return 0;
         const nsa::LuStructure& s = b->b->plans().structure(p);
@@ -675,7 +675,7 @@ int nsa_bundle_plan_info(nsa_bundle* b, int32_t inst, int64_t out[8]) {
         out[2] = s.nnz_a;
         out[3] = s.nlu();
         out[4] = b->b->plans().levels(p);
-        out[5] = b->b->plans().solve_levels(p);
+        out[5] = b->b->plans().bwd_levels(p);
         // left-looking operation counts of the Sparse 1.3 factor (factor.cpp:1010-1026)
         // and of the forward and backward substitutions (solve.cpp:6-47)
         int64_t f = s.n, off = 0;
@@ -688,6 +688,11 @@ int nsa_bundle_plan_info(nsa_bundle* b, int32_t inst, int64_t out[8]) {
         }
         out[6] = f;
         out[7] = 2 * off + s.n;
+        // plan words a factor-and-solve launch reads: slot positions, gmin
+        // positions, column segments and U counts, rows and columns of the
+        // positions, both permutations, and the level / task / backward block
+        const int32_t* w = b->b->plans().host_words().data() + b->b->plans().host_offsets()[p];
+        out[8] = (int64_t)s.nnz_a + 5 * s.n + 1 + 2 * (int64_t)s.nlu() + w[nsa::SH_HOTLEN];
         return 0;
     });
 }

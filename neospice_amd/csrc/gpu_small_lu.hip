@@ -51,10 +51,17 @@ struct PlanView {
     }
 };
 
-// A program row record (small_lu.hpp): target, term count, diagonal
-// position, offset of terms 2.., then position and source of terms 0 and 1.
+// LDS as bytes: plan records hold byte addresses into the instance's
+// values (factor value p at 8p, vector entry i at 8 (nlu + i), a zero at
+// 8 (nlu + n)), so an access needs no address arithmetic.
+__device__ inline double lds_ld(const char* L, int b) { return *reinterpret_cast<const double*>(L + b); }
+__device__ inline void lds_st(char* L, int b, double x) { *reinterpret_cast<double*>(L + b) = x; }
+
+// A program row record (small_lu.hpp): target, number of terms after the
+// first two, diagonal value, offset of those terms, then value and source
+// of terms 0 and 1 (the zero cell for both when a row has fewer terms).
 struct RowRec {
-    int t, cnt, dp, toff, p0, s0, p1, s1;
+    int t, ntail, dp, toff, p0, s0, p1, s1;
 };
 
 __device__ inline RowRec load_row(const int32_t* rows, int r) {
@@ -63,10 +70,17 @@ __device__ inline RowRec load_row(const int32_t* rows, int r) {
     return RowRec{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 }
 
+// One term of a row's subtraction chain.
+template <int OP>
+__device__ inline double term(double acc, double a, double b) {
+    if ((OP & OP_SKIPZERO) && b == 0.0) return acc;
+    return acc - a * b;
+}
+
 // The rest of a row's subtraction chain (terms 2..), operand loads issued
 // eight terms ahead.
 template <int OP>
-__device__ inline double row_tail(double acc, const int32_t* tm, int cnt, const double* v, const double* y) {
+__device__ inline double row_tail(double acc, const int32_t* tm, int cnt, const char* L) {
     int q = 0;
     for (; q + 8 <= cnt; q += 8) {
         int pos[8], src[8];
@@ -78,20 +92,21 @@ __device__ inline double row_tail(double acc, const int32_t* tm, int cnt, const 
         double a[8], b[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            a[u] = v[pos[u]];
-            b[u] = y[src[u]];
+            a[u] = lds_ld(L, pos[u]);
+            b[u] = lds_ld(L, src[u]);
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            if ((OP & OP_SKIPZERO) && b[u] == 0.0) continue;
-            acc = acc - a[u] * b[u];
-        }
+        for (int u = 0; u < 8; ++u) acc = term<OP>(acc, a[u], b[u]);
     }
-    for (; q < cnt; ++q) {
-        const double b = y[tm[2 * q + 1]];
-        if ((OP & OP_SKIPZERO) && b == 0.0) continue;
-        acc = acc - v[tm[2 * q]] * b;
-    }
+    for (; q < cnt; ++q) acc = term<OP>(acc, lds_ld(L, tm[2 * q]), lds_ld(L, tm[2 * q + 1]));
+    return acc;
+}
+
+// The diagonal operation of a program row.
+template <int OP>
+__device__ inline double row_diag(double acc, double dv) {
+    if (OP & OP_MUL_NZ) return (acc != 0.0) ? acc * dv : acc;
+    if (OP & OP_MUL_DIAG) return acc * dv;
     return acc;
 }
 
@@ -101,21 +116,14 @@ __device__ inline double row_tail(double acc, const int32_t* tm, int cnt, const 
 // loaded ahead by the caller.
 template <int OP, bool STAGED>
 __device__ inline void run_row(const PlanView<STAGED>& pv, const RowRec& rec, double yt, double a0,
-                               double a1, double dv, const double* v, double* y) {
-    // both inline sources are loaded together, unconditionally (unused
-    // slots name entry 0), so the row waits for one LDS round trip
-    const double b0 = y[rec.s0];
-    const double b1 = y[rec.s1];
-    double acc = yt;
-    if (rec.cnt >= 1 && !((OP & OP_SKIPZERO) && b0 == 0.0)) acc = acc - a0 * b0;
-    if (rec.cnt >= 2 && !((OP & OP_SKIPZERO) && b1 == 0.0)) acc = acc - a1 * b1;
-    if (rec.cnt > 2) acc = row_tail<OP>(acc, pv.at(rec.toff), rec.cnt - 2, v, y);
-    if (OP & OP_MUL_NZ) {
-        if (acc != 0.0) acc = acc * dv;
-    } else if (OP & OP_MUL_DIAG) {
-        acc = acc * dv;
-    }
-    y[rec.t] = acc;
+                               double a1, double dv, char* L) {
+    // both inline sources are loaded together, so the row waits for one
+    // LDS round trip
+    const double b0 = lds_ld(L, rec.s0);
+    const double b1 = lds_ld(L, rec.s1);
+    double acc = term<OP>(term<OP>(yt, a0, b0), a1, b1);
+    if (rec.ntail > 0) acc = row_tail<OP>(acc, pv.at(rec.toff), rec.ntail, L);
+    lds_st(L, rec.t, row_diag<OP>(acc, dv));
 }
 
 // What a level needs of its first row, loaded ahead: the record and the
@@ -125,6 +133,13 @@ struct RowAhead {
     double yt, a0, a1, dv;
 };
 
+__device__ inline void fetch_ahead(RowAhead& a, const char* L) {
+    a.yt = lds_ld(L, a.rec.t);
+    a.a0 = lds_ld(L, a.rec.p0);
+    a.a1 = lds_ld(L, a.rec.p1);
+    a.dv = lds_ld(L, a.rec.dp);
+}
+
 // Level l of a program: runs the level's rows, then fetches what level l+1
 // needs.  `cur` holds this level's ahead data, `nxt` receives the next
 // level's; callers alternate two RowAhead variables so that no prefetched
@@ -132,32 +147,26 @@ struct RowAhead {
 template <int OP, bool STAGED>
 __device__ inline void program_level(const PlanView<STAGED>& pv, const int32_t* lev, const int32_t* rows,
                                      int nl, int last, int l, int& r0, int& r1, int& r2,
-                                     const RowAhead& cur, RowAhead& nxt, const double* v, double* y,
-                                     int lane) {
+                                     const RowAhead& cur, RowAhead& nxt, char* L, int lane) {
     const int r3 = (l + 3 <= nl) ? lev[l + 3] : r2;
     nxt.rec = load_row(rows, min(r1 + lane, last));
-    if (r0 + lane < r1) run_row<OP, STAGED>(pv, cur.rec, cur.yt, cur.a0, cur.a1, cur.dv, v, y);
+    if (r0 + lane < r1) run_row<OP, STAGED>(pv, cur.rec, cur.yt, cur.a0, cur.a1, cur.dv, L);
     for (int r = r0 + lane + kW; r < r1; r += kW) {
         const RowRec rec = load_row(rows, r);
-        run_row<OP, STAGED>(pv, rec, y[rec.t], v[rec.p0], v[rec.p1], v[rec.dp], v, y);
+        run_row<OP, STAGED>(pv, rec, lds_ld(L, rec.t), lds_ld(L, rec.p0), lds_ld(L, rec.p1),
+                            lds_ld(L, rec.dp), L);
     }
     __syncthreads();
     r0 = r1;
     r1 = r2;
     r2 = r3;
-    nxt.yt = y[nxt.rec.t];
-    nxt.a0 = v[nxt.rec.p0];
-    nxt.a1 = v[nxt.rec.p1];
-    nxt.dv = v[nxt.rec.dp];
+    fetch_ahead(nxt, L);
 }
 
+// Levels [0, nl) of `lev` run one row per lane, a barrier after each.
 template <int OP, bool STAGED>
-__device__ inline void run_program(const PlanView<STAGED>& pv, int prog, const double* v, double* y,
-                                   int lane) {
-    const int32_t* P = pv.P;
-    const int nl = P[SH_PROG + 3 * prog];
-    const int32_t* lev = pv.at(P[SH_PROG + 3 * prog + 1]);
-    const int32_t* rows = pv.at(P[SH_PROG + 3 * prog + 2]);
+__device__ inline void run_levels(const PlanView<STAGED>& pv, const int32_t* lev, const int32_t* rows,
+                                  int nl, char* L, int lane) {
     if (nl <= 0) return;
     const int last = lev[nl] - 1;  // rows are never empty (one per target)
     // Level bounds [r0, r1) and the next level's end r2.  Each level's
@@ -170,39 +179,123 @@ __device__ inline void run_program(const PlanView<STAGED>& pv, int prog, const d
     int r2 = (nl > 1) ? lev[2] : r1;
     RowAhead A, B;
     A.rec = load_row(rows, min(r0 + lane, last));
-    A.yt = y[A.rec.t];
-    A.a0 = v[A.rec.p0];
-    A.a1 = v[A.rec.p1];
-    A.dv = v[A.rec.dp];
+    fetch_ahead(A, L);
     int l = 0;
     for (; l + 1 < nl; l += 2) {
-        program_level<OP, STAGED>(pv, lev, rows, nl, last, l, r0, r1, r2, A, B, v, y, lane);
-        program_level<OP, STAGED>(pv, lev, rows, nl, last, l + 1, r0, r1, r2, B, A, v, y, lane);
+        program_level<OP, STAGED>(pv, lev, rows, nl, last, l, r0, r1, r2, A, B, L, lane);
+        program_level<OP, STAGED>(pv, lev, rows, nl, last, l + 1, r0, r1, r2, B, A, L, lane);
     }
-    if (l < nl) program_level<OP, STAGED>(pv, lev, rows, nl, last, l, r0, r1, r2, A, B, v, y, lane);
+    if (l < nl) program_level<OP, STAGED>(pv, lev, rows, nl, last, l, r0, r1, r2, A, B, L, lane);
+}
+
+// A row of a serial phase with its operands.
+struct SerialRow {
+    RowRec rec;
+    double yt, a0, a1, dv, b0, b1;
+};
+
+// A row index the compiler must treat as per-lane: loads through it stay in
+// vector registers.  (A wave-uniform index would let it move the loaded
+// record into scalar registers right after the load, which waits for the
+// load and undoes the prefetch.)
+__device__ inline int lane_index(int r) {
+    asm volatile("" : "+v"(r));
+    return r;
+}
+
+__device__ inline void serial_fetch(SerialRow& s, const char* L) {
+    s.yt = lds_ld(L, s.rec.t);
+    s.a0 = lds_ld(L, s.rec.p0);
+    s.a1 = lds_ld(L, s.rec.p1);
+    s.dv = lds_ld(L, s.rec.dp);
+    s.b0 = lds_ld(L, s.rec.s0);
+    s.b1 = lds_ld(L, s.rec.s1);
+}
+
+// Row r of a serial phase [r, re): fetches the record of row r + 2 and
+// the operands of row r + 1, then computes row r.  Operands fetched one
+// row ahead were read before row r's result was written, so a source equal
+// to the previous row's target takes the value passed on in registers;
+// every older target was written before the fetch (a wave's LDS accesses
+// complete in order).  Callers rotate three SerialRow variables, so that
+// no register with a load in flight is copied.
+template <int OP, bool STAGED>
+__device__ inline void serial_row(const PlanView<STAGED>& pv, const int32_t* rows, int r, int re,
+                                  const SerialRow& cur, SerialRow& nxt, SerialRow& far, int& tprev,
+                                  double& prev, char* L) {
+    far.rec = load_row(rows, lane_index(min(r + 2, re - 1)));
+    serial_fetch(nxt, L);
+    const double b0 = (cur.rec.s0 == tprev) ? prev : cur.b0;
+    const double b1 = (cur.rec.s1 == tprev) ? prev : cur.b1;
+    double acc = term<OP>(term<OP>(cur.yt, cur.a0, b0), cur.a1, b1);
+    if (cur.rec.ntail > 0) acc = row_tail<OP>(acc, pv.at(cur.rec.toff), cur.rec.ntail, L);
+    acc = row_diag<OP>(acc, cur.dv);
+    lds_st(L, cur.rec.t, acc);
+    tprev = cur.rec.t;
+    prev = acc;
+}
+
+// Rows [rb, re) in order on lane 0 (a chain of one-row levels).
+template <int OP, bool STAGED>
+__device__ inline void run_serial(const PlanView<STAGED>& pv, const int32_t* rows, int rb, int re, char* L,
+                                  int lane) {
+    if (lane == 0 && rb < re) {
+        SerialRow A, B, C;
+        A.rec = load_row(rows, lane_index(rb));
+        B.rec = load_row(rows, lane_index(min(rb + 1, re - 1)));
+        serial_fetch(A, L);
+        int tprev = -1;
+        double prev = 0.0;
+        int r = rb;
+        for (; r + 3 <= re; r += 3) {
+            serial_row<OP, STAGED>(pv, rows, r, re, A, B, C, tprev, prev, L);
+            serial_row<OP, STAGED>(pv, rows, r + 1, re, B, C, A, tprev, prev, L);
+            serial_row<OP, STAGED>(pv, rows, r + 2, re, C, A, B, tprev, prev, L);
+        }
+        if (r < re) serial_row<OP, STAGED>(pv, rows, r, re, A, B, C, tprev, prev, L);
+        if (r + 1 < re) serial_row<OP, STAGED>(pv, rows, r + 1, re, B, C, A, tprev, prev, L);
+    }
+    __syncthreads();
+}
+
+// Program `prog` phase by phase.
+template <int OP, bool STAGED>
+__device__ inline void run_program(const PlanView<STAGED>& pv, int prog, char* L, int lane) {
+    const int32_t* P = pv.P;
+    const int nl = P[SH_PROG + 4 * prog];
+    if (nl <= 0) return;
+    const int32_t* lev = pv.at(P[SH_PROG + 4 * prog + 1]);
+    const int32_t* rows = pv.at(P[SH_PROG + 4 * prog + 2]);
+    const int32_t* ph = pv.at(P[SH_PROG + 4 * prog + 3]);
+    const int nph = ph[0];
+    for (int q = 0; q < nph; ++q) {
+        const int4 h = reinterpret_cast<const int4*>(ph)[1 + q];
+        if (h.x == PH_SERIAL) run_serial<OP, STAGED>(pv, rows, lev[h.y], lev[h.z], L, lane);
+        else run_levels<OP, STAGED>(pv, lev + h.y, rows, h.z - h.y, L, lane);
+    }
 }
 
 // One task: x_t = x_t - (x_kj * (1 / x_kk)) * x_ik.  A forward-substitution
-// task (pivot stored as -(pos + 1)) is skipped when its y_k is zero, as the
-// forward loop skips a zero entry (solve.cpp:22-23).
-__device__ inline void run_task(const int4 tk, double* v) {
+// task (pivot stored as -(address + 1)) is skipped when its y_k is zero, as
+// the forward loop skips a zero entry (solve.cpp:22-23).
+__device__ inline void run_task(const int4 tk, char* L) {
     const bool fwd = tk.w < 0;
     const int piv = fwd ? -tk.w - 1 : tk.w;
-    const double xkk = v[piv], xkj = v[tk.y], xik = v[tk.z], xt = v[tk.x];
+    const double xkk = lds_ld(L, piv), xkj = lds_ld(L, tk.y), xik = lds_ld(L, tk.z), xt = lds_ld(L, tk.x);
     const double u = xkj * (1.0 / xkk);
     const double nv = xt - u * xik;
-    v[tk.x] = (fwd && xkj == 0.0) ? xt : nv;
+    lds_st(L, tk.x, (fwd && xkj == 0.0) ? xt : nv);
 }
 
 // Level l of the refactor: the level's tasks, then the next level's first
 // task (alternated like RowAhead).
 __device__ inline void factor_level(const int32_t* lev, const int4* task, int nlev, int last, int l,
-                                    int& t0, int& t1, int& t2, const int4& cur, int4& nxt, double* v,
+                                    int& t0, int& t1, int& t2, const int4& cur, int4& nxt, char* L,
                                     int lane) {
     const int t3 = (l + 3 <= nlev) ? lev[l + 3] : t2;
     nxt = task[min(t1 + lane, last)];
-    if (t0 + lane < t1) run_task(cur, v);
-    for (int t = t0 + lane + kW; t < t1; t += kW) run_task(task[t], v);
+    if (t0 + lane < t1) run_task(cur, L);
+    for (int t = t0 + lane + kW; t < t1; t += kW) run_task(task[t], L);
     __syncthreads();
     t0 = t1;
     t1 = t2;
@@ -228,16 +321,19 @@ __global__ __launch_bounds__(kW) void k_small_lu(const int32_t* __restrict__ wor
     const int32_t* __restrict__ P = words + offs[d.plan];
     const int n = P[SH_N], nlu = P[SH_NLU];
     // LDS: the factor values, the vector right after them (the forward
-    // substitution's tasks address entry i as value nlu + i), the reciprocal
-    // pivots, then the staged hot block on a 16-byte boundary
+    // substitution's tasks address entry i as value nlu + i), a zero (plan
+    // records name it for absent terms), the reciprocal pivots, then the
+    // staged hot block on a 16-byte boundary
     double* v = lds;
     double* y = lds + nlu;
-    double* dr = lds + max_nlu + max_n;
+    char* L = reinterpret_cast<char*>(lds);
+    double* dr = lds + max_nlu + max_n + 1;
+    if (lane == 0) y[n] = 0.0;
     PlanView<STAGED> pv{P, nullptr, P[SH_HOT]};
     if (STAGED) {
         // the hot block, 16 bytes per lane per step (sections start on
         // 4-word boundaries, so the block is whole int4s)
-        int32_t* hw = reinterpret_cast<int32_t*>(lds + ((max_nlu + 2 * max_n + 1) & ~1));
+        int32_t* hw = reinterpret_cast<int32_t*>(lds + ((max_nlu + 2 * max_n + 2) & ~1));
         const int4* src = reinterpret_cast<const int4*>(P + pv.hot0);
         int4* dst = reinterpret_cast<int4*>(hw);
         const int n4 = (P[SH_HOTLEN] + 3) / 4;
@@ -300,10 +396,10 @@ __global__ __launch_bounds__(kW) void k_small_lu(const int32_t* __restrict__ wor
             int4 ta = task[min(t0 + lane, last)], tb;
             int l = 0;
             for (; l + 1 < nlev; l += 2) {
-                factor_level(lev, task, nlev, last, l, t0, t1, t2, ta, tb, v, lane);
-                factor_level(lev, task, nlev, last, l + 1, t0, t1, t2, tb, ta, v, lane);
+                factor_level(lev, task, nlev, last, l, t0, t1, t2, ta, tb, L, lane);
+                factor_level(lev, task, nlev, last, l + 1, t0, t1, t2, tb, ta, L, lane);
             }
-            if (l < nlev) factor_level(lev, task, nlev, last, l, t0, t1, t2, ta, tb, v, lane);
+            if (l < nlev) factor_level(lev, task, nlev, last, l, t0, t1, t2, ta, tb, L, lane);
         }
         // checks (the first failing column, as the reference stops there)
         int bad = 0x7fffffff, code = FAIL_NONE;
@@ -371,19 +467,19 @@ __global__ __launch_bounds__(kW) void k_small_lu(const int32_t* __restrict__ wor
     if (!(mode & (SLU_SOLVE | SLU_SOLVE_T))) return;
     if (fused) {
         // forward substitution done with the factor: the backward one
-        run_program<OP_PLAIN, STAGED>(pv, 1, v, y, lane);
+        run_program<OP_PLAIN, STAGED>(pv, 1, L, lane);
     } else {
         __syncthreads();
         for (int k = lane; k < n; k += kW) y[k] = rhs[d.xoff + in_perm[k]];
         __syncthreads();
         if (!tr) {
-            run_program<OP_SKIPZERO | OP_MUL_NZ, false>(PlanView<false>{P, nullptr, 0}, 0, v, y, lane);
-            run_program<OP_PLAIN, STAGED>(pv, 1, v, y, lane);
+            run_program<OP_SKIPZERO | OP_MUL_NZ, false>(PlanView<false>{P, nullptr, 0}, 0, L, lane);
+            run_program<OP_PLAIN, STAGED>(pv, 1, L, lane);
         } else {
             // the transposed programs are not in the hot block
             const PlanView<false> g{P, nullptr, 0};
-            run_program<OP_SKIPZERO, false>(g, 2, v, y, lane);
-            run_program<OP_MUL_DIAG, false>(g, 3, v, y, lane);
+            run_program<OP_SKIPZERO, false>(g, 2, L, lane);
+            run_program<OP_MUL_DIAG, false>(g, 3, L, lane);
         }
     }
     for (int k = lane; k < n; k += kW) x[d.xoff + out_perm[k]] = y[k];
@@ -398,8 +494,8 @@ void launch_small_lu(const SmallLuPlans& plans, const SluInst* d_inst, int ninst
     if (ninst <= 0) return;
     const int max_nlu = plans.max_nlu(), max_n = plans.max_n();
     // the hot block starts on a 16-byte boundary after the values, the
-    // vector and the reciprocal pivots
-    const size_t base = sizeof(double) * (((size_t)max_nlu + 2 * (size_t)max_n + 1) & ~(size_t)1);
+    // vector, the zero and the reciprocal pivots
+    const size_t base = sizeof(double) * (((size_t)max_nlu + 2 * (size_t)max_n + 2) & ~(size_t)1);
     const size_t hot = sizeof(int32_t) * (((size_t)plans.max_hot() + 3) / 4 * 4);
     const int stage = (base + hot <= kLdsLimit) ? 1 : 0;
     const size_t lds = base + (stage ? hot : 0);

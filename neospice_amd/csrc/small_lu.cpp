@@ -86,21 +86,49 @@ void align4(std::vector<int32_t>& w) {
     while (w.size() % 4) w.push_back(0);
 }
 
-// A triangular program as level-ordered row records and terms.  A row
-// record is 8 words: target, term count, diagonal position (or -1), offset
-// of terms 2.. (plan-relative), then the LU position and source of terms 0
-// and 1 (the rows of circuit matrices are mostly that short, so a level
-// needs no term fetch before its operand loads).
-void append_program(const TriProgram& t, std::vector<int32_t>& w, int32_t base, int32_t* hdr3) {
+// A triangular program as level-ordered row records and terms, in LDS byte
+// addresses of the launch's layout (factor value p at 8p, vector entry i at
+// 8 (nlu + i), a zero at 8 (nlu + n)).  A row record is 8 words: target,
+// number of terms after the first two, diagonal value (the zero when the
+// program has none), plan offset of those terms, then value and source of
+// terms 0 and 1 (the zero for both when a row has fewer terms: x - 0 * 0 is
+// x, so every row runs two terms without a branch).  Circuit rows are
+// mostly that short, so a level needs no term fetch before its operand
+// loads.
+void append_program(const TriProgram& t, int nlu, int n, std::vector<int32_t>& w, int32_t base,
+                    int32_t* hdr4) {
+    const int32_t zero_b = 8 * (nlu + n);
+    auto vb = [&](int pos) { return (int32_t)(8 * pos); };
+    auto yb = [&](int i) { return (int32_t)(8 * (nlu + i)); };
     const int nl = t.nlevels();
-    hdr3[0] = nl;
+    hdr4[0] = nl;
     align4(w);
-    hdr3[1] = (int32_t)w.size() - base;
+    hdr4[1] = (int32_t)w.size() - base;
     for (int l = 0; l <= nl; ++l) w.push_back(t.level_ptr[l]);
     align4(w);
+    // phases: maximal runs of at least kSluMinSerialRun one-row levels are
+    // serial, what lies between them parallel
+    std::vector<int32_t> ph;
+    for (int l = 0; l < nl;) {
+        int e = l;
+        while (e < nl && t.level_ptr[e + 1] - t.level_ptr[e] == 1) ++e;
+        if (e - l >= kSluMinSerialRun) {
+            ph.insert(ph.end(), {PH_SERIAL, l, e, 0});
+            l = e;
+            continue;
+        }
+        // a short run or a wider level: part of a parallel phase
+        const int q = std::max(e, l + 1);
+        if (!ph.empty() && ph[ph.size() - 4] == PH_PARALLEL) ph[ph.size() - 2] = q;
+        else ph.insert(ph.end(), {PH_PARALLEL, l, q, 0});
+        l = q;
+    }
+    hdr4[3] = (int32_t)w.size() - base;
+    w.insert(w.end(), {(int32_t)(ph.size() / 4), 0, 0, 0});
+    w.insert(w.end(), ph.begin(), ph.end());
     const int nrows = (int)t.order.size();
     const int32_t rows_at = (int32_t)w.size();
-    hdr3[2] = rows_at - base;
+    hdr4[2] = rows_at - base;
     w.resize(w.size() + 8 * (size_t)nrows);
     const int32_t term_base = (int32_t)w.size() - base;
     std::vector<int32_t> terms;
@@ -108,20 +136,18 @@ void append_program(const TriProgram& t, std::vector<int32_t>& w, int32_t base, 
         const int tg = t.order[r];
         const int a = t.term_ptr[tg], b = t.term_ptr[tg + 1];
         int32_t* rec = &w[rows_at + 8 * (size_t)r];
-        rec[0] = tg;
-        rec[1] = b - a;
-        rec[2] = t.dpos.empty() ? -1 : t.dpos[tg];
+        rec[0] = yb(tg);
+        rec[1] = std::max(0, b - a - 2);
+        rec[2] = (t.dpos.empty() || t.dpos[tg] < 0) ? zero_b : vb(t.dpos[tg]);
         rec[3] = term_base + (int32_t)terms.size();
-        // unused inline slots point at position 0 / source 0 (always valid)
-        rec[4] = rec[5] = rec[6] = rec[7] = 0;
-        if (rec[2] < 0) rec[2] = 0;
+        rec[4] = rec[5] = rec[6] = rec[7] = zero_b;
         for (int q = a; q < b; ++q) {
             if (q - a < 2) {
-                rec[4 + 2 * (q - a)] = t.term_pos[q];
-                rec[5 + 2 * (q - a)] = t.term_src[q];
+                rec[4 + 2 * (q - a)] = vb(t.term_pos[q]);
+                rec[5 + 2 * (q - a)] = yb(t.term_src[q]);
             } else {
-                terms.push_back(t.term_pos[q]);
-                terms.push_back(t.term_src[q]);
+                terms.push_back(vb(t.term_pos[q]));
+                terms.push_back(yb(t.term_src[q]));
             }
         }
     }
@@ -162,15 +188,13 @@ int SmallLuPlans::add(const LuStructure& s) {
     put(SH_CPERM, s.col_perm);
     TriProgram fwd, bwd, tfwd, tbwd;
     build_solve_programs(s, fwd, bwd, tfwd, tbwd, 0);
-    int32_t h3[3];
+    int32_t h4[4];
     // the forward program (solve-only launches) and the transposed ones
     const TriProgram* cold[3] = {&fwd, &tfwd, &tbwd};
     const int cold_id[3] = {0, 2, 3};
     for (int c = 0; c < 3; ++c) {
-        append_program(*cold[c], words_, base, h3);
-        H(SH_PROG + 3 * cold_id[c]) = h3[0];
-        H(SH_PROG + 3 * cold_id[c] + 1) = h3[1];
-        H(SH_PROG + 3 * cold_id[c] + 2) = h3[2];
+        append_program(*cold[c], s.nlu(), n, words_, base, h4);
+        for (int q = 0; q < 4; ++q) H(SH_PROG + 4 * cold_id[c] + q) = h4[q];
     }
     // the hot block: what the level loops of a factor-and-solve launch read
     // (the task levels with the forward substitution folded in, and the
@@ -182,17 +206,22 @@ int SmallLuPlans::add(const LuStructure& s) {
     align4(words_);
     const int32_t hot0 = (int32_t)words_.size() - base;
     put(SH_LEV, lev_ptr);
+    // tasks in LDS byte addresses (a forward task's pivot as -(address + 1))
+    for (size_t q = 0; q < tasks.size(); q += 4) {
+        for (int f = 0; f < 3; ++f) tasks[q + f] *= 8;
+        const int32_t pv = tasks[q + 3];
+        tasks[q + 3] = pv >= 0 ? 8 * pv : -(8 * (-pv - 1) + 1);
+    }
     put(SH_TASK, tasks);
-    append_program(bwd, words_, base, h3);
-    H(SH_PROG + 3) = h3[0];
-    H(SH_PROG + 4) = h3[1];
-    H(SH_PROG + 5) = h3[2];
+    append_program(bwd, s.nlu(), n, words_, base, h4);
+    for (int q = 0; q < 4; ++q) H(SH_PROG + 4 + q) = h4[q];
     align4(words_);
     H(SH_HOT) = hot0;
     H(SH_HOTLEN) = (int32_t)words_.size() - base - hot0;
     st_.push_back(s);
     lev_.push_back(H(SH_NLEV));
     slev_.push_back(fwd.nlevels() + bwd.nlevels());
+    blev_.push_back(bwd.nlevels());
     max_nlu_ = std::max(max_nlu_, s.nlu());
     max_n_ = std::max(max_n_, n);
     max_hot_ = std::max(max_hot_, (int)words_[base + SH_HOTLEN]);

@@ -48,23 +48,32 @@ enum SluHdr : int32_t {
     SH_APOS,    // nnz_a: position receiving A slot p
     SH_GMIN,    // n: position receiving diag_gmin, or -1
     SH_LEV,     // nlev+1 task offsets (relative to SH_TASK, in tasks)
-    SH_TASK,    // tasks, 4 words each: target, U(k,j), L(i,k), pivot (k,k) positions;
-                // forward-substitution tasks: y_i, y_k (at nlu + index), L(i,k), -(pivot + 1)
+    SH_TASK,    // tasks, 4 words each, LDS byte addresses (value p at 8p, vector entry
+                // i at 8 (nlu + i)): target, U(k,j), L(i,k), pivot (k,k); forward-
+                // substitution tasks: y_i, y_k, L(i,k), -(pivot address + 1)
     SH_CPTR,    // n+1 column segments
     SH_NU,      // n: U entries per column
     SH_ROWS,    // nlu: pivot-space row of each position
     SH_PCOL,    // nlu: pivot-space column of each position
     SH_RPERM, SH_CPERM,
-    SH_PROG,    // 4 programs (fwd, bwd, tfwd, tbwd), SH_PROG + 3*p: nlev, levptr off, rows off
-    SH_HOT = SH_PROG + 12,  // first word and length of the block a factor-and-solve
+    SH_PROG,    // 4 programs (fwd, bwd, tfwd, tbwd), SH_PROG + 4*p: nlev, levptr off,
+                // rows off, phases off
+    SH_HOT = SH_PROG + 16,  // first word and length of the block a factor-and-solve
     SH_HOTLEN,              // launch reads in its level loops (LEV, TASK, bwd)
     SH_WORDS,
 };
-// Program rows: 8 words each (target, term count, diagonal position (0
-// when the program has no diagonal operation), offset of the terms after
-// the first two, then LU position and source of terms 0 and 1); terms: 2
-// words each (LU position, source).  The program op is fixed by the form:
+// Program rows: 8 words each, LDS byte addresses (value p at 8p, vector
+// entry i at 8 (nlu + i), a zero at 8 (nlu + n)): target, number of terms
+// after the first two, diagonal value (the zero when none), offset of those
+// terms, then value and source of terms 0 and 1 (both the zero when
+// absent); terms: 2 words each (value, source).  The program op is fixed by the form:
 // fwd SKIPZERO|MUL_NZ, bwd PLAIN, tfwd SKIPZERO, tbwd MUL_DIAG.
+// Phases: 4 words of header (count), then 4 words each (kind, first level,
+// end level, 0).  A parallel phase runs its levels one row per lane; a
+// serial phase is a run of one-row levels (a dependency chain) that one
+// lane runs in order, the previous row's result passed on in registers.
+enum SluPhase : int32_t { PH_PARALLEL = 0, PH_SERIAL = 1 };
+constexpr int kSluMinSerialRun = 4;
 
 // Kinds per instance (the same codes the interleaved kernels use).
 enum SluKind : int32_t { SLU_REFACTOR = 0, SLU_VERIFY = 1, SLU_TRUSTED = 2, SLU_SKIP = 3 };
@@ -92,6 +101,9 @@ public:
     // Refactor levels and the solve programs' levels of a plan (statistics).
     int levels(int plan) const { return lev_[plan]; }
     int solve_levels(int plan) const { return slev_[plan]; }
+    // Backward-substitution levels: a factor-and-solve launch runs
+    // levels() (the forward substitution folded in) and then these.
+    int bwd_levels(int plan) const { return blev_[plan]; }
     // Copies new plans to the device (stream-ordered); returns the device
     // words and offsets valid until the next sync() that adds plans.
     void sync(hipStream_t s);
@@ -104,7 +116,7 @@ public:
 private:
     std::vector<LuStructure> st_;
     std::vector<int32_t> words_, off_;
-    std::vector<int> lev_, slev_;
+    std::vector<int> lev_, slev_, blev_;
     int max_nlu_ = 0, max_n_ = 0, max_hot_ = 0;
     int32_t* d_words_ = nullptr;
     int32_t* d_off_ = nullptr;
