@@ -253,6 +253,13 @@ int nsa_lu_refactorize(nsa_lu* s, const double* values, double diag_gmin, int32_
 /* rhs: host, n x B, overwritten with the solution. */
 int nsa_lu_solve(nsa_lu* s, double* rhs);
 int nsa_lu_solve_transposed(nsa_lu* s, double* rhs);
+/* nsa_lu_solve without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * serving loops).  The staging vector lives in a 16 MiB device array the
+ * library holds on each GPU until the process ends, so n x batch may not
+ * exceed 2097152 values (larger: NSA_ERR_ARGUMENT).  Calls on one GPU share
+ * that array: calls that overlap on one GPU race. */
+int nsa_lu_solve_noalloc(nsa_lu* s, double* rhs);
 const char* nsa_lu_name(nsa_lu* s);
 /* Pivot order and fill: n, nnz(L+U), levels of refactor, levels of solve. */
 int nsa_lu_info(nsa_lu* s, int32_t out[4]);

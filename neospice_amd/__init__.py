@@ -442,6 +442,13 @@ class Solver:
         check(lib().nsa_lu_solve_transposed(self.h, f64p(b)))
         return b
 
+    def solve_noalloc(self, rhs):
+        """solve() through nsa_lu_solve_noalloc: no device allocation (the
+        library's static device array stages the vectors)."""
+        b = as_f64(np.array(rhs, np.float64).reshape(self.n, self.B))
+        check(lib().nsa_lu_solve_noalloc(self.h, f64p(b)))
+        return b
+
     @property
     def name(self):
         return lib().nsa_lu_name(self.h).decode()

@@ -87,6 +87,7 @@ _SIGS = {
     "nsa_lu_refactorize": (C.c_int, [VP, P_F64, C.c_double, P_I32]),
     "nsa_lu_solve": (C.c_int, [VP, P_F64]),
     "nsa_lu_solve_transposed": (C.c_int, [VP, P_F64]),
+    "nsa_lu_solve_noalloc": (C.c_int, [VP, P_F64]),
     "nsa_lu_name": (C.c_char_p, [VP]),
     "nsa_lu_info": (C.c_int, [VP, P_I32]),
     "nsa_lu_structure": (C.c_int, [VP, P_I32, P_I32, P_I32, P_I32]),

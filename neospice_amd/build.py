@@ -34,6 +34,7 @@ SOURCES = [
     "gpu_adjoint.hip",
     "gpu_small_lu.hip",
     "gpu_bundle.hip",
+    "noalloc.hip",
 ]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]

@@ -12,6 +12,7 @@
 #include "engine.hpp"
 #include "lu_cpu.hpp"
 #include "bundle.hpp"
+#include "noalloc.hpp"
 
 struct nsa_circuit {
     nsa::Circuit c;
@@ -435,6 +436,25 @@ int nsa_lu_solve(nsa_lu* s, double* rhs) {
 }
 int nsa_lu_solve_transposed(nsa_lu* s, double* rhs) {
     return guard([&] { return lu_solve(s, rhs, true); });
+}
+int nsa_lu_solve_noalloc(nsa_lu* s, double* rhs) {
+    return guard([&] {
+        const size_t bytes = sizeof(double) * (size_t)s->n * s->B;
+        if (bytes > nsa::kNoAllocScratchBytes)
+            throw nsa::Error(nsa::NSA_ERR_ARGUMENT,
+                             "nsa_lu_solve_noalloc: n x batch = " + std::to_string((size_t)s->n * s->B) +
+                                 " values exceed the " + std::to_string(nsa::kNoAllocScratchBytes >> 20) +
+                                 " MiB device scratch");
+        if (!s->lu->factored()) throw nsa::Error(nsa::NSA_ERR_LOGIC, "nsa_lu_solve_noalloc: numeric() not called");
+        // the staging vector is the library's static device array (its
+        // address on this device), not the handle's buffer
+        double* d = nsa::noalloc_scratch();
+        NSA_HIP(hipMemcpyAsync(d, rhs, bytes, hipMemcpyHostToDevice, s->stream));
+        s->lu->solve(d);
+        NSA_HIP(hipMemcpyAsync(rhs, d, bytes, hipMemcpyDeviceToHost, s->stream));
+        NSA_HIP(hipStreamSynchronize(s->stream));
+        return 0;
+    });
 }
 const char* nsa_lu_name(nsa_lu* s) { return s->lu->name(); }
 int nsa_lu_info(nsa_lu* s, int32_t out[4]) {
