@@ -81,6 +81,12 @@ int nsa_circuit_m(nsa_circuit* c, const char* name, int32_t d, int32_t g, int32_
  * doubles, in the order documented in neospice_amd/csrc/bsim4.hpp (B4Param). */
 #define NSA_BSIM4_NPARAM 110
 int nsa_circuit_bsim4_params(const nsa_circuit* c, int32_t dev, double* out);
+/* The same record (val) with its derivatives with respect to the
+ * instance's W (dw) and L (dl), NSA_BSIM4_NPARAM doubles each: the
+ * size-dependent path of BSIM4v7temp (bsim4v7_temp.cpp:416-419 onwards)
+ * differentiated in forward mode, for the W/L adjoint. */
+int nsa_circuit_bsim4_param_derivs(const nsa_circuit* c, int32_t dev, double* val, double* dw,
+                                   double* dl);
 /* Circuit::finalize (src/core/circuit.cpp:201-283). */
 int nsa_circuit_finalize(nsa_circuit* c);
 /* n (unknowns), nodes, nnz, states, is_linear */
@@ -196,8 +202,10 @@ int nsa_bundle_dc(nsa_bundle* b, int32_t mode, int32_t* converged, int32_t* iter
  * MODETRANOP, then gmin stepping (src/core/convergence.cpp:216-346) for the
  * instances that failed.  stage per instance: 0 direct, 1 gmin stepping,
  * -1 not converged (source stepping and pseudo-transient are refused).
- * skip_direct = 1 runs gmin stepping alone on every instance. */
-int nsa_bundle_dc_ladder(nsa_bundle* b, int32_t skip_direct, int32_t* converged, int32_t* iterations,
+ * flags: bit 0 runs gmin stepping alone on every instance (no direct
+ * attempt); bit 1 uses solve_dc's MODEDCOP (src/core/dc.cpp:104-150) in
+ * place of MODETRANOP. */
+int nsa_bundle_dc_ladder(nsa_bundle* b, int32_t flags, int32_t* converged, int32_t* iterations,
                          int32_t* stage);
 int nsa_bundle_get_solution(nsa_bundle* b, double* x);
 int nsa_bundle_set_solution(nsa_bundle* b, const double* x);
@@ -241,6 +249,18 @@ int nsa_bundle_set_graph(nsa_bundle* b, int32_t on);
    solve (src/solver/solve.cpp:6-47), and the int32 plan words such a
    launch reads */
 int nsa_bundle_plan_info(nsa_bundle* b, int32_t inst, int64_t out[9]);
+/* Device offsets (count + 1): instance i's device d is entry doff[i] + d of
+ * the per-device outputs below. */
+int nsa_bundle_device_offsets(nsa_bundle* b, int32_t* doff);
+/* DC adjoint at every instance's last operating point (the quantities of
+ * .sens, src/core/sens.cpp:105-186, by the adjoint method): lambda (total
+ * unknowns, packed) solves J^T lambda = dL/dx with the factors of the last
+ * Newton iteration; g_value = dL/d(value) per device for R (ohms), C (zero
+ * in DC), V and I; g_w, g_l = dL/dW and dL/dL of BSIM4v7 devices through the
+ * size-dependent parameters (bsim4v7_temp.cpp:416-419 onwards).  Outputs may
+ * be NULL. */
+int nsa_bundle_dc_adjoint(nsa_bundle* b, const double* dLdx, double* lambda, double* g_value, double* g_w,
+                          double* g_l);
 
 /* ---- solver seam (ISolver) ---- */
 int nsa_lu_create(int32_t batch, int32_t n, const int32_t* col_ptr, const int32_t* row_idx,

@@ -545,14 +545,36 @@ class Bundle:
         check(lib().nsa_bundle_dc(self.h, int(mode), i32p(conv), i32p(its)))
         return conv.astype(bool), its
 
-    def op_ladder(self, x0=0.0, skip_direct=False):
+    def op_ladder(self, x0=0.0, skip_direct=False, dcop=False):
         """The transient preamble's operating point: direct Newton in
-        MODETRANOP, then gmin stepping for the instances that failed.
-        Returns (converged, iterations, stage) per instance."""
+        MODETRANOP (MODEDCOP with dcop, as solve_dc), then gmin stepping for
+        the instances that failed.  Returns (converged, iterations, stage)
+        per instance."""
         self.set_solution(x0)
         conv, its, st = (np.zeros(self.N, np.int32) for _ in range(3))
-        check(lib().nsa_bundle_dc_ladder(self.h, int(bool(skip_direct)), i32p(conv), i32p(its), i32p(st)))
+        flags = int(bool(skip_direct)) | (2 if dcop else 0)
+        check(lib().nsa_bundle_dc_ladder(self.h, flags, i32p(conv), i32p(its), i32p(st)))
         return conv.astype(bool), its, st
+
+    def device_offsets(self):
+        d = np.zeros(self.N + 1, np.int32)
+        check(lib().nsa_bundle_device_offsets(self.h, i32p(d)))
+        return d
+
+    def dc_adjoint(self, dLdx):
+        """Gradients of a loss through every instance's operating point,
+        given dL/dx (total unknowns, packed): dict with lambda (packed) and,
+        per device (instance i's device d at device_offsets()[i] + d),
+        value (dL/dR, dL/dC = 0, dL/dV, dL/dI), w and l (BSIM4v7 dL/dW, dL/dL)."""
+        g = as_f64(np.asarray(dLdx, np.float64).reshape(self.total_n))
+        nd = int(self.device_offsets()[-1])
+        out = {"lambda": np.zeros(self.total_n), "value": np.zeros(max(nd, 1)), "w": np.zeros(max(nd, 1)),
+               "l": np.zeros(max(nd, 1))}
+        check(lib().nsa_bundle_dc_adjoint(self.h, f64p(g), f64p(out["lambda"]), f64p(out["value"]),
+                                          f64p(out["w"]), f64p(out["l"])))
+        for k in ("value", "w", "l"):
+            out[k] = out[k][:nd]
+        return out
 
     def tran_begin(self, tstep, tstop, probes=None, max_points=1):
         """solve_transient's preamble for every instance; probes: one list

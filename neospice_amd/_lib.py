@@ -48,6 +48,7 @@ _SIGS = {
     "nsa_circuit_m": (C.c_int, [VP, C.c_char_p, I32, I32, I32, I32, C.c_char_p, C.c_double,
                                 C.c_double, P_I32]),
     "nsa_circuit_bsim4_params": (C.c_int, [VP, I32, P_F64]),
+    "nsa_circuit_bsim4_param_derivs": (C.c_int, [VP, I32, P_F64, P_F64, P_F64]),
     "nsa_circuit_finalize": (C.c_int, [VP]),
     "nsa_circuit_info": (C.c_int, [VP, P_I32]),
     "nsa_circuit_pattern": (C.c_int, [VP, P_I32, P_I32]),
@@ -111,6 +112,8 @@ _SIGS = {
     "nsa_bundle_timing": (C.c_int, [VP, P_F64]),
     "nsa_bundle_set_graph": (C.c_int, [VP, I32]),
     "nsa_bundle_plan_info": (C.c_int, [VP, I32, P_I64]),
+    "nsa_bundle_device_offsets": (C.c_int, [VP, P_I32]),
+    "nsa_bundle_dc_adjoint": (C.c_int, [VP, P_F64, P_F64, P_F64, P_F64, P_F64]),
     "nsa_bundle_dc_ladder": (C.c_int, [VP, I32, P_I32, P_I32, P_I32]),
 }
 

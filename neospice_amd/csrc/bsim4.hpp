@@ -31,25 +31,29 @@
 
 namespace nsa {
 
-struct B4Param {
+// The record with its numeric fields of type T (double for the load; dual
+// numbers for the derivatives with respect to W and L).
+template <class T>
+struct B4ParamT {
     // model
-    double type, vtm, coxe, coxp, toxe, toxp, factor1, vcrit, bdos, ados, temp_ratio;
-    double PhiBS, PhiBSWS, PhiBSWGS, PhiBD, PhiBSWD, PhiBSWGD;
-    double MJS, MJSWS, MJSWGS, MJD, MJSWD, MJSWGD;
+    T type, vtm, coxe, coxp, toxe, toxp, factor1, vcrit, bdos, ados, temp_ratio;
+    T PhiBS, PhiBSWS, PhiBSWGS, PhiBD, PhiBSWD, PhiBSWGD;
+    T MJS, MJSWS, MJSWGS, MJD, MJSWD, MJSWGD;
     // junction capacitances of this instance (zero-bias, area/perimeter applied)
-    double czbs, czbssw, czbsswg, czbd, czbdsw, czbdswg;
+    T czbs, czbssw, czbsswg, czbd, czbdsw, czbdswg;
     // size-dependent (pParam)
-    double leff, weff, leffCV, weffCV, phi, sqrtPhi, Xdep0, vbi, k1, k1ox, k3, k3b;
-    double dvt0, dvt1, dvt2, dvt0w, dvt1w, dvt2w, lpe0, lpeb, kt1, kt1l, kt2, w0;
-    double theta0vb0, etab, nfactor, cdsc, cdscb, cdscd, cit, mstar, voffcbn, cdep0;
-    double dwg, dwb, prwg, prwb, rds0, rdswmin, xj, a0, ags, b0, b1, keta;
-    double ua, ub, uc, ud, a2, delta, pvag, pclm, litl, thetaRout, pdiblb, pditsd;
-    double pscbe1, pscbe2, noff, voffcv, abulkCVfactor, acde, ldeb, moin;
-    double cgdo, cgso, cgbo, cgdl, cgsl, ckappad, ckappas;
+    T leff, weff, leffCV, weffCV, phi, sqrtPhi, Xdep0, vbi, k1, k1ox, k3, k3b;
+    T dvt0, dvt1, dvt2, dvt0w, dvt1w, dvt2w, lpe0, lpeb, kt1, kt1l, kt2, w0;
+    T theta0vb0, etab, nfactor, cdsc, cdscb, cdscd, cit, mstar, voffcbn, cdep0;
+    T dwg, dwb, prwg, prwb, rds0, rdswmin, xj, a0, ags, b0, b1, keta;
+    T ua, ub, uc, ud, a2, delta, pvag, pclm, litl, thetaRout, pdiblb, pditsd;
+    T pscbe1, pscbe2, noff, voffcv, abulkCVfactor, acde, ldeb, moin;
+    T cgdo, cgso, cgbo, cgdl, cgsl, ckappad, ckappas;
     // instance
-    double vth0, k2ox, eta0, vbsc, u0temp, vsattemp, vtfbphi2, vfbzb;
+    T vth0, k2ox, eta0, vbsc, u0temp, vsattemp, vtfbphi2, vfbzb;
     int32_t off, pad;
 };
+using B4Param = B4ParamT<double>;
 
 // The doubles before `off`: the record's numeric fields.
 constexpr int kB4Fields = 110;
@@ -57,6 +61,11 @@ constexpr int kB4Fields = 110;
 // Throws Error(NSA_ERR_UNSUPPORTED) for a card, instance or temperature
 // outside the scope above and Error(NSA_ERR_ARGUMENT) for an invalid one.
 B4Param bsim4_param(const ModelCard& card, const Device& d, const DeviceEnv& env);
+// The record (val) and its derivatives with respect to the instance's W
+// (dw) and L (dl), each kB4Fields doubles in field order, by forward-mode
+// dual numbers through the same computation (val equals bsim4_param's).
+void bsim4_param_derivs(const ModelCard& card, const Device& d, const DeviceEnv& env, double* val,
+                        double* dw, double* dl);
 
 // Whether a model card selects BSIM4v7 (LEVEL=14, mosfet_common.cpp:479-481).
 inline bool is_bsim4_level(int level) { return level == 14; }

@@ -7,6 +7,7 @@
 #include <exception>
 #include <thread>
 
+#include "bsim4.hpp"
 #include "common.hpp"
 #include "devices.hpp"
 #include "gpu_bundle.hpp"
@@ -94,6 +95,7 @@ Bundle::Bundle(const std::vector<const Circuit*>& circuits, const Options& opt)
     NSA_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     xoff_.assign(NI_ + 1, 0);
     aoff_.assign(NI_ + 1, 0);
+    doff_.assign(NI_ + 1, 0);
     n_.resize(NI_);
     nnodes_.resize(NI_);
     for (int i = 0; i < NI_; ++i) {
@@ -101,6 +103,7 @@ Bundle::Bundle(const std::vector<const Circuit*>& circuits, const Options& opt)
         nnodes_[i] = ckt_[i]->num_nodes();
         xoff_[i + 1] = xoff_[i] + n_[i];
         aoff_[i + 1] = aoff_[i] + ckt_[i]->nnz();
+        doff_[i + 1] = doff_[i] + (int32_t)ckt_[i]->devices().size();
     }
     tot_n_ = xoff_[NI_];
     tot_nnz_ = aoff_[NI_];
@@ -1361,6 +1364,117 @@ void Bundle::tran_probes(int i, int* count, double* out) const {
     NSA_HIP(hipMemcpyAsync(out, d_pvals_ + (size_t)pptr_[i] * max_points_, sizeof(double) * rows * np,
                            hipMemcpyDeviceToHost, stream_));
     NSA_HIP(hipStreamSynchronize(stream_));
+}
+
+// ---------------------------------------------------------------- DC adjoint
+
+void Bundle::ensure_adjoint() {
+    if (adj_ready_) return;
+    std::vector<int32_t> rn, vb, vi, in, ii;
+    auto abs_node = [&](int i, int node) { return node < 0 ? -1 : xoff_[i] + node; };
+    for (int i = 0; i < NI_; ++i) {
+        const Circuit& c = *ckt_[i];
+        const auto& devs = c.devices();
+        for (int d : c.family(DK_R)) {
+            rn.push_back(abs_node(i, devs[d].node[0]));
+            rn.push_back(abs_node(i, devs[d].node[1]));
+            r_owner_.push_back(doff_[i] + d);
+        }
+        for (int d : c.family(DK_V)) {
+            vb.push_back(xoff_[i] + devs[d].branch);
+            vi.push_back(i);
+            v_owner_.push_back(doff_[i] + d);
+        }
+        for (int d : c.family(DK_I)) {
+            in.push_back(abs_node(i, devs[d].node[0]));
+            in.push_back(abs_node(i, devs[d].node[1]));
+            ii.push_back(i);
+            i_owner_.push_back(doff_[i] + d);
+        }
+    }
+    if ((int)r_owner_.size() != nR_ || (int)v_owner_.size() != vsrc_.count || (int)i_owner_.size() != isrc_.count)
+        throw Error(NSA_ERR_RUNTIME, "Bundle adjoint: family tables");
+    // d(record)/dW and d(record)/dL of every BSIM4v7 lane, in the lane
+    // order and field-major layout of the load's records
+    const int G4 = b4_lanes_;
+    std::vector<double> dW((size_t)kB4Fields * std::max(1, G4)), dL(dW.size());
+    {
+        const DeviceEnv env{};
+        std::vector<double> val(kB4Fields), w(kB4Fields), l(kB4Fields);
+        int g = 0;
+        for (int i = 0; i < NI_; ++i) {
+            const Circuit& c = *ckt_[i];
+            for (int d : c.family(DK_B4)) {
+                const Device& dv = c.devices()[d];
+                bsim4_param_derivs(c.models().at(dv.model), dv, env, val.data(), w.data(), l.data());
+                for (int q = 0; q < kB4Fields; ++q) {
+                    dW[(size_t)q * G4 + g] = w[q];
+                    dL[(size_t)q * G4 + g] = l[q];
+                }
+                b4_owner_.push_back(doff_[i] + d);
+                ++g;
+            }
+        }
+        if (g != G4) throw Error(NSA_ERR_RUNTIME, "Bundle adjoint: BSIM4v7 lanes");
+    }
+    d_r_nodes_ = dupload(allocs_, rn);
+    d_v_branch_ = dupload(allocs_, vb);
+    d_v_inst_ = dupload(allocs_, vi);
+    d_i_nodes_ = dupload(allocs_, in);
+    d_i_inst_ = dupload(allocs_, ii);
+    d_b4_dW_ = dupload(allocs_, dW);
+    d_b4_dL_ = dupload(allocs_, dL);
+    d_lam_ = dalloc<double>(allocs_, tot_n_);
+    d_gR_ = dalloc<double>(allocs_, nR_);
+    d_gV_ = dalloc<double>(allocs_, vsrc_.count);
+    d_gI_ = dalloc<double>(allocs_, isrc_.count);
+    d_gW_ = dalloc<double>(allocs_, G4);
+    d_gL_ = dalloc<double>(allocs_, G4);
+    adj_ready_ = true;
+}
+
+void Bundle::dc_adjoint(const double* dLdx, double* lambda, double* g_value, double* g_w, double* g_l) {
+    for (int i = 0; i < NI_; ++i)
+        if (plan_[i] < 0) throw Error(NSA_ERR_LOGIC, "Bundle::dc_adjoint: no factors (solve the operating point first)");
+    ensure_adjoint();
+    NSA_HIP(hipStreamSynchronize(stream_));
+    NSA_HIP(hipMemcpyAsync(d_lam_, dLdx, sizeof(double) * tot_n_, hipMemcpyHostToDevice, stream_));
+    // J^T lambda = dL/dx with the factors of each instance's last iteration
+    launch_small_lu(plans_, d_inst_, NI_, SLU_SOLVE_T, nullptr, 0.0, nullptr, nullptr, nullptr, d_LU_, d_lam_,
+                    d_lam_, stream_);
+    launch_bundle_grad_lin(nR_, d_r_nodes_, d_r_ohms_, vsrc_.count, d_v_branch_, d_v_inst_, isrc_.count,
+                           d_i_nodes_, d_i_inst_, d_ctx_, d_x_, d_lam_, d_gR_, d_gV_, d_gI_, stream_);
+    if (b4_lanes_) {
+        launch_bsim4_grad(b4_, d_ctx_, d_x_, d_b4_dW_, d_lam_, d_gW_, tol_, b4_lanes_, stream_);
+        launch_bsim4_grad(b4_, d_ctx_, d_x_, d_b4_dL_, d_lam_, d_gL_, tol_, b4_lanes_, stream_);
+    }
+    std::vector<double> gR(nR_), gV(vsrc_.count), gI(isrc_.count), gW(b4_lanes_), gL(b4_lanes_);
+    auto down = [&](std::vector<double>& h, const double* dptr) {
+        if (!h.empty())
+            NSA_HIP(hipMemcpyAsync(h.data(), dptr, sizeof(double) * h.size(), hipMemcpyDeviceToHost, stream_));
+    };
+    down(gR, d_gR_);
+    down(gV, d_gV_);
+    down(gI, d_gI_);
+    down(gW, d_gW_);
+    down(gL, d_gL_);
+    if (lambda) NSA_HIP(hipMemcpyAsync(lambda, d_lam_, sizeof(double) * tot_n_, hipMemcpyDeviceToHost, stream_));
+    NSA_HIP(hipStreamSynchronize(stream_));
+    const size_t nd = (size_t)doff_[NI_];
+    if (g_value) {
+        std::fill(g_value, g_value + nd, 0.0);
+        for (size_t k = 0; k < gR.size(); ++k) g_value[r_owner_[k]] = gR[k];
+        for (size_t k = 0; k < gV.size(); ++k) g_value[v_owner_[k]] = gV[k];
+        for (size_t k = 0; k < gI.size(); ++k) g_value[i_owner_[k]] = gI[k];
+    }
+    if (g_w) {
+        std::fill(g_w, g_w + nd, 0.0);
+        for (size_t k = 0; k < gW.size(); ++k) g_w[b4_owner_[k]] = gW[k];
+    }
+    if (g_l) {
+        std::fill(g_l, g_l + nd, 0.0);
+        for (size_t k = 0; k < gL.size(); ++k) g_l[b4_owner_[k]] = gL[k];
+    }
 }
 
 }  // namespace nsa

@@ -140,6 +140,18 @@ public:
     void tran_probes(int i, int* count, double* out) const;
     int tran_probe_count(int i) const { return pptr_.empty() ? 0 : pptr_[i + 1] - pptr_[i]; }
 
+    // ---- DC adjoint (sens.cpp's quantities, by the adjoint instead of
+    // finite differences): at every instance's last converged operating
+    // point, lambda = J^-T dL/dx through the stored factors of the last
+    // iteration, then dL/dp = -lambda^T dF/dp.  Outputs are per circuit
+    // device (instance i's device d at device_offset(i) + d): g_value is
+    // dL/d(value) of R (ohms), C (farads: zero in DC), V and I sources;
+    // g_w / g_l are dL/dW and dL/dL of BSIM4v7 devices (zero elsewhere).
+    // Any output may be null.
+    void dc_adjoint(const double* dLdx, double* lambda, double* g_value, double* g_w, double* g_l);
+    int device_offset(int i) const { return doff_[i]; }
+    int total_devices() const { return doff_[NI_]; }
+
     // ---- statistics
     long n_cpu_pivot = 0, n_iterations = 0, n_graph_launches = 0, n_host_iterations = 0;
     int num_plans() const { return plans_.count(); }
@@ -165,6 +177,7 @@ private:
     void masked_copy_state(const std::vector<int32_t>& mask, double* const* src, double* const* dst);
     std::vector<double> instance_matrix(int i);
     void upload_inst();
+    void ensure_adjoint();
     void tran_attempt_setup(int i, DevCtx& c);
     void tran_post_round(const std::vector<int32_t>& attempt, const std::vector<int32_t>& conv,
                          const std::vector<int32_t>& its);
@@ -175,6 +188,14 @@ private:
     hipStream_t stream_ = nullptr;
     std::vector<void*> allocs_;
     std::vector<int32_t> xoff_, aoff_, n_, nnodes_;
+    std::vector<int32_t> doff_;  // NI+1 device offsets (circuit device order)
+    // adjoint: family tables with the owners of their entries (global device positions)
+    bool adj_ready_ = false;
+    std::vector<int32_t> r_owner_, v_owner_, i_owner_, b4_owner_;
+    int32_t *d_r_nodes_ = nullptr, *d_v_branch_ = nullptr, *d_v_inst_ = nullptr, *d_i_nodes_ = nullptr,
+            *d_i_inst_ = nullptr;
+    double *d_lam_ = nullptr, *d_gR_ = nullptr, *d_gV_ = nullptr, *d_gI_ = nullptr, *d_gW_ = nullptr,
+           *d_gL_ = nullptr, *d_b4_dW_ = nullptr, *d_b4_dL_ = nullptr;
     // device vectors
     double *d_x_ = nullptr, *d_xold_ = nullptr, *d_rhs_ = nullptr, *d_A_ = nullptr, *d_LU_ = nullptr;
     double* d_s_[4] = {nullptr, nullptr, nullptr, nullptr};

@@ -164,6 +164,17 @@ int nsa_circuit_bsim4_params(const nsa_circuit* c, int32_t dev, double* out) {
         return 0;
     });
 }
+int nsa_circuit_bsim4_param_derivs(const nsa_circuit* c, int32_t dev, double* val, double* dw,
+                                   double* dl) {
+    return guard([&] {
+        if (!val || !dw || !dl) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "output is NULL");
+        const auto& devs = c->c.devices();
+        if (dev < 0 || dev >= (int32_t)devs.size() || devs[dev].kind != nsa::DK_B4)
+            throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "not a BSIM4v7 device");
+        nsa::bsim4_param_derivs(c->c.models().at(devs[dev].model), devs[dev], nsa::DeviceEnv{}, val, dw, dl);
+        return 0;
+    });
+}
 int nsa_circuit_finalize(nsa_circuit* c) {
     return guard([&] { c->c.finalize(); return 0; });
 }
@@ -671,15 +682,31 @@ int nsa_bundle_timing(nsa_bundle* b, double out[5]) {
 int nsa_bundle_set_graph(nsa_bundle* b, int32_t on) {
     return guard([&] { b->b->use_graph = on != 0; return 0; });
 }
-int nsa_bundle_dc_ladder(nsa_bundle* b, int32_t skip_direct, int32_t* converged, int32_t* iterations,
+int nsa_bundle_dc_ladder(nsa_bundle* b, int32_t flags, int32_t* converged, int32_t* iterations,
                          int32_t* stage) {
     return guard([&] {
         std::vector<int32_t> c, it, st;
-        b->b->op_ladder(0x20 | 0x200, 0x20 | 0x100, c, it, st, skip_direct != 0);
+        // MODETRANOP (solve_transient's preamble) or MODEDCOP (solve_dc), | MODEINITJCT / MODEINITFLOAT
+        const int op = (flags & 2) ? 0x10 : 0x20;
+        b->b->op_ladder(op | 0x200, op | 0x100, c, it, st, (flags & 1) != 0);
         const int n = b->b->size();
         if (converged) std::memcpy(converged, c.data(), sizeof(int32_t) * n);
         if (iterations) std::memcpy(iterations, it.data(), sizeof(int32_t) * n);
         if (stage) std::memcpy(stage, st.data(), sizeof(int32_t) * n);
+        return 0;
+    });
+}
+int nsa_bundle_device_offsets(nsa_bundle* b, int32_t* doff) {
+    return guard([&] {
+        for (int i = 0; i <= b->b->size(); ++i) doff[i] = i < b->b->size() ? b->b->device_offset(i) : b->b->total_devices();
+        return 0;
+    });
+}
+int nsa_bundle_dc_adjoint(nsa_bundle* b, const double* dLdx, double* lambda, double* g_value, double* g_w,
+                          double* g_l) {
+    return guard([&] {
+        if (!dLdx) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "dLdx is NULL");
+        b->b->dc_adjoint(dLdx, lambda, g_value, g_w, g_l);
         return 0;
     });
 }

@@ -19,7 +19,9 @@
 #include <cstddef>
 
 #include "bsim4.hpp"
+#include "common.hpp"
 #include "dev_limits.hpp"
+#include "dual.hpp"
 #include "gpu_dev.hpp"
 
 namespace nsa {
@@ -43,21 +45,27 @@ constexpr double kDelta4 = 0.02;
 constexpr double kVt0 = 1.38064852e-23 * 300.15 / 1.6021766208e-19;
 
 // field f of the device's constant record (pb, ps set in the kernel)
-#define PF(f) (pb[(offsetof(B4Param, f) / sizeof(double)) * ps])
+#define PF(f) P((int)(offsetof(B4Param, f) / sizeof(double)))
 static_assert(kB4Fields == (int)(offsetof(B4Param, off) / sizeof(double)), "B4Param layout");
 
 // UCB MAX (ucb_compat.hpp:38-43)
-__device__ inline double ucb_max(double a, double b) { return (a > b) ? a : b; }
+template <class T>
+__device__ inline T ucb_max(const T& a, const T& b) { return (a > b) ? a : b; }
 
 // The lane's live state (~140 doubles at the charge model) fits in 280
 // VGPRs without spilling when the kernel may run one wave per SIMD; at the
 // compiler's default occupancy target it spilled 680 bytes per lane to
 // scratch, which doubled its HBM traffic and its duration.
-__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_bsim4(B4Table t, const DevCtx* __restrict__ ctx, const double* __restrict__ x,
-                        double* __restrict__ s0, double* __restrict__ s1,
-                        const double* __restrict__ s2, const int32_t* __restrict__ done,
-                        double* __restrict__ cm, double* __restrict__ cr,
-                        int32_t* __restrict__ devnc, DevTol tol, int B) {
+template <class T>
+__device__ __forceinline__ void b4_lane(const B4Table& t, const DevCtx* __restrict__ ctx,
+                                        const double* __restrict__ x, double* __restrict__ s0,
+                                        double* __restrict__ s1, const double* __restrict__ s2,
+                                        const int32_t* __restrict__ done, double* __restrict__ cm,
+                                        double* __restrict__ cr, int32_t* __restrict__ devnc, DevTol tol,
+                                        int B, const double* __restrict__ dpar,
+                                        const double* __restrict__ lam, double* __restrict__ gout) {
+    // the gradient form: dual numbers seeded by d(record)/dp, DC adjoint only
+    constexpr bool kGrad = is_dual<T>::value;
     const long gid = (long)blockIdx.x * TPB + threadIdx.x;
     if (gid >= (long)t.count * B) return;
     const int d = (int)(gid / B), inst = (int)(gid % B);
@@ -70,7 +78,17 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     const double* pb = t.par_inst ? t.par_inst + (size_t)d * kB4Fields * sB + inst
                                   : reinterpret_cast<const double*>(t.par + d);
     const size_t ps = t.par_inst ? sB : 1;
+    const double* dpb = kGrad ? dpar + (size_t)d * kB4Fields * sB + inst : nullptr;
+    auto P = [&](int q) -> T {
+        T v = pb[(size_t)q * ps];
+        if constexpr (kGrad) v.d[0] = dpb[(size_t)q * sB];
+        return v;
+    };
     const int mode = c.mode;
+    // stores of the load (none in the gradient form)
+    auto put = [&](double* a, size_t k, const T& v) {
+        if constexpr (!kGrad) a[k] = value_of(v);
+    };
     const size_t st = (size_t)t.state[d] * sB + inst;
     auto S = [&](int k) { return st + (size_t)k * sB; };
     // state slots (bsim4v7_def.hpp:514-545)
@@ -81,35 +99,37 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
               nB = t.nodes[4 * d + 3];
     // terminal k's voltage: node of the device (batch of one circuit) or the
     // lane's own solution index (packed batch of circuits)
-    auto V = [&](int node, int k) -> double {
+    auto V = [&](int node, int k) -> T {
         if (t.pin) {
             const int q = t.pin[(size_t)k * sB + inst];
             return q >= 0 ? x[q] : 0.0;
         }
         return sol(x, node, sB, inst);
     };
-    const double type = PF(type);
-    const bool charge = ((mode & (MODEDCTRANCURVE | MODEAC | MODETRAN | MODEINITSMSIG)) ||
+    const T type = PF(type);
+    const bool charge = !kGrad && ((mode & (MODEDCTRANCURVE | MODEAC | MODETRAN | MODEINITSMSIG)) ||
                          ((mode & MODETRANOP) && (mode & MODEUIC)));
 
     // instance fields carried from the previous load
-    const double h_von = aux[B4_VON * sB];
-    const double h_mode = aux[B4_MODE * sB];
-    const double h_cd = aux[B4_CD * sB], h_cbs = aux[B4_CBS * sB], h_cbd = aux[B4_CBD * sB];
-    const double h_gbs = aux[B4_GBS * sB], h_gbd = aux[B4_GBD * sB];
-    const double h_gm = aux[B4_GM * sB], h_gds = aux[B4_GDS * sB], h_gmbs = aux[B4_GMBS * sB];
+    const T h_von = aux[B4_VON * sB];
+    const T h_mode = aux[B4_MODE * sB];
+    const T h_cd = aux[B4_CD * sB], h_cbs = aux[B4_CBS * sB], h_cbd = aux[B4_CBD * sB];
+    const T h_gbs = aux[B4_GBS * sB], h_gbd = aux[B4_GBD * sB];
+    const T h_gm = aux[B4_GM * sB], h_gds = aux[B4_GDS * sB], h_gmbs = aux[B4_GMBS * sB];
 
     // ---- terminal voltages (bsim4v7_load.cpp:237-663)
     int Check = 1;
-    double vds, vgs, vbs, vbd, vgd;
-    double cdhat = 0.0, cbhat = 0.0;
-    if (mode & MODEADJOINT) {
+    T vds, vgs, vbs, vbd, vgd;
+    T cdhat = 0.0, cbhat = 0.0;
+    if (kGrad || (mode & MODEADJOINT)) {
         // the converged solution itself, unlimited (the adjoint's Jacobian)
         vds = type * (V(nD, 0) - V(nS, 2));
         vgs = type * (V(nG, 1) - V(nS, 2));
         vbs = type * (V(nB, 3) - V(nS, 2));
         Check = 0;
-    } else if (mode & MODEINITTRAN) {
+    } else {
+    if constexpr (!kGrad) {
+    if (mode & MODEINITTRAN) {
         vds = s1[S(vdsS)];
         vgs = s1[S(vgsS)];
         vbs = s1[S(vbsS)];
@@ -126,44 +146,44 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         }
     } else {
         if (mode & MODEINITPRED) {
-            const double xfact = c.delta / c.delta_old1;
-            s0[S(vdsS)] = s1[S(vdsS)];
+            const T xfact = c.delta / c.delta_old1;
+            put(s0, S(vdsS), s1[S(vdsS)]);
             vds = (1.0 + xfact) * (s1[S(vdsS)]) - (xfact * (s2[S(vdsS)]));
-            s0[S(vgsS)] = s1[S(vgsS)];
+            put(s0, S(vgsS), s1[S(vgsS)]);
             vgs = (1.0 + xfact) * (s1[S(vgsS)]) - (xfact * (s2[S(vgsS)]));
-            s0[S(vbsS)] = s1[S(vbsS)];
+            put(s0, S(vbsS), s1[S(vbsS)]);
             vbs = (1.0 + xfact) * (s1[S(vbsS)]) - (xfact * (s2[S(vbsS)]));
-            s0[S(vbdS)] = s0[S(vbsS)] - s0[S(vdsS)];
+            put(s0, S(vbdS), s0[S(vbsS)] - s0[S(vdsS)]);
         } else {
             vds = type * (V(nD, 0) - V(nS, 2));
             vgs = type * (V(nG, 1) - V(nS, 2));
             vbs = type * (V(nB, 3) - V(nS, 2));
         }
-        const double s0vds = s0[S(vdsS)], s0vgs = s0[S(vgsS)], s0vbs = s0[S(vbsS)],
+        const T s0vds = s0[S(vdsS)], s0vgs = s0[S(vgsS)], s0vbs = s0[S(vbsS)],
                      s0vbd = s0[S(vbdS)];
-        const double vgdo = s0vgs - s0vds;
+        const T vgdo = s0vgs - s0vds;
         vbd = vbs - vds;
         vgd = vgs - vds;
-        const double delvbd = vbd - s0vbd;
-        const double delvgd = vgd - vgdo;
-        const double delvds = vds - s0vds;
-        const double delvgs = vgs - s0vgs;
-        const double delvbs = vbs - s0vbs;
+        const T delvbd = vbd - s0vbd;
+        const T delvgd = vgd - vgdo;
+        const T delvds = vds - s0vds;
+        const T delvgs = vgs - s0vgs;
+        const T delvbs = vbs - s0vbs;
         // the linearised currents of the previous load at this iterate, for
         // the current-based convergence test (:421-473)
         if (h_mode >= 0) {
-            const double Idtot = h_cd - h_cbd;
+            const T Idtot = h_cd - h_cbd;
             cdhat = Idtot - h_gbd * delvbd + h_gmbs * delvbs + h_gm * delvgs + h_gds * delvds;
-            const double Ibtot = h_cbs + h_cbd;
+            const T Ibtot = h_cbs + h_cbd;
             cbhat = Ibtot + h_gbd * delvbd + h_gbs * delvbs;
         } else {
-            const double Idtot = h_cd + h_cbd;
+            const T Idtot = h_cd + h_cbd;
             cdhat = Idtot + h_gbd * delvbd + h_gmbs * delvbd + h_gm * delvgd - h_gds * delvds;
-            const double Ibtot = h_cbs + h_cbd;
+            const T Ibtot = h_cbs + h_cbd;
             cbhat = Ibtot + h_gbs * delvbs + h_gbd * delvbd;
         }
         // limiting (:582-662)
-        const double von = h_von;
+        const T von = h_von;
         if (s0vds >= 0.0) {
             vgs = fetlim(vgs, s0vgs, von);
             vds = vgs - vgd;
@@ -183,17 +203,19 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
             vbs = vbd + vds;
         }
     }
+    }
+    }
     vbd = vbs - vds;
     vgd = vgs - vds;
-    const double vgb = vgs - vbs;
-    const double vbs_jct = vbs, vbd_jct = vbd;
+    const T vgb = vgs - vbs;
+    const T vbs_jct = vbs, vbd_jct = vbd;
 
     // ---- junction currents: zero saturation current, gmin only (:682-795)
-    const double gbs = tol.gmin, cbs = gbs * vbs_jct;
-    const double gbd = tol.gmin, cbd = gbd * vbd_jct;
+    const T gbs = tol.gmin, cbs = gbs * vbs_jct;
+    const T gbd = tol.gmin, cbd = gbd * vbd_jct;
 
     int dmode;
-    double Vds, Vbs;
+    T Vds, Vbs;
     if (vds >= 0.0) {
         dmode = 1;
         Vds = vds;
@@ -203,11 +225,11 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         Vds = -vds;
         Vbs = vbd;
     }
-    const double toxe = PF(toxe), epssub = kEpsSi;
+    const T toxe = PF(toxe), epssub = kEpsSi;
 
     // ---- Vbseff, Phis, Xdep (:1013-1038)
-    double T0, T1, T2, T3, T4, T5, T6, T7, T8, T9, T10, T11, T12, T13;
-    double Vbseff, dVbseff_dVb;
+    T T0, T1, T2, T3, T4, T5, T6, T7, T8, T9, T10, T11, T12, T13;
+    T Vbseff, dVbseff_dVb;
     T0 = Vbs - PF(vbsc) - 0.001;
     T1 = sqrt(T0 * T0 - 0.004 * PF(vbsc));
     if (T0 >= 0.0) {
@@ -223,18 +245,18 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     T1 = sqrt(T0 * T0 + 0.004 * T9);
     Vbseff = T9 - 0.5 * (T0 + T1);
     dVbseff_dVb *= 0.5 * (1.0 + T0 / T1);
-    const double Phis = PF(phi) - Vbseff;
-    const double dPhis_dVb = -1.0;
-    const double sqrtPhis = sqrt(Phis);
-    const double dsqrtPhis_dVb = -0.5 / sqrtPhis;
-    const double Xdep = PF(Xdep0) * sqrtPhis / PF(sqrtPhi);
-    const double dXdep_dVb = (PF(Xdep0) / PF(sqrtPhi)) * dsqrtPhis_dVb;
-    const double Leff = PF(leff);
-    const double Vtm = PF(vtm);
+    const T Phis = PF(phi) - Vbseff;
+    const T dPhis_dVb = -1.0;
+    const T sqrtPhis = sqrt(Phis);
+    const T dsqrtPhis_dVb = -0.5 / sqrtPhis;
+    const T Xdep = PF(Xdep0) * sqrtPhis / PF(sqrtPhi);
+    const T dXdep_dVb = (PF(Xdep0) / PF(sqrtPhi)) * dsqrtPhis_dVb;
+    const T Leff = PF(leff);
+    const T Vtm = PF(vtm);
 
     // ---- Vth (:1044-1141)
     T3 = sqrt(Xdep);
-    const double V0 = PF(vbi) - PF(phi);
+    const T V0 = PF(vbi) - PF(phi);
     T0 = PF(dvt2) * Vbseff;
     if (T0 >= -0.5) {
         T1 = 1.0 + T0;
@@ -244,8 +266,8 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         T1 = (1.0 + 3.0 * T0) * T4;
         T2 = PF(dvt2) * T4 * T4;
     }
-    const double lt1 = PF(factor1) * T3 * T1;
-    const double dlt1_dVb = PF(factor1) * (0.5 / T3 * T1 * dXdep_dVb + T3 * T2);
+    const T lt1 = PF(factor1) * T3 * T1;
+    const T dlt1_dVb = PF(factor1) * (0.5 / T3 * T1 * dXdep_dVb + T3 * T2);
     T0 = PF(dvt2w) * Vbseff;
     if (T0 >= -0.5) {
         T1 = 1.0 + T0;
@@ -255,9 +277,9 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         T1 = (1.0 + 3.0 * T0) * T4;
         T2 = PF(dvt2w) * T4 * T4;
     }
-    const double ltw = PF(factor1) * T3 * T1;
-    const double dltw_dVb = PF(factor1) * (0.5 / T3 * T1 * dXdep_dVb + T3 * T2);
-    double Theta0, dTheta0_dVb;
+    const T ltw = PF(factor1) * T3 * T1;
+    const T dltw_dVb = PF(factor1) * (0.5 / T3 * T1 * dXdep_dVb + T3 * T2);
+    T Theta0, dTheta0_dVb;
     T0 = PF(dvt1) * Leff / lt1;
     if (T0 < kExpThreshold) {
         T1 = exp(T0);
@@ -265,16 +287,16 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         T3 = T2 * T2;
         T4 = T3 + 2.0 * T1 * kMinExp;
         Theta0 = T1 / T4;
-        const double dT1_dVb = -T0 * T1 * dlt1_dVb / lt1;
+        const T dT1_dVb = -T0 * T1 * dlt1_dVb / lt1;
         dTheta0_dVb = dT1_dVb * (T4 - 2.0 * T1 * (T2 + kMinExp)) / T4 / T4;
     } else {
         Theta0 = 1.0 / (kMaxExp - 2.0);
         dTheta0_dVb = 0.0;
     }
-    const double thetavth = PF(dvt0) * Theta0;
-    const double Delt_vth = thetavth * V0;
-    const double dDelt_vth_dVb = PF(dvt0) * dTheta0_dVb * V0;
-    double dT5_dVb;
+    const T thetavth = PF(dvt0) * Theta0;
+    const T Delt_vth = thetavth * V0;
+    const T dDelt_vth_dVb = PF(dvt0) * dTheta0_dVb * V0;
+    T dT5_dVb;
     T0 = PF(dvt1w) * PF(weff) * Leff / ltw;
     if (T0 < kExpThreshold) {
         T1 = exp(T0);
@@ -282,7 +304,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         T3 = T2 * T2;
         T4 = T3 + 2.0 * T1 * kMinExp;
         T5 = T1 / T4;
-        const double dT1_dVb = -T0 * T1 * dltw_dVb / ltw;
+        const T dT1_dVb = -T0 * T1 * dltw_dVb / ltw;
         dT5_dVb = dT1_dVb * (T4 - 2.0 * T1 * (T2 + kMinExp)) / T4 / T4;
     } else {
         T5 = 1.0 / (kMaxExp - 2.0);
@@ -290,11 +312,11 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     }
     T0 = PF(dvt0w) * T5;
     T2 = T0 * V0;
-    const double dT2_dVb_w = PF(dvt0w) * dT5_dVb * V0;
-    const double TempRatio = PF(temp_ratio);
+    const T dT2_dVb_w = PF(dvt0w) * dT5_dVb * V0;
+    const T TempRatio = PF(temp_ratio);
     T0 = sqrt(1.0 + PF(lpe0) / Leff);
     T1 = PF(k1ox) * (T0 - 1.0) * PF(sqrtPhi) + (PF(kt1) + PF(kt1l) / Leff + PF(kt2) * Vbseff) * TempRatio;
-    const double Vth_NarrowW = toxe * PF(phi) / (PF(weff) + PF(w0));
+    const T Vth_NarrowW = toxe * PF(phi) / (PF(weff) + PF(w0));
     T3 = PF(eta0) + PF(etab) * Vbseff;
     if (T3 < 1.0e-4) {
         T9 = 1.0 / (3.0 - 2.0e4 * T3);
@@ -303,24 +325,24 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     } else {
         T4 = 1.0;
     }
-    const double dDIBL_Sft_dVd = T3 * PF(theta0vb0);
-    const double DIBL_Sft = dDIBL_Sft_dVd * Vds;
-    const double Lpe_Vb = sqrt(1.0 + PF(lpeb) / Leff);
-    const double Vth = type * PF(vth0) + (PF(k1ox) * sqrtPhis - PF(k1) * PF(sqrtPhi)) * Lpe_Vb -
+    const T dDIBL_Sft_dVd = T3 * PF(theta0vb0);
+    const T DIBL_Sft = dDIBL_Sft_dVd * Vds;
+    const T Lpe_Vb = sqrt(1.0 + PF(lpeb) / Leff);
+    const T Vth = type * PF(vth0) + (PF(k1ox) * sqrtPhis - PF(k1) * PF(sqrtPhi)) * Lpe_Vb -
                        PF(k2ox) * Vbseff - Delt_vth - T2 + (PF(k3) + PF(k3b) * Vbseff) * Vth_NarrowW +
                        T1 - DIBL_Sft;
-    const double dVth_dVb = Lpe_Vb * PF(k1ox) * dsqrtPhis_dVb - PF(k2ox) - dDelt_vth_dVb - dT2_dVb_w +
+    const T dVth_dVb = Lpe_Vb * PF(k1ox) * dsqrtPhis_dVb - PF(k2ox) - dDelt_vth_dVb - dT2_dVb_w +
                             PF(k3b) * Vth_NarrowW - PF(etab) * Vds * PF(theta0vb0) * T4 +
                             PF(kt2) * TempRatio;
-    const double dVth_dVd = -dDIBL_Sft_dVd;
+    const T dVth_dVd = -dDIBL_Sft_dVd;
 
     // ---- n (:1145-1165)
-    double n, dn_dVb, dn_dVd;
+    T n, dn_dVb, dn_dVd;
     {
-        const double tmp1 = epssub / Xdep;
-        const double tmp2 = PF(nfactor) * tmp1;
-        const double tmp3 = PF(cdsc) + PF(cdscb) * Vbseff + PF(cdscd) * Vds;
-        const double tmp4 = (tmp2 + tmp3 * Theta0 + PF(cit)) / PF(coxe);
+        const T tmp1 = epssub / Xdep;
+        const T tmp2 = PF(nfactor) * tmp1;
+        const T tmp3 = PF(cdsc) + PF(cdscb) * Vbseff + PF(cdscd) * Vds;
+        const T tmp4 = (tmp2 + tmp3 * Theta0 + PF(cit)) / PF(coxe);
         if (tmp4 >= -0.5) {
             n = 1.0 + tmp4;
             dn_dVb = (-tmp2 / Xdep * dXdep_dVb + tmp3 * dTheta0_dVb + PF(cdscb) * Theta0) / PF(coxe);
@@ -333,13 +355,13 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
             dn_dVd = PF(cdscd) * Theta0 / PF(coxe) * T0;
         }
     }
-    const double von_new = Vth;
+    const T von_new = Vth;
 
     // ---- Vgsteff (:1222-1307); no poly depletion (NGATE = 0)
-    const double Vgs_eff = (dmode > 0) ? vgs : vgd;
-    const double dVgs_eff_dVg = 1.0;
-    const double Vgst = Vgs_eff - Vth;
-    double dT10_dVg, dT10_dVd, dT10_dVb, dT9_dVg, dT9_dVd, dT9_dVb;
+    const T Vgs_eff = (dmode > 0) ? vgs : vgd;
+    const T dVgs_eff_dVg = 1.0;
+    const T Vgst = Vgs_eff - Vth;
+    T dT10_dVg, dT10_dVd, dT10_dVb, dT9_dVg, dT9_dVd, dT9_dVb;
     T0 = n * Vtm;
     T1 = PF(mstar) * Vgst;
     T2 = T1 / T0;
@@ -355,7 +377,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         dT10_dVb = T10 * dn_dVb;
         T10 *= n;
     } else {
-        const double ExpVgst = exp(T2);
+        const T ExpVgst = exp(T2);
         T3 = Vtm * log(1.0 + ExpVgst);
         T10 = n * T3;
         dT10_dVg = PF(mstar) * ExpVgst / (1.0 + ExpVgst);
@@ -378,7 +400,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         dT9_dVd = dn_dVd * T3;
         dT9_dVb = dn_dVb * T3;
     } else {
-        const double ExpVgst = exp(T2);
+        const T ExpVgst = exp(T2);
         T3 = PF(coxe) / PF(cdep0);
         T4 = T3 * ExpVgst;
         T5 = T1 * T4 / T0;
@@ -388,17 +410,17 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         dT9_dVd = T4 * dn_dVd - dT9_dVg * dVth_dVd - T5 * dn_dVd;
         dT9_dVg *= dVgs_eff_dVg;
     }
-    double Vgsteff = T10 / T9;
+    T Vgsteff = T10 / T9;
     T11 = T9 * T9;
-    double dVgsteff_dVg = (T9 * dT10_dVg - T10 * dT9_dVg) / T11;
-    double dVgsteff_dVd = (T9 * dT10_dVd - T10 * dT9_dVd) / T11;
-    double dVgsteff_dVb = (T9 * dT10_dVb - T10 * dT9_dVb) / T11;
+    T dVgsteff_dVg = (T9 * dT10_dVg - T10 * dT9_dVg) / T11;
+    T dVgsteff_dVd = (T9 * dT10_dVd - T10 * dT9_dVd) / T11;
+    T dVgsteff_dVb = (T9 * dT10_dVb - T10 * dT9_dVb) / T11;
 
     // ---- effective width, Rds (:1310-1347)
     T9 = sqrtPhis - PF(sqrtPhi);
-    double Weff = PF(weff) - 2.0 * (PF(dwg) * Vgsteff + PF(dwb) * T9);
-    double dWeff_dVg = -2.0 * PF(dwg);
-    double dWeff_dVb = -2.0 * PF(dwb) * dsqrtPhis_dVb;
+    T Weff = PF(weff) - 2.0 * (PF(dwg) * Vgsteff + PF(dwb) * T9);
+    T dWeff_dVg = -2.0 * PF(dwg);
+    T dWeff_dVb = -2.0 * PF(dwb) * dsqrtPhis_dVb;
     if (Weff < 2.0e-8) {
         T0 = 1.0 / (6.0e-8 - 2.0 * Weff);
         Weff = 2.0e-8 * (4.0e-8 - Weff) * T0;
@@ -406,16 +428,16 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         dWeff_dVg *= T0;
         dWeff_dVb *= T0;
     }
-    double Rds, dRds_dVg, dRds_dVb;
+    T Rds, dRds_dVg, dRds_dVb;
     {
         T0 = 1.0 + PF(prwg) * Vgsteff;
-        const double dT0_dVg = -PF(prwg) / T0 / T0;
+        const T dT0_dVg = -PF(prwg) / T0 / T0;
         T1 = PF(prwb) * T9;
-        const double dT1_dVb = PF(prwb) * dsqrtPhis_dVb;
+        const T dT1_dVb = PF(prwb) * dsqrtPhis_dVb;
         T2 = 1.0 / T0 + T1;
         T3 = T2 + sqrt(T2 * T2 + 0.01);
-        double dT3_dVg = 1.0 + T2 / (T3 - T2);
-        const double dT3_dVb = dT3_dVg * dT1_dVb;
+        T dT3_dVg = 1.0 + T2 / (T3 - T2);
+        const T dT3_dVb = dT3_dVg * dT1_dVb;
         dT3_dVg *= dT0_dVg;
         T4 = PF(rds0) * 0.5;
         Rds = PF(rdswmin) + T3 * T4;
@@ -424,19 +446,19 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     }
 
     // ---- Abulk (:1350-1403)
-    double Abulk, Abulk0, dAbulk_dVg, dAbulk_dVb, dAbulk0_dVb;
+    T Abulk, Abulk0, dAbulk_dVg, dAbulk_dVb, dAbulk0_dVb;
     {
         T9 = 0.5 * PF(k1ox) * Lpe_Vb / sqrtPhis;
         T1 = T9 + PF(k2ox) - PF(k3b) * Vth_NarrowW;
-        const double dT1_dVb = -T9 / sqrtPhis * dsqrtPhis_dVb;
+        const T dT1_dVb = -T9 / sqrtPhis * dsqrtPhis_dVb;
         T9 = sqrt(PF(xj) * Xdep);
-        const double tmp1 = Leff + 2.0 * T9;
+        const T tmp1 = Leff + 2.0 * T9;
         T5 = Leff / tmp1;
-        const double tmp2 = PF(a0) * T5;
-        const double tmp3 = PF(weff) + PF(b1);
-        const double tmp4 = PF(b0) / tmp3;
+        const T tmp2 = PF(a0) * T5;
+        const T tmp3 = PF(weff) + PF(b1);
+        const T tmp4 = PF(b0) / tmp3;
         T2 = tmp2 + tmp4;
-        const double dT2_dVb = -T9 / tmp1 / Xdep * dXdep_dVb;
+        const T dT2_dVb = -T9 / tmp1 / Xdep * dXdep_dVb;
         T6 = T5 * T5;
         T7 = T5 * T6;
         Abulk0 = 1.0 + T1 * T2;
@@ -459,7 +481,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         dAbulk_dVg *= T10;
     }
     {
-        double dT0_dVb;
+        T dT0_dVb;
         T2 = PF(keta) * Vbseff;
         if (T2 >= -0.9) {
             T0 = 1.0 / (1.0 + T2);
@@ -477,9 +499,9 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     }
 
     // ---- mobility, mobMod = 0 (:1405-1514)
-    double ueff, dueff_dVg, dueff_dVd, dueff_dVb;
+    T ueff, dueff_dVg, dueff_dVd, dueff_dVb;
     {
-        const double T14 = 0.0;
+        const T T14 = 0.0;
         T0 = Vgsteff + Vth + Vth - T14;
         T2 = PF(ua) + PF(uc) * Vbseff;
         T3 = T0 / toxe;
@@ -491,12 +513,12 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         T5 = T3 * (T2 + PF(ub) * T3) + T6;
         T7 = -2.0 * T6 * T9;
         T11 = T7 * Vth / T12;
-        double dDenomi_dVg = (T2 + 2.0 * PF(ub) * T3) / toxe;
+        T dDenomi_dVg = (T2 + 2.0 * PF(ub) * T3) / toxe;
         T13 = 2.0 * (dDenomi_dVg + T11 + T8);
-        double dDenomi_dVd = T13 * dVth_dVd;
-        double dDenomi_dVb = T13 * dVth_dVb + PF(uc) * T3;
+        T dDenomi_dVd = T13 * dVth_dVd;
+        T dDenomi_dVb = T13 * dVth_dVb + PF(uc) * T3;
         dDenomi_dVg += T7;
-        double Denomi;
+        T Denomi;
         if (T5 >= -0.8) {
             Denomi = 1.0 + T5;
         } else {
@@ -515,18 +537,18 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     }
 
     // ---- Vdsat (:1517-1616); A1 = 0: Lambda = A2
-    const double WVCox = Weff * PF(vsattemp) * PF(coxe);
-    const double WVCoxRds = WVCox * Rds;
-    const double Esat = 2.0 * PF(vsattemp) / ueff;
-    const double EsatL = Esat * Leff;
+    const T WVCox = Weff * PF(vsattemp) * PF(coxe);
+    const T WVCoxRds = WVCox * Rds;
+    const T Esat = 2.0 * PF(vsattemp) / ueff;
+    const T EsatL = Esat * Leff;
     T0 = -EsatL / ueff;
-    const double dEsatL_dVg = T0 * dueff_dVg;
-    const double dEsatL_dVd = T0 * dueff_dVd;
-    const double dEsatL_dVb = T0 * dueff_dVb;
-    const double Lambda = PF(a2);
-    const double dLambda_dVg = 0.0;
-    const double Vgst2Vtm = Vgsteff + 2.0 * Vtm;
-    double tmp1, tmp2, tmp3;
+    const T dEsatL_dVg = T0 * dueff_dVg;
+    const T dEsatL_dVd = T0 * dueff_dVd;
+    const T dEsatL_dVb = T0 * dueff_dVb;
+    const T Lambda = PF(a2);
+    const T dLambda_dVg = 0.0;
+    const T Vgst2Vtm = Vgsteff + 2.0 * Vtm;
+    T tmp1, tmp2, tmp3;
     if (Rds > 0) {
         tmp2 = dRds_dVg / Rds + dWeff_dVg / Weff;
         tmp3 = dRds_dVb / Rds + dWeff_dVb / Weff;
@@ -534,7 +556,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         tmp2 = dWeff_dVg / Weff;
         tmp3 = dWeff_dVb / Weff;
     }
-    double Vdsat, dVdsat_dVg, dVdsat_dVd, dVdsat_dVb;
+    T Vdsat, dVdsat_dVg, dVdsat_dVd, dVdsat_dVb;
     if ((Rds == 0.0) && (Lambda == 1.0)) {
         T0 = 1.0 / (Abulk * EsatL + Vgst2Vtm);
         tmp1 = 0.0;
@@ -542,9 +564,9 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         T2 = Vgst2Vtm * T0;
         T3 = EsatL * Vgst2Vtm;
         Vdsat = T3 * T0;
-        const double dT0_dVg = -(Abulk * dEsatL_dVg + EsatL * dAbulk_dVg + 1.0) * T1;
-        const double dT0_dVd = -(Abulk * dEsatL_dVd) * T1;
-        const double dT0_dVb = -(Abulk * dEsatL_dVb + dAbulk_dVb * EsatL) * T1;
+        const T dT0_dVg = -(Abulk * dEsatL_dVg + EsatL * dAbulk_dVg + 1.0) * T1;
+        const T dT0_dVd = -(Abulk * dEsatL_dVd) * T1;
+        const T dT0_dVb = -(Abulk * dEsatL_dVb + dAbulk_dVb * EsatL) * T1;
         dVdsat_dVg = T3 * dT0_dVg + T2 * dEsatL_dVg + EsatL * T0;
         dVdsat_dVd = T3 * dT0_dVd + T2 * dEsatL_dVd;
         dVdsat_dVb = T3 * dT0_dVb + T2 * dEsatL_dVb;
@@ -555,21 +577,21 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         T7 = Vgst2Vtm * T9;
         T6 = Vgst2Vtm * WVCoxRds;
         T0 = 2.0 * Abulk * (T9 - 1.0 + 1.0 / Lambda);
-        const double dT0_dVg = 2.0 * (T8 * tmp2 - Abulk * tmp1 +
+        const T dT0_dVg = 2.0 * (T8 * tmp2 - Abulk * tmp1 +
                                       (2.0 * T9 + 1.0 / Lambda - 1.0) * dAbulk_dVg);
-        const double dT0_dVb = 2.0 * (T8 * (2.0 / Abulk * dAbulk_dVb + tmp3) +
+        const T dT0_dVb = 2.0 * (T8 * (2.0 / Abulk * dAbulk_dVb + tmp3) +
                                       (1.0 / Lambda - 1.0) * dAbulk_dVb);
-        const double dT0_dVd = 0.0;
+        const T dT0_dVd = 0.0;
         T1 = Vgst2Vtm * (2.0 / Lambda - 1.0) + Abulk * EsatL + 3.0 * T7;
-        const double dT1_dVg = (2.0 / Lambda - 1.0) - 2.0 * Vgst2Vtm * tmp1 + Abulk * dEsatL_dVg +
+        const T dT1_dVg = (2.0 / Lambda - 1.0) - 2.0 * Vgst2Vtm * tmp1 + Abulk * dEsatL_dVg +
                                EsatL * dAbulk_dVg + 3.0 * (T9 + T7 * tmp2 + T6 * dAbulk_dVg);
-        const double dT1_dVb =
+        const T dT1_dVb =
             Abulk * dEsatL_dVb + EsatL * dAbulk_dVb + 3.0 * (T6 * dAbulk_dVb + T7 * tmp3);
-        const double dT1_dVd = Abulk * dEsatL_dVd;
+        const T dT1_dVd = Abulk * dEsatL_dVd;
         T2 = Vgst2Vtm * (EsatL + 2.0 * T6);
-        const double dT2_dVg = EsatL + Vgst2Vtm * dEsatL_dVg + T6 * (4.0 + 2.0 * Vgst2Vtm * tmp2);
-        const double dT2_dVb = Vgst2Vtm * (dEsatL_dVb + 2.0 * T6 * tmp3);
-        const double dT2_dVd = Vgst2Vtm * dEsatL_dVd;
+        const T dT2_dVg = EsatL + Vgst2Vtm * dEsatL_dVg + T6 * (4.0 + 2.0 * Vgst2Vtm * tmp2);
+        const T dT2_dVb = Vgst2Vtm * (dEsatL_dVb + 2.0 * T6 * tmp3);
+        const T dT2_dVd = Vgst2Vtm * dEsatL_dVd;
         T3 = sqrt(T1 * T1 - 2.0 * T0 * T2);
         Vdsat = (T1 - T3) / T0;
         dVdsat_dVg = (dT1_dVg - (T1 * dT1_dVg - dT0_dVg * T2 - T0 * dT2_dVg) / T3 - Vdsat * dT0_dVg) / T0;
@@ -579,19 +601,19 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     }
 
     // ---- Vdseff (:1619-1657)
-    double Vdseff, dVdseff_dVg, dVdseff_dVd, dVdseff_dVb;
+    T Vdseff, dVdseff_dVg, dVdseff_dVd, dVdseff_dVb;
     {
         T1 = Vdsat - Vds - PF(delta);
-        const double dT1_dVg = dVdsat_dVg;
-        const double dT1_dVd = dVdsat_dVd - 1.0;
-        const double dT1_dVb = dVdsat_dVb;
+        const T dT1_dVg = dVdsat_dVg;
+        const T dT1_dVd = dVdsat_dVd - 1.0;
+        const T dT1_dVb = dVdsat_dVb;
         T2 = sqrt(T1 * T1 + 4.0 * PF(delta) * Vdsat);
         T0 = T1 / T2;
         T9 = 2.0 * PF(delta);
         T3 = T9 / T2;
-        const double dT2_dVg = T0 * dT1_dVg + T3 * dVdsat_dVg;
-        const double dT2_dVd = T0 * dT1_dVd + T3 * dVdsat_dVd;
-        const double dT2_dVb = T0 * dT1_dVb + T3 * dVdsat_dVb;
+        const T dT2_dVg = T0 * dT1_dVg + T3 * dVdsat_dVg;
+        const T dT2_dVd = T0 * dT1_dVd + T3 * dVdsat_dVd;
+        const T dT2_dVb = T0 * dT1_dVb + T3 * dVdsat_dVb;
         if (T1 >= 0.0) {
             Vdseff = Vdsat - 0.5 * (T1 + T2);
             dVdseff_dVg = dVdsat_dVg - 0.5 * (dT1_dVg + dT2_dVg);
@@ -613,25 +635,25 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         }
         if (Vdseff > Vds) Vdseff = Vds;
     }
-    const double diffVds = Vds - Vdseff;
+    const T diffVds = Vds - Vdseff;
 
     // ---- Vasat (:1702-1724)
-    double Vasat, dVasat_dVg, dVasat_dVb, dVasat_dVd;
+    T Vasat, dVasat_dVg, dVasat_dVb, dVasat_dVd;
     {
-        const double tmp4 = 1.0 - 0.5 * Abulk * Vdsat / Vgst2Vtm;
+        const T tmp4 = 1.0 - 0.5 * Abulk * Vdsat / Vgst2Vtm;
         T9 = WVCoxRds * Vgsteff;
         T8 = T9 / Vgst2Vtm;
         T0 = EsatL + Vdsat + 2.0 * T9 * tmp4;
         T7 = 2.0 * WVCoxRds * tmp4;
-        const double dT0_dVg = dEsatL_dVg + dVdsat_dVg + T7 * (1.0 + tmp2 * Vgsteff) -
+        const T dT0_dVg = dEsatL_dVg + dVdsat_dVg + T7 * (1.0 + tmp2 * Vgsteff) -
                                T8 * (Abulk * dVdsat_dVg - Abulk * Vdsat / Vgst2Vtm + Vdsat * dAbulk_dVg);
-        const double dT0_dVb = dEsatL_dVb + dVdsat_dVb + T7 * tmp3 * Vgsteff -
+        const T dT0_dVb = dEsatL_dVb + dVdsat_dVb + T7 * tmp3 * Vgsteff -
                                T8 * (dAbulk_dVb * Vdsat + Abulk * dVdsat_dVb);
-        const double dT0_dVd = dEsatL_dVd + dVdsat_dVd - T8 * Abulk * dVdsat_dVd;
+        const T dT0_dVd = dEsatL_dVd + dVdsat_dVd - T8 * Abulk * dVdsat_dVd;
         T9 = WVCoxRds * Abulk;
         T1 = 2.0 / Lambda - 1.0 + T9;
-        const double dT1_dVg = -2.0 * tmp1 + WVCoxRds * (Abulk * tmp2 + dAbulk_dVg);
-        const double dT1_dVb = dAbulk_dVb * WVCoxRds + T9 * tmp3;
+        const T dT1_dVg = -2.0 * tmp1 + WVCoxRds * (Abulk * tmp2 + dAbulk_dVg);
+        const T dT1_dVb = dAbulk_dVb * WVCoxRds + T9 * tmp3;
         Vasat = T0 / T1;
         dVasat_dVg = (dT0_dVg - Vasat * dT1_dVg) / T1;
         dVasat_dVb = (dT0_dVb - Vasat * dT1_dVb) / T1;
@@ -639,43 +661,43 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     }
 
     // ---- Idl (:1728-1785)
-    double Idl, dIdl_dVg, dIdl_dVd, dIdl_dVb;
+    T Idl, dIdl_dVg, dIdl_dVd, dIdl_dVb;
     {
         tmp1 = PF(vtfbphi2);
         tmp2 = 2.0e8 * PF(toxp);
-        double dT0_dVg = 1.0 / tmp2;
+        T dT0_dVg = 1.0 / tmp2;
         T0 = (Vgsteff + tmp1) * dT0_dVg;
         tmp3 = exp(PF(bdos) * 0.7 * log(T0));
         T1 = 1.0 + tmp3;
         T2 = PF(bdos) * 0.7 * tmp3 / T0;
-        const double Tcen = PF(ados) * 1.9e-9 / T1;
-        const double dTcen_dVg = -Tcen * T2 * dT0_dVg / T1;
-        const double Coxeff = epssub * PF(coxp) / (epssub + PF(coxp) * Tcen);
-        const double dCoxeff_dVg = -Coxeff * Coxeff * dTcen_dVg / epssub;
-        const double CoxeffWovL = Coxeff * Weff / Leff;
-        const double beta = ueff * CoxeffWovL;
+        const T Tcen = PF(ados) * 1.9e-9 / T1;
+        const T dTcen_dVg = -Tcen * T2 * dT0_dVg / T1;
+        const T Coxeff = epssub * PF(coxp) / (epssub + PF(coxp) * Tcen);
+        const T dCoxeff_dVg = -Coxeff * Coxeff * dTcen_dVg / epssub;
+        const T CoxeffWovL = Coxeff * Weff / Leff;
+        const T beta = ueff * CoxeffWovL;
         T3 = ueff / Leff;
-        const double dbeta_dVg = CoxeffWovL * dueff_dVg + T3 * (Weff * dCoxeff_dVg + Coxeff * dWeff_dVg);
-        const double dbeta_dVd = CoxeffWovL * dueff_dVd;
-        const double dbeta_dVb = CoxeffWovL * dueff_dVb + T3 * Coxeff * dWeff_dVb;
-        const double AbovVgst2Vtm = Abulk / Vgst2Vtm;
+        const T dbeta_dVg = CoxeffWovL * dueff_dVg + T3 * (Weff * dCoxeff_dVg + Coxeff * dWeff_dVg);
+        const T dbeta_dVd = CoxeffWovL * dueff_dVd;
+        const T dbeta_dVb = CoxeffWovL * dueff_dVb + T3 * Coxeff * dWeff_dVb;
+        const T AbovVgst2Vtm = Abulk / Vgst2Vtm;
         T0 = 1.0 - 0.5 * Vdseff * AbovVgst2Vtm;
         dT0_dVg = -0.5 * (Abulk * dVdseff_dVg - Abulk * Vdseff / Vgst2Vtm + Vdseff * dAbulk_dVg) / Vgst2Vtm;
-        const double dT0_dVd = -0.5 * Abulk * dVdseff_dVd / Vgst2Vtm;
-        const double dT0_dVb = -0.5 * (Abulk * dVdseff_dVb + dAbulk_dVb * Vdseff) / Vgst2Vtm;
-        const double fgche1 = Vgsteff * T0;
-        const double dfgche1_dVg = Vgsteff * dT0_dVg + T0;
-        const double dfgche1_dVd = Vgsteff * dT0_dVd;
-        const double dfgche1_dVb = Vgsteff * dT0_dVb;
+        const T dT0_dVd = -0.5 * Abulk * dVdseff_dVd / Vgst2Vtm;
+        const T dT0_dVb = -0.5 * (Abulk * dVdseff_dVb + dAbulk_dVb * Vdseff) / Vgst2Vtm;
+        const T fgche1 = Vgsteff * T0;
+        const T dfgche1_dVg = Vgsteff * dT0_dVg + T0;
+        const T dfgche1_dVd = Vgsteff * dT0_dVd;
+        const T dfgche1_dVb = Vgsteff * dT0_dVb;
         T9 = Vdseff / EsatL;
-        const double fgche2 = 1.0 + T9;
-        const double dfgche2_dVg = (dVdseff_dVg - T9 * dEsatL_dVg) / EsatL;
-        const double dfgche2_dVd = (dVdseff_dVd - T9 * dEsatL_dVd) / EsatL;
-        const double dfgche2_dVb = (dVdseff_dVb - T9 * dEsatL_dVb) / EsatL;
-        const double gche = beta * fgche1 / fgche2;
-        const double dgche_dVg = (beta * dfgche1_dVg + fgche1 * dbeta_dVg - gche * dfgche2_dVg) / fgche2;
-        const double dgche_dVd = (beta * dfgche1_dVd + fgche1 * dbeta_dVd - gche * dfgche2_dVd) / fgche2;
-        const double dgche_dVb = (beta * dfgche1_dVb + fgche1 * dbeta_dVb - gche * dfgche2_dVb) / fgche2;
+        const T fgche2 = 1.0 + T9;
+        const T dfgche2_dVg = (dVdseff_dVg - T9 * dEsatL_dVg) / EsatL;
+        const T dfgche2_dVd = (dVdseff_dVd - T9 * dEsatL_dVd) / EsatL;
+        const T dfgche2_dVb = (dVdseff_dVb - T9 * dEsatL_dVb) / EsatL;
+        const T gche = beta * fgche1 / fgche2;
+        const T dgche_dVg = (beta * dfgche1_dVg + fgche1 * dbeta_dVg - gche * dfgche2_dVg) / fgche2;
+        const T dgche_dVd = (beta * dfgche1_dVd + fgche1 * dbeta_dVd - gche * dfgche2_dVd) / fgche2;
+        const T dgche_dVb = (beta * dfgche1_dVb + fgche1 * dbeta_dVb - gche * dfgche2_dVb) / fgche2;
         T0 = 1.0 + gche * Rds;
         Idl = gche / T0;
         T1 = (1.0 - Idl * Rds) / T0;
@@ -686,8 +708,8 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     }
 
     // ---- output resistance: FP = 1 (FPROUT = 0), VACLM, VADIBL (:1789-1897)
-    const double FP = 1.0, dFP_dVg = 0.0;
-    double PvagTerm, dPvagTerm_dVg, dPvagTerm_dVb, dPvagTerm_dVd;
+    const T FP = 1.0, dFP_dVg = 0.0;
+    T PvagTerm, dPvagTerm_dVg, dPvagTerm_dVb, dPvagTerm_dVd;
     T8 = PF(pvag) / EsatL;
     T9 = T8 * Vgsteff;
     if (T9 > -0.9) {
@@ -704,17 +726,17 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         dPvagTerm_dVb = -T9 * dEsatL_dVb;
         dPvagTerm_dVd = -T9 * dEsatL_dVd;
     }
-    double Cclm, dCclm_dVg, dCclm_dVd, dCclm_dVb, VACLM, dVACLM_dVg, dVACLM_dVd, dVACLM_dVb;
+    T Cclm, dCclm_dVg, dCclm_dVd, dCclm_dVb, VACLM, dVACLM_dVg, dVACLM_dVd, dVACLM_dVb;
     if ((PF(pclm) > kMinExp) && (diffVds > 1.0e-10)) {
         T0 = 1.0 + Rds * Idl;
-        const double dT0_dVg = dRds_dVg * Idl + Rds * dIdl_dVg;
-        const double dT0_dVd = Rds * dIdl_dVd;
-        const double dT0_dVb = dRds_dVb * Idl + Rds * dIdl_dVb;
+        const T dT0_dVg = dRds_dVg * Idl + Rds * dIdl_dVg;
+        const T dT0_dVd = Rds * dIdl_dVd;
+        const T dT0_dVb = dRds_dVb * Idl + Rds * dIdl_dVb;
         T2 = Vdsat / Esat;
         T1 = Leff + T2;
-        const double dT1_dVg = (dVdsat_dVg - T2 * dEsatL_dVg / Leff) / Esat;
-        const double dT1_dVd = (dVdsat_dVd - T2 * dEsatL_dVd / Leff) / Esat;
-        const double dT1_dVb = (dVdsat_dVb - T2 * dEsatL_dVb / Leff) / Esat;
+        const T dT1_dVg = (dVdsat_dVg - T2 * dEsatL_dVg / Leff) / Esat;
+        const T dT1_dVd = (dVdsat_dVd - T2 * dEsatL_dVd / Leff) / Esat;
+        const T dT1_dVb = (dVdsat_dVb - T2 * dEsatL_dVb / Leff) / Esat;
         Cclm = FP * PvagTerm * T0 * T1 / (PF(pclm) * PF(litl));
         dCclm_dVg = Cclm * (dFP_dVg / FP + dPvagTerm_dVg / PvagTerm + dT0_dVg / T0 + dT1_dVg / T1);
         dCclm_dVb = Cclm * (dPvagTerm_dVb / PvagTerm + dT0_dVb / T0 + dT1_dVb / T1);
@@ -728,17 +750,17 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         dVACLM_dVd = dVACLM_dVg = dVACLM_dVb = 0.0;
         dCclm_dVd = dCclm_dVg = dCclm_dVb = 0.0;
     }
-    double VADIBL, dVADIBL_dVg, dVADIBL_dVb, dVADIBL_dVd;
+    T VADIBL, dVADIBL_dVg, dVADIBL_dVb, dVADIBL_dVd;
     if (PF(thetaRout) > kMinExp) {
         T8 = Abulk * Vdsat;
         T0 = Vgst2Vtm * T8;
-        const double dT0_dVg = Vgst2Vtm * Abulk * dVdsat_dVg + T8 + Vgst2Vtm * Vdsat * dAbulk_dVg;
-        const double dT0_dVb = Vgst2Vtm * (dAbulk_dVb * Vdsat + Abulk * dVdsat_dVb);
-        const double dT0_dVd = Vgst2Vtm * Abulk * dVdsat_dVd;
+        const T dT0_dVg = Vgst2Vtm * Abulk * dVdsat_dVg + T8 + Vgst2Vtm * Vdsat * dAbulk_dVg;
+        const T dT0_dVb = Vgst2Vtm * (dAbulk_dVb * Vdsat + Abulk * dVdsat_dVb);
+        const T dT0_dVd = Vgst2Vtm * Abulk * dVdsat_dVd;
         T1 = Vgst2Vtm + T8;
-        const double dT1_dVg = 1.0 + Abulk * dVdsat_dVg + Vdsat * dAbulk_dVg;
-        const double dT1_dVb = Abulk * dVdsat_dVb + dAbulk_dVb * Vdsat;
-        const double dT1_dVd = Abulk * dVdsat_dVd;
+        const T dT1_dVg = 1.0 + Abulk * dVdsat_dVg + Vdsat * dAbulk_dVg;
+        const T dT1_dVb = Abulk * dVdsat_dVb + dAbulk_dVb * Vdsat;
+        const T dT1_dVd = Abulk * dVdsat_dVd;
         T9 = T1 * T1;
         T2 = PF(thetaRout);
         VADIBL = (Vgst2Vtm - T0 / T1) / T2;
@@ -768,14 +790,14 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         VADIBL = kMaxExp;
         dVADIBL_dVd = dVADIBL_dVg = dVADIBL_dVb = 0.0;
     }
-    const double Va = Vasat + VACLM;
-    const double dVa_dVg = dVasat_dVg + dVACLM_dVg;
-    const double dVa_dVb = dVasat_dVb + dVACLM_dVb;
-    const double dVa_dVd = dVasat_dVd + dVACLM_dVd;
+    const T Va = Vasat + VACLM;
+    const T dVa_dVg = dVasat_dVg + dVACLM_dVg;
+    const T dVa_dVb = dVasat_dVb + dVACLM_dVb;
+    const T dVa_dVd = dVasat_dVd + dVACLM_dVd;
     // VADITS with PDITS = 0 (:1905-1926)
-    const double VADITS = kMaxExp, dVADITS_dVg = 0.0, dVADITS_dVd = 0.0;
+    const T VADITS = kMaxExp, dVADITS_dVg = 0.0, dVADITS_dVd = 0.0;
     // VASCBE (:1929-1947)
-    double VASCBE, dVASCBE_dVg, dVASCBE_dVd, dVASCBE_dVb;
+    T VASCBE, dVASCBE_dVg, dVASCBE_dVd, dVASCBE_dVb;
     if ((PF(pscbe2) > 0.0) && (PF(pscbe1) >= 0.0)) {
         if (diffVds > PF(pscbe1) * PF(litl) / kExpThreshold) {
             T0 = PF(pscbe1) * PF(litl) / diffVds;
@@ -794,7 +816,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     }
 
     // ---- Ids (:1950-2047)
-    double Idsa, dIdsa_dVg, dIdsa_dVd, dIdsa_dVb;
+    T Idsa, dIdsa_dVg, dIdsa_dVd, dIdsa_dVb;
     T9 = diffVds / VADIBL;
     T0 = 1.0 + T9;
     Idsa = Idl * T0;
@@ -809,14 +831,14 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     Idsa *= T0;
     {
         T0 = log(Va / Vasat);
-        const double dT0_dVg = dVa_dVg / Va - dVasat_dVg / Vasat;
-        const double dT0_dVb = dVa_dVb / Va - dVasat_dVb / Vasat;
-        const double dT0_dVd = dVa_dVd / Va - dVasat_dVd / Vasat;
+        const T dT0_dVg = dVa_dVg / Va - dVasat_dVg / Vasat;
+        const T dT0_dVb = dVa_dVb / Va - dVasat_dVb / Vasat;
+        const T dT0_dVd = dVa_dVd / Va - dVasat_dVd / Vasat;
         T1 = T0 / Cclm;
         T9 = 1.0 + T1;
-        const double dT9_dVg2 = (dT0_dVg - T1 * dCclm_dVg) / Cclm;
-        const double dT9_dVb2 = (dT0_dVb - T1 * dCclm_dVb) / Cclm;
-        const double dT9_dVd2 = (dT0_dVd - T1 * dCclm_dVd) / Cclm;
+        const T dT9_dVg2 = (dT0_dVg - T1 * dCclm_dVg) / Cclm;
+        const T dT9_dVb2 = (dT0_dVb - T1 * dCclm_dVb) / Cclm;
+        const T dT9_dVd2 = (dT0_dVd - T1 * dCclm_dVd) / Cclm;
         dIdsa_dVg = dIdsa_dVg * T9 + Idsa * dT9_dVg2;
         dIdsa_dVb = dIdsa_dVb * T9 + Idsa * dT9_dVb2;
         dIdsa_dVd = dIdsa_dVd * T9 + Idsa * dT9_dVd2;
@@ -824,25 +846,25 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     }
     T9 = diffVds / VASCBE;
     T0 = 1.0 + T9;
-    const double Ids = Idsa * T0;
-    double Gm = T0 * dIdsa_dVg - Idsa * (dVdseff_dVg + T9 * dVASCBE_dVg) / VASCBE;
-    double Gds = T0 * dIdsa_dVd + Idsa * (1.0 - dVdseff_dVd - T9 * dVASCBE_dVd) / VASCBE;
-    double Gmb = T0 * dIdsa_dVb - Idsa * (dVdseff_dVb + T9 * dVASCBE_dVb) / VASCBE;
+    const T Ids = Idsa * T0;
+    T Gm = T0 * dIdsa_dVg - Idsa * (dVdseff_dVg + T9 * dVASCBE_dVg) / VASCBE;
+    T Gds = T0 * dIdsa_dVd + Idsa * (1.0 - dVdseff_dVd - T9 * dVASCBE_dVd) / VASCBE;
+    T Gmb = T0 * dIdsa_dVb - Idsa * (dVdseff_dVb + T9 * dVASCBE_dVb) / VASCBE;
     tmp1 = Gds + Gm * dVgsteff_dVd;
     tmp2 = Gmb + Gm * dVgsteff_dVb;
     tmp3 = Gm;
     Gm = (Ids * dVdseff_dVg + Vdseff * tmp3) * dVgsteff_dVg;
     Gds = Ids * (dVdseff_dVd + dVdseff_dVg * dVgsteff_dVd) + Vdseff * tmp1;
     Gmb = (Ids * (dVdseff_dVb + dVdseff_dVg * dVgsteff_dVb) + Vdseff * tmp2) * dVbseff_dVb;
-    const double cdrain = Ids * Vdseff;
-    const double h_gds_new = Gds, h_gm_new = Gm, h_gmbs_new = Gmb;
+    const T cdrain = Ids * Vdseff;
+    const T h_gds_new = Gds, h_gm_new = Gm, h_gmbs_new = Gmb;
 
     // ---- C-V, capMod = 2 (charge-thickness model, :2910-3810), XPART < 0.5
-    double qgate = 0.0, qbulk = 0.0, qdrn = 0.0;
-    double cggb = 0.0, cgsb = 0.0, cgdb = 0.0, cdgb = 0.0, cdsb = 0.0, cddb = 0.0;
-    double cbgb = 0.0, cbsb = 0.0, cbdb = 0.0;
+    T qgate = 0.0, qbulk = 0.0, qdrn = 0.0;
+    T cggb = 0.0, cgsb = 0.0, cgdb = 0.0, cdgb = 0.0, cdsb = 0.0, cddb = 0.0;
+    T cbgb = 0.0, cbsb = 0.0, cbdb = 0.0;
     if (charge) {
-        double VbseffCV, dVbseffCV_dVb;
+        T VbseffCV, dVbseffCV_dVb;
         if (Vbseff < 0.0) {
             VbseffCV = Vbseff;
             dVbseffCV_dVb = 1.0;
@@ -850,14 +872,14 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
             VbseffCV = PF(phi) - Phis;
             dVbseffCV_dVb = -dPhis_dVb;
         }
-        const double CoxWL = PF(coxe) * PF(weffCV) * PF(leffCV) * 1.0;
+        const T CoxWL = PF(coxe) * PF(weffCV) * PF(leffCV) * 1.0;
         {
-            const double noff = n * PF(noff);
-            const double dnoff_dVd = PF(noff) * dn_dVd;
-            const double dnoff_dVb = PF(noff) * dn_dVb;
+            const T noff = n * PF(noff);
+            const T dnoff_dVd = PF(noff) * dn_dVd;
+            const T dnoff_dVb = PF(noff) * dn_dVb;
             T0 = Vtm * noff;
-            const double voffcv = PF(voffcv);
-            const double VgstNVt = (Vgst - voffcv) / T0;
+            const T voffcv = PF(voffcv);
+            const T VgstNVt = (Vgst - voffcv) / T0;
             if (VgstNVt > kExpThreshold) {
                 Vgsteff = Vgst - voffcv;
                 dVgsteff_dVg = dVgs_eff_dVg;
@@ -870,7 +892,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
                 dVgsteff_dVb = dVgsteff_dVd * dnoff_dVb;
                 dVgsteff_dVd *= dnoff_dVd;
             } else {
-                const double ExpVgst = exp(VgstNVt);
+                const T ExpVgst = exp(VgstNVt);
                 Vgsteff = T0 * log(1.0 + ExpVgst);
                 dVgsteff_dVg = ExpVgst / (1.0 + ExpVgst);
                 dVgsteff_dVd = -dVgsteff_dVg * (dVth_dVd + (Vgst - voffcv) / noff * dnoff_dVd) +
@@ -880,20 +902,20 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
                 dVgsteff_dVg *= dVgs_eff_dVg;
             }
         }
-        double V3 = PF(vfbzb) - Vgs_eff + VbseffCV - kDelta3;
+        T V3 = PF(vfbzb) - Vgs_eff + VbseffCV - kDelta3;
         if (PF(vfbzb) <= 0.0) T0 = sqrt(V3 * V3 - 4.0 * kDelta3 * PF(vfbzb));
         else T0 = sqrt(V3 * V3 + 4.0 * kDelta3 * PF(vfbzb));
         T1 = 0.5 * (1.0 + V3 / T0);
-        const double Vfbeff = PF(vfbzb) - 0.5 * (V3 + T0);
-        const double dVfbeff_dVg = T1 * dVgs_eff_dVg;
-        const double dVfbeff_dVb = -T1 * dVbseffCV_dVb;
-        const double Cox = PF(coxp);
-        double Tox = 1.0e8 * PF(toxp);
+        const T Vfbeff = PF(vfbzb) - 0.5 * (V3 + T0);
+        const T dVfbeff_dVg = T1 * dVgs_eff_dVg;
+        const T dVfbeff_dVb = -T1 * dVbseffCV_dVb;
+        const T Cox = PF(coxp);
+        T Tox = 1.0e8 * PF(toxp);
         T0 = (Vgs_eff - VbseffCV - PF(vfbzb)) / Tox;
-        const double dT0_dVg = dVgs_eff_dVg / Tox;
-        const double dT0_dVb = -dVbseffCV_dVb / Tox;
-        double Tcen, dTcen_dVg, dTcen_dVb, dTcen_dVd;
-        double tmp = T0 * PF(acde);
+        const T dT0_dVg = dVgs_eff_dVg / Tox;
+        const T dT0_dVb = -dVbseffCV_dVb / Tox;
+        T Tcen, dTcen_dVg, dTcen_dVb, dTcen_dVd;
+        T tmp = T0 * PF(acde);
         if ((-kExpThreshold < tmp) && (tmp < kExpThreshold)) {
             Tcen = PF(ldeb) * exp(tmp);
             dTcen_dVg = PF(acde) * Tcen;
@@ -906,25 +928,25 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
             Tcen = PF(ldeb) * kMaxExp;
             dTcen_dVg = dTcen_dVb = 0.0;
         }
-        const double LINK = 1.0e-3 * PF(toxp);
+        const T LINK = 1.0e-3 * PF(toxp);
         V3 = PF(ldeb) - Tcen - LINK;
-        const double V4 = sqrt(V3 * V3 + 4.0 * LINK * PF(ldeb));
+        const T V4 = sqrt(V3 * V3 + 4.0 * LINK * PF(ldeb));
         Tcen = PF(ldeb) - 0.5 * (V3 + V4);
         T1 = 0.5 * (1.0 + V3 / V4);
         dTcen_dVg *= T1;
         dTcen_dVb *= T1;
-        double Ccen = epssub / Tcen;
+        T Ccen = epssub / Tcen;
         T2 = Cox / (Cox + Ccen);
-        double Coxeff = T2 * Ccen;
+        T Coxeff = T2 * Ccen;
         T3 = -Ccen / Tcen;
-        double dCoxeff_dVg = T2 * T2 * T3;
-        double dCoxeff_dVb = dCoxeff_dVg * dTcen_dVb;
+        T dCoxeff_dVg = T2 * T2 * T3;
+        T dCoxeff_dVb = dCoxeff_dVg * dTcen_dVb;
         dCoxeff_dVg *= dTcen_dVg;
-        double CoxWLcen = CoxWL * Coxeff / PF(coxe);
-        const double Qac0 = CoxWLcen * (Vfbeff - PF(vfbzb));
-        double QovCox = Qac0 / Coxeff;
-        const double dQac0_dVg = CoxWLcen * dVfbeff_dVg + QovCox * dCoxeff_dVg;
-        const double dQac0_dVb = CoxWLcen * dVfbeff_dVb + QovCox * dCoxeff_dVb;
+        T CoxWLcen = CoxWL * Coxeff / PF(coxe);
+        const T Qac0 = CoxWLcen * (Vfbeff - PF(vfbzb));
+        T QovCox = Qac0 / Coxeff;
+        const T dQac0_dVg = CoxWLcen * dVfbeff_dVg + QovCox * dCoxeff_dVg;
+        const T dQac0_dVb = CoxWLcen * dVfbeff_dVb + QovCox * dCoxeff_dVb;
         T0 = 0.5 * PF(k1ox);
         T3 = Vgs_eff - Vfbeff - VbseffCV - Vgsteff;
         if (PF(k1ox) == 0.0) {
@@ -937,14 +959,14 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
             T1 = sqrt(T0 * T0 + T3);
             T2 = CoxWLcen * T0 / T1;
         }
-        const double Qsub0 = CoxWLcen * PF(k1ox) * (T1 - T0);
+        const T Qsub0 = CoxWLcen * PF(k1ox) * (T1 - T0);
         QovCox = Qsub0 / Coxeff;
-        const double dQsub0_dVg = T2 * (dVgs_eff_dVg - dVfbeff_dVg - dVgsteff_dVg) + QovCox * dCoxeff_dVg;
-        const double dQsub0_dVd = -T2 * dVgsteff_dVd;
-        const double dQsub0_dVb =
+        const T dQsub0_dVg = T2 * (dVgs_eff_dVg - dVfbeff_dVg - dVgsteff_dVg) + QovCox * dCoxeff_dVg;
+        const T dQsub0_dVd = -T2 * dVgsteff_dVd;
+        const T dQsub0_dVb =
             -T2 * (dVfbeff_dVb + dVbseffCV_dVb + dVgsteff_dVb) + QovCox * dCoxeff_dVb;
         // gate-bias dependent delta Phis
-        double Denomi;
+        T Denomi;
         if (PF(k1ox) <= 0.0) {
             Denomi = 0.25 * PF(moin) * Vtm;
             T0 = 0.5 * PF(sqrtPhi);
@@ -953,13 +975,13 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
             T0 = PF(k1ox) * PF(sqrtPhi);
         }
         T1 = 2.0 * T0 + Vgsteff;
-        const double DeltaPhi = Vtm * log(1.0 + T1 * Vgsteff / Denomi);
-        const double dDeltaPhi_dVg = 2.0 * Vtm * (T1 - T0) / (Denomi + T1 * Vgsteff);
+        const T DeltaPhi = Vtm * log(1.0 + T1 * Vgsteff / Denomi);
+        const T dDeltaPhi_dVg = 2.0 * Vtm * (T1 - T0) / (Denomi + T1 * Vgsteff);
         T0 = Vgsteff - DeltaPhi - 0.001;
-        const double dT0_dVg2 = 1.0 - dDeltaPhi_dVg;
+        const T dT0_dVg2 = 1.0 - dDeltaPhi_dVg;
         T1 = sqrt(T0 * T0 + Vgsteff * 0.004);
-        const double VgDP = 0.5 * (T0 + T1);
-        const double dVgDP_dVg = 0.5 * (dT0_dVg2 + (T0 * dT0_dVg2 + 0.002) / T1);
+        const T VgDP = 0.5 * (T0 + T1);
+        const T dVgDP_dVg = 0.5 * (dT0_dVg2 + (T0 * dT0_dVg2 + 0.002) / T1);
         Tox += Tox;
         T0 = (Vgsteff + PF(vtfbphi2)) / Tox;
         tmp = exp(PF(bdos) * 0.7 * log(T0));
@@ -975,22 +997,22 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         Coxeff = T0 * Ccen;
         T1 = -Ccen / Tcen;
         dCoxeff_dVg = T0 * T0 * T1;
-        const double dCoxeff_dVd = dCoxeff_dVg * dTcen_dVd;
+        const T dCoxeff_dVd = dCoxeff_dVg * dTcen_dVd;
         dCoxeff_dVb = dCoxeff_dVg * dTcen_dVb;
         dCoxeff_dVg *= dTcen_dVg;
         CoxWLcen = CoxWL * Coxeff / PF(coxe);
-        const double AbulkCV = Abulk0 * PF(abulkCVfactor);
-        const double dAbulkCV_dVb = PF(abulkCVfactor) * dAbulk0_dVb;
-        const double VdsatCV = VgDP / AbulkCV;
+        const T AbulkCV = Abulk0 * PF(abulkCVfactor);
+        const T dAbulkCV_dVb = PF(abulkCVfactor) * dAbulk0_dVb;
+        const T VdsatCV = VgDP / AbulkCV;
         T0 = VdsatCV - Vds - kDelta4;
-        const double dT0_dVg3 = dVgDP_dVg / AbulkCV;
-        const double dT0_dVb3 = -VdsatCV * dAbulkCV_dVb / AbulkCV;
+        const T dT0_dVg3 = dVgDP_dVg / AbulkCV;
+        const T dT0_dVb3 = -VdsatCV * dAbulkCV_dVb / AbulkCV;
         T1 = sqrt(T0 * T0 + 4.0 * kDelta4 * VdsatCV);
-        double dT1_dVg = (T0 + kDelta4 + kDelta4) / T1;
-        const double dT1_dVd = -T0 / T1;
-        const double dT1_dVb = dT1_dVg * dT0_dVb3;
+        T dT1_dVg = (T0 + kDelta4 + kDelta4) / T1;
+        const T dT1_dVd = -T0 / T1;
+        const T dT1_dVb = dT1_dVg * dT0_dVb3;
         dT1_dVg *= dT0_dVg3;
-        double VdseffCV, dVdseffCV_dVg, dVdseffCV_dVd, dVdseffCV_dVb;
+        T VdseffCV, dVdseffCV_dVg, dVdseffCV_dVd, dVdseffCV_dVb;
         if (T0 >= 0.0) {
             VdseffCV = VdsatCV - 0.5 * (T0 + T1);
             dVdseffCV_dVg = 0.5 * (dT0_dVg3 - dT1_dVg);
@@ -1019,9 +1041,9 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         T6 = T5 * VdseffCV / AbulkCV;
         qgate = CoxWLcen * (T1 - T0 * (0.5 - T3));
         QovCox = qgate / Coxeff;
-        double Cgg1 = CoxWLcen * (T4 * dVgDP_dVg + T5 * dVdseffCV_dVg);
-        const double Cgd1 = CoxWLcen * T5 * dVdseffCV_dVd + Cgg1 * dVgsteff_dVd + QovCox * dCoxeff_dVd;
-        const double Cgb1 = CoxWLcen * (T5 * dVdseffCV_dVb + T6 * dAbulkCV_dVb) +
+        T Cgg1 = CoxWLcen * (T4 * dVgDP_dVg + T5 * dVdseffCV_dVg);
+        const T Cgd1 = CoxWLcen * T5 * dVdseffCV_dVd + Cgg1 * dVgsteff_dVd + QovCox * dCoxeff_dVd;
+        const T Cgb1 = CoxWLcen * (T5 * dVdseffCV_dVb + T6 * dAbulkCV_dVb) +
                             Cgg1 * dVgsteff_dVb + QovCox * dCoxeff_dVb;
         Cgg1 = Cgg1 * dVgsteff_dVg + QovCox * dCoxeff_dVg;
         T7 = 1.0 - AbulkCV;
@@ -1032,34 +1054,34 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         T12 = -(T9 * T1 / AbulkCV + VdseffCV * (0.5 - T0 / T2));
         qbulk = CoxWLcen * T7 * (0.5 * VdseffCV - T0 * VdseffCV / T2);
         QovCox = qbulk / Coxeff;
-        double Cbg1 = CoxWLcen * (T10 + T11 * dVdseffCV_dVg);
-        const double Cbd1 = CoxWLcen * T11 * dVdseffCV_dVd + Cbg1 * dVgsteff_dVd + QovCox * dCoxeff_dVd;
-        const double Cbb1 = CoxWLcen * (T11 * dVdseffCV_dVb + T12 * dAbulkCV_dVb) +
+        T Cbg1 = CoxWLcen * (T10 + T11 * dVdseffCV_dVg);
+        const T Cbd1 = CoxWLcen * T11 * dVdseffCV_dVd + Cbg1 * dVgsteff_dVd + QovCox * dCoxeff_dVd;
+        const T Cbb1 = CoxWLcen * (T11 * dVdseffCV_dVb + T12 * dAbulkCV_dVb) +
                             Cbg1 * dVgsteff_dVb + QovCox * dCoxeff_dVb;
         Cbg1 = Cbg1 * dVgsteff_dVg + QovCox * dCoxeff_dVg;
         // 40/60 partition
         T2 = T2 / 12.0;
         T3 = 0.5 * CoxWLcen / (T2 * T2);
         T4 = T1 * (2.0 * T0 * T0 / 3.0 + T1 * (T1 - 4.0 * T0 / 3.0)) - 2.0 * T0 * T0 * T0 / 15.0;
-        const double qsrc0 = -T3 * T4;
+        const T qsrc0 = -T3 * T4;
         QovCox = qsrc0 / Coxeff;
         T8 = 4.0 / 3.0 * T1 * (T1 - T0) + 0.4 * T0 * T0;
         T5 = -2.0 * qsrc0 / T2 - T3 * (T1 * (3.0 * T1 - 8.0 * T0 / 3.0) + 2.0 * T0 * T0 / 3.0);
         T6 = AbulkCV * (qsrc0 / T2 + T3 * T8);
         T7 = T6 * VdseffCV / AbulkCV;
-        double Csg = T5 * dVgDP_dVg + T6 * dVdseffCV_dVg;
-        const double Csd = Csg * dVgsteff_dVd + T6 * dVdseffCV_dVd + QovCox * dCoxeff_dVd;
-        double Csb = Csg * dVgsteff_dVb + T6 * dVdseffCV_dVb + T7 * dAbulkCV_dVb + QovCox * dCoxeff_dVb;
+        T Csg = T5 * dVgDP_dVg + T6 * dVdseffCV_dVg;
+        const T Csd = Csg * dVgsteff_dVd + T6 * dVdseffCV_dVd + QovCox * dCoxeff_dVd;
+        T Csb = Csg * dVgsteff_dVb + T6 * dVdseffCV_dVb + T7 * dAbulkCV_dVb + QovCox * dCoxeff_dVb;
         Csg = Csg * dVgsteff_dVg + QovCox * dCoxeff_dVg;
         qgate += Qac0 + Qsub0 - qbulk;
         qbulk -= (Qac0 + Qsub0);
         qdrn = -(qgate + qbulk + qsrc0);
-        const double Cbg = Cbg1 - dQac0_dVg - dQsub0_dVg;
-        const double Cbd = Cbd1 - dQsub0_dVd;
-        double Cbb = Cbb1 - dQac0_dVb - dQsub0_dVb;
-        const double Cgg = Cgg1 - Cbg;
-        const double Cgd = Cgd1 - Cbd;
-        double Cgb = Cgb1 - Cbb;
+        const T Cbg = Cbg1 - dQac0_dVg - dQsub0_dVg;
+        const T Cbd = Cbd1 - dQsub0_dVd;
+        T Cbb = Cbb1 - dQac0_dVb - dQsub0_dVb;
+        const T Cgg = Cgg1 - Cbg;
+        const T Cgd = Cgd1 - Cbd;
+        T Cgb = Cgb1 - Cbb;
         Cgb *= dVbseff_dVb;
         Cbb *= dVbseff_dVb;
         Csb *= dVbseff_dVb;
@@ -1075,17 +1097,18 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     }
 
     // ---- junction C-V (:3843-3960)
-    double qbs = 0.0, qbd = 0.0, capbs = 0.0, capbd = 0.0;
+    T qbs = 0.0, qbd = 0.0, capbs = 0.0, capbd = 0.0;
+    if constexpr (!kGrad) {
     if (charge) {
-        const double czbd = PF(czbd), czbs = PF(czbs), czbdsw = PF(czbdsw), czbdswg = PF(czbdswg);
-        const double czbssw = PF(czbssw), czbsswg = PF(czbsswg);
+        const T czbd = PF(czbd), czbs = PF(czbs), czbdsw = PF(czbdsw), czbdswg = PF(czbdswg);
+        const T czbssw = PF(czbssw), czbsswg = PF(czbsswg);
         if (vbs_jct == 0.0) {
             qbs = 0.0;
             capbs = czbs + czbssw + czbsswg;
         } else if (vbs_jct < 0.0) {
             if (czbs > 0.0) {
-                const double arg = 1.0 - vbs_jct / PF(PhiBS);
-                const double sarg = (PF(MJS) == 0.5) ? 1.0 / sqrt(arg) : exp(-PF(MJS) * log(arg));
+                const T arg = 1.0 - vbs_jct / PF(PhiBS);
+                const T sarg = (PF(MJS) == 0.5) ? 1.0 / sqrt(arg) : exp(-PF(MJS) * log(arg));
                 qbs = PF(PhiBS) * czbs * (1.0 - arg * sarg) / (1.0 - PF(MJS));
                 capbs = czbs * sarg;
             } else {
@@ -1093,14 +1116,14 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
                 capbs = 0.0;
             }
             if (czbssw > 0.0) {
-                const double arg = 1.0 - vbs_jct / PF(PhiBSWS);
-                const double sarg = (PF(MJSWS) == 0.5) ? 1.0 / sqrt(arg) : exp(-PF(MJSWS) * log(arg));
+                const T arg = 1.0 - vbs_jct / PF(PhiBSWS);
+                const T sarg = (PF(MJSWS) == 0.5) ? 1.0 / sqrt(arg) : exp(-PF(MJSWS) * log(arg));
                 qbs += PF(PhiBSWS) * czbssw * (1.0 - arg * sarg) / (1.0 - PF(MJSWS));
                 capbs += czbssw * sarg;
             }
             if (czbsswg > 0.0) {
-                const double arg = 1.0 - vbs_jct / PF(PhiBSWGS);
-                const double sarg = (PF(MJSWGS) == 0.5) ? 1.0 / sqrt(arg) : exp(-PF(MJSWGS) * log(arg));
+                const T arg = 1.0 - vbs_jct / PF(PhiBSWGS);
+                const T sarg = (PF(MJSWGS) == 0.5) ? 1.0 / sqrt(arg) : exp(-PF(MJSWGS) * log(arg));
                 qbs += PF(PhiBSWGS) * czbsswg * (1.0 - arg * sarg) / (1.0 - PF(MJSWGS));
                 capbs += czbsswg * sarg;
             }
@@ -1116,8 +1139,8 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
             capbd = czbd + czbdsw + czbdswg;
         } else if (vbd_jct < 0.0) {
             if (czbd > 0.0) {
-                const double arg = 1.0 - vbd_jct / PF(PhiBD);
-                const double sarg = (PF(MJD) == 0.5) ? 1.0 / sqrt(arg) : exp(-PF(MJD) * log(arg));
+                const T arg = 1.0 - vbd_jct / PF(PhiBD);
+                const T sarg = (PF(MJD) == 0.5) ? 1.0 / sqrt(arg) : exp(-PF(MJD) * log(arg));
                 qbd = PF(PhiBD) * czbd * (1.0 - arg * sarg) / (1.0 - PF(MJD));
                 capbd = czbd * sarg;
             } else {
@@ -1125,14 +1148,14 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
                 capbd = 0.0;
             }
             if (czbdsw > 0.0) {
-                const double arg = 1.0 - vbd_jct / PF(PhiBSWD);
-                const double sarg = (PF(MJSWD) == 0.5) ? 1.0 / sqrt(arg) : exp(-PF(MJSWD) * log(arg));
+                const T arg = 1.0 - vbd_jct / PF(PhiBSWD);
+                const T sarg = (PF(MJSWD) == 0.5) ? 1.0 / sqrt(arg) : exp(-PF(MJSWD) * log(arg));
                 qbd += PF(PhiBSWD) * czbdsw * (1.0 - arg * sarg) / (1.0 - PF(MJSWD));
                 capbd += czbdsw * sarg;
             }
             if (czbdswg > 0.0) {
-                const double arg = 1.0 - vbd_jct / PF(PhiBSWGD);
-                const double sarg = (PF(MJSWGD) == 0.5) ? 1.0 / sqrt(arg) : exp(-PF(MJSWGD) * log(arg));
+                const T arg = 1.0 - vbd_jct / PF(PhiBSWGD);
+                const T sarg = (PF(MJSWGD) == 0.5) ? 1.0 / sqrt(arg) : exp(-PF(MJSWGD) * log(arg));
                 qbd += PF(PhiBSWGD) * czbdswg * (1.0 - arg * sarg) / (1.0 - PF(MJSWGD));
                 capbd += czbdswg * sarg;
             }
@@ -1143,8 +1166,9 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
             qbd = vbd_jct * (T0 + 0.5 * T1);
             capbd = T0 + T1;
         }
-        s0[S(qbsS)] = qbs;
-        s0[S(qbdS)] = qbd;
+        put(s0, S(qbsS), qbs);
+        put(s0, S(qbdS), qbd);
+    }
     }
 
     // ---- convergence (:3967-4011), with this load's currents
@@ -1152,44 +1176,45 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     if (Check == 1) {
         noncon = 1;
     } else if (!(mode & MODEADJOINT)) {
-        const double Idtot = (dmode >= 0) ? cdrain - cbd : cdrain + cbd;
-        const double tol0 = tol.reltol * ucb_max(fabs(cdhat), fabs(Idtot)) + tol.abstol;
+        const T Idtot = (dmode >= 0) ? cdrain - cbd : cdrain + cbd;
+        const T tol0 = tol.reltol * ucb_max(fabs(cdhat), fabs(Idtot)) + tol.abstol;
         if (fabs(cdhat - Idtot) >= tol0) {
             noncon = 1;
         } else {
-            const double Ibtot = cbs + cbd;
-            const double tol6 = tol.reltol * ucb_max(fabs(cbhat), fabs(Ibtot)) + tol.abstol;
+            const T Ibtot = cbs + cbd;
+            const T tol6 = tol.reltol * ucb_max(fabs(cbhat), fabs(Ibtot)) + tol.abstol;
             if (fabs(cbhat - Ibtot) > tol6) noncon = 1;
         }
     }
-    s0[S(vdsS)] = vds;
-    s0[S(vgsS)] = vgs;
-    s0[S(vbsS)] = vbs;
-    s0[S(vbdS)] = vbd;
+    put(s0, S(vdsS), vds);
+    put(s0, S(vgsS), vgs);
+    put(s0, S(vbsS), vbs);
+    put(s0, S(vbdS), vbd);
 
     // ---- charges and their companion conductances (:4026-4546)
-    double gcggb = 0.0, gcgdb = 0.0, gcgsb = 0.0, gcgbb = 0.0, gcdgb = 0.0, gcddb = 0.0,
+    T gcggb = 0.0, gcgdb = 0.0, gcgsb = 0.0, gcgbb = 0.0, gcdgb = 0.0, gcddb = 0.0,
            gcdsb = 0.0, gcdbb = 0.0, gcsgb = 0.0, gcsdb = 0.0, gcssb = 0.0, gcsbb = 0.0,
            gcbdb = 0.0, gcbgb = 0.0, gcbsb = 0.0, gcbbb = 0.0;
-    double ceqqg = 0.0, ceqqd = 0.0, ceqqb = 0.0;
+    T ceqqg = 0.0, ceqqd = 0.0, ceqqb = 0.0;
+    if constexpr (!kGrad) {
     if (charge) {
-        const double vgdx = vgd, vgsx = vgs;
+        const T vgdx = vgd, vgsx = vgs;
         T0 = vgdx + kDelta1;
         T1 = sqrt(T0 * T0 + 4.0 * kDelta1);
         T2 = 0.5 * (T0 - T1);
         T3 = PF(weffCV) * PF(cgdl);
         T4 = sqrt(1.0 - 4.0 * T2 / PF(ckappad));
-        const double cgdo = PF(cgdo) + T3 - T3 * (1.0 - 1.0 / T4) * (0.5 - 0.5 * T0 / T1);
-        const double qgdo = (PF(cgdo) + T3) * vgdx - T3 * (T2 + 0.5 * PF(ckappad) * (T4 - 1.0));
+        const T cgdo = PF(cgdo) + T3 - T3 * (1.0 - 1.0 / T4) * (0.5 - 0.5 * T0 / T1);
+        const T qgdo = (PF(cgdo) + T3) * vgdx - T3 * (T2 + 0.5 * PF(ckappad) * (T4 - 1.0));
         T0 = vgsx + kDelta1;
         T1 = sqrt(T0 * T0 + 4.0 * kDelta1);
         T2 = 0.5 * (T0 - T1);
         T3 = PF(weffCV) * PF(cgsl);
         T4 = sqrt(1.0 - 4.0 * T2 / PF(ckappas));
-        const double cgso = PF(cgso) + T3 - T3 * (1.0 - 1.0 / T4) * (0.5 - 0.5 * T0 / T1);
-        const double qgso = (PF(cgso) + T3) * vgsx - T3 * (T2 + 0.5 * PF(ckappas) * (T4 - 1.0));
-        const double ag0 = c.ag0;
-        double qsrc;
+        const T cgso = PF(cgso) + T3 - T3 * (1.0 - 1.0 / T4) * (0.5 - 0.5 * T0 / T1);
+        const T qgso = (PF(cgso) + T3) * vgsx - T3 * (T2 + 0.5 * PF(ckappas) * (T4 - 1.0));
+        const T ag0 = c.ag0;
+        T qsrc;
         if (dmode > 0) {
             qdrn -= qgdo;
             gcggb = (cggb + cgdo + cgso + PF(cgbo)) * ag0;
@@ -1199,7 +1224,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
             gcdgb = (cdgb - cgdo) * ag0;
             gcsgb = -(cggb + cbgb + cdgb + cgso) * ag0;
             gcbgb = (cbgb - PF(cgbo)) * ag0;
-            const double qgb = PF(cgbo) * vgb;
+            const T qgb = PF(cgbo) * vgb;
             qgate += qgdo + qgso + qgb;
             qbulk -= qgb;
             qsrc = -(qgate + qbulk + qdrn);
@@ -1221,7 +1246,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
             gcdgb = -(cggb + cbgb + cdgb + cgdo) * ag0;
             gcsgb = (cdgb - cgso) * ag0;
             gcbgb = (cbgb - PF(cgbo)) * ag0;
-            const double qgb = PF(cgbo) * vgb;
+            const T qgb = PF(cgbo) * vgb;
             qgate += qgdo + qgso + qgb;
             qbulk -= qgb;
             qdrn = -(qgate + qbulk + qsrc);
@@ -1235,32 +1260,33 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
             gcbsb = (cbdb - capbs) * ag0;
             gcbbb = -(gcbgb + gcbdb + gcbsb + 0.0);
         }
-        s0[S(qgS)] = qgate;
-        s0[S(qdS)] = qdrn - qbd;
-        s0[S(qsS)] = qsrc - qbs;
-        s0[S(qbS)] = qbulk + qbd + qbs;
+        put(s0, S(qgS), qgate);
+        put(s0, S(qdS), qdrn - qbd);
+        put(s0, S(qsS), qsrc - qbs);
+        put(s0, S(qbS), qbulk + qbd + qbs);
         if (mode & MODEINITTRAN) {
-            s1[S(qbS)] = s0[S(qbS)];
-            s1[S(qgS)] = s0[S(qgS)];
-            s1[S(qdS)] = s0[S(qdS)];
+            put(s1, S(qbS), s0[S(qbS)]);
+            put(s1, S(qgS), s0[S(qgS)]);
+            put(s1, S(qdS), s0[S(qdS)]);
         }
-        double geq, ceq;
+        T geq, ceq;
         niintegrate(c, s0, s1, s2, S(qbS), sB, 0.0, &geq, &ceq);
         niintegrate(c, s0, s1, s2, S(qgS), sB, 0.0, &geq, &ceq);
         niintegrate(c, s0, s1, s2, S(qdS), sB, 0.0, &geq, &ceq);
-        const double cqgate = s0[S(cqgS)], cqbody = s0[S(cqbS)], cqdrn = s0[S(cqdS)];
+        const T cqgate = s0[S(cqgS)], cqbody = s0[S(cqbS)], cqdrn = s0[S(cqdS)];
         ceqqg = cqgate - gcggb * vgb + gcgdb * vbd + gcgsb * vbs;
         ceqqd = cqdrn - gcdgb * vgb + gcddb * vbd + gcdsb * vbs;
         ceqqb = cqbody - gcbgb * vgb + gcbdb * vbd + gcbsb * vbs;
         if (mode & MODEINITTRAN) {
-            s1[S(cqbS)] = s0[S(cqbS)];
-            s1[S(cqgS)] = s0[S(cqgS)];
-            s1[S(cqdS)] = s0[S(cqdS)];
+            put(s1, S(cqbS), s0[S(cqbS)]);
+            put(s1, S(cqgS), s0[S(cqgS)]);
+            put(s1, S(cqdS), s0[S(cqdS)]);
         }
+    }
     }
 
     // ---- currents into the right-hand side (:4643-4914)
-    double Gmr, Gmbsr, FwdSum, RevSum, ceqdrn;
+    T Gmr, Gmbsr, FwdSum, RevSum, ceqdrn;
     if (dmode >= 0) {
         Gmr = Gm;
         Gmbsr = Gmb;
@@ -1274,7 +1300,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         RevSum = -(Gmr + Gmbsr);
         ceqdrn = -type * (cdrain + Gds * vds + Gmr * vgd + Gmbsr * vbd);
     }
-    double ceqjs, ceqjd;
+    T ceqjs, ceqjd;
     if (type > 0) {
         ceqjs = (cbs - gbs * vbs_jct);
         ceqjd = (cbd - gbd * vbd_jct);
@@ -1285,46 +1311,100 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         ceqqd = -ceqqd;
         ceqqb = -ceqqb;
     }
-    double* r = cr + (size_t)(t.rsite + 4 * d) * sB + inst;
-    r[0] = (ceqjd - ceqdrn - ceqqd);                              // dNodePrime +=
-    r[sB] = -(ceqqg);                                             // gNodePrime -=
-    r[2 * sB] = (-ceqjd - ceqjs - ceqqb);                         // bNodePrime +=
-    r[3 * sB] = (ceqdrn + ceqjs + ceqqg + ceqqb + ceqqd);         // sNodePrime +=
+    T rv[4], mv[16];
+    rv[0] = (ceqjd - ceqdrn - ceqqd);                              // dNodePrime +=
+    rv[1] = -(ceqqg);                                             // gNodePrime -=
+    rv[2] = (-ceqjd - ceqjs - ceqqb);                         // bNodePrime +=
+    rv[3] = (ceqdrn + ceqjs + ceqqg + ceqqb + ceqqd);         // sNodePrime +=
 
     // ---- matrix (:4980-5024), stamp order of the reference
-    const double gjbd = gbd, gjbs = gbs;
-    double* m = cm + (size_t)(t.site + 16 * d) * sB + inst;
-    m[0] = gcggb;                                                  // GPgp
-    m[sB] = gcgdb;                                                 // GPdp
-    m[2 * sB] = gcgsb;                                             // GPsp
-    m[3 * sB] = gcgbb;                                             // GPbp
-    m[4 * sB] = (Gds + gbd + RevSum + gcddb);                      // DPdp
-    m[5 * sB] = (Gmr + gcdgb);                                     // DPgp
-    m[6 * sB] = -(Gds + FwdSum - gcdsb);                           // DPsp
-    m[7 * sB] = -(gjbd - Gmbsr - gcdbb);                           // DPbp
-    m[8 * sB] = -(Gds + RevSum - gcsdb);                           // SPdp
-    m[9 * sB] = (gcsgb - Gmr);                                     // SPgp
-    m[10 * sB] = (Gds + gbs + FwdSum + gcssb);                     // SPsp
-    m[11 * sB] = -(gjbs + Gmbsr - gcsbb);                          // SPbp
-    m[12 * sB] = (gcbdb - gjbd);                                   // BPdp
-    m[13 * sB] = (gcbgb);                                          // BPgp
-    m[14 * sB] = (gcbsb - gjbs);                                   // BPsp
-    m[15 * sB] = (gjbd + gjbs + gcbbb);                            // BPbp
+    const T gjbd = gbd, gjbs = gbs;
+    mv[0] = gcggb;                                                  // GPgp
+    mv[1] = gcgdb;                                                 // GPdp
+    mv[2] = gcgsb;                                             // GPsp
+    mv[3] = gcgbb;                                             // GPbp
+    mv[4] = (Gds + gbd + RevSum + gcddb);                      // DPdp
+    mv[5] = (Gmr + gcdgb);                                     // DPgp
+    mv[6] = -(Gds + FwdSum - gcdsb);                           // DPsp
+    mv[7] = -(gjbd - Gmbsr - gcdbb);                           // DPbp
+    mv[8] = -(Gds + RevSum - gcsdb);                           // SPdp
+    mv[9] = (gcsgb - Gmr);                                     // SPgp
+    mv[10] = (Gds + gbs + FwdSum + gcssb);                     // SPsp
+    mv[11] = -(gjbs + Gmbsr - gcsbb);                          // SPbp
+    mv[12] = (gcbdb - gjbd);                                   // BPdp
+    mv[13] = (gcbgb);                                          // BPgp
+    mv[14] = (gcbsb - gjbs);                                   // BPsp
+    mv[15] = (gjbd + gjbs + gcbbb);                            // BPbp
 
-    aux[B4_VON * sB] = von_new;
-    aux[B4_MODE * sB] = (double)dmode;
-    aux[B4_CD * sB] = cdrain;
-    aux[B4_CBS * sB] = cbs;
-    aux[B4_CBD * sB] = cbd;
-    aux[B4_GBS * sB] = gbs;
-    aux[B4_GBD * sB] = gbd;
-    aux[B4_GM * sB] = h_gm_new;
-    aux[B4_GDS * sB] = h_gds_new;
-    aux[B4_GMBS * sB] = h_gmbs_new;
-    if (noncon) atomicOr(&devnc[ci], 1);
+    if constexpr (!kGrad) {
+        double* r = cr + (size_t)(t.rsite + 4 * d) * sB + inst;
+        for (int k = 0; k < 4; ++k) r[(size_t)k * sB] = rv[k];
+        double* m = cm + (size_t)(t.site + 16 * d) * sB + inst;
+        for (int k = 0; k < 16; ++k) m[(size_t)k * sB] = mv[k];
+    } else {
+        // dL/dp = -lambda^T dF/dp with F_row = sum_col A(row, col) x_col - rhs_row
+        // over the device's rows; stamp order: matrix rows G, D, S, B with
+        // columns (g d s b), (d g s b), (d g s b), (d g s b); rhs rows D, G, B, S
+        const double vd = value_of(V(nD, 0)), vg = value_of(V(nG, 1)), vs = value_of(V(nS, 2)),
+                     vb = value_of(V(nB, 3));
+        auto Lm = [&](int node, int k) -> double {
+            if (t.pin) {
+                const int q = t.pin[(size_t)k * sB + inst];
+                return q >= 0 ? lam[q] : 0.0;
+            }
+            return sol(lam, node, sB, inst);
+        };
+        const double ld = Lm(nD, 0), lg = Lm(nG, 1), ls = Lm(nS, 2), lb = Lm(nB, 3);
+        const double dFg = mv[0].d[0] * vg + mv[1].d[0] * vd + mv[2].d[0] * vs + mv[3].d[0] * vb - rv[1].d[0];
+        const double dFd = mv[4].d[0] * vd + mv[5].d[0] * vg + mv[6].d[0] * vs + mv[7].d[0] * vb - rv[0].d[0];
+        const double dFs = mv[8].d[0] * vd + mv[9].d[0] * vg + mv[10].d[0] * vs + mv[11].d[0] * vb - rv[3].d[0];
+        const double dFb = mv[12].d[0] * vd + mv[13].d[0] * vg + mv[14].d[0] * vs + mv[15].d[0] * vb - rv[2].d[0];
+        gout[gid] = -(lg * dFg + ld * dFd + ls * dFs + lb * dFb);
+    }
+
+    put(aux, B4_VON * sB, von_new);
+    put(aux, B4_MODE * sB, T(dmode));
+    put(aux, B4_CD * sB, cdrain);
+    put(aux, B4_CBS * sB, cbs);
+    put(aux, B4_CBD * sB, cbd);
+    put(aux, B4_GBS * sB, gbs);
+    put(aux, B4_GBD * sB, gbd);
+    put(aux, B4_GM * sB, h_gm_new);
+    put(aux, B4_GDS * sB, h_gds_new);
+    put(aux, B4_GMBS * sB, h_gmbs_new);
+    if constexpr (!kGrad) {
+        if (noncon) atomicOr(&devnc[ci], 1);
+    }
+}
+
+
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_bsim4(
+    B4Table t, const DevCtx* __restrict__ ctx, const double* __restrict__ x, double* __restrict__ s0,
+    double* __restrict__ s1, const double* __restrict__ s2, const int32_t* __restrict__ done,
+    double* __restrict__ cm, double* __restrict__ cr, int32_t* __restrict__ devnc, DevTol tol, int B) {
+    b4_lane<double>(t, ctx, x, s0, s1, s2, done, cm, cr, devnc, tol, B, nullptr, nullptr, nullptr);
+}
+
+// dL/dp of every (device, instance) for one direction p of the record:
+// dpar holds d(record)/dp in the per-instance layout of par_inst.
+__global__ __launch_bounds__(TPB) void k_bsim4_grad(B4Table t, const DevCtx* __restrict__ ctx,
+                                                    const double* __restrict__ x,
+                                                    const double* __restrict__ dpar,
+                                                    const double* __restrict__ lam,
+                                                    double* __restrict__ g, DevTol tol, int B) {
+    b4_lane<Dual<1>>(t, ctx, x, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, tol, B, dpar,
+                     lam, g);
 }
 
 }  // namespace
+
+void launch_bsim4_grad(const B4Table& t, const DevCtx* ctx, const double* x, const double* dpar,
+                       const double* lam, double* g, DevTol tol, int B, hipStream_t s) {
+    if (!t.count) return;
+    if (!t.par_inst) throw Error(NSA_ERR_LOGIC, "BSIM4v7 gradient: needs per-instance records");
+    hipLaunchKernelGGL(k_bsim4_grad, dim3(blocks_for((long)t.count * B)), dim3(TPB), 0, s, t, ctx, x, dpar,
+                       lam, g, tol, B);
+}
 
 void launch_bsim4(const B4Table& t, const DevCtx* ctx, const double* x, double* s0, double* s1,
                   const double* s2, const int32_t* done, double* cm, double* cr, int32_t* devnc,

@@ -288,4 +288,45 @@ void launch_copy_f64(double* dst, const double* src, long count, hipStream_t s) 
                        src, count);
 }
 
+
+namespace {
+__device__ inline double at_abs(const double* v, int i) { return i >= 0 ? v[i] : 0.0; }
+
+__global__ void k_bundle_grad_lin(int nr, const int32_t* __restrict__ r_nodes,
+                                  const double* __restrict__ ohms, int nv,
+                                  const int32_t* __restrict__ v_branch, const int32_t* __restrict__ v_inst,
+                                  int ni, const int32_t* __restrict__ i_nodes,
+                                  const int32_t* __restrict__ i_inst, const DevCtx* __restrict__ ctx,
+                                  const double* __restrict__ x, const double* __restrict__ lam,
+                                  double* __restrict__ gR, double* __restrict__ gV, double* __restrict__ gI) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nr) {
+        const int a = r_nodes[2 * k], b = r_nodes[2 * k + 1];
+        const double R = ohms[k];
+        gR[k] = (at_abs(lam, a) - at_abs(lam, b)) * (at_abs(x, a) - at_abs(x, b)) / (R * R);
+        return;
+    }
+    const long kv = k - nr;
+    if (kv < nv) {
+        gV[kv] = lam[v_branch[kv]] * ctx[v_inst[kv]].src_fact;
+        return;
+    }
+    const long ki = kv - nv;
+    if (ki < ni)
+        gI[ki] = (at_abs(lam, i_nodes[2 * ki + 1]) - at_abs(lam, i_nodes[2 * ki])) * ctx[i_inst[ki]].src_fact;
+}
+}  // namespace
+
+void launch_bundle_grad_lin(int nr, const int32_t* r_nodes, const double* ohms, int nv,
+                            const int32_t* v_branch, const int32_t* v_inst, int ni,
+                            const int32_t* i_nodes, const int32_t* i_inst, const DevCtx* ctx,
+                            const double* x, const double* lam, double* gR, double* gV, double* gI,
+                            hipStream_t s) {
+    const long total = (long)nr + nv + ni;
+    if (!total) return;
+    hipLaunchKernelGGL(k_bundle_grad_lin, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, nr, r_nodes,
+                       ohms, nv, v_branch, v_inst, ni, i_nodes, i_inst, ctx, x, lam, gR, gV, gI);
+    NSA_HIP(hipGetLastError());
+}
+
 }  // namespace nsa

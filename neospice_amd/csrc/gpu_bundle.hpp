@@ -60,6 +60,17 @@ void launch_tr_states(long ns, long ns_lane, int lanes, const int32_t* lane_inst
 void launch_masked_copy(long ns, long ns_lane, int lanes, const int32_t* lane_inst,
                         const int32_t* slot_inst, const int32_t* mask, const double* src, double* dst,
                         hipStream_t s);
+// DC-adjoint gradients of the linear families of a packed batch (absolute
+// unknown indices, -1 = ground): resistors dL/dR = (lam_a - lam_b)(x_a -
+// x_b) / R^2; voltage sources (rhs[branch] = V src_fact, vsource.cpp:193-212)
+// dL/dV = lam_branch src_fact; current sources (rhs[p] -= I, rhs[n] += I,
+// isource.cpp:157-174) dL/dI = (lam_n - lam_p) src_fact, src_fact from the
+// owning instance's context.
+void launch_bundle_grad_lin(int nr, const int32_t* r_nodes, const double* ohms, int nv,
+                            const int32_t* v_branch, const int32_t* v_inst, int ni,
+                            const int32_t* i_nodes, const int32_t* i_inst, const DevCtx* ctx,
+                            const double* x, const double* lam, double* gR, double* gV, double* gI,
+                            hipStream_t s);
 // dst[i] = src[i] for i < count.
 void launch_copy_f64(double* dst, const double* src, long count, hipStream_t s);
 

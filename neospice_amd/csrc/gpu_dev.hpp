@@ -108,7 +108,9 @@ struct Mos1Table {
 enum B4Aux : int32_t {
     B4_VON = 0, B4_MODE, B4_CD, B4_CBS, B4_CBD, B4_GBS, B4_GBD, B4_GM, B4_GDS, B4_GMBS, kB4Aux
 };
-struct B4Param;
+template <class T>
+struct B4ParamT;
+using B4Param = B4ParamT<double>;
 struct B4Table {
     int32_t count;
     const int32_t* nodes;   // 4 per device: d, g, s, b
@@ -161,6 +163,14 @@ void launch_mos1(const Mos1Table& t, const DevCtx* ctx, const double* x, double*
 void launch_bsim4(const B4Table& t, const DevCtx* ctx, const double* x, double* s0, double* s1,
                   const double* s2, const int32_t* done, double* cm, double* cr, int32_t* devnc,
                   DevTol tol, int B, hipStream_t s);
+// dL/dp of every BSIM4v7 (device, instance) at a converged DC point for one
+// direction p of the constant record: the load's stamps in MODEADJOINT
+// differentiated by dual numbers seeded with dpar = d(record)/dp (the
+// per-instance layout of par_inst), contracted with the adjoint solution
+// lam: g = -lam^T (dA/dp x - drhs/dp) over the device's rows.  ctx must
+// carry MODEADJOINT.
+void launch_bsim4_grad(const B4Table& t, const DevCtx* ctx, const double* x, const double* dpar,
+                       const double* lam, double* g, DevTol tol, int B, hipStream_t s);
 // MOS1Device::conv_test on the proposed solution x (mos1_device.cpp:376-416).
 void launch_mos1_conv(const Mos1Table& t, const double* x, const double* s0,
                       const int32_t* skip, int32_t* devnc, DevTol tol, int B, hipStream_t s);

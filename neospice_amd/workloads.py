@@ -195,3 +195,61 @@ def c3_rings(count=256, config_index=3, stages=None, **kw):
         ckt, _ = ring_oscillator(int(s), mismatch_rng=r, **kw)
         out.append(ckt)
     return out, np.asarray(stages)
+
+
+# Two-stage Miller op-amp of the DC-adjoint configuration: transistor order
+# M1..M8 (inputs, mirror load, tail, second stage, output sink, bias).
+OPAMP_DEVICES = ("m1", "m2", "m3", "m4", "m5", "m6", "m7", "m8")
+OPAMP_NMOS = (True, True, False, False, True, False, True, True)
+
+
+def opamp_miller(w, l, rc=1e3, cc=0.5e-12, rl=10e3, cl=1e-12, rb=20e3, vin=0.5, vdd=1.0,
+                 toxe=1.8e-9, vth0=0.4):
+    """Two-stage Miller op-amp of 8 BSIM4v7 (LEVEL=14) transistors as a
+    unity-gain buffer: NMOS input pair M1 (gate at the output) / M2 (gate at
+    the input source `vin`), PMOS mirror load M3/M4, NMOS tail M5, PMOS
+    common-source second stage M6 with NMOS sink M7, both current sources
+    biased by the diode M8 fed through rb from VDD; Miller compensation rc
+    in series with cc from the first-stage output to the output; load rl || cl.
+    w, l: the eight transistor widths and lengths.  Returns (circuit, info)
+    with the device indices of the transistors, resistors (rc, rl, rb) and
+    capacitors (cc, cl) and the output node."""
+    ckt = Circuit()
+    ckt.model(".model nmod nmos level=14 toxe=%.17g vth0=%.17g" % (toxe, vth0))
+    ckt.model(".model pmod pmos level=14 toxe=%.17g vth0=%.17g" % (toxe, -vth0))
+    n = {k: ckt.node(k) for k in ("vdd", "inp", "out", "n1", "n2", "tail", "vb", "nc")}
+    ckt.V("vsup", n["vdd"], GND, vdd)
+    ckt.V("vin", n["inp"], GND, vin)
+    pins = {  # d, g, s, b
+        "m1": ("n1", "out", "tail", None), "m2": ("n2", "inp", "tail", None),
+        "m3": ("n1", "n1", "vdd", "vdd"), "m4": ("n2", "n1", "vdd", "vdd"),
+        "m5": ("tail", "vb", None, None), "m6": ("out", "n2", "vdd", "vdd"),
+        "m7": ("out", "vb", None, None), "m8": ("vb", "vb", None, None),
+    }
+    node = lambda k: GND if k is None else n[k]
+    mos = []
+    for k, name in enumerate(OPAMP_DEVICES):
+        dn, gn, sn, bn = pins[name]
+        mos.append(ckt.M(name, node(dn), node(gn), node(sn), node(bn),
+                         "nmod" if OPAMP_NMOS[k] else "pmod", float(w[k]), float(l[k])))
+    res = [ckt.R("rc", n["n2"], n["nc"], rc), ckt.R("rl", n["out"], GND, rl),
+           ckt.R("rb", n["vdd"], n["vb"], rb)]
+    cap = [ckt.C("cc", n["nc"], n["out"], cc), ckt.C("cl", n["out"], GND, cl)]
+    return ckt, {"mos": mos, "res": res, "cap": cap, "out": n["out"]}
+
+
+OPAMP_BIASES = (0.3, 0.4, 0.5, 0.6)
+
+
+def opamp_instances(count=1024, config_index=4):
+    """Seeded op-amp sizes of the DC-adjoint batch: per instance eight W
+    uniform in [0.2u, 5u], eight L uniform in [0.05u, 0.5u], and the
+    resistors (rc, rl, rb) and capacitors (cc, cl) scaled by U[0.8, 1.2]."""
+    out = []
+    for r in instance_rngs(config_index, count):
+        w = r.uniform(0.2e-6, 5e-6, 8)
+        l = r.uniform(0.05e-6, 0.5e-6, 8)
+        rs = np.array([1e3, 10e3, 20e3]) * r.uniform(0.8, 1.2, 3)
+        cs = np.array([0.5e-12, 1e-12]) * r.uniform(0.8, 1.2, 2)
+        out.append({"w": w, "l": l, "r": rs, "c": cs})
+    return out

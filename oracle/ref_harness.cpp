@@ -831,7 +831,8 @@ int ref_amd_ordering(int n, const int32_t* col_ptr, const int32_t* row_idx, int3
 // solution, then gmin stepping, source stepping and pseudo-transient on one
 // solver.  stage_out: 0..3 for the method that converged, -1 if none did.
 // Returns the iterations of the converged method (total over its solves).
-int ref_dc_ladder(void* p, double* solution, int* stage_out) {
+// op: 0x20 (MODETRANOP, compute_dc_operating_point) or 0x10 (MODEDCOP).
+int ref_dc_ladder_op(void* p, double* solution, int* stage_out, int op) {
     auto* h = static_cast<RefCkt*>(p);
     return guarded([&] {
         Circuit& ckt = h->ckt;
@@ -840,7 +841,7 @@ int ref_dc_ladder(void* p, double* solution, int* stage_out) {
         auto solver = make_solver(ckt.num_vars(), ckt.is_linear());
         solver->symbolic(ckt.pattern());
         std::vector<double> sol(solution, solution + ckt.num_vars());
-        const int jct = 0x20 | 0x200, flt = 0x20 | 0x100;
+        const int jct = op | 0x200, flt = op | 0x100;
         *stage_out = -1;
         ckt.integrator_ctx.mode = jct;
         NewtonResult r = newton_solve(ckt, *solver, sol, ckt.options);
@@ -871,6 +872,10 @@ int ref_dc_ladder(void* p, double* solution, int* stage_out) {
 // gmin_stepping (convergence.cpp:216-346) alone, from the current solution,
 // in MODETRANOP (first MODEINITJCT, then MODEINITFLOAT), on a fresh solver.
 // Returns its iterations, negative when it did not converge.
+int ref_dc_ladder(void* p, double* solution, int* stage_out) {
+    return ref_dc_ladder_op(p, solution, stage_out, 0x20);
+}
+
 int ref_gmin_stepping(void* p, double* solution) {
     auto* h = static_cast<RefCkt*>(p);
     return guarded([&] {

@@ -295,13 +295,15 @@ class RefCircuit:
         its = _chk(self.L.ref_dc(self.h, x.ctypes.data_as(P_F64), C.byref(sec)))
         return x, its, sec.value
 
-    def dc_ladder(self):
+    def dc_ladder(self, dcop=False):
         """The transient preamble's operating point from zeros: direct
-        Newton, gmin stepping, source stepping, pseudo-transient (ref_dc_ladder).
+        Newton, gmin stepping, source stepping, pseudo-transient
+        (ref_dc_ladder_op), in MODETRANOP or, with dcop, MODEDCOP.
         Returns (solution, iterations of the method that converged, stage)."""
         x = np.zeros(self.n)
         st = C.c_int()
-        its = _chk(self.L.ref_dc_ladder(self.h, x.ctypes.data_as(P_F64), C.byref(st)))
+        its = _chk(self.L.ref_dc_ladder_op(self.h, x.ctypes.data_as(P_F64), C.byref(st),
+                                           0x10 if dcop else 0x20))
         return x, its, st.value
 
     def gmin_stepping(self):

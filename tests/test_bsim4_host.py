@@ -98,3 +98,41 @@ def test_unsupported_cards_refused(extra):
         c.M("m1", d, d, na.GND, na.GND, "nb", 1e-6, 1e-7)
         c.finalize()
     assert e.value.code == -5  # NSA_ERR_UNSUPPORTED
+
+
+def derivs(ckt, dev):
+    val, dw, dl = np.zeros(NPARAM), np.zeros(NPARAM), np.zeros(NPARAM)
+    P = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    assert lib().nsa_circuit_bsim4_param_derivs(ckt._native, dev, P(val), P(dw), P(dl)) == 0
+    return val, dw, dl
+
+
+@pytest.mark.parametrize("card", CARDS)
+def test_size_derivatives(card):
+    """d(record)/dW and d(record)/dL by dual numbers: the value part is the
+    plain record bit for bit, and each derivative matches a central
+    difference of the plain record (h = 1e-6 relative; Richardson-checked
+    against h = 2e-6 so that a wrong term cannot hide in FD noise)."""
+    for w, l in ((1e-6, 1e-7), (0.5e-6, 0.05e-6), (3.1e-6, 0.37e-6)):
+        val, dw, dl = derivs(one_device(card, w, l), 2)
+        _, plain = params(one_device(card, w, l), 2)
+        assert np.array_equal(val, plain)
+        for which, x0, d_ad in (("w", w, dw), ("l", l, dl)):
+            def rec(x):
+                ww, ll = (x, l) if which == "w" else (w, x)
+                return params(one_device(card, ww, ll), 2)[1]
+            fd = {}
+            for rel in (1e-6, 2e-6):
+                h = rel * x0
+                fd[rel] = (rec(x0 + h) - rec(x0 - h)) / (2 * h)
+            scale = np.maximum(np.abs(val) / x0, 1e-300)
+            for q in range(NPARAM):
+                a, f1, f2 = d_ad[q], fd[1e-6][q], fd[2e-6][q]
+                tol = 1e-6 * max(abs(a), abs(f1)) + 1e-9 * scale[q] + abs(f1 - f2)
+                assert abs(a - f1) <= tol, (card, w, l, which, NAMES[q], a, f1, f2)
+        # fields that do not depend on the size have zero derivative
+        for nm in ("type", "vtm", "coxe", "toxe", "phi", "vbi", "vth0"):
+            q = NAMES.index(nm)
+            assert dw[q] == 0.0 and dl[q] == 0.0, nm
+        # leff = L, weff = W here (no offsets): unit derivatives
+        assert dl[NAMES.index("leff")] == 1.0 and dw[NAMES.index("weff")] == 1.0
