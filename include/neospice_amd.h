@@ -261,6 +261,14 @@ int nsa_bundle_device_offsets(nsa_bundle* b, int32_t* doff);
  * be NULL. */
 int nsa_bundle_dc_adjoint(nsa_bundle* b, const double* dLdx, double* lambda, double* g_value, double* g_w,
                           double* g_l);
+/* New per-device values for later solves, in the layout of
+ * nsa_bundle_dc_adjoint (NULL arrays are left alone): value sets R, C, V and
+ * I (Resistor::set_resistance and its siblings, as .sens perturbs them,
+ * src/core/sens.cpp:74-130); w and l set BSIM4v7 W and L (their constants
+ * are recomputed: BSIM4v7temp, bsim4v7_temp.cpp:416-419 onwards). */
+int nsa_bundle_set_device_values(nsa_bundle* b, const double* value, const double* w, const double* l);
+/* The current values in the same layout (zero where a kind does not apply). */
+int nsa_bundle_get_device_values(nsa_bundle* b, double* value, double* w, double* l);
 
 /* ---- solver seam (ISolver) ---- */
 int nsa_lu_create(int32_t batch, int32_t n, const int32_t* col_ptr, const int32_t* row_idx,

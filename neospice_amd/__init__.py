@@ -556,6 +556,23 @@ class Bundle:
         check(lib().nsa_bundle_dc_ladder(self.h, flags, i32p(conv), i32p(its), i32p(st)))
         return conv.astype(bool), its, st
 
+    def set_device_values(self, value=None, w=None, l=None):
+        """New per-device values (arrays over all devices, layout of
+        dc_adjoint's outputs; None leaves a kind alone): value for R, C, V
+        and I; w and l for BSIM4v7 devices."""
+        nd = int(self.device_offsets()[-1])
+        arrs = [None if a is None else as_f64(np.asarray(a, np.float64).reshape(nd)) for a in (value, w, l)]
+        ptr = [None if a is None else f64p(a) for a in arrs]
+        check(lib().nsa_bundle_set_device_values(self.h, *ptr))
+
+    def device_values(self):
+        """The current per-device values (R, C, V, I), W and L (BSIM4v7):
+        arrays over all devices, zero where a kind does not apply."""
+        nd = int(self.device_offsets()[-1])
+        value, w, l = np.zeros(max(nd, 1)), np.zeros(max(nd, 1)), np.zeros(max(nd, 1))
+        check(lib().nsa_bundle_get_device_values(self.h, f64p(value), f64p(w), f64p(l)))
+        return value[:nd], w[:nd], l[:nd]
+
     def device_offsets(self):
         d = np.zeros(self.N + 1, np.int32)
         check(lib().nsa_bundle_device_offsets(self.h, i32p(d)))

@@ -114,6 +114,8 @@ _SIGS = {
     "nsa_bundle_plan_info": (C.c_int, [VP, I32, P_I64]),
     "nsa_bundle_device_offsets": (C.c_int, [VP, P_I32]),
     "nsa_bundle_dc_adjoint": (C.c_int, [VP, P_F64, P_F64, P_F64, P_F64, P_F64]),
+    "nsa_bundle_set_device_values": (C.c_int, [VP, P_F64, P_F64, P_F64]),
+    "nsa_bundle_get_device_values": (C.c_int, [VP, P_F64, P_F64, P_F64]),
     "nsa_bundle_dc_ladder": (C.c_int, [VP, I32, P_I32, P_I32, P_I32]),
 }
 

@@ -9,6 +9,10 @@ A Function keeps a reference to the engine and reads the engine's state in
 backward (the last Newton factors for the DC adjoint, the last transient
 record for the transient adjoint), so call backward before the next forward
 on the same engine.
+
+BundleDCOperatingPoint does the same for a bundle of circuits (any
+topologies): its tensors are per device over every instance (the layout of
+Bundle.device_values), and the solution is packed instance after instance.
 """
 import numpy as np
 import torch
@@ -64,3 +68,29 @@ class TransientProbes(torch.autograd.Function):
     def backward(ctx, gout):
         gR, gC = ctx.engine.tran_adjoint(_np(gout))
         return (None, None, None, None, None, _t(gR, ctx.like), _t(gC, ctx.like))
+
+
+class BundleDCOperatingPoint(torch.autograd.Function):
+    """Every instance's DC operating point (packed unknowns, total_n) as a
+    function of the per-device values (R ohms, C farads, V and I dc), W and
+    L (BSIM4v7 devices), each a tensor over all devices of the bundle.
+    Forward: solve_dc's ladder (direct Newton, gmin stepping for instances
+    that need it, MODEDCOP); backward: the bundle's DC adjoint (transposed
+    solves with the last factors, dL/dW and dL/dL through BSIM4v7temp's
+    size-dependent constants).  C receives a zero gradient (open in DC)."""
+
+    @staticmethod
+    def forward(ctx, bundle, value, w, l):
+        bundle.set_device_values(_np(value), _np(w), _np(l))
+        conv, _, _ = bundle.op_ladder(dcop=True)
+        if not conv.all():
+            raise RuntimeError("DC operating point failed for instances %s" % np.nonzero(~conv)[0][:10].tolist())
+        ctx.bundle = bundle
+        ctx.like = value
+        return _t(bundle.solution(), value)
+
+    @staticmethod
+    def backward(ctx, gx):
+        g = ctx.bundle.dc_adjoint(_np(gx))
+        like = ctx.like
+        return None, _t(g["value"], like), _t(g["w"], like), _t(g["l"], like)

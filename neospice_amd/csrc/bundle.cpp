@@ -1391,52 +1391,66 @@ void Bundle::ensure_adjoint() {
             ii.push_back(i);
             i_owner_.push_back(doff_[i] + d);
         }
+        for (int d : c.family(DK_C)) c_owner_.push_back(doff_[i] + d);
+        for (int d : c.family(DK_B4)) b4_owner_.push_back(doff_[i] + d);
     }
-    if ((int)r_owner_.size() != nR_ || (int)v_owner_.size() != vsrc_.count || (int)i_owner_.size() != isrc_.count)
+    if ((int)r_owner_.size() != nR_ || (int)v_owner_.size() != vsrc_.count || (int)i_owner_.size() != isrc_.count ||
+        (int)b4_owner_.size() != b4_lanes_)
         throw Error(NSA_ERR_RUNTIME, "Bundle adjoint: family tables");
-    // d(record)/dW and d(record)/dL of every BSIM4v7 lane, in the lane
-    // order and field-major layout of the load's records
-    const int G4 = b4_lanes_;
-    std::vector<double> dW((size_t)kB4Fields * std::max(1, G4)), dL(dW.size());
-    {
-        const DeviceEnv env{};
-        std::vector<double> val(kB4Fields), w(kB4Fields), l(kB4Fields);
-        int g = 0;
-        for (int i = 0; i < NI_; ++i) {
-            const Circuit& c = *ckt_[i];
-            for (int d : c.family(DK_B4)) {
-                const Device& dv = c.devices()[d];
-                bsim4_param_derivs(c.models().at(dv.model), dv, env, val.data(), w.data(), l.data());
-                for (int q = 0; q < kB4Fields; ++q) {
-                    dW[(size_t)q * G4 + g] = w[q];
-                    dL[(size_t)q * G4 + g] = l[q];
-                }
-                b4_owner_.push_back(doff_[i] + d);
-                ++g;
-            }
-        }
-        if (g != G4) throw Error(NSA_ERR_RUNTIME, "Bundle adjoint: BSIM4v7 lanes");
-    }
     d_r_nodes_ = dupload(allocs_, rn);
     d_v_branch_ = dupload(allocs_, vb);
     d_v_inst_ = dupload(allocs_, vi);
     d_i_nodes_ = dupload(allocs_, in);
     d_i_inst_ = dupload(allocs_, ii);
-    d_b4_dW_ = dupload(allocs_, dW);
-    d_b4_dL_ = dupload(allocs_, dL);
     d_lam_ = dalloc<double>(allocs_, tot_n_);
     d_gR_ = dalloc<double>(allocs_, nR_);
     d_gV_ = dalloc<double>(allocs_, vsrc_.count);
     d_gI_ = dalloc<double>(allocs_, isrc_.count);
-    d_gW_ = dalloc<double>(allocs_, G4);
-    d_gL_ = dalloc<double>(allocs_, G4);
+    d_gW_ = dalloc<double>(allocs_, b4_lanes_);
+    d_gL_ = dalloc<double>(allocs_, b4_lanes_);
+    const size_t nf = (size_t)kB4Fields * std::max(1, b4_lanes_);
+    d_b4_dW_ = dalloc<double>(allocs_, nf);
+    d_b4_dL_ = dalloc<double>(allocs_, nf);
     adj_ready_ = true;
+    size_derivs_dirty_ = true;
+}
+
+// d(record)/dW and d(record)/dL of every BSIM4v7 lane at its current size,
+// in the lane order and field-major layout of the load's records.
+void Bundle::refresh_size_derivs() {
+    if (!size_derivs_dirty_) return;
+    const int G4 = b4_lanes_;
+    std::vector<double> dW((size_t)kB4Fields * std::max(1, G4)), dL(dW.size());
+    const DeviceEnv env{};
+    std::vector<double> val(kB4Fields), w(kB4Fields), l(kB4Fields);
+    int g = 0;
+    for (int i = 0; i < NI_; ++i) {
+        const Circuit& c = *ckt_[i];
+        for (int d : c.family(DK_B4)) {
+            Device dv = c.devices()[d];
+            if (!b4_w_.empty()) {
+                dv.w = b4_w_[g];
+                dv.l = b4_l_[g];
+            }
+            bsim4_param_derivs(c.models().at(dv.model), dv, env, val.data(), w.data(), l.data());
+            for (int q = 0; q < kB4Fields; ++q) {
+                dW[(size_t)q * G4 + g] = w[q];
+                dL[(size_t)q * G4 + g] = l[q];
+            }
+            ++g;
+        }
+    }
+    if (g != G4) throw Error(NSA_ERR_RUNTIME, "Bundle adjoint: BSIM4v7 lanes");
+    NSA_HIP(hipMemcpy(d_b4_dW_, dW.data(), sizeof(double) * dW.size(), hipMemcpyHostToDevice));
+    NSA_HIP(hipMemcpy(d_b4_dL_, dL.data(), sizeof(double) * dL.size(), hipMemcpyHostToDevice));
+    size_derivs_dirty_ = false;
 }
 
 void Bundle::dc_adjoint(const double* dLdx, double* lambda, double* g_value, double* g_w, double* g_l) {
     for (int i = 0; i < NI_; ++i)
         if (plan_[i] < 0) throw Error(NSA_ERR_LOGIC, "Bundle::dc_adjoint: no factors (solve the operating point first)");
     ensure_adjoint();
+    refresh_size_derivs();
     NSA_HIP(hipStreamSynchronize(stream_));
     NSA_HIP(hipMemcpyAsync(d_lam_, dLdx, sizeof(double) * tot_n_, hipMemcpyHostToDevice, stream_));
     // J^T lambda = dL/dx with the factors of each instance's last iteration
@@ -1475,6 +1489,103 @@ void Bundle::dc_adjoint(const double* dLdx, double* lambda, double* g_value, dou
         std::fill(g_l, g_l + nd, 0.0);
         for (size_t k = 0; k < gL.size(); ++k) g_l[b4_owner_[k]] = gL[k];
     }
+}
+
+void Bundle::set_device_values(const double* value, const double* w, const double* l) {
+    ensure_adjoint();
+    NSA_HIP(hipStreamSynchronize(stream_));
+    if (value) {
+        auto put = [&](const std::vector<int32_t>& owner, const double* dptr, const char* what) {
+            if (owner.empty()) return;
+            std::vector<double> v(owner.size());
+            for (size_t k = 0; k < owner.size(); ++k) {
+                v[k] = value[owner[k]];
+                if (!std::isfinite(v[k])) throw Error(NSA_ERR_ARGUMENT, std::string("set_device_values: ") + what);
+            }
+            NSA_HIP(hipMemcpy(const_cast<double*>(dptr), v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
+        };
+        for (int32_t o : r_owner_)
+            if (value[o] == 0.0) throw Error(NSA_ERR_ARGUMENT, "set_device_values: zero resistance");
+        put(r_owner_, d_r_ohms_, "resistance");
+        put(c_owner_, cap_.farads, "capacitance");
+        put(v_owner_, vsrc_.dc, "source value");
+        put(i_owner_, isrc_.dc, "source value");
+        // the resistors' conductances are constant matrix sites
+        launch_res_sites(nR_, d_r_ohms_, d_cm_, r_site_, 1, stream_);
+    }
+    if ((w || l) && b4_lanes_) {
+        const int G4 = b4_lanes_;
+        if (b4_w_.empty()) {
+            b4_w_.resize(G4);
+            b4_l_.resize(G4);
+            int g = 0;
+            for (int i = 0; i < NI_; ++i)
+                for (int d : ckt_[i]->family(DK_B4)) {
+                    b4_w_[g] = ckt_[i]->devices()[d].w;
+                    b4_l_[g] = ckt_[i]->devices()[d].l;
+                    ++g;
+                }
+        }
+        std::vector<double> par((size_t)kB4Fields * G4);
+        NSA_HIP(hipMemcpy(par.data(), b4_.par_inst, sizeof(double) * par.size(), hipMemcpyDeviceToHost));
+        const DeviceEnv env{};
+        bool changed = false;
+        int g = 0;
+        for (int i = 0; i < NI_; ++i) {
+            const Circuit& c = *ckt_[i];
+            for (int d : c.family(DK_B4)) {
+                const double nw = w ? w[doff_[i] + d] : b4_w_[g];
+                const double nl = l ? l[doff_[i] + d] : b4_l_[g];
+                if (nw != b4_w_[g] || nl != b4_l_[g]) {
+                    Device dv = c.devices()[d];
+                    dv.w = nw;
+                    dv.l = nl;
+                    const B4Param p = bsim4_param(c.models().at(dv.model), dv, env);
+                    const double* f = reinterpret_cast<const double*>(&p);
+                    for (int q = 0; q < kB4Fields; ++q) par[(size_t)q * G4 + g] = f[q];
+                    b4_w_[g] = nw;
+                    b4_l_[g] = nl;
+                    changed = true;
+                }
+                ++g;
+            }
+        }
+        if (changed) {
+            NSA_HIP(hipMemcpy(const_cast<double*>(b4_.par_inst), par.data(), sizeof(double) * par.size(),
+                              hipMemcpyHostToDevice));
+            // the size derivatives follow the new sizes
+            size_derivs_dirty_ = true;
+        }
+    }
+}
+
+void Bundle::get_device_values(double* value, double* w, double* l) {
+    ensure_adjoint();
+    NSA_HIP(hipStreamSynchronize(stream_));
+    const size_t nd = (size_t)doff_[NI_];
+    if (value) {
+        std::fill(value, value + nd, 0.0);
+        auto get = [&](const std::vector<int32_t>& owner, const double* dptr) {
+            if (owner.empty()) return;
+            std::vector<double> v(owner.size());
+            NSA_HIP(hipMemcpy(v.data(), dptr, sizeof(double) * v.size(), hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < owner.size(); ++k) value[owner[k]] = v[k];
+        };
+        get(r_owner_, d_r_ohms_);
+        get(c_owner_, cap_.farads);
+        get(v_owner_, vsrc_.dc);
+        get(i_owner_, isrc_.dc);
+    }
+    if (w) std::fill(w, w + nd, 0.0);
+    if (l) std::fill(l, l + nd, 0.0);
+    int g = 0;
+    for (int i = 0; i < NI_; ++i)
+        for (int d : ckt_[i]->family(DK_B4)) {
+            const Device& dv = ckt_[i]->devices()[d];
+            if (w) w[doff_[i] + d] = b4_w_.empty() ? dv.w : b4_w_[g];
+            if (l) l[doff_[i] + d] = b4_l_.empty() ? dv.l : b4_l_[g];
+            ++g;
+        }
 }
 
 }  // namespace nsa

@@ -149,6 +149,13 @@ public:
     // g_w / g_l are dL/dW and dL/dL of BSIM4v7 devices (zero elsewhere).
     // Any output may be null.
     void dc_adjoint(const double* dLdx, double* lambda, double* g_value, double* g_w, double* g_l);
+    // New device values for later solves, per circuit device as for
+    // dc_adjoint (null arrays leave theirs): value sets R (ohms), C (farads),
+    // V (dc) and I (dc); w and l set the W and L of BSIM4v7 devices (their
+    // constant records are recomputed).  Entries of other devices are
+    // ignored.
+    void set_device_values(const double* value, const double* w, const double* l);
+    void get_device_values(double* value, double* w, double* l);
     int device_offset(int i) const { return doff_[i]; }
     int total_devices() const { return doff_[NI_]; }
 
@@ -178,6 +185,7 @@ private:
     std::vector<double> instance_matrix(int i);
     void upload_inst();
     void ensure_adjoint();
+    void refresh_size_derivs();
     void tran_attempt_setup(int i, DevCtx& c);
     void tran_post_round(const std::vector<int32_t>& attempt, const std::vector<int32_t>& conv,
                          const std::vector<int32_t>& its);
@@ -190,8 +198,9 @@ private:
     std::vector<int32_t> xoff_, aoff_, n_, nnodes_;
     std::vector<int32_t> doff_;  // NI+1 device offsets (circuit device order)
     // adjoint: family tables with the owners of their entries (global device positions)
-    bool adj_ready_ = false;
-    std::vector<int32_t> r_owner_, v_owner_, i_owner_, b4_owner_;
+    bool adj_ready_ = false, size_derivs_dirty_ = true;
+    std::vector<int32_t> r_owner_, v_owner_, i_owner_, c_owner_, b4_owner_;
+    std::vector<double> b4_w_, b4_l_;  // per lane: the W and L its record was computed with
     int32_t *d_r_nodes_ = nullptr, *d_v_branch_ = nullptr, *d_v_inst_ = nullptr, *d_i_nodes_ = nullptr,
             *d_i_inst_ = nullptr;
     double *d_lam_ = nullptr, *d_gR_ = nullptr, *d_gV_ = nullptr, *d_gI_ = nullptr, *d_gW_ = nullptr,

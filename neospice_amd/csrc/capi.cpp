@@ -702,6 +702,18 @@ int nsa_bundle_device_offsets(nsa_bundle* b, int32_t* doff) {
         return 0;
     });
 }
+int nsa_bundle_set_device_values(nsa_bundle* b, const double* value, const double* w, const double* l) {
+    return guard([&] {
+        b->b->set_device_values(value, w, l);
+        return 0;
+    });
+}
+int nsa_bundle_get_device_values(nsa_bundle* b, double* value, double* w, double* l) {
+    return guard([&] {
+        b->b->get_device_values(value, w, l);
+        return 0;
+    });
+}
 int nsa_bundle_dc_adjoint(nsa_bundle* b, const double* dLdx, double* lambda, double* g_value, double* g_w,
                           double* g_l) {
     return guard([&] {

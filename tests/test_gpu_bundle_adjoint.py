@@ -222,3 +222,49 @@ def test_opamp_adjoint_vs_sens():
             assert abs(ad - ref) <= tol, (name, ad, ref)
             checked += 1
         assert checked == 5
+
+
+def test_set_device_values_and_autograd():
+    """New W, L and R values set on a built bundle give the operating points
+    of circuits built with those values (bit for bit), and the torch front
+    end returns the adjoint's gradients."""
+    import torch
+    from neospice_amd.autograd import BundleDCOperatingPoint
+    inst = W.opamp_instances(2)
+    cks, infos = batch(inst, biases=(0.4, 0.55))
+    b = na.Bundle(cks)
+    value, w, l = b.device_values()
+    doff = b.device_offsets()
+    # scale every transistor's W by 1.1, L by 0.9 and the load resistors by 1.05
+    q = [dict(p) for p in inst]
+    for p in q:
+        p["w"] = p["w"] * 1.1
+        p["l"] = p["l"] * 0.9
+        p["r"] = p["r"] * 1.05
+    cks2, _ = batch(q, biases=(0.4, 0.55))
+    w2, l2, value2 = w * 1.1, l * 0.9, value.copy()
+    for j, info in enumerate(infos):
+        for k in info["res"]:
+            value2[doff[j] + k] *= 1.05
+    b.set_device_values(value2, w2, l2)
+    conv, _, _ = b.op_ladder(dcop=True)
+    ref = na.Bundle(cks2)
+    conv2, _, _ = ref.op_ladder(dcop=True)
+    assert conv.all() and conv2.all()
+    assert np.array_equal(b.solution(), ref.solution())
+    v_now, w_now, l_now = b.device_values()
+    assert np.array_equal(v_now, value2) and np.array_equal(w_now, w2) and np.array_equal(l_now, l2)
+    # torch: L = sum (Vout - 0.5)^2 over the circuits
+    tv, tw, tl = (torch.tensor(a, dtype=torch.float64, requires_grad=True) for a in (value2, w2, l2))
+    x = BundleDCOperatingPoint.apply(b, tv, tw, tl)
+    outs = torch.tensor([int(b.xoff[j] + info["out"]) for j, info in enumerate(infos)])
+    loss = ((x[outs] - 0.5) ** 2).sum()
+    loss.backward()
+    dLdx = np.zeros(b.total_n)
+    xs = b.solution()
+    for j, o in enumerate(outs.tolist()):
+        dLdx[o] = 2.0 * (xs[o] - 0.5)
+    g = b.dc_adjoint(dLdx)
+    assert np.array_equal(tw.grad.numpy(), g["w"]) and np.array_equal(tl.grad.numpy(), g["l"])
+    assert np.array_equal(tv.grad.numpy(), g["value"])
+    assert np.abs(tw.grad.numpy()).max() > 0
