@@ -260,6 +260,64 @@ def rc_mesh(na, W, args, B=256, dt=1e-12):
 
 
 
+def adjoint_c4(na, W, count=1024):
+    """DC-operating-point adjoint of the two-stage Miller op-amp batch: 1024
+    seeded instances (8 BSIM4v7 transistors with W in [0.2u, 5u] and L in
+    [0.05u, 0.5u], RC compensation and the 10k / 1pF load scaled by
+    U[0.8, 1.2]) at 4 input biases 0.3..0.6 V, 4096 circuits in one bundle;
+    loss = sum over the biases of (Vout - 0.5)^2; gradient w.r.t. every W,
+    L, R and C through the torch front end (BundleDCOperatingPoint).
+    forward = solve_dc's ladder of all 4096 circuits; grad = the backward
+    (transposed solves, R/V/I gradients, dual-number BSIM4v7 W and L
+    gradients).  Timed after one warm-up forward and backward (pivot orders
+    kept from it)."""
+    import torch
+    from neospice_amd.autograd import BundleDCOperatingPoint
+    inst = W.opamp_instances(count)
+    t0 = time.perf_counter()
+    cks, outs = [], []
+    for p in inst:
+        for v in W.OPAMP_BIASES:
+            c, info = W.opamp_miller(p["w"], p["l"], rc=p["r"][0], rl=p["r"][1], rb=p["r"][2], cc=p["c"][0],
+                                     cl=p["c"][1], vin=v)
+            c.finalize()
+            cks.append(c)
+            outs.append(info["out"])
+    b = na.Bundle(cks)
+    build_s = time.perf_counter() - t0
+    out_idx = torch.tensor([int(b.xoff[j] + o) for j, o in enumerate(outs)])
+    value, w, l = b.device_values()
+
+    def step():
+        tv, tw, tl = (torch.tensor(a, dtype=torch.float64, requires_grad=True) for a in (value, w, l))
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        x = BundleDCOperatingPoint.apply(b, tv, tw, tl)
+        loss = ((x[out_idx] - 0.5) ** 2).sum()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        return t2 - t1, t3 - t2, float(loss.detach()), tw.grad
+
+    cold_f, cold_g, _, _ = step()
+    it0 = b.stats()["iterations"]
+    fwd, grad, loss, gw = step()
+    its = b.stats()["iterations"] - it0
+    log("bench: C4 forward %.4f s (cold %.4f), gradient %.4f s (cold %.4f), ratio %.3f" %
+        (fwd, cold_f, grad, cold_g, grad / fwd))
+    return {"workload": "miller_opamp_bsim4v7_%dx%d_biases_dc_adjoint" % (count, len(W.OPAMP_BIASES)),
+            "circuits": len(cks), "unknowns_total": b.total_n, "params": int(2 * 8 * count + 5 * count),
+            "forward_ms": round(fwd * 1e3, 3), "grad_ms": round(grad * 1e3, 3),
+            "grad_over_forward": round(grad / fwd, 4),
+            "forward_newton_iterations": int(its), "loss": loss,
+            "cold_forward_ms": round(cold_f * 1e3, 3), "cold_grad_ms": round(cold_g * 1e3, 3),
+            "build_s": round(build_s, 3),
+            "note": "forward = DC ladder (MODEDCOP) of every circuit from zero; grad = transposed packed "
+                    "solves + dL/dR,V,I + dual-number dL/dW,dL/dL kernels; both through torch.autograd"}
+
+
 def adjoint_c5(na, W, mesh=32, batch=128, nsteps=500, dt=2e-12, ckpt=25, nprobe=64):
     """Transient adjoint of the diode-clamped RC mesh (32x32, diode IS=1e-14
     N=1 per node, 16-point PWL drive): 500 trapezoidal steps of 2 ps over
@@ -584,7 +642,9 @@ def main():
 
     c3 = c3_headline(na, W, args)
     mesh = None if (args.no_mesh or args.headline_only) else rc_mesh(na, W, args)
-    adjoint = None if (args.no_adjoint or args.headline_only) else adjoint_c5(na, W, nsteps=args.adjoint_steps)
+    adjoint = None
+    if not (args.no_adjoint or args.headline_only):
+        adjoint = {"dc_opamp": adjoint_c4(na, W), "transient_diode_mesh": adjoint_c5(na, W, nsteps=args.adjoint_steps)}
     out = {
         "metric": "batched Newton iters/s vs batch-1 GPU @1e-9 rel; adjoint grad/forward time",
         "value": round(c3["value"], 2),
