@@ -78,10 +78,11 @@ std::vector<int32_t> min_degree_order(int n, const int32_t* col_ptr, const int32
             deg[v] = d;
         }
     }
-    // Dense nodes last by ascending degree.  A stable order is what the
-    // reference's std::sort yields for 16 or fewer elements.
-    std::stable_sort(dense_nodes.begin(), dense_nodes.end(),
-                     [&](int32_t a, int32_t b) { return deg[a] < deg[b]; });
+    // Dense nodes last by ascending degree, with the reference's call
+    // (amd.cpp:134-137): std::sort is not stable, and for more than 16 dense
+    // nodes the tie order is the one its introsort leaves.
+    std::sort(dense_nodes.begin(), dense_nodes.end(),
+              [&](int32_t a, int32_t b) { return deg[a] < deg[b]; });
     for (int32_t d : dense_nodes) perm.push_back(d);
     return perm;
 }

@@ -105,7 +105,8 @@ def diode_mesh(m=32, config_index=5, r_range=(1.0, 100.0), c_range=(10e-15, 100e
 
 def inverter_chain(stages=50, vdd=1.8, r_load=1e3, c_load=2e-15):
     """MOS1 inverter chain: LEVEL=1 NMOS/PMOS (VTO +-0.5, KP 120u/40u,
-    LAMBDA 0.02), a PULSE input 0 -> vdd (50 ps edges, 1 ns period) and,
+    LAMBDA 0.02, W/L 2u/0.18u and 1u/0.18u), a PULSE input 0 -> vdd (50 ps
+    edges, 1 ns period) and,
     per stage, an RC load (series r_load to a node with c_load to ground)
     that drives the next stage: 2 nodes per stage, 2 + 2*stages + 2
     unknowns."""
@@ -119,8 +120,8 @@ def inverter_chain(stages=50, vdd=1.8, r_load=1e3, c_load=2e-15):
     for k in range(stages):
         o = ckt.node("o%d" % k)
         l = ckt.node("l%d" % k)
-        ckt.M("mn%d" % k, o, drive, GND, GND, "nch", 1e-6, 0.18e-6)
-        ckt.M("mp%d" % k, o, drive, nvdd, nvdd, "pch", 2e-6, 0.18e-6)
+        ckt.M("mn%d" % k, o, drive, GND, GND, "nch", 2e-6, 0.18e-6)
+        ckt.M("mp%d" % k, o, drive, nvdd, nvdd, "pch", 1e-6, 0.18e-6)
         ckt.R("rl%d" % k, o, l, r_load)
         ckt.C("cl%d" % k, l, GND, c_load)
         drive = l

@@ -182,6 +182,32 @@ def test_torch_autograd_dc():
     assert not vals["C"].grad.abs().max().item()
 
 
+def test_torch_autograd_transient():
+    """TransientProbes: the torch front end returns the engine's own
+    transient-adjoint gradients (same checkpointing) and its forward the
+    engine's probe waveforms."""
+    torch = pytest.importorskip("torch")
+    from neospice_amd.autograd import TransientProbes
+    ckt = diode_mesh(4)
+    B, nsteps, dt, ckpt = 2, 20, 2e-12, 5
+    probes = np.array([1, 6, ckt.num_nodes - 1], np.int32)
+    eng = na.BatchEngine(ckt, batch=B)
+    R = torch.tensor(eng.family_values("R"), dtype=torch.float64, requires_grad=True)
+    C = torch.tensor(eng.family_values("C") * 1.05, dtype=torch.float64, requires_grad=True)
+    v = TransientProbes.apply(eng, nsteps, dt, probes, ckpt, R, C)
+    target = torch.full_like(v, 0.2)
+    loss = ((v - target) ** 2).sum()
+    loss.backward()
+    v_np = v.detach().numpy()
+    eng2 = na.BatchEngine(ckt, batch=B)
+    eng2.set_family_values("C", C.detach().numpy())
+    v2 = eng2.tran_forward(nsteps, dt, probes, ckpt_every=ckpt)
+    assert np.array_equal(v_np, v2)
+    gR, gC = eng2.tran_adjoint(2.0 * (v2 - 0.2))
+    assert np.array_equal(R.grad.numpy(), gR) and np.array_equal(C.grad.numpy(), gC)
+    assert np.abs(gR).max() > 0 and np.abs(gC).max() > 0
+
+
 @pytest.mark.parametrize("which", ["dc", "tran"])
 def test_adjoint_linear_complex_step(which):
     if which == "dc":
@@ -269,6 +295,13 @@ def test_tran_adjoint_refuses_bsim4():
     c.M("m1", a, a, na.GND, na.GND, "nb", 0.5e-6, 0.05e-6)
     ckt = c.finalize()
     eng = na.BatchEngine(ckt, batch=1)
-    with pytest.raises(Exception):
-        eng.tran_forward(3, 1e-12, np.array([0]))
+    from neospice_amd._lib import NsaError
+    # no forward record yet: a misuse-order error
+    with pytest.raises(NsaError) as e0:
         eng.tran_adjoint(np.zeros((4, 1, 1)))
+    assert e0.value.code == -1
+    # the forward run itself is fine; the adjoint refuses the BSIM4v7 charges
+    eng.tran_forward(3, 1e-12, np.array([0]))
+    with pytest.raises(NsaError) as e1:
+        eng.tran_adjoint(np.zeros((4, 1, 1)))
+    assert e1.value.code == -5 and "BSIM4v7" in str(e1.value)

@@ -181,3 +181,15 @@ def test_bundle_tran_mixed_families():
     for c in cks:
         c.finalize()
     adaptive_parity(cks, 2e-12, 200e-12)
+
+
+def test_bundle_tran_c1_full_size():
+    """The MOS1 configuration at its full size: a 50-stage CMOS inverter
+    chain (NMOS 2u/0.18u, PMOS 1u/0.18u, VDD 1.8 V, PULSE 0 -> 1.8 V with
+    50 ps edges and 1 ns period), DC operating point then `.tran 10p 2n`
+    (solve_transient: steps capped at 10 ps, smaller at the breakpoints):
+    the bundle's own step sequence and waveform against the reference's."""
+    c, _ = W.inverter_chain(50)
+    c.finalize()
+    b, worst = adaptive_parity([c], 10e-12, 2e-9)
+    print("worst", worst, "accepted", b.tran_status(0)["accepted"])

@@ -111,3 +111,35 @@ def test_diode_and_mos_circuit_pattern():
     cp, ri = rc.pattern()
     assert np.array_equal(cp, c.col_ptr) and np.array_equal(ri, c.row_idx)
     assert np.array_equal(rc.load_order(), c.load_order)
+
+
+def test_min_degree_order_many_dense_nodes():
+    """More than 16 dense nodes (degree > max(10 sqrt(n), 16)) with tied
+    degrees: the dense tail keeps the reference's std::sort order
+    (amd.cpp:134-137), not a stable one."""
+    import ctypes as C
+    n = 400
+    rng = np.random.default_rng(3)
+    dense = rng.choice(n, 24, replace=False)
+    cols = [set([j]) for j in range(n)]
+    for d in dense:
+        # every dense node touches the same number of others: tied degrees
+        others = rng.choice(np.setdiff1d(np.arange(n), [d]), 260, replace=False)
+        for o in others:
+            cols[d].add(int(o))
+            cols[int(o)].add(int(d))
+    for j in range(n - 1):
+        cols[j].add(j + 1)
+        cols[j + 1].add(j)
+    cp = np.zeros(n + 1, np.int32)
+    ri = []
+    for j in range(n):
+        r = sorted(cols[j])
+        ri.extend(r)
+        cp[j + 1] = len(ri)
+    ri = np.asarray(ri, np.int32)
+    q_ref = np.zeros(n, np.int32)
+    P = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+    assert RB.ref().ref_amd_ordering(n, P(cp), P(ri), P(q_ref)) == n
+    q = na.min_degree_order(n, cp, ri)
+    assert np.array_equal(q, q_ref)
