@@ -240,6 +240,11 @@ int nsa_bundle_set_timing(nsa_bundle* b, int32_t on);
 int nsa_bundle_timing(nsa_bundle* b, double out[5]);
 /* 1 (default): device-driven iterations replay a captured HIP graph */
 int nsa_bundle_set_graph(nsa_bundle* b, int32_t on);
+/* 1 (default): the transient's step controller (solve_transient's loop body
+ * with its TimeStepController, src/core/transient.cpp:873-1063) runs on the
+ * device, every instance on its own step sequence; 0: on the host between
+ * batched Newton rounds.  Both give the same results per instance. */
+int nsa_bundle_set_device_control(nsa_bundle* b, int32_t on);
 /* pivot structure of an instance: plan index (-1 before its first
    factorization), n, nnz(A), nnz(L+U), levels of a factor-and-solve launch
    (the refactor with the forward substitution folded in, then the

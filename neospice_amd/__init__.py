@@ -666,6 +666,12 @@ class Bundle:
     def set_graph(self, on=True):
         check(lib().nsa_bundle_set_graph(self.h, int(bool(on))))
 
+    def set_device_control(self, on=True):
+        """True (default): the transient's step controller runs on the device
+        (every instance on its own step sequence, no host round trip per
+        step); False: on the host between batched Newton rounds."""
+        check(lib().nsa_bundle_set_device_control(self.h, int(bool(on))))
+
     def plan_info(self, i):
         """Pivot structure of instance i: plan id, sizes, levels and the
         operation counts of one refactor and one forward + backward solve."""

@@ -64,18 +64,6 @@ void parallel_for(int count, F&& f) {
         if (e) std::rethrow_exception(e);
 }
 
-// solve_transient's constants (transient.cpp:45-78).
-constexpr double kInitialStopDivisor = 100.0;
-constexpr double kInitialStepDivisor = 10.0;
-constexpr double kMinTimeStepRatio = 1e-11;
-constexpr int kMaxTransientIterations = 500000;
-constexpr int kNoRecentBreakpoint = 1000;
-constexpr double kNewtonFailureDtFactor = 8.0;
-constexpr double kDeviceLteRejectRatio = 0.9;
-constexpr double kMaxDtGrowthFactor = 2.0;
-constexpr int kRingingMinStepsAfterBp = 5;
-constexpr int kLteMinStepCount = 2;
-
 }  // namespace
 
 Bundle::Bundle(const std::vector<const Circuit*>& circuits, const Options& opt)
@@ -125,7 +113,8 @@ Bundle::Bundle(const std::vector<const Circuit*>& circuits, const Options& opt)
     NSA_HIP(hipMemset(d_devnc_, 0, sizeof(int32_t) * NI_));
     NSA_HIP(hipHostMalloc((void**)&h_ctx_pin_, sizeof(DevCtx) * NI_, hipHostMallocDefault));
     NSA_HIP(hipHostMalloc((void**)&h_nt_pin_, sizeof(NtInst) * NI_, hipHostMallocDefault));
-    NSA_HIP(hipHostMalloc((void**)&h_kind_pin_, sizeof(int32_t) * 2 * NI_, hipHostMallocDefault));
+    // kinds, fail codes, and the modes and completion flags of attempts the host ends
+    NSA_HIP(hipHostMalloc((void**)&h_kind_pin_, sizeof(int32_t) * 4 * NI_, hipHostMallocDefault));
     std::vector<int32_t> nn(n_.begin(), n_.end()), nd(nnodes_.begin(), nnodes_.end());
     d_ixoff_ = dupload(allocs_, std::vector<int32_t>(xoff_.begin(), xoff_.end()));
     {
@@ -163,6 +152,8 @@ Bundle::~Bundle() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
+    if (tick_exec_) (void)hipGraphExecDestroy(tick_exec_);
+    if (tick_graph_) (void)hipGraphDestroy(tick_graph_);
     for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
     for (void* p : tran_allocs_) (void)hipFree(p);
     for (void* p : allocs_) (void)hipFree(p);
@@ -174,6 +165,7 @@ Bundle::~Bundle() {
     if (h_acc_pin_) (void)hipHostFree(h_acc_pin_);
     if (h_tmin_pin_) (void)hipHostFree(h_tmin_pin_);
     if (h_ring_pin_) (void)hipHostFree(h_ring_pin_);
+    if (h_tcs_pin_) (void)hipHostFree(h_tcs_pin_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -583,6 +575,15 @@ void Bundle::capture_graph() {
     graph_words_ = plans_.d_words();
     graph_nlu_ = plans_.max_nlu();
     graph_n_ = plans_.max_n();
+    graph_hot_ = plans_.max_hot();
+}
+
+// Whether a graph captured with these plan words and launch-shape limits
+// would launch the packed LU differently now (its LDS size and staging
+// follow the largest factor, vector and hot block of the plans).
+bool Bundle::graph_stale(const int32_t* words, int nlu, int n, int hot) const {
+    return words != plans_.d_words() || nlu != plans_.max_nlu() || n != plans_.max_n() ||
+           hot != plans_.max_hot();
 }
 
 void Bundle::iteration_device() {
@@ -600,9 +601,7 @@ void Bundle::iteration_device() {
         return;
     }
     if (use_graph) {
-        if (!graph_exec_ || graph_words_ != plans_.d_words() || graph_nlu_ != plans_.max_nlu() ||
-            graph_n_ != plans_.max_n())
-            capture_graph();
+        if (!graph_exec_ || graph_stale(graph_words_, graph_nlu_, graph_n_, graph_hot_)) capture_graph();
         NSA_HIP(hipGraphLaunch(graph_exec_, stream_));
         ++n_graph_launches;
         return;
@@ -687,12 +686,14 @@ void Bundle::host_lu_post(std::vector<NtInst>& nt, const std::vector<int>& repiv
     NSA_HIP(hipMemcpyAsync(d_kind_, h_kind_pin_, sizeof(int32_t) * NI_, hipMemcpyHostToDevice, stream_));
     NSA_HIP(hipMemcpyAsync(d_fail_, h_kind_pin_ + NI_, sizeof(int32_t) * NI_, hipMemcpyHostToDevice,
                            stream_));
-    // a failed attempt leaves the caller's mode in the context (the pinned
-    // context still holds it) and its completion flag set
+    // a failed attempt leaves the caller's mode in the context and its
+    // completion flag set
     for (int i : failed) {
-        NSA_HIP(hipMemcpyAsync(&d_ctx_[i].mode, &h_ctx_pin_[i].mode, sizeof(int32_t),
+        h_kind_pin_[2 * NI_ + i] = nt[i].saved_mode;
+        h_kind_pin_[3 * NI_ + i] = nt[i].done;
+        NSA_HIP(hipMemcpyAsync(&d_ctx_[i].mode, h_kind_pin_ + 2 * NI_ + i, sizeof(int32_t),
                                hipMemcpyHostToDevice, stream_));
-        NSA_HIP(hipMemcpyAsync(d_done_ + i, &nt[i].done, sizeof(int32_t), hipMemcpyHostToDevice,
+        NSA_HIP(hipMemcpyAsync(d_done_ + i, h_kind_pin_ + 3 * NI_ + i, sizeof(int32_t), hipMemcpyHostToDevice,
                                stream_));
     }
     launch_lu(d_kind_, SLU_FACTOR | SLU_SOLVE);
@@ -945,64 +946,31 @@ void Bundle::dc_op(int mode, std::vector<int32_t>& converged, std::vector<int32_
 
 namespace {
 
-// TimeStepController (src/core/timestep.cpp), restated on TranCtl.
-void add_source_breakpoint(TranCtl& c, double t, bool hard) {
+// The controller's breakpoint sets while they are built (TimeStepController::
+// add_breakpoint with CKTminBreak, src/core/timestep.cpp).
+struct BpSets {
+    double time = 0.0, tstop = 0.0, min_break = 0.0;
+    std::set<double> breakpoints;
+    std::map<double, bool> src;  // time -> HARD
+};
+
+void add_source_breakpoint(BpSets& c, double t, bool hard) {
     if (t > c.time && t <= c.tstop) {
         // CKTminBreak: a breakpoint this close to an existing one only promotes it
         if (c.min_break > 0.0) {
-            auto lb = c.src_breakpoints.lower_bound(t - c.min_break);
-            if (lb != c.src_breakpoints.end() && lb->first <= t + c.min_break) {
+            auto lb = c.src.lower_bound(t - c.min_break);
+            if (lb != c.src.end() && lb->first <= t + c.min_break) {
                 if (hard) lb->second = true;
                 return;
             }
         }
         c.breakpoints.insert(t);
-        auto it = c.src_breakpoints.find(t);
-        if (it != c.src_breakpoints.end()) {
+        auto it = c.src.find(t);
+        if (it != c.src.end()) {
             if (hard) it->second = true;
         } else {
-            c.src_breakpoints.emplace(t, hard);
+            c.src.emplace(t, hard);
         }
-    }
-}
-
-double next_breakpoint_gap(const TranCtl& c) {
-    if (!c.src_breakpoints.empty()) {
-        const double bp = c.src_breakpoints.begin()->first;
-        if (bp > c.time + 1e-18) return bp - c.time;
-    }
-    if (!c.breakpoints.empty()) {
-        const double bp = *c.breakpoints.begin();
-        if (bp > c.time + 1e-18) return bp - c.time;
-    }
-    return c.tstop - c.time;
-}
-
-double clamp_to_breakpoint(const TranCtl& c, double dt) {
-    const double t_next = c.time + dt;
-    if (!c.breakpoints.empty()) {
-        const double bp = *c.breakpoints.begin();
-        if (t_next > bp - 1e-18) return bp - c.time;
-        if (t_next > bp - 0.1 * dt) return bp - c.time;
-    }
-    return dt;
-}
-
-double clamp_to_end(const TranCtl& c, double dt) {
-    if (c.time + dt > c.tstop) return c.tstop - c.time;
-    return dt;
-}
-
-void advance(TranCtl& c, double dt) {
-    c.time += dt;
-    c.crossed_src_bp = false;
-    c.last_bp_hard = false;
-    while (!c.breakpoints.empty() && *c.breakpoints.begin() <= c.time + 1e-18)
-        c.breakpoints.erase(c.breakpoints.begin());
-    while (!c.src_breakpoints.empty() && c.src_breakpoints.begin()->first <= c.time + 1e-18) {
-        if (c.src_breakpoints.begin()->second) c.last_bp_hard = true;
-        c.src_breakpoints.erase(c.src_breakpoints.begin());
-        c.crossed_src_bp = true;
     }
 }
 
@@ -1036,8 +1004,14 @@ void Bundle::tran_begin(double tstep, double tstop, const int32_t* probe_ptr, co
         throw Error(NSA_ERR_ARGUMENT, "Bundle::tran_begin: tstep and tstop must be positive");
     tstep_ = tstep;
     tstop_ = tstop;
+    NSA_HIP(hipStreamSynchronize(stream_));
     for (void* p : tran_allocs_) NSA_HIP(hipFree(p));
     tran_allocs_.clear();
+    tcd_.clear();
+    tc_on_device_ = false;
+    log_on_device_ = false;
+    d_log_ = nullptr;
+    log_rows_ = 0;
     // 1. DC operating point in MODETRANOP from zeros (transient.cpp:710-726, 195-205)
     std::vector<double> z(tot_n_, 0.0);
     set_solution(z.data());
@@ -1082,10 +1056,13 @@ void Bundle::tran_begin(double tstep, double tstop, const int32_t* probe_ptr, co
     d_acc_ = (int32_t*)tal(sizeof(int32_t) * NI_);
     d_att_ = (int32_t*)tal(sizeof(int32_t) * NI_);
     d_prow_ = (int32_t*)tal(sizeof(int32_t) * NI_);
+    d_tc_ = (TranDev*)tal(sizeof(TranDev) * NI_);
+    d_tcs_ = (TcStatus*)tal(sizeof(TcStatus));
     if (!h_acc_pin_) {
         NSA_HIP(hipHostMalloc((void**)&h_acc_pin_, sizeof(int32_t) * 3 * NI_, hipHostMallocDefault));
         NSA_HIP(hipHostMalloc((void**)&h_tmin_pin_, sizeof(unsigned long long) * 2 * NI_, hipHostMallocDefault));
         NSA_HIP(hipHostMalloc((void**)&h_ring_pin_, sizeof(int32_t) * NI_, hipHostMallocDefault));
+        NSA_HIP(hipHostMalloc((void**)&h_tcs_pin_, sizeof(TcStatus), hipHostMallocDefault));
     }
     pptr_.assign(NI_ + 1, 0);
     std::vector<int32_t> pr;
@@ -1114,12 +1091,18 @@ void Bundle::tran_begin(double tstep, double tstop, const int32_t* probe_ptr, co
     launch_tr_hist(NI_, d_ixoff_, d_in_, d_x_, d_h1_, d_h2_, d_h3_, d_acc_, d_pptr_, d_probe_, d_prow_,
                    d_pvals_, max_points_, stream_);
     // 5. step controllers (transient.cpp:826-903)
-    tc_.assign(NI_, TranCtl());
+    tcd_.assign(NI_, TranDev{});
+    hsteps_.assign(NI_, {});
+    bpt_.clear();
+    bph_.clear();
     const double max_step = std::min(tstep, tstop / 50.0);
     std::vector<double> bps;
     int vcur = 0;
     for (int i = 0; i < NI_; ++i) {
-        TranCtl& c = tc_[i];
+        TranDev& c = tcd_[i];
+        c.order = 1;
+        c.last_bp_hard = 1;
+        c.steps_after_bp = kNoRecentBreakpoint;
         c.dt_min = max_step * kMinTimeStepRatio;
         c.dt_max = max_step;
         c.ctrl_dt = tstep;
@@ -1127,7 +1110,16 @@ void Bundle::tran_begin(double tstep, double tstop, const int32_t* probe_ptr, co
         c.time = 0.0;
         c.proposed_dt = tstep;
         c.min_break = (max_step > 0.0) ? 5e-5 * max_step : 0.0;
+        c.xmu = opt_.xmu;
+        c.src_fact = opt_.src_fact;
+        c.soft_scale = std::sqrt(kRestartStepScale);
+        c.itl = opt_.itl4;
+        c.phase = TP_IDLE;
         // collect_breakpoints (transient.cpp:135-151): sources in device order
+        BpSets b;
+        b.time = c.time;
+        b.tstop = tstop;
+        b.min_break = c.min_break;
         const Circuit& ck = *ckt_[i];
         const auto& fv = ck.family(DK_V);
         std::vector<int> vfam_pos(ck.devices().size(), -1);
@@ -1136,15 +1128,28 @@ void Bundle::tran_begin(double tstep, double tstop, const int32_t* probe_ptr, co
             const Device& dv = ck.devices()[d];
             if (dv.kind != DK_V) continue;
             source_breakpoints(dv, &pulse[7 * (size_t)(vcur + vfam_pos[d])], tstop, bps);
-            for (double bp : bps) add_source_breakpoint(c, bp, true);
+            for (double bp : bps) add_source_breakpoint(b, bp, true);
         }
         vcur += (int)fv.size();
+        // both sets hold the same times (sources are the only producers)
+        if (b.breakpoints.size() != b.src.size())
+            throw Error(NSA_ERR_LOGIC, "Bundle::tran_begin: breakpoint sets differ");
+        c.bp_head = (int32_t)bpt_.size();
+        for (const auto& kv : b.src) {
+            if (!b.breakpoints.count(kv.first))
+                throw Error(NSA_ERR_LOGIC, "Bundle::tran_begin: breakpoint sets differ");
+            bpt_.push_back(kv.first);
+            bph_.push_back(kv.second ? 1 : 0);
+        }
+        c.bp_end = (int32_t)bpt_.size();
         c.dt = std::min(tstop / kInitialStopDivisor, tstep) / kInitialStepDivisor;
         c.use_gear = opt_.method == 1;
         c.integrate_method = c.use_gear ? 1 : 0;
         c.saved_delta = c.dt;
-        c.steps_after_bp = kNoRecentBreakpoint;
-        const double bp_gap = next_breakpoint_gap(c);
+    }
+    for (int i = 0; i < NI_; ++i) {
+        TranDev& c = tcd_[i];
+        const double bp_gap = tc_next_bp_gap(c, bpt_.data());
         if (bp_gap < tstop) {
             const double init_save_delta = tstop / 50.0;
             c.dt = std::min(c.dt, 0.1 * std::min(init_save_delta, bp_gap));
@@ -1154,82 +1159,63 @@ void Bundle::tran_begin(double tstep, double tstop, const int32_t* probe_ptr, co
         c.saved_delta = c.dt;
         c.prev_prev_dt = c.dt;
     }
+    d_bpt_ = (double*)tal(sizeof(double) * std::max<size_t>(1, bpt_.size()));
+    d_bph_ = (int32_t*)tal(sizeof(int32_t) * std::max<size_t>(1, bph_.size()));
+    if (!bpt_.empty()) {
+        NSA_HIP(hipMemcpyAsync(d_bpt_, bpt_.data(), sizeof(double) * bpt_.size(), hipMemcpyHostToDevice, stream_));
+        NSA_HIP(hipMemcpyAsync(d_bph_, bph_.data(), sizeof(int32_t) * bph_.size(), hipMemcpyHostToDevice, stream_));
+    }
     NSA_HIP(hipStreamSynchronize(stream_));
 }
 
 bool Bundle::tran_done() const {
-    for (const auto& c : tc_)
+    for (const auto& c : tcd_)
         if (!c.finished) return false;
     return true;
 }
 
+std::vector<TranStep> Bundle::tran_steps(int i) const {
+    if (i < 0 || i >= NI_) throw Error(NSA_ERR_ARGUMENT, "Bundle::tran_steps: instance");
+    if (tcd_.empty()) return {};
+    if (!log_on_device_) return hsteps_[i];
+    const int rows = std::min(tcd_[i].step_count, log_rows_);
+    std::vector<TranStep> out(rows);
+    if (rows)
+        NSA_HIP(hipMemcpyAsync(out.data(), d_log_ + (size_t)i * log_rows_, sizeof(TranStep) * rows,
+                               hipMemcpyDeviceToHost, stream_));
+    NSA_HIP(hipStreamSynchronize(stream_));
+    return out;
+}
+
 int Bundle::tran_advance(double t_until) {
-    if (tc_.empty()) throw Error(NSA_ERR_LOGIC, "Bundle::tran_advance: tran_begin not called");
+    if (tcd_.empty()) throw Error(NSA_ERR_LOGIC, "Bundle::tran_advance: tran_begin not called");
+    return device_control ? tran_advance_device(t_until) : tran_advance_host(t_until);
+}
+
+// Host control: every unfinished instance attempts one step per round; the
+// round's Newton solves run together, then the host takes every decision.
+int Bundle::tran_advance_host(double t_until) {
+    if (log_on_device_)
+        throw Error(NSA_ERR_LOGIC, "Bundle::tran_advance: a transient under device control stays there");
+    if (tc_on_device_) {
+        NSA_HIP(hipMemcpyAsync(tcd_.data(), d_tc_, sizeof(TranDev) * NI_, hipMemcpyDeviceToHost, stream_));
+        NSA_HIP(hipStreamSynchronize(stream_));
+        tc_on_device_ = false;
+    }
     int rounds = 0;
     std::vector<int32_t> conv, its;
     const TruncOpts to{opt_.abstol, opt_.reltol, opt_.chgtol, opt_.trtol};
     const double big = 1e30;
     unsigned long long big_bits;
     std::memcpy(&big_bits, &big, sizeof(big));
+    for (auto& c : tcd_) c.t_until = t_until;
     for (;;) {
         std::vector<int32_t> att(NI_, 0);
         std::vector<DevCtx> ctx = h_ctx_;
         bool any = false;
         for (int i = 0; i < NI_; ++i) {
-            TranCtl& c = tc_[i];
-            if (c.finished) continue;
-            if (!(c.time < c.tstop - 1e-18)) { c.finished = true; continue; }
-            if (c.time >= t_until) continue;
-            if (++c.total_iterations > kMaxTransientIterations) {
-                c.failed = c.finished = true;
-                continue;
-            }
-            // clamp to bounds and breakpoints (transient.cpp:894-903)
-            double dt = std::min(c.dt, c.dt_max);
-            dt = std::max(dt, c.dt_min);
-            const double unclamped = dt;
-            dt = clamp_to_breakpoint(c, dt);
-            if (dt < unclamped - 1e-18) c.saved_delta = unclamped;
-            dt = clamp_to_end(c, dt);
-            if (dt < 1e-20) { c.finished = true; continue; }
-            c.dt = dt;
-            // fill_integrator_context (transient.cpp:398-447)
-            DevCtx x{};
-            const bool first = c.step_count == 0;
-            x.order = c.order;
-            x.delta = dt;
-            x.time = c.time + dt;
-            x.delta_old1 = first ? dt : c.prev_dt;
-            x.delta_old2 = c.prev_prev_dt;
-            x.mode = MODETRAN | (first ? MODEINITTRAN : MODEINITPRED);
-            x.method = c.integrate_method;
-            x.src_fact = opt_.src_fact;
-            if (c.order == 1) {
-                x.ag0 = 1.0 / dt;
-                x.ag1 = -1.0 / dt;
-                x.ag2 = 0.0;
-                x.xmu_ratio = 0.0;
-            } else if (c.integrate_method != 1) {
-                const double xmu = opt_.xmu;
-                const double one_minus_xmu = 1.0 - xmu;
-                x.ag0 = 1.0 / (dt * one_minus_xmu);
-                x.ag1 = -1.0 / (dt * one_minus_xmu);
-                x.ag2 = 0.0;
-                x.xmu_ratio = xmu / one_minus_xmu;
-            } else {
-                const double h_old = c.prev_dt;
-                if (h_old > 0.0) {
-                    const double sum = dt + h_old;
-                    x.ag0 = (2.0 * dt + h_old) / (dt * sum);
-                    x.ag1 = -(dt + h_old) / (dt * h_old);
-                    x.ag2 = dt / (h_old * sum);
-                } else {
-                    x.ag0 = 1.0 / dt;
-                    x.ag1 = -1.0 / dt;
-                    x.ag2 = 0.0;
-                }
-                x.xmu_ratio = 0.0;
-            }
+            DevCtx x;
+            if (!tc_attempt(tcd_[i], bpt_.data(), x)) continue;
             ctx[i] = x;
             att[i] = 1;
             any = true;
@@ -1257,87 +1243,16 @@ int Bundle::tran_advance(double t_until) {
         std::vector<int32_t> acc(NI_, 0);
         for (int i = 0; i < NI_; ++i) {
             if (!att[i]) continue;
-            TranCtl& c = tc_[i];
-            c.all_iters += its[i];
-            double dt = c.dt;
-            if (!conv[i]) {
-                // Newton failure (transient.cpp:917-948)
-                dt /= kNewtonFailureDtFactor;
-                c.order = 1;
-                c.order_promote_cooldown = 5;
-                if (dt < c.dt_min) {
-                    if (!c.tried_delmin) {
-                        dt = c.dt_min;
-                        c.tried_delmin = true;
-                    } else {
-                        c.failed = c.finished = true;
-                    }
-                }
-                c.dt = dt;
-                acc[i] = 2;
-                continue;
-            }
-            c.newton_iters += its[i];
-            // evaluate_lte (transient.cpp:469-513), device LTE only
             double tmin_k, tmin_2;
             std::memcpy(&tmin_k, &h_tmin_pin_[i], sizeof(double));
             std::memcpy(&tmin_2, &h_tmin_pin_[NI_ + i], sizeof(double));
-            double device_dt = 1e30;
-            bool accepted = true;
-            if (c.step_count >= kLteMinStepCount) {
-                device_dt = std::min(device_dt, tmin_k);
-                device_dt = std::min(kMaxDtGrowthFactor * dt, device_dt);
-                if (device_dt < dt * kDeviceLteRejectRatio && dt > c.dt_min * 1.01) {
-                    accepted = false;
-                    ++c.rejected;
-                    c.ctrl_dt = dt;
-                }
+            TranStep rec{};
+            acc[i] = tc_decide(tcd_[i], bpt_.data(), bph_.data(), conv[i], its[i], tmin_k, tmin_2,
+                               h_ring_pin_[i], h_ctx_[i].order, h_ctx_[i].method, rec);
+            if (acc[i] == 1) {
+                hsteps_[i].push_back(rec);
+                prow_[i] = tcd_[i].step_count;
             }
-            if (!accepted) {
-                double proposed = c.proposed_dt;
-                if (device_dt < proposed) proposed = device_dt;
-                c.dt = std::max(proposed, c.dt_min);
-                acc[i] = 2;
-                continue;
-            }
-            // accept (transient.cpp:976-1063)
-            const double accepted_dt = dt;
-            c.prev_prev_dt = c.prev_dt;
-            c.prev_dt = dt;
-            advance(c, dt);
-            c.ctrl_dt = dt;
-            ++c.step_count;
-            c.tried_delmin = false;
-            if (c.steps_after_bp < kNoRecentBreakpoint) ++c.steps_after_bp;
-            c.steps.push_back(TranStep{c.time, accepted_dt, h_ctx_[i].order, h_ctx_[i].method, its[i], 0});
-            if (c.crossed_src_bp && c.step_count > 2) {
-                c.steps_after_bp = 0;
-                c.order = 1;
-                const double bp_gap = next_breakpoint_gap(c);
-                double scale = 0.1;  // SimOptions::restart_step_scale
-                if (!c.last_bp_hard) scale = std::sqrt(scale);
-                dt = std::min(dt, scale * std::min(c.saved_delta, bp_gap));
-                dt = std::max(dt, c.dt_min * 2.0);
-            }
-            if (c.order_promote_cooldown > 0) --c.order_promote_cooldown;
-            if (c.order == 1 && c.step_count >= 2 && c.order_promote_cooldown == 0) {
-                if (tmin_2 > 1.05 * dt) c.order = 2;
-            }
-            acc[i] = 1;
-            // detect_and_handle_ringing (transient.cpp:672-688)
-            if (!(c.step_count < 3 || c.use_gear || c.steps_after_bp < kRingingMinStepsAfterBp)) {
-                c.ringing = h_ring_pin_[i] != 0;
-                if (c.ringing) c.ringing_cooldown = 3;
-                if (c.ringing_cooldown > 0 && !c.ringing) --c.ringing_cooldown;
-                c.integrate_method = (c.ringing || c.ringing_cooldown > 0) ? 1 : 0;
-            }
-            if (c.step_count >= kLteMinStepCount) {
-                double proposed = device_dt;
-                proposed = std::min(proposed, kMaxDtGrowthFactor * dt);
-                dt = std::max(proposed, c.dt_min);
-            }
-            c.dt = dt;
-            prow_[i] = c.step_count;
         }
         // rotate accepted instances, restore the others (transient.cpp:919-975, 1004, 1027-1037)
         std::memcpy(h_acc_pin_, acc.data(), sizeof(int32_t) * NI_);
@@ -1355,10 +1270,184 @@ int Bundle::tran_advance(double t_until) {
     return rounds;
 }
 
+// The part of a tick after the Newton iteration: the instances whose Newton
+// solve ended take their step decision (truncation errors, ringing test,
+// tc_decide), then the accepted ones rotate states and history and the
+// others go back to their last accepted solution.
+void Bundle::launch_tick_tail() {
+    const TruncOpts to{opt_.abstol, opt_.reltol, opt_.chgtol, opt_.trtol};
+    const double big = 1e30;
+    unsigned long long big_bits;
+    std::memcpy(&big_bits, &big, sizeof(big));
+    launch_tc_mid(d_tc_, d_nt_, d_att_, d_tmin_, big_bits, NI_, stream_);
+    launch_trunc(trunc1_, d_ctx_, d_s_[0], d_s_[1], d_s_[2], d_s_[3], 1, to, nullptr, d_tmin_, stream_, 0,
+                 d_tmin_ + NI_, d_att_);
+    launch_trunc(trunc4_, d_ctx_, d_s_[0], d_s_[1], d_s_[2], d_s_[3], b4_lanes_, to, nullptr, d_tmin_, stream_,
+                 0, d_tmin_ + NI_, d_att_);
+    launch_tr_ring(NI_, d_ixoff_, d_inn_, d_x_, d_h1_, d_h2_, opt_.reltol, opt_.vntol, d_att_, d_ring_,
+                   stream_);
+    launch_tc_post(d_tc_, d_bpt_, d_bph_, d_nt_, d_ctx_, d_att_, d_tmin_, d_ring_, d_acc_, d_prow_, d_log_,
+                   log_rows_, NI_, stream_);
+    launch_tr_states(ns_tot_, ns_lane_, b4_lanes_, b4_.inst_of, d_slot_inst_, d_acc_, d_s_[0], d_s_[1],
+                     d_s_[2], d_s_[3], stream_);
+    launch_tr_hist(NI_, d_ixoff_, d_in_, d_x_, d_h1_, d_h2_, d_h3_, d_acc_, d_pptr_, d_probe_, d_prow_,
+                   d_pvals_, max_points_, stream_);
+}
+
+void Bundle::capture_tick_graph() {
+    if (tick_exec_) { NSA_HIP(hipGraphExecDestroy(tick_exec_)); tick_exec_ = nullptr; }
+    if (tick_graph_) { NSA_HIP(hipGraphDestroy(tick_graph_)); tick_graph_ = nullptr; }
+    NSA_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    launch_tc_pre(d_tc_, d_bpt_, d_nt_, d_ctx_, NI_, stream_);
+    launch_begin();
+    launch_loads();
+    launch_lu(d_kind_, SLU_FACTOR | SLU_SOLVE);
+    launch_post(0);
+    launch_tick_tail();
+    NSA_HIP(hipStreamEndCapture(stream_, &tick_graph_));
+    NSA_HIP(hipGraphInstantiate(&tick_exec_, tick_graph_, nullptr, nullptr, 0));
+    tick_words_ = plans_.d_words();
+    tick_nlu_ = plans_.max_nlu();
+    tick_n_ = plans_.max_n();
+    tick_hot_ = plans_.max_hot();
+    tick_log_ = d_log_;
+}
+
+// One tick: every instance at its loop head sets up its next step, every
+// instance in a Newton solve runs one iteration, and every instance whose
+// solve ended takes its step decision.
+void Bundle::tick_device() {
+    if (inst_dirty_) upload_inst();
+    if (timing_) {
+        launch_tc_pre(d_tc_, d_bpt_, d_nt_, d_ctx_, NI_, stream_);
+        NSA_HIP(hipEventRecord(ev_[0], stream_));
+        launch_begin();
+        launch_loads();
+        NSA_HIP(hipEventRecord(ev_[3], stream_));
+        launch_lu(d_kind_, SLU_FACTOR | SLU_SOLVE);
+        NSA_HIP(hipEventRecord(ev_[4], stream_));
+        launch_post(0);
+        NSA_HIP(hipEventRecord(ev_[5], stream_));
+        timed_pending_ = true;
+        launch_tick_tail();
+    } else if (use_graph) {
+        if (!tick_exec_ || graph_stale(tick_words_, tick_nlu_, tick_n_, tick_hot_) || tick_log_ != d_log_)
+            capture_tick_graph();
+        NSA_HIP(hipGraphLaunch(tick_exec_, stream_));
+        ++n_graph_launches;
+    } else {
+        launch_tc_pre(d_tc_, d_bpt_, d_nt_, d_ctx_, NI_, stream_);
+        launch_begin();
+        launch_loads();
+        launch_lu(d_kind_, SLU_FACTOR | SLU_SOLVE);
+        launch_post(0);
+        launch_tick_tail();
+    }
+    ++n_ticks;
+}
+
+// Room in the device step logs for `rows` accepted steps per instance
+// (stream-ordered; the logs keep what they hold).
+void Bundle::tran_log_reserve(int rows) {
+    if (rows <= log_rows_) return;
+    const int cap = std::max(rows, std::max(2 * log_rows_, 1024));
+    TranStep* nl = nullptr;
+    NSA_HIP(hipStreamSynchronize(stream_));
+    NSA_HIP(hipMalloc(&nl, sizeof(TranStep) * (size_t)cap * NI_));
+    if (d_log_ && log_rows_)
+        NSA_HIP(hipMemcpy2DAsync(nl, sizeof(TranStep) * cap, d_log_, sizeof(TranStep) * log_rows_,
+                                 sizeof(TranStep) * log_rows_, NI_, hipMemcpyDeviceToDevice, stream_));
+    NSA_HIP(hipStreamSynchronize(stream_));
+    for (auto it = tran_allocs_.begin(); it != tran_allocs_.end(); ++it)
+        if (*it == d_log_) {
+            NSA_HIP(hipFree(d_log_));
+            tran_allocs_.erase(it);
+            break;
+        }
+    tran_allocs_.push_back(nl);
+    d_log_ = nl;
+    log_rows_ = cap;
+}
+
+// Instances waiting in a Newton iteration for the host: their matrices are
+// ordered on the CPU (a fresh solver's first factorization, or a reused
+// order that failed its threshold test) and their iteration ends on the
+// device; no tick is in flight.
+void Bundle::handle_stalls() {
+    std::vector<NtInst> nt(NI_);
+    NSA_HIP(hipMemcpyAsync(h_nt_pin_, d_nt_, sizeof(NtInst) * NI_, hipMemcpyDeviceToHost, stream_));
+    NSA_HIP(hipStreamSynchronize(stream_));
+    std::memcpy(nt.data(), h_nt_pin_, sizeof(NtInst) * NI_);
+    std::vector<int> stalled;
+    for (int i = 0; i < NI_; ++i)
+        if (!nt[i].done && nt[i].stalled) stalled.push_back(i);
+    if (stalled.empty()) return;
+    if (inst_dirty_) upload_inst();
+    host_lu_post(nt, stalled, false);
+}
+
+// Device control: ticks in batches sized from the work left in the window;
+// between batches the host reads a status summary and serves stalled
+// instances.
+int Bundle::tran_advance_device(double t_until) {
+    if (!log_on_device_ && !hsteps_.empty()) {
+        // steps taken under host control move into the device logs
+        int most = 0;
+        for (const auto& v : hsteps_) most = std::max(most, (int)v.size());
+        tran_log_reserve(most + 64);
+        for (int i = 0; i < NI_; ++i)
+            if (!hsteps_[i].empty())
+                NSA_HIP(hipMemcpyAsync(d_log_ + (size_t)i * log_rows_, hsteps_[i].data(),
+                                       sizeof(TranStep) * hsteps_[i].size(), hipMemcpyHostToDevice, stream_));
+        NSA_HIP(hipStreamSynchronize(stream_));
+        log_on_device_ = true;
+    }
+    if (!tc_on_device_) {
+        NSA_HIP(hipMemcpyAsync(d_tc_, tcd_.data(), sizeof(TranDev) * NI_, hipMemcpyHostToDevice, stream_));
+        NSA_HIP(hipMemcpyAsync(d_prow_, prow_.data(), sizeof(int32_t) * NI_, hipMemcpyHostToDevice, stream_));
+        NSA_HIP(hipStreamSynchronize(stream_));
+        tc_on_device_ = true;
+    }
+    int64_t it0 = 0;
+    for (const auto& c : tcd_) it0 += c.all_iters;
+    launch_tc_resume(d_tc_, NI_, t_until, stream_);
+    int ticks = 0;
+    for (;;) {
+        launch_tc_status(d_tc_, d_nt_, NI_, d_tcs_, stream_);
+        NSA_HIP(hipMemcpyAsync(h_tcs_pin_, d_tcs_, sizeof(TcStatus), hipMemcpyDeviceToHost, stream_));
+        NSA_HIP(hipStreamSynchronize(stream_));
+        collect_timing();
+        const TcStatus st = *h_tcs_pin_;
+        if (st.stalled > 0) {
+            handle_stalls();
+            continue;
+        }
+        if (st.active == 0) break;
+        // at least two iterations per step (iteration 0 never converges)
+        int batch = 64;
+        if (st.est_steps < 32.0) batch = std::max(2, 2 * (int)std::ceil(st.est_steps));
+        // a fresh solver's instances stall in their first iteration
+        if (timing_ || std::find(plan_.begin(), plan_.end(), -1) != plan_.end()) batch = 1;
+        // a tick accepts at most one step per instance
+        tran_log_reserve(st.max_steps + batch + 1);
+        for (int k = 0; k < batch; ++k) tick_device();
+        ticks += batch;
+    }
+    NSA_HIP(hipMemcpyAsync(tcd_.data(), d_tc_, sizeof(TranDev) * NI_, hipMemcpyDeviceToHost, stream_));
+    NSA_HIP(hipMemcpyAsync(h_ctx_pin_, d_ctx_, sizeof(DevCtx) * NI_, hipMemcpyDeviceToHost, stream_));
+    NSA_HIP(hipMemcpyAsync(prow_.data(), d_prow_, sizeof(int32_t) * NI_, hipMemcpyDeviceToHost, stream_));
+    NSA_HIP(hipStreamSynchronize(stream_));
+    std::memcpy(h_ctx_.data(), h_ctx_pin_, sizeof(DevCtx) * NI_);
+    int64_t it1 = 0;
+    for (const auto& c : tcd_) it1 += c.all_iters;
+    n_iterations += (long)(it1 - it0);
+    return ticks;
+}
+
 void Bundle::tran_probes(int i, int* count, double* out) const {
     if (i < 0 || i >= NI_) throw Error(NSA_ERR_ARGUMENT, "Bundle::tran_probes: instance");
     const int np = tran_probe_count(i);
-    const int rows = std::min(max_points_, (tc_.empty() ? 0 : tc_[i].step_count) + 1);
+    const int rows = std::min(max_points_, (tcd_.empty() ? 0 : tcd_[i].step_count) + 1);
     *count = rows;
     if (!np || !out || !d_pvals_) return;
     NSA_HIP(hipMemcpyAsync(out, d_pvals_ + (size_t)pptr_[i] * max_points_, sizeof(double) * rows * np,

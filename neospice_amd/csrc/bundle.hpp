@@ -14,10 +14,13 @@
 //   * Newton control: newton_solve's per-iteration bookkeeping
 //     (src/core/newton.cpp:136-412) in a kernel with per-instance state, so
 //     an iteration is a fixed sequence of launches captured in a HIP graph.
+//   * transient step control: solve_transient's loop body per instance
+//     (tran_ctl.hpp) in kernels, so that in a "tick" every instance runs one
+//     Newton iteration of its own current step, and an instance whose
+//     Newton solve ends takes its step decision and starts its next step in
+//     the same tick, without waiting for the others or for the host.
 // The host keeps what the reference decides from values on the CPU: the
-// Markowitz pivot orders (one Sparse 1.3 state per instance) and the
-// transient step controller of solve_transient (src/core/transient.cpp:
-// 694-1066, src/core/timestep.cpp), restated per instance.
+// Markowitz pivot orders (one Sparse 1.3 state per instance).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -29,9 +32,11 @@
 #include "bsim4.hpp"
 #include "circuit.hpp"
 #include "engine.hpp"
+#include "gpu_bundle.hpp"
 #include "gpu_dev.hpp"
 #include "lu_cpu.hpp"
 #include "small_lu.hpp"
+#include "tran_ctl.hpp"
 
 namespace nsa {
 
@@ -47,35 +52,6 @@ struct NtInst {
     int32_t kind;            // LU call of the current iteration (SluKind)
     int32_t stalled;         // a numeric() whose pivot order failed: the host re-pivots
     int32_t pad[3];
-};
-
-// One accepted transient step of an instance.
-struct TranStep {
-    double t, dt;
-    int32_t order, method, iters, pad;
-};
-
-// Per-instance state of the adaptive transient loop (solve_transient with
-// its TimeStepController): everything the loop body reads or writes.
-struct TranCtl {
-    // TimeStepController (timestep.hpp:9-85)
-    double ctrl_dt = 0.0, time = 0.0, tstop = 0.0, proposed_dt = 0.0, prev_dt = 0.0;
-    double min_break = 0.0;
-    int rejected = 0, order = 1;
-    bool crossed_src_bp = false, last_bp_hard = true, ringing = false;
-    int ringing_cooldown = 0;
-    std::set<double> breakpoints;
-    std::map<double, bool> src_breakpoints;  // time -> HARD
-    // loop locals of solve_transient
-    double dt = 0.0, saved_delta = 0.0, prev_prev_dt = 0.0, dt_min = 0.0, dt_max = 0.0;
-    int step_count = 0, steps_after_bp = 1000, order_promote_cooldown = 0;
-    int total_iterations = 0, integrate_method = 0;
-    bool use_gear = false, tried_delmin = false;
-    bool finished = false, failed = false;
-    long newton_iters = 0;       // iterations of converged steps (TransientResult)
-    long all_iters = 0;          // every iteration run, including failed and rejected steps
-    double attempt_dt = 0.0;     // the step being attempted
-    std::vector<TranStep> steps;
 };
 
 class Bundle {
@@ -131,11 +107,19 @@ public:
     // (probe_ptr[NI+1] into probes; may be null), up to max_points rows.
     void tran_begin(double tstep, double tstop, const int32_t* probe_ptr, const int32_t* probes,
                     int max_points);
-    // Runs batched steps until every instance has reached t_until (or tstop,
-    // or failed).  Returns the number of batched rounds.
+    // Runs steps until every instance has reached t_until (or tstop, or
+    // failed).  Returns the number of ticks (device control) or batched
+    // rounds (host control).
     int tran_advance(double t_until);
     bool tran_done() const;
-    const TranCtl& tran_ctl(int i) const { return tc_[i]; }
+    // controller state of instance i (current after tran_advance returns)
+    const TranDev& tran_state(int i) const { return tcd_[i]; }
+    // accepted steps of instance i
+    std::vector<TranStep> tran_steps(int i) const;
+    // true (default): the step controller runs on the device in ticks; false:
+    // on the host between batched Newton rounds (the same decisions; every
+    // instance's results are bit-identical either way).
+    bool device_control = true;
     // probe rows of instance i: (count, values row-major count x nprobe_i)
     void tran_probes(int i, int* count, double* out) const;
     int tran_probe_count(int i) const { return pptr_.empty() ? 0 : pptr_[i + 1] - pptr_[i]; }
@@ -161,6 +145,7 @@ public:
 
     // ---- statistics
     long n_cpu_pivot = 0, n_iterations = 0, n_graph_launches = 0, n_host_iterations = 0;
+    long n_ticks = 0;  // transient ticks run under device control
     int num_plans() const { return plans_.count(); }
     const SmallLuPlans& plans() const { return plans_; }
     int plan_of(int i) const { return plan_[i]; }
@@ -186,9 +171,14 @@ private:
     void upload_inst();
     void ensure_adjoint();
     void refresh_size_derivs();
-    void tran_attempt_setup(int i, DevCtx& c);
-    void tran_post_round(const std::vector<int32_t>& attempt, const std::vector<int32_t>& conv,
-                         const std::vector<int32_t>& its);
+    int tran_advance_host(double t_until);
+    int tran_advance_device(double t_until);
+    void launch_tick_tail();
+    void tick_device();
+    void capture_tick_graph();
+    bool graph_stale(const int32_t* words, int nlu, int n, int hot) const;
+    void tran_log_reserve(int rows);
+    void handle_stalls();
 
     std::vector<const Circuit*> ckt_;
     int NI_ = 0, tot_n_ = 0, tot_nnz_ = 0, ns_tot_ = 0;
@@ -243,11 +233,17 @@ private:
     // graph of one device-driven iteration
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
+    // graph of one transient tick, and the launch shape it was captured with
+    hipGraph_t tick_graph_ = nullptr;
+    hipGraphExec_t tick_exec_ = nullptr;
+    const int32_t* tick_words_ = nullptr;
+    int tick_nlu_ = 0, tick_n_ = 0, tick_hot_ = 0;
+    TranStep* tick_log_ = nullptr;
     bool timing_ = false;
     hipEvent_t ev_[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     bool timed_pending_ = false;
     const int32_t* graph_words_ = nullptr;
-    int graph_nlu_ = 0, graph_n_ = 0;
+    int graph_nlu_ = 0, graph_n_ = 0, graph_hot_ = 0;
     int b4_lanes_ = 0;
     long ns_lane_ = 0;
     int32_t* d_slot_inst_ = nullptr;
@@ -258,8 +254,21 @@ private:
     int32_t* h_mask_pin_ = nullptr;
     double* d_lad_[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     std::vector<int32_t> dc_stage_;
-    // transient
-    std::vector<TranCtl> tc_;
+    // transient: controllers (host mirror and device copy), breakpoints,
+    // step logs
+    std::vector<TranDev> tcd_;
+    TranDev* d_tc_ = nullptr;
+    bool tc_on_device_ = false;                 // d_tc_ holds the current controllers
+    std::vector<double> bpt_;                   // breakpoint times, packed per instance
+    std::vector<int32_t> bph_;                  // their HARD flags
+    double* d_bpt_ = nullptr;
+    int32_t* d_bph_ = nullptr;
+    std::vector<std::vector<TranStep>> hsteps_;  // host-control step logs
+    TranStep* d_log_ = nullptr;                 // device-control step logs, log_rows_ per instance
+    int log_rows_ = 0;
+    bool log_on_device_ = false;
+    TcStatus* d_tcs_ = nullptr;
+    TcStatus* h_tcs_pin_ = nullptr;
     double tstep_ = 0.0, tstop_ = 0.0;
     unsigned long long* d_tmin_ = nullptr;   // 2 x NI: order-k proposals, order-2 proposals
     int32_t* d_ring_ = nullptr;              // NI ringing flags

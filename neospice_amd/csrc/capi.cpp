@@ -611,8 +611,8 @@ int nsa_bundle_tran_advance(nsa_bundle* b, double t_until, int32_t* rounds) {
 int nsa_bundle_tran_status(nsa_bundle* b, int32_t inst, int64_t out[8]) {
     return guard([&] {
         if (inst < 0 || inst >= b->b->size()) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "instance out of range");
-        const nsa::TranCtl& c = b->b->tran_ctl(inst);
-        out[0] = (int64_t)c.steps.size();
+        const nsa::TranDev& c = b->b->tran_state(inst);
+        out[0] = c.step_count;
         out[1] = c.rejected;
         out[2] = c.failed ? 1 : 0;
         out[3] = c.finished ? 1 : 0;
@@ -627,7 +627,7 @@ int nsa_bundle_tran_steps(nsa_bundle* b, int32_t inst, double* t, double* dt, in
                           int32_t* method, int32_t* iters) {
     return guard([&] {
         if (inst < 0 || inst >= b->b->size()) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "instance out of range");
-        const auto& st = b->b->tran_ctl(inst).steps;
+        const std::vector<nsa::TranStep> st = b->b->tran_steps(inst);
         for (size_t k = 0; k < st.size(); ++k) {
             if (t) t[k] = st[k].t;
             if (dt) dt[k] = st[k].dt;
@@ -681,6 +681,9 @@ int nsa_bundle_timing(nsa_bundle* b, double out[5]) {
 }
 int nsa_bundle_set_graph(nsa_bundle* b, int32_t on) {
     return guard([&] { b->b->use_graph = on != 0; return 0; });
+}
+int nsa_bundle_set_device_control(nsa_bundle* b, int32_t on) {
+    return guard([&] { b->b->device_control = on != 0; return 0; });
 }
 int nsa_bundle_dc_ladder(nsa_bundle* b, int32_t flags, int32_t* converged, int32_t* iterations,
                          int32_t* stage) {

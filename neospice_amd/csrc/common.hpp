@@ -39,6 +39,7 @@ enum FailCode : int32_t {
     FAIL_GROWTH = 2,       // GP replay pivot-growth check failed
     FAIL_REORDER = 3,      // pivot no longer threshold-acceptable: CPU re-pivot
     FAIL_TIMEOUT = 4,      // a kernel's bounded wait for a dependency ran out (a bug)
+    FAIL_NOPLAN = 5,       // no factorization yet (a fresh solver): the host orders it
 };
 
 }  // namespace nsa

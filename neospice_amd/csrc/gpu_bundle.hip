@@ -62,6 +62,12 @@ __global__ __launch_bounds__(kW) void k_nt_post(NtInst* __restrict__ nt, const i
     const int iter = s.iter;
     int mode = ctx[i].mode;
     bool finish = false, restore = false;
+    if (f == FAIL_NOPLAN) {
+        // a fresh solver's first factorization: the host orders this instance
+        s.stalled = 1;
+        if (lane == 0) nt[i] = s;
+        return;
+    }
     if (f != FAIL_NONE) {
         if (kd == SLU_REFACTOR) {
             // singular refactorize(): numeric() next iteration, nothing solved
@@ -227,6 +233,119 @@ __global__ void k_copy_f64(double* __restrict__ dst, const double* __restrict__ 
     if (k < count) dst[k] = src[k];
 }
 
+// ---- the transient's tick (one thread per instance)
+
+__global__ void k_tc_pre(TranDev* __restrict__ tc, const double* __restrict__ bpt, NtInst* __restrict__ nt,
+                         DevCtx* __restrict__ ctx, int ni) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ni) return;
+    TranDev c = tc[i];
+    if (c.phase != TP_READY) return;
+    DevCtx x;
+    if (tc_attempt(c, bpt, x)) {
+        ctx[i] = x;
+        NtInst s{};
+        s.saved_mode = x.mode;
+        s.done = 0;
+        s.itl = c.itl;
+        s.kind = SLU_SKIP;
+        nt[i] = s;
+        c.phase = TP_NEWTON;
+    } else {
+        c.phase = TP_IDLE;
+    }
+    tc[i] = c;
+}
+
+__global__ void k_tc_mid(const TranDev* __restrict__ tc, const NtInst* __restrict__ nt, int32_t* __restrict__ att,
+                         unsigned long long* __restrict__ tmin, unsigned long long big, int ni) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ni) return;
+    const bool fin = tc[i].phase == TP_NEWTON && nt[i].done;
+    att[i] = fin ? 1 : 0;
+    if (fin) {
+        tmin[i] = big;
+        tmin[ni + i] = big;
+    }
+}
+
+__global__ void k_tc_post(TranDev* __restrict__ tc, const double* __restrict__ bpt,
+                          const int32_t* __restrict__ bph, const NtInst* __restrict__ nt,
+                          const DevCtx* __restrict__ ctx, const int32_t* __restrict__ att,
+                          const unsigned long long* __restrict__ tmin, const int32_t* __restrict__ ring,
+                          int32_t* __restrict__ acc, int32_t* __restrict__ prow, TranStep* __restrict__ log,
+                          int max_log, int ni) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ni) return;
+    if (!att[i]) {
+        acc[i] = 0;
+        return;
+    }
+    TranDev c = tc[i];
+    const NtInst s = nt[i];
+    const DevCtx x = ctx[i];
+    TranStep rec{};
+    const int a = tc_decide(c, bpt, bph, s.converged, s.iter, __longlong_as_double((long long)tmin[i]),
+                            __longlong_as_double((long long)tmin[ni + i]), ring[i], x.order, x.method, rec);
+    if (a == 1) {
+        const int row = c.step_count - 1;
+        if (row < max_log) log[(size_t)i * max_log + row] = rec;
+        prow[i] = c.step_count;
+    }
+    acc[i] = a;
+    c.phase = TP_READY;
+    tc[i] = c;
+}
+
+__global__ void k_tc_resume(TranDev* __restrict__ tc, int ni, double t_until) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ni) return;
+    TranDev c = tc[i];
+    c.t_until = t_until;
+    if (!c.finished && c.phase == TP_IDLE) c.phase = TP_READY;
+    tc[i] = c;
+}
+
+__global__ __launch_bounds__(256) void k_tc_status(const TranDev* __restrict__ tc, const NtInst* __restrict__ nt,
+                                                   int ni, TcStatus* __restrict__ out) {
+    __shared__ int s_act[256], s_st[256], s_ms[256];
+    __shared__ double s_est[256];
+    const int t = threadIdx.x;
+    int act = 0, st = 0, ms = 0;
+    double est = 0.0;
+    for (int i = t; i < ni; i += 256) {
+        const TranDev& c = tc[i];
+        ms = c.step_count > ms ? c.step_count : ms;
+        if (c.phase == TP_IDLE) continue;
+        ++act;
+        if (c.phase == TP_NEWTON && nt[i].stalled && !nt[i].done) ++st;
+        const double end = c.t_until < c.tstop ? c.t_until : c.tstop;
+        const double e = c.dt > 0.0 ? (end - c.time) / c.dt : 0.0;
+        est = e > est ? e : est;
+    }
+    s_act[t] = act;
+    s_st[t] = st;
+    s_est[t] = est;
+    s_ms[t] = ms;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) {
+            s_act[t] += s_act[t + o];
+            s_st[t] += s_st[t + o];
+            s_ms[t] = s_ms[t + o] > s_ms[t] ? s_ms[t + o] : s_ms[t];
+            s_est[t] = s_est[t + o] > s_est[t] ? s_est[t + o] : s_est[t];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out->active = s_act[0];
+        out->stalled = s_st[0];
+        out->max_steps = s_ms[0];
+        out->pad = 0;
+        out->est_steps = s_est[0];
+    }
+}
+
 }  // namespace
 
 void launch_nt_begin(NtInst* nt, int ni, const int32_t* xoff, const int32_t* n, const double* x,
@@ -279,6 +398,35 @@ void launch_masked_copy(long ns, long ns_lane, int lanes, const int32_t* lane_in
     const int tpb = 256;
     hipLaunchKernelGGL(k_masked_copy, dim3((unsigned)((ns + tpb - 1) / tpb)), dim3(tpb), 0, s, ns, ns_lane,
                        lanes > 0 ? lanes : 1, lane_inst, slot_inst, mask, src, dst, src ? 0 : 1);
+}
+
+void launch_tc_pre(TranDev* tc, const double* bpt, NtInst* nt, DevCtx* ctx, int ni, hipStream_t s) {
+    if (ni <= 0) return;
+    hipLaunchKernelGGL(k_tc_pre, dim3((ni + kW - 1) / kW), dim3(kW), 0, s, tc, bpt, nt, ctx, ni);
+}
+
+void launch_tc_mid(const TranDev* tc, const NtInst* nt, int32_t* att, unsigned long long* tmin,
+                   unsigned long long big, int ni, hipStream_t s) {
+    if (ni <= 0) return;
+    hipLaunchKernelGGL(k_tc_mid, dim3((ni + kW - 1) / kW), dim3(kW), 0, s, tc, nt, att, tmin, big, ni);
+}
+
+void launch_tc_post(TranDev* tc, const double* bpt, const int32_t* bph, const NtInst* nt,
+                    const DevCtx* ctx, const int32_t* att, const unsigned long long* tmin,
+                    const int32_t* ring, int32_t* acc, int32_t* prow, TranStep* log, int max_log,
+                    int ni, hipStream_t s) {
+    if (ni <= 0) return;
+    hipLaunchKernelGGL(k_tc_post, dim3((ni + kW - 1) / kW), dim3(kW), 0, s, tc, bpt, bph, nt, ctx, att, tmin,
+                       ring, acc, prow, log, max_log, ni);
+}
+
+void launch_tc_resume(TranDev* tc, int ni, double t_until, hipStream_t s) {
+    if (ni <= 0) return;
+    hipLaunchKernelGGL(k_tc_resume, dim3((ni + kW - 1) / kW), dim3(kW), 0, s, tc, ni, t_until);
+}
+
+void launch_tc_status(const TranDev* tc, const NtInst* nt, int ni, TcStatus* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_tc_status, dim3(1), dim3(256), 0, s, tc, nt, ni, out);
 }
 
 void launch_copy_f64(double* dst, const double* src, long count, hipStream_t s) {

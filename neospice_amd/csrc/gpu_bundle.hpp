@@ -12,6 +12,17 @@ namespace nsa {
 
 struct NtInst;
 struct DevCtx;
+struct TranDev;
+struct TranStep;
+
+// Summary of a bundle's transient for the host between batches of ticks.
+struct TcStatus {
+    int32_t active;    // instances not idle (taking steps in the current window)
+    int32_t stalled;   // instances waiting for the host to order their matrix
+    int32_t max_steps; // most accepted steps of any instance
+    int32_t pad;
+    double est_steps;  // max over active instances of (window end - time) / dt
+};
 
 struct NtConvTol {
     double reltol, vntol, abstol;
@@ -71,6 +82,33 @@ void launch_bundle_grad_lin(int nr, const int32_t* r_nodes, const double* ohms, 
                             const int32_t* i_nodes, const int32_t* i_inst, const DevCtx* ctx,
                             const double* x, const double* lam, double* gR, double* gV, double* gI,
                             hipStream_t s);
+// ---- the transient's tick: one Newton iteration of every instance, each
+// at its own step.  A tick is tc_pre, the iteration's launches, tc_mid, the
+// truncation and ringing kernels (masked by att), tc_post, then the state
+// rotation and history kernels (masked by acc).
+//
+// Loop head of every READY instance (tc_attempt): its step context and a
+// fresh Newton state (newton_solve's locals, limit tc.itl); instances that
+// take no step go IDLE.
+void launch_tc_pre(TranDev* tc, const double* bpt, NtInst* nt, DevCtx* ctx, int ni, hipStream_t s);
+// Instances whose Newton solve ended in this tick: att[i] = 1 and their
+// truncation minima reset to `big`; att[i] = 0 for the rest.
+void launch_tc_mid(const TranDev* tc, const NtInst* nt, int32_t* att, unsigned long long* tmin,
+                   unsigned long long big, int ni, hipStream_t s);
+// The step decision of every instance with att[i] (tc_decide): acc[i] = 1
+// accepted (the step logged at row step_count - 1 of log, up to max_log;
+// prow[i] = step_count), 2 restore; 0 for the rest.  The instance is READY
+// again.
+void launch_tc_post(TranDev* tc, const double* bpt, const int32_t* bph, const NtInst* nt,
+                    const DevCtx* ctx, const int32_t* att, const unsigned long long* tmin,
+                    const int32_t* ring, int32_t* acc, int32_t* prow, TranStep* log, int max_log,
+                    int ni, hipStream_t s);
+// A new window end for every instance; idle instances that are not
+// finished become READY.
+void launch_tc_resume(TranDev* tc, int ni, double t_until, hipStream_t s);
+// The status summary (one workgroup).
+void launch_tc_status(const TranDev* tc, const NtInst* nt, int ni, TcStatus* out, hipStream_t s);
+
 // dst[i] = src[i] for i < count.
 void launch_copy_f64(double* dst, const double* src, long count, hipStream_t s);
 

@@ -409,12 +409,17 @@ __global__ void k_trunc(TruncTable t, const DevCtx* __restrict__ ctx, const doub
                         const double* __restrict__ s1, const double* __restrict__ s2,
                         const double* __restrict__ s3, int B, TruncOpts o, double* __restrict__ dt_dev,
                         unsigned long long* __restrict__ dt_min, int order_override,
-                        unsigned long long* __restrict__ dt_min2) {
-    // lanes past the end stay for the wave-wide reductions, with no instance
+                        unsigned long long* __restrict__ dt_min2, const int32_t* __restrict__ sel) {
+    // lanes past the end, and lanes of instances sel leaves out, stay for
+    // the wave-wide reductions with no instance
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = idx < t.ndev * B;
+    bool valid = idx < t.ndev * B;
     const int inst = valid ? idx % B : 0, dev = valid ? idx / B : 0;
-    const int ci = !valid ? -1 : (t.inst_of ? t.inst_of[idx] : inst);
+    int ci = !valid ? -1 : (t.inst_of ? t.inst_of[idx] : inst);
+    if (valid && sel && !sel[ci]) {
+        valid = false;
+        ci = -1;
+    }
     DevCtx c = ctx[valid ? ci : 0];
     if (order_override > 0) c.order = order_override;
     // the device's proposal at context c (compute_trunc's body)
@@ -460,11 +465,11 @@ __global__ void k_fill_u64(unsigned long long* p, unsigned long long v, int coun
 void launch_trunc(const TruncTable& t, const DevCtx* ctx, const double* s0, const double* s1,
                   const double* s2, const double* s3, int B, const TruncOpts& o, double* dt_dev,
                   unsigned long long* dt_min, hipStream_t s, int order_override,
-                  unsigned long long* dt_min2) {
+                  unsigned long long* dt_min2, const int32_t* sel) {
     const int total = t.ndev * B;
     if (total <= 0) return;
     hipLaunchKernelGGL(k_trunc, dim3((total + 255) / 256), dim3(256), 0, s, t, ctx, s0, s1, s2, s3, B, o,
-                       dt_dev, dt_min, order_override, dt_min2);
+                       dt_dev, dt_min, order_override, dt_min2, sel);
 }
 
 void launch_fill_u64(unsigned long long* p, unsigned long long v, int count, hipStream_t s) {

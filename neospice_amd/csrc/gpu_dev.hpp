@@ -224,11 +224,12 @@ struct TruncOpts {
 // order_override > 0 evaluates the devices at that integration order instead
 // of the context's; dt_min2 (optional) receives the per-instance minimum at
 // order 2 from the same states (the order-promotion test,
-// transient.cpp:946-958).
+// transient.cpp:946-958).  sel (optional, per instance): only instances with
+// sel[i] != 0 are evaluated; the others' dt_min entries are left alone.
 void launch_trunc(const TruncTable& t, const DevCtx* ctx, const double* s0, const double* s1,
                   const double* s2, const double* s3, int B, const TruncOpts& o, double* dt_dev,
                   unsigned long long* dt_min, hipStream_t s, int order_override = 0,
-                  unsigned long long* dt_min2 = nullptr);
+                  unsigned long long* dt_min2 = nullptr, const int32_t* sel = nullptr);
 void launch_fill_u64(unsigned long long* p, unsigned long long v, int count, hipStream_t s);
 // out[k] = A[k * B + inst], k < count (one instance of a slot-major batch)
 void launch_take_instance(const double* A, int B, int inst, int count, double* out, hipStream_t s);

@@ -318,6 +318,14 @@ __global__ __launch_bounds__(kW) void k_small_lu(const int32_t* __restrict__ wor
     const int kd = kind ? kind[ii] : SLU_REFACTOR;
     if (kd == SLU_SKIP) return;
     const SluInst d = insts[ii];
+    if (d.plan < 0) {
+        // no pivot order yet: nothing to factor or solve on the device
+        if (lane == 0 && fail) {
+            fail[ii] = FAIL_NOPLAN;
+            if (fail_col) fail_col[ii] = 0;
+        }
+        return;
+    }
     const int32_t* __restrict__ P = words + offs[d.plan];
     const int n = P[SH_N], nlu = P[SH_NLU];
     // LDS: the factor values, the vector right after them (the forward

@@ -198,3 +198,45 @@ def test_bundle_tran_c1_full_size():
     # the step sizes at ~1e-8 (the times and the step sequence stay the same)
     b, worst = adaptive_parity([c], 10e-12, 2e-9, dt_rtol=1e-7)
     print("worst", worst, "accepted", b.tran_status(0)["accepted"])
+
+
+def test_bundle_tran_device_control_matches_host():
+    """The step controller on the device (every instance on its own step
+    sequence, one Newton iteration per tick) against the same controller on
+    the host between batched rounds: per instance the same accepted steps,
+    rejections, iteration counts, probed waveform and final solution, bit
+    for bit -- also when the device run is cut into windows."""
+    cks = []
+    for make in (lambda: W.diode_mesh(4, t_stop=100e-12), lambda: W.inverter_chain(4)):
+        c, _ = make()
+        cks.append(c)
+    rings, _ = W.c3_rings(stages=[9, 31, 11])
+    cks += rings
+    for c in cks:
+        c.finalize()
+    tstep, tstop = 2e-12, 150e-12
+    probes = [range(c.n) for c in cks]
+    bh = na.Bundle(cks)
+    bh.set_device_control(False)
+    bh.tran_begin(tstep, tstop, probes=probes, max_points=4000)
+    rounds = 0
+    for w in (40e-12, 95e-12, tstop):
+        assert bh.tran_advance(w) == (w == tstop)
+        rounds += bh.last_rounds
+    bd = na.Bundle(cks)
+    bd.tran_begin(tstep, tstop, probes=probes, max_points=4000)
+    ticks = 0
+    for w in (40e-12, 95e-12, tstop):
+        assert bd.tran_advance(w) == (w == tstop)
+        ticks += bd.last_rounds
+    print("host rounds", rounds, "device ticks", ticks, flush=True)
+    for i in range(len(cks)):
+        sh, sd = bh.tran_status(i), bd.tran_status(i)
+        assert sh == sd, (i, sh, sd)
+        assert sh["finished"] and not sh["failed"]
+        th, td = bh.tran_steps(i), bd.tran_steps(i)
+        for k in th:
+            assert np.array_equal(th[k], td[k]), (i, k)
+        assert np.array_equal(bh.tran_probes(i), bd.tran_probes(i)), i
+        assert np.array_equal(bh.solution(i), bd.solution(i)), i
+    assert bh.stats()["iterations"] == bd.stats()["iterations"]
