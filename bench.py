@@ -489,8 +489,10 @@ def c3_headline(na, W, args):
     acc = c3_check_status(b, N)
     value = iters / wall
     st = b.stats()
+    ticks = b.tick_stats()
     log("bench: C3 batched: %d windows, %d Newton iterations in %.3f s -> %.0f iters/s (operating point %.3f s)"
         % (K, iters, wall, value, dc_s))
+    log("bench: C3 ticks", ticks)
     info = [b.plan_info(i) for i in range(N)]
 
     # HIP events around every packed-LU launch over a replay of the whole
@@ -590,6 +592,9 @@ def c3_headline(na, W, args):
                               "mean": round(float(accepted.mean()), 2),
                               "rejected": int(sum(a["rejected"] for a in acc))},
            "operating_point_s": round(dc_s, 4),
+           "ticks": {"count": ticks["ticks"], "host_looks": ticks["batches"],
+                     "us_per_tick": round(wall / max(1, ticks["ticks"]) * 1e6, 2),
+                     "host_enqueue_s": round(ticks["enqueue_s"], 4), "host_wait_s": round(ticks["wait_s"], 4)},
            "batch1_gpu_value": round(batch1_value, 2) if batch1_value else None,
            "speedup_vs_batch1_gpu": round(value / batch1_value, 3) if batch1_value else None,
            "batch1_sample": b1_rows,
@@ -664,7 +669,7 @@ def main():
         "config": c3["config"],
     }
     for k in ("batch1_gpu_value", "speedup_vs_batch1_gpu", "batch1_bit_equal_to_batched", "batch1_sample",
-              "accepted_steps", "operating_point_s", "parity", "roofline", "kernels", "cpu_baseline"):
+              "accepted_steps", "operating_point_s", "ticks", "parity", "roofline", "kernels", "cpu_baseline"):
         out[k] = c3[k]
     out["adjoint"] = adjoint
     out["rc_mesh"] = mesh

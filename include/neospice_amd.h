@@ -245,6 +245,10 @@ int nsa_bundle_set_graph(nsa_bundle* b, int32_t on);
  * device, every instance on its own step sequence; 0: on the host between
  * batched Newton rounds.  Both give the same results per instance. */
 int nsa_bundle_set_device_control(nsa_bundle* b, int32_t on);
+/* device-controlled transient: ticks run, host looks between batches of
+ * ticks, host seconds spent enqueueing ticks and waiting for the device
+ * (no reference counterpart: statistics of this engine) */
+int nsa_bundle_tick_stats(nsa_bundle* b, double out[4]);
 /* pivot structure of an instance: plan index (-1 before its first
    factorization), n, nnz(A), nnz(L+U), levels of a factor-and-solve launch
    (the refactor with the forward substitution folded in, then the

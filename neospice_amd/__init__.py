@@ -666,6 +666,13 @@ class Bundle:
     def set_graph(self, on=True):
         check(lib().nsa_bundle_set_graph(self.h, int(bool(on))))
 
+    def tick_stats(self):
+        """Device-controlled transient: ticks, host looks between batches,
+        host seconds enqueueing ticks and waiting for the device."""
+        t = np.zeros(4)
+        check(lib().nsa_bundle_tick_stats(self.h, f64p(t)))
+        return {"ticks": int(t[0]), "batches": int(t[1]), "enqueue_s": float(t[2]), "wait_s": float(t[3])}
+
     def set_device_control(self, on=True):
         """True (default): the transient's step controller runs on the device
         (every instance on its own step sequence, no host round trip per

@@ -112,6 +112,7 @@ _SIGS = {
     "nsa_bundle_timing": (C.c_int, [VP, P_F64]),
     "nsa_bundle_set_graph": (C.c_int, [VP, I32]),
     "nsa_bundle_set_device_control": (C.c_int, [VP, I32]),
+    "nsa_bundle_tick_stats": (C.c_int, [VP, P_F64]),
     "nsa_bundle_plan_info": (C.c_int, [VP, I32, P_I64]),
     "nsa_bundle_device_offsets": (C.c_int, [VP, P_I32]),
     "nsa_bundle_dc_adjoint": (C.c_int, [VP, P_F64, P_F64, P_F64, P_F64, P_F64]),

@@ -1,5 +1,7 @@
 #include "bundle.hpp"
 
+#include <chrono>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -524,12 +526,22 @@ void Bundle::set_timing(bool on) {
 
 // ---------------------------------------------------------------- iteration
 
+namespace {
+unsigned long long bits_1e30() {
+    const double big = 1e30;
+    unsigned long long b;
+    std::memcpy(&b, &big, sizeof(big));
+    return b;
+}
+}  // namespace
+
 void Bundle::launch_begin() {
     launch_nt_begin(d_nt_, NI_, d_ixoff_, d_in_, d_x_, d_xold_, d_ctx_, d_kind_, d_fail_, d_devnc_,
                     d_done_, stream_);
 }
 
-void Bundle::launch_loads() {
+// Loads of the families other than BSIM4v7.
+void Bundle::launch_loads_small() {
     double* s0 = d_s_[0];
     double* s1 = d_s_[1];
     double* s2 = d_s_[2];
@@ -538,8 +550,12 @@ void Bundle::launch_loads() {
     launch_isrc(isrc_, d_ctx_, d_done_, d_cr_, 1, stream_);
     launch_diode(dio_, d_ctx_, d_x_, s0, s1, s2, d_done_, d_cm_, d_cr_, d_devnc_, tol_, 1, stream_);
     launch_mos1(mos_, d_ctx_, d_x_, s0, s1, s2, d_done_, d_cm_, d_cr_, d_devnc_, tol_, 1, stream_);
+}
+
+// The BSIM4v7 load, then the matrix and rhs gathers.
+void Bundle::launch_loads_b4_gather() {
     if (timing_ && b4_.count) NSA_HIP(hipEventRecord(ev_[1], stream_));
-    launch_bsim4(b4_, d_ctx_, d_x_, s0, s1, s2, d_done_, d_cm_, d_cr_, d_devnc_, tol_, b4_lanes_,
+    launch_bsim4(b4_, d_ctx_, d_x_, d_s_[0], d_s_[1], d_s_[2], d_done_, d_cm_, d_cr_, d_devnc_, tol_, b4_lanes_,
                  stream_);
     if (timing_ && b4_.count) NSA_HIP(hipEventRecord(ev_[2], stream_));
     GatherPacked m = gm_, r = gr_;
@@ -550,16 +566,22 @@ void Bundle::launch_loads() {
     launch_gather_packed(m, r, stream_);
 }
 
+void Bundle::launch_loads() {
+    launch_loads_small();
+    launch_loads_b4_gather();
+}
+
 void Bundle::launch_lu(const int32_t* d_kind, int mode) {
     launch_small_lu(plans_, d_inst_, NI_, mode, d_A_, opt_.diag_gmin, d_kind, d_fail_, d_fail_col_,
                     d_LU_, d_rhs_, d_x_, stream_, d_gmin_);
 }
 
-void Bundle::launch_post(int) {
+void Bundle::launch_post(bool tick) {
     // MOS1Device::conv_test on the proposed solution (skipped where nothing was solved)
     launch_mos1_conv(mos_, d_x_, d_s_[0], d_fail_, d_devnc_, tol_, 1, stream_);
     launch_nt_post(d_nt_, NI_, d_ixoff_, d_in_, d_inn_, d_x_, d_xold_, d_ctx_, d_kind_, d_fail_,
-                   d_devnc_, d_done_, NtConvTol{opt_.reltol, opt_.vntol, opt_.abstol}, stream_);
+                   d_devnc_, d_done_, NtConvTol{opt_.reltol, opt_.vntol, opt_.abstol}, stream_,
+                   tick ? d_tc_ : nullptr, tick ? d_att_ : nullptr);
 }
 
 void Bundle::capture_graph() {
@@ -569,7 +591,7 @@ void Bundle::capture_graph() {
     launch_begin();
     launch_loads();
     launch_lu(d_kind_, SLU_FACTOR | SLU_SOLVE);
-    launch_post(0);
+    launch_post();
     NSA_HIP(hipStreamEndCapture(stream_, &graph_));
     NSA_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
     graph_words_ = plans_.d_words();
@@ -595,7 +617,7 @@ void Bundle::iteration_device() {
         NSA_HIP(hipEventRecord(ev_[3], stream_));
         launch_lu(d_kind_, SLU_FACTOR | SLU_SOLVE);
         NSA_HIP(hipEventRecord(ev_[4], stream_));
-        launch_post(0);
+        launch_post();
         NSA_HIP(hipEventRecord(ev_[5], stream_));
         timed_pending_ = true;
         return;
@@ -609,7 +631,7 @@ void Bundle::iteration_device() {
     launch_begin();
     launch_loads();
     launch_lu(d_kind_, SLU_FACTOR | SLU_SOLVE);
-    launch_post(0);
+    launch_post();
 }
 
 std::vector<double> Bundle::instance_matrix(int i) {
@@ -697,7 +719,7 @@ void Bundle::host_lu_post(std::vector<NtInst>& nt, const std::vector<int>& repiv
                                stream_));
     }
     launch_lu(d_kind_, SLU_FACTOR | SLU_SOLVE);
-    launch_post(0);
+    launch_post();
     NSA_HIP(hipStreamSynchronize(stream_));
     ++n_host_iterations;
 }
@@ -1007,10 +1029,14 @@ void Bundle::tran_begin(double tstep, double tstop, const int32_t* probe_ptr, co
     NSA_HIP(hipStreamSynchronize(stream_));
     for (void* p : tran_allocs_) NSA_HIP(hipFree(p));
     tran_allocs_.clear();
+    // the tick graph names the buffers freed above
+    if (tick_exec_) { NSA_HIP(hipGraphExecDestroy(tick_exec_)); tick_exec_ = nullptr; }
+    if (tick_graph_) { NSA_HIP(hipGraphDestroy(tick_graph_)); tick_graph_ = nullptr; }
     tcd_.clear();
     tc_on_device_ = false;
     log_on_device_ = false;
     d_log_ = nullptr;
+    d_spec_ = nullptr;
     log_rows_ = 0;
     // 1. DC operating point in MODETRANOP from zeros (transient.cpp:710-726, 195-205)
     std::vector<double> z(tot_n_, 0.0);
@@ -1057,6 +1083,7 @@ void Bundle::tran_begin(double tstep, double tstop, const int32_t* probe_ptr, co
     d_att_ = (int32_t*)tal(sizeof(int32_t) * NI_);
     d_prow_ = (int32_t*)tal(sizeof(int32_t) * NI_);
     d_tc_ = (TranDev*)tal(sizeof(TranDev) * NI_);
+    d_spec_ = (int32_t*)tal(sizeof(int32_t) * NI_);
     d_tcs_ = (TcStatus*)tal(sizeof(TcStatus));
     if (!h_acc_pin_) {
         NSA_HIP(hipHostMalloc((void**)&h_acc_pin_, sizeof(int32_t) * 3 * NI_, hipHostMallocDefault));
@@ -1270,40 +1297,54 @@ int Bundle::tran_advance_host(double t_until) {
     return rounds;
 }
 
-// The part of a tick after the Newton iteration: the instances whose Newton
-// solve ended take their step decision (truncation errors, ringing test,
-// tc_decide), then the accepted ones rotate states and history and the
-// others go back to their last accepted solution.
-void Bundle::launch_tick_tail() {
+// One transient tick.  Two launches carry independent work in extra
+// workgroups, on SIMDs their own one-wave-per-instance grids leave idle:
+// nt_begin carries the BSIM4v7 (lane-strided) state rotation the previous
+// tick's accepted steps owe (acc), and the packed LU carries the BSIM4v7
+// step proposals (compute_trunc) of the iterations that could end a
+// converged solve (spec).  After the LU come the other families'
+// proposals (att), the step decisions and their history (tc_step), and the
+// other families' rotation (acc).  With `timed`, HIP events bracket the
+// iteration's phases (loads, LU, end of iteration).
+void Bundle::launch_tick(bool timed) {
     const TruncOpts to{opt_.abstol, opt_.reltol, opt_.chgtol, opt_.trtol};
-    const double big = 1e30;
-    unsigned long long big_bits;
-    std::memcpy(&big_bits, &big, sizeof(big));
-    launch_tc_mid(d_tc_, d_nt_, d_att_, d_tmin_, big_bits, NI_, stream_);
+    const RotSide rs{b4_lanes_ > 0 ? (int)(ns_lane_ / b4_lanes_) : 0, b4_lanes_, b4_.inst_of, d_acc_, d_s_[0],
+                     d_s_[1], d_s_[2], d_s_[3]};
+    if (timed) NSA_HIP(hipEventRecord(ev_[0], stream_));
+    launch_nt_begin(d_nt_, NI_, d_ixoff_, d_in_, d_x_, d_xold_, d_ctx_, d_kind_, d_fail_, d_devnc_, d_done_,
+                    stream_, d_tmin_, bits_1e30(), d_spec_, &rs);
+    launch_loads_small();
+    launch_loads_b4_gather();
+    if (timed) NSA_HIP(hipEventRecord(ev_[3], stream_));
+    const SluSide side{trunc4_, d_ctx_, d_s_[0], d_s_[1], d_s_[2], d_s_[3], b4_lanes_, to, d_tmin_, NI_, d_spec_};
+    launch_small_lu(plans_, d_inst_, NI_, SLU_FACTOR | SLU_SOLVE, d_A_, opt_.diag_gmin, d_kind_, d_fail_,
+                    d_fail_col_, d_LU_, d_rhs_, d_x_, stream_, d_gmin_, &side);
+    if (timed) NSA_HIP(hipEventRecord(ev_[4], stream_));
+    launch_post(true);
+    if (timed) NSA_HIP(hipEventRecord(ev_[5], stream_));
     launch_trunc(trunc1_, d_ctx_, d_s_[0], d_s_[1], d_s_[2], d_s_[3], 1, to, nullptr, d_tmin_, stream_, 0,
                  d_tmin_ + NI_, d_att_);
-    launch_trunc(trunc4_, d_ctx_, d_s_[0], d_s_[1], d_s_[2], d_s_[3], b4_lanes_, to, nullptr, d_tmin_, stream_,
-                 0, d_tmin_ + NI_, d_att_);
-    launch_tr_ring(NI_, d_ixoff_, d_inn_, d_x_, d_h1_, d_h2_, opt_.reltol, opt_.vntol, d_att_, d_ring_,
-                   stream_);
-    launch_tc_post(d_tc_, d_bpt_, d_bph_, d_nt_, d_ctx_, d_att_, d_tmin_, d_ring_, d_acc_, d_prow_, d_log_,
-                   log_rows_, NI_, stream_);
-    launch_tr_states(ns_tot_, ns_lane_, b4_lanes_, b4_.inst_of, d_slot_inst_, d_acc_, d_s_[0], d_s_[1],
-                     d_s_[2], d_s_[3], stream_);
-    launch_tr_hist(NI_, d_ixoff_, d_in_, d_x_, d_h1_, d_h2_, d_h3_, d_acc_, d_pptr_, d_probe_, d_prow_,
+    launch_tc_step(d_tc_, d_bpt_, d_bph_, d_nt_, d_ctx_, d_att_, d_tmin_, NI_, d_ixoff_, d_in_, d_inn_, d_x_,
+                   d_h1_, d_h2_, d_h3_, opt_.reltol, opt_.vntol, d_acc_, d_log_, log_rows_, d_pptr_, d_probe_,
                    d_pvals_, max_points_, stream_);
+    launch_tr_states(ns_tot_, ns_lane_, b4_lanes_, b4_.inst_of, d_slot_inst_, d_acc_, d_s_[0], d_s_[1],
+                     d_s_[2], d_s_[3], stream_, ns_lane_);
+}
+
+// The BSIM4v7 rotations the last tick's accepted steps still owe, so the
+// states are current when control returns to the caller; acc is cleared
+// so that the next tick owes none.
+void Bundle::flush_rotations() {
+    launch_tr_states(ns_tot_, ns_lane_, b4_lanes_, b4_.inst_of, d_slot_inst_, d_acc_, d_s_[0], d_s_[1], d_s_[2],
+                     d_s_[3], stream_, 0, ns_lane_);
+    launch_fill_i32(d_acc_, 0, NI_, stream_);
 }
 
 void Bundle::capture_tick_graph() {
     if (tick_exec_) { NSA_HIP(hipGraphExecDestroy(tick_exec_)); tick_exec_ = nullptr; }
     if (tick_graph_) { NSA_HIP(hipGraphDestroy(tick_graph_)); tick_graph_ = nullptr; }
     NSA_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-    launch_tc_pre(d_tc_, d_bpt_, d_nt_, d_ctx_, NI_, stream_);
-    launch_begin();
-    launch_loads();
-    launch_lu(d_kind_, SLU_FACTOR | SLU_SOLVE);
-    launch_post(0);
-    launch_tick_tail();
+    launch_tick(false);
     NSA_HIP(hipStreamEndCapture(stream_, &tick_graph_));
     NSA_HIP(hipGraphInstantiate(&tick_exec_, tick_graph_, nullptr, nullptr, 0));
     tick_words_ = plans_.d_words();
@@ -1313,35 +1354,21 @@ void Bundle::capture_tick_graph() {
     tick_log_ = d_log_;
 }
 
-// One tick: every instance at its loop head sets up its next step, every
-// instance in a Newton solve runs one iteration, and every instance whose
-// solve ended takes its step decision.
+// One tick: every instance in a Newton solve runs one iteration, and every
+// instance whose solve ended takes its step decision and sets up its next
+// step.
 void Bundle::tick_device() {
     if (inst_dirty_) upload_inst();
     if (timing_) {
-        launch_tc_pre(d_tc_, d_bpt_, d_nt_, d_ctx_, NI_, stream_);
-        NSA_HIP(hipEventRecord(ev_[0], stream_));
-        launch_begin();
-        launch_loads();
-        NSA_HIP(hipEventRecord(ev_[3], stream_));
-        launch_lu(d_kind_, SLU_FACTOR | SLU_SOLVE);
-        NSA_HIP(hipEventRecord(ev_[4], stream_));
-        launch_post(0);
-        NSA_HIP(hipEventRecord(ev_[5], stream_));
+        launch_tick(true);
         timed_pending_ = true;
-        launch_tick_tail();
     } else if (use_graph) {
         if (!tick_exec_ || graph_stale(tick_words_, tick_nlu_, tick_n_, tick_hot_) || tick_log_ != d_log_)
             capture_tick_graph();
         NSA_HIP(hipGraphLaunch(tick_exec_, stream_));
         ++n_graph_launches;
     } else {
-        launch_tc_pre(d_tc_, d_bpt_, d_nt_, d_ctx_, NI_, stream_);
-        launch_begin();
-        launch_loads();
-        launch_lu(d_kind_, SLU_FACTOR | SLU_SOLVE);
-        launch_post(0);
-        launch_tick_tail();
+        launch_tick(false);
     }
     ++n_ticks;
 }
@@ -1403,19 +1430,24 @@ int Bundle::tran_advance_device(double t_until) {
         log_on_device_ = true;
     }
     if (!tc_on_device_) {
+        // ticks rotate the BSIM4v7 states the previous tick's acc names: none
+        // are owed on entry (host control rotates at once)
+        launch_fill_i32(d_acc_, 0, NI_, stream_);
         NSA_HIP(hipMemcpyAsync(d_tc_, tcd_.data(), sizeof(TranDev) * NI_, hipMemcpyHostToDevice, stream_));
-        NSA_HIP(hipMemcpyAsync(d_prow_, prow_.data(), sizeof(int32_t) * NI_, hipMemcpyHostToDevice, stream_));
         NSA_HIP(hipStreamSynchronize(stream_));
         tc_on_device_ = true;
     }
     int64_t it0 = 0;
     for (const auto& c : tcd_) it0 += c.all_iters;
-    launch_tc_resume(d_tc_, NI_, t_until, stream_);
+    launch_tc_resume(d_tc_, d_bpt_, d_nt_, d_ctx_, NI_, t_until, stream_);
     int ticks = 0;
     for (;;) {
         launch_tc_status(d_tc_, d_nt_, NI_, d_tcs_, stream_);
         NSA_HIP(hipMemcpyAsync(h_tcs_pin_, d_tcs_, sizeof(TcStatus), hipMemcpyDeviceToHost, stream_));
+        const auto tw = std::chrono::steady_clock::now();
         NSA_HIP(hipStreamSynchronize(stream_));
+        tick_wait_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - tw).count();
+        ++n_tick_batches;
         collect_timing();
         const TcStatus st = *h_tcs_pin_;
         if (st.stalled > 0) {
@@ -1430,12 +1462,14 @@ int Bundle::tran_advance_device(double t_until) {
         if (timing_ || std::find(plan_.begin(), plan_.end(), -1) != plan_.end()) batch = 1;
         // a tick accepts at most one step per instance
         tran_log_reserve(st.max_steps + batch + 1);
+        const auto te = std::chrono::steady_clock::now();
         for (int k = 0; k < batch; ++k) tick_device();
+        tick_enqueue_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - te).count();
         ticks += batch;
     }
+    flush_rotations();
     NSA_HIP(hipMemcpyAsync(tcd_.data(), d_tc_, sizeof(TranDev) * NI_, hipMemcpyDeviceToHost, stream_));
     NSA_HIP(hipMemcpyAsync(h_ctx_pin_, d_ctx_, sizeof(DevCtx) * NI_, hipMemcpyDeviceToHost, stream_));
-    NSA_HIP(hipMemcpyAsync(prow_.data(), d_prow_, sizeof(int32_t) * NI_, hipMemcpyDeviceToHost, stream_));
     NSA_HIP(hipStreamSynchronize(stream_));
     std::memcpy(h_ctx_.data(), h_ctx_pin_, sizeof(DevCtx) * NI_);
     int64_t it1 = 0;

@@ -146,6 +146,9 @@ public:
     // ---- statistics
     long n_cpu_pivot = 0, n_iterations = 0, n_graph_launches = 0, n_host_iterations = 0;
     long n_ticks = 0;  // transient ticks run under device control
+    long n_tick_batches = 0;     // host looks between batches of ticks
+    double tick_enqueue_s = 0.0;  // host time spent enqueueing ticks
+    double tick_wait_s = 0.0;     // host time spent waiting for the device between batches
     int num_plans() const { return plans_.count(); }
     const SmallLuPlans& plans() const { return plans_; }
     int plan_of(int i) const { return plan_[i]; }
@@ -159,8 +162,10 @@ private:
     void build_tables();
     void launch_begin();
     void launch_loads();
+    void launch_loads_small();
+    void launch_loads_b4_gather();
     void launch_lu(const int32_t* d_kind, int mode);
-    void launch_post(int iter_base);
+    void launch_post(bool tick = false);
     void iteration_device();
     void capture_graph();
     void host_lu_post(std::vector<NtInst>& nt, const std::vector<int>& repivot, bool others);
@@ -173,7 +178,8 @@ private:
     void refresh_size_derivs();
     int tran_advance_host(double t_until);
     int tran_advance_device(double t_until);
-    void launch_tick_tail();
+    void launch_tick(bool timed);
+    void flush_rotations();
     void tick_device();
     void capture_tick_graph();
     bool graph_stale(const int32_t* words, int nlu, int n, int hot) const;
@@ -258,6 +264,8 @@ private:
     // step logs
     std::vector<TranDev> tcd_;
     TranDev* d_tc_ = nullptr;
+    int32_t* d_spec_ = nullptr;                 // per instance: this iteration could end a converged solve
+
     bool tc_on_device_ = false;                 // d_tc_ holds the current controllers
     std::vector<double> bpt_;                   // breakpoint times, packed per instance
     std::vector<int32_t> bph_;                  // their HARD flags

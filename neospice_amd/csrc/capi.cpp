@@ -682,6 +682,16 @@ int nsa_bundle_timing(nsa_bundle* b, double out[5]) {
 int nsa_bundle_set_graph(nsa_bundle* b, int32_t on) {
     return guard([&] { b->b->use_graph = on != 0; return 0; });
 }
+int nsa_bundle_tick_stats(nsa_bundle* b, double out[4]) {
+    return guard([&] {
+        const nsa::Bundle& x = *b->b;
+        out[0] = (double)x.n_ticks;
+        out[1] = (double)x.n_tick_batches;
+        out[2] = x.tick_enqueue_s;
+        out[3] = x.tick_wait_s;
+        return 0;
+    });
+}
 int nsa_bundle_set_device_control(nsa_bundle* b, int32_t on) {
     return guard([&] { b->b->device_control = on != 0; return 0; });
 }

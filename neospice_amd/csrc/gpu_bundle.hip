@@ -1,5 +1,6 @@
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "bundle.hpp"
@@ -12,20 +13,60 @@ namespace {
 
 constexpr int kW = 64;
 
+// newton_solve's locals at entry, in the caller's mode (newton.cpp:117-131)
+__device__ inline NtInst fresh_newton(int mode, int itl) {
+    NtInst s{};
+    s.saved_mode = mode;
+    s.itl = itl;
+    s.kind = SLU_SKIP;
+    return s;
+}
+
 __global__ __launch_bounds__(kW) void k_nt_begin(NtInst* __restrict__ nt, const int32_t* __restrict__ xoff,
                                                 const int32_t* __restrict__ nn,
                                                 const double* __restrict__ x,
                                                 double* __restrict__ xold,
-                                                const DevCtx* __restrict__ ctx,
+                                                DevCtx* __restrict__ ctx,
                                                 int32_t* __restrict__ kind, int32_t* __restrict__ fail,
-                                                int32_t* __restrict__ devnc, int32_t* __restrict__ done) {
+                                                int32_t* __restrict__ devnc, int32_t* __restrict__ done,
+                                                unsigned long long* __restrict__ tmin, unsigned long long big,
+                                                int32_t* __restrict__ spec, int ni, RotSide rs) {
     const int i = blockIdx.x, lane = threadIdx.x;
+    if (i >= ni) {
+        // a side workgroup: the lane-strided state rotation the previous
+        // tick's accepted steps owe (rotate_state, circuit.cpp:328-334), one
+        // thread per lane over its slots g, g + lanes, ... (all loads issued
+        // before the stores)
+        const long g = (long)(i - ni) * kW + lane;
+        if (g >= rs.lanes || rs.acc[rs.lane_inst[g]] != 1) return;
+        constexpr int kU = 8;
+        for (int j0 = 0; j0 < rs.slots; j0 += kU) {
+            double v0[kU], v1[kU], v2[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const long k = (long)min(j0 + u, rs.slots - 1) * rs.lanes + g;
+                v0[u] = rs.s0[k];
+                v1[u] = rs.s1[k];
+                v2[u] = rs.s2[k];
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                if (j0 + u >= rs.slots) break;
+                const long k = (long)(j0 + u) * rs.lanes + g;
+                rs.s3[k] = v2[u];
+                rs.s2[k] = v1[u];
+                rs.s1[k] = v0[u];
+            }
+        }
+        return;
+    }
     NtInst s = nt[i];
     if (s.done || s.stalled) {
         if (lane == 0) {
             kind[i] = SLU_SKIP;
             fail[i] = -1;
             done[i] = 1;
+            if (spec) spec[i] = 0;
         }
         return;
     }
@@ -41,6 +82,15 @@ __global__ __launch_bounds__(kW) void k_nt_begin(NtInst* __restrict__ nt, const 
         fail[i] = 0;
         done[i] = 0;
         nt[i] = s;
+        ctx[i].iter = s.iter;
+        if (tmin) {
+            // the truncation minima of this iteration's loads
+            tmin[i] = big;
+            tmin[ni + i] = big;
+        }
+        // iteration 0 never ends a converged solve (newton.cpp:318-319):
+        // the devices' step proposals are needed from iteration 1 on
+        if (spec) spec[i] = s.iter >= 1 ? 1 : 0;
     }
 }
 
@@ -53,11 +103,21 @@ __global__ __launch_bounds__(kW) void k_nt_post(NtInst* __restrict__ nt, const i
                                                const int32_t* __restrict__ kind,
                                                const int32_t* __restrict__ fail,
                                                const int32_t* __restrict__ devnc,
-                                               int32_t* __restrict__ done, NtConvTol tol) {
+                                               int32_t* __restrict__ done, NtConvTol tol,
+                                               const TranDev* __restrict__ tc, int32_t* __restrict__ att) {
     const int i = blockIdx.x, lane = threadIdx.x;
     NtInst s = nt[i];
     const int kd = kind[i];
-    if (s.done || s.stalled || kd == SLU_SKIP) return;
+    // the transient's tick: instances whose Newton solve has ended take their
+    // step decision (att)
+    auto tick_end = [&](const NtInst& sf) {
+        if (!tc || lane != 0) return;
+        att[i] = (tc[i].phase == TP_NEWTON && sf.done) ? 1 : 0;
+    };
+    if (s.done || s.stalled || kd == SLU_SKIP) {
+        tick_end(s);
+        return;
+    }
     const int f = fail[i];
     const int iter = s.iter;
     int mode = ctx[i].mode;
@@ -66,6 +126,7 @@ __global__ __launch_bounds__(kW) void k_nt_post(NtInst* __restrict__ nt, const i
         // a fresh solver's first factorization: the host orders this instance
         s.stalled = 1;
         if (lane == 0) nt[i] = s;
+        tick_end(s);
         return;
     }
     if (f != FAIL_NONE) {
@@ -77,6 +138,7 @@ __global__ __launch_bounds__(kW) void k_nt_post(NtInst* __restrict__ nt, const i
             // the order no longer holds: the host re-pivots this instance
             s.stalled = 1;
             if (lane == 0) nt[i] = s;
+            tick_end(s);
             return;
         } else {
             // singular numeric(): this Newton attempt fails (newton.cpp:220-224)
@@ -141,6 +203,7 @@ __global__ __launch_bounds__(kW) void k_nt_post(NtInst* __restrict__ nt, const i
         ctx[i].mode = mode;
         done[i] = s.done;
     }
+    tick_end(s);
 }
 
 __global__ __launch_bounds__(kW) void k_tr_ring(const int32_t* __restrict__ xoff,
@@ -205,11 +268,11 @@ __global__ __launch_bounds__(kW) void k_tr_hist(const int32_t* __restrict__ xoff
     }
 }
 
-__global__ void k_tr_states(long ns, long ns_lane, int lanes, const int32_t* __restrict__ lane_inst,
+__global__ void k_tr_states(long start, long ns, long ns_lane, int lanes, const int32_t* __restrict__ lane_inst,
                             const int32_t* __restrict__ slot_inst, const int32_t* __restrict__ acc,
                             const double* __restrict__ s0, double* __restrict__ s1,
                             double* __restrict__ s2, double* __restrict__ s3) {
-    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long k = start + (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ns) return;
     const int inst = (k < ns_lane) ? lane_inst[k % lanes] : slot_inst[k - ns_lane];
     if (acc[inst] != 1) return;
@@ -235,74 +298,97 @@ __global__ void k_copy_f64(double* __restrict__ dst, const double* __restrict__ 
 
 // ---- the transient's tick (one thread per instance)
 
-__global__ void k_tc_pre(TranDev* __restrict__ tc, const double* __restrict__ bpt, NtInst* __restrict__ nt,
-                         DevCtx* __restrict__ ctx, int ni) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ni) return;
-    TranDev c = tc[i];
-    if (c.phase != TP_READY) return;
-    DevCtx x;
-    if (tc_attempt(c, bpt, x)) {
-        ctx[i] = x;
-        NtInst s{};
-        s.saved_mode = x.mode;
-        s.done = 0;
-        s.itl = c.itl;
-        s.kind = SLU_SKIP;
-        nt[i] = s;
-        c.phase = TP_NEWTON;
-    } else {
-        c.phase = TP_IDLE;
-    }
-    tc[i] = c;
-}
-
-__global__ void k_tc_mid(const TranDev* __restrict__ tc, const NtInst* __restrict__ nt, int32_t* __restrict__ att,
-                         unsigned long long* __restrict__ tmin, unsigned long long big, int ni) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ni) return;
-    const bool fin = tc[i].phase == TP_NEWTON && nt[i].done;
-    att[i] = fin ? 1 : 0;
-    if (fin) {
-        tmin[i] = big;
-        tmin[ni + i] = big;
-    }
-}
-
-__global__ void k_tc_post(TranDev* __restrict__ tc, const double* __restrict__ bpt,
-                          const int32_t* __restrict__ bph, const NtInst* __restrict__ nt,
-                          const DevCtx* __restrict__ ctx, const int32_t* __restrict__ att,
-                          const unsigned long long* __restrict__ tmin, const int32_t* __restrict__ ring,
-                          int32_t* __restrict__ acc, int32_t* __restrict__ prow, TranStep* __restrict__ log,
-                          int max_log, int ni) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ni) return;
+// check_ringing (as k_tr_ring), the step decision (tc_decide) and the
+// history rotation / restore (as k_tr_hist) of one instance whose Newton
+// solve ended in this tick.
+__global__ __launch_bounds__(kW) void k_tc_step(TranDev* __restrict__ tc, const double* __restrict__ bpt,
+                                               const int32_t* __restrict__ bph, NtInst* __restrict__ nt,
+                                               DevCtx* __restrict__ ctx, const int32_t* __restrict__ att,
+                                               const unsigned long long* __restrict__ tmin,
+                                               const int32_t* __restrict__ xoff, const int32_t* __restrict__ nn,
+                                               const int32_t* __restrict__ nnodes, double* __restrict__ x,
+                                               double* __restrict__ h1, double* __restrict__ h2,
+                                               double* __restrict__ h3, double reltol, double vntol,
+                                               int32_t* __restrict__ acc, TranStep* __restrict__ log, int max_log,
+                                               const int32_t* __restrict__ pptr, const int32_t* __restrict__ probe,
+                                               double* __restrict__ pvals, int max_points) {
+    const int i = blockIdx.x, lane = threadIdx.x, ni = gridDim.x;
     if (!att[i]) {
-        acc[i] = 0;
+        if (lane == 0) acc[i] = 0;
         return;
     }
+    const int o = xoff[i], n = nn[i], nnod = nnodes[i];
+    // ringing on the post-solve buffers: sol = sol_prev = x, sol_prev2 = h1,
+    // sol_prev3 = h2 (timestep.cpp:191-229, transient.cpp:1039-1048)
+    bool found = false;
+    for (int k = lane; k < nnod; k += kW) {
+        const double sol = x[o + k], prev = x[o + k], prev2 = h1[o + k], prev3 = h2[o + k];
+        const double d2c = sol - 2.0 * prev + prev2;
+        const double d2p = prev - 2.0 * prev2 + prev3;
+        const double t = reltol * fabs(sol) + vntol;
+        if (d2c * d2p < 0.0 && fabs(d2c) > t && fabs(d2p) > t) {
+            const double d1c = sol - prev, d1p = prev - prev2;
+            double d1max = fmax(fabs(d1c), fabs(d1p));
+            if (d1max < 1e-30) d1max = 1e-30;
+            const double ratio = fmax(fabs(d2c), fabs(d2p)) / d1max;
+            if (ratio > 0.5) found = true;
+        }
+    }
+    const int ring = __any(found) ? 1 : 0;
     TranDev c = tc[i];
     const NtInst s = nt[i];
-    const DevCtx x = ctx[i];
+    const DevCtx xc = ctx[i];
     TranStep rec{};
     const int a = tc_decide(c, bpt, bph, s.converged, s.iter, __longlong_as_double((long long)tmin[i]),
-                            __longlong_as_double((long long)tmin[ni + i]), ring[i], x.order, x.method, rec);
-    if (a == 1) {
-        const int row = c.step_count - 1;
-        if (row < max_log) log[(size_t)i * max_log + row] = rec;
-        prow[i] = c.step_count;
+                            __longlong_as_double((long long)tmin[ni + i]), ring, xc.order, xc.method, rec);
+    const int step_row = c.step_count;
+    // the loop head of the instance's next step (every lane takes the same
+    // decisions; lane 0 writes them)
+    DevCtx xn;
+    const bool next = tc_attempt(c, bpt, xn);
+    c.phase = next ? TP_NEWTON : TP_IDLE;
+    if (lane == 0) {
+        if (a == 1 && step_row - 1 < max_log) log[(size_t)i * max_log + step_row - 1] = rec;
+        acc[i] = a;
+        if (next) {
+            ctx[i] = xn;
+            nt[i] = fresh_newton(xn.mode, c.itl);
+        }
+        tc[i] = c;
     }
-    acc[i] = a;
-    c.phase = TP_READY;
-    tc[i] = c;
+    if (a == 1) {
+        for (int k = lane; k < n; k += kW) {
+            const double v = x[o + k];
+            h3[o + k] = h2[o + k];
+            h2[o + k] = h1[o + k];
+            h1[o + k] = v;
+        }
+        const int r = step_row;
+        if (pvals && r < max_points) {
+            const int p0 = pptr[i], np = pptr[i + 1] - p0;
+            for (int q = lane; q < np; q += kW)
+                pvals[(size_t)p0 * max_points + (size_t)r * np + q] = x[o + probe[p0 + q]];
+        }
+    } else {
+        for (int k = lane; k < n; k += kW) x[o + k] = h1[o + k];
+    }
 }
 
-__global__ void k_tc_resume(TranDev* __restrict__ tc, int ni, double t_until) {
+__global__ void k_tc_resume(TranDev* __restrict__ tc, const double* __restrict__ bpt, NtInst* __restrict__ nt,
+                            DevCtx* __restrict__ ctx, int ni, double t_until) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ni) return;
     TranDev c = tc[i];
     c.t_until = t_until;
-    if (!c.finished && c.phase == TP_IDLE) c.phase = TP_READY;
+    if (!c.finished && c.phase == TP_IDLE) {
+        // the loop head of the instance's next step
+        DevCtx xn;
+        if (tc_attempt(c, bpt, xn)) {
+            ctx[i] = xn;
+            nt[i] = fresh_newton(xn.mode, c.itl);
+            c.phase = TP_NEWTON;
+        }
+    }
     tc[i] = c;
 }
 
@@ -349,20 +435,27 @@ __global__ __launch_bounds__(256) void k_tc_status(const TranDev* __restrict__ t
 }  // namespace
 
 void launch_nt_begin(NtInst* nt, int ni, const int32_t* xoff, const int32_t* n, const double* x,
-                     double* xold, const DevCtx* ctx, int32_t* kind, int32_t* fail,
-                     int32_t* devnc, int32_t* done, hipStream_t s) {
+                     double* xold, DevCtx* ctx, int32_t* kind, int32_t* fail,
+                     int32_t* devnc, int32_t* done, hipStream_t s, unsigned long long* tmin,
+                     unsigned long long big, int32_t* spec, const RotSide* rot) {
     if (ni <= 0) return;
-    hipLaunchKernelGGL(k_nt_begin, dim3(ni), dim3(kW), 0, s, nt, xoff, n, x, xold, ctx, kind, fail,
-                       devnc, done);
+    RotSide rs{};
+    int extra = 0;
+    if (rot && rot->slots > 0 && rot->lanes > 0) {
+        rs = *rot;
+        extra = (rs.lanes + kW - 1) / kW;
+    }
+    hipLaunchKernelGGL(k_nt_begin, dim3(ni + extra), dim3(kW), 0, s, nt, xoff, n, x, xold, ctx, kind, fail,
+                       devnc, done, tmin, big, spec, ni, rs);
 }
 
 void launch_nt_post(NtInst* nt, int ni, const int32_t* xoff, const int32_t* n,
                     const int32_t* num_nodes, double* x, const double* xold, DevCtx* ctx,
                     const int32_t* kind, const int32_t* fail, const int32_t* devnc, int32_t* done,
-                    NtConvTol tol, hipStream_t s) {
+                    NtConvTol tol, hipStream_t s, const TranDev* tc, int32_t* att) {
     if (ni <= 0) return;
     hipLaunchKernelGGL(k_nt_post, dim3(ni), dim3(kW), 0, s, nt, xoff, n, num_nodes, x, xold, ctx,
-                       kind, fail, devnc, done, tol);
+                       kind, fail, devnc, done, tol, tc, att);
 }
 
 void launch_tr_ring(int ni, const int32_t* xoff, const int32_t* num_nodes, const double* x,
@@ -384,10 +477,11 @@ void launch_tr_hist(int ni, const int32_t* xoff, const int32_t* n, double* x, do
 
 void launch_tr_states(long ns, long ns_lane, int lanes, const int32_t* lane_inst,
                       const int32_t* slot_inst, const int32_t* acc, double* s0, double* s1,
-                      double* s2, double* s3, hipStream_t s) {
-    if (ns <= 0) return;
+                      double* s2, double* s3, hipStream_t s, long start, long end) {
+    if (end < 0) end = ns;
+    if (end <= start) return;
     const int tpb = 256;
-    hipLaunchKernelGGL(k_tr_states, dim3((unsigned)((ns + tpb - 1) / tpb)), dim3(tpb), 0, s, ns,
+    hipLaunchKernelGGL(k_tr_states, dim3((unsigned)((end - start + tpb - 1) / tpb)), dim3(tpb), 0, s, start, end,
                        ns_lane, lanes > 0 ? lanes : 1, lane_inst, slot_inst, acc, s0, s1, s2, s3);
 }
 
@@ -400,29 +494,21 @@ void launch_masked_copy(long ns, long ns_lane, int lanes, const int32_t* lane_in
                        lanes > 0 ? lanes : 1, lane_inst, slot_inst, mask, src, dst, src ? 0 : 1);
 }
 
-void launch_tc_pre(TranDev* tc, const double* bpt, NtInst* nt, DevCtx* ctx, int ni, hipStream_t s) {
+void launch_tc_step(TranDev* tc, const double* bpt, const int32_t* bph, NtInst* nt,
+                    DevCtx* ctx, const int32_t* att, const unsigned long long* tmin, int ni,
+                    const int32_t* xoff, const int32_t* n, const int32_t* num_nodes, double* x, double* h1,
+                    double* h2, double* h3, double reltol, double vntol, int32_t* acc, TranStep* log,
+                    int max_log, const int32_t* pptr, const int32_t* probe, double* pvals, int max_points,
+                    hipStream_t s) {
     if (ni <= 0) return;
-    hipLaunchKernelGGL(k_tc_pre, dim3((ni + kW - 1) / kW), dim3(kW), 0, s, tc, bpt, nt, ctx, ni);
+    hipLaunchKernelGGL(k_tc_step, dim3(ni), dim3(kW), 0, s, tc, bpt, bph, nt, ctx, att, tmin, xoff, n, num_nodes,
+                       x, h1, h2, h3, reltol, vntol, acc, log, max_log, pptr, probe, pvals, max_points);
 }
 
-void launch_tc_mid(const TranDev* tc, const NtInst* nt, int32_t* att, unsigned long long* tmin,
-                   unsigned long long big, int ni, hipStream_t s) {
+void launch_tc_resume(TranDev* tc, const double* bpt, NtInst* nt, DevCtx* ctx, int ni, double t_until,
+                      hipStream_t s) {
     if (ni <= 0) return;
-    hipLaunchKernelGGL(k_tc_mid, dim3((ni + kW - 1) / kW), dim3(kW), 0, s, tc, nt, att, tmin, big, ni);
-}
-
-void launch_tc_post(TranDev* tc, const double* bpt, const int32_t* bph, const NtInst* nt,
-                    const DevCtx* ctx, const int32_t* att, const unsigned long long* tmin,
-                    const int32_t* ring, int32_t* acc, int32_t* prow, TranStep* log, int max_log,
-                    int ni, hipStream_t s) {
-    if (ni <= 0) return;
-    hipLaunchKernelGGL(k_tc_post, dim3((ni + kW - 1) / kW), dim3(kW), 0, s, tc, bpt, bph, nt, ctx, att, tmin,
-                       ring, acc, prow, log, max_log, ni);
-}
-
-void launch_tc_resume(TranDev* tc, int ni, double t_until, hipStream_t s) {
-    if (ni <= 0) return;
-    hipLaunchKernelGGL(k_tc_resume, dim3((ni + kW - 1) / kW), dim3(kW), 0, s, tc, ni, t_until);
+    hipLaunchKernelGGL(k_tc_resume, dim3((ni + kW - 1) / kW), dim3(kW), 0, s, tc, bpt, nt, ctx, ni, t_until);
 }
 
 void launch_tc_status(const TranDev* tc, const NtInst* nt, int ni, TcStatus* out, hipStream_t s) {

@@ -14,7 +14,7 @@ struct DevCtx {
     int32_t mode;
     int32_t order;
     int32_t method;
-    int32_t pad;
+    int32_t iter;     // Newton iteration of the current load (0: the solve's first)
     double ag0, ag1, ag2;
     double xmu_ratio;
     double time;      // current_time of the step being solved

@@ -23,6 +23,7 @@
 
 #include "common.hpp"
 #include "small_lu.hpp"
+#include "dev_terr.hpp"
 
 namespace nsa {
 
@@ -311,10 +312,18 @@ __global__ __launch_bounds__(kW) void k_small_lu(const int32_t* __restrict__ wor
                                                 const int32_t* __restrict__ kind,
                                                 int32_t* __restrict__ fail,
                                                 int32_t* __restrict__ fail_col, double* LU,
-                                                const double* rhs, double* x, int max_nlu, int max_n) {
+                                                const double* rhs, double* x, int max_nlu, int max_n,
+                                                int ninst, SluSide side) {
     extern __shared__ double lds[];
     const int lane = threadIdx.x;
     const int ii = blockIdx.x;
+    if (ii >= ninst) {
+        // a side workgroup: truncation proposals that run beside the
+        // instances' factor-and-solve chains on otherwise idle SIMDs
+        trunc_lane(side.t, side.ctx, side.s0, side.s1, side.s2, side.s3, side.B, side.o, nullptr, side.tmin, 0,
+                   side.tmin + side.ni, side.sel, (ii - ninst) * kW + lane);
+        return;
+    }
     const int kd = kind ? kind[ii] : SLU_REFACTOR;
     if (kd == SLU_SKIP) return;
     const SluInst d = insts[ii];
@@ -498,8 +507,15 @@ __global__ __launch_bounds__(kW) void k_small_lu(const int32_t* __restrict__ wor
 void launch_small_lu(const SmallLuPlans& plans, const SluInst* d_inst, int ninst, int mode,
                      const double* A, double diag_gmin, const int32_t* kind, int32_t* fail,
                      int32_t* fail_col, double* LU, const double* rhs, double* x, hipStream_t s,
-                     const double* gmin_inst) {
+                     const double* gmin_inst, const SluSide* side) {
     if (ninst <= 0) return;
+    SluSide sd{};
+    int side_blocks = 0;
+    if (side && side->t.ndev > 0) {
+        sd = *side;
+        side_blocks = (int)(((long)sd.t.ndev * sd.B + kW - 1) / kW);
+    }
+    const unsigned grid = (unsigned)(ninst + side_blocks);
     const int max_nlu = plans.max_nlu(), max_n = plans.max_n();
     // the hot block starts on a 16-byte boundary after the values, the
     // vector, the zero and the reciprocal pivots
@@ -508,13 +524,13 @@ void launch_small_lu(const SmallLuPlans& plans, const SluInst* d_inst, int ninst
     const int stage = (base + hot <= kLdsLimit) ? 1 : 0;
     const size_t lds = base + (stage ? hot : 0);
     if (stage)
-        hipLaunchKernelGGL(k_small_lu<true>, dim3(ninst), dim3(kW), lds, s, plans.d_words(),
+        hipLaunchKernelGGL(k_small_lu<true>, dim3(grid), dim3(kW), lds, s, plans.d_words(),
                            plans.d_offsets(), d_inst, mode, A, diag_gmin, gmin_inst, kind, fail, fail_col,
-                           LU, rhs, x, max_nlu, max_n);
+                           LU, rhs, x, max_nlu, max_n, ninst, sd);
     else
-        hipLaunchKernelGGL(k_small_lu<false>, dim3(ninst), dim3(kW), lds, s, plans.d_words(),
+        hipLaunchKernelGGL(k_small_lu<false>, dim3(grid), dim3(kW), lds, s, plans.d_words(),
                            plans.d_offsets(), d_inst, mode, A, diag_gmin, gmin_inst, kind, fail, fail_col,
-                           LU, rhs, x, max_nlu, max_n);
+                           LU, rhs, x, max_nlu, max_n, ninst, sd);
     NSA_HIP(hipGetLastError());
 }
 

@@ -28,6 +28,7 @@
 #include <cstdint>
 #include <vector>
 
+#include "gpu_dev.hpp"
 #include "lu_structure.hpp"
 
 namespace nsa {
@@ -123,14 +124,30 @@ private:
     size_t d_words_cap_ = 0, d_off_cap_ = 0, synced_words_ = 0, synced_plans_ = 0;
 };
 
+// Independent work a packed launch may carry in extra workgroups: the
+// truncation proposals of a device table (as launch_trunc with dt_min =
+// tmin, dt_min2 = tmin + ni and the mask sel), which then run beside the
+// instances' factor-and-solve chains, whose one wave each leaves most
+// SIMDs idle.
+struct SluSide {
+    TruncTable t;
+    const DevCtx* ctx;
+    const double *s0, *s1, *s2, *s3;
+    int B;
+    TruncOpts o;
+    unsigned long long* tmin;
+    int ni;
+    const int32_t* sel;
+};
+
 // One packed launch: factor (per-instance kinds, fail codes and first
 // failing column), solve (rhs -> x, may alias) or transposed solve.  A, LU,
 // rhs, x are packed device arrays addressed through `inst`.  diag_gmin is
 // added to every pivot's diagonal, or gmin_inst[i] per instance when given.
-// Asynchronous.
+// side (optional): extra work carried by the same launch.  Asynchronous.
 void launch_small_lu(const SmallLuPlans& plans, const SluInst* d_inst, int ninst, int mode,
                      const double* A, double diag_gmin, const int32_t* kind, int32_t* fail,
                      int32_t* fail_col, double* LU, const double* rhs, double* x, hipStream_t s,
-                     const double* gmin_inst = nullptr);
+                     const double* gmin_inst = nullptr, const SluSide* side = nullptr);
 
 }  // namespace nsa
