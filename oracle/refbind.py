@@ -14,12 +14,15 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_SO = os.path.join(HERE, "_ref", "libneospice_ref.so")
+# the same reference with neospice_amd behind its solver seam (solver_wrap.cpp)
+REF_GPU_SO = os.path.join(HERE, "_ref", "libneospice_ref_gpu.so")
 ORC_SO = os.path.join(HERE, "_ref", "liboracle.so")
 
 P_I32 = C.POINTER(C.c_int32)
 P_F64 = C.POINTER(C.c_double)
 VP = C.c_void_p
 _ref = None
+_ref_gpu = None
 _orc = None
 
 
@@ -30,69 +33,91 @@ def available():
 def ref():
     global _ref
     if _ref is None:
-        L = C.CDLL(REF_SO)
-        sig = {
-            "ref_last_error": (C.c_char_p, []),
-            "ref_ckt_new": (VP, [C.c_int]),
-            "ref_ckt_free": (None, [VP]),
-            "ref_ckt_model": (C.c_int, [VP, C.c_char_p]),
-            "ref_ckt_r": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int, C.c_double]),
-            "ref_ckt_c": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int, C.c_double]),
-            "ref_ckt_v": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int, C.c_double]),
-            "ref_ckt_i": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int, C.c_double]),
-            "ref_ckt_v_pulse": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int] + [C.c_double] * 7),
-            "ref_ckt_v_pwl": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int, C.c_int, P_F64, P_F64]),
-            "ref_ckt_d": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int, C.c_char_p]),
-            "ref_ckt_m": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p,
-                                    C.c_double, C.c_double]),
-            "ref_bsim4_fields": (C.c_int, [VP, C.c_int, P_F64]),
-            "ref_ckt_devnoncon": (C.c_int, [VP, P_F64]),
-            "ref_ckt_ic": (C.c_int, [VP, C.c_int, C.c_double]),
-            "ref_ckt_option": (C.c_int, [VP, C.c_char_p, C.c_double]),
-            "ref_ckt_finalize": (C.c_int, [VP, C.c_int]),
-            "ref_ckt_num_vars": (C.c_int, [VP]),
-            "ref_ckt_num_nodes": (C.c_int, [VP]),
-            "ref_ckt_num_states": (C.c_int, [VP]),
-            "ref_ckt_nnz": (C.c_int, [VP]),
-            "ref_ckt_is_linear": (C.c_int, [VP]),
-            "ref_ckt_num_devices": (C.c_int, [VP]),
-            "ref_ckt_pattern": (None, [VP, P_I32, P_I32]),
-            "ref_ckt_load_order": (C.c_int, [VP, P_I32]),
-            "ref_ckt_branch_index": (C.c_int, [VP, C.c_int]),
-            "ref_ckt_dead_nodes": (C.c_int, [VP, P_I32]),
-            "ref_ckt_state": (P_F64, [VP, C.c_int]),
-            "ref_ckt_set_ctx": (None, [VP, C.c_int, C.c_int, C.c_int] + [C.c_double] * 7),
-            "ref_ckt_load": (C.c_int, [VP, P_F64, P_F64, P_F64]),
-            "ref_dc": (C.c_int, [VP, P_F64, P_F64]),
-            "ref_tran": (C.c_int, [VP, C.c_double, C.c_double, C.c_int, P_F64, P_F64, P_F64,
-                                   P_I32]),
-            "ref_tran_fixed": (C.c_int, [VP, C.c_double, C.c_int, P_F64, P_F64]),
-            "ref_tran_num_records": (C.c_int, [VP]),
-            "ref_tran_records": (None, [VP, P_I32, P_F64]),
-            "ref_newton": (C.c_int, [VP, P_F64, C.c_int]),
-            "ref_sens": (C.c_int, [VP, C.c_int, C.c_int, P_F64, C.c_char_p, C.c_int]),
-            "ref_tran_steps": (C.c_int, [VP, C.c_double, C.c_int, P_F64, P_F64]),
-            "ref_tran_trunc": (C.c_int, [VP, C.c_double, C.c_int, P_F64, P_F64]),
-            "ref_num_devices": (C.c_int, [VP]),
-            "ref_solver_new": (VP, [C.c_int, C.c_int, P_I32, P_I32]),
-            "ref_solver_free": (None, [VP]),
-            "ref_solver_numeric": (C.c_int, [VP, P_F64, C.c_double]),
-            "ref_solver_refactorize": (C.c_int, [VP, P_F64, C.c_double]),
-            "ref_solver_solve": (C.c_int, [VP, P_F64]),
-            "ref_solver_solve_transposed": (C.c_int, [VP, P_F64]),
-            "ref_solver_fast_count": (C.c_int, [VP]),
-            "ref_solver_fallback_count": (C.c_int, [VP]),
-            "ref_neo_dump": (C.c_int, [VP, P_I32, P_I32, P_I32, P_I32, P_F64]),
-            "ref_neo_fillins": (C.c_int, [VP]),
-            "ref_amd_dump": (C.c_int64, [VP, P_I32, P_I32, P_I32, P_I32, P_F64, P_I32, P_I32, P_F64]),
-            "ref_amd_ordering": (C.c_int, [C.c_int, P_I32, P_I32, P_I32]),
-        }
-        for k, (r, a) in sig.items():
-            f = getattr(L, k)
-            f.restype = r
-            f.argtypes = a
-        _ref = L
+        _ref = _bind(C.CDLL(REF_SO))
     return _ref
+
+
+def gpu_available():
+    return os.path.exists(REF_GPU_SO)
+
+
+def ref_gpu():
+    """The reference whose make_solver can hand out the MI355X solver
+    (ref_set_gpu_solver)."""
+    global _ref_gpu
+    if _ref_gpu is None:
+        L = _bind(C.CDLL(REF_GPU_SO))
+        L.ref_set_gpu_solver.restype = None
+        L.ref_set_gpu_solver.argtypes = [C.c_int]
+        L.ref_gpu_solvers_made.restype = C.c_int
+        L.ref_gpu_solvers_made.argtypes = []
+        _ref_gpu = L
+    return _ref_gpu
+
+
+def _bind(L):
+    """Sets the result and argument types of the harness functions."""
+    sig = {
+        "ref_last_error": (C.c_char_p, []),
+        "ref_ckt_new": (VP, [C.c_int]),
+        "ref_ckt_free": (None, [VP]),
+        "ref_ckt_model": (C.c_int, [VP, C.c_char_p]),
+        "ref_ckt_r": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int, C.c_double]),
+        "ref_ckt_c": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int, C.c_double]),
+        "ref_ckt_v": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int, C.c_double]),
+        "ref_ckt_i": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int, C.c_double]),
+        "ref_ckt_v_pulse": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int] + [C.c_double] * 7),
+        "ref_ckt_v_pwl": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int, C.c_int, P_F64, P_F64]),
+        "ref_ckt_d": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int, C.c_char_p]),
+        "ref_ckt_m": (C.c_int, [VP, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p,
+                                C.c_double, C.c_double]),
+        "ref_bsim4_fields": (C.c_int, [VP, C.c_int, P_F64]),
+        "ref_ckt_devnoncon": (C.c_int, [VP, P_F64]),
+        "ref_ckt_ic": (C.c_int, [VP, C.c_int, C.c_double]),
+        "ref_ckt_option": (C.c_int, [VP, C.c_char_p, C.c_double]),
+        "ref_ckt_finalize": (C.c_int, [VP, C.c_int]),
+        "ref_ckt_num_vars": (C.c_int, [VP]),
+        "ref_ckt_num_nodes": (C.c_int, [VP]),
+        "ref_ckt_num_states": (C.c_int, [VP]),
+        "ref_ckt_nnz": (C.c_int, [VP]),
+        "ref_ckt_is_linear": (C.c_int, [VP]),
+        "ref_ckt_num_devices": (C.c_int, [VP]),
+        "ref_ckt_pattern": (None, [VP, P_I32, P_I32]),
+        "ref_ckt_load_order": (C.c_int, [VP, P_I32]),
+        "ref_ckt_branch_index": (C.c_int, [VP, C.c_int]),
+        "ref_ckt_dead_nodes": (C.c_int, [VP, P_I32]),
+        "ref_ckt_state": (P_F64, [VP, C.c_int]),
+        "ref_ckt_set_ctx": (None, [VP, C.c_int, C.c_int, C.c_int] + [C.c_double] * 7),
+        "ref_ckt_load": (C.c_int, [VP, P_F64, P_F64, P_F64]),
+        "ref_dc": (C.c_int, [VP, P_F64, P_F64]),
+        "ref_tran": (C.c_int, [VP, C.c_double, C.c_double, C.c_int, P_F64, P_F64, P_F64,
+                               P_I32]),
+        "ref_tran_fixed": (C.c_int, [VP, C.c_double, C.c_int, P_F64, P_F64]),
+        "ref_tran_num_records": (C.c_int, [VP]),
+        "ref_tran_records": (None, [VP, P_I32, P_F64]),
+        "ref_newton": (C.c_int, [VP, P_F64, C.c_int]),
+        "ref_sens": (C.c_int, [VP, C.c_int, C.c_int, P_F64, C.c_char_p, C.c_int]),
+        "ref_tran_steps": (C.c_int, [VP, C.c_double, C.c_int, P_F64, P_F64]),
+        "ref_tran_trunc": (C.c_int, [VP, C.c_double, C.c_int, P_F64, P_F64]),
+        "ref_num_devices": (C.c_int, [VP]),
+        "ref_solver_new": (VP, [C.c_int, C.c_int, P_I32, P_I32]),
+        "ref_solver_free": (None, [VP]),
+        "ref_solver_numeric": (C.c_int, [VP, P_F64, C.c_double]),
+        "ref_solver_refactorize": (C.c_int, [VP, P_F64, C.c_double]),
+        "ref_solver_solve": (C.c_int, [VP, P_F64]),
+        "ref_solver_solve_transposed": (C.c_int, [VP, P_F64]),
+        "ref_solver_fast_count": (C.c_int, [VP]),
+        "ref_solver_fallback_count": (C.c_int, [VP]),
+        "ref_neo_dump": (C.c_int, [VP, P_I32, P_I32, P_I32, P_I32, P_F64]),
+        "ref_neo_fillins": (C.c_int, [VP]),
+        "ref_amd_dump": (C.c_int64, [VP, P_I32, P_I32, P_I32, P_I32, P_F64, P_I32, P_I32, P_F64]),
+        "ref_amd_ordering": (C.c_int, [C.c_int, P_I32, P_I32, P_I32]),
+    }
+    for k, (r, a) in sig.items():
+        f = getattr(L, k)
+        f.restype = r
+        f.argtypes = a
+    return L
 
 
 def orc():
@@ -226,7 +251,10 @@ def f64(a):
 
 def _chk(rc):
     if rc < 0:
-        raise RuntimeError("reference: " + ref().ref_last_error().decode())
+        msg = ref().ref_last_error().decode()
+        if not msg and _ref_gpu is not None:
+            msg = _ref_gpu.ref_last_error().decode()
+        raise RuntimeError("reference: " + msg)
     return rc
 
 
@@ -234,8 +262,8 @@ class RefCircuit:
     """A reference Circuit built from a neospice_amd.Circuit's recorded calls.
     `values` optionally overrides device primary values {device index: value}."""
 
-    def __init__(self, ckt, values=None, options=None, probe=True):
-        L = ref()
+    def __init__(self, ckt, values=None, options=None, probe=True, gpu=False):
+        L = ref_gpu() if gpu else ref()
         self.L = L
         nn = len(ckt.nodes)
         self.h = C.c_void_p(L.ref_ckt_new(nn))

@@ -83,13 +83,8 @@ def test_refactorize_batched(case, engine):
     V = np.stack([vals * (1 + 1e-3 * rng.standard_normal(len(vals))) for _ in range(B)], axis=1)
     s = na.Solver(n, cp, ri, batch=B, engine=engine)
     assert not s.numeric(np.repeat(vals[:, None], B, axis=1)).any()
-    try:
-        st = s.refactorize(V)
-    except na._lib.NsaError as e:
-        # an instance whose re-pivot changes the order cannot stay in the
-        # shared plan: reported as unsupported, never silently wrong
-        assert e.code == -5 and engine == "amdlu"
-        pytest.skip("pivot order diverged within the batch")
+    # instances whose re-pivot changes the order move to a plan of their own
+    st = s.refactorize(V)
     b = rng.standard_normal((n, B))
     x = s.solve(b)
     for i in range(B):
@@ -115,3 +110,41 @@ def test_pivot_order_matches_reference():
     assert np.array_equal(st["col_perm"], d["i2e_col"] - 1)
     assert np.array_equal(st["seg_ptr"], d["col_ptr"])
     assert np.array_equal(st["rows"], d["rows"] - 1)
+
+
+@pytest.mark.parametrize("engine", ["markowitz", "amdlu"])
+def test_refactorize_batched_diverging_orders(engine):
+    """One instance of a batch whose diagonal collapses: its reused order
+    fails the threshold (Markowitz, factor.cpp:929-946) or the replay's
+    growth check (AMD-LU, amd_lu_solver.cpp:214-271), so it re-pivots on
+    the CPU and leaves the batch's shared plan for one of its own; every
+    instance -- moved or not -- equals the reference refactorize of its own
+    values, and a later solve uses each instance's own order."""
+    name, n, cp, ri, vals = CASES[2]
+    rs = RB.RefSolver(KIND[engine], n, cp, ri)
+    if rs.numeric(vals):
+        pytest.skip("singular in the reference")
+    B = 4
+    rng = np.random.default_rng(11)
+    V = np.stack([vals * (1 + 1e-3 * rng.standard_normal(len(vals))) for _ in range(B)], axis=1)
+    rows = np.asarray(ri)
+    cols = np.repeat(np.arange(n), np.diff(np.asarray(cp)))
+    diag = rows == cols
+    V[diag, 2] *= 1e-9
+    s = na.Solver(n, cp, ri, batch=B, engine=engine)
+    assert not s.numeric(np.repeat(vals[:, None], B, axis=1)).any()
+    st = s.refactorize(V)
+    b = rng.standard_normal((n, B))
+    x = s.solve(b)
+    moved = 0
+    for i in range(B):
+        r2 = RB.RefSolver(KIND[engine], n, cp, ri)
+        assert not r2.numeric(vals)
+        sing = r2.refactorize(V[:, i])
+        assert bool(st[i]) == sing, i
+        if sing:
+            continue
+        assert np.array_equal(x[:, i], r2.solve(b[:, i])), i
+        if i == 2:
+            moved = 1
+    assert moved
