@@ -287,6 +287,12 @@ void nsa_lu_destroy(nsa_lu* s);
  * 1 if any instance is singular. */
 int nsa_lu_numeric(nsa_lu* s, const double* values, double diag_gmin, int32_t* status);
 int nsa_lu_refactorize(nsa_lu* s, const double* values, double diag_gmin, int32_t* status);
+/* ISolver::set_equilibrate (src/core/solver_iface.hpp:33-36): radix-2
+ * row/column equilibration of the following numeric/refactorize and the
+ * solves on their factors (NeoSolver, neo_solver.cpp:52-102, 204-225).
+ * Honoured by Markowitz solvers only, as in the reference (AmdLuSolver and
+ * the AMD-LU auto-fallback keep the interface's no-op). */
+int nsa_lu_set_equilibrate(nsa_lu* s, int32_t on);
 /* rhs: host, n x B, overwritten with the solution. */
 int nsa_lu_solve(nsa_lu* s, double* rhs);
 int nsa_lu_solve_transposed(nsa_lu* s, double* rhs);

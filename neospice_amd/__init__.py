@@ -453,6 +453,12 @@ class Solver:
     def name(self):
         return lib().nsa_lu_name(self.h).decode()
 
+    def set_equilibrate(self, on=True):
+        """ISolver::set_equilibrate: radix-2 row/column scaling of the next
+        factorizations and their solves (Markowitz solvers only, as in the
+        reference)."""
+        check(lib().nsa_lu_set_equilibrate(self.h, int(bool(on))))
+
     def info(self):
         o = np.zeros(4, np.int32)
         check(lib().nsa_lu_info(self.h, i32p(o)))

@@ -86,6 +86,7 @@ _SIGS = {
     "nsa_lu_destroy": (None, [VP]),
     "nsa_lu_numeric": (C.c_int, [VP, P_F64, C.c_double, P_I32]),
     "nsa_lu_refactorize": (C.c_int, [VP, P_F64, C.c_double, P_I32]),
+    "nsa_lu_set_equilibrate": (C.c_int, [VP, I32]),
     "nsa_lu_solve": (C.c_int, [VP, P_F64]),
     "nsa_lu_solve_transposed": (C.c_int, [VP, P_F64]),
     "nsa_lu_solve_noalloc": (C.c_int, [VP, P_F64]),

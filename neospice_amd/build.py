@@ -35,6 +35,7 @@ SOURCES = [
     "gpu_small_lu.hip",
     "gpu_bundle.hip",
     "noalloc.hip",
+    "gpu_equil.hip",
 ]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]

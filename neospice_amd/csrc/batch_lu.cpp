@@ -5,6 +5,8 @@
 
 #include "common.hpp"
 #include "gpu_dev.hpp"
+#include "gpu_equil.hpp"
+#include "parallel.hpp"
 
 namespace nsa {
 
@@ -37,6 +39,8 @@ BatchLu::~BatchLu() {
     (void)hipFree(d_fail_col_);
     (void)hipFree(d_kind_);
     (void)hipFree(d_vals_);
+    for (void* p : {(void*)d_eqA_, (void*)d_dr_, (void*)d_dc_, (void*)d_erows_, (void*)d_ecols_})
+        if (p) (void)hipFree(p);
     (void)hipHostFree(h_kind_pin_);
     (void)hipHostFree(h_fail_pin_);
     if (h_vals_pin_) (void)hipHostFree(h_vals_pin_);
@@ -50,7 +54,6 @@ void BatchLu::symbolic(int n, const int32_t* col_ptr, const int32_t* row_idx, bo
     groups_.clear();
     group_of_.assign(B_, -1);
     factored_ = false;
-    committed_ = false;
     auto_amd_ = false;
     if (engine == ENGINE_AMDLU) {
         form_ = FORM_GP;
@@ -60,6 +63,12 @@ void BatchLu::symbolic(int n, const int32_t* col_ptr, const int32_t* row_idx, bo
         form_ = (n >= kAmdLuAutoThreshold && is_linear) ? FORM_GP : FORM_S13;
         auto_amd_ = form_ == FORM_GP;
     }
+    honor_equil_ = form_ == FORM_S13;
+    iform_.assign(B_, form_);
+    for (void* p : {(void*)d_eqA_, (void*)d_dr_, (void*)d_dc_, (void*)d_erows_, (void*)d_ecols_})
+        if (p) NSA_HIP(hipFree(p));
+    d_eqA_ = d_dr_ = d_dc_ = nullptr;
+    d_erows_ = d_ecols_ = nullptr;
     if (form_ == FORM_GP) {
         q_ = min_degree_order(n, acp_.data(), ari_.data());
         if ((int)q_.size() != n) {
@@ -87,10 +96,9 @@ std::vector<double> BatchLu::instance_values(const double* dA, int inst) {
 // from `from` (the state of the instance's previous numeric(), reuse path)
 // or starts fresh; AMD-LU always re-pivots from scratch (factor_).  Returns
 // true when singular.
-bool BatchLu::cpu_order(const std::vector<double>& vals, double gmin, const Sparse13* from,
+bool BatchLu::cpu_order(LuForm form, const std::vector<double>& vals, double gmin, const Sparse13* from,
                         LuStructure& out, std::unique_ptr<Sparse13>& state) {
-    ++n_cpu_pivot;
-    if (form_ == FORM_S13) {
+    if (form == FORM_S13) {
         state = from ? from->clone() : std::make_unique<Sparse13>(n_, acp_.data(), ari_.data());
         if (state->numeric(vals.data(), gmin) != 0) return true;
         out = state->structure();
@@ -169,46 +177,118 @@ void BatchLu::run_gpu(const double* dA, double gmin, const std::vector<int32_t>&
     collect_timing();
 }
 
+// The equilibrated system of dA: D_r A D_c with the scaled diag_gmin on
+// the diagonal slots (gmin is then 0 for the factorization, as NeoSolver
+// skips add_diag_gmin when equilibrating, neo_solver.cpp:160-166).
+const double* BatchLu::equilibrate(const double* dA, double& gmin) {
+    const int nnz = acp_[n_];
+    if (!d_eqA_) {
+        std::vector<int32_t> cols(nnz);
+        for (int j = 0; j < n_; ++j)
+            for (int p = acp_[j]; p < acp_[j + 1]; ++p) cols[p] = j;
+        NSA_HIP(hipMalloc(&d_eqA_, sizeof(double) * std::max(1, nnz) * B_));
+        NSA_HIP(hipMalloc(&d_dr_, sizeof(double) * n_ * B_));
+        NSA_HIP(hipMalloc(&d_dc_, sizeof(double) * n_ * B_));
+        NSA_HIP(hipMalloc(&d_erows_, sizeof(int32_t) * std::max(1, nnz)));
+        NSA_HIP(hipMalloc(&d_ecols_, sizeof(int32_t) * std::max(1, nnz)));
+        NSA_HIP(hipMemcpyAsync(d_erows_, ari_.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice, stream_));
+        NSA_HIP(hipMemcpyAsync(d_ecols_, cols.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice, stream_));
+        NSA_HIP(hipStreamSynchronize(stream_));
+    }
+    launch_equilibrate(n_, nnz, d_erows_, d_ecols_, dA, gmin, B_, d_eqA_, d_dr_, d_dc_, stream_);
+    gmin = 0.0;
+    return d_eqA_;
+}
+
+// Every instance without a factorization is ordered on the CPU from its own
+// values, as the first numeric() of its own solver does (a refactorize()
+// there is a numeric(): NeoSolver throws and newton_solve retries with
+// numeric(), newton.cpp:235-242; AmdLuSolver runs factor_,
+// amd_lu_solver.cpp:280-301).  Under the auto policy a singular AMD-LU
+// order falls back to Markowitz for that instance (AutoFallbackSolver,
+// make_solver.cpp:85-101).  Instances whose structures are identical share
+// a group (one GPU plan); those are TRUSTED in this call's factorization,
+// and singular ones keep no group.  Returns the number of singular ones.
+int BatchLu::order_fresh(const double* dA, double gmin, std::vector<int32_t>& kind, int32_t* status) {
+    std::vector<int> fresh;
+    for (int i = 0; i < B_; ++i)
+        if (group_of_[i] < 0) fresh.push_back(i);
+    if (fresh.empty()) return 0;
+    const int nnz = acp_[n_];
+    std::vector<std::vector<double>> vals(fresh.size());
+    if (fresh.size() == 1) {
+        vals[0] = instance_values(dA, fresh[0]);
+    } else {
+        std::vector<double> all((size_t)nnz * B_);
+        NSA_HIP(hipMemcpyAsync(all.data(), dA, sizeof(double) * all.size(), hipMemcpyDeviceToHost, stream_));
+        NSA_HIP(hipStreamSynchronize(stream_));
+        for (size_t t = 0; t < fresh.size(); ++t) {
+            vals[t].resize(nnz);
+            for (int k = 0; k < nnz; ++k) vals[t][k] = all[(size_t)k * B_ + fresh[t]];
+        }
+    }
+    std::vector<LuStructure> st(fresh.size());
+    std::vector<std::unique_ptr<Sparse13>> s13(fresh.size());
+    std::vector<char> sing(fresh.size(), 0);
+    std::vector<LuForm> form(fresh.size());
+    parallel_for((int)fresh.size(), [&](int t) {
+        LuForm f = iform_[fresh[t]];
+        bool sg = cpu_order(f, vals[t], gmin, nullptr, st[t], s13[t]);
+        if (sg && f == FORM_GP && auto_amd_) {
+            f = FORM_S13;
+            sg = cpu_order(f, vals[t], gmin, nullptr, st[t], s13[t]);
+        }
+        sing[t] = sg ? 1 : 0;
+        form[t] = f;
+    });
+    n_cpu_pivot += (long)fresh.size();
+    int nsing = 0;
+    std::vector<int> touched;
+    for (size_t t = 0; t < fresh.size(); ++t) {
+        const int i = fresh[t];
+        // the auto policy's fallback is decided at an instance's first
+        // factorization only (AutoFallbackSolver commits after it)
+        iform_[i] = form[t];
+        if (sing[t]) {
+            status[i] = 1;
+            ++nsing;
+            kind[i] = KIND_SKIP;
+            continue;
+        }
+        int g = find_group(st[t]);
+        if (g < 0) {
+            Group G;
+            std::vector<int32_t> one{i};
+            G.lu = std::make_unique<GpuLu>(st[t], 1, stream_, &one, B_);
+            G.s13 = std::move(s13[t]);
+            groups_.push_back(std::move(G));
+            g = (int)groups_.size() - 1;
+        }
+        groups_[g].inst.push_back(i);
+        group_of_[i] = g;
+        kind[i] = KIND_TRUSTED;  // its order was just computed from its own values
+        if (std::find(touched.begin(), touched.end(), g) == touched.end()) touched.push_back(g);
+    }
+    auto_amd_ = false;
+    for (int g : touched) rebuild_group(g);
+    return nsing;
+}
+
 int BatchLu::factor(const double* dA, double gmin, const int32_t* numeric_mask, int32_t* status) {
     if (acp_.empty()) throw Error(NSA_ERR_LOGIC, "BatchLu: symbolic() not called");
+    if (equilibrating()) dA = equilibrate(dA, gmin);
     std::vector<int32_t> kind(B_, KIND_VERIFY);
     if (numeric_mask)
         for (int i = 0; i < B_; ++i) kind[i] = numeric_mask[i] ? KIND_VERIFY : KIND_REFACTOR;
     std::fill(status, status + B_, 0);
-    if (groups_.empty()) {
-        // First factorization.  A refactorize() on a fresh solver is a
-        // numeric(): NeoSolver throws and newton_solve retries with numeric()
-        // (newton.cpp:235-242); AmdLuSolver runs factor_ (amd_lu_solver.cpp:280-301).
-        std::fill(kind.begin(), kind.end(), KIND_VERIFY);
-        auto vals = instance_values(dA, 0);
-        LuStructure s;
-        std::unique_ptr<Sparse13> st;
-        bool sing = cpu_order(vals, gmin, nullptr, s, st);
-        if (sing && auto_amd_ && !committed_) {
-            // AutoFallbackSolver (make_solver.cpp:85-101): a singular first
-            // AMD-LU factorization switches to Markowitz.
-            form_ = FORM_S13;
-            auto_amd_ = false;
-            sing = cpu_order(vals, gmin, nullptr, s, st);
-        }
-        committed_ = true;
-        if (sing) {
-            std::fill(status, status + B_, 1);
-            return B_;
-        }
-        Group G;
-        G.inst.resize(B_);
-        for (int i = 0; i < B_; ++i) { G.inst[i] = i; group_of_[i] = 0; }
-        G.lu = std::make_unique<GpuLu>(s, B_, stream_);
-        G.s13 = std::move(st);
-        groups_.push_back(std::move(G));
-        kind[0] = KIND_TRUSTED;  // its order was just computed from its own values
-    }
+    // instances without a factorization: a fresh solver's, or one whose
+    // last numeric() was singular
+    int nsing = order_fresh(dA, gmin, kind, status);
+    if (groups_.empty()) return nsing;
     std::vector<int> all_groups(groups_.size());
     for (size_t g = 0; g < groups_.size(); ++g) all_groups[g] = (int)g;
     std::vector<int32_t> fail;
     run_gpu(dA, gmin, kind, all_groups, fail);
-    int nsing = 0;
     std::vector<char> changed(groups_.size(), 0);
     std::vector<int32_t> rerun_kind(B_, KIND_SKIP);
     bool moved = false;
@@ -216,12 +296,13 @@ int BatchLu::factor(const double* dA, double gmin, const int32_t* numeric_mask, 
         const int32_t f = fail[i];
         if (f == FAIL_NONE) continue;
         const int k = kind[i];
+        const LuForm fi = iform_[i];
         // Sparse 1.3 refactorize: exact zero pivot is reported as singular
         // (factor.cpp:997, 1025, 1045); newton_solve then asks for numeric().
-        if (form_ == FORM_S13 && k == KIND_REFACTOR) { status[i] = 1; ++nsing; continue; }
+        if (fi == FORM_S13 && k == KIND_REFACTOR) { status[i] = 1; ++nsing; continue; }
         // A fresh AMD-LU factorization would be singular, or an exact zero
         // on an order just computed for these values.
-        if ((form_ == FORM_GP && k == KIND_VERIFY && f == FAIL_ZERO_PIVOT) || k == KIND_TRUSTED) {
+        if ((fi == FORM_GP && k == KIND_VERIFY && f == FAIL_ZERO_PIVOT) || k == KIND_TRUSTED) {
             status[i] = 1;
             ++nsing;
             continue;
@@ -233,7 +314,8 @@ int BatchLu::factor(const double* dA, double gmin, const int32_t* numeric_mask, 
         auto vals = instance_values(dA, i);
         LuStructure s;
         std::unique_ptr<Sparse13> st;
-        if (cpu_order(vals, gmin, groups_[g].s13.get(), s, st)) {
+        ++n_cpu_pivot;
+        if (cpu_order(fi, vals, gmin, groups_[g].s13.get(), s, st)) {
             status[i] = 1;
             ++nsing;
             continue;
@@ -304,15 +386,23 @@ int BatchLu::refactorize(const double* dA, double gmin, int32_t* status) {
 
 void BatchLu::solve(double* db) {
     if (!factored()) throw Error(NSA_ERR_LOGIC, "BatchLu::solve: numeric() not called");
+    // equilibrated factors are of D_r A D_c: solve for D_r b, recover x = D_c y
+    const bool eq = equilibrating() && d_dr_;
+    if (eq) launch_scale_vec(n_, B_, d_dr_, db, stream_);
     if (timing_) NSA_HIP(hipEventRecord(ev_[2], stream_));
     for_groups(all_groups(), [&](int g, hipStream_t st) { groups_[g].lu->solve(db, false, st); });
     if (timing_) { NSA_HIP(hipEventRecord(ev_[3], stream_)); pend_sv_ = true; }
+    if (eq) launch_scale_vec(n_, B_, d_dc_, db, stream_);
     ++n_gpu_solve;
 }
 
 void BatchLu::solve_t(double* db) {
     if (!factored()) throw Error(NSA_ERR_LOGIC, "BatchLu::solve_t: numeric() not called");
+    // A^T x = b from the factors of M = D_r A D_c: x = D_r M^-T (D_c b)
+    const bool eq = equilibrating() && d_dr_;
+    if (eq) launch_scale_vec(n_, B_, d_dc_, db, stream_);
     for_groups(all_groups(), [&](int g, hipStream_t st) { groups_[g].lu->solve(db, true, st); });
+    if (eq) launch_scale_vec(n_, B_, d_dr_, db, stream_);
 }
 
 void BatchLu::set_timing(bool on) {

@@ -49,8 +49,20 @@ public:
     int refactorize(const double* dA, double diag_gmin, int32_t* status);
     void solve(double* db);
     void solve_t(double* db);
+    // Row/column equilibration (ISolver::set_equilibrate; NeoSolver's
+    // radix-2 scaling, neo_solver.cpp:52-102 and 204-225).  Honoured where
+    // the reference's solver would honour it: on Markowitz (NeoSolver)
+    // solvers; AMD-LU and the AMD-LU auto-fallback ignore it, as
+    // AmdLuSolver and AutoFallbackSolver inherit the interface's no-op.
+    void set_equilibrate(bool on) { equil_ = on; }
+    bool equilibrating() const { return equil_ && honor_equil_; }
 
-    const char* name() const { return form_ == FORM_S13 ? "markowitz" : "amdlu"; }
+    // the engine of instance 0 (AMD-LU auto-selection can fall back to
+    // Markowitz per instance, as AutoFallbackSolver does per solver)
+    const char* name() const {
+        const LuForm f = iform_.empty() ? form_ : iform_[0];
+        return f == FORM_S13 ? "markowitz" : "amdlu";
+    }
     bool factored() const { return !groups_.empty() && factored_; }
     int num_groups() const { return (int)groups_.size(); }
     // The plan of structure group g (group 0 holds the first instance's order).
@@ -79,19 +91,26 @@ private:
         std::unique_ptr<Sparse13> s13;  // Markowitz: CPU state after the numeric() that fixed the order
     };
     std::vector<double> instance_values(const double* dA, int inst);
-    bool cpu_order(const std::vector<double>& vals, double gmin, const Sparse13* from,
+    bool cpu_order(LuForm form, const std::vector<double>& vals, double gmin, const Sparse13* from,
                    LuStructure& out, std::unique_ptr<Sparse13>& state);
+    int order_fresh(const double* dA, double gmin, std::vector<int32_t>& kind, int32_t* status);
     int find_group(const LuStructure& s) const;
     void rebuild_group(int g);
     void run_gpu(const double* dA, double gmin, const std::vector<int32_t>& kind,
                  const std::vector<int>& which, std::vector<int32_t>& fail);
 
+    const double* equilibrate(const double* dA, double& gmin);
+
     int B_;
     hipStream_t stream_;
+    bool equil_ = false, honor_equil_ = false;
+    double *d_eqA_ = nullptr, *d_dr_ = nullptr, *d_dc_ = nullptr;
+    int32_t *d_erows_ = nullptr, *d_ecols_ = nullptr;
     int n_ = 0;
     std::vector<int32_t> acp_, ari_;
-    LuForm form_ = FORM_S13;
-    bool auto_amd_ = false, committed_ = false;
+    LuForm form_ = FORM_S13;      // the engine make_solver's policy picks
+    bool auto_amd_ = false;       // AMD-LU by the auto policy: a singular first factorization falls back
+    std::vector<LuForm> iform_;   // per instance: the engine factoring it
     std::vector<int32_t> q_;
     std::vector<Group> groups_;
     std::vector<int32_t> group_of_;

@@ -1,4 +1,5 @@
 #include "bundle.hpp"
+#include "parallel.hpp"
 
 #include <chrono>
 
@@ -41,30 +42,6 @@ void csr(const std::vector<std::vector<int32_t>>& L, std::vector<int32_t>& ptr, 
     for (auto& l : L) idx.insert(idx.end(), l.begin(), l.end());
 }
 
-// Runs f(i) for i < count on up to `threads` host threads.
-template <class F>
-void parallel_for(int count, F&& f) {
-    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
-    const int nt = std::min({hw, 16, count});
-    if (nt <= 1) {
-        for (int i = 0; i < count; ++i) f(i);
-        return;
-    }
-    std::vector<std::thread> th;
-    std::atomic<int> next{0};
-    std::vector<std::exception_ptr> err(nt);
-    for (int t = 0; t < nt; ++t)
-        th.emplace_back([&, t] {
-            try {
-                for (int i; (i = next.fetch_add(1)) < count;) f(i);
-            } catch (...) {
-                err[t] = std::current_exception();
-            }
-        });
-    for (auto& x : th) x.join();
-    for (auto& e : err)
-        if (e) std::rethrow_exception(e);
-}
 
 }  // namespace
 

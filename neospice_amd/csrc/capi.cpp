@@ -442,6 +442,12 @@ static int lu_solve(nsa_lu* s, double* rhs, bool t) {
     NSA_HIP(hipStreamSynchronize(s->stream));
     return 0;
 }
+int nsa_lu_set_equilibrate(nsa_lu* s, int32_t on) {
+    return guard([&] {
+        s->lu->set_equilibrate(on != 0);
+        return 0;
+    });
+}
 int nsa_lu_solve(nsa_lu* s, double* rhs) {
     return guard([&] { return lu_solve(s, rhs, false); });
 }

@@ -65,6 +65,12 @@ public:
         if ((int)rhs.size() != n_) throw std::invalid_argument("Mi355xSolver::solve_transposed: rhs size");
         check(nsa_lu_solve_transposed(h_, rhs.data()));
     }
+    // Honoured on Markowitz solvers, as NeoSolver does (the DC ladder's last
+    // fallback, dc.cpp:367-380).
+    void set_equilibrate(bool on) override {
+        if (!h_) throw std::logic_error("Mi355xSolver::set_equilibrate: symbolic() not called");
+        check(nsa_lu_set_equilibrate(h_, on ? 1 : 0));
+    }
     void numeric_complex(const SparsityPattern&, const std::vector<double>&) override { unsupported(); }
     bool refactorize_complex(const std::vector<double>&) override { unsupported(); }
     void solve_complex(std::vector<double>&) override { unsupported(); }

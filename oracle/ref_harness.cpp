@@ -728,6 +728,12 @@ static void load_vals(RefSolver* s, const double* v) {
     std::copy(v, v + s->mat->nnz(), s->mat->data());
 }
 
+// ISolver::set_equilibrate on the NeoSolver (a no-op on AmdLuSolver).
+void ref_solver_set_equilibrate(void* p, int on) {
+    auto* s = static_cast<RefSolver*>(p);
+    if (s->kind == 0) s->neo.set_equilibrate(on != 0); else s->amd.set_equilibrate(on != 0);
+}
+
 int ref_solver_numeric(void* p, const double* values, double gmin) {
     auto* s = static_cast<RefSolver*>(p);
     return guarded([&] {

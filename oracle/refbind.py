@@ -103,6 +103,7 @@ def _bind(L):
         "ref_solver_new": (VP, [C.c_int, C.c_int, P_I32, P_I32]),
         "ref_solver_free": (None, [VP]),
         "ref_solver_numeric": (C.c_int, [VP, P_F64, C.c_double]),
+        "ref_solver_set_equilibrate": (None, [VP, C.c_int]),
         "ref_solver_refactorize": (C.c_int, [VP, P_F64, C.c_double]),
         "ref_solver_solve": (C.c_int, [VP, P_F64]),
         "ref_solver_solve_transposed": (C.c_int, [VP, P_F64]),
@@ -434,6 +435,9 @@ class RefSolver:
         if getattr(self, "h", None):
             self.L.ref_solver_free(self.h)
             self.h = None
+
+    def set_equilibrate(self, on=True):
+        self.L.ref_solver_set_equilibrate(self.h, 1 if on else 0)
 
     def numeric(self, vals, gmin=0.0):
         v, pv = f64(vals)
