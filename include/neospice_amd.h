@@ -197,14 +197,17 @@ int nsa_bundle_offsets(nsa_bundle* b, int32_t* xoff, int32_t* aoff);
  * instance in `mode` (MODEDCOP or MODETRANOP with MODEINITJCT) from the
  * current solution; converged / iterations per instance. */
 int nsa_bundle_dc(nsa_bundle* b, int32_t mode, int32_t* converged, int32_t* iterations);
-/* The transient preamble's operating point (compute_dc_operating_point,
- * src/core/transient.cpp:172-238) from the current solution: direct Newton in
- * MODETRANOP, then gmin stepping (src/core/convergence.cpp:216-346) for the
- * instances that failed.  stage per instance: 0 direct, 1 gmin stepping,
- * -1 not converged (source stepping and pseudo-transient are refused).
- * flags: bit 0 runs gmin stepping alone on every instance (no direct
- * attempt); bit 1 uses solve_dc's MODEDCOP (src/core/dc.cpp:104-150) in
- * place of MODETRANOP. */
+/* The DC operating-point ladder per instance, from the current solution:
+ * the transient preamble's (compute_dc_operating_point,
+ * src/core/transient.cpp:172-238: direct Newton in MODETRANOP, then gmin
+ * stepping and source stepping, src/core/convergence.cpp:216-346, 472-582)
+ * or, with flags bit 1, solve_dc's (src/core/dc.cpp:104-184: MODEDCOP,
+ * with true gmin stepping, convergence.cpp:348-470, before gmin stepping).
+ * Every instance follows its own schedule.  stage per instance: 0 direct,
+ * 1 gmin stepping, 2 source stepping, 4 true gmin stepping, -1 not
+ * converged (pseudo-transient continuation and the later fallbacks are not
+ * restated).  flags bit 0: one method alone on every instance (no direct
+ * attempt), bits 4-5 naming it: 0 gmin, 1 source, 2 true gmin stepping. */
 int nsa_bundle_dc_ladder(nsa_bundle* b, int32_t flags, int32_t* converged, int32_t* iterations,
                          int32_t* stage);
 int nsa_bundle_get_solution(nsa_bundle* b, double* x);

@@ -551,14 +551,18 @@ class Bundle:
         check(lib().nsa_bundle_dc(self.h, int(mode), i32p(conv), i32p(its)))
         return conv.astype(bool), its
 
-    def op_ladder(self, x0=0.0, skip_direct=False, dcop=False):
-        """The transient preamble's operating point: direct Newton in
-        MODETRANOP (MODEDCOP with dcop, as solve_dc), then gmin stepping for
-        the instances that failed.  Returns (converged, iterations, stage)
-        per instance."""
+    def op_ladder(self, x0=0.0, skip_direct=False, dcop=False, method="gmin"):
+        """The operating-point ladder per instance: direct Newton in
+        MODETRANOP (MODEDCOP with dcop, as solve_dc), then for the instances
+        that failed true gmin stepping (dcop only), gmin stepping and source
+        stepping.  Returns (converged, iterations, stage) per instance, stage
+        0 direct, 1 gmin, 2 source, 4 true gmin stepping, -1 none.  With
+        skip_direct, `method` ("gmin", "source", "true_gmin") runs alone on
+        every instance."""
         self.set_solution(x0)
         conv, its, st = (np.zeros(self.N, np.int32) for _ in range(3))
-        flags = int(bool(skip_direct)) | (2 if dcop else 0)
+        m = {"gmin": 0, "source": 1, "true_gmin": 2}[method]
+        flags = int(bool(skip_direct)) | (2 if dcop else 0) | (m << 4)
         check(lib().nsa_bundle_dc_ladder(self.h, flags, i32p(conv), i32p(its), i32p(st)))
         return conv.astype(bool), its, st
 

@@ -103,12 +103,14 @@ Bundle::Bundle(const std::vector<const Circuit*>& circuits, const Options& opt)
         d_xinst_ = dupload(allocs_, xo);
     }
     d_gmin_ = dupload(allocs_, std::vector<double>(NI_, opt_.diag_gmin));
+    d_devgmin_ = dupload(allocs_, std::vector<double>(NI_, opt_.gmin));
     d_mask_ = dalloc<int32_t>(allocs_, 3 * (size_t)NI_);
     NSA_HIP(hipHostMalloc((void**)&h_mask_pin_, sizeof(int32_t) * 3 * NI_, hipHostMallocDefault));
     NSA_HIP(hipHostMalloc((void**)&h_gmin_pin_, sizeof(double) * NI_, hipHostMallocDefault));
     d_in_ = dupload(allocs_, nn);
     d_inn_ = dupload(allocs_, nd);
     tol_.gmin = opt_.gmin;
+    tol_.gmin_inst = d_devgmin_;
     tol_.reltol = opt_.reltol;
     tol_.abstol = opt_.abstol;
     build_tables();
@@ -784,6 +786,13 @@ void Bundle::newton(int itl, const int32_t* active, std::vector<int32_t>& conver
 
 // ---------------------------------------------------------------- DC ladder
 
+void Bundle::set_dev_gmin(const std::vector<double>& g) {
+    NSA_HIP(hipStreamSynchronize(stream_));
+    std::memcpy(h_gmin_pin_, g.data(), sizeof(double) * NI_);
+    NSA_HIP(hipMemcpyAsync(d_devgmin_, h_gmin_pin_, sizeof(double) * NI_, hipMemcpyHostToDevice, stream_));
+    NSA_HIP(hipStreamSynchronize(stream_));
+}
+
 void Bundle::set_gmin(const std::vector<double>& g) {
     NSA_HIP(hipStreamSynchronize(stream_));
     std::memcpy(h_gmin_pin_, g.data(), sizeof(double) * NI_);
@@ -808,8 +817,9 @@ void Bundle::masked_copy_state(const std::vector<int32_t>& mask, double* const* 
 
 void Bundle::op_ladder(int firstmode, int continuemode, std::vector<int32_t>& converged,
                        std::vector<int32_t>& iterations, std::vector<int32_t>& stage,
-                       bool skip_direct) {
-    // 1. direct Newton (transient.cpp:195-205)
+                       int alone) {
+    const bool skip_direct = alone != LADDER_NONE;
+    // 1. direct Newton (dc.cpp:104-107, transient.cpp:195-205)
     if (skip_direct) {
         converged.assign(NI_, 0);
         iterations.assign(NI_, 0);
@@ -819,19 +829,47 @@ void Bundle::op_ladder(int firstmode, int continuemode, std::vector<int32_t>& co
     } else {
         dc_op(firstmode, converged, iterations);
     }
-    stage.assign(NI_, -1);
-    std::vector<int32_t> failing(NI_, 0);
-    bool any = false;
-    for (int i = 0; i < NI_; ++i) {
-        if (converged[i]) stage[i] = 0;
-        failing[i] = !converged[i];
-        any |= failing[i] != 0;
-    }
-    if (!any) return;
-    // 2. gmin stepping (convergence.cpp:216-346) for the instances that failed
+    stage.assign(NI_, LADDER_NONE);
+    for (int i = 0; i < NI_; ++i)
+        if (converged[i]) stage[i] = LADDER_DIRECT;
     if (!d_lad_[0])
         for (int k = 0; k < 8; ++k)
             d_lad_[k] = dalloc<double>(allocs_, k % 4 == 0 ? tot_n_ : std::max(1, ns_tot_));
+    // 2. for the instances still failing, the continuation methods in the
+    //    reference's order: solve_dc (MODEDCOP) tries true gmin stepping
+    //    first (dc.cpp:120-145); both ladders then gmin stepping and source
+    //    stepping (dc.cpp:146-184, transient.cpp:206-222)
+    if (alone == LADDER_GMIN || alone == LADDER_TRUE_GMIN) {
+        ladder_gmin(firstmode, continuemode, alone == LADDER_TRUE_GMIN, converged, iterations, stage);
+        return;
+    }
+    if (alone == LADDER_SOURCE) {
+        ladder_source(firstmode & ~INITF_MASK, converged, iterations, stage);
+        return;
+    }
+    if (skip_direct) throw Error(NSA_ERR_ARGUMENT, "Bundle::op_ladder: unknown method");
+    if (firstmode & MODEDCOP) ladder_gmin(firstmode, continuemode, true, converged, iterations, stage);
+    ladder_gmin(firstmode, continuemode, false, converged, iterations, stage);
+    ladder_source(firstmode & ~INITF_MASK, converged, iterations, stage);
+}
+
+// gmin stepping (convergence.cpp:216-346) or, with device, true gmin
+// stepping (convergence.cpp:348-470) for every instance whose stage is
+// still LADDER_NONE, each on its own schedule: the stepped conductance is
+// the diagonal gmin (gmin stepping) or the devices' gmin (true gmin
+// stepping), from 1e-3 down to max(gmin, gshunt) with ngspice's factor
+// adaptation; then one more solve at the target (diag_gmin = gshunt, or
+// the original device gmin).  An instance that fails goes back to its
+// entry solution and states.
+void Bundle::ladder_gmin(int firstmode, int continuemode, bool device, std::vector<int32_t>& converged,
+                         std::vector<int32_t>& iterations, std::vector<int32_t>& stage) {
+    std::vector<int32_t> failing(NI_, 0);
+    bool any = false;
+    for (int i = 0; i < NI_; ++i) {
+        failing[i] = stage[i] == LADDER_NONE;
+        any |= failing[i] != 0;
+    }
+    if (!any) return;
     double* cur[4] = {d_x_, d_s_[0], d_s_[1], d_s_[2]};
     double* entry[4] = {d_lad_[0], d_lad_[1], d_lad_[2], d_lad_[3]};
     double* saved[4] = {d_lad_[4], d_lad_[5], d_lad_[6], d_lad_[7]};
@@ -850,7 +888,7 @@ void Bundle::op_ladder(int firstmode, int continuemode, std::vector<int32_t>& co
     std::vector<int32_t> conv, its;
     for (int round = 0; round < 100000; ++round) {
         std::vector<int32_t> active(NI_, 0);
-        std::vector<double> g(NI_, opt_.diag_gmin);
+        std::vector<double> g(NI_, opt_.diag_gmin), dev(NI_, opt_.gmin);
         std::vector<DevCtx> ctx = h_ctx_;
         bool live = false;
         for (int i = 0; i < NI_; ++i) {
@@ -858,11 +896,16 @@ void Bundle::op_ladder(int firstmode, int continuemode, std::vector<int32_t>& co
             active[i] = 1;
             live = true;
             ctx[i].mode = mode[i];
-            g[i] = (phase[i] == L_STEP) ? dg[i] : 0.0;  // the final solve at gshunt
+            if (device) {
+                if (phase[i] == L_STEP) dev[i] = dg[i];  // the final solve at the original gmin
+            } else {
+                g[i] = (phase[i] == L_STEP) ? dg[i] : 0.0;  // the final solve at gshunt
+            }
         }
         if (!live) break;
         set_ctx(ctx);
         set_gmin(g);
+        set_dev_gmin(dev);
         // both kinds of solve run the same iteration limit (max(itl2, 100) and
         // SimOptions::max_iter, 100 by default)
         newton(std::max(step_itl, opt_.itl1), active.data(), conv, its);
@@ -913,16 +956,141 @@ void Bundle::op_ladder(int firstmode, int continuemode, std::vector<int32_t>& co
         masked_copy_state(to_entry, entry, cur);
     }
     set_gmin(std::vector<double>(NI_, opt_.diag_gmin));
+    set_dev_gmin(std::vector<double>(NI_, opt_.gmin));
     for (int i = 0; i < NI_; ++i) {
-        if (!failing[i]) continue;
-        if (phase[i] == L_DONE) {
-            converged[i] = 1;
-            iterations[i] = total[i];
-            stage[i] = 1;
-        } else {
-            converged[i] = 0;
-        }
+        if (!failing[i] || phase[i] != L_DONE) continue;
+        converged[i] = 1;
+        iterations[i] = total[i];
+        stage[i] = device ? LADDER_TRUE_GMIN : LADDER_GMIN;
     }
+}
+
+// Source stepping (convergence.cpp:472-582) for every instance whose stage
+// is still LADDER_NONE: from zeros, a solve with every independent source
+// at zero (src_fact 0; if that fails, diagonal-gmin steps 1e10 gmin ...
+// gmin at zero sources), then src_fact raised to 1 in adaptive steps
+// (0.001 to start, x1.5 after an easy solve, /2 after a hard one, /10 and
+// a retry from the last accepted point after a failure).  base_mode is
+// the analysis's mode without its init bits.
+void Bundle::ladder_source(int base_mode, std::vector<int32_t>& converged, std::vector<int32_t>& iterations,
+                           std::vector<int32_t>& stage) {
+    std::vector<int32_t> want(NI_, 0);
+    bool any = false;
+    for (int i = 0; i < NI_; ++i) {
+        want[i] = stage[i] == LADDER_NONE;
+        any |= want[i] != 0;
+    }
+    if (!any) return;
+    double* cur[4] = {d_x_, d_s_[0], d_s_[1], d_s_[2]};
+    double* acc[4] = {d_lad_[4], d_lad_[5], d_lad_[6], d_lad_[7]};
+    masked_copy_state(want, nullptr, cur);   // zero solution, clear_state
+    const int max_iter = opt_.itl1;           // SimOptions::max_iter
+    const double min_step = 1e-7;
+    enum { S_ZERO = 0, S_ZG, S_STEP, S_DONE, S_FAILED, S_IDLE };
+    std::vector<int> phase(NI_, S_IDLE), mode(NI_, base_mode | MODEINITJCT), total(NI_, 0), zg_i(NI_, 0);
+    std::vector<int> first_its(NI_, 0);
+    std::vector<double> fraction(NI_, 0.0), step(NI_, 0.001), next(NI_, 0.0), src(NI_, 0.0), zdiag(NI_, 0.0);
+    const double zg0 = std::max(opt_.gmin, 0.0) == 0.0 ? opt_.gmin : std::max(opt_.gmin, 0.0);
+    for (int i = 0; i < NI_; ++i)
+        if (want[i]) phase[i] = S_ZERO;
+    std::vector<int32_t> conv, its;
+    for (int round = 0; round < 1000000; ++round) {
+        std::vector<int32_t> active(NI_, 0);
+        std::vector<double> g(NI_, opt_.diag_gmin);
+        std::vector<DevCtx> ctx = h_ctx_;
+        bool live = false;
+        for (int i = 0; i < NI_; ++i) {
+            if (phase[i] != S_ZERO && phase[i] != S_ZG && phase[i] != S_STEP) continue;
+            active[i] = 1;
+            live = true;
+            ctx[i].mode = mode[i];
+            ctx[i].src_fact = src[i];
+            if (phase[i] == S_ZG) g[i] = zdiag[i];
+        }
+        if (!live) break;
+        set_ctx(ctx);
+        set_gmin(g);
+        newton(max_iter, active.data(), conv, its);
+        std::vector<int32_t> save(NI_, 0), back(NI_, 0), zero(NI_, 0);
+        for (int i = 0; i < NI_; ++i) {
+            if (!active[i]) continue;
+            if (phase[i] == S_ZERO) {
+                first_its[i] = its[i];
+                if (conv[i]) {
+                    total[i] += its[i];
+                    save[i] = 1;
+                    mode[i] = base_mode | MODEINITFLOAT;
+                    phase[i] = S_STEP;
+                } else {
+                    // gmin stepping at zero sources from 1e10 x gmin
+                    double d = zg0;
+                    for (int k = 0; k < 10; ++k) d *= 10.0;
+                    zdiag[i] = d;
+                    zg_i[i] = 0;
+                    zero[i] = 1;
+                    mode[i] = base_mode | MODEINITJCT;
+                    phase[i] = S_ZG;
+                }
+                continue;
+            }
+            if (phase[i] == S_ZG) {
+                if (!conv[i]) {
+                    phase[i] = S_FAILED;
+                    continue;
+                }
+                total[i] += its[i];
+                zdiag[i] /= 10.0;
+                mode[i] = base_mode | MODEINITFLOAT;
+                if (zg_i[i]++ == 10) {
+                    // the failed zero-source solve's iterations count too
+                    total[i] += first_its[i];
+                    save[i] = 1;
+                    phase[i] = S_STEP;
+                }
+                continue;
+            }
+            // S_STEP: the solve at src[i] = next[i]
+            if (conv[i]) {
+                total[i] += its[i];
+                fraction[i] = next[i];
+                save[i] = 1;
+                if (its[i] < max_iter / 4) step[i] = std::min(0.1, step[i] * 1.5);
+                else if (its[i] > max_iter / 2) step[i] = std::max(min_step, step[i] * 0.5);
+                if (!(fraction[i] < 1.0)) phase[i] = S_DONE;
+            } else {
+                if (step[i] * (1.0 - fraction[i]) < 1e-8) {
+                    phase[i] = S_FAILED;
+                    continue;
+                }
+                back[i] = 1;
+                step[i] *= 0.1;
+                if (step[i] > 0.01) step[i] = 0.01;
+                if (step[i] < min_step) phase[i] = S_FAILED;
+            }
+        }
+        masked_copy_state(zero, nullptr, cur);
+        masked_copy_state(save, cur, acc);
+        masked_copy_state(back, acc, cur);
+        // the next solve of every stepping instance starts from its accepted point
+        for (int i = 0; i < NI_; ++i) {
+            if (phase[i] != S_STEP) continue;
+            next[i] = std::min(1.0, fraction[i] + step[i]);
+            src[i] = next[i];
+        }
+        std::vector<int32_t> from_acc(NI_, 0);
+        for (int i = 0; i < NI_; ++i) from_acc[i] = (phase[i] == S_STEP && !back[i] && !save[i]) ? 1 : 0;
+        masked_copy_state(from_acc, acc, cur);
+    }
+    set_gmin(std::vector<double>(NI_, opt_.diag_gmin));
+    std::vector<DevCtx> ctx = h_ctx_;
+    for (int i = 0; i < NI_; ++i) {
+        ctx[i].src_fact = opt_.src_fact;
+        if (!want[i] || phase[i] != S_DONE) continue;
+        converged[i] = 1;
+        iterations[i] = total[i];
+        stage[i] = LADDER_SOURCE;
+    }
+    set_ctx(ctx);
 }
 
 void Bundle::dc_op(int mode, std::vector<int32_t>& converged, std::vector<int32_t>& iterations) {

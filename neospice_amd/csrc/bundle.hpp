@@ -40,6 +40,15 @@
 
 namespace nsa {
 
+// The method of the DC ladder that converged for an instance.
+enum LadderStage : int32_t {
+    LADDER_NONE = -1,
+    LADDER_DIRECT = 0,
+    LADDER_GMIN = 1,       // gmin stepping (convergence.cpp:216-346)
+    LADDER_SOURCE = 2,     // source stepping (convergence.cpp:472-582)
+    LADDER_TRUE_GMIN = 4,  // true gmin stepping (convergence.cpp:348-470)
+};
+
 // Per-instance Newton state on the device (newton_solve's locals).
 struct NtInst {
     int32_t saved_mode;      // the caller's mode, restored on exit (newton.cpp:117, 417)
@@ -81,16 +90,21 @@ public:
     // Direct Newton attempt of the DC operating point (dc.cpp:104-107 /
     // transient.cpp:195-205) for every instance, from the current solution.
     void dc_op(int mode, std::vector<int32_t>& converged, std::vector<int32_t>& iterations);
-    // The operating-point ladder of the transient preamble
-    // (compute_dc_operating_point, transient.cpp:172-238): direct Newton in
-    // firstmode, then gmin stepping (convergence.cpp:216-346) with
-    // per-instance diag_gmin for the instances that failed.  stage[i]: 0
-    // direct, 1 gmin stepping, -1 neither converged (source stepping and
-    // pseudo-transient are not restated).
-    // skip_direct: gmin stepping for every instance, from the current solution.
+    // The operating-point ladder, per instance: direct Newton in firstmode,
+    // then for the instances that failed the continuation methods of the
+    // reference's ladder in its order -- solve_dc's (MODEDCOP, dc.cpp:
+    // 104-184): true gmin stepping, gmin stepping, source stepping; the
+    // transient preamble's (MODETRANOP, compute_dc_operating_point,
+    // transient.cpp:172-238): gmin stepping, source stepping -- each
+    // instance on its own schedule (diag_gmin, device gmin and src_fact per
+    // instance).  stage[i] is the method that converged (LadderStage).
+    // Pseudo-transient continuation, OPtran, the node-classified guess and
+    // the equilibrated retry that follow in the reference are not
+    // restated.  alone (a LadderStage other than LADDER_NONE): that method
+    // alone for every instance, without the direct attempt.
     void op_ladder(int firstmode, int continuemode, std::vector<int32_t>& converged,
                    std::vector<int32_t>& iterations, std::vector<int32_t>& stage,
-                   bool skip_direct = false);
+                   int alone = LADDER_NONE);
     const std::vector<int32_t>& dc_stage() const { return dc_stage_; }
     // A fresh solver for every instance (make_solver + symbolic).
     void reset_solver();
@@ -171,6 +185,11 @@ private:
     void host_lu_post(std::vector<NtInst>& nt, const std::vector<int>& repivot, bool others);
     void collect_timing();
     void set_gmin(const std::vector<double>& g);
+    void set_dev_gmin(const std::vector<double>& g);
+    void ladder_gmin(int firstmode, int continuemode, bool device, std::vector<int32_t>& converged,
+                     std::vector<int32_t>& iterations, std::vector<int32_t>& stage);
+    void ladder_source(int base_mode, std::vector<int32_t>& converged, std::vector<int32_t>& iterations,
+                       std::vector<int32_t>& stage);
     void masked_copy_state(const std::vector<int32_t>& mask, double* const* src, double* const* dst);
     std::vector<double> instance_matrix(int i);
     void upload_inst();
@@ -255,6 +274,7 @@ private:
     int32_t* d_slot_inst_ = nullptr;
     int32_t* d_xinst_ = nullptr;   // owner instance of every unknown
     double* d_gmin_ = nullptr;     // diag_gmin per instance
+    double* d_devgmin_ = nullptr;  // device gmin per instance (true gmin stepping)
     double* h_gmin_pin_ = nullptr;
     int32_t* d_mask_ = nullptr;
     int32_t* h_mask_pin_ = nullptr;

@@ -707,7 +707,13 @@ int nsa_bundle_dc_ladder(nsa_bundle* b, int32_t flags, int32_t* converged, int32
         std::vector<int32_t> c, it, st;
         // MODETRANOP (solve_transient's preamble) or MODEDCOP (solve_dc), | MODEINITJCT / MODEINITFLOAT
         const int op = (flags & 2) ? 0x10 : 0x20;
-        b->b->op_ladder(op | 0x200, op | 0x100, c, it, st, (flags & 1) != 0);
+        // bit 0: one method alone (bits 4-5: 0 gmin, 1 source, 2 true gmin stepping)
+        int alone = nsa::LADDER_NONE;
+        if (flags & 1) {
+            const int m = (flags >> 4) & 3;
+            alone = m == 0 ? nsa::LADDER_GMIN : m == 1 ? nsa::LADDER_SOURCE : nsa::LADDER_TRUE_GMIN;
+        }
+        b->b->op_ladder(op | 0x200, op | 0x100, c, it, st, alone);
         const int n = b->b->size();
         if (converged) std::memcpy(converged, c.data(), sizeof(int32_t) * n);
         if (iterations) std::memcpy(iterations, it.data(), sizeof(int32_t) * n);

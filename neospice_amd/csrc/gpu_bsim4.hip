@@ -211,8 +211,8 @@ __device__ __forceinline__ void b4_lane(const B4Table& t, const DevCtx* __restri
     const T vbs_jct = vbs, vbd_jct = vbd;
 
     // ---- junction currents: zero saturation current, gmin only (:682-795)
-    const T gbs = tol.gmin, cbs = gbs * vbs_jct;
-    const T gbd = tol.gmin, cbd = gbd * vbd_jct;
+    const T gbs = dev_gmin(tol, ci), cbs = gbs * vbs_jct;
+    const T gbd = dev_gmin(tol, ci), cbd = gbd * vbd_jct;
 
     int dmode;
     T Vds, Vbs;

@@ -138,7 +138,15 @@ struct B4Table {
 // Tolerances a device load reads from SimOptions.
 struct DevTol {
     double gmin, reltol, abstol;
+    // per-instance device gmin (SimOptions::gmin as true gmin stepping
+    // publishes it, convergence.cpp:383-386), or null: gmin for all
+    const double* gmin_inst = nullptr;
 };
+
+// The device gmin an instance's loads use.
+__host__ __device__ inline double dev_gmin(const DevTol& t, int inst) {
+    return t.gmin_inst ? t.gmin_inst[inst] : t.gmin;
+}
 
 void launch_res_sites(int count, const double* ohms, double* contrib, int site, int B,
                       hipStream_t s);

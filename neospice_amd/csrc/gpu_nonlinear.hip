@@ -210,7 +210,7 @@ __global__ void k_diode(DioTable t, const DevCtx* __restrict__ ctx, const double
     }
     double cd = cdb + cdsw;
     double gd = gdb + gdsw;
-    const double gmin = tol.gmin;
+    const double gmin = dev_gmin(tol, ci);
     if (vd >= -3 * vte) {
         if ((fl & DIO_IKF) && (cd > 1.0e-18)) {
             const double ik = p.ikf;
@@ -342,7 +342,7 @@ __global__ void k_mos1(Mos1Table t, const DevCtx* __restrict__ ctx, const double
               nB = t.nodes[4 * d + 3];
     const double type = p.type;
     const double vt = p.vt;
-    const double gmin = tol.gmin;
+    const double gmin = dev_gmin(tol, ci);
     int Check = 1;
     double vbs, vgs, vds, vbd, vgd, vgb, von, vdsat;
     double xfact = 0.0;

@@ -236,6 +236,7 @@ int ref_ckt_option(void* p, const char* key, double v) {
     else if (k == "itl4") o.itl4 = (int)v;
     else if (k == "itl1") o.itl1 = (int)v;
     else if (k == "method") o.method = (v != 0.0) ? "gear" : "trap";
+    else if (k == "verbose") o.verbose = v != 0.0;
     else { g_err = "unknown option " + k; return -1; }
     return 0;
 }
@@ -851,8 +852,13 @@ int ref_dc_ladder_op(void* p, double* solution, int* stage_out, int op) {
         *stage_out = -1;
         ckt.integrator_ctx.mode = jct;
         NewtonResult r = newton_solve(ckt, *solver, sol, ckt.options);
+        // solve_dc tries true gmin stepping before gmin stepping (dc.cpp:120-145)
+        if (!r.converged && op == 0x10) {
+            r = true_gmin_stepping(ckt, *solver, sol, ckt.options, jct, flt);
+            if (r.converged) *stage_out = 4;
+        }
         if (r.converged) {
-            *stage_out = 0;
+            if (*stage_out < 0) *stage_out = 0;
         } else {
             r = gmin_stepping(ckt, *solver, sol, ckt.options, jct, flt);
             if (r.converged) {
@@ -880,6 +886,28 @@ int ref_dc_ladder_op(void* p, double* solution, int* stage_out, int op) {
 // Returns its iterations, negative when it did not converge.
 int ref_dc_ladder(void* p, double* solution, int* stage_out) {
     return ref_dc_ladder_op(p, solution, stage_out, 0x20);
+}
+
+// One continuation method alone from the given solution on a fresh
+// solver, in MODETRANOP (op 0x20) or MODEDCOP (0x10): 1 source stepping
+// (convergence.cpp:472-582), 2 true gmin stepping (:348-470).  Returns its
+// iterations, negative when it did not converge.
+int ref_ladder_method(void* p, double* solution, int method, int op) {
+    auto* h = static_cast<RefCkt*>(p);
+    return guarded([&] {
+        Circuit& ckt = h->ckt;
+        ckt.finalize_if_needed();
+        ckt.integrator_ctx.options = &ckt.options;
+        auto solver = make_solver(ckt.num_vars(), ckt.is_linear());
+        solver->symbolic(ckt.pattern());
+        std::vector<double> sol(solution, solution + ckt.num_vars());
+        NewtonResult r;
+        ckt.integrator_ctx.mode = op | 0x200;
+        if (method == 1) r = source_stepping(ckt, *solver, sol, ckt.options);
+        else r = true_gmin_stepping(ckt, *solver, sol, ckt.options, op | 0x200, op | 0x100);
+        std::copy(sol.begin(), sol.end(), solution);
+        return r.converged ? r.iterations : -r.iterations;
+    });
 }
 
 int ref_gmin_stepping(void* p, double* solution) {

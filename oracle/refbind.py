@@ -104,6 +104,7 @@ def _bind(L):
         "ref_solver_free": (None, [VP]),
         "ref_solver_numeric": (C.c_int, [VP, P_F64, C.c_double]),
         "ref_solver_set_equilibrate": (None, [VP, C.c_int]),
+        "ref_ladder_method": (C.c_int, [VP, P_F64, C.c_int, C.c_int]),
         "ref_solver_refactorize": (C.c_int, [VP, P_F64, C.c_double]),
         "ref_solver_solve": (C.c_int, [VP, P_F64]),
         "ref_solver_solve_transposed": (C.c_int, [VP, P_F64]),
@@ -334,6 +335,15 @@ class RefCircuit:
         its = _chk(self.L.ref_dc_ladder_op(self.h, x.ctypes.data_as(P_F64), C.byref(st),
                                            0x10 if dcop else 0x20))
         return x, its, st.value
+
+    def ladder_method(self, method, dcop=False):
+        """source stepping ("source") or true gmin stepping ("true_gmin")
+        alone from zeros on a fresh solver; returns (solution, iterations),
+        iterations negative when it failed."""
+        x = np.zeros(self.n)
+        its = self.L.ref_ladder_method(self.h, x.ctypes.data_as(P_F64), {"source": 1, "true_gmin": 2}[method],
+                                       0x10 if dcop else 0x20)
+        return x, its
 
     def gmin_stepping(self):
         """gmin_stepping alone from zeros on a fresh solver; returns

@@ -97,6 +97,33 @@ def test_bundle_gmin_stepping():
         assert (np.abs(b.solution(i) - x_ref) <= tol).all(), i
 
 
+@pytest.mark.parametrize("method,dcop", [("source", False), ("source", True), ("true_gmin", True)])
+def test_bundle_ladder_methods(method, dcop):
+    """Source stepping (convergence.cpp:472-582) and true gmin stepping
+    (:348-470) alone for every instance of a bundle of rings and an
+    inverter chain, each on its own schedule (src_fact, device gmin):
+    iteration counts (which fix the schedules) and operating points against
+    the reference's own methods."""
+    cks, st = W.c3_rings(stages=[5, 7, 9, 11])
+    chain, _ = W.inverter_chain(4)
+    cks.append(chain)
+    for c in cks:
+        c.finalize()
+    b = na.Bundle(cks)
+    conv, its, stage = b.op_ladder(skip_direct=True, dcop=dcop, method=method)
+    code = {"source": 2, "true_gmin": 4}[method]
+    for i, c in enumerate(cks):
+        x_ref, it_ref = RB.RefCircuit(c, probe=False).ladder_method(method, dcop=dcop)
+        print(method, "instance", i, "n", c.n, "iterations", its[i], it_ref, flush=True)
+        assert (it_ref > 0) == bool(conv[i]), (i, it_ref, conv[i])
+        if it_ref <= 0:
+            continue
+        assert stage[i] == code
+        assert its[i] == it_ref, (i, its[i], it_ref)
+        tol = DC_RTOL * np.maximum(np.abs(x_ref), 1e-3) + 1e-12
+        assert (np.abs(b.solution(i) - x_ref) <= tol).all(), (i, float(np.max(np.abs(b.solution(i) - x_ref))))
+
+
 def test_bundle_op_ladder_long_rings():
     """Rings of the uneven-size batch whose direct Newton fails in the
     reference (95-125 stages): the bundle's operating-point ladder reaches
