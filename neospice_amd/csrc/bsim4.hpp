@@ -67,7 +67,9 @@ B4Param bsim4_param(const ModelCard& card, const Device& d, const DeviceEnv& env
 void bsim4_param_derivs(const ModelCard& card, const Device& d, const DeviceEnv& env, double* val,
                         double* dw, double* dl);
 
-// Whether a model card selects BSIM4v7 (LEVEL=14, mosfet_common.cpp:479-481).
-inline bool is_bsim4_level(int level) { return level == 14; }
+// Whether a model card selects BSIM4v7: LEVEL=14, as the reference's
+// registry has it (mosfet_common.cpp:479-481, bsim4v7_factory.cpp:8-10), or
+// ngspice's LEVEL=54 for the same model.
+inline bool is_bsim4_level(int level) { return level == 14 || level == 54; }
 
 }  // namespace nsa

@@ -342,17 +342,22 @@ void BatchEngine::set_model_values(int dev, const std::string& param, const doub
     }
     std::string key = param;
     for (auto& ch : key) ch = (char)std::tolower((unsigned char)ch);
+    // every instance's card is built from a copy of its overrides; they are
+    // kept only when no instance's card is refused (an unsupported parameter
+    // or value), so a refused call leaves the device as it was
     std::vector<double> col(per_dev);
+    std::vector<std::map<std::string, double>> over = b4_over_[d.family_index];
     for (int i = 0; i < B_; ++i) {
-        b4_over_[d.family_index][i][key] = vals[i];
+        over[i][key] = vals[i];
         ModelCard card = ckt_.models().at(d.model);
-        for (const auto& kv : b4_over_[d.family_index][i]) card.p[kv.first] = kv.second;
+        for (const auto& kv : over[i]) card.p[kv.first] = kv.second;
         const B4Param p = bsim4_param(card, d, env_);
         const double* f = reinterpret_cast<const double*>(&p);
         for (int j = 0; j < kB4Fields; ++j) col[(size_t)j * B_ + i] = f[j];
     }
     NSA_HIP(hipMemcpy(d_b4_inst_ + (size_t)d.family_index * per_dev, col.data(),
                       per_dev * sizeof(double), hipMemcpyHostToDevice));
+    b4_over_[d.family_index] = std::move(over);
 }
 
 void BatchEngine::set_ctx(const DevCtx& c) {

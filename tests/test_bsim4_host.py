@@ -136,3 +136,14 @@ def test_size_derivatives(card):
             assert dw[q] == 0.0 and dl[q] == 0.0, nm
         # leff = L, weff = W here (no offsets): unit derivatives
         assert dl[NAMES.index("leff")] == 1.0 and dw[NAMES.index("weff")] == 1.0
+
+
+@pytest.mark.parametrize("card", CARDS)
+def test_level54_alias(card):
+    """ngspice's LEVEL=54 names the same BSIM4v7 model as the reference's
+    LEVEL=14: the same constants, bit for bit."""
+    c14 = one_device(card, 0.5e-6, 0.05e-6)
+    c54 = one_device(card.replace("level=14", "level=54"), 0.5e-6, 0.05e-6)
+    _, a = params(c14, 2)
+    _, b = params(c54, 2)
+    assert np.array_equal(a, b)

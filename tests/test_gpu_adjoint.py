@@ -254,17 +254,23 @@ def test_adjoint_linear_complex_step(which):
         np.testing.assert_allclose(g[f][:, 0], cs, rtol=1e-9, atol=1e-11 * scale, err_msg=f)
 
 
-def test_dc_adjoint_bsim4_central_fd():
+@pytest.mark.parametrize("vin", [0.42, 0.36])
+def test_dc_adjoint_bsim4_central_fd(vin):
     """BSIM4v7 inverters with resistive loads at a DC bias: dL/dR and dL/dV
-    through the BSIM4v7 Jacobian (its load at the converged solution,
-    unlimited) against central differences."""
+    through the BSIM4v7 Jacobian against central differences.  The
+    transposed solve uses the factors of the last Newton iteration, whose
+    load ran at the returned iterate with junction and FET limiting in
+    force (newton.cpp:378-381); at a converged point the limiting leaves the
+    voltages alone, so those are the Jacobian at the solution.  The second
+    bias sits just below threshold, where the limiting acts most during the
+    solve."""
     B = 2
     c = na.Circuit()
     c.model(".model nb nmos level=14 toxe=1.8e-9 vth0=0.4")
     c.model(".model pb pmos level=14 toxe=1.8e-9 vth0=-0.4")
     vdd, nin = c.node("vdd"), c.node("in")
     c.V("vsup", vdd, na.GND, 1.0)
-    c.V("vin", nin, na.GND, 0.42)
+    c.V("vin", nin, na.GND, vin)
     drive = nin
     rng = np.random.default_rng(4)
     for k in range(3):

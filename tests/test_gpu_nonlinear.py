@@ -254,3 +254,10 @@ def test_bsim4_per_instance_cards():
         assert (np.abs(x[:, i] - x_ref) <= tol).all(), (i, np.max(np.abs(x[:, i] - x_ref)))
     # the instances differ from each other
     assert np.abs(x[:, 0] - x[:, 1]).max() > 1e-9
+    # a refused card value (an unsupported parameter) leaves every instance's
+    # card as it was: the same operating point again
+    from neospice_amd._lib import NsaError
+    with pytest.raises(NsaError):
+        eng.set_model_values(mdevs[0], "capmod", np.full(B, 1.0))
+    x2, its2 = eng.dc_op()
+    assert np.array_equal(x2, x) and np.array_equal(its2, its)
