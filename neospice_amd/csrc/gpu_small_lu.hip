@@ -19,6 +19,7 @@
 // level's first descriptor is fetched while the previous level finishes.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "common.hpp"
@@ -31,6 +32,17 @@ namespace {
 
 constexpr int kW = 64;
 constexpr size_t kLdsLimit = 64 * 1024;
+
+// The end of a level.  A workgroup is one wave, and a wave's LDS accesses
+// are performed in the order it issues them, so the next level's loads see
+// this level's stores without waiting for the stores to complete: only the
+// compiler must keep the accesses on their side (a workgroup barrier would
+// also wait for every outstanding LDS access).
+__device__ inline void level_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 __device__ inline int wave_min_i32(int v) {
     for (int o = kW / 2; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
@@ -157,7 +169,7 @@ __device__ inline void program_level(const PlanView<STAGED>& pv, const int32_t* 
         run_row<OP, STAGED>(pv, rec, lds_ld(L, rec.t), lds_ld(L, rec.p0), lds_ld(L, rec.p1),
                             lds_ld(L, rec.dp), L);
     }
-    __syncthreads();
+    level_sync();
     r0 = r1;
     r1 = r2;
     r2 = r3;
@@ -256,7 +268,7 @@ __device__ inline void run_serial(const PlanView<STAGED>& pv, const int32_t* row
         if (r < re) serial_row<OP, STAGED>(pv, rows, r, re, A, B, C, tprev, prev, L);
         if (r + 1 < re) serial_row<OP, STAGED>(pv, rows, r + 1, re, B, C, A, tprev, prev, L);
     }
-    __syncthreads();
+    level_sync();
 }
 
 // Program `prog` phase by phase.
@@ -276,28 +288,44 @@ __device__ inline void run_program(const PlanView<STAGED>& pv, int prog, char* L
     }
 }
 
-// One task: x_t = x_t - (x_kj * (1 / x_kk)) * x_ik.  A forward-substitution
-// task (pivot stored as -(address + 1)) is skipped when its y_k is zero, as
-// the forward loop skips a zero entry (solve.cpp:22-23).
-__device__ inline void run_task(const int4 tk, char* L) {
+// One task: x_t = x_t - (x_kj * (1 / x_kk)) * x_ik, the result stored at
+// `dst`.  A forward-substitution task (pivot stored as -(address + 1)) is
+// skipped when its y_k is zero, as the forward loop skips a zero entry
+// (solve.cpp:22-23).
+__device__ inline void run_task(const int4 tk, char* L, int dst, const int4& ahead, int far_w) {
     const bool fwd = tk.w < 0;
     const int piv = fwd ? -tk.w - 1 : tk.w;
     const double xkk = lds_ld(L, piv), xkj = lds_ld(L, tk.y), xik = lds_ld(L, tk.z), xt = lds_ld(L, tk.x);
-    const double u = xkj * (1.0 / xkk);
-    const double nv = xt - u * xik;
-    lds_st(L, tk.x, (fwd && xkj == 0.0) ? xt : nv);
+    double r = 1.0 / xkk;
+    // what this level fetched for the next ones before its own operands --
+    // the next level's task (a level earlier) and the pivot word of the one
+    // after -- is in registers by now; saying so (once the reciprocal is
+    // done, so the division does not wait behind the claim) keeps the next
+    // level from waiting for this task's store to complete before it can
+    // use them
+    asm volatile("" : "+v"(r) : "v"(ahead.x), "v"(ahead.y), "v"(ahead.z), "v"(ahead.w), "v"(far_w));
+    const double nv = xt - (xkj * r) * xik;
+    lds_st(L, dst, (fwd && xkj == 0.0) ? xt : nv);
 }
 
-// Level l of the refactor: the level's tasks, then the next level's first
-// task (alternated like RowAhead).
-__device__ inline void factor_level(const int32_t* lev, const int4* task, int nlev, int last, int l,
-                                    int& t0, int& t1, int& t2, const int4& cur, int4& nxt, char* L,
-                                    int lane) {
-    const int t3 = (l + 3 <= nlev) ? lev[l + 3] : t2;
-    nxt = task[min(t1 + lane, last)];
-    if (t0 + lane < t1) run_task(cur, L);
-    for (int t = t0 + lane + kW; t < t1; t += kW) run_task(task[t], L);
-    __syncthreads();
+// Level l of the refactor: the level's tasks, and the fetch of the first
+// task of level l+2 (`far`; `cur` and `nxt` were fetched two levels and one
+// level ahead, the callers rotate three variables so that no fetched
+// register is ever copied).  Every lane runs a first task, the lanes past
+// the level's end the clamped one with its result sent to `junk`, so that
+// the common one-pass level has no branch.  t0..t2 are the task offsets of
+// levels l..l+2.
+__device__ inline void factor_level(const int32_t* lev, const int4* task, int last, int l, int& t0,
+                                    int& t1, int& t2, const int4& cur, const int4& nxt, int4& far, char* L,
+                                    int lane, int junk) {
+    const int t3 = lev[l + 3];
+    far = task[min(t2 + lane, last)];
+    run_task(cur, L, (t0 + lane < t1) ? cur.x : junk, nxt, far.w);
+    for (int t = t0 + lane + kW; t < t1; t += kW) {
+        const int4 tk = task[t];
+        run_task(tk, L, tk.x, nxt, far.w);
+    }
+    level_sync();
     t0 = t1;
     t1 = t2;
     t2 = t3;
@@ -339,8 +367,8 @@ __global__ __launch_bounds__(kW) void k_small_lu(const int32_t* __restrict__ wor
     const int n = P[SH_N], nlu = P[SH_NLU];
     // LDS: the factor values, the vector right after them (the forward
     // substitution's tasks address entry i as value nlu + i), a zero (plan
-    // records name it for absent terms), the reciprocal pivots, then the
-    // staged hot block on a 16-byte boundary
+    // records name it for absent terms), the reciprocal pivots (at least one
+    // cell per lane), then the staged hot block on a 16-byte boundary
     double* v = lds;
     double* y = lds + nlu;
     char* L = reinterpret_cast<char*>(lds);
@@ -350,7 +378,7 @@ __global__ __launch_bounds__(kW) void k_small_lu(const int32_t* __restrict__ wor
     if (STAGED) {
         // the hot block, 16 bytes per lane per step (sections start on
         // 4-word boundaries, so the block is whole int4s)
-        int32_t* hw = reinterpret_cast<int32_t*>(lds + ((max_nlu + 2 * max_n + 2) & ~1));
+        int32_t* hw = reinterpret_cast<int32_t*>(lds + ((max_nlu + max_n + 2 + max(max_n, kW)) & ~1));
         const int4* src = reinterpret_cast<const int4*>(P + pv.hot0);
         int4* dst = reinterpret_cast<int4*>(hw);
         const int n4 = (P[SH_HOTLEN] + 3) / 4;
@@ -400,23 +428,29 @@ __global__ __launch_bounds__(kW) void k_small_lu(const int32_t* __restrict__ wor
             }
             __syncthreads();
         }
-        // right-looking steps, level by level; as in run_program, each
-        // level's first task is fetched one level ahead at a clamped index
+        // right-looking steps, level by level; each level's first task is
+        // fetched two levels ahead at a clamped index
         const int nlev = P[SH_NLEV];
         const int ntask = P[SH_NTASK];
         if (nlev > 0 && ntask > 0) {
             const int32_t* lev = pv.at(P[SH_LEV]);
             const int4* task = reinterpret_cast<const int4*>(pv.at(P[SH_TASK]));
             const int last = ntask - 1;
-            int t0 = lev[0], t1 = lev[1];
-            int t2 = (nlev > 1) ? lev[2] : t1;
-            int4 ta = task[min(t0 + lane, last)], tb;
+            int t0 = lev[0], t1 = lev[1], t2 = lev[2];
+            int4 ta = task[min(t0 + lane, last)];
+            int4 tb = task[min(t1 + lane, last)], tc;
+            // results of the lanes without a task: a reciprocal pivot cell
+            // of their own (one address for all would serialize the store),
+            // written after the factor
+            const int junk = (int)(reinterpret_cast<char*>(dr + lane) - L);
             int l = 0;
-            for (; l + 1 < nlev; l += 2) {
-                factor_level(lev, task, nlev, last, l, t0, t1, t2, ta, tb, L, lane);
-                factor_level(lev, task, nlev, last, l + 1, t0, t1, t2, tb, ta, L, lane);
+            for (; l + 2 < nlev; l += 3) {
+                factor_level(lev, task, last, l, t0, t1, t2, ta, tb, tc, L, lane, junk);
+                factor_level(lev, task, last, l + 1, t0, t1, t2, tb, tc, ta, L, lane, junk);
+                factor_level(lev, task, last, l + 2, t0, t1, t2, tc, ta, tb, L, lane, junk);
             }
-            if (l < nlev) factor_level(lev, task, nlev, last, l, t0, t1, t2, ta, tb, L, lane);
+            if (l < nlev) factor_level(lev, task, last, l, t0, t1, t2, ta, tb, tc, L, lane, junk);
+            if (l + 1 < nlev) factor_level(lev, task, last, l + 1, t0, t1, t2, tb, tc, ta, L, lane, junk);
         }
         // checks (the first failing column, as the reference stops there)
         int bad = 0x7fffffff, code = FAIL_NONE;
@@ -518,8 +552,9 @@ void launch_small_lu(const SmallLuPlans& plans, const SluInst* d_inst, int ninst
     const unsigned grid = (unsigned)(ninst + side_blocks);
     const int max_nlu = plans.max_nlu(), max_n = plans.max_n();
     // the hot block starts on a 16-byte boundary after the values, the
-    // vector, the zero and the reciprocal pivots
-    const size_t base = sizeof(double) * (((size_t)max_nlu + 2 * (size_t)max_n + 2) & ~(size_t)1);
+    // vector, the zero and the reciprocal pivots (at least one per lane)
+    const size_t base =
+        sizeof(double) * (((size_t)max_nlu + max_n + 2 + std::max(max_n, kW)) & ~(size_t)1);
     const size_t hot = sizeof(int32_t) * (((size_t)plans.max_hot() + 3) / 4 * 4);
     const int stage = (base + hot <= kLdsLimit) ? 1 : 0;
     const size_t lds = base + (stage ? hot : 0);

@@ -201,8 +201,12 @@ int SmallLuPlans::add(const LuStructure& s) {
     // backward program), contiguous so that a launch can stage it in LDS
     std::vector<int32_t> lev_ptr, tasks;
     build_tasks(s, lev_ptr, tasks, true);
-    H(SH_NLEV) = (int32_t)lev_ptr.size() - 1;
+    const int nlev = (int)lev_ptr.size() - 1;
+    H(SH_NLEV) = nlev;
     H(SH_NTASK) = (int32_t)(tasks.size() / 4);
+    // three copies of the end, so that a level reads the word three levels
+    // ahead without a bounds test
+    lev_ptr.insert(lev_ptr.end(), 3, lev_ptr.back());
     align4(words_);
     const int32_t hot0 = (int32_t)words_.size() - base;
     put(SH_LEV, lev_ptr);

@@ -48,7 +48,8 @@ enum SluHdr : int32_t {
     SH_N = 0, SH_NLU, SH_NNZA, SH_NLEV, SH_NTASK,
     SH_APOS,    // nnz_a: position receiving A slot p
     SH_GMIN,    // n: position receiving diag_gmin, or -1
-    SH_LEV,     // nlev+1 task offsets (relative to SH_TASK, in tasks)
+    SH_LEV,     // nlev+1 task offsets (relative to SH_TASK, in tasks) and three
+                // more copies of the last
     SH_TASK,    // tasks, 4 words each, LDS byte addresses (value p at 8p, vector entry
                 // i at 8 (nlu + i)): target, U(k,j), L(i,k), pivot (k,k); forward-
                 // substitution tasks: y_i, y_k, L(i,k), -(pivot address + 1)
