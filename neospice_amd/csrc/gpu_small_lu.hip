@@ -44,6 +44,32 @@ __device__ inline void level_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Elements [0, n) of a lane-strided loop, U per lane at a time: all U
+// loads are issued before any is used, so a batch costs one memory round
+// trip (a plain strided loop waits for each element's load in turn).
+template <int U, class Load, class Use>
+__device__ inline void batched(int n, int lane, Load ld, Use use) {
+    for (int k0 = lane; k0 < n; k0 += U * kW) {
+        decltype(ld(0)) val[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (k0 + u * kW < n) val[u] = ld(k0 + u * kW);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (k0 + u * kW < n) use(k0 + u * kW, val[u]);
+    }
+}
+
+struct PosVal {
+    int q;
+    double w;
+};
+
+__device__ inline int wave_max_i32(int v) {
+    for (int o = kW / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
 __device__ inline int wave_min_i32(int v) {
     for (int o = kW / 2; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
     return v;
@@ -201,74 +227,102 @@ __device__ inline void run_levels(const PlanView<STAGED>& pv, const int32_t* lev
     if (l < nl) program_level<OP, STAGED>(pv, lev, rows, nl, last, l, r0, r1, r2, A, B, L, lane);
 }
 
-// A row of a serial phase with its operands.
-struct SerialRow {
-    RowRec rec;
-    double yt, a0, a1, dv, b0, b1;
-};
+__device__ inline int readlane_i32(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 
-// A row index the compiler must treat as per-lane: loads through it stay in
-// vector registers.  (A wave-uniform index would let it move the loaded
-// record into scalar registers right after the load, which waits for the
-// load and undoes the prefetch.)
-__device__ inline int lane_index(int r) {
-    asm volatile("" : "+v"(r));
-    return r;
+__device__ inline double readlane_f64(double v, int lane) {
+    const unsigned long long b = __double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)b, lane);
+    const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-__device__ inline void serial_fetch(SerialRow& s, const char* L) {
-    s.yt = lds_ld(L, s.rec.t);
-    s.a0 = lds_ld(L, s.rec.p0);
-    s.a1 = lds_ld(L, s.rec.p1);
-    s.dv = lds_ld(L, s.rec.dp);
-    s.b0 = lds_ld(L, s.rec.s0);
-    s.b1 = lds_ld(L, s.rec.s1);
+// v of lane i - 1 in lane i (lane 0: 0), a DPP wave shift.
+__device__ inline double shift_up_f64(double v) {
+    const unsigned long long b = __double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, 0x138, 0xf, 0xf, false);
+    const unsigned hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), 0x138, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// Row r of a serial phase [r, re): fetches the record of row r + 2 and
-// the operands of row r + 1, then computes row r.  Operands fetched one
-// row ahead were read before row r's result was written, so a source equal
-// to the previous row's target takes the value passed on in registers;
-// every older target was written before the fetch (a wave's LDS accesses
-// complete in order).  Callers rotate three SerialRow variables, so that
-// no register with a load in flight is copied.
+// Rows [rb, re) of a serial phase (a chain of one-row levels), in batches
+// of 64 rows, row rb' + j in lane j: every lane loads its row's record and
+// operands -- each source read where it is final before the batch (the
+// phase's earlier batches stored theirs) -- and a term whose source a row
+// of the same batch produces (srow) takes that row's result from
+// registers.
+//
+// When every such source is the row just before or one common row of the
+// batch (a hub), and no row has terms beyond the first two, the batch runs
+// as a wave-wide recurrence: each step every lane evaluates its row from
+// its own operands, the previous lane's value of the step before (a DPP
+// shift) and, once the hub's step is done, the hub's value (broadcast
+// once), so lane j's value is final from step j on and 64 steps finish the
+// batch without a branch or an LDS access.  Otherwise the rows run one after the
+// other, each taking earlier rows' results from their lanes and storing its
+// own (terms beyond the first two are read from LDS when their row runs).
 template <int OP, bool STAGED>
-__device__ inline void serial_row(const PlanView<STAGED>& pv, const int32_t* rows, int r, int re,
-                                  const SerialRow& cur, SerialRow& nxt, SerialRow& far, int& tprev,
-                                  double& prev, char* L) {
-    far.rec = load_row(rows, lane_index(min(r + 2, re - 1)));
-    serial_fetch(nxt, L);
-    const double b0 = (cur.rec.s0 == tprev) ? prev : cur.b0;
-    const double b1 = (cur.rec.s1 == tprev) ? prev : cur.b1;
-    double acc = term<OP>(term<OP>(cur.yt, cur.a0, b0), cur.a1, b1);
-    if (cur.rec.ntail > 0) acc = row_tail<OP>(acc, pv.at(cur.rec.toff), cur.rec.ntail, L);
-    acc = row_diag<OP>(acc, cur.dv);
-    lds_st(L, cur.rec.t, acc);
-    tprev = cur.rec.t;
-    prev = acc;
-}
-
-// Rows [rb, re) in order on lane 0 (a chain of one-row levels).
-template <int OP, bool STAGED>
-__device__ inline void run_serial(const PlanView<STAGED>& pv, const int32_t* rows, int rb, int re, char* L,
-                                  int lane) {
-    if (lane == 0 && rb < re) {
-        SerialRow A, B, C;
-        A.rec = load_row(rows, lane_index(rb));
-        B.rec = load_row(rows, lane_index(min(rb + 1, re - 1)));
-        serial_fetch(A, L);
-        int tprev = -1;
-        double prev = 0.0;
-        int r = rb;
-        for (; r + 3 <= re; r += 3) {
-            serial_row<OP, STAGED>(pv, rows, r, re, A, B, C, tprev, prev, L);
-            serial_row<OP, STAGED>(pv, rows, r + 1, re, B, C, A, tprev, prev, L);
-            serial_row<OP, STAGED>(pv, rows, r + 2, re, C, A, B, tprev, prev, L);
+__device__ inline void run_serial(const PlanView<STAGED>& pv, const int32_t* rows, const int32_t* srow, int rb,
+                                  int re, char* L, int lane) {
+    for (int r0 = rb; r0 < re; r0 += kW) {
+        const int m = min(kW, re - r0);
+        const int my = min(r0 + lane, re - 1);
+        const RowRec rec = load_row(rows, my);
+        const int sr = srow[my];
+        const double yt = lds_ld(L, rec.t), a0 = lds_ld(L, rec.p0), a1 = lds_ld(L, rec.p1);
+        const double dv = lds_ld(L, rec.dp), b0 = lds_ld(L, rec.s0), b1 = lds_ld(L, rec.s1);
+        // the batch's loads are complete before its rows run: said here, so
+        // that no row waits for the stores of the rows before it
+        asm volatile("" ::"v"(yt), "v"(a0), "v"(a1), "v"(dv), "v"(b0), "v"(b1), "v"(sr), "v"(rec.t),
+                     "v"(rec.ntail), "v"(rec.toff));
+        // producing rows relative to the batch (< 0: final before it)
+        const int q0 = (sr & 0xffff) - 1 - r0, q1 = (sr >> 16) - 1 - r0;
+        const bool adj0 = q0 >= 0 && q0 == lane - 1, adj1 = q1 >= 0 && q1 == lane - 1;
+        // one other row of the batch (a hub) may feed any number of rows
+        const bool hub0 = lane < m && q0 >= 0 && !adj0, hub1 = lane < m && q1 >= 0 && !adj1;
+        const int hub = wave_max_i32(hub0 ? q0 : hub1 ? q1 : -1);
+        const bool other = lane < m && ((hub0 && q0 != hub) || (hub1 && q1 != hub) || rec.ntail > 0);
+        if (__ballot(other) == 0) {
+            // steps up to the hub's own (its consumers are not final before
+            // it), then the rest with the hub's value broadcast once
+            double acc = 0.0, up = 0.0, x0 = b0, x1 = b1;
+            const int hend = min(hub + 1, m);
+            int j = 0;
+#pragma unroll 4
+            for (; j < hend; ++j) {
+                acc = row_diag<OP>(term<OP>(term<OP>(yt, a0, adj0 ? up : x0), a1, adj1 ? up : x1), dv);
+                up = shift_up_f64(acc);
+            }
+            if (hub >= 0) {
+                const double hv = readlane_f64(acc, hub);
+                x0 = hub0 ? hv : b0;
+                x1 = hub1 ? hv : b1;
+            }
+#pragma unroll 4
+            for (; j < m; ++j) {
+                acc = row_diag<OP>(term<OP>(term<OP>(yt, a0, adj0 ? up : x0), a1, adj1 ? up : x1), dv);
+                up = shift_up_f64(acc);
+            }
+            if (lane < m) lds_st(L, rec.t, acc);
+        } else {
+            double res = 0.0, prev = 0.0;
+            for (int j = 0; j < m; ++j) {
+                const int srj = readlane_i32(sr, j);
+                const int Q0 = (srj & 0xffff) - 1 - r0, Q1 = (srj >> 16) - 1 - r0;
+                double B0 = b0, B1 = b1;
+                if (Q0 >= 0) B0 = (Q0 == j - 1) ? prev : readlane_f64(res, Q0);
+                if (Q1 >= 0) B1 = (Q1 == j - 1) ? prev : readlane_f64(res, Q1);
+                double acc = term<OP>(term<OP>(yt, a0, B0), a1, B1);
+                if (lane == j) {
+                    if (rec.ntail > 0) acc = row_tail<OP>(acc, pv.at(rec.toff), rec.ntail, L);
+                    acc = row_diag<OP>(acc, dv);
+                    lds_st(L, rec.t, acc);
+                    res = acc;
+                }
+                prev = readlane_f64(acc, j);
+            }
         }
-        if (r < re) serial_row<OP, STAGED>(pv, rows, r, re, A, B, C, tprev, prev, L);
-        if (r + 1 < re) serial_row<OP, STAGED>(pv, rows, r + 1, re, B, C, A, tprev, prev, L);
+        level_sync();
     }
-    level_sync();
 }
 
 // Program `prog` phase by phase.
@@ -281,9 +335,10 @@ __device__ inline void run_program(const PlanView<STAGED>& pv, int prog, char* L
     const int32_t* rows = pv.at(P[SH_PROG + 4 * prog + 2]);
     const int32_t* ph = pv.at(P[SH_PROG + 4 * prog + 3]);
     const int nph = ph[0];
+    const int32_t* srow = pv.at(ph[1]);
     for (int q = 0; q < nph; ++q) {
         const int4 h = reinterpret_cast<const int4*>(ph)[1 + q];
-        if (h.x == PH_SERIAL) run_serial<OP, STAGED>(pv, rows, lev[h.y], lev[h.z], L, lane);
+        if (h.x == PH_SERIAL) run_serial<OP, STAGED>(pv, rows, srow, lev[h.y], lev[h.z], L, lane);
         else run_levels<OP, STAGED>(pv, lev + h.y, rows, h.z - h.y, L, lane);
     }
 }
@@ -381,16 +436,8 @@ __global__ __launch_bounds__(kW) void k_small_lu(const int32_t* __restrict__ wor
         int32_t* hw = reinterpret_cast<int32_t*>(lds + ((max_nlu + max_n + 2 + max(max_n, kW)) & ~1));
         const int4* src = reinterpret_cast<const int4*>(P + pv.hot0);
         int4* dst = reinterpret_cast<int4*>(hw);
-        const int n4 = (P[SH_HOTLEN] + 3) / 4;
-        int k = lane;
-        for (; k + 3 * kW < n4; k += 4 * kW) {
-            const int4 a = src[k], b = src[k + kW], c = src[k + 2 * kW], e = src[k + 3 * kW];
-            dst[k] = a;
-            dst[k + kW] = b;
-            dst[k + 2 * kW] = c;
-            dst[k + 3 * kW] = e;
-        }
-        for (; k < n4; k += kW) dst[k] = src[k];
+        batched<8>((P[SH_HOTLEN] + 3) / 4, lane, [&](int k) { return src[k]; },
+                   [&](int k, int4 w) { dst[k] = w; });
         pv.hot = hw;
     }
     const int32_t* cptr = P + P[SH_CPTR];
@@ -404,28 +451,24 @@ __global__ __launch_bounds__(kW) void k_small_lu(const int32_t* __restrict__ wor
     if (mode & SLU_FACTOR) {
         for (int p = lane; p < nlu; p += kW) v[p] = 0.0;
         // the right-hand side rides along as column n (zeros: no solve asked)
-        for (int k = lane; k < n; k += kW) y[k] = fused ? rhs[d.xoff + in_perm[k]] : 0.0;
+        if (fused) {
+            batched<4>(n, lane, [&](int k) { return in_perm[k]; },
+                       [&](int k, int i) { y[k] = rhs[d.xoff + i]; });
+        } else {
+            for (int k = lane; k < n; k += kW) y[k] = 0.0;
+        }
         __syncthreads();
         const int32_t* apos = P + P[SH_APOS];
         const int nnz = P[SH_NNZA];
         const double* a = A + d.aoff;
-        int p = lane;
-        for (; p + 3 * kW < nnz; p += 4 * kW) {
-            const int q0 = apos[p], q1 = apos[p + kW], q2 = apos[p + 2 * kW], q3 = apos[p + 3 * kW];
-            const double w0 = a[p], w1 = a[p + kW], w2 = a[p + 2 * kW], w3 = a[p + 3 * kW];
-            v[q0] = w0;
-            v[q1] = w1;
-            v[q2] = w2;
-            v[q3] = w3;
-        }
-        for (; p < nnz; p += kW) v[apos[p]] = a[p];
+        batched<8>(nnz, lane, [&](int p) { return PosVal{apos[p], a[p]}; },
+                   [&](int, PosVal e) { v[e.q] = e.w; });
         __syncthreads();
         if (gmin != 0.0) {
             const int32_t* gp = P + P[SH_GMIN];
-            for (int k = lane; k < n; k += kW) {
-                const int q = gp[k];
+            batched<4>(n, lane, [&](int k) { return gp[k]; }, [&](int, int q) {
                 if (q >= 0) v[q] += gmin;
-            }
+            });
             __syncthreads();
         }
         // right-looking steps, level by level; each level's first task is
@@ -452,57 +495,36 @@ __global__ __launch_bounds__(kW) void k_small_lu(const int32_t* __restrict__ wor
             if (l < nlev) factor_level(lev, task, last, l, t0, t1, t2, ta, tb, tc, L, lane, junk);
             if (l + 1 < nlev) factor_level(lev, task, last, l + 1, t0, t1, t2, tb, tc, ta, L, lane, junk);
         }
-        // checks (the first failing column, as the reference stops there)
+        // checks (the first failing column, as the reference stops there),
+        // reciprocal pivots and the forward substitution's closing scale by
+        // the stored 1/pivot (solve.cpp:24, zero entries left alone)
         int bad = 0x7fffffff, code = FAIL_NONE;
-        for (int k = lane; k < n; k += kW) {
-            const int dp = cptr[k] + nU[k];
-            const double xkk = v[dp];
+        batched<4>(n, lane, [&](int k) { return int2{cptr[k] + nU[k], cptr[k + 1]}; }, [&](int k, int2 c) {
+            const double xkk = v[c.x];
             bool f;
             if (kd == SLU_VERIFY) {
                 double lmax = 0.0;
-                for (int p = dp + 1; p < cptr[k + 1]; ++p) lmax = fmax(lmax, fabs(v[p]));
+                for (int p = c.x + 1; p < c.y; ++p) lmax = fmax(lmax, fabs(v[p]));
                 f = !(lmax * 1e-3 < fabs(xkk));
             } else {
                 f = (xkk == 0.0);
             }
             if (f && k < bad) bad = k;
-        }
-        bad = wave_min_i32(bad);
-        if (bad != 0x7fffffff) code = (kd == SLU_VERIFY) ? FAIL_REORDER : FAIL_ZERO_PIVOT;
-        // reciprocal pivots, then the stored U values and diagonal, and the
-        // forward substitution's closing scale by the stored 1/pivot
-        // (solve.cpp:24, zero entries left alone)
-        for (int k = lane; k < n; k += kW) {
-            const double r = 1.0 / v[cptr[k] + nU[k]];
+            const double r = 1.0 / xkk;
             dr[k] = r;
             const double yk = y[k];
             if (yk != 0.0) y[k] = yk * r;
-        }
+        });
+        bad = wave_min_i32(bad);
+        if (bad != 0x7fffffff) code = (kd == SLU_VERIFY) ? FAIL_REORDER : FAIL_ZERO_PIVOT;
         __syncthreads();
+        // the stored U values and diagonal
         const int32_t* rows = P + P[SH_ROWS];
         const int32_t* pcol = P + P[SH_PCOL];
-        {
-            int p = lane;
-            for (; p + 3 * kW < nlu; p += 4 * kW) {
-                int r[4], c[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    r[u] = rows[p + u * kW];
-                    c[u] = pcol[p + u * kW];
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int q = p + u * kW;
-                    if (r[u] < c[u]) v[q] = v[q] * dr[r[u]];
-                    else if (r[u] == c[u]) v[q] = dr[r[u]];
-                }
-            }
-            for (; p < nlu; p += kW) {
-                const int r = rows[p], c = pcol[p];
-                if (r < c) v[p] = v[p] * dr[r];
-                else if (r == c) v[p] = dr[r];
-            }
-        }
+        batched<8>(nlu, lane, [&](int p) { return int2{rows[p], pcol[p]}; }, [&](int p, int2 rc) {
+            if (rc.x < rc.y) v[p] = v[p] * dr[rc.x];
+            else if (rc.x == rc.y) v[p] = dr[rc.x];
+        });
         __syncthreads();
         double* lu = LU + d.luoff;
         for (int p = lane; p < nlu; p += kW) lu[p] = v[p];
@@ -513,7 +535,7 @@ __global__ __launch_bounds__(kW) void k_small_lu(const int32_t* __restrict__ wor
         if (code != FAIL_NONE) return;
     } else {
         const double* lu = LU + d.luoff;
-        for (int p = lane; p < nlu; p += kW) v[p] = lu[p];
+        batched<8>(nlu, lane, [&](int p) { return lu[p]; }, [&](int p, double w) { v[p] = w; });
     }
     if (!(mode & (SLU_SOLVE | SLU_SOLVE_T))) return;
     if (fused) {
@@ -521,7 +543,7 @@ __global__ __launch_bounds__(kW) void k_small_lu(const int32_t* __restrict__ wor
         run_program<OP_PLAIN, STAGED>(pv, 1, L, lane);
     } else {
         __syncthreads();
-        for (int k = lane; k < n; k += kW) y[k] = rhs[d.xoff + in_perm[k]];
+        batched<4>(n, lane, [&](int k) { return in_perm[k]; }, [&](int k, int i) { y[k] = rhs[d.xoff + i]; });
         __syncthreads();
         if (!tr) {
             run_program<OP_SKIPZERO | OP_MUL_NZ, false>(PlanView<false>{P, nullptr, 0}, 0, L, lane);
@@ -533,7 +555,7 @@ __global__ __launch_bounds__(kW) void k_small_lu(const int32_t* __restrict__ wor
             run_program<OP_MUL_DIAG, false>(g, 3, L, lane);
         }
     }
-    for (int k = lane; k < n; k += kW) x[d.xoff + out_perm[k]] = y[k];
+    batched<4>(n, lane, [&](int k) { return out_perm[k]; }, [&](int k, int i) { x[d.xoff + i] = y[k]; });
 }
 
 }  // namespace

@@ -152,6 +152,30 @@ void append_program(const TriProgram& t, int nlu, int n, std::vector<int32_t>& w
         }
     }
     w.insert(w.end(), terms.begin(), terms.end());
+    // serial phases: for terms 0 and 1 of each row, 1 + the row of the same
+    // phase that produces the term's source (0: the source is final before
+    // the phase), packed low/high 16 bits; the phases header's second word
+    // points here
+    std::vector<int32_t> producer(n, -1);  // target -> row
+    for (int r = 0; r < nrows; ++r) producer[t.order[r]] = r;
+    std::vector<int32_t> srow(nrows, 0);
+    for (size_t q = 0; q < ph.size(); q += 4) {
+        if (ph[q] != PH_SERIAL) continue;
+        const int rb = t.level_ptr[ph[q + 1]], re = t.level_ptr[ph[q + 2]];
+        for (int r = rb; r < re; ++r) {
+            const int tg = t.order[r];
+            const int a = t.term_ptr[tg], b = t.term_ptr[tg + 1];
+            int32_t packed = 0;
+            for (int k = 0; k < 2 && a + k < b; ++k) {
+                const int pr = producer[t.term_src[a + k]];
+                if (pr >= rb && pr < r) packed |= (pr + 1) << (16 * k);
+            }
+            srow[r] = packed;
+        }
+    }
+    align4(w);
+    w[base + hdr4[3] + 1] = (int32_t)w.size() - base;
+    w.insert(w.end(), srow.begin(), srow.end());
 }
 
 }  // namespace

@@ -70,10 +70,15 @@ enum SluHdr : int32_t {
 // terms, then value and source of terms 0 and 1 (both the zero when
 // absent); terms: 2 words each (value, source).  The program op is fixed by the form:
 // fwd SKIPZERO|MUL_NZ, bwd PLAIN, tfwd SKIPZERO, tbwd MUL_DIAG.
-// Phases: 4 words of header (count), then 4 words each (kind, first level,
-// end level, 0).  A parallel phase runs its levels one row per lane; a
-// serial phase is a run of one-row levels (a dependency chain) that one
-// lane runs in order, the previous row's result passed on in registers.
+// Phases: 4 words of header (count, offset of the serial source rows, 0,
+// 0), then 4 words each (kind, first level, end level, 0).  A parallel
+// phase runs its levels one row per lane; a serial phase is a run of
+// one-row levels (a dependency chain) that the wave runs 64 rows at a time
+// from registers: every lane loads one row's operands, then the rows run
+// in order, each taking the results of earlier rows of its batch from
+// registers.  Serial source rows: one word per row, 1 + the row of the same
+// serial phase producing the source of term 0 (low 16 bits) and term 1
+// (high 16 bits), 0 when the source is final before the phase.
 enum SluPhase : int32_t { PH_PARALLEL = 0, PH_SERIAL = 1 };
 constexpr int kSluMinSerialRun = 4;
 
