@@ -1460,10 +1460,15 @@ void Bundle::launch_tick(bool timed) {
                     stream_, d_tmin_, bits_1e30(), d_spec_, &rs);
     launch_loads_small();
     launch_loads_b4_gather();
-    if (timed) NSA_HIP(hipEventRecord(ev_[3], stream_));
     const SluSide side{trunc4_, d_ctx_, d_s_[0], d_s_[1], d_s_[2], d_s_[3], b4_lanes_, to, d_tmin_, NI_, d_spec_};
+    // a timed tick runs the BSIM4v7 step proposals as a launch of their own,
+    // so that the LU launch's events (and counters) cover the LU alone
+    if (timed)
+        launch_trunc(trunc4_, d_ctx_, d_s_[0], d_s_[1], d_s_[2], d_s_[3], b4_lanes_, to, nullptr, d_tmin_, stream_,
+                     0, d_tmin_ + NI_, d_spec_);
+    if (timed) NSA_HIP(hipEventRecord(ev_[3], stream_));
     launch_small_lu(plans_, d_inst_, NI_, SLU_FACTOR | SLU_SOLVE, d_A_, opt_.diag_gmin, d_kind_, d_fail_,
-                    d_fail_col_, d_LU_, d_rhs_, d_x_, stream_, d_gmin_, &side);
+                    d_fail_col_, d_LU_, d_rhs_, d_x_, stream_, d_gmin_, timed ? nullptr : &side);
     if (timed) NSA_HIP(hipEventRecord(ev_[4], stream_));
     launch_post(true);
     if (timed) NSA_HIP(hipEventRecord(ev_[5], stream_));
