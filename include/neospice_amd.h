@@ -296,6 +296,32 @@ int nsa_lu_refactorize(nsa_lu* s, const double* values, double diag_gmin, int32_
  * Honoured by Markowitz solvers only, as in the reference (AmdLuSolver and
  * the AMD-LU auto-fallback keep the interface's no-op). */
 int nsa_lu_set_equilibrate(nsa_lu* s, int32_t on);
+/* The complex (AC) paths of ISolver (src/core/solver_iface.hpp:23-28), the
+ * reference's NeoSolver ones whatever the engine (AmdLuSolver delegates
+ * them, amd_lu_solver.cpp:347-349).  Host arrays in the real calls' layout,
+ * instance fastest, a complex value's real and imaginary parts in rows 2k
+ * and 2k + 1: ax (2 nnz) x B, rhs (2 n) x B.
+ * numeric_complex (neo_solver.cpp:228-245): every instance ordered by the
+ * complex Markowitz search on its own values (reusing its order on a later
+ * call when the order still holds, factor.cpp:929-947) and factored;
+ * refactorize_complex (neo_solver.cpp:247-258) factors on the orders held
+ * (factor_complex, factor.cpp:1054-1141); solve_complex (solve.cpp:49-100)
+ * solves in place.  status per instance: 0 ok, 1 singular; the factor calls
+ * return 1 if any instance is singular. */
+int nsa_lu_numeric_complex(nsa_lu* s, const double* ax, int32_t* status);
+int nsa_lu_refactorize_complex(nsa_lu* s, const double* ax, int32_t* status);
+int nsa_lu_solve_complex(nsa_lu* s, double* rhs);
+/* The AC analysis's frequency sweep (src/core/ac.cpp:196-238) for every
+ * instance and frequency in one launch: instance i's matrix at frequency f
+ * is g_i + j (2 pi f) c_i (g, c: nnz x B), ordered at freqs[0]
+ * (numeric_complex) and factored on that order at every frequency
+ * (refactorize_complex), then solved for rhs_i ((2 n) x B).  out: (2 n) x
+ * (nf B), column f B + i; status: nf B entries in the same order.
+ * Per-frequency device stamps (ac_stamp_freq) are the caller's: g and c
+ * hold the frequency-independent parts.  After a sweep the next solve
+ * needs a refactorize_complex. */
+int nsa_lu_ac_sweep(nsa_lu* s, const double* g, const double* c, const double* freqs, int32_t nf,
+                    const double* rhs, double* out, int32_t* status);
 /* rhs: host, n x B, overwritten with the solution. */
 int nsa_lu_solve(nsa_lu* s, double* rhs);
 int nsa_lu_solve_transposed(nsa_lu* s, double* rhs);
@@ -317,7 +343,10 @@ int nsa_lu_structure(nsa_lu* s, int32_t* row_perm, int32_t* col_perm, int32_t* s
  * The pivot order and fill pattern a fresh numeric factorization of
  * `values` fixes: engine 1 = Sparse 1.3 Markowitz (NeoSolver::numeric,
  * src/core/neo_solver.cpp:148-174), engine 2 = min-degree + Gilbert-Peierls
- * (AmdLuSolver::symbolic + factor_, src/core/amd_lu_solver.cpp:11-197).
+ * (AmdLuSolver::symbolic + factor_, src/core/amd_lu_solver.cpp:11-197),
+ * engine 3 = Sparse 1.3 Markowitz on a complex matrix (NeoSolver::
+ * numeric_complex, neo_solver.cpp:228-245; values 2 nnz, (re, im) per
+ * slot, diag_gmin 0).
  * Pivot space k = 0..n-1: b'[k] = b[row_perm[k]], x[col_perm[k]] = y[k];
  * column k holds nU[k] U rows (processing order), the diagonal, then L rows,
  * at rows[seg_ptr[k] .. seg_ptr[k+1]).  Pass rows = NULL to get *nnz_lu only.

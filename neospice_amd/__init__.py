@@ -449,6 +449,48 @@ class Solver:
         check(lib().nsa_lu_solve_noalloc(self.h, f64p(b)))
         return b
 
+    # ---- the complex (AC) paths: complex arrays in, complex arrays out
+    def _cvals(self, ax, rows):
+        a = np.asarray(ax, np.complex128).reshape(rows, self.B)
+        out = np.empty((2 * rows, self.B), np.float64)
+        out[0::2] = a.real
+        out[1::2] = a.imag
+        return as_f64(out)
+
+    def numeric_complex(self, ax):
+        """ISolver::numeric_complex for every instance: ax (nnz, B) complex,
+        CSC slot order.  Returns the per-instance singular flags."""
+        st = np.zeros(self.B, np.int32)
+        check(lib().nsa_lu_numeric_complex(self.h, f64p(self._cvals(ax, self.nnz)), i32p(st)))
+        return st.astype(bool)
+
+    def refactorize_complex(self, ax):
+        st = np.zeros(self.B, np.int32)
+        check(lib().nsa_lu_refactorize_complex(self.h, f64p(self._cvals(ax, self.nnz)), i32p(st)))
+        return st.astype(bool)
+
+    def solve_complex(self, rhs):
+        """rhs (n, B) complex; returns the solutions (n, B)."""
+        b = self._cvals(rhs, self.n)
+        check(lib().nsa_lu_solve_complex(self.h, f64p(b)))
+        return b[0::2] + 1j * b[1::2]
+
+    def ac_sweep(self, g, c, freqs, rhs):
+        """The AC analysis's sweep for every instance and frequency: matrix
+        g + j 2 pi f c (g, c: (nnz, B) real), ordered at freqs[0], solved for
+        rhs ((n, B) complex).  Returns (x, singular): x (nf, n, B) complex,
+        singular (nf, B)."""
+        freqs = as_f64(np.asarray(freqs, np.float64).ravel())
+        nf = len(freqs)
+        gg = as_f64(np.asarray(g, np.float64).reshape(self.nnz, self.B))
+        cc = as_f64(np.asarray(c, np.float64).reshape(self.nnz, self.B))
+        rb = self._cvals(rhs, self.n)
+        out = np.zeros((2 * self.n, nf * self.B), np.float64)
+        st = np.zeros(nf * self.B, np.int32)
+        check(lib().nsa_lu_ac_sweep(self.h, f64p(gg), f64p(cc), f64p(freqs), nf, f64p(rb), f64p(out), i32p(st)))
+        x = (out[0::2] + 1j * out[1::2]).reshape(self.n, nf, self.B).transpose(1, 0, 2)
+        return x, st.reshape(nf, self.B).astype(bool)
+
     @property
     def name(self):
         return lib().nsa_lu_name(self.h).decode()
@@ -479,7 +521,15 @@ def pivot_structure(engine, n, col_ptr, row_idx, values, diag_gmin=0.0):
     """Host-only: pivot order and fill of a fresh factorization (nsa_pivot_structure).
     Returns None when singular."""
     eng = ENGINES[engine] if isinstance(engine, str) else int(engine)
-    cp, ri, v = as_i32(col_ptr), as_i32(row_idx), as_f64(values)
+    cp, ri = as_i32(col_ptr), as_i32(row_idx)
+    if np.iscomplexobj(values):
+        # the complex Markowitz ordering (NeoSolver::numeric_complex)
+        z = np.asarray(values, np.complex128).ravel()
+        v = np.empty(2 * len(z), np.float64)
+        v[0::2], v[1::2] = z.real, z.imag
+        eng = 3
+    else:
+        v = as_f64(values)
     nlu = C.c_int32()
     rc = check(lib().nsa_pivot_structure(eng, n, i32p(cp), i32p(ri), f64p(v), float(diag_gmin),
                                          C.byref(nlu), None, None, None, None, None))

@@ -26,6 +26,7 @@ SOURCES = [
     "adjoint.cpp",
     "small_lu.cpp",
     "bundle.cpp",
+    "ac_lu.cpp",
     "capi.cpp",
     "gpu_lu.hip",
     "gpu_dev.hip",
@@ -36,6 +37,7 @@ SOURCES = [
     "gpu_bundle.hip",
     "noalloc.hip",
     "gpu_equil.hip",
+    "gpu_small_lu_c.hip",
 ]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]

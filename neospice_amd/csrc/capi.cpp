@@ -5,6 +5,7 @@
 #include <string>
 
 #include "../../include/neospice_amd.h"
+#include "ac_lu.hpp"
 #include "batch_lu.hpp"
 #include "circuit.hpp"
 #include "common.hpp"
@@ -30,10 +31,17 @@ struct nsa_bundle {
 struct nsa_lu {
     hipStream_t stream = nullptr;
     std::unique_ptr<nsa::BatchLu> lu;
+    std::unique_ptr<nsa::AcLu> ac;  // the complex paths, made on first use
+    std::vector<int32_t> cp, ri;
     double* dA = nullptr;
     double* db = nullptr;
     int n = 0, nnz = 0, B = 0;
+    nsa::AcLu& complex_lu() {
+        if (!ac) ac = std::make_unique<nsa::AcLu>(n, cp.data(), ri.data(), B, stream);
+        return *ac;
+    }
     ~nsa_lu() {
+        ac.reset();
         lu.reset();
         if (dA) (void)hipFree(dA);
         if (db) (void)hipFree(db);
@@ -409,6 +417,8 @@ int nsa_lu_create(int32_t batch, int32_t n, const int32_t* col_ptr, const int32_
         s->n = n;
         s->nnz = col_ptr[n];
         s->B = batch;
+        s->cp.assign(col_ptr, col_ptr + n + 1);
+        s->ri.assign(row_idx, row_idx + col_ptr[n]);
         s->lu = std::make_unique<nsa::BatchLu>(batch, s->stream);
         s->lu->symbolic(n, col_ptr, row_idx, is_linear != 0, engine);
         NSA_HIP(hipMalloc(&s->dA, sizeof(double) * std::max(1, s->nnz) * batch));
@@ -473,6 +483,63 @@ int nsa_lu_solve_noalloc(nsa_lu* s, double* rhs) {
         return 0;
     });
 }
+// The complex paths.  Host arrays in the real calls' layout, instance
+// fastest: value row r of instance i at r * B + i, a complex value's real
+// and imaginary parts in rows 2k and 2k + 1.
+static std::vector<double> to_blocks(const double* v, int rows, int B) {
+    std::vector<double> out((size_t)rows * B);
+    for (int i = 0; i < B; ++i)
+        for (int r = 0; r < rows; ++r) out[(size_t)i * rows + r] = v[(size_t)r * B + i];
+    return out;
+}
+static void from_blocks(const std::vector<double>& b, int rows, int B, double* v) {
+    for (int i = 0; i < B; ++i)
+        for (int r = 0; r < rows; ++r) v[(size_t)r * B + i] = b[(size_t)i * rows + r];
+}
+int nsa_lu_numeric_complex(nsa_lu* s, const double* ax, int32_t* status) {
+    return guard([&] {
+        if (!ax) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "nsa_lu_numeric_complex: ax");
+        const auto blk = to_blocks(ax, 2 * s->nnz, s->B);
+        return s->complex_lu().numeric_complex(blk.data(), status) ? 1 : 0;
+    });
+}
+int nsa_lu_refactorize_complex(nsa_lu* s, const double* ax, int32_t* status) {
+    return guard([&] {
+        if (!ax) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "nsa_lu_refactorize_complex: ax");
+        const auto blk = to_blocks(ax, 2 * s->nnz, s->B);
+        return s->complex_lu().refactorize_complex(blk.data(), status) ? 1 : 0;
+    });
+}
+int nsa_lu_solve_complex(nsa_lu* s, double* rhs) {
+    return guard([&] {
+        if (!rhs) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "nsa_lu_solve_complex: rhs");
+        auto blk = to_blocks(rhs, 2 * s->n, s->B);
+        s->complex_lu().solve_complex(blk.data());
+        from_blocks(blk, 2 * s->n, s->B, rhs);
+        return 0;
+    });
+}
+int nsa_lu_ac_sweep(nsa_lu* s, const double* g, const double* c, const double* freqs, int32_t nf,
+                    const double* rhs, double* out, int32_t* status) {
+    return guard([&] {
+        if (!g || !c || !freqs || !rhs || !out || nf <= 0)
+            throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "nsa_lu_ac_sweep: arguments");
+        const int B = s->B, n = s->n;
+        const auto gb = to_blocks(g, s->nnz, B), cb = to_blocks(c, s->nnz, B), rb = to_blocks(rhs, 2 * n, B);
+        std::vector<double> ob((size_t)2 * n * B * nf);
+        std::vector<int32_t> st((size_t)B * nf);
+        const int k = s->complex_lu().sweep(gb.data(), cb.data(), freqs, nf, rb.data(), ob.data(), st.data());
+        // (instance, frequency) blocks -> column f * B + i of 2n rows
+        const size_t cols = (size_t)B * nf;
+        for (int i = 0; i < B; ++i)
+            for (int f = 0; f < nf; ++f) {
+                const size_t blk = (size_t)i * nf + f, col = (size_t)f * B + i;
+                for (int r = 0; r < 2 * n; ++r) out[(size_t)r * cols + col] = ob[blk * 2 * n + r];
+                if (status) status[col] = st[blk];
+            }
+        return k ? 1 : 0;
+    });
+}
 const char* nsa_lu_name(nsa_lu* s) { return s->lu->name(); }
 int nsa_lu_info(nsa_lu* s, int32_t out[4]) {
     return guard([&] {
@@ -512,8 +579,15 @@ int nsa_pivot_structure(int32_t engine, int32_t n, const int32_t* col_ptr, const
             nsa::Sparse13 s13(n, col_ptr, row_idx);
             if (s13.numeric(values, diag_gmin) != 0) return 1;
             st = s13.structure();
+        } else if (engine == 3) {
+            if (diag_gmin != 0.0)
+                throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "nsa_pivot_structure: no diag_gmin on a complex matrix");
+            nsa::Sparse13 s13(n, col_ptr, row_idx);
+            if (s13.numeric_complex(values) != 0) return 1;
+            st = s13.structure();
         } else {
-            throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "engine must be 1 (markowitz) or 2 (amdlu)");
+            throw nsa::Error(nsa::NSA_ERR_ARGUMENT,
+                             "engine must be 1 (markowitz), 2 (amdlu) or 3 (complex markowitz)");
         }
         if (nnz_lu) *nnz_lu = st.nlu();
         if (rows) {

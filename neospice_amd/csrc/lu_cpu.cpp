@@ -10,6 +10,8 @@
 // reference's pointer-to-pointer splicing.
 #include "lu_cpu.hpp"
 
+#include "cplx.hpp"
+
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -178,7 +180,8 @@ bool gp_factor_structure(int n, const int32_t* cp, const int32_t* ri, const doub
 struct Sparse13::Impl {
     int n = 0;
     // element arrays
-    std::vector<double> re;
+    std::vector<double> re, im;  // im: zero unless the matrix is complex
+    bool cplx = false;
     std::vector<int32_t> erow, ecol, nrow, ncol;
     // list heads (1-based, [0] unused)
     std::vector<int32_t> fc, fr, dg;
@@ -200,6 +203,7 @@ struct Sparse13::Impl {
 
     int new_element(int r, int c, bool fill) {
         re.push_back(0.0);
+        im.push_back(0.0);
         erow.push_back(r);
         ecol.push_back(c);
         nrow.push_back(-1);
@@ -336,7 +340,8 @@ struct Sparse13::Impl {
         }
     }
 
-    double mag(int e) const { return std::fabs(re[e]); }
+    // element_mag (element.hpp:18-20); im is zero on a real matrix
+    double mag(int e) const { return cmag(re[e], im[e]); }
 
     double biggest_in_col_excluding(int e, int step) {  // factor.cpp:38-55
         const int r = erow[e], c = ecol[e];
@@ -690,7 +695,26 @@ struct Sparse13::Impl {
         return ne;
     }
 
+    // complex_row_col_elimination (factor.cpp:769-804)
+    bool eliminate_complex(int piv) {
+        if (mag(piv) == 0.0) return false;
+        crecip(re[piv], im[piv], re[piv], im[piv]);
+        for (int up = nrow[piv]; up >= 0; up = nrow[up]) {
+            cmul(re[up], im[up], re[piv], im[piv]);
+            int sub = ncol[up];
+            for (int lo = ncol[piv]; lo >= 0; lo = ncol[lo]) {
+                const int r = erow[lo];
+                while (sub >= 0 && erow[sub] < r) sub = ncol[sub];
+                if (sub < 0 || erow[sub] > r) sub = create_fillin(r, ecol[up]);
+                cmsub(re[sub], im[sub], re[up], im[up], re[lo], im[lo]);
+                sub = ncol[sub];
+            }
+        }
+        return true;
+    }
+
     bool eliminate(int piv) {  // real_row_col_elimination, factor.cpp:735-767
+        if (cplx) return eliminate_complex(piv);
         if (std::fabs(re[piv]) == 0.0) return false;
         re[piv] = 1.0 / re[piv];
         for (int up = nrow[piv]; up >= 0; up = nrow[up]) {
@@ -803,6 +827,23 @@ int Sparse13::numeric(const double* values, double gmin) {
     if (!m.preordered) { m.mna_preorder(); m.preordered = true; }
     if (gmin != 0.0)
         for (int i = m.n; i > 0; --i) if (m.dg[i] >= 0) m.re[m.dg[i]] += gmin;
+    m.rel = 1e-3;
+    m.absth = 1e-13;
+    return m.order_and_factor();
+}
+
+int Sparse13::numeric_complex(const double* ax) {
+    Impl& m = *impl_;
+    // load_complex (neo_solver.cpp:104-111): fill-ins cleared, both parts
+    // of the originals reloaded
+    m.cplx = true;
+    for (size_t e = 0; e < m.re.size(); ++e)
+        if (m.is_fill[e]) { m.re[e] = 0.0; m.im[e] = 0.0; }
+    for (size_t p = 0; p < m.slot_elem.size(); ++p) {
+        m.re[m.slot_elem[p]] = ax[2 * p];
+        m.im[m.slot_elem[p]] = ax[2 * p + 1];
+    }
+    if (!m.preordered) { m.mna_preorder(); m.preordered = true; }
     m.rel = 1e-3;
     m.absth = 1e-13;
     return m.order_and_factor();

@@ -37,6 +37,11 @@ public:
     // threshold check and re-pivot from the first failing step
     // (factor.cpp:929-947).
     int numeric(const double* values, double diag_gmin);
+    // NeoSolver::numeric_complex (neo_solver.cpp:228-245): ax holds (real,
+    // imaginary) per slot; the same ordering on element magnitudes |re| +
+    // |im| with complex eliminations (factor.cpp:769-804), no diag_gmin.
+    // The matrix stays complex from then on.
+    int numeric_complex(const double* ax);
     LuStructure structure() const;
     int fillins() const;
     // Deep copy of the factorization state (pivot order, fill, counts).

@@ -156,4 +156,24 @@ void launch_small_lu(const SmallLuPlans& plans, const SluInst* d_inst, int ninst
                      int32_t* fail_col, double* LU, const double* rhs, double* x, hipStream_t s,
                      const double* gmin_inst = nullptr, const SluSide* side = nullptr);
 
+// The complex packed LU (gpu_small_lu_c.hip): factor_complex and
+// solve_complex on the same plans, one wave per SluInst, every array of
+// (re, im) pairs with offsets counted in complex values (ax at aoff, LU at
+// luoff -- LU may be null when only the solution is wanted -- rhs and x at
+// xoff; x may alias rhs).  Modes SLU_FACTOR and SLU_SOLVE (both: the
+// forward substitution folded into the factor).  fail[i] = FAIL_ZERO_PIVOT
+// with the first failing column in fail_col.
+constexpr size_t kSluComplexLds = 64 * 1024;  // beyond it a launch raises the kernel's LDS limit
+size_t small_lu_complex_lds(const SmallLuPlans& plans);
+void launch_small_lu_complex(const SmallLuPlans& plans, const SluInst* d_inst, int ninst, int mode,
+                             const double* ax, int32_t* fail, int32_t* fail_col, double* LU, const double* rhs,
+                             double* x, hipStream_t s);
+// The AC sweep's matrices and right-hand sides for B instances x nf
+// frequencies, (instance, frequency) blocks, instance-major: ax (complex,
+// nnz per block) = (g_i, w_f c_i) per slot; v (complex, n per block) =
+// rhs_i (complex, n per instance).
+void launch_ac_build(const double* g, const double* c, const double* w, int B, int nf, int nnz, double* ax,
+                     hipStream_t s);
+void launch_ac_rhs(const double* rhs, int B, int nf, int n, double* v, hipStream_t s);
+
 }  // namespace nsa

@@ -777,6 +777,38 @@ int ref_solver_solve_transposed(void* p, double* rhs) {
     });
 }
 
+// ISolver's complex paths (neo_solver.cpp:228-279; AmdLuSolver delegates
+// them to a NeoSolver): ax is 2 nnz values, (re, im) per CSC slot; rhs 2 n.
+int ref_solver_numeric_complex(void* p, const double* ax) {
+    auto* s = static_cast<RefSolver*>(p);
+    return guarded([&] {
+        std::vector<double> a(ax, ax + 2 * s->mat->nnz());
+        if (s->kind == 0) s->neo.numeric_complex(*s->pat, a); else s->amd.numeric_complex(*s->pat, a);
+        // numeric_complex reports nothing; its matrix's error state says
+        // whether the ordering met a singular matrix
+        const auto& m = s->kind == 0 ? *s->neo.matrix_ : *s->amd.complex_solver_->matrix_;
+        return m.error_ == neospice::solver::SparseError::Singular ? 1 : 0;
+    });
+}
+
+int ref_solver_refactorize_complex(void* p, const double* ax) {
+    auto* s = static_cast<RefSolver*>(p);
+    return guarded([&] {
+        std::vector<double> a(ax, ax + 2 * s->mat->nnz());
+        return s->kind == 0 ? (int)s->neo.refactorize_complex(a) : (int)s->amd.refactorize_complex(a);
+    });
+}
+
+int ref_solver_solve_complex(void* p, double* rhs) {
+    auto* s = static_cast<RefSolver*>(p);
+    return guarded([&] {
+        std::vector<double> r(rhs, rhs + 2 * s->pat->size());
+        if (s->kind == 0) s->neo.solve_complex(r); else s->amd.solve_complex(r);
+        std::copy(r.begin(), r.end(), rhs);
+        return 0;
+    });
+}
+
 int ref_solver_fast_count(void* p) { return static_cast<RefSolver*>(p)->amd.refactor_fast_count(); }
 int ref_solver_fallback_count(void* p) { return static_cast<RefSolver*>(p)->amd.refactor_fallback_count(); }
 

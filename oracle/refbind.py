@@ -108,6 +108,9 @@ def _bind(L):
         "ref_solver_refactorize": (C.c_int, [VP, P_F64, C.c_double]),
         "ref_solver_solve": (C.c_int, [VP, P_F64]),
         "ref_solver_solve_transposed": (C.c_int, [VP, P_F64]),
+        "ref_solver_numeric_complex": (C.c_int, [VP, P_F64]),
+        "ref_solver_refactorize_complex": (C.c_int, [VP, P_F64]),
+        "ref_solver_solve_complex": (C.c_int, [VP, P_F64]),
         "ref_solver_fast_count": (C.c_int, [VP]),
         "ref_solver_fallback_count": (C.c_int, [VP]),
         "ref_neo_dump": (C.c_int, [VP, P_I32, P_I32, P_I32, P_I32, P_F64]),
@@ -466,6 +469,33 @@ class RefSolver:
         b, pb = f64(np.array(b, dtype=np.float64))
         _chk(self.L.ref_solver_solve_transposed(self.h, pb))
         return b
+
+    # ---- complex paths: complex numpy arrays (slot order / unknowns)
+    @staticmethod
+    def _inter(z):
+        z = np.asarray(z, np.complex128).ravel()
+        out = np.empty(2 * len(z), np.float64)
+        out[0::2] = z.real
+        out[1::2] = z.imag
+        return out
+
+    def numeric_complex(self, ax):
+        """True when the ordering met a singular matrix (the reference's
+        numeric_complex returns nothing; its matrix's error state says so)."""
+        a, pa = f64(self._inter(ax))
+        return bool(_chk(self.L.ref_solver_numeric_complex(self.h, pa)))
+
+    def refactorize_complex(self, ax):
+        a, pa = f64(self._inter(ax))
+        r = self.L.ref_solver_refactorize_complex(self.h, pa)
+        if r < 0:
+            _chk(r)
+        return bool(r)
+
+    def solve_complex(self, b):
+        r, pr = f64(self._inter(b))
+        _chk(self.L.ref_solver_solve_complex(self.h, pr))
+        return r[0::2] + 1j * r[1::2]
 
     def neo_dump(self):
         n = self.n

@@ -191,3 +191,95 @@ def test_equilibrate_matches_reference(case, engine, gmin):
         assert bool(st1[i]) == sing1, i
         if not sing1:
             assert np.array_equal(x1[:, i], r.solve(b[:, i])), i
+
+
+def ac_pair(case, seed=9):
+    """g (the case's values) and c (capacitance-like values on the same
+    pattern) of an AC system g + j w c."""
+    name, n, cp, ri, vals = case
+    rng = np.random.default_rng(seed)
+    c = np.abs(np.asarray(vals)) * rng.uniform(0.5, 2.0, len(vals)) * 1e-12
+    return np.asarray(vals, np.float64), c
+
+
+@pytest.mark.parametrize("engine", ["markowitz", "amdlu"])
+@pytest.mark.parametrize("case", CASES, ids=lambda c: c[0])
+def test_complex_paths_bit_exact(case, engine):
+    """ISolver's complex paths on a batch of three instances, each at its
+    own frequency: numeric_complex (the complex Markowitz ordering on the
+    host, the complex factor on the GPU), solve_complex, refactorize_complex
+    at new frequencies, and a second numeric_complex (the reuse path):
+    every instance's solutions bit-equal to a reference solver of its own
+    values (AmdLuSolver delegates these to NeoSolver, so both engines
+    must equal it)."""
+    name, n, cp, ri, vals = case
+    g, c = ac_pair(case)
+    rng = np.random.default_rng(3)
+    B = 3
+    W0 = (1e3, 1e9, 1e13)
+    W1 = (2e3, 3e10, 5e15)
+    Z0 = np.stack([g + 1j * w * c for w in W0], axis=1)
+    Z1 = np.stack([g * (1 + 1e-3 * rng.standard_normal(len(g))) + 1j * w * c for w in W1], axis=1)
+    b = rng.standard_normal((n, B)) + 1j * rng.standard_normal((n, B))
+    s = na.Solver(n, cp, ri, batch=B, engine=engine)
+    st0 = s.numeric_complex(Z0)
+    x0 = s.solve_complex(b)
+    st1 = s.refactorize_complex(Z1)
+    x1 = s.solve_complex(b)
+    st2 = s.numeric_complex(Z1)
+    x2 = s.solve_complex(b)
+    checked = 0
+    for i in range(B):
+        r = RB.RefSolver(KIND[engine], n, cp, ri)
+        sing0 = r.numeric_complex(Z0[:, i])
+        assert bool(st0[i]) == sing0, i
+        if sing0:
+            continue
+        assert np.array_equal(x0[:, i], r.solve_complex(b[:, i])), (i, np.max(np.abs(x0[:, i] - r.solve_complex(b[:, i]))))
+        sing1 = r.refactorize_complex(Z1[:, i])
+        assert bool(st1[i]) == sing1, i
+        if not sing1:
+            assert np.array_equal(x1[:, i], r.solve_complex(b[:, i])), i
+        r2 = RB.RefSolver(KIND[engine], n, cp, ri)
+        r2.numeric_complex(Z0[:, i])
+        assert bool(st2[i]) == r2.numeric_complex(Z1[:, i]), i
+        if not st2[i]:
+            assert np.array_equal(x2[:, i], r2.solve_complex(b[:, i])), i
+        checked += 1
+    if checked == 0:
+        pytest.skip("singular in the reference")
+
+
+@pytest.mark.parametrize("case", [CASES[0], CASES[1], CASES[2]], ids=lambda c: c[0])
+def test_ac_sweep_matches_reference(case):
+    """The AC analysis's sweep (ac.cpp:196-238) for two instances and 24
+    frequencies in one launch: each instance ordered at the first frequency,
+    every frequency factored on that order and solved -- bit-equal to the
+    reference's own loop (numeric_complex at f0, then refactorize_complex
+    and solve_complex per frequency, on a NeoSolver as ac.cpp makes)."""
+    name, n, cp, ri, vals = case
+    g, c = ac_pair(case)
+    rng = np.random.default_rng(4)
+    B = 2
+    G = np.stack([g, g * (1 + 1e-2 * rng.standard_normal(len(g)))], axis=1)
+    Cm = np.stack([c, c * (1 + 1e-2 * rng.standard_normal(len(c)))], axis=1)
+    freqs = np.logspace(2, 14, 24)
+    rhs = rng.standard_normal((n, B)) + 1j * rng.standard_normal((n, B))
+    s = na.Solver(n, cp, ri, batch=B)
+    x, sing = s.ac_sweep(G, Cm, freqs, rhs)
+    assert x.shape == (24, n, B)
+    for i in range(B):
+        r = RB.RefSolver(0, n, cp, ri)
+        for f, fr in enumerate(freqs):
+            w = 2.0 * np.pi * fr
+            ax = G[:, i] + 1j * (w * Cm[:, i])
+            if f == 0:
+                sing0 = r.numeric_complex(ax)
+                assert bool(sing[0, i]) == sing0
+                if sing0:
+                    break
+            else:
+                assert bool(sing[f, i]) == r.refactorize_complex(ax), (i, f)
+            xr = r.solve_complex(rhs[:, i])
+            if not sing[f, i]:
+                assert np.array_equal(x[f, :, i], xr), (i, f, np.max(np.abs(x[f, :, i] - xr)))

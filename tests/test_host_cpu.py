@@ -143,3 +143,40 @@ def test_min_degree_order_many_dense_nodes():
     assert RB.ref().ref_amd_ordering(n, P(cp), P(ri), P(q_ref)) == n
     q = na.min_degree_order(n, cp, ri)
     assert np.array_equal(q, q_ref)
+
+
+def complex_cases():
+    """Complex matrices on the same patterns: g + j w c with c a second set
+    of values on the pattern, at a low, a middle and a high frequency (the
+    imaginary parts dominating moves the pivots)."""
+    out = []
+    rng = np.random.default_rng(7)
+    for name, n, cp, ri, v in MATS:
+        c = np.abs(np.asarray(v)) * rng.uniform(0.5, 2.0, len(v)) * 1e-12
+        for w in (1e3, 1e12, 1e16):
+            out.append(("%s_w%.0e" % (name, w), n, cp, ri, np.asarray(v) + 1j * w * c))
+    return out
+
+
+CMATS = complex_cases()
+
+
+@pytest.mark.parametrize("case", CMATS, ids=lambda c: c[0])
+def test_complex_markowitz_order_and_fill_match_reference(case):
+    """NeoSolver::numeric_complex's ordering (neo_solver.cpp:228-245):
+    pivots chosen on |re| + |im| with complex eliminations
+    (factor.cpp:769-804); the internal row/column translation and every
+    column list, bit-exact, and singular where the reference's ordering
+    meets a singular matrix."""
+    name, n, cp, ri, z = case
+    rs = RB.RefSolver(0, n, cp, ri)
+    if rs.numeric_complex(z):
+        assert na.pivot_structure("markowitz", n, cp, ri, z) is None
+        return
+    d = rs.neo_dump()
+    st = na.pivot_structure("markowitz", n, cp, ri, z)
+    assert st is not None
+    assert np.array_equal(st["row_perm"] + 1, d["i2e_row"])
+    assert np.array_equal(st["col_perm"] + 1, d["i2e_col"])
+    assert np.array_equal(st["seg_ptr"], d["col_ptr"])
+    assert np.array_equal(st["rows"] + 1, d["rows"])
