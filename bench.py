@@ -260,6 +260,93 @@ def rc_mesh(na, W, args, B=256, dt=1e-12):
 
 
 
+def ac_sweep(na, W, args, m=16, B=256, nf=64, cpu=True):
+    """Batched AC (the (f)2 row): the AC analysis's frequency sweep of a
+    16x16 RC mesh (257 unknowns), 256 instances with per-instance +-10 %
+    Gaussian R, 64 log-spaced frequencies 1 kHz .. 1 THz, unit AC drive on
+    the source's branch.  Instance i's matrix at frequency f is
+    g_i + j 2 pi f c_i, g the conductances and c the capacitances on the
+    pattern (both from the library's own loads: a DC load, and a transient
+    load with ag0 = 1 whose difference is c); each instance ordered at the
+    first frequency, every (instance, frequency) factored on that order and
+    solved in one launch (ac.cpp:196-238).  A step is one whole sweep; value =
+    frequency points (instances x frequencies) per second, wall time of the
+    call with its host orderings and host<->device copies."""
+    import torch
+    ckt, info = W.rc_mesh(m)
+    ckt.finalize()
+    R = W.mesh_instance_resistors(info, B)
+    eng = na.BatchEngine(ckt, batch=B, xmu=0.0)
+    for k, d in enumerate(info["res_devs"]):
+        eng.set_values(int(d), R[k])
+    eng.tran_begin()
+    eng.set_ctx(na.make_ctx(na.MODEDCOP))
+    g, _ = eng.load()
+    eng.set_ctx(na.make_ctx(na.MODETRAN | na.MODEINITPRED, 1, 0, (1.0, -1.0, 0.0)))
+    a, _ = eng.load()
+    c = a - g
+    n = ckt.n
+    rhs = np.zeros((n, B), np.complex128)
+    rhs[n - 1, :] = 1.0                         # the voltage source's branch row: unit AC drive
+    freqs = np.logspace(3, 12, nf)
+    s = na.Solver(n, ckt.col_ptr, ckt.row_idx, batch=B)
+    x, sing = s.ac_sweep(g, c, freqs, rhs)      # warm-up (allocations, plans)
+    torch.cuda.synchronize()
+    reps = 3
+    dev = order = inside = 0.0
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        x, sing = s.ac_sweep(g, c, freqs, rhs)
+        st = s.ac_stats()
+        dev += st["device_s"]
+        order += st["order_s"]
+        inside += st["total_s"]
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / reps
+    dev, order, inside = dev / reps, order / reps, inside / reps
+    value = B * nf / dev
+    log("bench: AC sweep %d instances x %d frequencies, n=%d: device %.3f ms (%.0f points/s), host orderings "
+        "%.2f ms, library call %.2f ms, Python call %.2f ms" % (B, nf, n, dev * 1e3, value, order * 1e3,
+                                                                  inside * 1e3, wall * 1e3))
+    res = {"workload": "rc_mesh_%dx%d_ac_sweep_b%d_f%d" % (m, m, B, nf), "value": round(value, 2),
+           "unit": "frequency points/s", "device_ms_per_sweep": round(dev * 1e3, 3),
+           "host_order_ms": round(order * 1e3, 3), "library_call_ms": round(inside * 1e3, 3),
+           "python_call_ms": round(wall * 1e3, 3),
+           "points_per_s_end_to_end": round(B * nf / wall, 2), "unknowns": n,
+           "nnz": ckt.nnz, "instances": B, "frequencies": nf, "singular_points": int(sing.sum()),
+           "note": "value: device work with the matrices' g, c resident (build, factor, solve of every point); "
+                   "the call also orders each instance on the host (at the first frequency) and copies "
+                   "host buffers both ways"}
+    if cpu:
+        # the reference's own loop (ac.cpp:215-222) on a NeoSolver per
+        # instance, for a bounded sample of the instances; also the parity
+        # check of those instances, bit for bit
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import refbind as RB
+        sample = [0, B // 2, B - 1]
+        t0 = time.perf_counter()
+        same = True
+        for i in sample:
+            r = RB.RefSolver(0, n, ckt.col_ptr, ckt.row_idx)
+            for f, fr in enumerate(freqs):
+                w = 2.0 * np.pi * fr
+                ax = g[:, i] + 1j * (w * c[:, i])
+                if f == 0:
+                    r.numeric_complex(ax)
+                else:
+                    r.refactorize_complex(ax)
+                y = r.solve_complex(rhs[:, i])
+                same = same and bool(np.array_equal(y, x[f, :, i]))
+        tc = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(len(sample) * nf / tc, 2), "unit": "frequency points/s", "cores": 1,
+                               "kind": "reference", "sample": "%d instances x %d frequencies, NeoSolver loop of "
+                               "ac.cpp on one core" % (len(sample), nf)}
+        res["parity_bit_equal"] = same
+        log("bench: AC sweep CPU reference %.0f points/s (1 core), sample bit-equal %s"
+            % (res["cpu_baseline"]["value"], same))
+    return res
+
+
 def adjoint_c4(na, W, count=1024):
     """DC-operating-point adjoint of the two-stage Miller op-amp batch: 1024
     seeded instances (8 BSIM4v7 transistors with W in [0.2u, 5u] and L in
@@ -650,6 +737,7 @@ def main():
     # already churned (the ring leg is not affected by the order)
     mesh = None if (args.no_mesh or args.headline_only) else rc_mesh(na, W, args)
     c3 = c3_headline(na, W, args)
+    ac = None if args.headline_only else ac_sweep(na, W, args, cpu=not args.no_cpu)
     adjoint = None
     if not (args.no_adjoint or args.headline_only):
         adjoint = {"dc_opamp": adjoint_c4(na, W), "transient_diode_mesh": adjoint_c5(na, W, nsteps=args.adjoint_steps)}
@@ -673,6 +761,7 @@ def main():
         out[k] = c3[k]
     out["adjoint"] = adjoint
     out["rc_mesh"] = mesh
+    out["ac_sweep"] = ac
     print(json.dumps(out))
 
 

@@ -313,15 +313,22 @@ int nsa_lu_refactorize_complex(nsa_lu* s, const double* ax, int32_t* status);
 int nsa_lu_solve_complex(nsa_lu* s, double* rhs);
 /* The AC analysis's frequency sweep (src/core/ac.cpp:196-238) for every
  * instance and frequency in one launch: instance i's matrix at frequency f
- * is g_i + j (2 pi f) c_i (g, c: nnz x B), ordered at freqs[0]
- * (numeric_complex) and factored on that order at every frequency
- * (refactorize_complex), then solved for rhs_i ((2 n) x B).  out: (2 n) x
- * (nf B), column f B + i; status: nf B entries in the same order.
- * Per-frequency device stamps (ac_stamp_freq) are the caller's: g and c
- * hold the frequency-independent parts.  After a sweep the next solve
- * needs a refactorize_complex. */
+ * is g_i + j (2 pi f) c_i, ordered at freqs[0] (numeric_complex) and
+ * factored on that order at every frequency (refactorize_complex), then
+ * solved for rhs_i.  Host arrays, instance-major (the layout of a batch of
+ * the reference's own arrays, and a complex array's memory): g, c: B blocks
+ * of nnz (CSC slot order); rhs: B blocks of 2 n ((re, im) per unknown);
+ * out: B nf blocks of 2 n, block i nf + f; status: B nf entries, same
+ * order.  Per-frequency device stamps (ac_stamp_freq) are the caller's: g
+ * and c hold the frequency-independent parts.  After a sweep the next
+ * solve needs a refactorize_complex. */
 int nsa_lu_ac_sweep(nsa_lu* s, const double* g, const double* c, const double* freqs, int32_t nf,
                     const double* rhs, double* out, int32_t* status);
+/* Seconds of the last nsa_lu_ac_sweep: out[0] the host orderings, out[1]
+ * the device work with inputs resident (matrix build, factor and solve, by
+ * HIP events), out[2] the whole call inside the library (no reference
+ * counterpart: statistics of this engine). */
+int nsa_lu_ac_stats(nsa_lu* s, double out[3]);
 /* rhs: host, n x B, overwritten with the solution. */
 int nsa_lu_solve(nsa_lu* s, double* rhs);
 int nsa_lu_solve_transposed(nsa_lu* s, double* rhs);

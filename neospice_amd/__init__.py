@@ -478,18 +478,26 @@ class Solver:
     def ac_sweep(self, g, c, freqs, rhs):
         """The AC analysis's sweep for every instance and frequency: matrix
         g + j 2 pi f c (g, c: (nnz, B) real), ordered at freqs[0], solved for
-        rhs ((n, B) complex).  Returns (x, singular): x (nf, n, B) complex,
-        singular (nf, B)."""
+        rhs ((n, B) complex).  Returns (x, singular): x (nf, n, B) complex
+        (a view of the library's instance-major output), singular (nf, B)."""
         freqs = as_f64(np.asarray(freqs, np.float64).ravel())
         nf = len(freqs)
-        gg = as_f64(np.asarray(g, np.float64).reshape(self.nnz, self.B))
-        cc = as_f64(np.asarray(c, np.float64).reshape(self.nnz, self.B))
-        rb = self._cvals(rhs, self.n)
-        out = np.zeros((2 * self.n, nf * self.B), np.float64)
-        st = np.zeros(nf * self.B, np.int32)
-        check(lib().nsa_lu_ac_sweep(self.h, f64p(gg), f64p(cc), f64p(freqs), nf, f64p(rb), f64p(out), i32p(st)))
-        x = (out[0::2] + 1j * out[1::2]).reshape(self.n, nf, self.B).transpose(1, 0, 2)
-        return x, st.reshape(nf, self.B).astype(bool)
+        # instance-major blocks: a batch of the reference's own arrays
+        gg = as_f64(np.asarray(g, np.float64).reshape(self.nnz, self.B).T)
+        cc = as_f64(np.asarray(c, np.float64).reshape(self.nnz, self.B).T)
+        rb = np.ascontiguousarray(np.asarray(rhs, np.complex128).reshape(self.n, self.B).T)
+        out = np.empty((self.B, nf, self.n), np.complex128)
+        st = np.zeros((self.B, nf), np.int32)
+        check(lib().nsa_lu_ac_sweep(self.h, f64p(gg), f64p(cc), f64p(freqs), nf, f64p(rb.view(np.float64)),
+                                    f64p(out.view(np.float64)), i32p(st)))
+        return out.transpose(1, 2, 0), st.T.astype(bool)
+
+    def ac_stats(self):
+        """Seconds of the last ac_sweep: host orderings, device work with the
+        inputs resident, the whole call inside the library."""
+        o = np.zeros(3, np.float64)
+        check(lib().nsa_lu_ac_stats(self.h, f64p(o)))
+        return {"order_s": o[0], "device_s": o[1], "total_s": o[2]}
 
     @property
     def name(self):

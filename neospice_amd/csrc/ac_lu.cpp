@@ -1,10 +1,12 @@
 #include "ac_lu.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
 #include "common.hpp"
+#include "parallel.hpp"
 
 namespace nsa {
 
@@ -36,19 +38,29 @@ T* AcLu::grow(T*& p, size_t& cap, size_t count) {
 
 // NeoSolver::numeric_complex of instance inst on its own matrix: the
 // complex Markowitz ordering (a fresh one on the first call, the reuse
-// test on an existing order after) fixes its structure and plan.
-int AcLu::order(const double* ax, int inst) {
+// test on an existing order after) fixes its structure.  Instances are
+// independent, so a batch orders them on host threads; their plans are
+// added in instance order afterwards.
+int AcLu::order_one(const double* ax, int inst, LuStructure& st) {
     if (!s13_[inst]) s13_[inst] = std::make_unique<Sparse13>(n_, acp_.data(), ari_.data());
-    ++n_cpu_order;
-    if (s13_[inst]->numeric_complex(ax)) {
-        plan_of_[inst] = -1;
-        return 1;
-    }
-    const LuStructure st = s13_[inst]->structure();
+    if (s13_[inst]->numeric_complex(ax)) return 1;
+    st = s13_[inst]->structure();
     if (!SmallLuPlans::fits(st))
         throw Error(NSA_ERR_UNSUPPORTED, "complex LU: the factor exceeds the packed kernel's limits");
-    plan_of_[inst] = plans_.add(st);
     return 0;
+}
+
+void AcLu::order(const std::vector<int>& which, const std::function<const double*(int)>& ax,
+                 std::vector<int32_t>& singular) {
+    std::vector<LuStructure> st(which.size());
+    std::vector<int32_t> sing(which.size(), 0);
+    parallel_for((int)which.size(), [&](int k) { sing[k] = order_one(ax(which[k]), which[k], st[k]); });
+    n_cpu_order += (long)which.size();
+    for (size_t k = 0; k < which.size(); ++k) {
+        const int i = which[k];
+        singular[i] = sing[k];
+        plan_of_[i] = sing[k] ? -1 : plans_.add(st[k]);
+    }
 }
 
 void AcLu::upload_inst(const std::vector<SluInst>& inst) {
@@ -59,7 +71,9 @@ void AcLu::upload_inst(const std::vector<SluInst>& inst) {
 
 int AcLu::numeric_complex(const double* ax, int32_t* status) {
     std::vector<int32_t> host(B_, 0), st(B_, 0);
-    for (int i = 0; i < B_; ++i) host[i] = order(ax + (size_t)2 * nnz_ * i, i);
+    std::vector<int> all(B_);
+    for (int i = 0; i < B_; ++i) all[i] = i;
+    order(all, [&](int i) { return ax + (size_t)2 * nnz_ * i; }, host);
     ordered_ = true;
     factor_gpu(ax, st);
     int k = 0;
@@ -76,8 +90,10 @@ int AcLu::refactorize_complex(const double* ax, int32_t* status) {
     // an instance whose ordering failed is ordered again, as the
     // reference's factor() orders a matrix that still needs ordering
     std::vector<int32_t> host(B_, 0), st(B_, 0);
+    std::vector<int> redo;
     for (int i = 0; i < B_; ++i)
-        if (plan_of_[i] < 0) host[i] = order(ax + (size_t)2 * nnz_ * i, i);
+        if (plan_of_[i] < 0) redo.push_back(i);
+    order(redo, [&](int i) { return ax + (size_t)2 * nnz_ * i; }, host);
     factor_gpu(ax, st);
     int k = 0;
     for (int i = 0; i < B_; ++i) {
@@ -122,20 +138,24 @@ void AcLu::solve_complex(double* rhs) {
 int AcLu::sweep(const double* g, const double* c, const double* freqs, int nf, const double* rhs, double* out,
                 int32_t* status) {
     if (nf <= 0) throw Error(NSA_ERR_ARGUMENT, "AC sweep: no frequencies");
+    const auto t_start = std::chrono::steady_clock::now();
     // omega = 2 pi f as ac.cpp:197 forms it
     std::vector<double> w(nf);
     for (int f = 0; f < nf; ++f) w[f] = 2.0 * M_PI * freqs[f];
     // every instance ordered at the first frequency
     std::vector<int32_t> host(B_, 0);
-    std::vector<double> ax0((size_t)2 * nnz_);
+    std::vector<double> ax0((size_t)2 * nnz_ * B_);
+    std::vector<int> all(B_);
     for (int i = 0; i < B_; ++i) {
+        all[i] = i;
         for (int k = 0; k < nnz_; ++k) {
-            ax0[2 * k] = g[(size_t)i * nnz_ + k];
-            ax0[2 * k + 1] = w[0] * c[(size_t)i * nnz_ + k];
+            ax0[(size_t)2 * nnz_ * i + 2 * k] = g[(size_t)i * nnz_ + k];
+            ax0[(size_t)2 * nnz_ * i + 2 * k + 1] = w[0] * c[(size_t)i * nnz_ + k];
         }
-        host[i] = order(ax0.data(), i);
     }
+    order(all, [&](int i) { return ax0.data() + (size_t)2 * nnz_ * i; }, host);
     ordered_ = true;
+    last_order_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
     plans_.sync(stream_);
     const long blocks = (long)B_ * nf;
     std::vector<SluInst> inst(blocks);
@@ -157,14 +177,25 @@ int AcLu::sweep(const double* g, const double* c, const double* freqs, int nf, c
     NSA_HIP(hipMemcpyAsync(d_c_, c, sizeof(double) * nnz_ * B_, hipMemcpyHostToDevice, stream_));
     NSA_HIP(hipMemcpyAsync(d_w_, w.data(), sizeof(double) * nf, hipMemcpyHostToDevice, stream_));
     NSA_HIP(hipMemcpyAsync(d_r_, rhs, sizeof(double) * 2 * n_ * B_, hipMemcpyHostToDevice, stream_));
+    hipEvent_t ev[2];
+    NSA_HIP(hipEventCreate(&ev[0]));
+    NSA_HIP(hipEventCreate(&ev[1]));
+    NSA_HIP(hipEventRecord(ev[0], stream_));
     launch_ac_build(d_g_, d_c_, d_w_, B_, nf, nnz_, d_ax_, stream_);
     launch_ac_rhs(d_r_, B_, nf, n_, d_v_, stream_);
     launch_small_lu_complex(plans_, d_inst_, (int)blocks, SLU_FACTOR | SLU_SOLVE, d_ax_, d_fail_, nullptr, nullptr,
                             d_v_, d_v_, stream_);
+    NSA_HIP(hipEventRecord(ev[1], stream_));
     std::vector<int32_t> fail(blocks);
     NSA_HIP(hipMemcpyAsync(out, d_v_, sizeof(double) * 2 * n_ * blocks, hipMemcpyDeviceToHost, stream_));
     NSA_HIP(hipMemcpyAsync(fail.data(), d_fail_, sizeof(int32_t) * blocks, hipMemcpyDeviceToHost, stream_));
     NSA_HIP(hipStreamSynchronize(stream_));
+    float ms = 0.0f;
+    NSA_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    NSA_HIP(hipEventDestroy(ev[0]));
+    NSA_HIP(hipEventDestroy(ev[1]));
+    last_device_s = ms * 1e-3;
+    last_total_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
     // the factor slabs hold factors of orders the sweep may have replaced:
     // solves need a refactorize_complex first
     factored_ = false;

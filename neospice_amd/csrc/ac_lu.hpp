@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <vector>
 
@@ -44,12 +45,18 @@ public:
     int sweep(const double* g, const double* c, const double* freqs, int nf, const double* rhs, double* out,
               int32_t* status);
 
+    // seconds of the last sweep: host orderings, device work (matrix build,
+    // factor and solve, by HIP events), whole call
+    double last_order_s = 0.0, last_device_s = 0.0, last_total_s = 0.0;
+
     int batch() const { return B_; }
     int plans() const { return plans_.count(); }
     long n_cpu_order = 0;  // host orderings
 
 private:
-    int order(const double* ax, int inst);
+    int order_one(const double* ax, int inst, LuStructure& st);
+    void order(const std::vector<int>& which, const std::function<const double*(int)>& ax,
+               std::vector<int32_t>& singular);
     void factor_gpu(const double* ax, std::vector<int32_t>& st);
     void upload_inst(const std::vector<SluInst>& inst);
     template <class T>

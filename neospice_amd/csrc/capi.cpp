@@ -524,20 +524,17 @@ int nsa_lu_ac_sweep(nsa_lu* s, const double* g, const double* c, const double* f
     return guard([&] {
         if (!g || !c || !freqs || !rhs || !out || nf <= 0)
             throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "nsa_lu_ac_sweep: arguments");
-        const int B = s->B, n = s->n;
-        const auto gb = to_blocks(g, s->nnz, B), cb = to_blocks(c, s->nnz, B), rb = to_blocks(rhs, 2 * n, B);
-        std::vector<double> ob((size_t)2 * n * B * nf);
-        std::vector<int32_t> st((size_t)B * nf);
-        const int k = s->complex_lu().sweep(gb.data(), cb.data(), freqs, nf, rb.data(), ob.data(), st.data());
-        // (instance, frequency) blocks -> column f * B + i of 2n rows
-        const size_t cols = (size_t)B * nf;
-        for (int i = 0; i < B; ++i)
-            for (int f = 0; f < nf; ++f) {
-                const size_t blk = (size_t)i * nf + f, col = (size_t)f * B + i;
-                for (int r = 0; r < 2 * n; ++r) out[(size_t)r * cols + col] = ob[blk * 2 * n + r];
-                if (status) status[col] = st[blk];
-            }
-        return k ? 1 : 0;
+        return s->complex_lu().sweep(g, c, freqs, nf, rhs, out, status) ? 1 : 0;
+    });
+}
+int nsa_lu_ac_stats(nsa_lu* s, double out[3]) {
+    return guard([&] {
+        if (!out) throw nsa::Error(nsa::NSA_ERR_ARGUMENT, "nsa_lu_ac_stats: out");
+        const nsa::AcLu& a = s->complex_lu();
+        out[0] = a.last_order_s;
+        out[1] = a.last_device_s;
+        out[2] = a.last_total_s;
+        return 0;
     });
 }
 const char* nsa_lu_name(nsa_lu* s) { return s->lu->name(); }

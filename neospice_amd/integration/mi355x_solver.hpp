@@ -12,8 +12,9 @@
 // (neo_solver.cpp:152, 170-202).  engine 0 is make_solver's own policy
 // (AMD-LU iff n >= 256 and linear, with the AutoFallbackSolver's switch to
 // Markowitz on a singular first factorization; make_solver.cpp:71-176).
-// The complex (AC/noise) paths are not offloaded: they throw, and
-// make_solver is only reached by the DC and transient analyses.
+// The complex (AC) paths are NeoSolver's, as AmdLuSolver's delegate to one
+// (amd_lu_solver.cpp:347-349): the complex Markowitz ordering on the host,
+// the complex factor and solves on the MI355X (nsa_lu_*_complex).
 #pragma once
 #include <neospice_amd.h>
 
@@ -39,7 +40,9 @@ public:
         if (h_) nsa_lu_destroy(h_);
         h_ = nullptr;
         n_ = p.size();
+        nnz_ = (int)csc.row_idx.size();
         factored_ = false;
+        complex_factored_ = false;
         check(nsa_lu_create(1, n_, csc.col_ptr.data(), csc.row_idx.data(), is_linear_ ? 1 : 0, engine_, &h_));
     }
     bool numeric(const SparsityPattern&, const NumericMatrix& m, double diag_gmin) override {
@@ -71,9 +74,32 @@ public:
         if (!h_) throw std::logic_error("Mi355xSolver::set_equilibrate: symbolic() not called");
         check(nsa_lu_set_equilibrate(h_, on ? 1 : 0));
     }
-    void numeric_complex(const SparsityPattern&, const std::vector<double>&) override { unsupported(); }
-    bool refactorize_complex(const std::vector<double>&) override { unsupported(); }
-    void solve_complex(std::vector<double>&) override { unsupported(); }
+    // ax: 2 nnz values, (re, im) per CSC slot.  Like NeoSolver's, it reports
+    // no singularity, and the real factors are gone after it
+    // (neo_solver.cpp:228-245).
+    void numeric_complex(const SparsityPattern&, const std::vector<double>& ax) override {
+        if (!h_) throw std::logic_error("Mi355xSolver::numeric_complex: symbolic() not called");
+        if ((int)ax.size() != 2 * nnz_) throw std::invalid_argument("Mi355xSolver::numeric_complex: ax size mismatch");
+        int32_t st = 0;
+        check(nsa_lu_numeric_complex(h_, ax.data(), &st));
+        complex_factored_ = true;
+        factored_ = false;
+    }
+    bool refactorize_complex(const std::vector<double>& ax) override {
+        if (!h_) throw std::logic_error("Mi355xSolver::refactorize_complex: symbolic() not called");
+        if (!complex_factored_)
+            throw std::logic_error("Mi355xSolver::refactorize_complex: numeric_complex() not called");
+        if ((int)ax.size() != 2 * nnz_)
+            throw std::invalid_argument("Mi355xSolver::refactorize_complex: ax size mismatch");
+        int32_t st = 0;
+        return check(nsa_lu_refactorize_complex(h_, ax.data(), &st)) == 1;
+    }
+    void solve_complex(std::vector<double>& rhs) override {
+        if (!h_) throw std::logic_error("Mi355xSolver::solve_complex: symbolic() not called");
+        if (!complex_factored_) throw std::logic_error("Mi355xSolver::solve_complex: numeric_complex() not called");
+        if ((int)rhs.size() != 2 * n_) throw std::invalid_argument("Mi355xSolver::solve_complex: rhs size must be 2*n");
+        check(nsa_lu_solve_complex(h_, rhs.data()));
+    }
     const char* name() const override { return h_ ? nsa_lu_name(h_) : "mi355x"; }
 
 private:
@@ -84,11 +110,9 @@ private:
         if (rc == -2) throw std::invalid_argument(msg);  // bad argument or size
         throw std::runtime_error(msg);
     }
-    [[noreturn]] static void unsupported() {
-        throw std::logic_error("Mi355xSolver: the complex (AC) path stays on NeoSolver");
-    }
     nsa_lu* h_ = nullptr;
-    int n_ = 0;
+    int n_ = 0, nnz_ = 0;
+    bool complex_factored_ = false;
     int engine_;
     bool is_linear_;
     bool factored_ = false;

@@ -86,3 +86,54 @@ def test_newton_solve_through_mi355x_solver(gpu_lib):
         gpu_lib.ref_set_gpu_solver(0)
     assert it_gpu == it_cpu
     assert np.array_equal(x1, x0)
+
+
+def test_mi355x_solver_complex_paths(gpu_lib):
+    """The adapter's complex paths as the AC analysis drives a solver
+    (ac.cpp:215-222): numeric_complex then solve_complex, refactorize_complex
+    at another frequency then solve_complex, through Mi355xSolver (compiled
+    against the reference's headers, linked into the reference) -- bit-equal
+    to the reference's own NeoSolver on the AC system g + j w c of a
+    145-unknown RC mesh."""
+    import ctypes as C
+    import neospice_amd as na
+    mesh, _ = W.rc_mesh(12)
+    mesh.finalize()
+    rc = RB.RefCircuit(mesh, probe=False)
+    x, _, _ = rc.dc()
+    g, _ = rc.load(x)                                     # DC conductances
+    dt = 1e-12
+    rc.set_ctx(na.MODETRAN | na.MODEINITPRED, 1, 0, (1 / dt, -1 / dt, 0), dt, dt, 0.0, dt)
+    a1, _ = rc.load(x)
+    c = (np.asarray(a1) - np.asarray(g)) * dt             # capacitances on the pattern
+    cp, ri = rc.pattern()
+    n = mesh.n
+    rng = np.random.default_rng(1)
+    rhs = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    z0 = np.asarray(g) + 1j * (2 * np.pi * 1e6) * c
+    z1 = np.asarray(g) + 1j * (2 * np.pi * 1e10) * c
+
+    def inter(z):
+        out = np.empty(2 * len(z))
+        out[0::2], out[1::2] = z.real, z.imag
+        return np.ascontiguousarray(out)
+
+    L = gpu_lib
+    L.ref_mi355x_complex.restype = C.c_int
+    L.ref_mi355x_complex.argtypes = [C.c_int] + [C.c_void_p] * 7 + [C.c_char_p]
+    cpa = np.ascontiguousarray(cp, np.int32)
+    ria = np.ascontiguousarray(ri, np.int32)
+    a0, a1_, r = inter(z0), inter(z1), inter(rhs)
+    x0 = np.zeros(2 * n)
+    x1 = np.zeros(2 * n)
+    err = C.create_string_buffer(256)
+    rcode = L.ref_mi355x_complex(n, cpa.ctypes.data, ria.ctypes.data, a0.ctypes.data, a1_.ctypes.data,
+                                 r.ctypes.data, x0.ctypes.data, x1.ctypes.data, err)
+    assert rcode >= 0, err.value.decode()
+    ref = RB.RefSolver(0, n, cp, ri)
+    assert not ref.numeric_complex(z0)
+    y0 = ref.solve_complex(rhs)
+    assert bool(rcode) == ref.refactorize_complex(z1)
+    y1 = ref.solve_complex(rhs)
+    assert np.array_equal(x0[0::2] + 1j * x0[1::2], y0)
+    assert np.array_equal(x1[0::2] + 1j * x1[1::2], y1)
