@@ -673,7 +673,7 @@ def c3_headline(na, W, args):
               "unknowns_total": b.total_n, "nnz_total": b.total_nnz,
               "tstop": C3_TSTOP, "max_step": C3_TSTEP, "integration": "trapezoidal, adaptive (device LTE)",
               "step": "one window of tstop/K of every instance's adaptive transient",
-              "pivot_plans": st["plans"]}
+              "pivot_plans": st["plans"], "lu_schedule": b.tuning()}
     out = {"value": value, "wall": wall, "K": K, "config": config,
            "accepted_steps": {"min": int(accepted.min()), "max": int(accepted.max()),
                               "mean": round(float(accepted.mean()), 2),

@@ -248,6 +248,17 @@ int nsa_bundle_set_graph(nsa_bundle* b, int32_t on);
  * device, every instance on its own step sequence; 0: on the host between
  * batched Newton rounds.  Both give the same results per instance. */
 int nsa_bundle_set_device_control(nsa_bundle* b, int32_t on);
+/* 1 (default): at a transient's start the packed LU's schedule (the
+ * shortest run of one-row levels solved as a wave recurrence, LDS staging
+ * of the plan) is chosen by timing every candidate on the batch's own
+ * matrices; every schedule gives the same results.  nsa_bundle_tuning
+ * writes up to max_out values -- the chosen candidate, the candidate count,
+ * then per candidate (min serial run, staging, microseconds per launch,
+ * as timed when the decision was made: a process reuses a decision for the
+ * same plans and batch size) -- and returns how many there are
+ * (no reference counterpart: the autotuner of this engine). */
+int nsa_bundle_set_tuning(nsa_bundle* b, int32_t on);
+int nsa_bundle_tuning(nsa_bundle* b, double* out, int32_t max_out);
 /* device-controlled transient: ticks run, host looks between batches of
  * ticks, host seconds spent enqueueing ticks and waiting for the device
  * (no reference counterpart: statistics of this engine) */

@@ -741,6 +741,23 @@ class Bundle:
         check(lib().nsa_bundle_tick_stats(self.h, f64p(t)))
         return {"ticks": int(t[0]), "batches": int(t[1]), "enqueue_s": float(t[2]), "wait_s": float(t[3])}
 
+    def set_tuning(self, on=True):
+        """The packed LU's schedule chosen by timing at a transient's start
+        (on by default; the same results under every schedule)."""
+        check(lib().nsa_bundle_set_tuning(self.h, int(bool(on))))
+
+    def tuning(self):
+        """The last tuning: the chosen schedule and every candidate's time."""
+        k = check(lib().nsa_bundle_tuning(self.h, None, 0))
+        if k <= 0:
+            return None
+        o = np.zeros(k, np.float64)
+        check(lib().nsa_bundle_tuning(self.h, f64p(o), k))
+        nc = int(o[1])
+        cands = [{"min_serial_run": int(o[2 + 3 * c]), "stage": bool(o[3 + 3 * c]), "us": float(o[4 + 3 * c])}
+                 for c in range(nc)]
+        return {"chosen": cands[int(o[0])], "candidates": cands}
+
     def set_device_control(self, on=True):
         """True (default): the transient's step controller runs on the device
         (every instance on its own step sequence, no host round trip per

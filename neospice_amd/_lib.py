@@ -92,6 +92,8 @@ _SIGS = {
     "nsa_lu_solve_complex": (C.c_int, [VP, P_F64]),
     "nsa_lu_ac_sweep": (C.c_int, [VP, P_F64, P_F64, P_F64, I32, P_F64, P_F64, P_I32]),
     "nsa_lu_ac_stats": (C.c_int, [VP, P_F64]),
+    "nsa_bundle_set_tuning": (C.c_int, [VP, I32]),
+    "nsa_bundle_tuning": (C.c_int, [VP, P_F64, I32]),
     "nsa_lu_solve": (C.c_int, [VP, P_F64]),
     "nsa_lu_solve_transposed": (C.c_int, [VP, P_F64]),
     "nsa_lu_solve_noalloc": (C.c_int, [VP, P_F64]),

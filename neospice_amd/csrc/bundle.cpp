@@ -4,6 +4,7 @@
 #include <chrono>
 
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <cstring>
 #include <atomic>
@@ -1165,6 +1166,76 @@ void source_breakpoints(const Device& d, const double* pulse, double tstop, std:
 
 }  // namespace
 
+// The packed LU's schedule by timing (the autotuner): every candidate
+// schedule (the serial-phase threshold and LDS staging, SluOpts) rebuilds
+// the plans and times a few factor-and-solve launches of the whole batch on
+// its current matrices into scratch buffers; the fastest is kept.  Every
+// schedule computes the same results, so the choice changes speed only.
+// A process-wide cache keyed by the plans and the batch size reuses a
+// decision (the bench's warm-up and timed bundles share one).
+void Bundle::tune_lu() {
+    if (NI_ <= 0 || plans_.count() == 0) return;
+    static std::map<uint64_t, std::pair<int, std::vector<double>>> cache;
+    const SluOpts cand[] = {{2, true}, {4, true}, {8, true}, {32, true}, {1 << 30, true},
+                            {2, false}, {4, false}, {8, false}, {32, false}, {1 << 30, false}};
+    const int nc = (int)(sizeof(cand) / sizeof(cand[0]));
+    plans_.rebuild(SluOpts{});
+    uint64_t key = 1469598103934665603ull;
+    auto mix = [&](uint64_t v) { key = (key ^ v) * 1099511628211ull; };
+    for (int32_t w : plans_.host_words()) mix((uint32_t)w);
+    for (int32_t w : plans_.host_offsets()) mix((uint32_t)w);
+    mix((uint64_t)NI_);
+    std::vector<double> us(nc, -1.0);
+    int best = -1;
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+        best = it->second.first;
+        us = it->second.second;
+    } else {
+        double *lu = nullptr, *x = nullptr;
+        int32_t* fail = nullptr;
+        NSA_HIP(hipMalloc(&lu, sizeof(double) * (size_t)NI_ * kSluMaxLu));
+        NSA_HIP(hipMalloc(&x, sizeof(double) * std::max<size_t>(1, (size_t)tot_n_)));
+        NSA_HIP(hipMalloc(&fail, sizeof(int32_t) * NI_));
+        hipEvent_t e0, e1;
+        NSA_HIP(hipEventCreate(&e0));
+        NSA_HIP(hipEventCreate(&e1));
+        constexpr int kReps = 5;
+        for (int c = 0; c < nc; ++c) {
+            plans_.rebuild(cand[c]);
+            plans_.sync(stream_);
+            double bestc = 1e30;
+            for (int r = 0; r <= kReps; ++r) {
+                NSA_HIP(hipEventRecord(e0, stream_));
+                launch_small_lu(plans_, d_inst_, NI_, SLU_FACTOR | SLU_SOLVE, d_A_, opt_.diag_gmin, nullptr, fail,
+                                nullptr, lu, d_rhs_, x, stream_, d_gmin_);
+                NSA_HIP(hipEventRecord(e1, stream_));
+                NSA_HIP(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                NSA_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (r > 0) bestc = std::min(bestc, 1e3 * (double)ms);  // the first launch warms up
+            }
+            us[c] = bestc;
+            if (best < 0 || us[c] < us[best]) best = c;
+        }
+        NSA_HIP(hipEventDestroy(e0));
+        NSA_HIP(hipEventDestroy(e1));
+        NSA_HIP(hipFree(lu));
+        NSA_HIP(hipFree(x));
+        NSA_HIP(hipFree(fail));
+        cache[key] = {best, us};
+    }
+    plans_.rebuild(cand[best]);
+    plans_.sync(stream_);
+    tune_report.assign(1, (double)best);
+    tune_report.push_back((double)nc);
+    for (int c = 0; c < nc; ++c) {
+        tune_report.push_back((double)cand[c].min_serial_run);
+        tune_report.push_back(cand[c].stage ? 1.0 : 0.0);
+        tune_report.push_back(us[c]);
+    }
+}
+
 void Bundle::tran_begin(double tstep, double tstop, const int32_t* probe_ptr, const int32_t* probes,
                         int max_points) {
     if (!(tstep > 0.0) || !(tstop > 0.0))
@@ -1193,6 +1264,11 @@ void Bundle::tran_begin(double tstep, double tstop, const int32_t* probe_ptr, co
         if (!conv[i])
             throw Error(NSA_ERR_RUNTIME, "Bundle::tran_begin: DC operating point did not converge for instance " +
                                              std::to_string(i));
+    // the packed LU's schedule, timed on the operating point's matrices
+    if (tuning) {
+        if (inst_dirty_) upload_inst();
+        tune_lu();
+    }
     // 2. a fresh solver for the transient (transient.cpp:728-730); state
     //    history seeded from the operating point (transient.cpp:791-802)
     reset_solver();

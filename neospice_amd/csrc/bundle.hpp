@@ -134,6 +134,12 @@ public:
     // on the host between batched Newton rounds (the same decisions; every
     // instance's results are bit-identical either way).
     bool device_control = true;
+    // The packed LU's schedule chosen by timing (tune_lu): at a transient's
+    // start when on.  tune_report: chosen candidate, then per candidate
+    // (min_serial_run, stage, microseconds per launch).
+    bool tuning = true;
+    std::vector<double> tune_report;
+    void tune_lu();
     // probe rows of instance i: (count, values row-major count x nprobe_i)
     void tran_probes(int i, int* count, double* out) const;
     int tran_probe_count(int i) const { return pptr_.empty() ? 0 : pptr_[i + 1] - pptr_[i]; }

@@ -769,6 +769,20 @@ int nsa_bundle_tick_stats(nsa_bundle* b, double out[4]) {
         return 0;
     });
 }
+int nsa_bundle_set_tuning(nsa_bundle* b, int32_t on) {
+    return guard([&] {
+        b->b->tuning = on != 0;
+        return 0;
+    });
+}
+int nsa_bundle_tuning(nsa_bundle* b, double* out, int32_t max_out) {
+    return guard([&] {
+        const auto& r = b->b->tune_report;
+        if (out)
+            for (int k = 0; k < std::min<int>(max_out, (int)r.size()); ++k) out[k] = r[k];
+        return (int)r.size();
+    });
+}
 int nsa_bundle_set_device_control(nsa_bundle* b, int32_t on) {
     return guard([&] { b->b->device_control = on != 0; return 0; });
 }

@@ -578,7 +578,7 @@ void launch_small_lu(const SmallLuPlans& plans, const SluInst* d_inst, int ninst
     const size_t base =
         sizeof(double) * (((size_t)max_nlu + max_n + 2 + std::max(max_n, kW)) & ~(size_t)1);
     const size_t hot = sizeof(int32_t) * (((size_t)plans.max_hot() + 3) / 4 * 4);
-    const int stage = (base + hot <= kLdsLimit) ? 1 : 0;
+    const int stage = (plans.opts().stage && base + hot <= kLdsLimit) ? 1 : 0;
     const size_t lds = base + (stage ? hot : 0);
     if (stage)
         hipLaunchKernelGGL(k_small_lu<true>, dim3(grid), dim3(kW), lds, s, plans.d_words(),

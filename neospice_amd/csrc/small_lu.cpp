@@ -96,7 +96,7 @@ void align4(std::vector<int32_t>& w) {
 // mostly that short, so a level needs no term fetch before its operand
 // loads.
 void append_program(const TriProgram& t, int nlu, int n, std::vector<int32_t>& w, int32_t base,
-                    int32_t* hdr4) {
+                    int32_t* hdr4, int min_run) {
     const int32_t zero_b = 8 * (nlu + n);
     auto vb = [&](int pos) { return (int32_t)(8 * pos); };
     auto yb = [&](int i) { return (int32_t)(8 * (nlu + i)); };
@@ -106,13 +106,13 @@ void append_program(const TriProgram& t, int nlu, int n, std::vector<int32_t>& w
     hdr4[1] = (int32_t)w.size() - base;
     for (int l = 0; l <= nl; ++l) w.push_back(t.level_ptr[l]);
     align4(w);
-    // phases: maximal runs of at least kSluMinSerialRun one-row levels are
+    // phases: maximal runs of at least min_run one-row levels are
     // serial, what lies between them parallel
     std::vector<int32_t> ph;
     for (int l = 0; l < nl;) {
         int e = l;
         while (e < nl && t.level_ptr[e + 1] - t.level_ptr[e] == 1) ++e;
-        if (e - l >= kSluMinSerialRun) {
+        if (e - l >= min_run) {
             ph.insert(ph.end(), {PH_SERIAL, l, e, 0});
             l = e;
             continue;
@@ -185,6 +185,10 @@ int SmallLuPlans::add(const LuStructure& s) {
     for (size_t p = 0; p < st_.size(); ++p)
         if (st_[p].same_as(s) && st_[p].nnz_a == s.nnz_a && st_[p].gmin_pos == s.gmin_pos)
             return (int)p;
+    return add_new(s);
+}
+
+int SmallLuPlans::add_new(const LuStructure& s) {
     const int n = s.n;
     align4(words_);
     const int32_t base = (int32_t)words_.size();
@@ -217,7 +221,7 @@ int SmallLuPlans::add(const LuStructure& s) {
     const TriProgram* cold[3] = {&fwd, &tfwd, &tbwd};
     const int cold_id[3] = {0, 2, 3};
     for (int c = 0; c < 3; ++c) {
-        append_program(*cold[c], s.nlu(), n, words_, base, h4);
+        append_program(*cold[c], s.nlu(), n, words_, base, h4, opts_.min_serial_run);
         for (int q = 0; q < 4; ++q) H(SH_PROG + 4 * cold_id[c] + q) = h4[q];
     }
     // the hot block: what the level loops of a factor-and-solve launch read
@@ -241,7 +245,7 @@ int SmallLuPlans::add(const LuStructure& s) {
         tasks[q + 3] = pv >= 0 ? 8 * pv : -(8 * (-pv - 1) + 1);
     }
     put(SH_TASK, tasks);
-    append_program(bwd, s.nlu(), n, words_, base, h4);
+    append_program(bwd, s.nlu(), n, words_, base, h4, opts_.min_serial_run);
     for (int q = 0; q < 4; ++q) H(SH_PROG + 4 + q) = h4[q];
     align4(words_);
     H(SH_HOT) = hot0;
@@ -254,6 +258,28 @@ int SmallLuPlans::add(const LuStructure& s) {
     max_n_ = std::max(max_n_, n);
     max_hot_ = std::max(max_hot_, (int)words_[base + SH_HOTLEN]);
     return (int)st_.size() - 1;
+}
+
+void SmallLuPlans::rebuild(const SluOpts& o) {
+    std::vector<LuStructure> st;
+    st.swap(st_);
+    opts_ = o;
+    words_.clear();
+    off_.clear();
+    lev_.clear();
+    slev_.clear();
+    blev_.clear();
+    max_nlu_ = max_n_ = max_hot_ = 0;
+    for (const LuStructure& x : st) {
+        // add() reuses an identical earlier plan; keep one plan per entry so
+        // that plan ids stay what the instances hold
+        const size_t before = st_.size();
+        (void)add_new(x);
+        if (st_.size() != before + 1) throw Error(NSA_ERR_RUNTIME, "SmallLuPlans::rebuild: plan ids moved");
+    }
+    synced_words_ = 0;
+    synced_plans_ = 0;
+    d_words_cap_ = 0;  // a fresh device copy: the old one may be in use
 }
 
 void SmallLuPlans::sync(hipStream_t s) {

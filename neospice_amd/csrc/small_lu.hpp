@@ -90,9 +90,23 @@ enum SluMode : int32_t { SLU_FACTOR = 1, SLU_SOLVE = 2, SLU_SOLVE_T = 4 };
 constexpr int kSluMaxLu = 6144;   // factor values per instance held in LDS
 constexpr int kSluMaxN = 1024;
 
+// Schedule choices of the packed LU, the same results under every choice
+// (per entry the same operations in the same order): the shortest run of
+// one-row levels that a solve program runs as a serial phase (a wave
+// recurrence) instead of level by level, and whether a launch stages the
+// plan's hot block in LDS when it fits.
+struct SluOpts {
+    int min_serial_run = kSluMinSerialRun;
+    bool stage = true;
+};
+
 // The plans of a packed batch, built on the host and mirrored on the device.
 class SmallLuPlans {
 public:
+    const SluOpts& opts() const { return opts_; }
+    // Rebuilds every plan under new schedule choices (plan ids unchanged);
+    // the next sync() uploads them afresh.
+    void rebuild(const SluOpts& o);
     ~SmallLuPlans();
     // Whether a structure fits the packed kernel.
     static bool fits(const LuStructure& s) {
@@ -100,6 +114,7 @@ public:
     }
     // Plan id of `s` (an existing identical plan is reused).
     int add(const LuStructure& s);
+    int add_new(const LuStructure& s);
     const LuStructure& structure(int plan) const { return st_[plan]; }
     int count() const { return (int)st_.size(); }
     int max_nlu() const { return max_nlu_; }
@@ -121,6 +136,7 @@ public:
     const std::vector<int32_t>& host_offsets() const { return off_; }
 
 private:
+    SluOpts opts_;
     std::vector<LuStructure> st_;
     std::vector<int32_t> words_, off_;
     std::vector<int> lev_, slev_, blev_;

@@ -267,3 +267,35 @@ def test_bundle_tran_device_control_matches_host():
         assert np.array_equal(bh.tran_probes(i), bd.tran_probes(i)), i
         assert np.array_equal(bh.solution(i), bd.solution(i)), i
     assert bh.stats()["iterations"] == bd.stats()["iterations"]
+
+
+def test_bundle_tuning_same_results():
+    """The packed LU's autotuner (every candidate schedule timed on the
+    batch's own matrices at the transient's start) changes speed only: a
+    bundle that tunes and one on the default schedule take the same steps
+    and probe the same waveforms, bit for bit; the tuner reports every
+    candidate and keeps the fastest."""
+    cks, _ = W.c3_rings(stages=[9, 31, 11, 45])
+    chain, _ = W.inverter_chain(4)
+    cks.append(chain)
+    for c in cks:
+        c.finalize()
+    probes = [range(c.n) for c in cks]
+    runs = []
+    for tune in (False, True):
+        b = na.Bundle(cks)
+        b.set_tuning(tune)
+        b.tran_begin(2e-12, 120e-12, probes=probes, max_points=4000)
+        assert b.tran_advance(120e-12)
+        runs.append(b)
+    assert runs[0].tuning() is None
+    t = runs[1].tuning()
+    print("tuning", t, flush=True)
+    assert t is not None and len(t["candidates"]) >= 2
+    timed = [c["us"] for c in t["candidates"]]
+    assert min(timed) > 0 and t["chosen"]["us"] == min(timed)
+    for i in range(len(cks)):
+        s0, s1 = runs[0].tran_steps(i), runs[1].tran_steps(i)
+        for k in s0:
+            assert np.array_equal(s0[k], s1[k]), (i, k)
+        assert np.array_equal(runs[0].tran_probes(i), runs[1].tran_probes(i)), i
