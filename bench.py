@@ -706,6 +706,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-mesh", action="store_true")
     ap.add_argument("--headline-only", action="store_true", help="only the ring batch")
+    ap.add_argument("--ac-only", action="store_true", help="only the AC-sweep leg (its JSON)")
     ap.add_argument("--mesh", type=int, default=64)
     ap.add_argument("--mesh-steps", type=int, default=100)
     ap.add_argument("--mesh-batch1-steps", type=int, default=20)
@@ -732,6 +733,9 @@ def main():
     import neospice_amd as na
     from neospice_amd import workloads as W
 
+    if args.ac_only:
+        print(json.dumps(ac_sweep(na, W, args, cpu=not args.no_cpu)))
+        return
     # the mesh leg first: its memory-bound refactor runs ~2x slower in a
     # process whose device heap the ring leg's many bundle allocations have
     # already churned (the ring leg is not affected by the order)
