@@ -39,6 +39,15 @@ __device__ inline C2 cld(const char* L, int realb) { return *reinterpret_cast<co
 __device__ inline void cst(char* L, int realb, C2 v) { *reinterpret_cast<C2*>(L + 2 * realb) = v; }
 __device__ inline bool cnz(C2 v) { return v.r != 0.0 || v.i != 0.0; }
 
+// The end of a level: a workgroup is one wave, whose LDS accesses are
+// performed in the order it issues them, so only the compiler must keep
+// the accesses on their side.
+__device__ inline void level_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ inline int wave_min_i32(int v) {
     for (int o = kW / 2; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
     return v;
@@ -93,9 +102,14 @@ __device__ inline void run_program_c(const int32_t* P, int prog, char* L, int la
     if (nl <= 0) return;
     const int32_t* lev = P + P[SH_PROG + 4 * prog + 1];
     const int32_t* rows = P + P[SH_PROG + 4 * prog + 2];
+    // the level bounds one level ahead (the plan is read from global memory)
+    int r0 = lev[0], r1 = lev[1];
     for (int l = 0; l < nl; ++l) {
-        for (int r = lev[l] + lane; r < lev[l + 1]; r += kW) run_row_c<OP>(P, rows + 8 * r, L);
-        __syncthreads();
+        const int r2 = (l + 2 <= nl) ? lev[l + 2] : r1;
+        for (int r = r0 + lane; r < r1; r += kW) run_row_c<OP>(P, rows + 8 * r, L);
+        level_sync();
+        r0 = r1;
+        r1 = r2;
     }
 }
 
@@ -139,9 +153,22 @@ __global__ __launch_bounds__(kW) void k_small_lu_c(const int32_t* __restrict__ w
         const int nlev = P[SH_NLEV];
         const int32_t* lev = P + P[SH_LEV];
         const int4* task = reinterpret_cast<const int4*>(P + P[SH_TASK]);
-        for (int l = 0; l < nlev; ++l) {
-            for (int t = lev[l] + lane; t < lev[l + 1]; t += kW) run_task_c(task[t], L);
-            __syncthreads();
+        const int last = P[SH_NTASK] - 1;
+        if (nlev > 0 && last >= 0) {
+            // each level's first task and the next bounds fetched a level
+            // ahead (the plan is read from global memory)
+            int t0 = lev[0], t1 = lev[1];
+            int4 cur = task[min(t0 + lane, last)];
+            for (int l = 0; l < nlev; ++l) {
+                const int t2 = (l + 2 <= nlev) ? lev[l + 2] : t1;
+                const int4 nxt = task[min(t1 + lane, last)];
+                if (t0 + lane < t1) run_task_c(cur, L);
+                for (int t = t0 + lane + kW; t < t1; t += kW) run_task_c(task[t], L);
+                level_sync();
+                cur = nxt;
+                t0 = t1;
+                t1 = t2;
+            }
         }
         const int32_t* cptr = P + P[SH_CPTR];
         const int32_t* nU = P + P[SH_NU];
