@@ -526,7 +526,9 @@ void Bundle::launch_loads_small() {
     double* s1 = d_s_[1];
     double* s2 = d_s_[2];
     launch_cap(cap_, d_ctx_, d_x_, s0, s1, s2, d_done_, d_cm_, d_cr_, 1, stream_);
-    launch_vsrc(vsrc_, d_ctx_, d_done_, d_cr_, 1, stream_);
+    // with BSIM4v7 devices present the voltage sources ride in the BSIM4v7
+    // load's launch (launch_loads_b4_gather)
+    if (!b4_.count) launch_vsrc(vsrc_, d_ctx_, d_done_, d_cr_, 1, stream_);
     launch_isrc(isrc_, d_ctx_, d_done_, d_cr_, 1, stream_);
     launch_diode(dio_, d_ctx_, d_x_, s0, s1, s2, d_done_, d_cm_, d_cr_, d_devnc_, tol_, 1, stream_);
     launch_mos1(mos_, d_ctx_, d_x_, s0, s1, s2, d_done_, d_cm_, d_cr_, d_devnc_, tol_, 1, stream_);
@@ -535,8 +537,9 @@ void Bundle::launch_loads_small() {
 // The BSIM4v7 load, then the matrix and rhs gathers.
 void Bundle::launch_loads_b4_gather() {
     if (timing_ && b4_.count) NSA_HIP(hipEventRecord(ev_[1], stream_));
+    const VsrcSide vs{vsrc_, d_ctx_, d_done_, d_cr_, 1};
     launch_bsim4(b4_, d_ctx_, d_x_, d_s_[0], d_s_[1], d_s_[2], d_done_, d_cm_, d_cr_, d_devnc_, tol_, b4_lanes_,
-                 stream_);
+                 stream_, &vs);
     if (timing_ && b4_.count) NSA_HIP(hipEventRecord(ev_[2], stream_));
     GatherPacked m = gm_, r = gr_;
     m.ptr = d_mptr_; m.sites = d_msites_; m.contrib = d_cm_; m.ptr2 = nullptr; m.sites2 = nullptr;

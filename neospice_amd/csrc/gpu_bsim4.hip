@@ -23,6 +23,7 @@
 #include "dev_limits.hpp"
 #include "dual.hpp"
 #include "gpu_dev.hpp"
+#include "dev_src.hpp"
 
 namespace nsa {
 
@@ -1381,7 +1382,13 @@ __device__ __forceinline__ void b4_lane(const B4Table& t, const DevCtx* __restri
 __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_bsim4(
     B4Table t, const DevCtx* __restrict__ ctx, const double* __restrict__ x, double* __restrict__ s0,
     double* __restrict__ s1, const double* __restrict__ s2, const int32_t* __restrict__ done,
-    double* __restrict__ cm, double* __restrict__ cr, int32_t* __restrict__ devnc, DevTol tol, int B) {
+    double* __restrict__ cm, double* __restrict__ cr, int32_t* __restrict__ devnc, DevTol tol, int B, int nb4,
+    VsrcSide vs) {
+    if ((int)blockIdx.x >= nb4) {
+        // an extra workgroup: voltage sources (their own sites)
+        vsrc_lane(vs.t, vs.ctx, vs.done, vs.cr, vs.B, (long)(blockIdx.x - nb4) * TPB + threadIdx.x);
+        return;
+    }
     b4_lane<double>(t, ctx, x, s0, s1, s2, done, cm, cr, devnc, tol, B, nullptr, nullptr, nullptr);
 }
 
@@ -1408,10 +1415,17 @@ void launch_bsim4_grad(const B4Table& t, const DevCtx* ctx, const double* x, con
 
 void launch_bsim4(const B4Table& t, const DevCtx* ctx, const double* x, double* s0, double* s1,
                   const double* s2, const int32_t* done, double* cm, double* cr, int32_t* devnc,
-                  DevTol tol, int B, hipStream_t s) {
+                  DevTol tol, int B, hipStream_t s, const VsrcSide* vs) {
     if (!t.count) return;
-    hipLaunchKernelGGL(k_bsim4, dim3(blocks_for((long)t.count * B)), dim3(TPB), 0, s, t, ctx, x,
-                       s0, s1, s2, done, cm, cr, devnc, tol, B);
+    const int nb4 = (int)blocks_for((long)t.count * B);
+    VsrcSide v{};
+    int nv = 0;
+    if (vs && vs->t.count) {
+        v = *vs;
+        nv = (int)blocks_for((long)v.t.count * v.B);
+    }
+    hipLaunchKernelGGL(k_bsim4, dim3(nb4 + nv), dim3(TPB), 0, s, t, ctx, x, s0, s1, s2, done, cm, cr, devnc, tol,
+                       B, nb4, v);
 }
 
 }  // namespace nsa

@@ -166,11 +166,22 @@ void launch_diode(const DioTable& t, const DevCtx* ctx, const double* x, double*
 void launch_mos1(const Mos1Table& t, const DevCtx* ctx, const double* x, double* s0, double* s1,
                  const double* s2, const int32_t* done, double* cm, double* cr, int32_t* devnc,
                  DevTol tol, int B, hipStream_t s);
+// Voltage sources a launch may evaluate in extra workgroups (vsrc_lane):
+// the BSIM4v7 load carries them, so that a batch of rings needs no launch
+// of its own for its supplies.
+struct VsrcSide {
+    VsrcTable t;
+    const DevCtx* ctx;
+    const int32_t* done;
+    double* cr;
+    int B;
+};
 // BSIM4v7load for the default-mode cards (bsim4v7_load.cpp:69-5104); the
-// load's own current test sets devnc (device_converged).
+// load's own current test sets devnc (device_converged).  vs: voltage
+// sources evaluated in extra workgroups of the same launch.
 void launch_bsim4(const B4Table& t, const DevCtx* ctx, const double* x, double* s0, double* s1,
                   const double* s2, const int32_t* done, double* cm, double* cr, int32_t* devnc,
-                  DevTol tol, int B, hipStream_t s);
+                  DevTol tol, int B, hipStream_t s, const VsrcSide* vs = nullptr);
 // dL/dp of every BSIM4v7 (device, instance) at a converged DC point for one
 // direction p of the constant record: the load's stamps in MODEADJOINT
 // differentiated by dual numbers seeded with dpar = d(record)/dp (the
